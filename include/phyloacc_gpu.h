@@ -1,0 +1,171 @@
+/*
+ * phyloacc_gpu.h -- C ABI of the MI355X engine for PhyloAcc-ST's per-element
+ * MCMC (Felsenstein pruning + Gibbs Z updates + Metropolis r1/r2 updates).
+ *
+ * Plain C types only (no C++ and no torch types cross this boundary).  Every
+ * call returns a pa_status; PA_OK is 0.  pa_last_error() gives the message
+ * of the last failing call on an engine.  No exceptions cross the ABI.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository).
+ */
+#ifndef PHYLOACC_GPU_H
+#define PHYLOACC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PA_OK = 0,
+    PA_E_INPUT = 1,     /* malformed input (tree, options, alignment, ids) */
+    PA_E_NUMERIC = 2,   /* numeric failure (NaN likelihood, bad eigensystem) */
+    PA_E_DEVICE = 3,    /* HIP runtime error or no usable gfx950 device */
+    PA_E_OOM = 4,       /* device memory exhausted / over the HBM budget */
+    PA_E_STATE = 5      /* call not valid in the engine's current state */
+} pa_status;
+
+enum { PA_FP64 = 0, PA_FP32 = 1 };                 /* partial-likelihood precision */
+enum { PA_RUN_BATCHED = 0, PA_RUN_PER_ELEMENT = 1 }; /* launch strategy */
+enum { PA_ELEM_OK = 0, PA_ELEM_FILTERED = 1, PA_ELEM_NUMERIC_FAIL = 2 };
+
+/* leaf codes produced by the encoder */
+enum { PA_CODE_A = 0, PA_CODE_C = 1, PA_CODE_G = 2, PA_CODE_T = 3, PA_CODE_GAP = 4,
+       PA_CODE_OTHER = 5, PA_CODE_R = 6, PA_CODE_Y = 7, PA_CODE_K = 8, PA_CODE_M = 9,
+       PA_CODE_S = 10, PA_CODE_W = 11 };
+
+/* Run parameters, one field per reference parameter-file key
+ * (src/PhyloAcc-ST/main.cpp:134-277, defaults main.cpp:35-90). */
+typedef struct {
+    int32_t burnin;            /* BURNIN (200) */
+    int32_t mcmc;              /* MCMC (800) */
+    int32_t adapt_freq;        /* ADAPT_FREQ (500) */
+    double init_crate;         /* INIT_CONSERVE_RATE (0.5) */
+    double init_nrate;         /* INIT_ACCE_RATE (1) */
+    double cprior_a, cprior_b; /* CONSERVE_PRIOR_A/B (5, 0.04) */
+    double nprior_a, nprior_b; /* ACCE_PRIOR_A/B (10, 0.2) */
+    int32_t ropt;              /* ROPT (1) */
+    double cub, nlb;           /* CUB (1), NLB (0.6) */
+    double init_grate, init_lrate, init_lrate2;   /* INIT_GRATE/LRATE/LRATE2 (0.5, 0.3, 0.1) */
+    double hyper_grate_a, hyper_grate_b;          /* HYPER_GRATE_A/B (3, 1) */
+    double hyper_lrate_a, hyper_lrate_b;          /* HYPER_LRATE_A/B (1, 9) */
+    double hyper_lrate2_a, hyper_lrate2_b;        /* HYPER_LRATE2_A/B (1, 9); A = 0 -> Dollo */
+    double gap_prop;           /* GAP_PROP (0.8) */
+    double conserve_prop;      /* CONSERVE_PROP (0.8) */
+    double constomis;          /* CONSTOMIS (0.01) */
+    double trim_gap_percent;   /* TRIM_GAP_PERCENT (1 = off) */
+    int32_t min_len;           /* MIN_LEN (50) */
+    int32_t prune_tree;        /* PRUNE_TREE (0) */
+    int32_t gapchar;           /* GAPCHAR ('-') */
+    uint64_t seed;             /* SEED (5) */
+    int32_t r1_shared_uniform; /* 1 = the reference's r1 step (one uniform proposes and accepts,
+                                  src/PhyloAcc-ST/bpp_c.cpp:993,1010,1078); 0 = fresh uniform */
+    int32_t compress_patterns; /* 1 = prune over unique site patterns (default); 0 = one per site */
+} pa_options;
+
+/* Fills o with the reference defaults. */
+void pa_default_options(pa_options *o);
+
+/* Tree + substitution model in the reference's node numbering
+ * (src/PhyloAcc-ST/newick.cpp:382-428; bpp.cpp:84-117): leaves 0..S-1,
+ * internal S..N-1 in post-order, root N-1, children[i][0] < children[i][1]. */
+typedef struct {
+    int32_t S;                 /* species (leaves); N = 2S-1 */
+    const int32_t *children;   /* [N*2], -1 for leaves */
+    const int32_t *parent;     /* [N], -1 for the root */
+    const double *dist;        /* [N] branch length above each node */
+    const double *pi;          /* [4] BACKGROUND */
+    const double *Q;           /* [16] RATE_MAT, row = from state */
+    const int32_t *outgroup;   int32_t n_outgroup;   /* OUTGROUP species ids */
+    const int32_t *conserve;   int32_t n_conserve;   /* CONSERVE species ids */
+} pa_model;
+
+typedef struct pa_engine pa_engine;
+
+/* Parsed .mod file.  Replaces LoadPhyloTree (src/PhyloAcc-ST/newick.cpp:431-517):
+ * same numbering, float-rounded branch lengths, names of every node.
+ * names receives N NUL-terminated strings of at most 63 chars at stride 64. */
+pa_status pa_parse_mod(const char *text, int32_t max_nodes, int32_t *S, int32_t *children,
+                       int32_t *parent, double *dist, double *pi, double *Q, char *names);
+
+/* Engine over one tree + model + options on device `device`.
+ * Replaces the BPP constructor (src/PhyloAcc-ST/bpp.hpp:173-404: eigensystem of Q,
+ * outgroup/conserve ancestor sets) and BPP::InitMCMC (bpp.cpp:186-224). */
+pa_status pa_engine_create(const pa_model *model, const pa_options *opts, int32_t device,
+                           pa_engine **out);
+void pa_engine_destroy(pa_engine *e);
+const char *pa_last_error(const pa_engine *e);
+
+/* Encodes elements from an alignment whose S rows are in tree leaf order
+ * (BPP::MatchProfAndTree, bpp.cpp:133-184, is the caller's job).  Element i
+ * is columns [starts[i], starts[i]+lengths[i]) of every row.  Replaces the
+ * BPP_C constructor (src/PhyloAcc-ST/bpp_c.hpp:143-476): encoding, trimming,
+ * missing species, CONSERVE filter, outgroup-lineage collapse.  Writes one
+ * engine element id and one PA_ELEM_* status per element. */
+pa_status pa_elements_add(pa_engine *e, const char *const *rows, const int64_t *starts,
+                          const int32_t *lengths, int32_t n, int32_t *ids, int32_t *status);
+
+/* Host-side view of one encoded element (for checks and drop-in outputs). */
+typedef struct {
+    int32_t filtered, GG, P, root, n_nodes, n_internal;
+} pa_element_info;
+pa_status pa_element_get(const pa_engine *e, int32_t id, pa_element_info *info,
+                         int32_t *children2 /*[N*2] or NULL*/, int32_t *parent2 /*[N]*/,
+                         double *dist2 /*[N]*/, int32_t *missing /*[N]*/, int32_t *nodes /*[N]*/,
+                         uint8_t *site_codes /*[GG*S] or NULL*/, int32_t *site_pattern /*[GG]*/,
+                         int32_t *weights /*[P]*/);
+
+/* Per-element log-likelihood at fixed Z and rates, on the device.
+ * Replaces BPP_C::log_f_Xz(log_pi, num_base, Z, caches) (src/PhyloAcc-ST/bpp_c2.cpp:392-425)
+ * evaluated at log_cache_TM_cons/neut built for r1/r2 (bpp_c.cpp:187-228).
+ * Z: [n*N] int32 (0 neutral, 1 conserved, 2 accelerated, -1 outside the element's tree);
+ * rates: [n*2] (r1 conserved, r2 accelerated).  out: [n]. */
+pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *Z,
+                    const double *rates, int32_t precision, double *out);
+
+/* Results of one model chain of one element (BPP_C::Gibbs bookkeeping,
+ * src/PhyloAcc-ST/bpp_c.cpp:470-485). */
+typedef struct {
+    double max_loglik;          /* log_liks_Z[resZ] */
+    int32_t max_m;              /* Max_m */
+    int32_t status;             /* PA_ELEM_* */
+    double final_nrate, final_crate, final_grate, final_lrate, final_lrate2;
+    int32_t accept_n, accept_c; /* adaptive-MH counters at chain end */
+} pa_chain_result;
+
+/* Optional device-to-host trace copy, per element and model, T = BURNIN+MCMC. */
+typedef struct {
+    double *loglik, *full, *nrate, *crate, *grate, *lrate, *l2rate; /* [n*3*T] or NULL */
+    int8_t *Z;                                                       /* [n*3*T*N] or NULL */
+    int8_t *max_Z;                                                   /* [n*3*N] or NULL */
+} pa_traces;
+
+typedef struct {
+    int32_t precision;          /* PA_FP64 / PA_FP32 */
+    int32_t mode;               /* PA_RUN_BATCHED / PA_RUN_PER_ELEMENT */
+    int32_t block_threads;      /* 0 = autotuned */
+    int32_t keep_traces;        /* keep full traces on the device (required for pa_traces) */
+} pa_run_config;
+
+/* Runs the three model chains M0, M1, M2 (reference order resZ 0, 2, 1,
+ * src/PhyloAcc-ST/main.cpp:466-494) of every listed element on the device.
+ * Replaces the OpenMP element loop body (main.cpp:452-500: BPP_C::initMCMC +
+ * BPP_C::Gibbs per model).  results: [n*3] in run order (M0, M1, M2).
+ * elapsed_ms (may be NULL) receives the device time of the chain launches. */
+pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
+                        pa_chain_result *results, pa_traces *traces, double *elapsed_ms);
+
+/* Number of visible HIP devices (0 when none); no reference counterpart. */
+int32_t pa_device_count(void);
+
+/* Host-side Philox4x32-10 uniform for stream slot (c0..c3) word `lane`, the
+ * generator every device sampler draws from (no reference counterpart: the
+ * reference draws from GSL's mt19937, src/PhyloAcc-ST/bpp_c.hpp:146-147). */
+double pa_rng_uniform(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int32_t lane);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,229 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  It loads oracle/libpa_oracle.so, built by oracle/Makefile.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libpa_oracle.so")
+MAXN = 512
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("burnin", C.c_int), ("mcmc", C.c_int), ("adapt_freq", C.c_int),
+        ("init_crate", C.c_double), ("init_nrate", C.c_double),
+        ("cprior_a", C.c_double), ("cprior_b", C.c_double),
+        ("nprior_a", C.c_double), ("nprior_b", C.c_double),
+        ("ropt", C.c_int), ("cub", C.c_double), ("nlb", C.c_double),
+        ("init_grate", C.c_double), ("init_lrate", C.c_double), ("init_lrate2", C.c_double),
+        ("hyper_grate_a", C.c_double), ("hyper_grate_b", C.c_double),
+        ("hyper_lrate_a", C.c_double), ("hyper_lrate_b", C.c_double),
+        ("hyper_lrate2_a", C.c_double), ("hyper_lrate2_b", C.c_double),
+        ("gap_prop", C.c_double), ("conserve_prop", C.c_double),
+        ("constomis", C.c_double), ("nconstomis", C.c_double),
+        ("trim_gap_percent", C.c_double), ("min_len", C.c_int), ("prune_tree", C.c_int),
+        ("gapchar", C.c_int), ("seed", C.c_uint64), ("r1_shared_uniform", C.c_int),
+    ]
+
+
+def default_options(**kw) -> Options:
+    """Reference defaults (src/PhyloAcc-ST/main.cpp:35-90)."""
+    o = Options(burnin=200, mcmc=800, adapt_freq=500, init_crate=0.5, init_nrate=1.0,
+                cprior_a=5, cprior_b=0.04, nprior_a=10, nprior_b=0.2, ropt=1, cub=1, nlb=0.6,
+                init_grate=0.5, init_lrate=0.3, init_lrate2=0.1,
+                hyper_grate_a=3, hyper_grate_b=1, hyper_lrate_a=1, hyper_lrate_b=9,
+                hyper_lrate2_a=1, hyper_lrate2_b=9, gap_prop=0.8, conserve_prop=0.8,
+                constomis=0.01, nconstomis=1.0, trim_gap_percent=1.0, min_len=50,
+                prune_tree=0, gapchar=ord("-"), seed=5, r1_shared_uniform=1)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Model(C.Structure):
+    _fields_ = [
+        ("S", C.c_int), ("N", C.c_int),
+        ("children", (C.c_int * 2) * MAXN), ("parent", C.c_int * MAXN),
+        ("dist", C.c_double * MAXN), ("pi", C.c_double * 4), ("Q", C.c_double * 16),
+        ("upper", C.c_int * MAXN), ("upper_conserve", C.c_int * MAXN),
+        ("conserve", C.c_int * MAXN), ("n_conserve", C.c_int),
+    ]
+
+
+class ChainOut(C.Structure):
+    _fields_ = [
+        ("loglik", C.POINTER(C.c_double)), ("full", C.POINTER(C.c_double)),
+        ("nrate", C.POINTER(C.c_double)), ("crate", C.POINTER(C.c_double)),
+        ("grate", C.POINTER(C.c_double)), ("lrate", C.POINTER(C.c_double)),
+        ("l2rate", C.POINTER(C.c_double)), ("Z", C.POINTER(C.c_byte)),
+        ("postZ", C.POINTER(C.c_double)),
+        ("max_loglik", C.c_double), ("max_m", C.c_int), ("max_Z", C.c_int * MAXN),
+        ("vb", C.c_double), ("med", C.c_double * 5), ("failure", C.c_int),
+        ("accept_n", C.c_int), ("accept_c", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        L.or_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        L.or_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
+        L.or_uniform.restype = C.c_double
+        L.or_gamma_stream.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double]
+        L.or_gamma_stream.restype = C.c_double
+        L.or_beta_stream.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double]
+        L.or_beta_stream.restype = C.c_double
+        L.or_parse_mod.argtypes = [C.c_char_p, P(Model), C.c_char_p, C.c_int]
+        L.or_set_groups.argtypes = [P(Model), P(C.c_int), C.c_int, P(C.c_int), C.c_int]
+        L.or_expm.argtypes = [P(C.c_double), C.c_double, P(C.c_double)]
+        L.or_log_trans.argtypes = [P(Model), C.c_double, P(C.c_double)]
+        L.or_encode.argtypes = [P(Model), P(Options), P(C.c_char_p), C.c_long, C.c_int]
+        L.or_encode.restype = C.c_void_p
+        L.or_free.argtypes = [C.c_void_p]
+        L.or_elem_filtered.argtypes = [C.c_void_p]
+        L.or_elem_info.argtypes = [C.c_void_p, P(C.c_int)]
+        L.or_elem_arrays.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_double), P(C.c_int),
+                                     P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_uint8)]
+        L.or_loglik.argtypes = [P(Model), C.c_void_p, P(C.c_int), C.c_double, C.c_double]
+        L.or_loglik.restype = C.c_double
+        L.or_run_element.argtypes = [P(Model), P(Options), C.c_void_p, P(ChainOut)]
+        L.or_run_batch.argtypes = [P(Model), P(Options), P(C.c_void_p), C.c_int, C.c_int, P(C.c_double)]
+        L.or_run_batch.restype = C.c_long
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def parse_mod(text: str):
+    m = Model()
+    names = C.create_string_buffer(64 * MAXN)
+    rc = lib().or_parse_mod(text.encode(), C.byref(m), names, MAXN)
+    if rc != 0:
+        raise ValueError(f"or_parse_mod failed: {rc}")
+    N = m.N
+    nm = [names.raw[64 * i: 64 * (i + 1)].split(b"\0")[0].decode() for i in range(N)]
+    return m, nm
+
+
+def model_from_workload(w):
+    m, names = parse_mod(w.mod_text)
+    out = np.asarray(w.outgroup, np.int32)
+    cons = np.asarray(w.conserve, np.int32)
+    lib().or_set_groups(C.byref(m), _ptr(out, C.c_int), len(out), _ptr(cons, C.c_int), len(cons))
+    return m
+
+
+class Element:
+    def __init__(self, model, opts, align, start, length):
+        self.rows_keep = [np.ascontiguousarray(align[s]).tobytes() for s in range(align.shape[0])]
+        rows = (C.c_char_p * len(self.rows_keep))(*self.rows_keep)
+        self.model = model
+        self.ptr = lib().or_encode(C.byref(model), C.byref(opts), rows, int(start), int(length))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().or_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def filtered(self):
+        return bool(lib().or_elem_filtered(self.ptr))
+
+    def info(self):
+        o = (C.c_int * 8)()
+        lib().or_elem_info(self.ptr, o)
+        return dict(GG=o[0], root=o[1], n_nodes=o[2], n_internal=o[3], S=o[4], N=o[5], filtered=o[6])
+
+    def arrays(self):
+        inf = self.info()
+        N, GG, S = inf["N"], inf["GG"], inf["S"]
+        ch = np.zeros(2 * N, np.int32)
+        par = np.zeros(N, np.int32)
+        d = np.zeros(N, np.float64)
+        mis = np.zeros(N, np.int32)
+        nodes = np.zeros(max(inf["n_nodes"], 1), np.int32)
+        internal = np.zeros(max(inf["n_internal"], 1), np.int32)
+        Tg = np.zeros(max(GG * N, 1), np.int32)
+        codes = np.zeros(max(GG * S, 1), np.uint8)
+        lib().or_elem_arrays(self.ptr, _ptr(ch, C.c_int), _ptr(par, C.c_int), _ptr(d, C.c_double),
+                             _ptr(mis, C.c_int), _ptr(nodes, C.c_int), _ptr(internal, C.c_int),
+                             _ptr(Tg, C.c_int), _ptr(codes, C.c_uint8))
+        return dict(children=ch.reshape(N, 2), parent=par, dist=d, missing=mis,
+                    nodes=nodes[: inf["n_nodes"]], internal=internal[: inf["n_internal"]],
+                    Tg=Tg[: GG * N].reshape(GG, N), codes=codes[: GG * S].reshape(GG, S), **inf)
+
+    def loglik(self, Z, r1, r2):
+        Z = np.ascontiguousarray(Z, np.int32)
+        return lib().or_loglik(C.byref(self.model), self.ptr, _ptr(Z, C.c_int), float(r1), float(r2))
+
+    def run(self, opts, traces=True):
+        """Run M0, M1, M2 (reference run order resZ 0, 2, 1)."""
+        inf = self.info()
+        T = opts.burnin + opts.mcmc
+        N = inf["N"]
+        outs = (ChainOut * 3)()
+        keep = []
+        for k in range(3):
+            arrs = {}
+            if traces:
+                for f in ("loglik", "full", "nrate", "crate", "grate", "lrate", "l2rate"):
+                    a = np.zeros(T, np.float64)
+                    arrs[f] = a
+                    setattr(outs[k], f, _ptr(a, C.c_double))
+                z = np.zeros(T * N, np.int8)
+                arrs["Z"] = z
+                outs[k].Z = _ptr(z, C.c_byte)
+            pz = np.zeros(N * 4, np.float64)
+            arrs["postZ"] = pz
+            outs[k].postZ = _ptr(pz, C.c_double)
+            keep.append(arrs)
+        rc = lib().or_run_element(C.byref(self.model), C.byref(opts), self.ptr, outs)
+        if rc != 0:
+            return None
+        res = []
+        for k in range(3):
+            o = outs[k]
+            d = dict(max_loglik=o.max_loglik, max_m=o.max_m, max_Z=np.array(o.max_Z[:N]),
+                     vb=o.vb, med=np.array(o.med[:]), failure=o.failure,
+                     accept_n=o.accept_n, accept_c=o.accept_c)
+            for f, a in keep[k].items():
+                d[f] = a.reshape(T, N) if f == "Z" else (a.reshape(N, 4) if f == "postZ" else a)
+            res.append(d)
+        return res
+
+
+def run_batch(model, opts, elements, nthreads=0):
+    arr = (C.c_void_p * len(elements))(*[e.ptr for e in elements])
+    ml = np.zeros(3 * len(elements), np.float64)
+    n = lib().or_run_batch(C.byref(model), C.byref(opts), arr, len(elements), nthreads, _ptr(ml, C.c_double))
+    return n, ml.reshape(-1, 3)

@@ -1,0 +1,1176 @@
+/*
+ * pa_oracle.c -- CPU restatement of PhyloAcc-ST's per-element MCMC path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline).  Nothing in
+ * phyloacc_amd/ links or calls this file.
+ *
+ * Everything is FP64, log space and one alignment site at a time, as in the
+ * reference.  Each function cites the reference lines (src/PhyloAcc-ST/...)
+ * it restates.  Departures, all documented where they occur:
+ *   - P(t) = expm(Q t) is computed by scaling-and-squaring Taylor series
+ *     rather than Armadillo's eig_gen (same function, independent method);
+ *     entries are clamped at 0 before the log.
+ *   - Random numbers: counter-based Philox4x32-10 with a fixed slot layout
+ *     instead of GSL's mt19937 stream; samplers: categorical by inverse CDF,
+ *     Gamma by Marsaglia-Tsang (2000) with Box-Muller normals, Beta as a
+ *     ratio of Gammas.  Gamma log-densities are closed forms in log space.
+ */
+#include "pa_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include <ctype.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NEG_INF (-INFINITY)
+
+/* ------------------------------------------------------------------ RNG -- */
+
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    /* Salmon, Moraes, Dror, Shaw (SC'11), Philox4x32 with 10 rounds. */
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline double u01(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+
+static inline uint32_t tagword(int model, int tag) { return ((uint32_t)model << 24) | ((uint32_t)tag << 16); }
+
+double or_uniform(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int lane)
+{
+    uint32_t ctr[4] = {c0, c1, c2, c3}, key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)}, o[4];
+    or_philox4x32_10(ctr, key, o);
+    return u01(o[lane & 3]);
+}
+
+/* uniform for a fixed slot t of sub-stream c1 */
+static inline double slot_u(uint64_t seed, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t t)
+{
+    return or_uniform(seed, t >> 2, c1, c2, c3, (int)(t & 3));
+}
+
+typedef struct { uint64_t seed; uint32_t c1, c2, c3, idx; } stream_t;
+
+static inline double st_next(stream_t *s) { double u = slot_u(s->seed, s->c1, s->c2, s->c3, s->idx); s->idx++; return u; }
+
+#define OR_GAMMA_MAXIT 64
+/* Marsaglia & Tsang (2000), shape a > 0, scale b; for a < 1 the boost
+ * U^(1/a) uses draw 0 and the main loop starts at draw 1. */
+static double gamma_draw(stream_t *s, double a, double b)
+{
+    double boost = 1.0;
+    if (a < 1.0) { double u = st_next(s); boost = pow(u, 1.0 / a); a += 1.0; }
+    double d = a - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
+    for (int it = 0; it < OR_GAMMA_MAXIT; it++) {
+        double u1 = st_next(s), u2 = st_next(s);
+        double x = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+        double v = 1.0 + c * x;
+        if (v <= 0.0) continue;
+        v = v * v * v;
+        double u = st_next(s);
+        if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v * b * boost;
+        if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v * b * boost;
+    }
+    return d * b * boost;
+}
+
+double or_gamma_stream(uint64_t seed, uint32_t c1, uint32_t c2, uint32_t c3, double shape, double scale)
+{
+    stream_t s = {seed, c1, c2, c3, 0};
+    return gamma_draw(&s, shape, scale);
+}
+
+double or_beta_stream(uint64_t seed, uint32_t c1x, uint32_t c1y, uint32_t c2, uint32_t c3, double a, double b)
+{
+    stream_t sx = {seed, c1x, c2, c3, 0}, sy = {seed, c1y, c2, c3, 0};
+    double x = gamma_draw(&sx, a, 1.0), y = gamma_draw(&sy, b, 1.0);
+    return x / (x + y);
+}
+
+/* categorical over k unnormalised non-negative weights, inverse CDF */
+static int categorical(const double *p, int k, double u)
+{
+    double tot = 0; int last = 0;
+    for (int i = 0; i < k; i++) { tot += p[i]; if (p[i] > 0) last = i; }
+    double target = u * tot, cum = 0;
+    for (int i = 0; i < k - 1; i++) { cum += p[i]; if (target < cum) return i; }
+    return last;
+}
+
+/* log of gsl_ran_gamma_pdf(x; a, scale b), closed form */
+static double log_gamma_pdf(double x, double a, double b)
+{
+    if (x < 0) return NEG_INF;
+    if (x == 0) { if (a == 1) return -log(b); return a > 1 ? NEG_INF : INFINITY; }
+    return (a - 1) * log(x) - x / b - lgamma(a) - a * log(b);
+}
+
+static double lnbeta(double a, double b) { return lgamma(a) + lgamma(b) - lgamma(a + b); }
+
+/* ---------------------------------------------------------- .mod text -- */
+/* Restates LoadPhyloTree (newick.cpp:431-517), parseTree (newick.cpp:145-331)
+ * and the TravelTree2/3 numbering (newick.cpp:382-428): post-order; leaves
+ * 0..S-1 in encounter order, internal nodes S.. in post-order, root N-1;
+ * children ordered by id (bpp.cpp:105-117); branch lengths rounded to float. */
+
+typedef struct pnode { int leaf; int kids[2]; int nk; double dist; char name[64]; int id; } pnode;
+
+typedef struct { const char *s; int pos; pnode *nodes; int n, cap; int err; } pstate;
+
+static int pnew(pstate *p)
+{
+    if (p->n >= p->cap) { p->err = 1; return 0; }
+    memset(&p->nodes[p->n], 0, sizeof(pnode));
+    return p->n++;
+}
+
+/* parse one subtree in s[a..b) */
+static int parse_sub(pstate *p, int a, int b)
+{
+    const char *s = p->s;
+    int me = pnew(p); if (p->err) return 0;
+    if (s[a] != '(') {
+        int colon = -1;
+        for (int i = a; i < b; i++) if (s[i] == ':') colon = i;
+        int e = colon < 0 ? b : colon;
+        int L = e - a; if (L > 63) L = 63;
+        memcpy(p->nodes[me].name, s + a, L); p->nodes[me].name[L] = 0;
+        p->nodes[me].dist = colon < 0 ? 0.0 : (double)(float)atof(s + colon + 1);
+        p->nodes[me].leaf = 1;
+        return me;
+    }
+    int depth = 0, start = a + 1, i;
+    for (i = a; i < b; i++) {
+        char c = s[i];
+        if (c == '(') depth++;
+        else if (c == ')') { depth--; if (depth == 0) break; }
+        if ((c == ',' && depth == 1)) {
+            if (p->nodes[me].nk >= 2) { p->err = 2; return me; }
+            int k = parse_sub(p, start, i); p->nodes[me].kids[p->nodes[me].nk++] = k; start = i + 1;
+        }
+    }
+    if (i >= b) { p->err = 3; return me; }
+    if (p->nodes[me].nk >= 2) { p->err = 2; return me; }
+    { int k = parse_sub(p, start, i); p->nodes[me].kids[p->nodes[me].nk++] = k; }
+    if (p->nodes[me].nk != 2) { p->err = 2; return me; }
+    /* label and distance after ')' */
+    int j = i + 1, colon = -1;
+    for (int q = j; q < b && s[q] != ';'; q++) if (s[q] == ':') { colon = q; break; }
+    int e = colon < 0 ? j : colon;
+    if (colon < 0) { e = j; while (e < b && s[e] != ';') e++; }
+    int L = e - j; if (L > 63) L = 63;
+    memcpy(p->nodes[me].name, s + j, L); p->nodes[me].name[L] = 0;
+    if (colon >= 0) {
+        p->nodes[me].dist = (double)(float)atof(s + colon + 1);
+    }
+    return me;
+}
+
+static void number_post(pstate *p, int v, int *leaf_id, int *int_id, int S)
+{
+    pnode *n = &p->nodes[v];
+    for (int k = 0; k < n->nk; k++) number_post(p, n->kids[k], leaf_id, int_id, S);
+    if (n->leaf) n->id = (*leaf_id)++;
+    else n->id = S + (*int_id)++;
+}
+
+static void trim(char *t)
+{
+    int L = (int)strlen(t), a = 0;
+    while (a < L && (t[a] == ' ' || t[a] == '\t' || t[a] == '\n')) a++;
+    while (L > a && (t[L - 1] == ' ' || t[L - 1] == '\t' || t[L - 1] == '\n')) L--;
+    memmove(t, t + a, L - a); t[L - a] = 0;
+}
+
+int or_parse_mod(const char *text, or_model *m, char *names, int max_names)
+{
+    memset(m, 0, sizeof(*m));
+    char *buf = strdup(text); if (!buf) return -1;
+    char *lines[4096]; int nl = 0;
+    for (char *q = buf; nl < 4096;) {
+        lines[nl++] = q;
+        char *e = strchr(q, '\n');
+        if (!e) break;
+        *e = 0; q = e + 1;
+    }
+    char *tree = NULL;
+    for (int li = 0; li < nl; li++) {
+        if (lines[li][0] == 0) break;           /* newick.cpp:457 stops at the first empty line */
+        trim(lines[li]);
+        char first[64] = {0}; sscanf(lines[li], "%63s", first);
+        if (lines[li][0] == ' ' || lines[li][0] == 0) continue;
+        if (!strcmp(first, "BACKGROUND:")) {
+            char *q = lines[li] + strlen("BACKGROUND:"); int ind = 0;
+            while (ind < 4) { char *e; double v = strtod(q, &e); if (e == q) break; m->pi[ind++] = v; q = e; }
+        } else if (!strcmp(first, "RATE_MAT:")) {
+            for (int i = 0; i < 4 && li + 1 < nl; i++) {
+                li++; char *q = lines[li]; int ind = 0;
+                while (ind < 4) { char *e; double v = strtod(q, &e); if (e == q) break; m->Q[i * 4 + ind++] = v; q = e; }
+            }
+        } else if (!strcmp(first, "TREE:")) {
+            char *q = lines[li] + 5; while (*q == ' ') q++;
+            char *e = q; while (*e && *e != ' ') e++; *e = 0;
+            tree = q;
+        }
+    }
+    if (!tree) { free(buf); return -2; }
+    int L = (int)strlen(tree);
+    pstate p = {tree, 0, (pnode *)calloc(2 * OR_MAXN, sizeof(pnode)), 0, 2 * OR_MAXN, 0};
+    int end = L; while (end > 0 && tree[end - 1] == ';') end--;
+    int root = parse_sub(&p, 0, end);
+    if (p.err) { free(p.nodes); free(buf); return -3; }
+    int S = 0; for (int i = 0; i < p.n; i++) S += p.nodes[i].leaf;
+    int N = 2 * S - 1;
+    if (N > OR_MAXN || p.n != N) { free(p.nodes); free(buf); return -4; }
+    int lid = 0, iid = 0; number_post(&p, root, &lid, &iid, S);
+    m->S = S; m->N = N;
+    for (int i = 0; i < N; i++) { m->children[i][0] = m->children[i][1] = -1; m->parent[i] = -1; }
+    for (int v = 0; v < p.n; v++) {
+        pnode *n = &p.nodes[v]; int id = n->id;
+        m->dist[id] = n->dist;
+        if (names && id < max_names) { strncpy(names + 64 * id, n->name, 63); names[64 * id + 63] = 0; }
+        if (!n->leaf) {
+            int a = p.nodes[n->kids[0]].id, b = p.nodes[n->kids[1]].id;
+            if (a > b) { int t = a; a = b; b = t; }
+            m->children[id][0] = a; m->children[id][1] = b;
+            m->parent[a] = id; m->parent[b] = id;
+        }
+    }
+    free(p.nodes); free(buf);
+    return 0;
+}
+
+/* BPP::getUppertree (bpp.cpp:486-504): the species themselves, their
+ * ancestors, and the root. */
+void or_set_groups(or_model *m, const int *outgroup, int n_out, const int *conserve, int n_cons)
+{
+    int root = m->N - 1;
+    memset(m->upper, 0, sizeof(m->upper)); memset(m->upper_conserve, 0, sizeof(m->upper_conserve));
+    for (int i = 0; i < n_out; i++) for (int p = outgroup[i]; p != root && p >= 0; p = m->parent[p]) m->upper[p] = 1;
+    m->upper[root] = 1;
+    for (int i = 0; i < n_cons; i++) for (int p = conserve[i]; p != root && p >= 0; p = m->parent[p]) m->upper_conserve[p] = 1;
+    m->upper_conserve[root] = 1;
+    m->n_conserve = n_cons;
+    for (int i = 0; i < n_cons; i++) m->conserve[i] = conserve[i];
+}
+
+/* ------------------------------------------------ transition matrices -- */
+
+static void mat4_mul(const double *A, const double *B, double *C)
+{
+    double T[16];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) {
+        double s = 0; for (int k = 0; k < 4; k++) s += A[i * 4 + k] * B[k * 4 + j];
+        T[i * 4 + j] = s;
+    }
+    memcpy(C, T, sizeof T);
+}
+
+/* expm(Q t) by scaling and squaring with a degree-24 Taylor sum */
+void or_expm(const double *Q, double t, double *P)
+{
+    double A[16], nrm = 0;
+    for (int i = 0; i < 16; i++) A[i] = Q[i] * t;
+    for (int i = 0; i < 4; i++) { double r = 0; for (int j = 0; j < 4; j++) r += fabs(A[i * 4 + j]); if (r > nrm) nrm = r; }
+    int sq = 0;
+    while (nrm > 0.125) { nrm *= 0.5; sq++; }
+    double sc = ldexp(1.0, -sq);
+    for (int i = 0; i < 16; i++) A[i] *= sc;
+    double term[16], sum[16];
+    for (int i = 0; i < 16; i++) { term[i] = (i % 5 == 0) ? 1.0 : 0.0; sum[i] = term[i]; }
+    for (int k = 1; k <= 24; k++) {
+        mat4_mul(term, A, term);
+        for (int i = 0; i < 16; i++) { term[i] /= k; sum[i] += term[i]; }
+    }
+    for (int s = 0; s < sq; s++) mat4_mul(sum, sum, sum);
+    memcpy(P, sum, sizeof sum);
+}
+
+/* log((V e^{L t} V^-1)^T), bpp.hpp:370-397 and bpp_c.cpp:187-228:
+ * logPt[i*4+j] = log P_{j->i}(t).  Used for t > threshold only by callers. */
+void or_log_trans(const or_model *m, double t, double *logPt)
+{
+    double P[16];
+    or_expm(m->Q, t, P);
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) {
+        double v = P[j * 4 + i]; if (v < 0) v = 0;
+        logPt[i * 4 + j] = log(v);
+    }
+}
+
+static void log_identity(double *L)
+{
+    for (int i = 0; i < 16; i++) L[i] = (i % 5 == 0) ? 0.0 : NEG_INF;
+}
+
+/* ------------------------------------------------------ log helpers -- */
+
+/* BPP::log_multi (bpp.hpp:427-445): out_j = log sum_i exp(L(i,j) + y_i),
+ * one global max over the 4x4 */
+static void log_multi(const double *L, const double *y, double *out)
+{
+    double x[16], mx = NEG_INF;
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) {
+        x[i * 4 + j] = L[i * 4 + j] + y[i];
+        if (x[i * 4 + j] > mx) mx = x[i * 4 + j];
+    }
+    if (mx == NEG_INF) { for (int j = 0; j < 4; j++) out[j] = NEG_INF; return; }
+    for (int j = 0; j < 4; j++) {
+        double s = 0; for (int i = 0; i < 4; i++) s += exp(x[i * 4 + j] - mx);
+        out[j] = log(s) + mx;
+    }
+}
+
+/* BPP::log_multi2 (bpp.hpp:447-460): per-column max, 3x3 */
+static void log_multi2(const double *L, const double *y, double *out)
+{
+    for (int j = 0; j < 3; j++) {
+        double x[3], mx = NEG_INF;
+        for (int i = 0; i < 3; i++) { x[i] = L[i * 3 + j] + y[i]; if (x[i] > mx) mx = x[i]; }
+        double s = 0; for (int i = 0; i < 3; i++) s += exp(x[i] - mx);
+        out[j] = log(s) + mx;
+    }
+}
+
+/* BPP::log_exp_sum (bpp.hpp:471-479) */
+static double log_exp_sum(const double *y, int k)
+{
+    double mx = NEG_INF;
+    for (int i = 0; i < k; i++) if (y[i] > mx) mx = y[i];
+    if (mx == NEG_INF) return NEG_INF;
+    double s = 0; for (int i = 0; i < k; i++) s += exp(y[i] - mx);
+    return log(s) + mx;
+}
+
+/* ----------------------------------------------------------- element -- */
+
+struct or_element {
+    int filtered, GG, S, N, root;
+    int *Tg;                 /* [GG*N] */
+    double *lam;             /* [GG*N*4] */
+    unsigned char *code;     /* [GG*S] */
+    int missing[OR_MAXN];
+    int ch2[OR_MAXN][2], par2[OR_MAXN];
+    double dist2[OR_MAXN];
+    int nodes[OR_MAXN], n_nodes, inodes[OR_MAXN], n_inodes;
+    int upper_c[OR_MAXN], upper_cons_c[OR_MAXN];   /* flags */
+    double log_emission[OR_MAXN][3];
+    /* chain state that persists across the three models of one element */
+    int Z[OR_MAXN], fixZ[OR_MAXN];
+    double TM[OR_MAXN][9];   /* log_TM_Int, [i*3+j] = log P(child i | parent j) */
+    double Lnull[OR_MAXN][16], Lcons[OR_MAXN][16], Lneut[OR_MAXN][16];
+    double *amb_null;        /* [GG*S*4] */
+    int accept_n, accept_c;
+    double prop_n, prop_c;
+};
+
+static void encode_char(int y, int gapchar, double *lam, int *Tg, unsigned char *code)
+{
+    /* bpp_c.hpp:213-261 */
+    for (int b = 0; b < 4; b++) lam[b] = NEG_INF;
+    switch (y) {
+    case 'a': lam[0] = 0; *Tg = 0; *code = OR_A; break;
+    case 'c': lam[1] = 0; *Tg = 1; *code = OR_C; break;
+    case 'g': lam[2] = 0; *Tg = 2; *code = OR_G; break;
+    case 't': lam[3] = 0; *Tg = 3; *code = OR_T; break;
+    case 'r': lam[0] = 0; lam[2] = 0; *Tg = -1; *code = OR_R; break;
+    case 'y': lam[1] = 0; lam[3] = 0; *Tg = -1; *code = OR_Y; break;
+    case 'k': lam[2] = 0; lam[3] = 0; *Tg = -1; *code = OR_K; break;
+    case 'm': lam[0] = 0; lam[1] = 0; *Tg = -1; *code = OR_M; break;
+    case 's': lam[1] = 0; lam[2] = 0; *Tg = -1; *code = OR_S; break;
+    case 'w': lam[0] = 0; lam[3] = 0; *Tg = -1; *code = OR_W; break;
+    default:
+        for (int b = 0; b < 4; b++) lam[b] = 0;
+        if (y == gapchar) { *Tg = 4; *code = OR_GAP; } else { *Tg = 5; *code = OR_OTHER; }
+    }
+}
+
+/* BPP_C::getSubtree_missing (bpp_c.cpp:64-113) */
+static void subtree_missing(or_element *e, int ss, const int *upper, int child)
+{
+    if (!upper[ss] || ss < e->S) return;
+    int s = ss;
+    while (upper[s] && s >= e->S) {
+        int p0 = e->ch2[s][0], p1 = e->ch2[s][1];
+        if (e->missing[p0] && !e->missing[p1]) {
+            e->par2[p0] = -1;
+            e->dist2[p1] += e->dist2[s];
+            if (s == e->root) e->root = p1;
+            else { e->ch2[e->par2[ss]][child] = p1; e->par2[p1] = e->par2[ss]; }
+            s = p1;
+        } else if (!e->missing[p0] && e->missing[p1]) {
+            e->par2[p1] = -1;
+            e->dist2[p0] += e->dist2[s];
+            if (s == e->root) e->root = p0;
+            else { e->ch2[e->par2[ss]][child] = p0; e->par2[p0] = e->par2[ss]; }
+            s = p0;
+        } else {
+            subtree_missing(e, p0, upper, 0);
+            subtree_missing(e, p1, upper, 1);
+            return;
+        }
+    }
+}
+
+/* BPP_C::getSubtree (bpp_c.cpp:44-60): post-order via children2 */
+static void post_order(or_element *e, int j)
+{
+    for (int c = 0; c < 2; c++) if (e->ch2[j][c] != -1) post_order(e, e->ch2[j][c]);
+    e->nodes[e->n_nodes++] = j;
+}
+
+/* BPP_C constructor (bpp_c.hpp:143-476) for element rows[s][start..start+GG) */
+or_element *or_encode(const or_model *m, const or_options *o, const char *const *rows, long start, int GG)
+{
+    or_element *e = (or_element *)calloc(1, sizeof(or_element));
+    int S = m->S, N = m->N;
+    e->S = S; e->N = N; e->GG = GG; e->root = N - 1;
+    if (GG < o->min_len) { e->filtered = 1; return e; }
+    int *Tg = (int *)malloc(sizeof(int) * (size_t)GG * N);
+    double *lam = (double *)malloc(sizeof(double) * (size_t)GG * N * 4);
+    unsigned char *code = (unsigned char *)malloc((size_t)GG * S);
+    for (size_t i = 0; i < (size_t)GG * N; i++) Tg[i] = -1;
+    for (int s = 0; s < S; s++) for (int g = 0; g < GG; g++) {
+        int y = tolower((unsigned char)rows[s][start + g]);
+        encode_char(y, o->gapchar, lam + ((size_t)g * N + s) * 4, Tg + (size_t)g * N + s, code + (size_t)g * S + s);
+    }
+    /* trimming, bpp_c.hpp:268-301 */
+    if (o->trim_gap_percent < 1) {
+        int keep = 0;
+        char *drop = (char *)calloc(GG, 1);
+        for (int g = 0; g < GG; g++) {
+            int mis = 0;
+            for (int s = 0; s < S; s++) if (Tg[(size_t)g * N + s] >= 4) mis++;
+            if (mis > S * o->trim_gap_percent) drop[g] = 1; else keep++;
+        }
+        if (keep < o->min_len) { free(drop); free(Tg); free(lam); free(code); e->filtered = 1; return e; }
+        int w = 0;
+        for (int g = 0; g < GG; g++) {
+            if (drop[g]) continue;
+            if (w != g) {
+                memmove(Tg + (size_t)w * N, Tg + (size_t)g * N, sizeof(int) * N);
+                memmove(lam + (size_t)w * N * 4, lam + (size_t)g * N * 4, sizeof(double) * N * 4);
+                memmove(code + (size_t)w * S, code + (size_t)g * S, S);
+            }
+            w++;
+        }
+        free(drop);
+        GG = keep; e->GG = GG;
+    }
+    /* gap-only counts and missing flags, bpp_c.hpp:304-351 */
+    int num_missing[OR_MAXN] = {0};
+    for (int s = 0; s < S; s++) for (int g = 0; g < GG; g++) if (Tg[(size_t)g * N + s] == 4) num_missing[s]++;
+    for (int i = 0; i < N; i++) {
+        e->ch2[i][0] = m->children[i][0]; e->ch2[i][1] = m->children[i][1];
+        e->par2[i] = m->parent[i]; e->dist2[i] = m->dist[i];
+    }
+    for (int s = S; s < N; s++) {
+        for (int c = 0; c < 2; c++) {
+            int chi = e->ch2[s][c];
+            if (chi < S && num_missing[chi] > o->gap_prop * GG) e->missing[chi] = 1;
+        }
+        if (e->missing[e->ch2[s][0]] && e->missing[e->ch2[s][1]]) e->missing[s] = 1;
+    }
+    /* CONSERVE filter, bpp_c.hpp:361-373 */
+    int ct = 0;
+    for (int i = 0; i < m->n_conserve; i++) if (e->missing[m->conserve[i]]) ct++;
+    if (ct > o->conserve_prop * m->n_conserve) { free(Tg); free(lam); free(code); e->filtered = 1; return e; }
+    /* collapse, bpp_c.hpp:379-390 */
+    if (o->prune_tree) { int all[OR_MAXN]; for (int i = 0; i < N; i++) all[i] = 1; subtree_missing(e, N - 1, all, -1); }
+    else subtree_missing(e, N - 1, m->upper, -1);
+    e->par2[e->root] = N;
+    post_order(e, e->root);
+    for (int i = 0; i < e->n_nodes; i++) {
+        int s = e->nodes[i];
+        if (s >= S) e->inodes[e->n_inodes++] = s;
+        e->upper_c[s] = m->upper[s];
+        e->upper_cons_c[s] = m->upper_conserve[s];
+    }
+    /* internal nodes whose children are both >= num_base, bpp_c.hpp:417-428 */
+    for (int k = 0; k < e->n_inodes; k++) {
+        int s = e->inodes[k], a = e->ch2[s][0], b = e->ch2[s][1];
+        for (int g = 0; g < GG; g++)
+            if (Tg[(size_t)g * N + a] >= 4 && Tg[(size_t)g * N + b] >= 4) Tg[(size_t)g * N + s] = 4;
+    }
+    /* missing-node emissions, bpp_c.hpp:451-474 */
+    for (int s = 0; s < N; s++) {
+        e->log_emission[s][0] = e->log_emission[s][1] = e->log_emission[s][2] = 0;
+        if (e->missing[s]) {
+            e->log_emission[s][2] = log(o->nconstomis);
+            e->log_emission[s][1] = log(o->constomis);
+            e->log_emission[s][0] = log(o->nconstomis);
+        }
+    }
+    for (int s = 0; s < N; s++) for (int k = 0; k < 9; k++) e->TM[s][k] = 0;   /* zeros<mat>(3,3) */
+    e->Tg = Tg; e->lam = lam; e->code = code;
+    e->amb_null = (double *)calloc((size_t)GG * S * 4, sizeof(double));
+    return e;
+}
+
+void or_free(or_element *e)
+{
+    if (!e) return;
+    free(e->Tg); free(e->lam); free(e->code); free(e->amb_null); free(e);
+}
+
+int or_elem_filtered(const or_element *e) { return e->filtered; }
+
+int or_elem_info(const or_element *e, int *out)
+{
+    out[0] = e->GG; out[1] = e->root; out[2] = e->n_nodes; out[3] = e->n_inodes;
+    out[4] = e->S; out[5] = e->N; out[6] = e->filtered; out[7] = 0;
+    return 0;
+}
+
+void or_elem_arrays(const or_element *e, int *children2, int *parent2, double *dist2,
+                    int *missing, int *nodes, int *internal, int *Tg, unsigned char *codes)
+{
+    int N = e->N;
+    for (int i = 0; i < N; i++) {
+        if (children2) { children2[2 * i] = e->ch2[i][0]; children2[2 * i + 1] = e->ch2[i][1]; }
+        if (parent2) parent2[i] = e->par2[i];
+        if (dist2) dist2[i] = e->dist2[i];
+        if (missing) missing[i] = e->missing[i];
+    }
+    if (nodes) for (int i = 0; i < e->n_nodes; i++) nodes[i] = e->nodes[i];
+    if (internal) for (int i = 0; i < e->n_inodes; i++) internal[i] = e->inodes[i];
+    if (!e->filtered) {
+        if (Tg) memcpy(Tg, e->Tg, sizeof(int) * (size_t)e->GG * N);
+        if (codes) memcpy(codes, e->code, (size_t)e->GG * e->S);
+    }
+}
+
+#define LAM(e, g, s) ((e)->lam + (((size_t)(g) * (e)->N + (s)) * 4))
+#define TG(e, g, s) ((e)->Tg[(size_t)(g) * (e)->N + (s)])
+
+/* class of rate on branch above s: Z=2 -> neut (r2), Z=0 -> null (1), else cons (r1) */
+static const double *Lclass(const or_element *e, int s)
+{
+    int z = e->Z[s];
+    return z == 2 ? e->Lneut[s] : (z == 0 ? e->Lnull[s] : e->Lcons[s]);
+}
+
+/* BPP_C::Update_Tg(tosample=false) for one site, bpp_c.cpp:647-671 */
+static void prune_site(or_element *e, int g, const char *visited, const double *log_pi)
+{
+    for (int k = 0; k < e->n_inodes; k++) {
+        int s = e->inodes[k];
+        if (visited[s] || e->missing[s] || TG(e, g, s) >= 4) continue;
+        double *ls = LAM(e, g, s);
+        for (int c = 0; c < 2; c++) {
+            int chi = e->ch2[s][c];
+            if (e->missing[chi] || TG(e, g, chi) >= 4) continue;
+            double o[4]; log_multi(Lclass(e, chi), LAM(e, g, chi), o);
+            for (int j = 0; j < 4; j++) ls[j] += o[j];
+        }
+    }
+    if (!visited[e->root]) { double *lr = LAM(e, g, e->root); for (int j = 0; j < 4; j++) lr[j] += log_pi[j]; }
+}
+
+static void zero_internal(or_element *e)
+{
+    for (int g = 0; g < e->GG; g++) for (int s = e->S; s < e->N; s++) { double *l = LAM(e, g, s); l[0] = l[1] = l[2] = l[3] = 0; }
+}
+
+static void model_caches(const or_model *m, or_element *e, double nrate, double crate)
+{
+    /* BPP null cache (bpp.hpp:366-397) then BPP_C::initMCMC caches (bpp_c.cpp:180-228) */
+    for (int s = 0; s < e->N; s++) {
+        if (s < e->N - 1 && m->dist[s] > 1e-10) or_log_trans(m, m->dist[s], e->Lnull[s]);
+        else if (s < e->N - 1) log_identity(e->Lnull[s]);
+        else for (int i = 0; i < 16; i++) e->Lnull[s][i] = 0;
+    }
+    for (int k = 0; k < e->n_nodes - 1; k++) {
+        int s = e->nodes[k];
+        if (e->dist2[s] > 1e-10) {
+            if (e->dist2[s] != m->dist[s]) or_log_trans(m, e->dist2[s], e->Lnull[s]);
+            or_log_trans(m, e->dist2[s] * nrate, e->Lneut[s]);
+            or_log_trans(m, e->dist2[s] * crate, e->Lcons[s]);
+        } else {
+            memcpy(e->Lcons[s], e->Lnull[s], sizeof(double) * 16);
+            memcpy(e->Lneut[s], e->Lnull[s], sizeof(double) * 16);
+        }
+    }
+}
+
+double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, double r2)
+{
+    /* log_f_Xz full form, bpp_c2.cpp:392-425, with caches at (r2, r1) */
+    if (e->filtered) return NAN;
+    model_caches(m, e, r2, r1);
+    for (int s = 0; s < e->N; s++) e->Z[s] = Z[s];
+    zero_internal(e);
+    double logpi[4]; for (int j = 0; j < 4; j++) logpi[j] = log(m->pi[j]);
+    char visited[OR_MAXN] = {0};
+    double res = 0;
+    for (int g = 0; g < e->GG; g++) {
+        prune_site(e, g, visited, logpi);
+        res += log_exp_sum(LAM(e, g, e->root), 4);
+    }
+    return res;
+}
+
+/* ------------------------------------------------------------- chain -- */
+
+typedef struct {
+    const or_model *m; const or_options *o; or_element *e;
+    int resZ, ms, T, m_cur;
+    double logpi[4];
+    double *tl, *tf, *tn, *tc, *tg, *tlr, *tl2; signed char *tZ;
+    double MaxLoglik; int Max_m; int Max_Z[OR_MAXN];
+} chain_t;
+
+static void set_TM(or_element *e, int s, double g, double l, double l2, int init)
+{
+    double *T = e->TM[s];
+    if (e->fixZ[s] == 1) {
+        if (init) T[0 * 3 + 1] = NEG_INF;
+        T[1 * 3 + 1] = 0; T[2 * 3 + 1] = NEG_INF;
+        T[0 * 3 + 0] = log(1 - g); T[1 * 3 + 0] = log(g); T[2 * 3 + 0] = NEG_INF;
+    } else {
+        T[0 * 3 + 0] = log(1 - g); T[1 * 3 + 0] = log(g); T[2 * 3 + 0] = NEG_INF;
+        double y = 1 - l;
+        if (init) T[0 * 3 + 1] = NEG_INF;
+        T[1 * 3 + 1] = log(y); T[2 * 3 + 1] = log(1 - y);
+        if (init) T[0 * 3 + 2] = NEG_INF;
+        T[1 * 3 + 2] = log(l2); T[2 * 3 + 2] = log(1 - l2);
+    }
+}
+
+/* BPP_C::initMCMC (bpp_c.cpp:131-343), iter == 0 */
+static void init_mcmc(chain_t *c)
+{
+    or_element *e = c->e; const or_options *o = c->o;
+    e->prop_c = 1.2;
+    e->prop_n = (double)1 / e->GG * 10;
+    for (int s = 0; s < e->N; s++) e->Z[s] = -1;
+    for (int k = 0; k < e->n_nodes; k++) e->Z[e->nodes[k]] = 1;
+    e->Z[e->root] = 0;
+    for (int s = 0; s < e->N; s++) e->fixZ[s] = -1;
+    for (int k = 0; k < e->n_nodes; k++) if (e->upper_c[e->nodes[k]]) e->fixZ[e->nodes[k]] = 1;
+    e->fixZ[e->root] = 1;
+    c->MaxLoglik = NEG_INF; c->Max_m = 0;
+    double lr2 = o->hyper_lrate2_a == 0 ? 0 : o->init_lrate2;
+    c->tn[0] = o->init_nrate; c->tc[0] = o->init_crate;
+    c->tlr[0] = o->init_lrate; c->tl2[0] = lr2; c->tg[0] = o->init_grate;
+    model_caches(c->m, e, o->init_nrate, o->init_crate);
+    zero_internal(e);
+    /* getEmission_ambig, bpp_c.cpp:116-128 */
+    for (int s = 0; s < e->S; s++) {
+        if (e->missing[s]) continue;
+        for (int g = 0; g < e->GG; g++) if (TG(e, g, s) == -1) log_multi(e->Lnull[s], LAM(e, g, s), e->amb_null + ((size_t)g * e->S + s) * 4);
+    }
+    if (c->resZ == 2) {
+        for (int k = 0; k < e->n_nodes; k++) if (e->upper_cons_c[e->nodes[k]]) e->fixZ[e->nodes[k]] = 1;
+    } else if (c->resZ == 0) {
+        for (int k = 0; k < e->n_nodes - 1; k++) e->fixZ[e->nodes[k]] = 1;
+    }
+    for (int k = 0; k < e->n_nodes; k++) set_TM(e, e->nodes[k], c->tg[0], c->tlr[0], c->tl2[0], 1);
+}
+
+/* BPP_C::MonitorChain (bpp_c.cpp:741-834) */
+static void monitor(chain_t *c, int m, double *loglik)
+{
+    or_element *e = c->e; const or_options *o = c->o;
+    for (int s = 0; s < e->N; s++) c->tZ[(size_t)m * e->N + s] = (signed char)e->Z[s];
+    if (m == 0) {
+        double t = 0;
+        for (int g = 0; g < e->GG; g++) t += log_exp_sum(LAM(e, g, e->root), 4);
+        c->tl[0] = t; *loglik = t;
+    } else c->tl[m] = *loglik;
+    c->tf[m] = c->tl[m] + log_gamma_pdf(c->tc[m], o->cprior_a, o->cprior_b);
+    if (c->resZ != 0) c->tf[m] += log_gamma_pdf(c->tn[m], o->nprior_a, o->nprior_b);
+    if (m >= o->burnin && c->tf[m] > c->MaxLoglik) {
+        c->MaxLoglik = c->tf[m]; c->Max_m = m;
+        for (int s = 0; s < e->N; s++) c->Max_Z[s] = e->Z[s];
+    }
+}
+
+/* BPP_C::Update_Tg(tosample=true), bpp_c.cpp:672-733; the uniform for
+ * site g and internal node at sampling position t (0 = root) is slot t of
+ * sub-stream g in the ANC stream of sweep m. */
+static void sample_site(chain_t *c, int g)
+{
+    or_element *e = c->e;
+    uint32_t c3 = tagword(c->ms, OR_TAG_ANC), c2 = (uint32_t)c->m_cur;
+    double p[4], *lr = LAM(e, g, e->root), mx = NEG_INF;
+    for (int j = 0; j < 4; j++) if (lr[j] > mx) mx = lr[j];
+    for (int j = 0; j < 4; j++) p[j] = exp(lr[j] - mx);
+    TG(e, g, e->root) = categorical(p, 4, slot_u(c->o->seed, (uint32_t)g, c2, c3, 0));
+    int t = 1;
+    for (int k = e->n_inodes - 2; k >= 0; k--, t++) {
+        int s = e->inodes[k], par = e->par2[s];
+        if (e->missing[s] || TG(e, g, s) >= 4) continue;
+        const double *L = Lclass(e, s);
+        int tp = TG(e, g, par);
+        double *ls = LAM(e, g, s), y[4];
+        mx = NEG_INF;
+        for (int i = 0; i < 4; i++) { y[i] = L[i * 4 + tp] + ls[i]; if (y[i] > mx) mx = y[i]; }
+        for (int i = 0; i < 4; i++) p[i] = exp(y[i] - mx);
+        TG(e, g, s) = categorical(p, 4, slot_u(c->o->seed, (uint32_t)g, c2, c3, (uint32_t)t));
+    }
+}
+
+/* BPP_C::getEmission (bpp_c2.cpp:355-390) */
+static void get_emission(chain_t *c)
+{
+    or_element *e = c->e;
+    for (int k = 0; k < e->n_nodes - 1; k++) {
+        int s = e->nodes[k], par = e->par2[s];
+        if (e->missing[s]) continue;
+        double e0 = 0, e1 = 0, e2 = 0;
+        for (int g = 0; g < e->GG; g++) {
+            int ts = TG(e, g, s), tp = TG(e, g, par);
+            if (ts >= 4) continue;
+            if (ts == -1) {
+                double y[4];
+                e0 += e->amb_null[((size_t)g * e->S + s) * 4 + tp];
+                for (int i = 0; i < 4; i++) y[i] = e->Lcons[s][i * 4 + tp] + LAM(e, g, s)[i];
+                e1 += log_exp_sum(y, 4);
+                for (int i = 0; i < 4; i++) y[i] = e->Lneut[s][i * 4 + tp] + LAM(e, g, s)[i];
+                e2 += log_exp_sum(y, 4);
+            } else {
+                e0 += e->Lnull[s][ts * 4 + tp];
+                e1 += e->Lcons[s][ts * 4 + tp];
+                e2 += e->Lneut[s][ts * 4 + tp];
+            }
+        }
+        e->log_emission[s][0] = e0; e->log_emission[s][1] = e1; e->log_emission[s][2] = e2;
+    }
+}
+
+/* BPP_C::Update_Z_subtree (bpp_c.cpp:1182-1293); the uniform for the node at
+ * position t of the top-down order (0 = root) is slot t of the Z stream. */
+static int update_Z(chain_t *c, int *changed)
+{
+    or_element *e = c->e;
+    get_emission(c);
+    double b[OR_MAXN][3];
+    for (int k = 0; k < e->n_nodes; k++) {
+        int s = e->nodes[k];
+        for (int j = 0; j < 3; j++) b[s][j] = e->log_emission[s][j];
+        if (e->fixZ[s] == 1) b[s][2] = NEG_INF;
+        else if (e->fixZ[s] == 2) b[s][0] = NEG_INF;
+    }
+    for (int k = 0; k < e->n_nodes - 1; k++) {
+        int s = e->nodes[k], p = e->par2[s];
+        double o[3]; log_multi2(e->TM[s], b[s], o);
+        for (int j = 0; j < 3; j++) b[p][j] += o[j];
+    }
+    double prior[3] = {log(0.5), log(0.5), NEG_INF};
+    int r = e->root, nch = 0;
+    for (int j = 0; j < 3; j++) b[r][j] += prior[j];
+    uint32_t c3 = tagword(c->ms, OR_TAG_Z), c2 = (uint32_t)c->m_cur;
+    double p[3], mx = NEG_INF;
+    for (int j = 0; j < 3; j++) if (b[r][j] > mx) mx = b[r][j];
+    for (int j = 0; j < 3; j++) p[j] = exp(b[r][j] - mx);
+    int nn = categorical(p, 3, slot_u(c->o->seed, 0, c2, c3, 0));
+    if (e->Z[r] != nn) changed[nch++] = r;
+    e->Z[r] = nn;
+    int t = 1;
+    for (int k = e->n_nodes - 2; k >= 0; k--, t++) {
+        int s = e->nodes[k], par = e->par2[s];
+        double y[3]; mx = NEG_INF;
+        for (int i = 0; i < 3; i++) { y[i] = e->TM[s][i * 3 + e->Z[par]] + b[s][i]; if (y[i] > mx) mx = y[i]; }
+        for (int i = 0; i < 3; i++) p[i] = exp(y[i] - mx);
+        int z = categorical(p, 3, slot_u(c->o->seed, 0, c2, c3, (uint32_t)t));
+        if (e->Z[s] != z) changed[nch++] = s;
+        e->Z[s] = z;
+    }
+    return nch;
+}
+
+/* BPP_C::getUpdateNode(changedZ) (bpp_c.cpp:620-642) */
+static void update_node_changed(or_element *e, const int *changed, int nch, char *visited)
+{
+    for (int i = 0; i < e->N; i++) visited[i] = 1;
+    for (int k = nch - 1; k >= 0; k--) {
+        int j = e->par2[changed[k]];
+        while (j != e->N) {
+            if (!visited[j]) break;
+            visited[j] = 0;
+            for (int g = 0; g < e->GG; g++) { double *l = LAM(e, g, j); l[0] = l[1] = l[2] = l[3] = 0; }
+            j = e->par2[j];
+        }
+    }
+}
+
+/* BPP_C::getUpdateNode(neut) (bpp_c.cpp:836-858) */
+static void update_node_class(or_element *e, int neut, char *visited)
+{
+    for (int i = 0; i < e->N; i++) visited[i] = 1;
+    for (int k = 0; k < e->n_nodes; k++) {
+        int s = e->nodes[k];
+        if (e->missing[s]) continue;
+        if (!neut && (e->Z[s] == 0 || e->Z[s] == 2)) continue;
+        if (neut && (e->Z[s] == 1 || e->Z[s] == 0)) continue;
+        int j = e->par2[s];
+        while (j != e->N) { if (!visited[j]) break; visited[j] = 0; j = e->par2[j]; }
+    }
+}
+
+/* BPP_C::log_f_Xz(visited, lambda_tmp, neut, propos) (bpp_c.cpp:860-928) */
+static double log_f_Xz_prop(chain_t *c, const char *visited, double *lt, int neut, double propos)
+{
+    or_element *e = c->e; int N = e->N;
+    double (*Lprop)[16] = (double (*)[16])malloc(sizeof(double) * 16 * N);
+    for (int k = 0; k < e->n_inodes; k++) {
+        int s = e->inodes[k];
+        if (visited[s]) continue;
+        for (int cc = 0; cc < 2; cc++) {
+            int chi = e->ch2[s][cc];
+            if (e->missing[chi]) continue;
+            const double *L;
+            if (!neut) {
+                if (e->Z[chi] == 1) { or_log_trans(c->m, e->dist2[chi] * propos, Lprop[chi]); L = Lprop[chi]; }
+                else if (e->Z[chi] == 2) L = e->Lneut[chi]; else L = e->Lnull[chi];
+            } else {
+                if (e->Z[chi] == 2) { or_log_trans(c->m, e->dist2[chi] * propos, Lprop[chi]); L = Lprop[chi]; }
+                else if (e->Z[chi] == 1) L = e->Lcons[chi]; else L = e->Lnull[chi];
+            }
+            for (int g = 0; g < e->GG; g++) {
+                if (TG(e, g, chi) < 4) {
+                    double o[4]; log_multi(L, lt + ((size_t)g * N + chi) * 4, o);
+                    double *d = lt + ((size_t)g * N + s) * 4;
+                    for (int j = 0; j < 4; j++) d[j] += o[j];
+                }
+            }
+        }
+    }
+    double res = 0;
+    for (int g = 0; g < e->GG; g++) {
+        double *lr = lt + ((size_t)g * N + e->root) * 4;
+        if (!visited[e->root]) for (int j = 0; j < 4; j++) lr[j] += c->logpi[j];
+        res += log_exp_sum(lr, 4);
+    }
+    free(Lprop);
+    return res;
+}
+
+/* BPP_C::sample_rate (bpp_c.cpp:930-1176), M = 1, adaptive, factor 0.5 */
+static double sample_rate(chain_t *c, double old_rate, int neut, const char *visited, double *loglik_old)
+{
+    or_element *e = c->e; const or_options *o = c->o; const or_model *mod = c->m;
+    int m = c->m_cur, N = e->N;
+    double cur = old_rate, prior_a, prior_b;
+    if (neut) { prior_a = o->nprior_a; prior_b = o->nprior_b; } else { prior_a = o->cprior_a; prior_b = o->cprior_b; }
+    uint32_t c3 = tagword(c->ms, neut ? OR_TAG_R2 : OR_TAG_R1), c2 = (uint32_t)m;
+    if (visited[e->root]) {
+        if (c->resZ != 0) {
+            stream_t sp = {o->seed, 2, c2, c3, 0};
+            cur = gamma_draw(&sp, prior_a, prior_b);
+            if (neut) {
+                if (o->ropt == 1 && cur < o->nlb) cur = old_rate;
+                else if (o->ropt == 2 && cur < c->tc[m]) cur = old_rate;
+            } else {
+                if (o->ropt == 1 && cur > o->cub) cur = old_rate;
+                else if (o->ropt == 2 && cur > c->tn[m + 1]) cur = old_rate;
+            }
+        } else cur = old_rate;
+    } else {
+        int skip = 0;
+        double r = slot_u(o->seed, 0, c2, c3, 0), proposal;
+        if (neut) {
+            stream_t sg = {o->seed, 1, c2, c3, 0};
+            proposal = gamma_draw(&sg, cur / e->prop_n, e->prop_n);
+            if (o->ropt == 1 && proposal < o->nlb) skip = 1;
+            else if (o->ropt == 2 && proposal < c->tc[m]) skip = 1;
+        } else {
+            double ru = o->r1_shared_uniform ? r : slot_u(o->seed, 1, c2, c3, 0);
+            double u = ru * (e->prop_c - 1 / e->prop_c) + 1 / e->prop_c;
+            proposal = cur * u;
+            if (o->ropt == 1 && proposal > o->cub) skip = 1;
+            else if (o->ropt == 2 && proposal > c->tn[m + 1]) skip = 1;
+        }
+        if (!skip && (proposal < 1e-12 || proposal > 1e8)) skip = 1;
+        if (!skip) {
+            size_t GG = e->GG;
+            double *lt = (double *)calloc(GG * N * 4, sizeof(double));
+            for (int k = 0; k < e->n_nodes - 1; k++) {
+                int s = e->nodes[k];
+                if (!visited[s]) continue;
+                for (size_t g = 0; g < GG; g++) memcpy(lt + (g * N + s) * 4, LAM(e, g, s), 4 * sizeof(double));
+            }
+            double loglik_new = log_f_Xz_prop(c, visited, lt, neut, proposal);
+            double MH;
+            if (neut)
+                MH = loglik_new - *loglik_old + log_gamma_pdf(proposal, prior_a, prior_b) - log_gamma_pdf(cur, prior_a, prior_b)
+                     + log_gamma_pdf(cur, proposal / e->prop_n, e->prop_n) - log_gamma_pdf(proposal, cur / e->prop_n, e->prop_n);
+            else
+                MH = loglik_new - *loglik_old + log_gamma_pdf(proposal, prior_a, prior_b) - log_gamma_pdf(cur, prior_a, prior_b)
+                     + log(cur) - log(proposal);
+            if (log(r) < MH) {
+                cur = proposal; *loglik_old = loglik_new;
+                for (int k = 0; k < e->n_inodes; k++) {
+                    int s = e->inodes[k];
+                    if (visited[s]) continue;
+                    for (size_t g = 0; g < GG; g++) memcpy(LAM(e, g, s), lt + (g * N + s) * 4, 4 * sizeof(double));
+                }
+            }
+            free(lt);
+        }
+    }
+    if (old_rate != cur) {
+        for (int k = 0; k < e->n_nodes - 1; k++) {
+            int s = e->nodes[k];
+            if (e->dist2[s] < 1e-10) continue;
+            or_log_trans(mod, e->dist2[s] * cur, neut ? e->Lneut[s] : e->Lcons[s]);
+        }
+        if (neut) e->accept_n += 1; else e->accept_c += 1;
+    }
+    int af = o->adapt_freq;
+    if ((m + 1) % af == 0) {
+        double fac = 1 / sqrt((double)((m + 1) / af)), adj;
+        if (fac > 0.5) fac = 0.5;
+        if (neut) {
+            double ar = (double)e->accept_n / af;
+            adj = ar > 0.44 ? exp(fac) : (ar < 0.23 ? exp(-fac) : 1);
+            e->prop_n = e->prop_n * adj; if (e->prop_n > 1) e->prop_n = 1;
+            e->accept_n = 0;
+        } else {
+            double ar = (double)e->accept_c / af;
+            adj = ar > 0.44 ? exp(fac) : (ar < 0.23 ? exp(-fac) : 1);
+            e->prop_c = pow(e->prop_c, adj); if (e->prop_c < 1.0001) e->prop_c = 1.0001;
+            e->accept_c = 0;
+        }
+    }
+    return cur;
+}
+
+/* BPP_C::sample_transition (bpp_c.cpp:522-613) */
+static void sample_transition(chain_t *c, double *gr, double *lr, double *lr2)
+{
+    or_element *e = c->e; const or_options *o = c->o;
+    double nZ[3][3] = {{0}};
+    for (int k = 0; k < e->n_nodes - 1; k++) { int s = e->nodes[k]; nZ[e->Z[s]][e->Z[e->par2[s]]] += 1; }
+    uint32_t c3 = tagword(c->ms, OR_TAG_TRANS), c2 = (uint32_t)c->m_cur;
+    *gr = or_beta_stream(o->seed, 0, 1, c2, c3, o->hyper_grate_a + nZ[1][0] + nZ[2][0], o->hyper_grate_b + nZ[0][0]);
+    for (int k = 0; k < e->n_nodes - 1; k++) {
+        int s = e->nodes[k];
+        if (e->fixZ[s] == 1) nZ[e->Z[s]][e->Z[e->par2[s]]] -= 1;
+    }
+    *lr = or_beta_stream(o->seed, 2, 3, c2, c3, o->hyper_lrate_a + nZ[2][1], o->hyper_lrate_b + nZ[1][1]);
+    if (o->hyper_lrate2_a == 0) *lr2 = 0;
+    else *lr2 = or_beta_stream(o->seed, 4, 5, c2, c3, o->hyper_lrate2_a + nZ[1][2], o->hyper_lrate2_b + nZ[2][2]);
+    for (int k = 0; k < e->n_nodes; k++) set_TM(e, e->nodes[k], *gr, *lr, *lr2, 0);
+}
+
+/* BPP_C::Gibbs (bpp_c.cpp:346-519) */
+static void gibbs(chain_t *c)
+{
+    or_element *e = c->e;
+    char visited[OR_MAXN];
+    memset(visited, 0, sizeof visited);
+    for (int g = 0; g < e->GG; g++) prune_site(e, g, visited, c->logpi);
+    double log_lik_old = 0;
+    int changed[OR_MAXN];
+    for (int m = 0; m < c->T; m++) {
+        c->m_cur = m;
+        monitor(c, m, &log_lik_old);
+        if (m == c->T - 1) break;
+        for (int g = 0; g < e->GG; g++) sample_site(c, g);
+        int nch = update_Z(c, changed);
+        update_node_changed(e, changed, nch, visited);
+        if (nch > 0) {
+            log_lik_old = 0;
+            for (int g = 0; g < e->GG; g++) { prune_site(e, g, visited, c->logpi); log_lik_old += log_exp_sum(LAM(e, g, e->root), 4); }
+        }
+        char vn[OR_MAXN], vc[OR_MAXN];
+        update_node_class(e, 1, vn);
+        update_node_class(e, 0, vc);
+        c->tn[m + 1] = sample_rate(c, c->tn[m], 1, vn, &log_lik_old);
+        c->tc[m + 1] = sample_rate(c, c->tc[m], 0, vc, &log_lik_old);
+        sample_transition(c, &c->tg[m + 1], &c->tlr[m + 1], &c->tl2[m + 1]);
+    }
+}
+
+/* BPP_C::Eval2 (bpp_c2.cpp:14-353) */
+static double eval2(chain_t *c)
+{
+    or_element *e = c->e; const or_options *o = c->o;
+    int N = e->N, T = c->T, nb = o->burnin, nm = o->mcmc;
+    int *post = (int *)calloc(N, sizeof(int)), *unc = (int *)malloc(sizeof(int) * N), nu = 0;
+    for (int s = 0; s < N; s++) e->Z[s] = c->Max_Z[s];
+    for (int s = 0; s < N; s++) {
+        for (int i = nb; i < T; i++) if (c->tZ[(size_t)i * N + s] == c->Max_Z[s]) post[s]++;
+        if (post[s] < nm) unc[nu++] = s;
+    }
+    int cap = T + 1, nconf = 1;
+    int *confrow = (int *)malloc(sizeof(int) * cap);   /* -1 means Max_Z */
+    int *num = (int *)calloc(cap, sizeof(int));
+    double *vc = (double *)calloc(cap, sizeof(double)), *vn = (double *)calloc(cap, sizeof(double));
+    double *mc = (double *)calloc(cap, sizeof(double)), *mn = (double *)calloc(cap, sizeof(double));
+    double *cov = (double *)calloc(cap, sizeof(double)), *sfl = (double *)calloc(cap, sizeof(double));
+    confrow[0] = -1;
+#define CONF(j, s) (confrow[j] < 0 ? c->Max_Z[s] : (int)c->tZ[(size_t)confrow[j] * N + (s)])
+    for (int i = T - 1; i >= nb; i--) {
+        int j = 0;
+        for (; j < nconf; j++) {
+            int k = 0;
+            for (; k < nu; k++) if (c->tZ[(size_t)i * N + unc[k]] != CONF(j, unc[k])) break;
+            if (k == nu) break;
+        }
+        if (j == nconf) { confrow[nconf] = i; nconf++; }
+        num[j]++;
+        vc[j] += pow(c->tc[i], 2); mc[j] += c->tc[i];
+        if (c->resZ != 0) { vn[j] += pow(c->tn[i], 2); mn[j] += c->tn[i]; cov[j] += c->tc[i] * c->tn[i]; }
+        sfl[j] += c->tf[i] / nm;
+    }
+    double VB = 0, logdet; int eff = 0;
+    double prior_z[3] = {log(0.5), log(0.5), NEG_INF};
+    for (int i = 0; i < nconf; i++) {
+        double w = (double)num[i] / nm;
+        if (w < 0.02) continue;
+        double nZ[3][3] = {{0}};
+        for (int k = 0; k < e->n_nodes - 1; k++) { int s = e->nodes[k]; nZ[CONF(i, s)][CONF(i, e->par2[s])] += 1; }
+        double priorZ = lnbeta(o->hyper_grate_a + nZ[1][0] + nZ[2][0], o->hyper_grate_b + nZ[0][0]) - lnbeta(o->hyper_grate_a, o->hyper_grate_b);
+        for (int k = 0; k < e->n_nodes - 1; k++) { int s = e->nodes[k]; if (e->fixZ[s] == 1) nZ[CONF(i, s)][CONF(i, e->par2[s])] -= 1; }
+        priorZ += lnbeta(o->hyper_lrate_a + nZ[2][1], o->hyper_lrate_b + nZ[1][1]) - lnbeta(o->hyper_lrate_a, o->hyper_lrate_b);
+        if (o->hyper_lrate2_a > 0) priorZ += lnbeta(o->hyper_lrate2_a + nZ[1][2], o->hyper_lrate2_b + nZ[2][2]) - lnbeta(o->hyper_lrate2_a, o->hyper_lrate2_b);
+        priorZ += prior_z[CONF(i, e->root)];
+        sfl[i] += num[i] * priorZ / nm;
+        if (c->resZ == 0) logdet = log(vc[i] - pow(mc[i], 2) / num[i]) - log(num[i]);
+        else {
+            double a = vc[i] - pow(mc[i], 2) / num[i], b = vn[i] - pow(mn[i], 2) / num[i];
+            double cv = cov[i] - mn[i] * mc[i] / num[i];
+            logdet = log(a * b - cv * cv) - 2 * log(num[i]);
+        }
+        if (isnan(logdet)) continue;
+        eff += num[i];
+        VB -= w * log(num[i]);
+        VB += sfl[i];
+        VB += 0.5 * logdet * num[i] / nm;
+    }
+    if (eff == 0) {
+        double varn = 0, varc = 0, cv = 0, meann = 0, meanc = 0;
+        VB = 0;
+        for (int i = 0; i < nconf; i++) {
+            varc += vc[i]; meanc += mc[i];
+            if (c->resZ != 0) { meann += mn[i]; varn += vn[i]; cv += cov[i]; }
+        }
+        double rn = meann / nm, rc = meanc / nm;
+        for (int k = 0; k < e->n_nodes - 1; k++) {
+            int s = e->nodes[k];
+            or_log_trans(c->m, e->dist2[s] * rc, e->Lcons[s]);
+            if (c->resZ != 0) or_log_trans(c->m, e->dist2[s] * rn, e->Lneut[s]);
+        }
+        for (int i = 0; i < nconf; i++) {
+            double nZ[3][3] = {{0}};
+            for (int k = 0; k < e->n_nodes - 1; k++) { int s = e->nodes[k]; nZ[CONF(i, s)][CONF(i, e->par2[s])] += 1; }
+            double priorZ = lnbeta(o->hyper_grate_a + nZ[1][0] + nZ[2][0], o->hyper_grate_b + nZ[0][0]) - lnbeta(o->hyper_grate_a, o->hyper_grate_b);
+            for (int k = 0; k < e->n_nodes - 1; k++) { int s = e->nodes[k]; if (e->fixZ[s] == 1) nZ[CONF(i, s)][CONF(i, e->par2[s])] -= 1; }
+            priorZ += lnbeta(o->hyper_lrate_a + nZ[2][1], o->hyper_lrate_b + nZ[1][1]) - lnbeta(o->hyper_lrate_a, o->hyper_lrate_b);
+            if (o->hyper_lrate2_a > 0) priorZ += lnbeta(o->hyper_lrate2_a + nZ[1][2], o->hyper_lrate2_b + nZ[2][2]) - lnbeta(o->hyper_lrate2_a, o->hyper_lrate2_b);
+            priorZ += prior_z[CONF(i, e->root)];
+            for (int s = 0; s < N; s++) e->Z[s] = CONF(i, s);
+            zero_internal(e);
+            char visited[OR_MAXN]; memset(visited, 0, sizeof visited);
+            sfl[i] = 0;
+            for (int g = 0; g < e->GG; g++) { prune_site(e, g, visited, c->logpi); sfl[i] += log_exp_sum(LAM(e, g, e->root), 4); }
+            sfl[i] += priorZ;
+        }
+        VB = log_exp_sum(sfl, nconf);
+        VB += log_gamma_pdf(rc, o->cprior_a, o->cprior_b);
+        if (c->resZ != 0) VB += log_gamma_pdf(rn, o->nprior_a, o->nprior_b);
+        if (c->resZ == 0) {
+            varc -= pow(meanc, 2) / nm;
+            logdet = log(varc) - log(nm);
+            VB += 0.5 * logdet + 0.5 * log(2 * M_PI);
+        } else {
+            varn -= pow(meann, 2) / nm; varc -= pow(meanc, 2) / nm; cv -= meann * meanc / nm;
+            logdet = log(varc * varn - cv * cv) - 2 * log(nm);
+            VB += 0.5 * logdet + log(2 * M_PI);
+        }
+    } else {
+        if (c->resZ != 0) VB = VB * ((double)nm / eff) + log(nm) + log(2 * M_PI) + 1;
+        else VB = VB * ((double)nm / eff) + log(nm) + 0.5 * (log(2 * M_PI) + 1);
+    }
+#undef CONF
+    free(post); free(unc); free(confrow); free(num); free(vc); free(vn); free(mc); free(mn); free(cov); free(sfl);
+    return VB;
+}
+
+static int cmp_d(const void *a, const void *b) { double x = *(const double *)a, y = *(const double *)b; return x < y ? -1 : (x > y); }
+
+/* BPP_C::Output_init (bpp_c2.cpp:739-785): medians over the whole trace and
+ * post-burn-in Z frequencies (missing nodes: column 0 set to MCMC) */
+static void output_init(chain_t *c, or_chain_out *out)
+{
+    or_element *e = c->e; int T = c->T, N = e->N;
+    double *tmp = (double *)malloc(sizeof(double) * T);
+    double *src[5] = {c->tn, c->tc, c->tg, c->tlr, c->tl2};
+    for (int k = 0; k < 5; k++) { memcpy(tmp, src[k], sizeof(double) * T); qsort(tmp, T, sizeof(double), cmp_d); out->med[k] = tmp[T / 2]; }
+    free(tmp);
+    if (!out->postZ) return;
+    for (int s = 0; s < N; s++) {
+        int cnt[4] = {0};
+        for (int i = c->o->burnin; i < T; i++) cnt[c->tZ[(size_t)i * N + s] + 1]++;
+        if (e->missing[s]) cnt[0] = c->o->mcmc;
+        for (int k = 0; k < 4; k++) out->postZ[s * 4 + k] = (double)cnt[k] / c->o->mcmc;
+    }
+}
+
+int or_run_element(const or_model *m, const or_options *o, or_element *e, or_chain_out *outs)
+{
+    if (e->filtered) return 1;
+    int T = o->burnin + o->mcmc, N = e->N;
+    static const int order[3] = {0, 2, 1};
+    e->accept_n = e->accept_c = 0;
+    for (int ms = 0; ms < 3; ms++) {
+        chain_t c; memset(&c, 0, sizeof c);
+        c.m = m; c.o = o; c.e = e; c.resZ = order[ms]; c.ms = ms; c.T = T;
+        for (int j = 0; j < 4; j++) c.logpi[j] = log(m->pi[j]);
+        or_chain_out *out = &outs[ms];
+        double *own[7] = {0}; signed char *ownZ = NULL;
+        double **dst[7] = {&c.tl, &c.tf, &c.tn, &c.tc, &c.tg, &c.tlr, &c.tl2};
+        double *given[7] = {out->loglik, out->full, out->nrate, out->crate, out->grate, out->lrate, out->l2rate};
+        for (int k = 0; k < 7; k++) {
+            if (given[k]) *dst[k] = given[k];
+            else { own[k] = (double *)calloc(T, sizeof(double)); *dst[k] = own[k]; }
+        }
+        if (out->Z) c.tZ = out->Z; else { ownZ = (signed char *)calloc((size_t)T * N, 1); c.tZ = ownZ; }
+        init_mcmc(&c);
+        gibbs(&c);
+        out->max_loglik = c.MaxLoglik; out->max_m = c.Max_m;
+        for (int s = 0; s < N; s++) out->max_Z[s] = c.Max_Z[s];
+        out->vb = eval2(&c);
+        out->failure = isnan(out->vb) ? 1 : 0;
+        output_init(&c, out);
+        out->accept_n = e->accept_n; out->accept_c = e->accept_c;
+        for (int k = 0; k < 7; k++) free(own[k]);
+        free(ownZ);
+    }
+    return 0;
+}
+
+long or_run_batch(const or_model *m, const or_options *o, or_element **es, int n, int nthreads, double *max_loglik)
+{
+    long sweeps = 0;
+    int T = o->burnin + o->mcmc;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(guided) num_threads(nthreads) reduction(+ : sweeps)
+#endif
+    for (int i = 0; i < n; i++) {
+        if (es[i]->filtered) continue;
+        or_chain_out outs[3]; memset(outs, 0, sizeof outs);
+        or_run_element(m, o, es[i], outs);
+        for (int k = 0; k < 3; k++) if (max_loglik) max_loglik[i * 3 + k] = outs[k].max_loglik;
+        sweeps += 3L * (T - 1);
+    }
+    return sweeps;
+}

@@ -1,0 +1,114 @@
+/*
+ * pa_oracle.h -- CPU restatement of PhyloAcc-ST's per-element MCMC path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (phyloacc_amd/) never links, loads or calls it.
+ *
+ * It restates, in plain C and FP64 log space, one site at a time, the
+ * reference routines cited per function in pa_oracle.c (src/PhyloAcc-ST/...).
+ * Random numbers come from a counter-based Philox4x32-10 stream whose slot
+ * layout is fixed (see or_slot_* below) so that the GPU engine, fed the same
+ * seed, consumes exactly the same uniforms.
+ */
+#ifndef PA_ORACLE_H
+#define PA_ORACLE_H
+#include <stdint.h>
+
+#define OR_MAXN 512
+
+/* leaf codes (shared meaning with the product's encoder) */
+enum { OR_A = 0, OR_C = 1, OR_G = 2, OR_T = 3, OR_GAP = 4, OR_OTHER = 5,
+       OR_R = 6, OR_Y = 7, OR_K = 8, OR_M = 9, OR_S = 10, OR_W = 11 };
+
+/* RNG stream tags (bits 16..23 of counter word 3) */
+enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5 };
+
+typedef struct {
+    int burnin, mcmc, adapt_freq;          /* BURNIN, MCMC, ADAPT_FREQ */
+    double init_crate, init_nrate;         /* INIT_CONSERVE_RATE, INIT_ACCE_RATE */
+    double cprior_a, cprior_b;             /* CONSERVE_PRIOR_A/B (shape, scale) */
+    double nprior_a, nprior_b;             /* ACCE_PRIOR_A/B */
+    int ropt; double cub, nlb;             /* ROPT, CUB, NLB */
+    double init_grate, init_lrate, init_lrate2;
+    double hyper_grate_a, hyper_grate_b;
+    double hyper_lrate_a, hyper_lrate_b;
+    double hyper_lrate2_a, hyper_lrate2_b;
+    double gap_prop;                       /* GAP_PROP (missing_thres) */
+    double conserve_prop;                  /* CONSERVE_PROP */
+    double constomis;                      /* CONSTOMIS */
+    double nconstomis;                     /* fixed 1 in the reference driver */
+    double trim_gap_percent;               /* TRIM_GAP_PERCENT (revgap) */
+    int min_len;                           /* MIN_LEN */
+    int prune_tree;                        /* PRUNE_TREE */
+    int gapchar;                           /* GAPCHAR (first char) */
+    uint64_t seed;                         /* SEED */
+    int r1_shared_uniform;                 /* 1: reference r1 quirk (one uniform proposes and accepts) */
+} or_options;
+
+typedef struct {
+    int S, N;
+    int children[OR_MAXN][2];
+    int parent[OR_MAXN];
+    double dist[OR_MAXN];
+    double pi[4];
+    double Q[16];                          /* row = from state */
+    int upper[OR_MAXN];                    /* BPP::upper membership (outgroup ancestors + root) */
+    int upper_conserve[OR_MAXN];           /* BPP::upper_conserve membership */
+    int conserve[OR_MAXN]; int n_conserve; /* CONSERVE species ids */
+} or_model;
+
+typedef struct or_element or_element;
+
+typedef struct {
+    /* traces, length T = burnin + mcmc (caller-owned, may be NULL) */
+    double *loglik, *full, *nrate, *crate, *grate, *lrate, *l2rate;
+    signed char *Z;                        /* [T*N] */
+    double *postZ;                         /* [N*4] Output_init frequencies (may be NULL) */
+    double max_loglik; int max_m; int max_Z[OR_MAXN];
+    double vb;                             /* Eval2 marginal likelihood */
+    double med[5];                         /* Output_init medians: n, c, g, l, l2 */
+    int failure;
+    int accept_n, accept_c;                /* counters after the chain (carry over) */
+} or_chain_out;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* RNG */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double or_uniform(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int lane);
+double or_gamma_stream(uint64_t seed, uint32_t c1, uint32_t c2, uint32_t c3, double shape, double scale);
+double or_beta_stream(uint64_t seed, uint32_t c1x, uint32_t c1y, uint32_t c2, uint32_t c3, double a, double b);
+
+/* tree / model text format */
+int or_parse_mod(const char *text, or_model *m, char *names /* [N*64] */, int max_names);
+void or_set_groups(or_model *m, const int *outgroup, int n_out, const int *conserve, int n_cons);
+
+/* transition matrices: logPt[i*4+j] = log P_{j->i}(t*r) */
+void or_expm(const double *Q, double t, double *P /* [16], P[i*4+j] = P_{i->j} */);
+void or_log_trans(const or_model *m, double t, double *logPt);
+
+/* element encoding (BPP_C constructor) */
+or_element *or_encode(const or_model *m, const or_options *o, const char *const *rows, long start, int GG);
+void or_free(or_element *e);
+int or_elem_filtered(const or_element *e);
+int or_elem_info(const or_element *e, int *out /* [8]: GG, root, n_nodes, n_internal, S, N, filtered, 0 */);
+void or_elem_arrays(const or_element *e, int *children2 /*[N*2]*/, int *parent2, double *dist2,
+                    int *missing, int *nodes, int *internal, int *Tg /*[GG*N]*/, unsigned char *codes /*[GG*S]*/);
+
+/* log-likelihood at fixed Z, rates (bpp_c2.cpp:392-425 form) */
+double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, double r2);
+
+/* full three-model chain of one element, outs[0..2] = M0, M1, M2 (run order resZ 0, 2, 1) */
+int or_run_element(const or_model *m, const or_options *o, or_element *e, or_chain_out *outs);
+
+/* CPU baseline: many elements, OpenMP guided schedule; returns sweeps done */
+long or_run_batch(const or_model *m, const or_options *o, or_element **es, int n, int nthreads,
+                  double *max_loglik /* [n*3] */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,954 @@
+// pa_host.cpp -- host side of the engine: .mod parsing, Q eigensystem,
+// element encoding and pattern compression, device arenas, and the C ABI
+// declared in include/phyloacc_gpu.h.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/phyloacc_gpu.h"
+#include "pa_common.h"
+#include "pa_kernels.h"
+#include "pa_rng.h"
+
+using namespace pa;
+
+// ------------------------------------------------------------- .mod parse
+
+namespace {
+
+struct PNode {
+    std::string name;
+    double dist = 0;
+    int kid[2] = {-1, -1};
+    int nk = 0;
+    int id = -1;
+};
+
+// Recursive-descent Newick reader with the reference's conventions
+// (src/PhyloAcc-ST/newick.cpp:145-331): a leaf's length follows its last ':';
+// an internal label sits between ')' and ':'; lengths are read as float.
+struct Newick {
+    const std::string &s;
+    std::vector<PNode> nodes;
+    bool bad = false;
+    explicit Newick(const std::string &t) : s(t) {}
+
+    int parse(size_t a, size_t b)
+    {
+        int me = (int)nodes.size();
+        nodes.emplace_back();
+        if (a >= b) { bad = true; return me; }
+        if (s[a] != '(') {
+            size_t colon = std::string::npos;
+            for (size_t i = a; i < b; i++) if (s[i] == ':') colon = i;
+            size_t e = colon == std::string::npos ? b : colon;
+            nodes[me].name = s.substr(a, e - a);
+            nodes[me].dist = colon == std::string::npos ? 0.0 : (double)(float)atof(s.c_str() + colon + 1);
+            return me;
+        }
+        int depth = 0;
+        size_t start = a + 1, i = a;
+        std::vector<int> kids;
+        for (; i < b; i++) {
+            char ch = s[i];
+            if (ch == '(') depth++;
+            else if (ch == ')') { if (--depth == 0) break; }
+            else if (ch == ',' && depth == 1) { kids.push_back(parse(start, i)); start = i + 1; }
+        }
+        if (i >= b) { bad = true; return me; }
+        kids.push_back(parse(start, i));
+        if (kids.size() != 2) { bad = true; return me; }
+        nodes[me].kid[0] = kids[0];
+        nodes[me].kid[1] = kids[1];
+        nodes[me].nk = 2;
+        size_t j = i + 1, colon = std::string::npos, e = j;
+        while (e < b && s[e] != ';' && s[e] != ':') e++;
+        if (e < b && s[e] == ':') colon = e;
+        nodes[me].name = s.substr(j, e - j);
+        if (colon != std::string::npos) nodes[me].dist = (double)(float)atof(s.c_str() + colon + 1);
+        return me;
+    }
+};
+
+std::string trim3(const std::string &x)
+{
+    size_t a = x.find_first_not_of(" \t\n");
+    if (a == std::string::npos) return "";
+    size_t b = x.find_last_not_of(" \t\n");
+    return x.substr(a, b - a + 1);
+}
+
+std::vector<std::string> tokens(const std::string &x)
+{
+    std::vector<std::string> out;
+    size_t i = 0;
+    while (i < x.size()) {
+        while (i < x.size() && (x[i] == ' ' || x[i] == '\t')) i++;
+        size_t j = i;
+        while (j < x.size() && x[j] != ' ' && x[j] != '\t') j++;
+        if (j > i) out.push_back(x.substr(i, j - i));
+        i = j;
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" pa_status pa_parse_mod(const char *text, int32_t max_nodes, int32_t *S_out, int32_t *children,
+                                  int32_t *parent, double *dist, double *pi, double *Q, char *names)
+{
+    if (!text || !S_out) return PA_E_INPUT;
+    std::vector<std::string> lines;
+    {
+        const char *p = text;
+        while (true) {
+            const char *e = strchr(p, '\n');
+            if (!e) { lines.emplace_back(p); break; }
+            lines.emplace_back(p, e - p);
+            p = e + 1;
+        }
+    }
+    std::string tree;
+    double pib[4] = {0, 0, 0, 0}, Qb[16] = {0};
+    for (size_t li = 0; li < lines.size(); li++) {
+        if (lines[li].empty()) break;   // LoadPhyloTree stops at the first empty line
+        std::vector<std::string> tk = tokens(trim3(lines[li]));
+        if (tk.empty()) continue;
+        if (tk[0] == "BACKGROUND:") {
+            for (size_t g = 1; g < tk.size() && g <= 4; g++) pib[g - 1] = atof(tk[g].c_str());
+        } else if (tk[0] == "RATE_MAT:") {
+            for (int i = 0; i < 4 && li + 1 < lines.size(); i++) {
+                std::vector<std::string> row = tokens(trim3(lines[++li]));
+                for (size_t g = 0; g < row.size() && g < 4; g++) Qb[i * 4 + g] = atof(row[g].c_str());
+            }
+        } else if (tk[0] == "TREE:" && tk.size() > 1) {
+            tree = tk[1];
+        }
+    }
+    if (tree.empty()) return PA_E_INPUT;
+    size_t end = tree.size();
+    while (end > 0 && tree[end - 1] == ';') end--;
+    Newick nw(tree);
+    int root = nw.parse(0, end);
+    if (nw.bad) return PA_E_INPUT;
+    int S = 0;
+    for (auto &n : nw.nodes) S += n.nk == 0;
+    int N = 2 * S - 1;
+    if ((int)nw.nodes.size() != N || N > max_nodes || N > MAXN) return PA_E_INPUT;
+    // TravelTree2 numbering: post-order, leaves and internal nodes counted apart
+    int lid = 0, iid = 0;
+    std::vector<int> stack{root}, post;
+    std::vector<int> state(nw.nodes.size(), 0);
+    while (!stack.empty()) {
+        int v = stack.back();
+        PNode &n = nw.nodes[v];
+        if (state[v] < n.nk) { stack.push_back(n.kid[state[v]++]); continue; }
+        stack.pop_back();
+        n.id = n.nk == 0 ? lid++ : S + iid++;
+    }
+    *S_out = S;
+    for (int i = 0; i < N; i++) {
+        if (children) { children[2 * i] = children[2 * i + 1] = -1; }
+        if (parent) parent[i] = -1;
+    }
+    for (auto &n : nw.nodes) {
+        if (dist) dist[n.id] = n.dist;
+        if (names) {
+            memset(names + 64 * n.id, 0, 64);
+            strncpy(names + 64 * n.id, n.name.c_str(), 63);
+        }
+        if (n.nk == 2) {
+            int a = nw.nodes[n.kid[0]].id, b = nw.nodes[n.kid[1]].id;
+            if (a > b) std::swap(a, b);
+            if (children) { children[2 * n.id] = a; children[2 * n.id + 1] = b; }
+            if (parent) { parent[a] = n.id; parent[b] = n.id; }
+        }
+    }
+    if (pi) memcpy(pi, pib, sizeof pib);
+    if (Q) memcpy(Q, Qb, sizeof Qb);
+    return PA_OK;
+}
+
+// ------------------------------------------------------------- eigensystem
+
+namespace {
+
+// Stationary distribution of Q: pi Q = 0, sum pi = 1 (Gaussian elimination).
+bool stationary(const double *Q, double *pi)
+{
+    double A[4][5];
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 4; j++) A[i][j] = Q[j * 4 + i];
+        A[i][4] = 0;
+    }
+    for (int j = 0; j < 4; j++) A[3][j] = 1;
+    A[3][4] = 1;
+    for (int c = 0; c < 4; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 4; r++) if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        if (fabs(A[piv][c]) < 1e-300) return false;
+        for (int k = 0; k < 5; k++) std::swap(A[c][k], A[piv][k]);
+        for (int r = 0; r < 4; r++) {
+            if (r == c) continue;
+            double f = A[r][c] / A[c][c];
+            for (int k = c; k < 5; k++) A[r][k] -= f * A[c][k];
+        }
+    }
+    for (int i = 0; i < 4; i++) pi[i] = A[i][4] / A[i][i];
+    for (int i = 0; i < 4; i++) if (!(pi[i] > 0)) return false;
+    return true;
+}
+
+// Cyclic Jacobi on a symmetric 4x4: A = U diag(w) U^T.
+void jacobi4(double A[4][4], double w[4], double U[4][4])
+{
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) U[i][j] = i == j;
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = 0;
+        for (int p = 0; p < 4; p++) for (int q = p + 1; q < 4; q++) off += A[p][q] * A[p][q];
+        if (off < 1e-300) break;
+        for (int p = 0; p < 4; p++)
+            for (int q = p + 1; q < 4; q++) {
+                if (A[p][q] == 0) continue;
+                double th = (A[q][q] - A[p][p]) / (2 * A[p][q]);
+                double t = (th >= 0 ? 1 : -1) / (fabs(th) + sqrt(th * th + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 4; k++) {
+                    double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 4; k++) {
+                    double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 4; k++) {
+                    double ukp = U[k][p], ukq = U[k][q];
+                    U[k][p] = c * ukp - s * ukq;
+                    U[k][q] = s * ukp + c * ukq;
+                }
+            }
+    }
+    for (int i = 0; i < 4; i++) w[i] = A[i][i];
+}
+
+// Q = V diag(lam) Vi for a time-reversible Q (BPP ctor eig_gen + inv,
+// src/PhyloAcc-ST/bpp.hpp:357-362), via the symmetric form D^1/2 Q D^-1/2.
+bool eigensystem(const double *Q, double *lam, double *V, double *Vi, std::string &err)
+{
+    double pq[4];
+    if (!stationary(Q, pq)) { err = "RATE_MAT has no positive stationary distribution"; return false; }
+    double A[4][4], amax = 0, asym = 0;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            A[i][j] = sqrt(pq[i]) * Q[i * 4 + j] / sqrt(pq[j]);
+            amax = std::max(amax, fabs(A[i][j]));
+        }
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) asym = std::max(asym, fabs(A[i][j] - A[j][i]));
+    if (asym > 1e-9 * amax) { err = "RATE_MAT is not time-reversible"; return false; }
+    for (int i = 0; i < 4; i++) for (int j = i + 1; j < 4; j++) A[i][j] = A[j][i] = 0.5 * (A[i][j] + A[j][i]);
+    double U[4][4];
+    jacobi4(A, lam, U);
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) {
+            V[i * 4 + k] = U[i][k] / sqrt(pq[i]);
+            Vi[k * 4 + i] = U[i][k] * sqrt(pq[i]);
+        }
+    double rec = 0, qmax = 0;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double s = 0;
+            for (int k = 0; k < 4; k++) s += V[i * 4 + k] * lam[k] * Vi[k * 4 + j];
+            rec = std::max(rec, fabs(s - Q[i * 4 + j]));
+            qmax = std::max(qmax, fabs(Q[i * 4 + j]));
+        }
+    if (rec > 1e-10 * qmax) { err = "eigendecomposition of RATE_MAT failed"; return false; }
+    return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- encoder
+
+namespace {
+
+struct HostElem {
+    int filtered = 0, GG = 0, P = 0, Ppad = 0, root = -1, has_amb = 0;
+    int n_up = 0, n_down = 0;
+    std::vector<int32_t> tab;         // node table (tab_words(N))
+    std::vector<uint8_t> codes;       // [S][Ppad]
+    std::vector<int32_t> w, site;     // [P], [GG]
+    std::vector<uint8_t> site_codes;  // [GG][S]
+    std::vector<int> nodes, internal;
+};
+
+inline uint8_t code_of(int y, int gapchar)
+{
+    // BPP_C constructor switch (src/PhyloAcc-ST/bpp_c.hpp:220-261)
+    switch (y) {
+    case 'a': return PA_CODE_A;
+    case 'c': return PA_CODE_C;
+    case 'g': return PA_CODE_G;
+    case 't': return PA_CODE_T;
+    case 'r': return PA_CODE_R;
+    case 'y': return PA_CODE_Y;
+    case 'k': return PA_CODE_K;
+    case 'm': return PA_CODE_M;
+    case 's': return PA_CODE_S;
+    case 'w': return PA_CODE_W;
+    default: return y == gapchar ? PA_CODE_GAP : PA_CODE_OTHER;
+    }
+}
+
+struct TreeModel {
+    int S, N;
+    std::vector<int> ch, par;
+    std::vector<double> dist;
+    std::vector<uint8_t> upper, upper_cons;
+    std::vector<int> conserve;
+};
+
+struct Collapse {
+    std::vector<int> &ch2, &par2;
+    std::vector<double> &dist2;
+    const std::vector<uint8_t> &missing, &upper;
+    int S, &root;
+    // BPP_C::getSubtree_missing (src/PhyloAcc-ST/bpp_c.cpp:64-113)
+    void run(int ss, int child)
+    {
+        if (!upper[ss] || ss < S) return;
+        int s = ss;
+        while (upper[s] && s >= S) {
+            int a = ch2[2 * s], b = ch2[2 * s + 1];
+            int keep;
+            if (missing[a] && !missing[b]) { par2[a] = -1; keep = b; }
+            else if (!missing[a] && missing[b]) { par2[b] = -1; keep = a; }
+            else { run(a, 0); run(b, 1); return; }
+            dist2[keep] += dist2[s];
+            if (s == root) root = keep;
+            else { ch2[2 * par2[ss] + child] = keep; par2[keep] = par2[ss]; }
+            s = keep;
+        }
+    }
+};
+
+void encode_element(const TreeModel &tm, const pa_options &o, const char *const *rows, int64_t start, int len,
+                    HostElem &E)
+{
+    const int S = tm.S, N = tm.N;
+    E = HostElem();
+    if (len < o.min_len) { E.filtered = 1; return; }
+    // site codes [len][S]
+    std::vector<uint8_t> sc((size_t)len * S);
+    for (int s = 0; s < S; s++) {
+        const char *r = rows[s] + start;
+        for (int g = 0; g < len; g++) sc[(size_t)g * S + s] = code_of(tolower((unsigned char)r[g]), o.gapchar);
+    }
+    // column trimming (bpp_c.hpp:268-301)
+    std::vector<int> keep;
+    keep.reserve(len);
+    for (int g = 0; g < len; g++) {
+        if (o.trim_gap_percent < 1) {
+            int mis = 0;
+            for (int s = 0; s < S; s++) mis += sc[(size_t)g * S + s] == PA_CODE_GAP || sc[(size_t)g * S + s] == PA_CODE_OTHER;
+            if (mis > S * o.trim_gap_percent) continue;
+        }
+        keep.push_back(g);
+    }
+    if (o.trim_gap_percent < 1 && (int)keep.size() < o.min_len) { E.filtered = 1; return; }
+    const int GG = (int)keep.size();
+    E.GG = GG;
+    E.site_codes.resize((size_t)GG * S);
+    for (int g = 0; g < GG; g++) memcpy(&E.site_codes[(size_t)g * S], &sc[(size_t)keep[g] * S], S);
+    // gap counts and missing species (bpp_c.hpp:304-351)
+    std::vector<int> ngap(S, 0);
+    for (int g = 0; g < GG; g++)
+        for (int s = 0; s < S; s++) ngap[s] += E.site_codes[(size_t)g * S + s] == PA_CODE_GAP;
+    std::vector<uint8_t> missing(N, 0);
+    std::vector<int> ch2(tm.ch), par2(tm.par);
+    std::vector<double> dist2(tm.dist);
+    for (int s = S; s < N; s++) {
+        for (int q = 0; q < 2; q++) {
+            int x = ch2[2 * s + q];
+            if (x < S && ngap[x] > o.gap_prop * GG) missing[x] = 1;
+        }
+        if (missing[ch2[2 * s]] && missing[ch2[2 * s + 1]]) missing[s] = 1;
+    }
+    // CONSERVE filter (bpp_c.hpp:361-373)
+    int ct = 0;
+    for (int x : tm.conserve) ct += missing[x];
+    if (ct > o.conserve_prop * (double)tm.conserve.size()) { E.filtered = 1; return; }
+    // outgroup-lineage (or whole-tree) collapse (bpp_c.hpp:379-390)
+    int root = N - 1;
+    std::vector<uint8_t> all(N, 1);
+    Collapse col{ch2, par2, dist2, missing, o.prune_tree ? all : tm.upper, S, root};
+    col.run(N - 1, -1);
+    par2[root] = N;
+    if (root < S) { E.filtered = 1; return; }   // nothing left to prune over
+    // post-order of the element's tree (BPP_C::getSubtree, bpp_c.cpp:44-60)
+    std::vector<int> nodes;
+    {
+        std::vector<std::pair<int, int>> st{{root, 0}};
+        while (!st.empty()) {
+            auto &[v, k] = st.back();
+            if (v >= S && k < 2) { int x = ch2[2 * v + k]; k++; if (x != -1) st.push_back({x, 0}); continue; }
+            nodes.push_back(v);
+            st.pop_back();
+        }
+    }
+    std::vector<int> internal;
+    for (int v : nodes) if (v >= S) internal.push_back(v);
+    // levels for the Z forward-backward
+    std::vector<int> height(N, 0), depth(N, 0);
+    int hmax = 0, dmax = 0;
+    for (int v : nodes)
+        if (v >= S) { height[v] = 1 + std::max(height[ch2[2 * v]], height[ch2[2 * v + 1]]); hmax = std::max(hmax, height[v]); }
+    for (int k = (int)nodes.size() - 1; k >= 0; k--) {
+        int v = nodes[k];
+        depth[v] = v == root ? 0 : depth[par2[v]] + 1;
+        dmax = std::max(dmax, depth[v]);
+    }
+    E.tab.assign(tab_words(N), 0);
+    int32_t *T = E.tab.data();
+    for (int i = 0; i < N; i++) {
+        T[tab_ch0(N) + i] = ch2[2 * i];
+        T[tab_ch1(N) + i] = ch2[2 * i + 1];
+        T[tab_par(N) + i] = par2[i];
+        T[tab_flags(N) + i] = (missing[i] ? NF_MISSING : 0) | (tm.upper[i] ? NF_UPPER : 0) |
+                              (tm.upper_cons[i] ? NF_UPPER_CONS : 0) | (i < S ? NF_LEAF : 0);
+    }
+    for (int v : nodes) T[tab_flags(N) + v] |= NF_IN_TREE;
+    for (size_t k = 0; k < nodes.size(); k++) {
+        T[tab_nodes(N) + k] = nodes[k];
+        T[tab_tz(N) + nodes[k]] = (int)nodes.size() - 1 - (int)k;
+    }
+    for (size_t k = 0; k < internal.size(); k++) T[tab_int(N) + k] = internal[k];
+    int q = 0;
+    for (int h = 1; h <= hmax; h++) {
+        for (int v : internal) if (height[v] == h) T[tab_up(N) + q++] = v;
+        T[tab_upoff(N) + h - 1] = q;
+    }
+    q = 0;
+    for (int d = 1; d <= dmax; d++) {
+        for (int k = (int)nodes.size() - 1; k >= 0; k--) if (depth[nodes[k]] == d) T[tab_down(N) + q++] = nodes[k];
+        T[tab_downoff(N) + d - 1] = q;
+    }
+    double *D = (double *)(T + tab_dist(N));
+    for (int i = 0; i < N; i++) D[i] = dist2[i];
+    E.n_up = hmax;
+    E.n_down = dmax;
+    E.root = root;
+    E.nodes = nodes;
+    E.internal = internal;
+    // pattern compression (first-appearance order) or one pattern per site
+    E.site.resize(GG);
+    std::vector<int> first;
+    if (o.compress_patterns) {
+        std::unordered_map<std::string, int> map;
+        map.reserve(GG * 2);
+        for (int g = 0; g < GG; g++) {
+            std::string key((const char *)&E.site_codes[(size_t)g * S], S);
+            auto it = map.find(key);
+            if (it == map.end()) {
+                int id = (int)first.size();
+                map.emplace(std::move(key), id);
+                first.push_back(g);
+                E.site[g] = id;
+                E.w.push_back(1);
+            } else {
+                E.site[g] = it->second;
+                E.w[it->second]++;
+            }
+        }
+    } else {
+        for (int g = 0; g < GG; g++) { E.site[g] = g; first.push_back(g); E.w.push_back(1); }
+    }
+    E.P = (int)first.size();
+    E.Ppad = (E.P + 63) & ~63;
+    E.codes.assign((size_t)S * E.Ppad, PA_CODE_GAP);
+    for (int p = 0; p < E.P; p++)
+        for (int s = 0; s < S; s++) {
+            uint8_t cd = E.site_codes[(size_t)first[p] * S + s];
+            E.codes[(size_t)s * E.Ppad + p] = cd;
+            if (cd >= PA_CODE_R) E.has_amb = 1;
+        }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- engine
+
+struct pa_engine {
+    int device = 0;
+    pa_options opt;
+    ModelDev M;
+    TreeModel tm;
+    std::string err;
+    std::vector<HostElem> elems;
+    bool dev_dirty = true;
+    // device arenas
+    ElemDev *d_elems = nullptr;
+    uint8_t *d_codes = nullptr;
+    int32_t *d_w = nullptr, *d_site = nullptr, *d_tab = nullptr;
+    int *d_queue = nullptr;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    size_t lds_per_cu = 0;
+};
+
+static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
+{
+    if (e) e->err = msg;
+    return st;
+}
+
+#define HIPCHK(e, x)                                                                      \
+    do {                                                                                  \
+        hipError_t _r = (x);                                                              \
+        if (_r != hipSuccess)                                                             \
+            return fail(e, _r == hipErrorOutOfMemory ? PA_E_OOM : PA_E_DEVICE,            \
+                        std::string(#x) + ": " + hipGetErrorString(_r));                  \
+    } while (0)
+
+extern "C" void pa_default_options(pa_options *o)
+{
+    memset(o, 0, sizeof *o);
+    o->burnin = 200; o->mcmc = 800; o->adapt_freq = 500;
+    o->init_crate = 0.5; o->init_nrate = 1.0;
+    o->cprior_a = 5; o->cprior_b = 0.04; o->nprior_a = 10; o->nprior_b = 0.2;
+    o->ropt = 1; o->cub = 1; o->nlb = 0.6;
+    o->init_grate = 0.5; o->init_lrate = 0.3; o->init_lrate2 = 0.1;
+    o->hyper_grate_a = 3; o->hyper_grate_b = 1; o->hyper_lrate_a = 1; o->hyper_lrate_b = 9;
+    o->hyper_lrate2_a = 1; o->hyper_lrate2_b = 9;
+    o->gap_prop = 0.8; o->conserve_prop = 0.8; o->constomis = 0.01; o->trim_gap_percent = 1;
+    o->min_len = 50; o->prune_tree = 0; o->gapchar = '-'; o->seed = 5;
+    o->r1_shared_uniform = 1; o->compress_patterns = 1;
+}
+
+extern "C" const char *pa_last_error(const pa_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+extern "C" int32_t pa_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" double pa_rng_uniform(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, int32_t lane)
+{
+    U4 v = philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return u01(pick(v, lane & 3));
+}
+
+extern "C" pa_status pa_engine_create(const pa_model *model, const pa_options *opts, int32_t device, pa_engine **out)
+{
+    if (!model || !out) return PA_E_INPUT;
+    *out = nullptr;
+    const int S = model->S, N = 2 * S - 1;
+    if (S < 2 || N > MAXN) return PA_E_INPUT;
+    pa_engine *e = new pa_engine();
+    if (opts) e->opt = *opts; else pa_default_options(&e->opt);
+    const pa_options &o = e->opt;
+    if (o.burnin < 0 || o.mcmc < 1 || o.adapt_freq < 1) { delete e; return PA_E_INPUT; }
+    TreeModel &tm = e->tm;
+    tm.S = S; tm.N = N;
+    tm.ch.assign(model->children, model->children + 2 * N);
+    tm.par.assign(model->parent, model->parent + N);
+    tm.dist.assign(model->dist, model->dist + N);
+    tm.upper.assign(N, 0);
+    tm.upper_cons.assign(N, 0);
+    // the tree must be the reference numbering: binary, post-order ids, root N-1
+    for (int i = 0; i < N; i++) {
+        bool leaf = i < S;
+        if (leaf != (tm.ch[2 * i] == -1)) { e->err = "children[] do not match leaves 0..S-1"; *out = e; return PA_E_INPUT; }
+        if (!leaf && (tm.ch[2 * i] >= i || tm.ch[2 * i + 1] >= i)) { e->err = "internal ids are not post-order"; *out = e; return PA_E_INPUT; }
+    }
+    // BPP::getUppertree (src/PhyloAcc-ST/bpp.cpp:486-504)
+    auto upper = [&](const int32_t *sp, int n, std::vector<uint8_t> &set) -> bool {
+        for (int i = 0; i < n; i++) {
+            int p = sp[i];
+            if (p < 0 || p >= S) return false;
+            while (p != N - 1) { set[p] = 1; p = tm.par[p]; }
+        }
+        set[N - 1] = 1;
+        return true;
+    };
+    if (!upper(model->outgroup, model->n_outgroup, tm.upper) || !upper(model->conserve, model->n_conserve, tm.upper_cons)) {
+        e->err = "OUTGROUP/CONSERVE species id out of range";
+        *out = e;
+        return PA_E_INPUT;
+    }
+    tm.conserve.assign(model->conserve, model->conserve + model->n_conserve);
+    ModelDev &M = e->M;
+    memset(&M, 0, sizeof M);
+    M.S = S; M.N = N; M.I = N - S;
+    M.T = o.burnin + o.mcmc; M.burnin = o.burnin; M.mcmc = o.mcmc; M.adapt_freq = o.adapt_freq; M.ropt = o.ropt;
+    M.cub = o.cub; M.nlb = o.nlb;
+    M.cprior_a = o.cprior_a; M.cprior_b = o.cprior_b; M.nprior_a = o.nprior_a; M.nprior_b = o.nprior_b;
+    M.init_crate = o.init_crate; M.init_nrate = o.init_nrate; M.init_grate = o.init_grate;
+    M.init_lrate = o.init_lrate; M.init_lrate2 = o.init_lrate2;
+    M.ha_g = o.hyper_grate_a; M.hb_g = o.hyper_grate_b; M.ha_l = o.hyper_lrate_a; M.hb_l = o.hyper_lrate_b;
+    M.ha_l2 = o.hyper_lrate2_a; M.hb_l2 = o.hyper_lrate2_b;
+    M.constomis = o.constomis; M.nconstomis = 1.0;
+    for (int i = 0; i < 4; i++) M.pi[i] = model->pi[i];
+    M.key0 = (uint32_t)o.seed; M.key1 = (uint32_t)(o.seed >> 32);
+    M.r1_shared = o.r1_shared_uniform;
+    std::string err;
+    if (!eigensystem(model->Q, M.lam, M.V, M.Vi, err)) { e->err = err; *out = e; return PA_E_NUMERIC; }
+    e->device = device;
+    *out = e;
+    return PA_OK;
+}
+
+// Device set-up happens on first use, so that encoding and inspection work
+// on hosts without a GPU.
+static pa_status ensure_device(pa_engine *e)
+{
+    if (e->stream) return PA_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= e->device) return fail(e, PA_E_DEVICE, "no HIP device");
+    HIPCHK(e, hipSetDevice(e->device));
+    hipDeviceProp_t prop;
+    HIPCHK(e, hipGetDeviceProperties(&prop, e->device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(e, PA_E_DEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
+    e->n_cu = prop.multiProcessorCount;
+    e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+    HIPCHK(e, hipStreamCreate(&e->stream));
+    HIPCHK(e, hipMalloc(&e->d_queue, 256));
+    return PA_OK;
+}
+
+static void free_arenas(pa_engine *e)
+{
+    (void)hipFree(e->d_elems); (void)hipFree(e->d_codes); (void)hipFree(e->d_w);
+    (void)hipFree(e->d_site); (void)hipFree(e->d_tab);
+    e->d_elems = nullptr; e->d_codes = nullptr; e->d_w = e->d_site = e->d_tab = nullptr;
+}
+
+extern "C" void pa_engine_destroy(pa_engine *e)
+{
+    if (!e) return;
+    if (e->stream) {
+        (void)hipSetDevice(e->device);
+        (void)hipStreamSynchronize(e->stream);
+        free_arenas(e);
+        (void)hipFree(e->d_queue);
+        (void)hipStreamDestroy(e->stream);
+    }
+    delete e;
+}
+
+extern "C" pa_status pa_elements_add(pa_engine *e, const char *const *rows, const int64_t *starts, const int32_t *lengths,
+                                     int32_t n, int32_t *ids, int32_t *status)
+{
+    if (!e || !rows || !starts || !lengths || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    const size_t base = e->elems.size();
+    e->elems.resize(base + n);
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 64) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (int i = (int)t; i < n; i += (int)nt) encode_element(e->tm, e->opt, rows, starts[i], lengths[i], e->elems[base + i]);
+        });
+    for (auto &x : th) x.join();
+    for (int i = 0; i < n; i++) {
+        if (ids) ids[i] = (int32_t)(base + i);
+        if (status) status[i] = e->elems[base + i].filtered ? PA_ELEM_FILTERED : PA_ELEM_OK;
+    }
+    e->dev_dirty = true;
+    return PA_OK;
+}
+
+extern "C" pa_status pa_element_get(const pa_engine *e, int32_t id, pa_element_info *info, int32_t *children2,
+                                    int32_t *parent2, double *dist2, int32_t *missing, int32_t *nodes,
+                                    uint8_t *site_codes, int32_t *site_pattern, int32_t *weights)
+{
+    if (!e || id < 0 || id >= (int)e->elems.size()) return PA_E_INPUT;
+    const HostElem &E = e->elems[id];
+    const int N = e->tm.N, S = e->tm.S;
+    if (info) {
+        info->filtered = E.filtered; info->GG = E.GG; info->P = E.P; info->root = E.root;
+        info->n_nodes = (int)E.nodes.size(); info->n_internal = (int)E.internal.size();
+    }
+    if (E.filtered) return PA_OK;
+    const int32_t *T = E.tab.data();
+    for (int i = 0; i < N; i++) {
+        if (children2) { children2[2 * i] = T[tab_ch0(N) + i]; children2[2 * i + 1] = T[tab_ch1(N) + i]; }
+        if (parent2) parent2[i] = T[tab_par(N) + i];
+        if (dist2) dist2[i] = ((const double *)(T + tab_dist(N)))[i];
+        if (missing) missing[i] = (T[tab_flags(N) + i] & NF_MISSING) ? 1 : 0;
+    }
+    if (nodes) for (size_t k = 0; k < E.nodes.size(); k++) nodes[k] = E.nodes[k];
+    if (site_codes) memcpy(site_codes, E.site_codes.data(), E.site_codes.size());
+    if (site_pattern) memcpy(site_pattern, E.site.data(), E.site.size() * 4);
+    if (weights) memcpy(weights, E.w.data(), E.w.size() * 4);
+    (void)S;
+    return PA_OK;
+}
+
+// Upload every element's encoding into flat device arenas.
+static pa_status upload(pa_engine *e)
+{
+    pa_status st = ensure_device(e);
+    if (st != PA_OK) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    if (!e->dev_dirty) return PA_OK;
+    free_arenas(e);
+    const int S = e->tm.S, N = e->tm.N;
+    std::vector<ElemDev> d(e->elems.size());
+    size_t nc = 0, nw = 0, ns = 0, nt = 0;
+    for (size_t i = 0; i < e->elems.size(); i++) {
+        const HostElem &E = e->elems[i];
+        ElemDev &x = d[i];
+        memset(&x, 0, sizeof x);
+        if (E.filtered) continue;
+        x.GG = E.GG; x.P = E.P; x.Ppad = E.Ppad; x.root = E.root;
+        x.n_nodes = (int)E.nodes.size(); x.n_int = (int)E.internal.size(); x.n_up = E.n_up; x.n_down = E.n_down;
+        x.has_amb = E.has_amb;
+        x.off_codes = nc; nc += (size_t)S * E.Ppad;
+        x.off_w = nw; nw += E.Ppad;
+        x.off_site = ns; ns += (E.GG + 15) & ~15;
+        x.off_tab = nt; nt += tab_words(N);
+    }
+    std::vector<uint8_t> codes(std::max<size_t>(nc, 1));
+    std::vector<int32_t> w(std::max<size_t>(nw, 1), 0), site(std::max<size_t>(ns, 1), 0), tab(std::max<size_t>(nt, 2), 0);
+    for (size_t i = 0; i < e->elems.size(); i++) {
+        const HostElem &E = e->elems[i];
+        if (E.filtered) continue;
+        memcpy(&codes[d[i].off_codes], E.codes.data(), E.codes.size());
+        memcpy(&w[d[i].off_w], E.w.data(), E.w.size() * 4);
+        memcpy(&site[d[i].off_site], E.site.data(), E.site.size() * 4);
+        memcpy(&tab[d[i].off_tab], E.tab.data(), E.tab.size() * 4);
+    }
+    HIPCHK(e, hipMalloc(&e->d_elems, sizeof(ElemDev) * std::max<size_t>(d.size(), 1)));
+    HIPCHK(e, hipMalloc(&e->d_codes, codes.size()));
+    HIPCHK(e, hipMalloc(&e->d_w, w.size() * 4));
+    HIPCHK(e, hipMalloc(&e->d_site, site.size() * 4));
+    HIPCHK(e, hipMalloc(&e->d_tab, tab.size() * 4));
+    if (!d.empty()) HIPCHK(e, hipMemcpy(e->d_elems, d.data(), sizeof(ElemDev) * d.size(), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(e->d_codes, codes.data(), codes.size(), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(e->d_w, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(e->d_site, site.data(), site.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(e->d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    e->dev_dirty = false;
+    return PA_OK;
+}
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+// Common launch path for the chain and log-likelihood kernels.
+static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, int kind, int mode, int bs_req,
+                            ChainArgs &A, DevBuf *bufs, double *elapsed_ms)
+{
+    const int S = e->tm.S, N = e->tm.N;
+    // ids arrive in schedule order (largest first for batched runs)
+    std::vector<int32_t> work(ids, ids + n);
+    int Pcap = 64;
+    double meanP = 0;
+    for (int i = 0; i < n; i++) {
+        Pcap = std::max(Pcap, e->elems[work[i]].Ppad);
+        meanP += e->elems[work[i]].P;
+    }
+    meanP /= std::max(1, n);
+    int BS = bs_req;
+    if (BS == 0) BS = (meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256);
+    if (BS != 64 && BS != 128 && BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
+    const size_t smem = chain_smem_bytes(N, S, BS, fp64);
+    if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+    int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
+    per_cu = std::max(1, std::min(per_cu, 2048 / BS));
+    int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, e->n_cu * per_cu);
+    size_t off[5];
+    const size_t slot = slot_scratch_bytes(N, S, Pcap, fp64, off);
+    HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
+    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n));
+    HIPCHK(e, hipMemcpyAsync(bufs[1].p, work.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+    A.queue = e->d_queue;
+    A.scratch = (char *)bufs[0].p; A.slot_bytes = (int64_t)slot;
+    A.off_parts = off[0]; A.off_exps = off[1]; A.off_plin = off[2]; A.off_logt = off[3]; A.off_amb = off[4];
+    hipEvent_t t0, t1;
+    HIPCHK(e, hipEventCreate(&t0));
+    HIPCHK(e, hipEventCreate(&t1));
+    HIPCHK(e, hipEventRecord(t0, e->stream));
+    if (mode == PA_RUN_PER_ELEMENT) {
+        for (int i = 0; i < n; i++) {
+            ChainArgs B = A;
+            B.work = (const int32_t *)bufs[1].p + i;
+            B.n_work = 1;
+            // per-element outputs shift with the work position
+            B.res_d = A.res_d ? A.res_d + (size_t)i * 24 : nullptr;
+            B.res_i = A.res_i ? A.res_i + (size_t)i * 12 : nullptr;
+            B.maxZ = A.maxZ ? A.maxZ + (size_t)i * 3 * N : nullptr;
+            B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * e->M.T : nullptr;
+            B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * e->M.T * N : nullptr;
+            B.Zin = A.Zin ? A.Zin + (size_t)i * N : nullptr;
+            B.rates_in = A.rates_in ? A.rates_in + (size_t)i * 2 : nullptr;
+            B.ll_out = A.ll_out ? A.ll_out + i : nullptr;
+            HIPCHK(e, launch_reset(e->d_queue, e->stream));
+            HIPCHK(e, launch_kernel(e->M, B, fp64, BS, 1, kind, e->stream));
+        }
+    } else {
+        A.work = (const int32_t *)bufs[1].p;
+        A.n_work = n;
+        HIPCHK(e, launch_reset(e->d_queue, e->stream));
+        HIPCHK(e, launch_kernel(e->M, A, fp64, BS, grid, kind, e->stream));
+    }
+    HIPCHK(e, hipEventRecord(t1, e->stream));
+    HIPCHK(e, hipEventSynchronize(t1));
+    float ms = 0;
+    HIPCHK(e, hipEventElapsedTime(&ms, t0, t1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return PA_OK;
+}
+
+static pa_status check_ids(pa_engine *e, const int32_t *ids, int32_t n)
+{
+    for (int i = 0; i < n; i++) {
+        if (ids[i] < 0 || ids[i] >= (int)e->elems.size()) return fail(e, PA_E_INPUT, "element id out of range");
+        if (e->elems[ids[i]].filtered) return fail(e, PA_E_INPUT, "element " + std::to_string(ids[i]) + " is filtered");
+    }
+    return PA_OK;
+}
+
+static std::vector<int32_t> schedule(pa_engine *e, const int32_t *ids, int32_t n, int mode)
+{
+    std::vector<int32_t> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    if (mode == PA_RUN_BATCHED)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const HostElem &x = e->elems[ids[a]], &y = e->elems[ids[b]];
+            return (int64_t)x.GG * 3 + x.P > (int64_t)y.GG * 3 + y.P;
+        });
+    return order;
+}
+
+extern "C" pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *Z, const double *rates,
+                               int32_t precision, double *out)
+{
+    if (!e || !ids || !Z || !rates || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    if (n == 0) return PA_OK;
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    const int N = e->tm.N;
+    std::vector<int32_t> order = schedule(e, ids, n, PA_RUN_BATCHED);
+    std::vector<int32_t> Zw((size_t)n * N);
+    std::vector<double> rw((size_t)n * 2);
+    for (int i = 0; i < n; i++) {
+        memcpy(&Zw[(size_t)i * N], Z + (size_t)order[i] * N, sizeof(int32_t) * N);
+        rw[2 * i] = rates[2 * order[i]];
+        rw[2 * i + 1] = rates[2 * order[i] + 1];
+    }
+    DevBuf bufs[6];
+    HIPCHK(e, hipMalloc(&bufs[3].p, Zw.size() * 4));
+    HIPCHK(e, hipMalloc(&bufs[4].p, rw.size() * 8));
+    HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
+    HIPCHK(e, hipMemcpy(bufs[3].p, Zw.data(), Zw.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(bufs[4].p, rw.data(), rw.size() * 8, hipMemcpyHostToDevice));
+    ChainArgs A;
+    memset(&A, 0, sizeof A);
+    A.Zin = (const int32_t *)bufs[3].p;
+    A.rates_in = (const double *)bufs[4].p;
+    A.ll_out = (double *)bufs[5].p;
+    std::vector<int32_t> sorted(n);
+    for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
+    st = run_kernel(e, sorted.data(), n, precision == PA_FP64, 1, PA_RUN_BATCHED, 0, A, bufs, nullptr);
+    if (st != PA_OK) return st;
+    std::vector<double> ll(n);
+    HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) out[order[i]] = ll[i];
+    return PA_OK;
+}
+
+extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
+                                   pa_chain_result *results, pa_traces *traces, double *elapsed_ms)
+{
+    if (!e || !ids || !cfg || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    if (n == 0) return PA_OK;
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    const int N = e->tm.N, T = e->M.T;
+    std::vector<int32_t> order = schedule(e, ids, n, cfg->mode);
+    std::vector<int32_t> sorted(n);
+    for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
+    const bool want_tr = traces && (traces->loglik || traces->full || traces->nrate || traces->crate || traces->grate ||
+                                    traces->lrate || traces->l2rate || traces->Z);
+    const bool keep = cfg->keep_traces || want_tr;
+    DevBuf bufs[8];
+    HIPCHK(e, hipMalloc(&bufs[3].p, (size_t)n * 3 * 8 * 8));
+    HIPCHK(e, hipMalloc(&bufs[4].p, (size_t)n * 3 * 4 * 4));
+    HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 3 * N));
+    if (keep) {
+        HIPCHK(e, hipMalloc(&bufs[6].p, (size_t)n * 3 * 7 * T * 8));
+        HIPCHK(e, hipMalloc(&bufs[7].p, (size_t)n * 3 * T * N));
+    }
+    ChainArgs A;
+    memset(&A, 0, sizeof A);
+    A.res_d = (double *)bufs[3].p;
+    A.res_i = (int32_t *)bufs[4].p;
+    A.maxZ = (int8_t *)bufs[5].p;
+    A.tr_d = keep ? (double *)bufs[6].p : nullptr;
+    A.tr_Z = keep ? (int8_t *)bufs[7].p : nullptr;
+    st = run_kernel(e, sorted.data(), n, cfg->precision == PA_FP64, 0, cfg->mode, cfg->block_threads, A, bufs, elapsed_ms);
+    if (st != PA_OK) return st;
+    std::vector<double> rd((size_t)n * 24);
+    std::vector<int32_t> ri((size_t)n * 12);
+    std::vector<int8_t> mz((size_t)n * 3 * N);
+    HIPCHK(e, hipMemcpy(rd.data(), bufs[3].p, rd.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(e, hipMemcpy(ri.data(), bufs[4].p, ri.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(e, hipMemcpy(mz.data(), bufs[5].p, mz.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        const int dst = order[i];
+        for (int k = 0; k < 3; k++) {
+            const double *d = &rd[((size_t)i * 3 + k) * 8];
+            const int32_t *q = &ri[((size_t)i * 3 + k) * 4];
+            if (results) {
+                pa_chain_result &r = results[(size_t)dst * 3 + k];
+                r.max_loglik = d[0]; r.final_nrate = d[1]; r.final_crate = d[2];
+                r.final_grate = d[3]; r.final_lrate = d[4]; r.final_lrate2 = d[5];
+                r.max_m = q[0]; r.status = q[1]; r.accept_n = q[2]; r.accept_c = q[3];
+            }
+            if (traces && traces->max_Z) memcpy(traces->max_Z + ((size_t)dst * 3 + k) * N, &mz[((size_t)i * 3 + k) * N], N);
+        }
+    }
+    if (want_tr) {
+        std::vector<double> td((size_t)n * 3 * 7 * T);
+        HIPCHK(e, hipMemcpy(td.data(), bufs[6].p, td.size() * 8, hipMemcpyDeviceToHost));
+        double *dst7[7] = {traces->loglik, traces->full, traces->nrate, traces->crate, traces->grate, traces->lrate, traces->l2rate};
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++)
+                for (int f = 0; f < 7; f++)
+                    if (dst7[f]) memcpy(dst7[f] + ((size_t)order[i] * 3 + k) * T, &td[(((size_t)i * 3 + k) * 7 + f) * T], (size_t)T * 8);
+        if (traces->Z) {
+            std::vector<int8_t> tz((size_t)n * 3 * T * N);
+            HIPCHK(e, hipMemcpy(tz.data(), bufs[7].p, tz.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; i++)
+                memcpy(traces->Z + (size_t)order[i] * 3 * T * N, &tz[(size_t)i * 3 * T * N], (size_t)3 * T * N);
+        }
+    }
+    return PA_OK;
+}

@@ -1,0 +1,292 @@
+"""Python binding of the engine's C ABI (include/phyloacc_gpu.h).
+
+The engine mirrors PhyloAcc-ST's in-process interface: ``Engine`` plays the
+role of ``BPP`` (tree, substitution model, priors, outgroup/conserve sets;
+src/PhyloAcc-ST/bpp.hpp:173-404), ``add_elements`` the ``BPP_C`` constructor
+(per-element encoding; src/PhyloAcc-ST/bpp_c.hpp:143-476), ``loglik`` the
+fixed-(Z, r) likelihood ``BPP_C::log_f_Xz`` (src/PhyloAcc-ST/bpp_c2.cpp:392-425)
+and ``run_chains`` the per-element loop body of main.cpp:452-500
+(``initMCMC`` + ``Gibbs`` for M0, M1, M2).
+
+All computation happens in libphyloacc_gpu.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libphyloacc_gpu.so")
+
+PA_OK, PA_E_INPUT, PA_E_NUMERIC, PA_E_DEVICE, PA_E_OOM, PA_E_STATE = range(6)
+PA_FP64, PA_FP32 = 0, 1
+PA_RUN_BATCHED, PA_RUN_PER_ELEMENT = 0, 1
+PA_ELEM_OK, PA_ELEM_FILTERED, PA_ELEM_NUMERIC_FAIL = 0, 1, 2
+
+EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_destroy", "pa_last_error",
+           "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count"]
+
+
+class PaError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"phyloacc_gpu status {status}: {msg}")
+        self.status = status
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("burnin", C.c_int32), ("mcmc", C.c_int32), ("adapt_freq", C.c_int32),
+        ("init_crate", C.c_double), ("init_nrate", C.c_double),
+        ("cprior_a", C.c_double), ("cprior_b", C.c_double),
+        ("nprior_a", C.c_double), ("nprior_b", C.c_double),
+        ("ropt", C.c_int32), ("cub", C.c_double), ("nlb", C.c_double),
+        ("init_grate", C.c_double), ("init_lrate", C.c_double), ("init_lrate2", C.c_double),
+        ("hyper_grate_a", C.c_double), ("hyper_grate_b", C.c_double),
+        ("hyper_lrate_a", C.c_double), ("hyper_lrate_b", C.c_double),
+        ("hyper_lrate2_a", C.c_double), ("hyper_lrate2_b", C.c_double),
+        ("gap_prop", C.c_double), ("conserve_prop", C.c_double), ("constomis", C.c_double),
+        ("trim_gap_percent", C.c_double), ("min_len", C.c_int32), ("prune_tree", C.c_int32),
+        ("gapchar", C.c_int32), ("seed", C.c_uint64), ("r1_shared_uniform", C.c_int32),
+        ("compress_patterns", C.c_int32),
+    ]
+
+
+class Model(C.Structure):
+    _fields_ = [
+        ("S", C.c_int32), ("children", C.POINTER(C.c_int32)), ("parent", C.POINTER(C.c_int32)),
+        ("dist", C.POINTER(C.c_double)), ("pi", C.POINTER(C.c_double)), ("Q", C.POINTER(C.c_double)),
+        ("outgroup", C.POINTER(C.c_int32)), ("n_outgroup", C.c_int32),
+        ("conserve", C.POINTER(C.c_int32)), ("n_conserve", C.c_int32),
+    ]
+
+
+class ElementInfo(C.Structure):
+    _fields_ = [("filtered", C.c_int32), ("GG", C.c_int32), ("P", C.c_int32), ("root", C.c_int32),
+                ("n_nodes", C.c_int32), ("n_internal", C.c_int32)]
+
+
+class ChainResult(C.Structure):
+    _fields_ = [("max_loglik", C.c_double), ("max_m", C.c_int32), ("status", C.c_int32),
+                ("final_nrate", C.c_double), ("final_crate", C.c_double), ("final_grate", C.c_double),
+                ("final_lrate", C.c_double), ("final_lrate2", C.c_double),
+                ("accept_n", C.c_int32), ("accept_c", C.c_int32)]
+
+
+class Traces(C.Structure):
+    _fields_ = [(f, C.POINTER(C.c_double)) for f in ("loglik", "full", "nrate", "crate", "grate", "lrate", "l2rate")] + \
+               [("Z", C.POINTER(C.c_int8)), ("max_Z", C.POINTER(C.c_int8))]
+
+
+class RunConfig(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("mode", C.c_int32), ("block_threads", C.c_int32),
+                ("keep_traces", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine library; raises if the HIP build is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build it with __graft_entry__.build() "
+                               "(make -C phyloacc_amd/csrc); there is no CPU fallback")
+        L = C.CDLL(LIBPATH)
+        P = C.POINTER
+        L.pa_default_options.argtypes = [P(Options)]
+        L.pa_parse_mod.argtypes = [C.c_char_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                   P(C.c_double), P(C.c_double), P(C.c_double), C.c_char_p]
+        L.pa_engine_create.argtypes = [P(Model), P(Options), C.c_int32, P(C.c_void_p)]
+        L.pa_engine_destroy.argtypes = [C.c_void_p]
+        L.pa_last_error.argtypes = [C.c_void_p]
+        L.pa_last_error.restype = C.c_char_p
+        L.pa_elements_add.argtypes = [C.c_void_p, P(C.c_char_p), P(C.c_int64), P(C.c_int32), C.c_int32,
+                                      P(C.c_int32), P(C.c_int32)]
+        L.pa_element_get.argtypes = [C.c_void_p, C.c_int32, P(ElementInfo), P(C.c_int32), P(C.c_int32),
+                                     P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_uint8), P(C.c_int32),
+                                     P(C.c_int32)]
+        L.pa_loglik.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_double), C.c_int32,
+                                P(C.c_double)]
+        L.pa_run_chains.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(ChainResult),
+                                    P(Traces), P(C.c_double)]
+        L.pa_rng_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32]
+        L.pa_rng_uniform.restype = C.c_double
+        L.pa_device_count.argtypes = []
+        L.pa_device_count.restype = C.c_int32
+        for f in ("pa_parse_mod", "pa_engine_create", "pa_elements_add", "pa_element_get", "pa_loglik",
+                  "pa_run_chains"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().pa_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def parse_mod(text: str, max_nodes: int = 512):
+    """Reads a .mod file (LoadPhyloTree, src/PhyloAcc-ST/newick.cpp:431-517)."""
+    S = C.c_int32()
+    ch = np.zeros(2 * max_nodes, np.int32)
+    par = np.zeros(max_nodes, np.int32)
+    d = np.zeros(max_nodes, np.float64)
+    pi = np.zeros(4, np.float64)
+    Q = np.zeros(16, np.float64)
+    names = C.create_string_buffer(64 * max_nodes)
+    st = lib().pa_parse_mod(text.encode(), max_nodes, C.byref(S), _p(ch, C.c_int32), _p(par, C.c_int32),
+                            _p(d, C.c_double), _p(pi, C.c_double), _p(Q, C.c_double), names)
+    if st != PA_OK:
+        raise PaError(st, "pa_parse_mod failed")
+    N = 2 * S.value - 1
+    nm = [names.raw[64 * i: 64 * (i + 1)].split(b"\0")[0].decode() for i in range(N)]
+    return dict(S=S.value, N=N, children=ch[: 2 * N].reshape(N, 2).copy(), parent=par[:N].copy(),
+                dist=d[:N].copy(), pi=pi, Q=Q.reshape(4, 4), names=nm)
+
+
+class Engine:
+    """One tree + substitution model + run options (BPP) and its encoded elements."""
+
+    def __init__(self, S, children, parent, dist, pi, Q, outgroup=(), conserve=(), options: Options | None = None,
+                 device: int = 0):
+        self._keep = dict(
+            children=np.ascontiguousarray(children, np.int32).reshape(-1),
+            parent=np.ascontiguousarray(parent, np.int32),
+            dist=np.ascontiguousarray(dist, np.float64),
+            pi=np.ascontiguousarray(pi, np.float64),
+            Q=np.ascontiguousarray(Q, np.float64).reshape(-1),
+            out=np.ascontiguousarray(outgroup, np.int32),
+            cons=np.ascontiguousarray(conserve, np.int32),
+        )
+        k = self._keep
+        self.S = int(S)
+        self.N = 2 * self.S - 1
+        m = Model(self.S, _p(k["children"], C.c_int32), _p(k["parent"], C.c_int32), _p(k["dist"], C.c_double),
+                  _p(k["pi"], C.c_double), _p(k["Q"], C.c_double), _p(k["out"], C.c_int32), len(k["out"]),
+                  _p(k["cons"], C.c_int32), len(k["cons"]))
+        self.options = options if options is not None else default_options()
+        h = C.c_void_p()
+        st = lib().pa_engine_create(C.byref(m), C.byref(self.options), int(device), C.byref(h))
+        self._h = h.value
+        if st != PA_OK:
+            msg = lib().pa_last_error(self._h).decode() if self._h else "create failed"
+            self.close()
+            raise PaError(st, msg)
+        self._rows = []
+
+    @classmethod
+    def from_workload(cls, w, options: Options | None = None, device: int = 0):
+        t = parse_mod(w.mod_text)
+        return cls(t["S"], t["children"], t["parent"], t["dist"], t["pi"], t["Q"], w.outgroup, w.conserve,
+                   options, device)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pa_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, st):
+        if st != PA_OK:
+            raise PaError(st, lib().pa_last_error(self._h).decode())
+
+    def add_elements(self, align: np.ndarray, starts, lengths):
+        """Encode elements (BPP_C constructor).  align: [S, G] uint8 characters in tree leaf order."""
+        align = np.ascontiguousarray(align, np.uint8)
+        assert align.shape[0] == self.S
+        self._rows.append(align)
+        ptrs = (C.c_char_p * self.S)(*[C.c_char_p(align[s].ctypes.data) for s in range(self.S)])
+        st = np.ascontiguousarray(starts, np.int64)
+        ln = np.ascontiguousarray(lengths, np.int32)
+        n = len(st)
+        ids = np.zeros(n, np.int32)
+        status = np.zeros(n, np.int32)
+        self._check(lib().pa_elements_add(self._h, ptrs, _p(st, C.c_int64), _p(ln, C.c_int32), n,
+                                          _p(ids, C.c_int32), _p(status, C.c_int32)))
+        return ids, status
+
+    def element(self, eid: int):
+        info = ElementInfo()
+        self._check(lib().pa_element_get(self._h, int(eid), C.byref(info), None, None, None, None, None, None,
+                                         None, None))
+        N, S = self.N, self.S
+        d = dict(filtered=info.filtered, GG=info.GG, P=info.P, root=info.root, n_nodes=info.n_nodes,
+                 n_internal=info.n_internal)
+        if info.filtered:
+            return d
+        ch = np.zeros(2 * N, np.int32)
+        par = np.zeros(N, np.int32)
+        dist = np.zeros(N, np.float64)
+        mis = np.zeros(N, np.int32)
+        nodes = np.zeros(N, np.int32)
+        codes = np.zeros(info.GG * S, np.uint8)
+        site = np.zeros(info.GG, np.int32)
+        w = np.zeros(max(info.P, 1), np.int32)
+        self._check(lib().pa_element_get(self._h, int(eid), C.byref(info), _p(ch, C.c_int32), _p(par, C.c_int32),
+                                         _p(dist, C.c_double), _p(mis, C.c_int32), _p(nodes, C.c_int32),
+                                         _p(codes, C.c_uint8), _p(site, C.c_int32), _p(w, C.c_int32)))
+        d.update(children=ch.reshape(N, 2), parent=par, dist=dist, missing=mis, nodes=nodes[: info.n_nodes],
+                 codes=codes.reshape(info.GG, S), site_pattern=site, weights=w[: info.P])
+        return d
+
+    def loglik(self, ids, Z, rates, precision=PA_FP64):
+        ids = np.ascontiguousarray(ids, np.int32)
+        Z = np.ascontiguousarray(Z, np.int32).reshape(len(ids), self.N)
+        rates = np.ascontiguousarray(rates, np.float64).reshape(len(ids), 2)
+        out = np.zeros(len(ids), np.float64)
+        self._check(lib().pa_loglik(self._h, _p(ids, C.c_int32), len(ids), _p(Z, C.c_int32),
+                                    _p(rates, C.c_double), int(precision), _p(out, C.c_double)))
+        return out
+
+    def run_chains(self, ids, precision=PA_FP32, mode=PA_RUN_BATCHED, traces=False, block_threads=0,
+                   keep_traces=False):
+        """Runs M0, M1, M2 for every id.  Returns per-(element, model) arrays, models in run order."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        n = len(ids)
+        T = self.options.burnin + self.options.mcmc
+        N = self.N
+        res = (ChainResult * (3 * n))()
+        tr = Traces()
+        out = {}
+        maxZ = np.zeros((n, 3, N), np.int8)
+        tr.max_Z = _p(maxZ, C.c_int8)
+        if traces:
+            for f in ("loglik", "full", "nrate", "crate", "grate", "lrate", "l2rate"):
+                a = np.zeros((n, 3, T), np.float64)
+                out[f] = a
+                setattr(tr, f, _p(a, C.c_double))
+            z = np.zeros((n, 3, T, N), np.int8)
+            out["Z"] = z
+            tr.Z = _p(z, C.c_int8)
+        cfg = RunConfig(int(precision), int(mode), int(block_threads), int(bool(keep_traces)))
+        ms = C.c_double()
+        self._check(lib().pa_run_chains(self._h, _p(ids, C.c_int32), n, C.byref(cfg), res, C.byref(tr),
+                                        C.byref(ms)))
+        out["max_Z"] = maxZ
+        out["elapsed_ms"] = ms.value
+        for f, t in (("max_loglik", np.float64), ("max_m", np.int32), ("status", np.int32),
+                     ("final_nrate", np.float64), ("final_crate", np.float64), ("final_grate", np.float64),
+                     ("final_lrate", np.float64), ("final_lrate2", np.float64), ("accept_n", np.int32),
+                     ("accept_c", np.int32)):
+            out[f] = np.array([getattr(res[i], f) for i in range(3 * n)], t).reshape(n, 3)
+        return out
+
+
+def device_count() -> int:
+    return int(lib().pa_device_count())
+
+
+def rng_uniform(seed, c0, c1, c2, c3, lane):
+    return lib().pa_rng_uniform(seed, c0, c1, c2, c3, lane)

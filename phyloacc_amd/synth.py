@@ -1,0 +1,304 @@
+"""Seeded synthetic PhyloAcc-ST workloads (trees, GTR models, alignments).
+
+The benchmark configurations are described by their shapes only (species
+count, branch-length law, element-length law, gap rate, rate classes), so the
+inputs here are simulated: a random rooted binary tree, a GTR substitution
+model, and per-element alignments evolved down the tree with per-branch rate
+multipliers (neutral 1, conserved r1, accelerated r2).
+
+Node numbering follows the reference's ``.mod`` reader
+(src/PhyloAcc-ST/newick.cpp:382-428): post-order, leaves 0..S-1 in encounter
+order, internal nodes S..N-1 in post-order, root N-1, children ordered by id
+(src/PhyloAcc-ST/bpp.cpp:105-117).  ``Workload.mod_text`` is a ``.mod`` file
+in the reference's format (BACKGROUND / RATE_MAT / TREE with every node
+labelled, newick.cpp:416-419).
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+CODES = np.frombuffer(b"acgt", dtype=np.uint8)
+
+
+@dataclasses.dataclass
+class Tree:
+    S: int
+    children: np.ndarray      # [N, 2] int32, -1 for leaves
+    parent: np.ndarray        # [N] int32, -1 for root
+    dist: np.ndarray          # [N] float64 (float32-rounded, as the reference parser stores it)
+    names: list
+    newick: str
+
+    @property
+    def N(self):
+        return 2 * self.S - 1
+
+
+def random_tree(S: int, mean_bl: float, rng: np.random.Generator, prefix="sp") -> Tree:
+    """Yule-style random rooted binary tree with Exp(mean_bl) branch lengths."""
+    # build as nested lists: node = [name or None, [kids], bl]
+    root = {"kids": [], "bl": 0.0}
+    leaves = [root]
+    while len(leaves) < S:
+        i = int(rng.integers(len(leaves)))
+        v = leaves.pop(i)
+        a = {"kids": [], "bl": 0.0}
+        b = {"kids": [], "bl": 0.0}
+        v["kids"] = [a, b]
+        leaves += [a, b]
+    for v in _walk(root):
+        if v is not root:
+            v["bl"] = float(np.float32(rng.exponential(mean_bl)))
+    return _finish(root, S, prefix)
+
+
+def _walk(v):
+    yield v
+    for k in v["kids"]:
+        yield from _walk(k)
+
+
+def _finish(root, S, prefix):
+    # reference numbering: post-order, leaves and internal nodes counted apart
+    leaf_id = [0]
+    int_id = [0]
+
+    def number(v):
+        for k in v["kids"]:
+            number(k)
+        if not v["kids"]:
+            v["id"] = leaf_id[0]
+            leaf_id[0] += 1
+        else:
+            v["id"] = S + int_id[0]
+            int_id[0] += 1
+
+    number(root)
+    N = 2 * S - 1
+    children = -np.ones((N, 2), np.int32)
+    parent = -np.ones(N, np.int32)
+    dist = np.zeros(N, np.float64)
+    names = [""] * N
+    for v in _walk(root):
+        i = v["id"]
+        dist[i] = v["bl"]
+        names[i] = f"{prefix}{i}" if not v["kids"] else f"n{i}"
+        if v["kids"]:
+            a, b = sorted(k["id"] for k in v["kids"])
+            children[i] = (a, b)
+            parent[a] = i
+            parent[b] = i
+
+    def nw(v):
+        if not v["kids"]:
+            return f"{names[v['id']]}:{v['bl']:.9g}"
+        inner = ",".join(nw(k) for k in v["kids"])
+        if v is root:
+            return f"({inner}){names[v['id']]};"
+        return f"({inner}){names[v['id']]}:{v['bl']:.9g}"
+
+    return Tree(S, children, parent, dist, names, nw(root))
+
+
+def gtr(rng: np.random.Generator):
+    """pi ~ Dirichlet(10,10,10,10); exchangeabilities ~ LogNormal(0, 0.5);
+    Q normalised to one expected substitution per unit time."""
+    pi = rng.dirichlet([10.0] * 4)
+    ex = rng.lognormal(0.0, 0.5, size=6)
+    R = np.zeros((4, 4))
+    k = 0
+    for i in range(4):
+        for j in range(i + 1, 4):
+            R[i, j] = R[j, i] = ex[k]
+            k += 1
+    Q = R * pi[None, :]
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    Q /= -(pi * np.diag(Q)).sum()
+    return pi, Q
+
+
+def mod_text(pi, Q, tree: Tree) -> str:
+    lines = ["ALPHABET: A C G T", "ORDER: 0", "SUBST_MOD: REV",
+             "BACKGROUND: " + " ".join(f"{x:.17g}" for x in pi), "RATE_MAT:"]
+    for i in range(4):
+        lines.append("  " + " ".join(f"{x:.17g}" for x in Q[i]))
+    lines.append("TREE: " + tree.newick)
+    return "\n".join(lines) + "\n"
+
+
+def _eig(pi, Q):
+    d = np.sqrt(pi)
+    A = (Q * d[:, None]) / d[None, :]
+    A = 0.5 * (A + A.T)
+    lam, U = np.linalg.eigh(A)
+    V = U / d[:, None]          # right eigenvectors
+    Vi = U.T * d[None, :]       # inverse
+    return lam, V, Vi
+
+
+def trans_batch(lam, V, Vi, t):
+    """P(t) for a vector of times t -> [len(t), 4, 4] (row = from)."""
+    t = np.asarray(t, np.float64)
+    E = np.exp(lam[None, :] * t[:, None])
+    P = np.einsum("ik,nk,kj->nij", V, E, Vi)
+    return np.clip(P, 0.0, None)
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    tree: Tree
+    pi: np.ndarray
+    Q: np.ndarray
+    mod_text: str
+    align: np.ndarray         # [S, G] uint8 lower-case characters, rows in tree leaf order
+    sizes: np.ndarray         # [C] element lengths (BED end - start)
+    starts: np.ndarray        # [C] cumulative starts
+    outgroup: list
+    conserve: list
+    target: list
+    rates: np.ndarray         # [C, N] simulated rate multiplier per branch
+    seed: int
+
+    @property
+    def C(self):
+        return len(self.sizes)
+
+
+def _lengths(kind, C, rng):
+    if kind == "uniform100_500":
+        return rng.integers(100, 501, size=C)
+    if kind == "lognormal250":
+        x = rng.lognormal(np.log(250.0), 0.6, size=C)
+        return np.clip(np.rint(x), 50, 3000).astype(np.int64)
+    if kind == "pareto":
+        a, lo, hi = 1.5, 50.0, 10000.0
+        u = rng.random(C)
+        x = lo / (1.0 - u * (1.0 - (lo / hi) ** a)) ** (1.0 / a)
+        return np.clip(np.floor(x), 50, 10000).astype(np.int64)
+    if kind == "lognormal300":
+        x = rng.lognormal(np.log(300.0), 0.6, size=C)
+        return np.clip(np.rint(x), 50, 10000).astype(np.int64)
+    raise ValueError(kind)
+
+
+def _clade_tips(tree, v):
+    out = []
+    stack = [v]
+    while stack:
+        x = stack.pop()
+        if x < tree.S:
+            out.append(x)
+        else:
+            stack += list(tree.children[x])
+    return sorted(out)
+
+
+def simulate(tree, pi, Q, sizes, rates, gap_frac, rng):
+    """Evolve every element down the tree.  rates: [C, N] per-branch multipliers."""
+    lam, V, Vi = _eig(pi, Q)
+    C = len(sizes)
+    G = int(np.sum(sizes))
+    elem = np.repeat(np.arange(C), sizes)
+    N = tree.N
+    states = np.empty((N, G), np.uint8)
+    cpi = np.cumsum(pi)
+    states[N - 1] = np.searchsorted(cpi, rng.random(G) * cpi[-1], side="right").clip(0, 3)
+    order = []
+    stack = [N - 1]
+    while stack:
+        v = stack.pop()
+        order.append(v)
+        if v >= tree.S:
+            stack += list(tree.children[v])
+    for v in order:
+        if v == N - 1:
+            continue
+        p = tree.parent[v]
+        P = trans_batch(lam, V, Vi, tree.dist[v] * rates[:, v])        # [C,4,4]
+        cum = np.cumsum(P, axis=2)                                      # [C,4,4]
+        rows = cum[elem, states[p]]                                     # [G,4]
+        u = rng.random(G) * rows[:, 3]
+        states[v] = (u[:, None] >= rows[:, :3]).sum(1).astype(np.uint8)
+    align = CODES[states[: tree.S]]
+    if gap_frac > 0:
+        mask = rng.random(align.shape) < gap_frac
+        gapc = np.where(rng.random(align.shape) < 0.5, ord("-"), ord("n")).astype(np.uint8)
+        align = np.where(mask, gapc, align)
+    return np.ascontiguousarray(align)
+
+
+def make_workload(config: int, seed: int = 1, n_elements: int | None = None) -> Workload:
+    """Build the synthetic inputs of one benchmark configuration.
+
+    1: S=10, Exp(0.1) branches, 200 elements U[100,500], 2% gap/N, types
+       neutral / conserved (r1=0.3) / accelerated (r2=2.0 on 2 target tips).
+    2: S=43, Exp(0.08), 5000 elements LogNormal(ln 250, 0.6) in [50,3000],
+       5% gaps, 4 target tips, same three types.
+    3: S=43, 50000 elements Pareto(1.5) on [50, 10000]; 1-20% of branches
+       accelerated per element, r1 ~ U[0.1,0.5], r2 ~ U[1.5,4]; 2% gaps.
+    ``n_elements`` scales the element count down (tests) keeping the laws.
+    """
+    rng = np.random.default_rng(seed)
+    if config == 1:
+        S, mbl, C, lk, gap, ntarget = 10, 0.1, 200, "uniform100_500", 0.02, 2
+    elif config in (2, 4):
+        S, mbl, C, lk, gap, ntarget = 43, 0.08, 5000, "lognormal250", 0.05, 4
+    elif config == 3:
+        S, mbl, C, lk, gap, ntarget = 43, 0.08, 50000, "pareto", 0.02, 4
+    else:
+        raise ValueError(config)
+    if n_elements is not None:
+        C = int(n_elements)
+    tree = random_tree(S, mbl, rng)
+    pi, Q = gtr(rng)
+    sizes = _lengths(lk, C, rng)
+    N = tree.N
+    # groups: the smaller root clade is the outgroup; targets are tips of the other side
+    r = N - 1
+    a, b = tree.children[r]
+    ca, cb = _clade_tips(tree, a), _clade_tips(tree, b)
+    small, big = (ca, cb) if len(ca) <= len(cb) else (cb, ca)
+    outgroup = small[:2]
+    target = [int(x) for x in rng.choice(big, size=min(ntarget, len(big) - 1), replace=False)]
+    conserve = [x for x in range(S) if x not in target and x not in outgroup]
+    rates = np.ones((C, N))
+    if config in (1, 2, 4):
+        typ = rng.integers(0, 3, size=C)       # 0 neutral, 1 conserved, 2 accelerated
+        rates[typ == 1] = 0.3
+        rates[typ == 2] = 0.3
+        for t in target:
+            rates[typ == 2, t] = 2.0
+    else:
+        r1 = rng.uniform(0.1, 0.5, size=C)
+        r2 = rng.uniform(1.5, 4.0, size=C)
+        rates[:] = r1[:, None]
+        frac = rng.uniform(0.01, 0.20, size=C)
+        nacc = np.maximum(1, np.rint(frac * (N - 1))).astype(int)
+        for c in range(C):
+            br = rng.choice(N - 1, size=nacc[c], replace=False)
+            rates[c, br] = r2[c]
+    align = simulate(tree, pi, Q, sizes, rates, gap, rng)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    return Workload(f"config{config}", tree, pi, Q, mod_text(pi, Q, tree), align,
+                    sizes.astype(np.int64), starts, [int(x) for x in outgroup],
+                    [int(x) for x in conserve], target, rates, seed)
+
+
+def small_workload(S=6, C=4, length=40, seed=3, gap=0.05, mean_bl=0.15):
+    """Tiny workload for exhaustive checks (brute force, golden vectors)."""
+    rng = np.random.default_rng(seed)
+    tree = random_tree(S, mean_bl, rng)
+    pi, Q = gtr(rng)
+    sizes = np.full(C, length, np.int64)
+    rates = np.ones((C, tree.N))
+    align = simulate(tree, pi, Q, sizes, rates, gap, rng)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    a, b = tree.children[tree.N - 1]
+    ca = _clade_tips(tree, a)
+    cb = _clade_tips(tree, b)
+    small_, big = (ca, cb) if len(ca) <= len(cb) else (cb, ca)
+    return Workload("small", tree, pi, Q, mod_text(pi, Q, tree), align, sizes, starts,
+                    small_[:1], [x for x in big[1:]], big[:1], rates, seed)

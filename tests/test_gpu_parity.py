@@ -1,0 +1,61 @@
+"""GPU parity: the HIP engine against the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from phyloacc_amd import engine as E
+from phyloacc_amd import synth
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(config=1, n=6, seed=11, **kw):
+    w = synth.make_workload(config, seed=seed, n_elements=n)
+    eng = E.Engine.from_workload(w, E.default_options(**kw))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    m = O.model_from_workload(w)
+    oo = O.default_options(**kw)
+    return w, eng, ids, st, m, oo
+
+
+def _random_Z(eng, ids, rng):
+    Z = np.full((len(ids), eng.N), -1, np.int32)
+    for k, i in enumerate(ids):
+        d = eng.element(i)
+        for s in d["nodes"]:
+            Z[k, s] = rng.integers(0, 3)
+        Z[k, d["root"]] = rng.integers(0, 2)
+    return Z
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_loglik_fp64_and_fp32_vs_oracle(require_gpu, config):
+    w, eng, ids, st, m, oo = _setup(config, n=8, seed=5)
+    rng = np.random.default_rng(0)
+    Z = _random_Z(eng, ids, rng)
+    rates = np.stack([rng.uniform(0.1, 0.9, len(ids)), rng.uniform(1.2, 4.0, len(ids))], 1)
+    ref = np.array([O.Element(m, oo, w.align, w.starts[i], w.sizes[i]).loglik(Z[k], rates[k, 0], rates[k, 1])
+                    for k, i in enumerate(ids)])
+    l64 = eng.loglik(ids, Z, rates, E.PA_FP64)
+    l32 = eng.loglik(ids, Z, rates, E.PA_FP32)
+    print("fp64 rel err", np.max(np.abs(l64 - ref) / np.abs(ref)), "fp32 rel err", np.max(np.abs(l32 - ref) / np.abs(ref)))
+    np.testing.assert_allclose(l64, ref, rtol=1e-9)     # FP64 path vs oracle: 1e-9 relative
+    np.testing.assert_allclose(l32, ref, rtol=1e-6)     # FP32 rescaled partials: 1e-6 relative
+
+
+def test_chain_fp64_matches_oracle_trajectory(require_gpu):
+    kw = dict(burnin=20, mcmc=60, adapt_freq=25)
+    w, eng, ids, st, m, oo = _setup(1, n=4, seed=7, **kw)
+    res = eng.run_chains(ids, precision=E.PA_FP64, traces=True)
+    for k, i in enumerate(ids):
+        ref = O.Element(m, oo, w.align, w.starts[i], w.sizes[i]).run(oo)
+        for ms in range(3):
+            r = ref[ms]
+            np.testing.assert_array_equal(res["Z"][k, ms], r["Z"])
+            np.testing.assert_allclose(res["loglik"][k, ms], r["loglik"], rtol=1e-9)
+            np.testing.assert_allclose(res["nrate"][k, ms], r["nrate"], rtol=1e-9)
+            np.testing.assert_allclose(res["crate"][k, ms], r["crate"], rtol=1e-9)
+            np.testing.assert_allclose(res["grate"][k, ms], r["grate"], rtol=1e-9)
+            np.testing.assert_allclose(res["max_loglik"][k, ms], r["max_loglik"], rtol=1e-9)
+            assert res["max_m"][k, ms] == r["max_m"]
+            np.testing.assert_array_equal(res["max_Z"][k, ms], r["max_Z"])
