@@ -1,0 +1,171 @@
+"""Benchmark: element-MCMC iterations/s of the batched MI355X engine.
+
+Workload (default): 43 species, 50,000 elements with Pareto(1.5) lengths on
+[50, 10000] bp, 1-20% of branches accelerated per element (synthetic,
+seeded).  One step = the full three-model chain (M0, M1, M2; BURNIN 200 +
+MCMC 800 states each, the reference defaults) of every element in one
+persistent launch.  One element-MCMC iteration = one sweep m -> m+1 of one
+model chain of one element; value sums the three models.
+
+Also reported (same JSON line):
+  roofline     - the chain kernel's counted algorithmic HBM bytes per launch
+                 over its HIP-event duration, against 8 TB/s;
+  cpu_baseline - the CPU oracle (FP64, per site, OpenMP over elements) on a
+                 length-stratified sample of the same workload;
+  plain_gpu    - the plain FP64 path: one launch per element, one pattern
+                 per site, on a length-stratified sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stratified(sizes, k, rng):
+    """k element indices at evenly spaced length quantiles."""
+    order = np.argsort(sizes, kind="stable")
+    q = ((np.arange(k) + 0.5) / k * len(order)).astype(int)
+    return order[np.clip(q, 0, len(order) - 1)]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--elements", type=int, default=None, help="scale the element count down")
+    ap.add_argument("--burnin", type=int, default=200)
+    ap.add_argument("--mcmc", type=int, default=800)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs shard elements across "
+                          "processes and are not part of this build", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from phyloacc_amd import engine as E
+    from phyloacc_amd import synth
+
+    t0 = time.time()
+    w = synth.make_workload(args.config, seed=args.seed, n_elements=args.elements)
+    log(f"workload config{args.config}: {w.C} elements, {int(w.sizes.sum())} sites, S={w.tree.S} "
+        f"({time.time() - t0:.1f}s)")
+    opts = E.default_options(burnin=args.burnin, mcmc=args.mcmc)
+    eng = E.Engine.from_workload(w, opts)
+    t1 = time.time()
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    keep = ids[st == E.PA_ELEM_OK]
+    log(f"encoded {len(ids)} elements ({len(keep)} kept) in {time.time() - t1:.1f}s")
+    prec = E.PA_FP32 if args.precision == "fp32" else E.PA_FP64
+    T = args.burnin + args.mcmc
+    sweeps_per_step = len(keep) * 3 * (T - 1)
+
+    for i in range(args.warmup):
+        r = eng.run_chains(keep, precision=prec, block_threads=args.block, keep_traces=True)
+        log(f"warmup {i}: {r['elapsed_ms'] / 1e3:.2f}s device")
+    ms, stats = [], None
+    for i in range(args.steps):
+        tw = time.time()
+        r = eng.run_chains(keep, precision=prec, block_threads=args.block, keep_traces=True)
+        wall = time.time() - tw
+        ms.append(r["elapsed_ms"])
+        stats = r["stats"]
+        bad = int(np.sum(r["status"] != 0))
+        log(f"step {i}: {r['elapsed_ms'] / 1e3:.3f}s device, {wall:.3f}s wall, numeric failures {bad}")
+    step_ms = float(np.mean(ms))
+    value = sweeps_per_step / (step_ms / 1e3)
+    achieved = stats["algorithmic_bytes"] / (step_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "chain_kernel", "algorithmic_bytes_per_launch": stats["algorithmic_bytes"],
+                "grid": stats["grid"], "block_threads": stats["block_threads"], "lds_bytes": stats["lds_bytes"]}
+
+    rng = np.random.default_rng(0)
+    plain = None
+    if not args.no_plain:
+        # plain FP64 path: per-element launches over an uncompressed copy of a stratified sample
+        idx = stratified(w.sizes, 24, rng)
+        po = E.default_options(burnin=20, mcmc=80, compress_patterns=0)
+        peng = E.Engine.from_workload(w, po)
+        pids, pst = peng.add_elements(w.align, w.starts[idx], w.sizes[idx])
+        pids = pids[pst == E.PA_ELEM_OK]
+        peng.run_chains(pids[:2], precision=E.PA_FP64, mode=E.PA_RUN_PER_ELEMENT)
+        rp = peng.run_chains(pids, precision=E.PA_FP64, mode=E.PA_RUN_PER_ELEMENT)
+        psweeps = len(pids) * 3 * (po.burnin + po.mcmc - 1)
+        mean_pop = float(np.mean(w.sizes))
+        mean_smp = float(np.mean(w.sizes[idx][pst == E.PA_ELEM_OK]))
+        prate = psweeps / (rp["elapsed_ms"] / 1e3) * (mean_smp / mean_pop)
+        plain = {"value": round(prate, 1), "unit": "element-MCMC iterations/s", "precision": "fp64",
+                 "launches": int(len(pids)), "sample": f"{len(pids)} length-stratified elements, 100-state chains, "
+                 "rate scaled by mean length (sample/population)"}
+        peng.close()
+        log(f"plain fp64 per-element: {prate:.1f} it/s")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle_py as O
+        idx = stratified(w.sizes, args.cpu_sample, rng)
+        m = O.model_from_workload(w)
+        oo = O.default_options(burnin=10, mcmc=40)
+        els = [O.Element(m, oo, w.align, w.starts[i], w.sizes[i]) for i in idx]
+        els = [e for e in els if not e.filtered]
+        nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        tc = time.time()
+        n_sw, _ = O.run_batch(m, oo, els, nthr)
+        dt = time.time() - tc
+        mean_smp = float(np.mean([e.info()["GG"] for e in els]))
+        crate = n_sw / dt * (mean_smp / float(np.mean(w.sizes)))
+        cpu = {"value": round(crate, 2), "unit": "element-MCMC iterations/s", "cores": nthr, "kind": "port",
+               "sample": f"{len(els)} length-stratified elements, 50-state chains, {dt:.1f}s; rate scaled by "
+               "mean length (sample/population)"}
+        log(f"cpu oracle: {crate:.1f} it/s on {nthr} threads ({dt:.1f}s)")
+
+    out = {
+        "metric": "element-MCMC iterations/s (batched MI355X engine)",
+        "value": round(value, 1),
+        "unit": "element-MCMC iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 partials / fp64 sums" if prec == E.PA_FP32 else "fp64",
+        "data": "synthetic (seeded simulation of the configuration's shapes)",
+        "config": {"workload": f"config{args.config}: S={w.tree.S}, {w.C} elements, lengths "
+                   f"{'Pareto(1.5) 50-10000 bp' if args.config == 3 else 'see synth.py'}, "
+                   f"BURNIN {args.burnin} + MCMC {args.mcmc}, 3 models",
+                   "elements": int(len(keep)), "sites": int(w.sizes.sum()), "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "speedup_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -149,13 +149,34 @@ typedef struct {
     int32_t keep_traces;        /* keep full traces on the device (required for pa_traces) */
 } pa_run_config;
 
+/* Launch statistics of one pa_run_chains call. */
+typedef struct {
+    double elapsed_ms;          /* device time of the chain launches (HIP events on the engine stream) */
+    double algorithmic_bytes;   /* partial/code/weight bytes the passes must move (counted on device) */
+    int32_t grid, block_threads;
+    int32_t lds_bytes;          /* LDS per workgroup */
+    int32_t n_launches;
+} pa_run_stats;
+
 /* Runs the three model chains M0, M1, M2 (reference order resZ 0, 2, 1,
  * src/PhyloAcc-ST/main.cpp:466-494) of every listed element on the device.
  * Replaces the OpenMP element loop body (main.cpp:452-500: BPP_C::initMCMC +
  * BPP_C::Gibbs per model).  results: [n*3] in run order (M0, M1, M2).
- * elapsed_ms (may be NULL) receives the device time of the chain launches. */
+ * stats may be NULL. */
 pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
-                        pa_chain_result *results, pa_traces *traces, double *elapsed_ms);
+                        pa_chain_result *results, pa_traces *traces, pa_run_stats *stats);
+
+/* Per-model chain summaries of one element from its traces (T = BURNIN+MCMC
+ * states; Z is [T*N], max_Z is [N]).  vb: marginal log-likelihood of
+ * BPP_C::Eval2 (src/PhyloAcc-ST/bpp_c2.cpp:14-353), whose fallback branch
+ * re-evaluates P(X | Z, r*) on the device at `precision`.  medians[5] (n, c,
+ * g, l, l2 rates) and postZ[N*4] (frequency of Z = -1, 0, 1, 2 after burn-in)
+ * are BPP_C::Output_init's row (bpp_c2.cpp:739-785).  model: 0 = M0, 1 = M1,
+ * 2 = M2.  Any output pointer may be NULL. */
+pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, const double *full, const double *nrate,
+                           const double *crate, const double *grate, const double *lrate, const double *l2rate,
+                           const int8_t *Z, const int8_t *max_Z, int32_t precision, double *vb, double *medians,
+                           double *postZ);
 
 /* Number of visible HIP devices (0 when none); no reference counterpart. */
 int32_t pa_device_count(void);

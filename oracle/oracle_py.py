@@ -106,6 +106,8 @@ def lib():
         L.or_loglik.argtypes = [P(Model), C.c_void_p, P(C.c_int), C.c_double, C.c_double]
         L.or_loglik.restype = C.c_double
         L.or_run_element.argtypes = [P(Model), P(Options), C.c_void_p, P(ChainOut)]
+        L.or_test_update_Z.argtypes = [C.c_int, P(C.c_int), C.c_int, P(C.c_int), P(C.c_int), P(C.c_double),
+                                       C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, C.c_int, P(C.c_int)]
         L.or_run_batch.argtypes = [P(Model), P(Options), P(C.c_void_p), C.c_int, C.c_int, P(C.c_double)]
         L.or_run_batch.restype = C.c_long
         _lib = L

@@ -755,10 +755,18 @@ static void get_emission(chain_t *c)
 
 /* BPP_C::Update_Z_subtree (bpp_c.cpp:1182-1293); the uniform for the node at
  * position t of the top-down order (0 = root) is slot t of the Z stream. */
+static int update_Z_core(chain_t *c, int *changed);
+
 static int update_Z(chain_t *c, int *changed)
 {
-    or_element *e = c->e;
     get_emission(c);
+    return update_Z_core(c, changed);
+}
+
+/* the message passing and draws of Update_Z_subtree, given log_emission */
+static int update_Z_core(chain_t *c, int *changed)
+{
+    or_element *e = c->e;
     double b[OR_MAXN][3];
     for (int k = 0; k < e->n_nodes; k++) {
         int s = e->nodes[k];
@@ -1173,4 +1181,29 @@ long or_run_batch(const or_model *m, const or_options *o, or_element **es, int n
         sweeps += 3L * (T - 1);
     }
     return sweeps;
+}
+
+/* Test hook: one Z update on a bare tree with given per-branch
+ * log-emissions E[N*3] and transition probabilities g, l, l2.
+ * nodes: post-order (root last); parent: [N] with parent[root] = N. */
+int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, const int *fixZ, const double *E,
+                     double g, double l, double l2, uint64_t seed, int m, int ms, int *Zout)
+{
+    or_element *e = (or_element *)calloc(1, sizeof(or_element));
+    or_options o; memset(&o, 0, sizeof o); o.seed = seed;
+    e->N = N; e->n_nodes = n_nodes; e->root = nodes[n_nodes - 1];
+    for (int k = 0; k < n_nodes; k++) e->nodes[k] = nodes[k];
+    for (int s = 0; s < N; s++) {
+        e->par2[s] = parent[s]; e->fixZ[s] = fixZ[s]; e->Z[s] = 1;
+        for (int j = 0; j < 3; j++) e->log_emission[s][j] = E[s * 3 + j];
+        for (int j = 0; j < 9; j++) e->TM[s][j] = 0;
+    }
+    for (int k = 0; k < n_nodes; k++) set_TM(e, nodes[k], g, l, l2, 1);
+    chain_t c; memset(&c, 0, sizeof c);
+    c.e = e; c.o = &o; c.ms = ms; c.m_cur = m;
+    int changed[OR_MAXN];
+    update_Z_core(&c, changed);
+    for (int s = 0; s < N; s++) Zout[s] = e->Z[s];
+    free(e);
+    return 0;
 }

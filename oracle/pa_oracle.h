@@ -104,6 +104,10 @@ double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, doub
 /* full three-model chain of one element, outs[0..2] = M0, M1, M2 (run order resZ 0, 2, 1) */
 int or_run_element(const or_model *m, const or_options *o, or_element *e, or_chain_out *outs);
 
+/* test hook: one Z forward-backward draw on a bare tree */
+int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, const int *fixZ, const double *E,
+                     double g, double l, double l2, uint64_t seed, int m, int ms, int *Zout);
+
 /* CPU baseline: many elements, OpenMP guided schedule; returns sweeps done */
 long or_run_batch(const or_model *m, const or_options *o, or_element **es, int n, int nthreads,
                   double *max_loglik /* [n*3] */);

@@ -73,6 +73,7 @@ struct Ctx {
     double loglik, nrate, crate, g, l, l2, prop_n, prop_c, proposal, r, max_loglik;
     double TMf[9], TMn[9];  // linear log_TM_Int for fixZ / free nodes, [i*3+j] = P(child i | parent j)
     double red[16];
+    unsigned long long bytes;   // algorithmic HBM bytes of this element's passes (see prune / sample_sites)
     int accept_n, accept_c, max_m, skip, accept, status, work, nch;
 };
 
@@ -82,16 +83,19 @@ using MP = LDS const ModelDev *;
 
 __host__ __device__ inline size_t al(size_t x, size_t a) { return (x + a - 1) & ~(a - 1); }
 
-// Dynamic-LDS layout: returns the byte size; with a context, also binds its array pointers.
-template <typename R>
+// Dynamic-LDS layout: returns the byte size; BIND also points the context's
+// array fields into the block.  The context may sit at LDS address 0, which
+// compares equal to a null pointer, so binding is selected at compile time and
+// never by testing the context pointer.
+template <typename R, bool BIND>
 __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<R> c)
 {
     const int I = N - S;
     size_t o = 0;
 #define PA_TAKE(field, T, count)                                                   \
     do {                                                                           \
-        o = al(o, sizeof(T) > 4 ? sizeof(T) : 4);                                  \
-        if (c) c->field = (LDS T *)(base + o);                                     \
+        o = al(o, 32);  /* rows of PT are read as 4-vectors */                      \
+        if constexpr (BIND) c->field = (LDS T *)(base + o);                        \
         o += sizeof(T) * (size_t)(count);                                          \
     } while (0)
     PA_TAKE(dist2, double, N);
@@ -125,7 +129,8 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
 
 size_t chain_smem_bytes(int N, int S, int BS, int fp64)
 {
-    return (fp64 ? smem_arrays<double>(N, S, BS, nullptr, nullptr) : smem_arrays<float>(N, S, BS, nullptr, nullptr)) +
+    return (fp64 ? smem_arrays<double, false>(N, S, BS, nullptr, nullptr)
+                 : smem_arrays<float, false>(N, S, BS, nullptr, nullptr)) +
            (fp64 ? sizeof(Ctx<double>) : sizeof(Ctx<float>)) + sizeof(ModelDev);
 }
 
@@ -309,6 +314,8 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
         const GBL int32_t *er0 = exps + o0, *er1 = exps + o1;
         GBL V4<R> *dstp = parts + wb * bufI + (size_t)(s - S) * Ppad;
         GBL int32_t *dste = exps + wb * bufI + (size_t)(s - S) * Ppad;
+        if (tid == 0)   // write partial + exponent; read each child's partial + exponent, or its leaf code
+            c->bytes += (unsigned long long)P * ((4 * sizeof(R) + 4) + (leaf0 ? 1 : 4 * sizeof(R) + 4) + (leaf1 ? 1 : 4 * sizeof(R) + 4));
         for (int p = tid; p < P; p += BS) {
             R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
             int e = 0;
@@ -337,6 +344,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
         const GBL V4<R> *rp = parts + rb * bufI + (size_t)(r - S) * Ppad;
         const GBL int32_t *re = exps + rb * bufI + (size_t)(r - S) * Ppad;
         const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
+        if (tid == 0) c->bytes += (unsigned long long)P * (4 * sizeof(R) + 4 + 4);   // root partial, exponent, weight
         for (int p = tid; p < P; p += BS) {
             const int ex = re[p];
             if (ex == GAPT) continue;
@@ -366,6 +374,13 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
     const GBL int32_t *exps = c->exps;
     const GBL uint8_t *codes = c->codes;
     LDS uint8_t *tgs = c->tgs;
+    if (tid == 0) {
+        // per site: pattern index, root partial, each non-missing internal node's partial, each leaf code
+        int nint = 0, nleaf = 0;
+        for (int k = 0; k < n_int; k++) nint += !(c->flags[c->inodes[k]] & NF_MISSING);
+        for (int q = 0; q < S; q++) nleaf += (c->flags[q] & NF_IN_TREE) && !(c->flags[q] & NF_MISSING);
+        c->bytes += (unsigned long long)GG * (4 + (4 * sizeof(R) + 4) * nint + nleaf);
+    }
     for (int g = tid; g < GG; g += BS) {
         const int p = c->site[g];
         U4 blk = philox(0, (uint32_t)g, c2, c3, k0, k1);
@@ -763,6 +778,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
         c->amb = (GBL uint32_t *)(base + A->off_amb);
         c->accept_n = 0;
         c->accept_c = 0;
+        c->bytes = 0;
     }
     const int32_t *tab = A->tab + E.off_tab;
     for (int i = tid; i < N; i += BS) {
@@ -926,7 +942,7 @@ __global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs A)
     MP M = model_to_lds(Mk, (LDS ModelDev *)&Ms);
     CP<R> c = (CP<R>)&Cs;
     __syncthreads();
-    if (threadIdx.x == 0) smem_arrays<R>(M->N, M->S, BS, (LDS char *)smem_raw, c);
+    if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
     __syncthreads();
     for (;;) {
         if (threadIdx.x == 0) c->work = atomicAdd(A.queue, 1);
@@ -936,6 +952,7 @@ __global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs A)
         if (w >= A.n_work) break;
         bind_element<R, BS>(M, c, &A, A.work[w]);
         for (int ms = 0; ms < 3; ms++) run_model<R, BS>(M, c, &A, ms, w);
+        if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes);
     }
 }
 
@@ -949,7 +966,7 @@ __global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs A)
     MP M = model_to_lds(Mk, (LDS ModelDev *)&Ms);
     CP<R> c = (CP<R>)&Cs;
     __syncthreads();
-    if (threadIdx.x == 0) smem_arrays<R>(M->N, M->S, BS, (LDS char *)smem_raw, c);
+    if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
     __syncthreads();
     for (;;) {
         if (threadIdx.x == 0) c->work = atomicAdd(A.queue, 1);
@@ -979,7 +996,7 @@ __global__ void reset_queue(int *q) { *q = 0; }
 template <typename R, int BS>
 static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, int kind, hipStream_t st)
 {
-    const size_t sm = smem_arrays<R>(M.N, M.S, BS, nullptr, nullptr);
+    const size_t sm = smem_arrays<R, false>(M.N, M.S, BS, nullptr, nullptr);
     hipError_t err;
     if (kind == 0) {
         err = hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);

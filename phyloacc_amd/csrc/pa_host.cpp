@@ -756,7 +756,7 @@ struct DevBuf {
 
 // Common launch path for the chain and log-likelihood kernels.
 static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, int kind, int mode, int bs_req,
-                            ChainArgs &A, DevBuf *bufs, double *elapsed_ms)
+                            ChainArgs &A, DevBuf *bufs, pa_run_stats *stats)
 {
     const int S = e->tm.S, N = e->tm.N;
     // ids arrive in schedule order (largest first for batched runs)
@@ -779,7 +779,10 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     size_t off[5];
     const size_t slot = slot_scratch_bytes(N, S, Pcap, fp64, off);
     HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
-    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n));
+    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + 64));
+    unsigned long long *d_cnt = (unsigned long long *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
+    HIPCHK(e, hipMemsetAsync(d_cnt, 0, 16, e->stream));
+    A.counters = d_cnt;
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, work.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
     A.queue = e->d_queue;
@@ -816,7 +819,16 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     HIPCHK(e, hipEventSynchronize(t1));
     float ms = 0;
     HIPCHK(e, hipEventElapsedTime(&ms, t0, t1));
-    if (elapsed_ms) *elapsed_ms = ms;
+    if (stats) {
+        unsigned long long cnt = 0;
+        HIPCHK(e, hipMemcpy(&cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+        stats->elapsed_ms = ms;
+        stats->algorithmic_bytes = (double)cnt;
+        stats->grid = grid;
+        stats->block_threads = BS;
+        stats->lds_bytes = (int32_t)smem;
+        stats->n_launches = mode == PA_RUN_PER_ELEMENT ? n : 1;
+    }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     return PA_OK;
@@ -883,7 +895,7 @@ extern "C" pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, cons
 }
 
 extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
-                                   pa_chain_result *results, pa_traces *traces, double *elapsed_ms)
+                                   pa_chain_result *results, pa_traces *traces, pa_run_stats *stats)
 {
     if (!e || !ids || !cfg || n < 0) return fail(e, PA_E_INPUT, "null argument");
     if (n == 0) return PA_OK;
@@ -913,7 +925,7 @@ extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, 
     A.maxZ = (int8_t *)bufs[5].p;
     A.tr_d = keep ? (double *)bufs[6].p : nullptr;
     A.tr_Z = keep ? (int8_t *)bufs[7].p : nullptr;
-    st = run_kernel(e, sorted.data(), n, cfg->precision == PA_FP64, 0, cfg->mode, cfg->block_threads, A, bufs, elapsed_ms);
+    st = run_kernel(e, sorted.data(), n, cfg->precision == PA_FP64, 0, cfg->mode, cfg->block_threads, A, bufs, stats);
     if (st != PA_OK) return st;
     std::vector<double> rd((size_t)n * 24);
     std::vector<int32_t> ri((size_t)n * 12);
@@ -950,5 +962,168 @@ extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, 
                 memcpy(traces->Z + (size_t)order[i] * 3 * T * N, &tz[(size_t)i * 3 * T * N], (size_t)3 * T * N);
         }
     }
+    return PA_OK;
+}
+
+// ------------------------------------------------------- chain summaries
+
+namespace {
+
+double lnbeta(double a, double b) { return lgamma(a) + lgamma(b) - lgamma(a + b); }
+
+double log_gamma_density(double x, double a, double b)
+{
+    if (x < 0) return -INFINITY;
+    if (x == 0) return a == 1 ? -log(b) : (a > 1 ? -INFINITY : INFINITY);
+    return (a - 1) * log(x) - x / b - lgamma(a) - a * log(b);
+}
+
+}  // namespace
+
+// Eval2 (src/PhyloAcc-ST/bpp_c2.cpp:14-353) and Output_init
+// (bpp_c2.cpp:739-785) of one element's model chain from its traces.
+extern "C" pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, const double *full,
+                                      const double *nrate, const double *crate, const double *grate,
+                                      const double *lrate, const double *l2rate, const int8_t *Z,
+                                      const int8_t *max_Z, int32_t precision, double *vb, double *medians,
+                                      double *postZ)
+{
+    if (!e || id < 0 || id >= (int)e->elems.size() || model < 0 || model > 2) return fail(e, PA_E_INPUT, "bad argument");
+    const HostElem &E = e->elems[id];
+    if (E.filtered) return fail(e, PA_E_INPUT, "element is filtered");
+    const pa_options &o = e->opt;
+    const int N = e->tm.N, T = o.burnin + o.mcmc, nb = o.burnin, nm = o.mcmc;
+    const int resZ = model == 0 ? 0 : (model == 1 ? 2 : 1);
+    const int32_t *tab = E.tab.data();
+    const std::vector<int> &nodes = E.nodes;
+    const int root = E.root;
+    auto par = [&](int s) { return tab[tab_par(N) + s]; };
+    auto flag = [&](int s) { return (uint32_t)tab[tab_flags(N) + s]; };
+    std::vector<int> fixZ(N, 0);
+    for (int s : nodes) {
+        int fz = (flag(s) & NF_UPPER) || s == root;
+        if (resZ == 2 && (flag(s) & NF_UPPER_CONS)) fz = 1;
+        if (resZ == 0 && s != root) fz = 1;
+        fixZ[s] = fz;
+    }
+    if (medians) {
+        const double *tr[5] = {nrate, crate, grate, lrate, l2rate};
+        std::vector<double> tmp(T);
+        for (int k = 0; k < 5; k++) {
+            if (!tr[k]) { medians[k] = NAN; continue; }
+            tmp.assign(tr[k], tr[k] + T);
+            std::sort(tmp.begin(), tmp.end());
+            medians[k] = tmp[T / 2];
+        }
+    }
+    if (postZ) {
+        for (int s = 0; s < N; s++) {
+            int cnt[4] = {0, 0, 0, 0};
+            for (int i = nb; i < T; i++) cnt[Z[(size_t)i * N + s] + 1]++;
+            if (flag(s) & NF_MISSING) cnt[0] = nm;   // missing nodes: column 0 is MCMC (bpp_c2.cpp:768-771)
+            for (int k = 0; k < 4; k++) postZ[s * 4 + k] = (double)cnt[k] / nm;
+        }
+    }
+    if (!vb) return PA_OK;
+    // configurations of the uncertain nodes after burn-in, newest first, MaxZ first
+    std::vector<int> unc;
+    for (int s = 0; s < N; s++) {
+        int agree = 0;
+        for (int i = nb; i < T; i++) agree += Z[(size_t)i * N + s] == max_Z[s];
+        if (agree < nm) unc.push_back(s);
+    }
+    std::vector<const int8_t *> conf{max_Z};
+    std::vector<int> num{0};
+    std::vector<double> vc{0}, vn{0}, mc{0}, mn{0}, cov{0}, sfl{0};
+    for (int i = T - 1; i >= nb; i--) {
+        const int8_t *row = Z + (size_t)i * N;
+        size_t j = 0;
+        for (; j < conf.size(); j++) {
+            bool same = true;
+            for (int s : unc) if (row[s] != conf[j][s]) { same = false; break; }
+            if (same) break;
+        }
+        if (j == conf.size()) {
+            conf.push_back(row);
+            num.push_back(0); vc.push_back(0); vn.push_back(0); mc.push_back(0); mn.push_back(0);
+            cov.push_back(0); sfl.push_back(0);
+        }
+        num[j]++;
+        vc[j] += crate[i] * crate[i];
+        mc[j] += crate[i];
+        if (resZ != 0) { vn[j] += nrate[i] * nrate[i]; mn[j] += nrate[i]; cov[j] += crate[i] * nrate[i]; }
+        sfl[j] += full[i] / nm;
+    }
+    const double lp[3] = {log(0.5), log(0.5), -INFINITY};
+    auto priorZ = [&](const int8_t *cz) {
+        double n[3][3] = {{0}};
+        for (size_t k = 0; k + 1 < nodes.size(); k++) n[cz[nodes[k]]][cz[par(nodes[k])]] += 1;
+        double p = lnbeta(o.hyper_grate_a + n[1][0] + n[2][0], o.hyper_grate_b + n[0][0]) - lnbeta(o.hyper_grate_a, o.hyper_grate_b);
+        for (size_t k = 0; k + 1 < nodes.size(); k++)
+            if (fixZ[nodes[k]]) n[cz[nodes[k]]][cz[par(nodes[k])]] -= 1;
+        p += lnbeta(o.hyper_lrate_a + n[2][1], o.hyper_lrate_b + n[1][1]) - lnbeta(o.hyper_lrate_a, o.hyper_lrate_b);
+        if (o.hyper_lrate2_a > 0)
+            p += lnbeta(o.hyper_lrate2_a + n[1][2], o.hyper_lrate2_b + n[2][2]) - lnbeta(o.hyper_lrate2_a, o.hyper_lrate2_b);
+        return p + lp[cz[root]];
+    };
+    double VB = 0;
+    int eff = 0;
+    for (size_t i = 0; i < conf.size(); i++) {
+        const double w = (double)num[i] / nm;
+        if (w < 0.02) continue;
+        sfl[i] += num[i] * priorZ(conf[i]) / nm;
+        double logdet;
+        if (resZ == 0) logdet = log(vc[i] - mc[i] * mc[i] / num[i]) - log((double)num[i]);
+        else {
+            const double a = vc[i] - mc[i] * mc[i] / num[i], b = vn[i] - mn[i] * mn[i] / num[i];
+            const double cv = cov[i] - mn[i] * mc[i] / num[i];
+            logdet = log(a * b - cv * cv) - 2 * log((double)num[i]);
+        }
+        if (std::isnan(logdet)) continue;
+        eff += num[i];
+        VB += -w * log((double)num[i]) + sfl[i] + 0.5 * logdet * num[i] / nm;
+    }
+    const double TWO_PI = 6.283185307179586;
+    if (eff == 0) {
+        // rates and Z treated as independent: P(X | Z_i, r*) for every configuration at r* = posterior means
+        double varn = 0, varc = 0, cv = 0, meann = 0, meanc = 0;
+        for (size_t i = 0; i < conf.size(); i++) {
+            varc += vc[i]; meanc += mc[i];
+            if (resZ != 0) { meann += mn[i]; varn += vn[i]; cv += cov[i]; }
+        }
+        const double rn = meann / nm, rc = meanc / nm;
+        const int nc = (int)conf.size();
+        std::vector<int32_t> ids(nc, id), Zs((size_t)nc * N);
+        std::vector<double> rates((size_t)nc * 2), ll(nc);
+        for (int i = 0; i < nc; i++) {
+            for (int s = 0; s < N; s++) Zs[(size_t)i * N + s] = conf[i][s];
+            rates[2 * i] = rc;
+            rates[2 * i + 1] = resZ != 0 ? rn : 1.0;
+        }
+        pa_status st = pa_loglik(e, ids.data(), nc, Zs.data(), rates.data(), precision, ll.data());
+        if (st != PA_OK) return st;
+        double mx = -INFINITY;
+        for (int i = 0; i < nc; i++) { sfl[i] = ll[i] + priorZ(conf[i]); mx = std::max(mx, sfl[i]); }
+        double acc = 0;
+        for (int i = 0; i < nc; i++) acc += exp(sfl[i] - mx);
+        VB = mx == -INFINITY ? -INFINITY : log(acc) + mx;
+        VB += log_gamma_density(rc, o.cprior_a, o.cprior_b);
+        if (resZ != 0) VB += log_gamma_density(rn, o.nprior_a, o.nprior_b);
+        double logdet;
+        if (resZ == 0) {
+            varc -= meanc * meanc / nm;
+            logdet = log(varc) - log((double)nm);
+            VB += 0.5 * logdet + 0.5 * log(TWO_PI);
+        } else {
+            varn -= meann * meann / nm; varc -= meanc * meanc / nm; cv -= meann * meanc / nm;
+            logdet = log(varc * varn - cv * cv) - 2 * log((double)nm);
+            VB += 0.5 * logdet + log(TWO_PI);
+        }
+    } else if (resZ != 0) {
+        VB = VB * ((double)nm / eff) + log((double)nm) + log(TWO_PI) + 1;
+    } else {
+        VB = VB * ((double)nm / eff) + log((double)nm) + 0.5 * (log(TWO_PI) + 1);
+    }
+    *vb = VB;
     return PA_OK;
 }

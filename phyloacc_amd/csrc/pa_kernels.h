@@ -34,6 +34,7 @@ struct ChainArgs {
     const int32_t *Zin;       // [n_work][N]
     const double *rates_in;   // [n_work][2] (r1, r2)
     double *ll_out;           // [n_work]
+    unsigned long long *counters;   // [0]: algorithmic HBM bytes of the launch (or null)
 };
 
 size_t chain_smem_bytes(int N, int S, int BS, int fp64);

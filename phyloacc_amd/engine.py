@@ -26,7 +26,8 @@ PA_RUN_BATCHED, PA_RUN_PER_ELEMENT = 0, 1
 PA_ELEM_OK, PA_ELEM_FILTERED, PA_ELEM_NUMERIC_FAIL = 0, 1, 2
 
 EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_destroy", "pa_last_error",
-           "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count"]
+           "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count",
+           "pa_chain_summary"]
 
 
 class PaError(RuntimeError):
@@ -79,6 +80,11 @@ class Traces(C.Structure):
                [("Z", C.POINTER(C.c_int8)), ("max_Z", C.POINTER(C.c_int8))]
 
 
+class RunStats(C.Structure):
+    _fields_ = [("elapsed_ms", C.c_double), ("algorithmic_bytes", C.c_double), ("grid", C.c_int32),
+                ("block_threads", C.c_int32), ("lds_bytes", C.c_int32), ("n_launches", C.c_int32)]
+
+
 class RunConfig(C.Structure):
     _fields_ = [("precision", C.c_int32), ("mode", C.c_int32), ("block_threads", C.c_int32),
                 ("keep_traces", C.c_int32)]
@@ -111,9 +117,12 @@ def lib():
         L.pa_loglik.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_double), C.c_int32,
                                 P(C.c_double)]
         L.pa_run_chains.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(ChainResult),
-                                    P(Traces), P(C.c_double)]
+                                    P(Traces), P(RunStats)]
         L.pa_rng_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32]
         L.pa_rng_uniform.restype = C.c_double
+        L.pa_chain_summary.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [P(C.c_double)] * 6 + \
+            [P(C.c_int8), P(C.c_int8), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.pa_chain_summary.restype = C.c_int
         L.pa_device_count.argtypes = []
         L.pa_device_count.restype = C.c_int32
         for f in ("pa_parse_mod", "pa_engine_create", "pa_elements_add", "pa_element_get", "pa_loglik",
@@ -250,6 +259,24 @@ class Engine:
                                     _p(rates, C.c_double), int(precision), _p(out, C.c_double)))
         return out
 
+    def chain_summary(self, eid, model, traces, precision=PA_FP64, want_vb=True):
+        """Eval2 marginal likelihood, Output_init medians and Z frequencies of one
+        element's model chain from its traces (dict of [T] arrays and Z [T, N])."""
+        T = self.options.burnin + self.options.mcmc
+        arr = {f: np.ascontiguousarray(traces[f], np.float64).reshape(T)
+               for f in ("full", "nrate", "crate", "grate", "lrate", "l2rate")}
+        Z = np.ascontiguousarray(traces["Z"], np.int8).reshape(T, self.N)
+        mz = np.ascontiguousarray(traces["max_Z"], np.int8).reshape(self.N)
+        vb = np.zeros(1)
+        med = np.zeros(5)
+        pz = np.zeros(self.N * 4)
+        self._check(lib().pa_chain_summary(
+            self._h, int(eid), int(model), *[_p(arr[f], C.c_double) for f in ("full", "nrate", "crate", "grate",
+                                                                                 "lrate", "l2rate")],
+            _p(Z, C.c_int8), _p(mz, C.c_int8), int(precision), _p(vb, C.c_double) if want_vb else None,
+            _p(med, C.c_double), _p(pz, C.c_double)))
+        return dict(vb=float(vb[0]) if want_vb else None, med=med, postZ=pz.reshape(self.N, 4))
+
     def run_chains(self, ids, precision=PA_FP32, mode=PA_RUN_BATCHED, traces=False, block_threads=0,
                    keep_traces=False):
         """Runs M0, M1, M2 for every id.  Returns per-(element, model) arrays, models in run order."""
@@ -271,11 +298,12 @@ class Engine:
             out["Z"] = z
             tr.Z = _p(z, C.c_int8)
         cfg = RunConfig(int(precision), int(mode), int(block_threads), int(bool(keep_traces)))
-        ms = C.c_double()
+        stats = RunStats()
         self._check(lib().pa_run_chains(self._h, _p(ids, C.c_int32), n, C.byref(cfg), res, C.byref(tr),
-                                        C.byref(ms)))
+                                        C.byref(stats)))
         out["max_Z"] = maxZ
-        out["elapsed_ms"] = ms.value
+        out["elapsed_ms"] = stats.elapsed_ms
+        out["stats"] = {f: getattr(stats, f) for f, _ in RunStats._fields_}
         for f, t in (("max_loglik", np.float64), ("max_m", np.int32), ("status", np.int32),
                      ("final_nrate", np.float64), ("final_crate", np.float64), ("final_grate", np.float64),
                      ("final_lrate", np.float64), ("final_lrate2", np.float64), ("accept_n", np.int32),

@@ -142,7 +142,7 @@ def trans_batch(lam, V, Vi, t):
     """P(t) for a vector of times t -> [len(t), 4, 4] (row = from)."""
     t = np.asarray(t, np.float64)
     E = np.exp(lam[None, :] * t[:, None])
-    P = np.einsum("ik,nk,kj->nij", V, E, Vi)
+    P = (V[None, :, :] * E[:, None, :]) @ Vi
     return np.clip(P, 0.0, None)
 
 
@@ -202,6 +202,7 @@ def simulate(tree, pi, Q, sizes, rates, gap_frac, rng):
     C = len(sizes)
     G = int(np.sum(sizes))
     elem = np.repeat(np.arange(C), sizes)
+    elem16 = elem.astype(np.int64) * 16
     N = tree.N
     states = np.empty((N, G), np.uint8)
     cpi = np.cumsum(pi)
@@ -218,10 +219,13 @@ def simulate(tree, pi, Q, sizes, rates, gap_frac, rng):
             continue
         p = tree.parent[v]
         P = trans_batch(lam, V, Vi, tree.dist[v] * rates[:, v])        # [C,4,4]
-        cum = np.cumsum(P, axis=2)                                      # [C,4,4]
-        rows = cum[elem, states[p]]                                     # [G,4]
-        u = rng.random(G) * rows[:, 3]
-        states[v] = (u[:, None] >= rows[:, :3]).sum(1).astype(np.uint8)
+        cum = np.cumsum(P, axis=2).reshape(-1)                          # flat [C*16]
+        base = elem16 + states[p].astype(np.int64) * 4
+        u = rng.random(G) * np.take(cum, base + 3)
+        st = (u >= np.take(cum, base)).astype(np.uint8)
+        st += u >= np.take(cum, base + 1)
+        st += u >= np.take(cum, base + 2)
+        states[v] = st
     align = CODES[states[: tree.S]]
     if gap_frac > 0:
         mask = rng.random(align.shape) < gap_frac
@@ -277,9 +281,10 @@ def make_workload(config: int, seed: int = 1, n_elements: int | None = None) -> 
         rates[:] = r1[:, None]
         frac = rng.uniform(0.01, 0.20, size=C)
         nacc = np.maximum(1, np.rint(frac * (N - 1))).astype(int)
-        for c in range(C):
-            br = rng.choice(N - 1, size=nacc[c], replace=False)
-            rates[c, br] = r2[c]
+        # nacc[c] distinct random branches per element: ranks of a random key
+        rank = np.argsort(np.argsort(rng.random((C, N - 1)), axis=1), axis=1)
+        acc = rank < nacc[:, None]
+        rates[:, : N - 1] = np.where(acc, r2[:, None], r1[:, None])
     align = simulate(tree, pi, Q, sizes, rates, gap, rng)
     starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
     return Workload(f"config{config}", tree, pi, Q, mod_text(pi, Q, tree), align,

@@ -126,3 +126,34 @@ def test_uniform_slots():
     us = np.array([O.lib().or_uniform(5, c0, 3, 4, 5, lane) for c0 in range(2000) for lane in range(4)])
     assert us.min() > 0 and us.max() < 1
     assert scipy.stats.kstest(us, "uniform").pvalue > 1e-3
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_z_update_matches_exact_enumeration(case):
+    """Update_Z_subtree's forward-backward draw (bpp_c.cpp:1182-1293) against the
+    exact distribution over every Z configuration of a 4-tip tree."""
+    C = O.C
+    nodes = np.ascontiguousarray(G["zc_nodes"], np.int32)
+    parent = np.array(G["zc_parent"], np.int32)
+    N = len(parent)
+    parent[N - 1] = N
+    fixZ = np.ascontiguousarray(G[f"zc{case}_fixZ"], np.int32)
+    E = np.ascontiguousarray(G["zc_E"], np.float64)
+    E[N - 1] = 0.0        # the root carries no branch emission
+    confs, w = G[f"zc{case}_confs"], G[f"zc{case}_w"]
+    index = {tuple(cf): i for i, cf in enumerate(confs)}
+    counts = np.zeros(len(confs))
+    Zout = np.zeros(N, np.int32)
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    n = 20000
+    for m in range(n):
+        O.lib().or_test_update_Z(N, P(nodes, C.c_int), len(nodes), P(parent, C.c_int), P(fixZ, C.c_int),
+                                 P(E, C.c_double), C.c_double(0.4), C.c_double(0.2), C.c_double(0.1),
+                                 C.c_uint64(99), m, 1, P(Zout, C.c_int))
+        counts[index[tuple(Zout[nodes])]] += 1
+    exp = w * n
+    keep = exp > 5
+    chi2 = np.sum((counts[keep] - exp[keep]) ** 2 / exp[keep])
+    dof = keep.sum() - 1
+    assert scipy.stats.chi2.sf(chi2, dof) > 1e-4, (chi2, dof)
+    assert counts[~keep].sum() <= max(20, 3 * exp[~keep].sum())
