@@ -107,6 +107,23 @@ const char *pa_last_error(const pa_engine *e);
 pa_status pa_elements_add(pa_engine *e, const char *const *rows, const int64_t *starts,
                           const int32_t *lengths, int32_t n, int32_t *ids, int32_t *status);
 
+/* Elements that carry their own tree over the same S leaves: the per-element
+ * gene trees of PhyloAcc-GT (src/PhyloAcc-GT/genetree.hpp:16-127).  children:
+ * [n*N*2], internal ids in post-order, root N-1.  Encoding as pa_elements_add
+ * without the outgroup collapse and the CONSERVE filter. */
+pa_status pa_elements_add_trees(pa_engine *e, const char *const *rows, const int64_t *starts,
+                                const int32_t *lengths, int32_t n, const int32_t *children, int32_t *ids,
+                                int32_t *status);
+
+/* Substitution length of each gene branch from its pieces in species branches
+ * (K pieces: gene node, species branch, length) at rates 1 / r1 / r2 for
+ * species-branch Z = 0 / 1 / 2.  Replaces the per-piece getlogTMc composition
+ * of GTree::Update_Tg (src/PhyloAcc-GT/genetree.cpp:1330-1426;
+ * src/PhyloAcc-GT/bpp.cpp:221-235): P(a)P(b) = P(a+b) for one Q, and pieces
+ * shorter than 1e-8 are the identity there and add nothing here. */
+pa_status pa_gene_lengths(int32_t N, int32_t K, const int32_t *seg_gene, const int32_t *seg_species,
+                          const double *seg_len, const int32_t *Z, double r1, double r2, double *out);
+
 /* Host-side view of one encoded element (for checks and drop-in outputs). */
 typedef struct {
     int32_t filtered, GG, P, root, n_nodes, n_internal;
@@ -124,6 +141,11 @@ pa_status pa_element_get(const pa_engine *e, int32_t id, pa_element_info *info,
  * rates: [n*2] (r1 conserved, r2 accelerated).  out: [n]. */
 pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *Z,
                     const double *rates, int32_t precision, double *out);
+
+/* Per-element log-likelihood of the element's own tree with per-call branch
+ * lengths [n*N] (all at rate 1): batched pruning over uneven gene trees. */
+pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                            int32_t precision, double *out);
 
 /* Results of one model chain of one element (BPP_C::Gibbs bookkeeping,
  * src/PhyloAcc-ST/bpp_c.cpp:470-485). */

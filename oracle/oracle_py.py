@@ -108,6 +108,9 @@ def lib():
         L.or_run_element.argtypes = [P(Model), P(Options), C.c_void_p, P(ChainOut)]
         L.or_test_update_Z.argtypes = [C.c_int, P(C.c_int), C.c_int, P(C.c_int), P(C.c_int), P(C.c_double),
                                        C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, C.c_int, P(C.c_int)]
+        L.or_gt_loglik.argtypes = [P(Model), C.c_int, P(C.c_int), C.c_int, P(C.c_int), P(C.c_int), P(C.c_double),
+                                   P(C.c_int), C.c_double, C.c_double, P(C.c_char_p), C.c_long, C.c_int, C.c_int, C.c_double]
+        L.or_gt_loglik.restype = C.c_double
         L.or_run_batch.argtypes = [P(Model), P(Options), P(C.c_void_p), C.c_int, C.c_int, P(C.c_double)]
         L.or_run_batch.restype = C.c_long
         _lib = L
@@ -229,3 +232,17 @@ def run_batch(model, opts, elements, nthreads=0):
     ml = np.zeros(3 * len(elements), np.float64)
     n = lib().or_run_batch(C.byref(model), C.byref(opts), arr, len(elements), nthreads, _ptr(ml, C.c_double))
     return n, ml.reshape(-1, 3)
+
+
+def gt_loglik(model, S, gchildren, seg_gene, seg_sp, seg_len, Z, r1, r2, align, start, length, gapchar="-",
+              gap_prop=0.8):
+    rows_keep = [np.ascontiguousarray(align[s]).tobytes() for s in range(align.shape[0])]
+    rows = (C.c_char_p * len(rows_keep))(*rows_keep)
+    ch = np.ascontiguousarray(gchildren, np.int32).reshape(-1)
+    sg = np.ascontiguousarray(seg_gene, np.int32)
+    ss = np.ascontiguousarray(seg_sp, np.int32)
+    sl = np.ascontiguousarray(seg_len, np.float64)
+    z = np.ascontiguousarray(Z, np.int32)
+    return lib().or_gt_loglik(C.byref(model), int(S), _ptr(ch, C.c_int), len(sg), _ptr(sg, C.c_int),
+                              _ptr(ss, C.c_int), _ptr(sl, C.c_double), _ptr(z, C.c_int), float(r1), float(r2),
+                              rows, int(start), int(length), ord(gapchar), float(gap_prop))

@@ -1207,3 +1207,68 @@ int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, co
     free(e);
     return 0;
 }
+
+/* Gene-tree pruning at fixed species-branch Z and rates: GTree::Update_Tg
+ * (tosample = false, src/PhyloAcc-GT/genetree.cpp:1330-1426).  Every gene
+ * branch's message is composed piece by piece, bottom-up, one log_multi per
+ * piece it spends in a species branch, at that branch's rate; pieces shorter
+ * than 1e-8 are the identity (getlogTMc, src/PhyloAcc-GT/bpp.cpp:221-235).
+ * Gene leaves are the species 0..S-1; internal gene ids are post-order. */
+double or_gt_loglik(const or_model *m, int S, const int *gchildren, int K, const int *seg_gene, const int *seg_sp,
+                    const double *seg_len, const int *Z, double r1, double r2, const char *const *rows, long start,
+                    int GG, int gapchar, double gap_prop)
+{
+    int N = 2 * S - 1;
+    double logpi[4];
+    for (int j = 0; j < 4; j++) logpi[j] = log(m->pi[j]);
+    /* per piece transition (log P transposed) */
+    double (*L)[16] = (double (*)[16])malloc(sizeof(double) * 16 * (K > 0 ? K : 1));
+    for (int k = 0; k < K; k++) {
+        int z = Z[seg_sp[k]];
+        double rate = z == 2 ? r2 : (z == 1 ? r1 : 1.0);
+        if (seg_len[k] < 1e-8) log_identity(L[k]);
+        else or_log_trans(m, seg_len[k] * rate, L[k]);
+    }
+    double (*lam)[4] = (double (*)[4])malloc(sizeof(double) * 4 * N);
+    int *gap = (int *)malloc(sizeof(int) * N);
+    /* missing genes: a leaf with more than gap_prop*GG gap characters, an
+     * internal gene node whose children are both missing (bpp_c.cpp:34-60);
+     * the pruning skips them (bpp_c.cpp:1130, 1156, 1186) */
+    int *miss = (int *)calloc(N, sizeof(int));
+    for (int s = 0; s < S; s++) {
+        int ng = 0;
+        for (int g = 0; g < GG; g++) ng += tolower((unsigned char)rows[s][start + g]) == gapchar;
+        miss[s] = ng > gap_prop * GG;
+    }
+    for (int s = S; s < N; s++) miss[s] = miss[gchildren[2 * s]] && miss[gchildren[2 * s + 1]];
+    double total = 0;
+    for (int g = 0; g < GG; g++) {
+        for (int s = 0; s < S; s++) {
+            int Tg; unsigned char code;
+            encode_char(tolower((unsigned char)rows[s][start + g]), gapchar, lam[s], &Tg, &code);
+            gap[s] = Tg >= 4;
+        }
+        for (int s = S; s < N; s++) {
+            for (int j = 0; j < 4; j++) lam[s][j] = 0;
+            int a = gchildren[2 * s], b = gchildren[2 * s + 1];
+            gap[s] = gap[a] && gap[b];
+            for (int q = 0; q < 2; q++) {
+                int x = q ? b : a;
+                if (gap[x] || miss[x]) continue;
+                double msg[4], tmp[4];
+                memcpy(msg, lam[x], sizeof msg);
+                for (int k = 0; k < K; k++) {
+                    if (seg_gene[k] != x) continue;
+                    log_multi(L[k], msg, tmp);
+                    memcpy(msg, tmp, sizeof msg);
+                }
+                for (int j = 0; j < 4; j++) lam[s][j] += msg[j];
+            }
+        }
+        double y[4];
+        for (int j = 0; j < 4; j++) y[j] = lam[N - 1][j] + logpi[j];
+        total += log_exp_sum(y, 4);
+    }
+    free(L); free(lam); free(gap); free(miss);
+    return total;
+}

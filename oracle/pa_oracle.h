@@ -104,6 +104,11 @@ double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, doub
 /* full three-model chain of one element, outs[0..2] = M0, M1, M2 (run order resZ 0, 2, 1) */
 int or_run_element(const or_model *m, const or_options *o, or_element *e, or_chain_out *outs);
 
+/* gene-tree log-likelihood at fixed species-branch Z and rates (PhyloAcc-GT pruning) */
+double or_gt_loglik(const or_model *m, int S, const int *gchildren, int K, const int *seg_gene, const int *seg_sp,
+                    const double *seg_len, const int *Z, double r1, double r2, const char *const *rows, long start,
+                    int GG, int gapchar, double gap_prop);
+
 /* test hook: one Z forward-backward draw on a bare tree */
 int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, const int *fixZ, const double *E,
                      double g, double l, double l2, uint64_t seed, int m, int ms, int *Zout);

@@ -977,6 +977,12 @@ __global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs A)
         bind_element<R, BS>(M, c, &A, A.work[w]);
         const double r1 = A.rates_in[2 * w], r2 = A.rates_in[2 * w + 1];
         for (int i = threadIdx.x; i < M->N; i += BS) c->Z[i] = (int8_t)A.Zin[(size_t)w * M->N + i];
+        if (A.len_in) {
+            // own-tree element: per-call lengths, every branch in the rate-1 class
+            for (int i = threadIdx.x; i < M->N; i += BS) c->dist2[i] = A.len_in[(size_t)w * M->N + i];
+            __syncthreads();
+            for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, 0, c->nodes[q], 1.0, 0);
+        }
         for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
             const int s = c->nodes[q];
             class_table<R>(M, c, 1, s, r1, 0);

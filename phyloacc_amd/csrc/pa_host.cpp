@@ -342,7 +342,7 @@ struct Collapse {
 };
 
 void encode_element(const TreeModel &tm, const pa_options &o, const char *const *rows, int64_t start, int len,
-                    HostElem &E)
+                    HostElem &E, const int32_t *own_children = nullptr)
 {
     const int S = tm.S, N = tm.N;
     E = HostElem();
@@ -376,6 +376,13 @@ void encode_element(const TreeModel &tm, const pa_options &o, const char *const 
     std::vector<uint8_t> missing(N, 0);
     std::vector<int> ch2(tm.ch), par2(tm.par);
     std::vector<double> dist2(tm.dist);
+    if (own_children) {
+        // an element with its own tree (a gene tree over the same leaves): lengths come per call
+        ch2.assign(own_children, own_children + 2 * N);
+        for (int i = 0; i < N; i++) par2[i] = -1;
+        for (int i = S; i < N; i++) { par2[ch2[2 * i]] = i; par2[ch2[2 * i + 1]] = i; }
+        std::fill(dist2.begin(), dist2.end(), 0.0);
+    }
     for (int s = S; s < N; s++) {
         for (int q = 0; q < 2; q++) {
             int x = ch2[2 * s + q];
@@ -386,11 +393,11 @@ void encode_element(const TreeModel &tm, const pa_options &o, const char *const 
     // CONSERVE filter (bpp_c.hpp:361-373)
     int ct = 0;
     for (int x : tm.conserve) ct += missing[x];
-    if (ct > o.conserve_prop * (double)tm.conserve.size()) { E.filtered = 1; return; }
+    if (!own_children && ct > o.conserve_prop * (double)tm.conserve.size()) { E.filtered = 1; return; }
     // outgroup-lineage (or whole-tree) collapse (bpp_c.hpp:379-390)
     int root = N - 1;
-    std::vector<uint8_t> all(N, 1);
-    Collapse col{ch2, par2, dist2, missing, o.prune_tree ? all : tm.upper, S, root};
+    std::vector<uint8_t> all(N, 1), none(N, 0);   // gene trees are never collapsed
+    Collapse col{ch2, par2, dist2, missing, own_children ? none : (o.prune_tree ? all : tm.upper), S, root};
     col.run(N - 1, -1);
     par2[root] = N;
     if (root < S) { E.filtered = 1; return; }   // nothing left to prune over
@@ -423,8 +430,8 @@ void encode_element(const TreeModel &tm, const pa_options &o, const char *const 
         T[tab_ch0(N) + i] = ch2[2 * i];
         T[tab_ch1(N) + i] = ch2[2 * i + 1];
         T[tab_par(N) + i] = par2[i];
-        T[tab_flags(N) + i] = (missing[i] ? NF_MISSING : 0) | (tm.upper[i] ? NF_UPPER : 0) |
-                              (tm.upper_cons[i] ? NF_UPPER_CONS : 0) | (i < S ? NF_LEAF : 0);
+        T[tab_flags(N) + i] = (missing[i] ? NF_MISSING : 0) | (!own_children && tm.upper[i] ? NF_UPPER : 0) |
+                              (!own_children && tm.upper_cons[i] ? NF_UPPER_CONS : 0) | (i < S ? NF_LEAF : 0);
     }
     for (int v : nodes) T[tab_flags(N) + v] |= NF_IN_TREE;
     for (size_t k = 0; k < nodes.size(); k++) {
@@ -670,6 +677,59 @@ extern "C" pa_status pa_elements_add(pa_engine *e, const char *const *rows, cons
     return PA_OK;
 }
 
+// Elements that carry their own tree over the same leaves (gene trees,
+// PhyloAcc-GT): children is [n][N][2] in post-order numbering, root N-1.
+extern "C" pa_status pa_elements_add_trees(pa_engine *e, const char *const *rows, const int64_t *starts,
+                                           const int32_t *lengths, int32_t n, const int32_t *children, int32_t *ids,
+                                           int32_t *status)
+{
+    if (!e || !rows || !starts || !lengths || !children || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    const int N = e->tm.N, S = e->tm.S;
+    for (int i = 0; i < n; i++) {
+        const int32_t *ch = children + (size_t)i * 2 * N;
+        for (int v = 0; v < N; v++) {
+            const bool leaf = v < S;
+            if (leaf != (ch[2 * v] == -1) || (!leaf && (ch[2 * v] >= v || ch[2 * v + 1] >= v || ch[2 * v] < 0)))
+                return fail(e, PA_E_INPUT, "element tree " + std::to_string(i) + " is not in post-order numbering");
+        }
+    }
+    const size_t base = e->elems.size();
+    e->elems.resize(base + n);
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 64) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (int i = (int)t; i < n; i += (int)nt)
+                encode_element(e->tm, e->opt, rows, starts[i], lengths[i], e->elems[base + i], children + (size_t)i * 2 * N);
+        });
+    for (auto &x : th) x.join();
+    for (int i = 0; i < n; i++) {
+        if (ids) ids[i] = (int32_t)(base + i);
+        if (status) status[i] = e->elems[base + i].filtered ? PA_ELEM_FILTERED : PA_ELEM_OK;
+    }
+    e->dev_dirty = true;
+    return PA_OK;
+}
+
+// Substitution length of every gene branch from the pieces it spends in
+// species branches: sum of piece length x rate of the piece's species branch
+// (Z = 0 / 1 / 2 -> 1 / r1 / r2); pieces shorter than 1e-8 add nothing, as the
+// identity matrix of getlogTMc (src/PhyloAcc-GT/bpp.cpp:221-235).
+extern "C" pa_status pa_gene_lengths(int32_t N, int32_t K, const int32_t *seg_gene, const int32_t *seg_species,
+                                     const double *seg_len, const int32_t *Z, double r1, double r2, double *out)
+{
+    if (N <= 0 || K < 0 || !out) return PA_E_INPUT;
+    for (int i = 0; i < N; i++) out[i] = 0;
+    for (int k = 0; k < K; k++) {
+        if (seg_gene[k] < 0 || seg_gene[k] >= N || seg_species[k] < 0) return PA_E_INPUT;
+        if (seg_len[k] < 1e-8) continue;
+        const int z = Z[seg_species[k]];
+        out[seg_gene[k]] += seg_len[k] * (z == 2 ? r2 : (z == 1 ? r1 : 1.0));
+    }
+    return PA_OK;
+}
+
 extern "C" pa_status pa_element_get(const pa_engine *e, int32_t id, pa_element_info *info, int32_t *children2,
                                     int32_t *parent2, double *dist2, int32_t *missing, int32_t *nodes,
                                     uint8_t *site_codes, int32_t *site_pattern, int32_t *weights)
@@ -884,6 +944,46 @@ extern "C" pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, cons
     A.Zin = (const int32_t *)bufs[3].p;
     A.rates_in = (const double *)bufs[4].p;
     A.ll_out = (double *)bufs[5].p;
+    std::vector<int32_t> sorted(n);
+    for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
+    st = run_kernel(e, sorted.data(), n, precision == PA_FP64, 1, PA_RUN_BATCHED, 0, A, bufs, nullptr);
+    if (st != PA_OK) return st;
+    std::vector<double> ll(n);
+    HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) out[order[i]] = ll[i];
+    return PA_OK;
+}
+
+// Log-likelihood with per-call branch lengths (every branch at rate 1):
+// the pruning of an element's own tree, e.g. a gene tree whose lengths come
+// from pa_gene_lengths.
+extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                                       int32_t precision, double *out)
+{
+    if (!e || !ids || !lengths || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    if (n == 0) return PA_OK;
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    const int N = e->tm.N;
+    std::vector<int32_t> order = schedule(e, ids, n, PA_RUN_BATCHED);
+    std::vector<int32_t> Zw((size_t)n * N, 0);
+    std::vector<double> lw((size_t)n * N), rw((size_t)n * 2, 1.0);
+    for (int i = 0; i < n; i++) memcpy(&lw[(size_t)i * N], lengths + (size_t)order[i] * N, sizeof(double) * N);
+    DevBuf bufs[7];
+    HIPCHK(e, hipMalloc(&bufs[3].p, Zw.size() * 4));
+    HIPCHK(e, hipMalloc(&bufs[4].p, rw.size() * 8));
+    HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
+    HIPCHK(e, hipMalloc(&bufs[6].p, lw.size() * 8));
+    HIPCHK(e, hipMemcpy(bufs[3].p, Zw.data(), Zw.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(bufs[4].p, rw.data(), rw.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(bufs[6].p, lw.data(), lw.size() * 8, hipMemcpyHostToDevice));
+    ChainArgs A;
+    memset(&A, 0, sizeof A);
+    A.Zin = (const int32_t *)bufs[3].p;
+    A.rates_in = (const double *)bufs[4].p;
+    A.ll_out = (double *)bufs[5].p;
+    A.len_in = (const double *)bufs[6].p;
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
     st = run_kernel(e, sorted.data(), n, precision == PA_FP64, 1, PA_RUN_BATCHED, 0, A, bufs, nullptr);

@@ -34,6 +34,7 @@ struct ChainArgs {
     const int32_t *Zin;       // [n_work][N]
     const double *rates_in;   // [n_work][2] (r1, r2)
     double *ll_out;           // [n_work]
+    const double *len_in;     // [n_work][N] per-call branch lengths (own-tree elements) or null
     unsigned long long *counters;   // [0]: algorithmic HBM bytes of the launch (or null)
 };
 

@@ -27,7 +27,7 @@ PA_ELEM_OK, PA_ELEM_FILTERED, PA_ELEM_NUMERIC_FAIL = 0, 1, 2
 
 EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_destroy", "pa_last_error",
            "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count",
-           "pa_chain_summary"]
+           "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths"]
 
 
 class PaError(RuntimeError):
@@ -123,6 +123,15 @@ def lib():
         L.pa_chain_summary.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [P(C.c_double)] * 6 + \
             [P(C.c_int8), P(C.c_int8), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.pa_chain_summary.restype = C.c_int
+        L.pa_elements_add_trees.argtypes = [C.c_void_p, P(C.c_char_p), P(C.c_int64), P(C.c_int32), C.c_int32,
+                                            P(C.c_int32), P(C.c_int32), P(C.c_int32)]
+        L.pa_elements_add_trees.restype = C.c_int
+        L.pa_gene_lengths.argtypes = [C.c_int32, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double),
+                                      P(C.c_int32), C.c_double, C.c_double, P(C.c_double)]
+        L.pa_gene_lengths.restype = C.c_int
+        L.pa_loglik_lengths.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), C.c_int32,
+                                        P(C.c_double)]
+        L.pa_loglik_lengths.restype = C.c_int
         L.pa_device_count.argtypes = []
         L.pa_device_count.restype = C.c_int32
         for f in ("pa_parse_mod", "pa_engine_create", "pa_elements_add", "pa_element_get", "pa_loglik",
@@ -226,6 +235,29 @@ class Engine:
                                           _p(ids, C.c_int32), _p(status, C.c_int32)))
         return ids, status
 
+    def add_elements_with_trees(self, align: np.ndarray, starts, lengths, children):
+        """Elements carrying their own tree (gene trees): children [n, N, 2]."""
+        align = np.ascontiguousarray(align, np.uint8)
+        self._rows.append(align)
+        ptrs = (C.c_char_p * self.S)(*[C.c_char_p(align[s].ctypes.data) for s in range(self.S)])
+        st = np.ascontiguousarray(starts, np.int64)
+        ln = np.ascontiguousarray(lengths, np.int32)
+        ch = np.ascontiguousarray(children, np.int32)
+        n = len(st)
+        ids = np.zeros(n, np.int32)
+        status = np.zeros(n, np.int32)
+        self._check(lib().pa_elements_add_trees(self._h, ptrs, _p(st, C.c_int64), _p(ln, C.c_int32), n,
+                                                _p(ch, C.c_int32), _p(ids, C.c_int32), _p(status, C.c_int32)))
+        return ids, status
+
+    def loglik_lengths(self, ids, lengths, precision=PA_FP64):
+        ids = np.ascontiguousarray(ids, np.int32)
+        L = np.ascontiguousarray(lengths, np.float64).reshape(len(ids), self.N)
+        out = np.zeros(len(ids), np.float64)
+        self._check(lib().pa_loglik_lengths(self._h, _p(ids, C.c_int32), len(ids), _p(L, C.c_double),
+                                            int(precision), _p(out, C.c_double)))
+        return out
+
     def element(self, eid: int):
         info = ElementInfo()
         self._check(lib().pa_element_get(self._h, int(eid), C.byref(info), None, None, None, None, None, None,
@@ -310,6 +342,19 @@ class Engine:
                      ("accept_c", np.int32)):
             out[f] = np.array([getattr(res[i], f) for i in range(3 * n)], t).reshape(n, 3)
         return out
+
+
+def gene_lengths(N, seg_gene, seg_species, seg_len, Z, r1, r2):
+    sg = np.ascontiguousarray(seg_gene, np.int32)
+    ss = np.ascontiguousarray(seg_species, np.int32)
+    sl = np.ascontiguousarray(seg_len, np.float64)
+    z = np.ascontiguousarray(Z, np.int32)
+    out = np.zeros(N, np.float64)
+    st = lib().pa_gene_lengths(int(N), len(sg), _p(sg, C.c_int32), _p(ss, C.c_int32), _p(sl, C.c_double),
+                               _p(z, C.c_int32), float(r1), float(r2), _p(out, C.c_double))
+    if st != PA_OK:
+        raise PaError(st, "pa_gene_lengths failed")
+    return out
 
 
 def device_count() -> int:

@@ -307,3 +307,241 @@ def small_workload(S=6, C=4, length=40, seed=3, gap=0.05, mean_bl=0.15):
     small_, big = (ca, cb) if len(ca) <= len(cb) else (cb, ca)
     return Workload("small", tree, pi, Q, mod_text(pi, Q, tree), align, sizes, starts,
                     small_[:1], [x for x in big[1:]], big[:1], rates, seed)
+
+
+# ----------------------------------------------------------------- gene trees
+
+@dataclasses.dataclass
+class GeneTree:
+    """One element's gene tree embedded in the species tree (PhyloAcc-GT style).
+
+    Leaves 0..S-1 are the species' sequences; internal gene nodes S..N-1 are
+    numbered in post-order of the gene tree; root N-1.  seg_* list, for every
+    gene branch, the pieces it spends in each species branch: (gene child,
+    species branch, length in coalescent units)."""
+    children: np.ndarray      # [N, 2]
+    parent: np.ndarray        # [N]
+    height: np.ndarray        # [N] coalescent-unit heights
+    seg_gene: np.ndarray      # [K] gene node whose parent branch holds the piece
+    seg_species: np.ndarray   # [K] species branch (node id of its lower end)
+    seg_len: np.ndarray       # [K] length in coalescent units
+
+
+def ultrametric_tree(S, mean_interval, rng, prefix="sp", mean_bl=None):
+    """Random ultrametric species tree: lineages merge pairwise at Exp(mean)
+    intervals; with ``mean_bl`` all heights are rescaled to that mean branch length."""
+    lineages = [{"kids": [], "h": 0.0} for _ in range(S)]
+    h = 0.0
+    while len(lineages) > 1:
+        h += rng.exponential(mean_interval)
+        i, j = rng.choice(len(lineages), 2, replace=False)
+        a, b = lineages[i], lineages[j]
+        v = {"kids": [a, b], "h": h}
+        lineages = [x for k, x in enumerate(lineages) if k not in (i, j)] + [v]
+    root = lineages[0]
+    if mean_bl is not None:
+        tot = sum(v["h"] - k["h"] for v in _walk(root) for k in v["kids"])
+        f = mean_bl * (2 * S - 2) / tot
+        for v in _walk(root):
+            v["h"] *= f
+    for v in _walk(root):
+        for k in v["kids"]:
+            k["bl"] = float(np.float32(v["h"] - k["h"]))
+    root["bl"] = 0.0
+    t = _finish(root, S, prefix)
+    heights = np.zeros(t.N)
+    for v in _walk(root):
+        heights[v["id"]] = v["h"]
+    return t, heights
+
+
+def msc_gene_tree(tree: Tree, sp_height, rng) -> GeneTree:
+    """Multispecies-coalescent draw: within species branch s, k lineages
+    coalesce at rate k(k-1)/2 per coalescent unit until the branch's top."""
+    S, N = tree.S, tree.N
+    post = []
+    stack = [(N - 1, 0)]
+    while stack:
+        v, k = stack.pop()
+        if v >= S and k == 0:
+            stack.append((v, 1))
+            stack.append((tree.children[v][1], 0))
+            stack.append((tree.children[v][0], 0))
+        else:
+            post.append(v)
+    # gene nodes: dicts with kids, height, species branch where they were born
+    nodes = [{"kids": [], "h": 0.0, "sp": s, "id": s} for s in range(S)]
+    pieces = []                      # (gene node index, species branch, length)
+    lineages = {s: [s] for s in range(S)}   # species node -> gene node indices entering its branch
+    for s in post:
+        if s >= S:
+            a, b = tree.children[s]
+            lin = lineages[a] + lineages[b]
+        else:
+            lin = list(lineages[s])
+        bottom = sp_height[s]
+        top = sp_height[tree.parent[s]] if s != N - 1 else np.inf
+        h = bottom
+        enter = {g: bottom for g in lin}
+        while len(lin) > 1:
+            k = len(lin)
+            h_next = h + rng.exponential(2.0 / (k * (k - 1)))
+            if h_next >= top:
+                break
+            i, j = rng.choice(k, 2, replace=False)
+            gi, gj = lin[i], lin[j]
+            nid = len(nodes)
+            nodes.append({"kids": [gi, gj], "h": h_next, "sp": s})
+            for g in (gi, gj):
+                pieces.append((g, s, h_next - max(enter[g], nodes[g]["h"])))
+            lin = [x for q, x in enumerate(lin) if q not in (i, j)] + [nid]
+            enter[nid] = h_next
+            h = h_next
+        if s != N - 1:
+            for g in lin:
+                pieces.append((g, s, top - max(enter[g], nodes[g]["h"])))
+            lineages[s] = lin
+    # renumber internal gene nodes in gene-tree post-order (leaves keep species ids)
+    groot = len(nodes) - 1
+    order = []
+    st = [(groot, 0)]
+    while st:
+        v, k = st.pop()
+        if nodes[v]["kids"] and k == 0:
+            st.append((v, 1))
+            st.append((nodes[v]["kids"][1], 0))
+            st.append((nodes[v]["kids"][0], 0))
+        else:
+            order.append(v)
+    newid = {}
+    nxt = S
+    for v in order:
+        if nodes[v]["kids"]:
+            newid[v] = nxt
+            nxt += 1
+        else:
+            newid[v] = v
+    children = -np.ones((N, 2), np.int32)
+    parent = -np.ones(N, np.int32)
+    height = np.zeros(N)
+    for v, d in enumerate(nodes):
+        i = newid[v]
+        height[i] = d["h"]
+        if d["kids"]:
+            a, b = sorted(newid[k] for k in d["kids"])
+            children[i] = (a, b)
+            parent[a] = i
+            parent[b] = i
+    seg_gene = np.array([newid[g] for g, _, _ in pieces], np.int32)
+    seg_sp = np.array([s for _, s, _ in pieces], np.int32)
+    seg_len = np.array([l for _, _, l in pieces], np.float64)
+    return GeneTree(children, parent, height, seg_gene, seg_sp, seg_len)
+
+
+def gene_effective_lengths(gt: GeneTree, Z, r1, r2, scale):
+    """Substitution length of every gene branch: sum of its pieces' lengths
+    times the rate class of the species branch holding each piece."""
+    rate = np.where(Z[gt.seg_species] == 2, r2, np.where(Z[gt.seg_species] == 1, r1, 1.0))
+    N = len(gt.parent)
+    eff = np.zeros(N)
+    np.add.at(eff, gt.seg_gene, gt.seg_len * rate * scale)
+    return eff
+
+
+@dataclasses.dataclass
+class GTWorkload:
+    """Elements that each carry their own gene tree (PhyloAcc-GT style)."""
+    name: str
+    tree: Tree                 # species tree, lengths in coalescent units
+    sp_height: np.ndarray      # [N] species node heights
+    pi: np.ndarray
+    Q: np.ndarray
+    mod_text: str
+    align: np.ndarray          # [S, G]
+    sizes: np.ndarray          # [C]
+    starts: np.ndarray         # [C]
+    genes: list                # [C] GeneTree
+    Z: np.ndarray              # [C, N] simulated species-branch classes
+    r1: np.ndarray             # [C]
+    r2: np.ndarray             # [C]
+    scale: float               # substitutions per coalescent unit
+    seed: int
+
+    outgroup = ()              # no outgroup collapse and no CONSERVE filter for gene trees
+    conserve = ()
+
+    @property
+    def C(self):
+        return len(self.sizes)
+
+    def children(self):
+        return np.stack([g.children for g in self.genes]).astype(np.int32)
+
+    def lengths(self, Z=None, r1=None, r2=None):
+        """[C, N] substitution lengths of every gene branch."""
+        Z = self.Z if Z is None else Z
+        r1 = self.r1 if r1 is None else r1
+        r2 = self.r2 if r2 is None else r2
+        return np.stack([gene_effective_lengths(g, Z[c], r1[c], r2[c], self.scale)
+                         for c, g in enumerate(self.genes)])
+
+
+def simulate_gene_trees(children, lengths, pi, Q, sizes, gap_frac, rng, S):
+    """Evolve every element down its own tree.  Ids are post-order (a parent's
+    id exceeds its children's), so visiting ids from the root down is a
+    valid top-down order for every element at once."""
+    lam, V, Vi = _eig(pi, Q)
+    C, N = lengths.shape
+    G = int(np.sum(sizes))
+    elem = np.repeat(np.arange(C), sizes)
+    elem16 = elem.astype(np.int64) * 16
+    parent = np.full((C, N), -1, np.int64)
+    for q in range(2):
+        np.put_along_axis(parent, children[:, S:, q].astype(np.int64), np.arange(S, N)[None, :].repeat(C, 0), 1)
+    states = np.empty((N, G), np.uint8)
+    cpi = np.cumsum(pi)
+    states[N - 1] = np.searchsorted(cpi, rng.random(G) * cpi[-1], side="right").clip(0, 3)
+    cols = np.arange(G)
+    for v in range(N - 2, -1, -1):
+        P = trans_batch(lam, V, Vi, lengths[:, v])
+        cum = np.cumsum(P, axis=2).reshape(-1)
+        par = states[parent[elem, v], cols]
+        base = elem16 + par.astype(np.int64) * 4
+        u = rng.random(G) * np.take(cum, base + 3)
+        st = (u >= np.take(cum, base)).astype(np.uint8)
+        st += u >= np.take(cum, base + 1)
+        st += u >= np.take(cum, base + 2)
+        states[v] = st
+    align = CODES[states[:S]]
+    if gap_frac > 0:
+        mask = rng.random(align.shape) < gap_frac
+        gapc = np.where(rng.random(align.shape) < 0.5, ord("-"), ord("n")).astype(np.uint8)
+        align = np.where(mask, gapc, align)
+    return np.ascontiguousarray(align)
+
+
+def make_gt_workload(seed: int = 5, n_elements: int = 20000, S: int = 120, mean_bl: float = 0.5,
+                     scale: float = 0.1, gap: float = 0.02) -> GTWorkload:
+    """Gene-tree workload: an ultrametric species tree in coalescent units whose
+    branch lengths average ``mean_bl``; every element draws its own gene tree
+    from the multispecies coalescent; lengths LogNormal(ln 300, 0.6) clipped to
+    [50, 3000]; 1-20% of species branches accelerated (r2 ~ U[1.5, 4]), the
+    rest conserved (r1 ~ U[0.1, 0.5]); ``scale`` substitutions per unit."""
+    rng = np.random.default_rng(seed)
+    tree, h = ultrametric_tree(S, 1.0, rng, mean_bl=mean_bl)
+    N = tree.N
+    pi, Q = gtr(rng)
+    C = int(n_elements)
+    sizes = np.clip(np.rint(rng.lognormal(np.log(300), 0.6, C)), 50, 3000).astype(np.int64)
+    genes = [msc_gene_tree(tree, h, rng) for _ in range(C)]
+    r1 = rng.uniform(0.1, 0.5, size=C)
+    r2 = rng.uniform(1.5, 4.0, size=C)
+    frac = rng.uniform(0.01, 0.20, size=C)
+    nacc = np.maximum(1, np.rint(frac * (N - 1))).astype(int)
+    rank = np.argsort(np.argsort(rng.random((C, N - 1)), axis=1), axis=1)
+    Z = np.ones((C, N), np.int32)
+    Z[:, : N - 1] = np.where(rank < nacc[:, None], 2, 1)
+    w = GTWorkload(f"gt{S}", tree, h, pi, Q, mod_text(pi, Q, tree), None, sizes,
+                   np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64), genes, Z, r1, r2, scale, seed)
+    w.align = simulate_gene_trees(w.children(), w.lengths(), pi, Q, sizes, gap, rng, S)
+    return w

@@ -1,0 +1,141 @@
+"""Gene-tree pruning (PhyloAcc-GT style): elements that carry their own tree.
+
+The oracle's gene-tree likelihood (one transition matrix per piece of a gene
+branch inside a species branch, composed in log space) is pinned against a
+brute-force sum over every ancestral assignment with scipy's expm, and
+against the species-tree likelihood when the gene tree equals the species
+tree.  The engine's gene lengths, encoding and (on the GPU) batched pruning
+are then checked against the oracle.
+"""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import oracle_py as O
+from phyloacc_amd import engine as E
+from phyloacc_amd import synth
+
+
+def _tiny(seed, S=4, GG=12, gap=0.1):
+    rng = np.random.default_rng(seed)
+    tree, h = synth.ultrametric_tree(S, 0.4, rng)
+    gt = synth.msc_gene_tree(tree, h, rng)
+    pi, Q = synth.gtr(rng)
+    align = rng.choice(np.frombuffer(b"acgt-nr", np.uint8), size=(S, GG),
+                       p=[(1 - gap) / 4] * 4 + [gap / 3] * 3)
+    return tree, h, gt, pi, Q, np.ascontiguousarray(align)
+
+
+LEAF = {ord("a"): [1, 0, 0, 0], ord("c"): [0, 1, 0, 0], ord("g"): [0, 0, 1, 0], ord("t"): [0, 0, 0, 1],
+        ord("r"): [1, 0, 1, 0], ord("-"): [1, 1, 1, 1], ord("n"): [1, 1, 1, 1]}
+
+
+def _brute(gt, pi, Q, align, Z, r1, r2, scale=1.0):
+    """Sum over all 4^I internal assignments; each gene branch's matrix is the
+    product of expm over its pieces (rate by the piece's species-branch Z)."""
+    S = align.shape[0]
+    N = 2 * S - 1
+    rate = {0: 1.0, 1: r1, 2: r2}
+    P = [np.eye(4) for _ in range(N)]
+    for g, s, L in zip(gt.seg_gene, gt.seg_species, gt.seg_len):
+        if L * scale < 1e-8:
+            continue
+        P[g] = P[g] @ scipy.linalg.expm(Q * L * scale * rate[int(Z[s])])
+    total = 0.0
+    for col in range(align.shape[1]):
+        leaf = [np.array(LEAF[int(align[s, col])], float) for s in range(S)]
+        acc = 0.0
+        for states in itertools.product(range(4), repeat=N - S):
+            st = dict(zip(range(S, N), states))
+            p = pi[st[N - 1]]
+            for v in range(N - 1):
+                a = st[gt.parent[v]]
+                if v < S:
+                    p *= P[v][a] @ leaf[v]
+                else:
+                    p *= P[v][a, st[v]]
+            acc += p
+        total += np.log(acc)
+    return total
+
+
+def _model(tree, pi, Q):
+    m, _ = O.parse_mod(synth.mod_text(pi, Q, tree))
+    return m
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_gene_tree_brute_force(seed):
+    tree, h, gt, pi, Q, align = _tiny(seed)
+    m = _model(tree, pi, Q)
+    rng = np.random.default_rng(seed + 10)
+    Z = rng.integers(0, 3, size=tree.N).astype(np.int32)
+    ref = _brute(gt, np.array(m.pi[:4]), np.array(m.Q[:16]).reshape(4, 4), align, Z, 0.3, 2.2)
+    ll = O.gt_loglik(m, tree.S, gt.children, gt.seg_gene, gt.seg_species, gt.seg_len, Z, 0.3, 2.2,
+                     align, 0, align.shape[1], gap_prop=1.0)
+    assert abs(ll - ref) <= 1e-11 * abs(ref), (ll, ref)
+
+
+def test_oracle_gene_tree_equal_to_species_tree():
+    # a gene tree identical to the species tree, one piece per branch, gives the
+    # species-tree likelihood (bpp_c2.cpp:392-425 form) at the same Z and rates
+    w = synth.small_workload(S=6, C=1, length=50, seed=9, gap=0.03)
+    m = O.model_from_workload(w)
+    t = w.tree
+    seg_gene = np.arange(t.N - 1, dtype=np.int32)
+    Z = np.random.default_rng(2).integers(0, 3, size=t.N).astype(np.int32)
+    Z[t.N - 1] = 1
+    gl = O.gt_loglik(m, t.S, t.children, seg_gene, seg_gene, t.dist[: t.N - 1], Z, 0.35, 1.9,
+                     w.align, 0, 50, gap_prop=1.0)
+    opts = O.default_options(min_len=1, gap_prop=1.0)
+    st = O.Element(m, opts, w.align, 0, 50).loglik(Z, 0.35, 1.9)
+    assert abs(gl - st) <= 1e-11 * abs(st), (gl, st)
+
+
+def test_gene_lengths_match_pieces():
+    w = synth.make_gt_workload(seed=3, n_elements=12, S=20)
+    for c in range(w.C):
+        g = w.genes[c]
+        out = E.gene_lengths(w.tree.N, g.seg_gene, g.seg_species, g.seg_len * w.scale, w.Z[c], w.r1[c], w.r2[c])
+        np.testing.assert_allclose(out, w.lengths()[c], rtol=1e-14, atol=1e-300)
+        # at rate 1 everywhere, each gene branch spans its two heights
+        ones = E.gene_lengths(w.tree.N, g.seg_gene, g.seg_species, g.seg_len, np.zeros(w.tree.N, np.int32), 1, 1)
+        span = np.array([g.height[g.parent[v]] - g.height[v] for v in range(w.tree.N - 1)])
+        leaf_span = np.array([g.height[g.parent[v]] - w.sp_height[v] for v in range(w.tree.S)])
+        span[: w.tree.S] = leaf_span
+        np.testing.assert_allclose(ones[:-1], span, rtol=1e-9, atol=1e-12)
+
+
+def test_add_trees_encoding():
+    w = synth.make_gt_workload(seed=4, n_elements=6, S=12)
+    eng = E.Engine.from_workload(w, E.default_options(min_len=1))
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    assert (st == E.PA_ELEM_OK).all()
+    for c, i in enumerate(ids):
+        el = eng.element(int(i))
+        np.testing.assert_array_equal(el["children"], w.genes[c].children)
+        assert el["root"] == w.tree.N - 1
+    bad = w.children().copy()
+    bad[0, w.tree.N - 1] = [w.tree.N - 1, 0]
+    with pytest.raises(E.PaError):
+        eng.add_elements_with_trees(w.align, w.starts, w.sizes, bad)
+
+
+@pytest.mark.gpu
+def test_gene_tree_loglik_gpu_vs_oracle(require_gpu):
+    w = synth.make_gt_workload(seed=5, n_elements=48, S=40)
+    opts = E.default_options(min_len=1)
+    eng = E.Engine.from_workload(w, opts)
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    assert (st == E.PA_ELEM_OK).all()
+    m = O.parse_mod(w.mod_text)[0]
+    L = w.lengths()
+    ref = np.array([O.gt_loglik(m, w.tree.S, g.children, g.seg_gene, g.seg_species, g.seg_len * w.scale, w.Z[c],
+                                w.r1[c], w.r2[c], w.align, w.starts[c], w.sizes[c], gap_prop=opts.gap_prop)
+                    for c, g in enumerate(w.genes)])
+    got64 = eng.loglik_lengths(ids, L, E.PA_FP64)
+    np.testing.assert_allclose(got64, ref, rtol=1e-9)
+    got32 = eng.loglik_lengths(ids, L, E.PA_FP32)
+    np.testing.assert_allclose(got32, ref, rtol=1e-6)
