@@ -200,6 +200,56 @@ pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, const double
                            const int8_t *Z, const int8_t *max_Z, int32_t precision, double *vb, double *medians,
                            double *postZ);
 
+/* ------------------------------------------------------------------ serving
+ * Continuous batching of element chains.  The reference has no serving mode:
+ * its OpenMP loop (src/PhyloAcc-ST/main.cpp:452-500) runs a fixed id list to
+ * completion.  Here requests (already encoded element ids) arrive at any time;
+ * each pa_serve_step retires the chains that finished, admits waiting
+ * requests while a chain record and scratch for the element's partials fit
+ * in the HBM budget, and launches one persistent kernel that advances every
+ * admitted chain by sweep_budget sweeps.  Chain state stays on the device
+ * between launches.  A chain's result does not depend on what else is in
+ * flight: its random streams are keyed by SEED and the element, and the
+ * launch boundaries fall on whole sweeps. */
+typedef struct pa_server pa_server;
+
+typedef struct {
+    int32_t precision;          /* PA_FP64 / PA_FP32 */
+    int32_t block_threads;      /* 64, 128 or 256; 0 = 128 */
+    int32_t sweep_budget;       /* sweeps per chain per launch; 0 = 100 */
+    int32_t max_chains;         /* chains in flight at most; 0 = 4096 */
+    double hbm_budget_bytes;    /* scratch for in-flight chains; 0 = 4 GiB */
+} pa_serve_config;
+
+typedef struct {
+    int64_t ticket;             /* from pa_serve_submit */
+    int32_t element;            /* element id */
+    int32_t status;             /* PA_ELEM_OK / PA_ELEM_NUMERIC_FAIL */
+    pa_chain_result result[3];  /* M0, M1, M2 */
+    double arrival_s, admit_s, finish_s;   /* pa_serve_clock() times */
+} pa_serve_result;
+
+typedef struct {
+    int64_t submitted, admitted, finished, launches;
+    int32_t waiting, in_flight;
+    double scratch_bytes_in_use, device_ms;
+} pa_serve_stats;
+
+pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **out);
+/* Queue n element ids (FIFO).  arrival_s: per-request arrival times on the
+ * pa_serve_clock() scale, or NULL for "now".  tickets: [n] or NULL. */
+pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n, const double *arrival_s, int64_t *tickets);
+/* One turn of the request loop.  With wait = 0 it returns at once while the
+ * current launch is still running (n_finished = 0). */
+pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finished);
+/* Moves up to max finished results out of the server, oldest first. */
+pa_status pa_serve_poll(pa_server *s, pa_serve_result *out, int32_t max, int32_t *n);
+pa_status pa_serve_get_stats(pa_server *s, pa_serve_stats *out);
+/* Waits for the launch in flight, then frees the server. */
+void pa_serve_destroy(pa_server *s);
+/* Monotonic host clock in seconds. */
+double pa_serve_clock(void);
+
 /* Number of visible HIP devices (0 when none); no reference counterpart. */
 int32_t pa_device_count(void);
 

@@ -134,19 +134,6 @@ size_t chain_smem_bytes(int N, int S, int BS, int fp64)
            (fp64 ? sizeof(Ctx<double>) : sizeof(Ctx<float>)) + sizeof(ModelDev);
 }
 
-// Per-workgroup HBM scratch layout (bytes), shared with the host.
-size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off)
-{
-    int I = N - S;
-    size_t rs = fp64 ? 8 : 4, o = 0;
-    off[0] = o; o = al(o + (size_t)2 * I * Pcap * 4 * rs, 256);   // parts
-    off[1] = o; o = al(o + (size_t)2 * I * Pcap * 4, 256);        // exps
-    off[2] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // plin
-    off[3] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // logt
-    off[4] = o; o = al(o + (size_t)S * 24 * 4, 256);              // amb
-    return o;
-}
-
 // ------------------------------------------------------------------ helpers
 
 template <typename R>
@@ -759,7 +746,7 @@ __device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
 // ---------------------------------------------------------- element setup
 
 template <typename R, int BS>
-__device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int eid)
+__device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int eid, int rec)
 {
     const int tid = threadIdx.x, N = M->N;
     const ElemDev E = A->elems[eid];
@@ -770,12 +757,14 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
         c->codes = (const GBL uint8_t *)(A->codes + E.off_codes);
         c->wts = (const GBL int32_t *)(A->wts + E.off_w);
         c->site = (const GBL int32_t *)(A->site + E.off_site);
-        char *base = A->scratch + (size_t)blockIdx.x * A->slot_bytes;
-        c->parts = (GBL V4<R> *)(base + A->off_parts);
-        c->exps = (GBL int32_t *)(base + A->off_exps);
-        c->plin = (GBL double *)(base + A->off_plin);
-        c->logt = (GBL double *)(base + A->off_logt);
-        c->amb = (GBL uint32_t *)(base + A->off_amb);
+        char *base = A->scratch + (A->slot_off ? A->slot_off[rec] : (int64_t)blockIdx.x * A->slot_bytes);
+        size_t off[5];
+        slot_layout(N, M->S, E.Ppad, sizeof(R) == 8, off);
+        c->parts = (GBL V4<R> *)(base + off[0]);
+        c->exps = (GBL int32_t *)(base + off[1]);
+        c->plin = (GBL double *)(base + off[2]);
+        c->logt = (GBL double *)(base + off[3]);
+        c->amb = (GBL uint32_t *)(base + off[4]);
         c->accept_n = 0;
         c->accept_c = 0;
         c->bytes = 0;
@@ -872,38 +861,89 @@ __device__ __noinline__ void monitor(MP M, CP<R> c, int m, int resZ, double *trd
 }
 
 template <typename R, int BS>
-__device__ __noinline__ void write_results(MP M, CP<R> c, const ChainArgs *A, int ms, int work)
+__device__ __noinline__ void write_results(MP M, CP<R> c, const ChainArgs *A, int ms, int rec)
 {
     const int tid = threadIdx.x, N = M->N;
     if (tid == 0) {
-        double *rd = A->res_d + ((size_t)work * 3 + ms) * 8;
-        int32_t *ri = A->res_i + ((size_t)work * 3 + ms) * 4;
+        double *rd = A->res_d + ((size_t)rec * 3 + ms) * 8;
+        int32_t *ri = A->res_i + ((size_t)rec * 3 + ms) * 4;
         rd[0] = c->max_loglik; rd[1] = c->nrate; rd[2] = c->crate; rd[3] = c->g; rd[4] = c->l; rd[5] = c->l2;
         ri[0] = c->max_m; ri[1] = c->status; ri[2] = c->accept_n; ri[3] = c->accept_c;
     }
-    int8_t *mz = A->maxZ + ((size_t)work * 3 + ms) * N;
+    int8_t *mz = A->maxZ + ((size_t)rec * 3 + ms) * N;
     for (int i = tid; i < N; i += BS) mz[i] = c->maxZ[i];
 }
 
+// Chain state across launches: everything a sweep hands to the next one
+// besides the partials and class tables, which stay in the scratch slot.
 template <typename R, int BS>
-__device__ __forceinline__ void run_model(MP M, CP<R> c, const ChainArgs *A, int ms, int work)
+__device__ __noinline__ void save_state(MP M, CP<R> c, GBL ChainState *st, int ms, int m)
+{
+    const int tid = threadIdx.x, N = M->N;
+    if (tid == 0) {
+        st->ms = ms; st->m = m;
+        st->accept_n = c->accept_n; st->accept_c = c->accept_c; st->max_m = c->max_m; st->status = c->status;
+        st->loglik = c->loglik; st->nrate = c->nrate; st->crate = c->crate;
+        st->g = c->g; st->l = c->l; st->l2 = c->l2;
+        st->prop_n = c->prop_n; st->prop_c = c->prop_c; st->max_loglik = c->max_loglik;
+        st->bytes = c->bytes;
+    }
+    for (int i = tid; i < N; i += BS) {
+        st->Z[i] = c->Z[i]; st->maxZ[i] = c->maxZ[i]; st->cur[i] = c->cur[i]; st->fixZ[i] = c->fixZ[i];
+    }
+}
+
+template <typename R, int BS>
+__device__ __noinline__ void load_state(MP M, CP<R> c, const GBL ChainState *st)
+{
+    const int tid = threadIdx.x, N = M->N;
+    if (tid == 0) {
+        c->accept_n = st->accept_n; c->accept_c = st->accept_c; c->max_m = st->max_m; c->status = st->status;
+        c->loglik = st->loglik; c->nrate = st->nrate; c->crate = st->crate;
+        c->g = st->g; c->l = st->l; c->l2 = st->l2;
+        c->prop_n = st->prop_n; c->prop_c = st->prop_c; c->max_loglik = st->max_loglik;
+        c->bytes = st->bytes;
+        set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
+    }
+    for (int i = tid; i < N; i += BS) {
+        c->Z[i] = st->Z[i]; c->maxZ[i] = st->maxZ[i]; c->cur[i] = st->cur[i]; c->fixZ[i] = st->fixZ[i];
+    }
+    __syncthreads();
+    refresh_PTc<R, BS>(c, -1);
+    __syncthreads();
+}
+
+// One model chain from iteration m0 (-1: from initMCMC).  Spends at most
+// *budget sweeps; returns false, with the state saved, when the budget ends
+// first.
+template <typename R, int BS>
+__device__ __forceinline__ bool run_model(MP M, CP<R> c, const ChainArgs *A, int ms, int rec, int m0, int *budget)
 {
     const int tid = threadIdx.x;
     const int resZ = ms == 0 ? 0 : (ms == 1 ? 2 : 1);
     const int T = M->T, N = M->N;
-    init_model<R, BS>(M, c, resZ);
-    {
+    GBL ChainState *st = A->state ? (GBL ChainState *)(A->state + rec) : nullptr;
+    if (m0 < 0) {
+        init_model<R, BS>(M, c, resZ);
         const double l0 = prune<R, BS>(M, c, PR_FULL, 0);
         if (tid == 0) c->loglik = l0;
         __syncthreads();
+        m0 = 0;
+    } else {
+        load_state<R, BS>(M, c, st);
     }
-    double *trd = A->tr_d ? A->tr_d + ((size_t)work * 3 + ms) * 7 * T : nullptr;
-    int8_t *trz = A->tr_Z ? A->tr_Z + ((size_t)work * 3 + ms) * (size_t)T * N : nullptr;
-    for (int m = 0; m < T; m++) {
+    double *trd = A->tr_d ? A->tr_d + ((size_t)rec * 3 + ms) * 7 * T : nullptr;
+    int8_t *trz = A->tr_Z ? A->tr_Z + ((size_t)rec * 3 + ms) * (size_t)T * N : nullptr;
+    for (int m = m0; m < T; m++) {
+        if (*budget == 0) {
+            save_state<R, BS>(M, c, st, ms, m);
+            return false;
+        }
         if (trz)
             for (int i = tid; i < N; i += BS) trz[(size_t)m * N + i] = c->Z[i];
         if (tid == 0) monitor<R>(M, c, m, resZ, trd, T);
         if (m == T - 1) break;
+        --*budget;
         sample_sites<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_ANC));
         __syncthreads();
         emissions<R, BS>(c);
@@ -921,8 +961,34 @@ __device__ __forceinline__ void run_model(MP M, CP<R> c, const ChainArgs *A, int
         sample_transition<R, BS>(M, c, ms, m);
     }
     __syncthreads();
-    write_results<R, BS>(M, c, A, ms, work);
+    write_results<R, BS>(M, c, A, ms, rec);
     __syncthreads();
+    return true;
+}
+
+// The three model chains of one element (main.cpp:466-494), resumed from
+// the record's state in resumable runs.
+template <typename R, int BS>
+__device__ __noinline__ void run_element(MP M, CP<R> c, const ChainArgs *A, int rec)
+{
+    int ms = 0, m0 = -1, budget = A->state ? A->sweep_budget : 0x7fffffff;
+    if (A->state) {
+        ms = A->state[rec].ms;
+        m0 = A->state[rec].m;
+    }
+    if (ms == 0 && m0 < 0 && threadIdx.x == 0) {
+        c->accept_n = 0;
+        c->accept_c = 0;
+        c->bytes = 0;
+    }
+    __syncthreads();
+    for (; ms < 3; ms++, m0 = -1)
+        if (!run_model<R, BS>(M, c, A, ms, rec, m0, &budget)) return;
+    if (A->state && threadIdx.x == 0) {
+        A->state[rec].ms = 3;
+        A->state[rec].bytes = c->bytes;
+        A->done[rec] = 1;
+    }
 }
 
 __device__ __forceinline__ MP model_to_lds(const ModelDev &Mk, LDS ModelDev *Ms)
@@ -950,9 +1016,12 @@ __global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs A)
         const int w = c->work;
         __syncthreads();
         if (w >= A.n_work) break;
-        bind_element<R, BS>(M, c, &A, A.work[w]);
-        for (int ms = 0; ms < 3; ms++) run_model<R, BS>(M, c, &A, ms, w);
-        if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes);
+        const int rec = A.rec ? A.rec[w] : w;
+        if (A.state && A.state[rec].ms >= 3) continue;
+        const unsigned long long b0 = A.state && A.state[rec].m >= 0 ? A.state[rec].bytes : 0ull;
+        bind_element<R, BS>(M, c, &A, A.work[w], rec);
+        run_element<R, BS>(M, c, &A, rec);
+        if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes - b0);
     }
 }
 
@@ -974,7 +1043,7 @@ __global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs A)
         const int w = c->work;
         __syncthreads();
         if (w >= A.n_work) break;
-        bind_element<R, BS>(M, c, &A, A.work[w]);
+        bind_element<R, BS>(M, c, &A, A.work[w], w);
         const double r1 = A.rates_in[2 * w], r2 = A.rates_in[2 * w + 1];
         for (int i = threadIdx.x; i < M->N; i += BS) c->Z[i] = (int8_t)A.Zin[(size_t)w * M->N + i];
         if (A.len_in) {

@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cctype>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -764,6 +768,7 @@ static pa_status upload(pa_engine *e)
     if (st != PA_OK) return st;
     HIPCHK(e, hipSetDevice(e->device));
     if (!e->dev_dirty) return PA_OK;
+    HIPCHK(e, hipStreamSynchronize(e->stream));   // no launch may still read the old arenas
     free_arenas(e);
     const int S = e->tm.S, N = e->tm.N;
     std::vector<ElemDev> d(e->elems.size());
@@ -837,7 +842,7 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     per_cu = std::max(1, std::min(per_cu, 2048 / BS));
     int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, e->n_cu * per_cu);
     size_t off[5];
-    const size_t slot = slot_scratch_bytes(N, S, Pcap, fp64, off);
+    const size_t slot = slot_layout(N, S, Pcap, fp64, off);
     HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
     HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + 64));
     unsigned long long *d_cnt = (unsigned long long *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
@@ -847,7 +852,6 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
     A.queue = e->d_queue;
     A.scratch = (char *)bufs[0].p; A.slot_bytes = (int64_t)slot;
-    A.off_parts = off[0]; A.off_exps = off[1]; A.off_plin = off[2]; A.off_logt = off[3]; A.off_amb = off[4];
     hipEvent_t t0, t1;
     HIPCHK(e, hipEventCreate(&t0));
     HIPCHK(e, hipEventCreate(&t1));
@@ -1226,4 +1230,320 @@ extern "C" pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, c
     }
     *vb = VB;
     return PA_OK;
+}
+
+// ------------------------------------------------------------ serving loop
+
+namespace {
+
+// First-fit allocator over one device scratch pool, with coalescing.
+struct Pool {
+    std::map<size_t, size_t> free_;   // offset -> length
+    size_t used = 0;
+    void init(size_t n) { free_.clear(); free_[0] = n; used = 0; }
+    bool take(size_t n, size_t *off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                *off = it->first;
+                const size_t rest = it->second - n, at = it->first + n;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                used += n;
+                return true;
+            }
+        return false;
+    }
+    void give(size_t off, size_t n)
+    {
+        used -= n;
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.end() && off + n == nx->first) { n += nx->second; nx = free_.erase(nx); }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) { pv->second += n; return; }
+        }
+        free_[off] = n;
+    }
+};
+
+struct Request {
+    int64_t ticket;
+    int32_t eid;
+    double arrival, admit;
+    size_t off, bytes;
+};
+
+}  // namespace
+
+struct pa_server {
+    pa_engine *e = nullptr;
+    int fp64 = 0, BS = 128, budget = 100, max_chains = 4096;
+    size_t pool_bytes = 0;
+    char *d_pool = nullptr;
+    Pool pool;
+    ChainState *d_state = nullptr;
+    int32_t *d_done = nullptr, *d_list = nullptr;   // done flags [max]; work ids + records [2*max]
+    int64_t *d_slot = nullptr;
+    double *d_res_d = nullptr;
+    int32_t *d_res_i = nullptr;
+    int8_t *d_maxZ = nullptr;
+    int32_t *h_list = nullptr, *h_done = nullptr;  // pinned staging
+    int64_t *h_slot = nullptr;
+    std::vector<int32_t> free_recs;
+    std::vector<Request> rec_req;                 // by record
+    std::vector<int32_t> active;                  // records in flight
+    std::deque<Request> waiting;
+    std::deque<pa_serve_result> finished;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool in_flight = false;
+    int64_t next_ticket = 0;
+    pa_serve_stats st{};
+};
+
+extern "C" double pa_serve_clock(void)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void serve_free(pa_server *s)
+{
+    (void)hipFree(s->d_pool); (void)hipFree(s->d_state); (void)hipFree(s->d_done); (void)hipFree(s->d_list);
+    (void)hipFree(s->d_slot); (void)hipFree(s->d_res_d); (void)hipFree(s->d_res_i); (void)hipFree(s->d_maxZ);
+    (void)hipHostFree(s->h_list); (void)hipHostFree(s->h_done); (void)hipHostFree(s->h_slot);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+}
+
+extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **out)
+{
+    if (!e || !cfg || !out) return fail(e, PA_E_INPUT, "null argument");
+    *out = nullptr;
+    pa_status st = ensure_device(e);
+    if (st != PA_OK) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    std::unique_ptr<pa_server> s(new pa_server());
+    s->e = e;
+    s->fp64 = cfg->precision == PA_FP64;
+    s->BS = cfg->block_threads ? cfg->block_threads : 128;
+    if (s->BS != 64 && s->BS != 128 && s->BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
+    if (chain_smem_bytes(e->tm.N, e->tm.S, s->BS, s->fp64) > e->lds_per_cu)
+        return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+    s->budget = cfg->sweep_budget > 0 ? cfg->sweep_budget : 100;
+    s->max_chains = cfg->max_chains > 0 ? cfg->max_chains : 4096;
+    s->pool_bytes = cfg->hbm_budget_bytes > 0 ? (size_t)cfg->hbm_budget_bytes : ((size_t)4 << 30);
+    const int M = s->max_chains, N = e->tm.N;
+    struct Alloc { pa_server *s; ~Alloc() { if (s) serve_free(s); } } guard{s.get()};
+    HIPCHK(e, hipMalloc(&s->d_pool, s->pool_bytes));
+    HIPCHK(e, hipMalloc(&s->d_state, sizeof(ChainState) * M));
+    HIPCHK(e, hipMalloc(&s->d_done, sizeof(int32_t) * M));
+    HIPCHK(e, hipMalloc(&s->d_list, sizeof(int32_t) * 2 * M));
+    HIPCHK(e, hipMalloc(&s->d_slot, sizeof(int64_t) * M));
+    HIPCHK(e, hipMalloc(&s->d_res_d, sizeof(double) * 24 * M));
+    HIPCHK(e, hipMalloc(&s->d_res_i, sizeof(int32_t) * 12 * M));
+    HIPCHK(e, hipMalloc(&s->d_maxZ, (size_t)3 * N * M));
+    HIPCHK(e, hipHostMalloc(&s->h_list, sizeof(int32_t) * 2 * M));
+    HIPCHK(e, hipHostMalloc(&s->h_done, sizeof(int32_t) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_slot, sizeof(int64_t) * M));
+    HIPCHK(e, hipEventCreate(&s->ev0));
+    HIPCHK(e, hipEventCreate(&s->ev1));
+    s->pool.init(s->pool_bytes);
+    s->rec_req.resize(M);
+    for (int r = M - 1; r >= 0; r--) s->free_recs.push_back(r);
+    guard.s = nullptr;
+    *out = s.release();
+    return PA_OK;
+}
+
+extern "C" pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n, const double *arrival_s, int64_t *tickets)
+{
+    if (!s || (!ids && n > 0) || n < 0) return PA_E_INPUT;
+    pa_engine *e = s->e;
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    const double now = pa_serve_clock();
+    size_t off[5];
+    for (int i = 0; i < n; i++) {
+        Request q{};
+        q.ticket = s->next_ticket++;
+        q.eid = ids[i];
+        q.arrival = arrival_s ? arrival_s[i] : now;
+        q.bytes = slot_layout(e->tm.N, e->tm.S, e->elems[ids[i]].Ppad, s->fp64, off);
+        if (q.bytes > s->pool_bytes) return fail(e, PA_E_INPUT, "element " + std::to_string(ids[i]) + " exceeds the HBM budget");
+        if (tickets) tickets[i] = q.ticket;
+        s->waiting.push_back(q);
+    }
+    s->st.submitted += n;
+    return PA_OK;
+}
+
+// Retires the chains of the finished launch.
+static pa_status serve_retire(pa_server *s, int32_t *n_finished)
+{
+    pa_engine *e = s->e;
+    const int N = e->tm.N;
+    HIPCHK(e, hipEventSynchronize(s->ev1));
+    float ms = 0;
+    HIPCHK(e, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->st.device_ms += ms;
+    s->in_flight = false;
+    const double now = pa_serve_clock();
+    std::vector<int32_t> keep;
+    std::vector<double> rd(24);
+    std::vector<int32_t> ri(12);
+    for (int r : s->active) {
+        if (!s->h_done[r]) { keep.push_back(r); continue; }
+        HIPCHK(e, hipMemcpy(rd.data(), s->d_res_d + (size_t)r * 24, 24 * 8, hipMemcpyDeviceToHost));
+        HIPCHK(e, hipMemcpy(ri.data(), s->d_res_i + (size_t)r * 12, 12 * 4, hipMemcpyDeviceToHost));
+        const Request &q = s->rec_req[r];
+        pa_serve_result out{};
+        out.ticket = q.ticket;
+        out.element = q.eid;
+        out.status = PA_ELEM_OK;
+        for (int k = 0; k < 3; k++) {
+            const double *d = &rd[k * 8];
+            const int32_t *x = &ri[k * 4];
+            pa_chain_result &c = out.result[k];
+            c.max_loglik = d[0]; c.final_nrate = d[1]; c.final_crate = d[2];
+            c.final_grate = d[3]; c.final_lrate = d[4]; c.final_lrate2 = d[5];
+            c.max_m = x[0]; c.status = x[1]; c.accept_n = x[2]; c.accept_c = x[3];
+            if (c.status) out.status = PA_ELEM_NUMERIC_FAIL;
+        }
+        out.arrival_s = q.arrival;
+        out.admit_s = q.admit;
+        out.finish_s = now;
+        s->finished.push_back(out);
+        s->pool.give(q.off, q.bytes);
+        s->free_recs.push_back(r);
+        s->st.finished++;
+        (*n_finished)++;
+    }
+    (void)N;
+    s->active.swap(keep);
+    return PA_OK;
+}
+
+// Admits waiting requests (FIFO) and launches the next sweep batch.
+static pa_status serve_launch(pa_server *s)
+{
+    pa_engine *e = s->e;
+    const double now = pa_serve_clock();
+    std::vector<int32_t> fresh;
+    while (!s->waiting.empty() && !s->free_recs.empty()) {
+        Request q = s->waiting.front();
+        size_t off;
+        if (!s->pool.take(q.bytes, &off)) break;   // wait for scratch to free up
+        s->waiting.pop_front();
+        const int r = s->free_recs.back();
+        s->free_recs.pop_back();
+        q.off = off;
+        q.admit = now;
+        s->rec_req[r] = q;
+        s->active.push_back(r);
+        fresh.push_back(r);
+        s->st.admitted++;
+    }
+    if (s->active.empty()) return PA_OK;
+    pa_status st = upload(e);
+    if (st != PA_OK) return st;
+    // new chains start from initMCMC: header {ms = 0, m = -1}, done = 0
+    static const int32_t hdr[2] = {0, -1};
+    for (int r : fresh) {
+        HIPCHK(e, hipMemcpyAsync(&s->d_state[r], hdr, sizeof hdr, hipMemcpyHostToDevice, e->stream));
+        s->h_slot[r] = (int64_t)s->rec_req[r].off;
+        HIPCHK(e, hipMemcpyAsync(s->d_slot + r, s->h_slot + r, 8, hipMemcpyHostToDevice, e->stream));
+    }
+    if (!fresh.empty()) HIPCHK(e, hipMemsetAsync(s->d_done, 0, sizeof(int32_t) * s->max_chains, e->stream));
+    // largest elements first
+    std::sort(s->active.begin(), s->active.end(), [&](int a, int b) {
+        const HostElem &x = e->elems[s->rec_req[a].eid], &y = e->elems[s->rec_req[b].eid];
+        if (x.P != y.P) return x.P > y.P;
+        return a < b;
+    });
+    const int n = (int)s->active.size();
+    for (int i = 0; i < n; i++) {
+        s->h_list[i] = s->rec_req[s->active[i]].eid;
+        s->h_list[s->max_chains + i] = s->active[i];
+    }
+    HIPCHK(e, hipMemcpyAsync(s->d_list, s->h_list, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(s->d_list + s->max_chains, s->h_list + s->max_chains, sizeof(int32_t) * n,
+                             hipMemcpyHostToDevice, e->stream));
+    ChainArgs A;
+    memset(&A, 0, sizeof A);
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+    A.queue = e->d_queue;
+    A.work = s->d_list;
+    A.rec = s->d_list + s->max_chains;
+    A.n_work = n;
+    A.scratch = s->d_pool;
+    A.slot_off = s->d_slot;
+    A.state = s->d_state;
+    A.done = s->d_done;
+    A.sweep_budget = s->budget;
+    A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
+    const size_t smem = chain_smem_bytes(e->tm.N, e->tm.S, s->BS, s->fp64);
+    int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
+    per_cu = std::max(1, std::min(per_cu, 2048 / s->BS));
+    const int grid = std::min(n, e->n_cu * per_cu);
+    HIPCHK(e, hipEventRecord(s->ev0, e->stream));
+    HIPCHK(e, launch_reset(e->d_queue, e->stream));
+    HIPCHK(e, launch_kernel(e->M, A, s->fp64, s->BS, grid, 0, e->stream));
+    HIPCHK(e, hipMemcpyAsync(s->h_done, s->d_done, sizeof(int32_t) * s->max_chains, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipEventRecord(s->ev1, e->stream));
+    s->in_flight = true;
+    s->st.launches++;
+    return PA_OK;
+}
+
+extern "C" pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finished)
+{
+    if (!s) return PA_E_INPUT;
+    pa_engine *e = s->e;
+    int32_t nf = 0;
+    if (n_finished) *n_finished = 0;
+    HIPCHK(e, hipSetDevice(e->device));
+    if (s->in_flight) {
+        if (!wait) {
+            const hipError_t q = hipEventQuery(s->ev1);
+            if (q == hipErrorNotReady) return PA_OK;
+            if (q != hipSuccess) return fail(e, PA_E_DEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
+        }
+        pa_status st = serve_retire(s, &nf);
+        if (st != PA_OK) return st;
+    }
+    pa_status st = serve_launch(s);
+    if (n_finished) *n_finished = nf;
+    return st;
+}
+
+extern "C" pa_status pa_serve_poll(pa_server *s, pa_serve_result *out, int32_t max, int32_t *n)
+{
+    if (!s || !n || (max > 0 && !out)) return PA_E_INPUT;
+    int k = 0;
+    while (k < max && !s->finished.empty()) {
+        out[k++] = s->finished.front();
+        s->finished.pop_front();
+    }
+    *n = k;
+    return PA_OK;
+}
+
+extern "C" pa_status pa_serve_get_stats(pa_server *s, pa_serve_stats *out)
+{
+    if (!s || !out) return PA_E_INPUT;
+    *out = s->st;
+    out->waiting = (int32_t)s->waiting.size();
+    out->in_flight = (int32_t)s->active.size();
+    out->scratch_bytes_in_use = (double)s->pool.used;
+    return PA_OK;
+}
+
+extern "C" void pa_serve_destroy(pa_server *s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->e->device);
+    if (s->in_flight) (void)hipEventSynchronize(s->ev1);
+    (void)hipStreamSynchronize(s->e->stream);
+    serve_free(s);
+    delete s;
 }

@@ -27,7 +27,9 @@ PA_ELEM_OK, PA_ELEM_FILTERED, PA_ELEM_NUMERIC_FAIL = 0, 1, 2
 
 EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_destroy", "pa_last_error",
            "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count",
-           "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths"]
+           "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths",
+           "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
+           "pa_serve_destroy", "pa_serve_clock"]
 
 
 class PaError(RuntimeError):
@@ -90,6 +92,23 @@ class RunConfig(C.Structure):
                 ("keep_traces", C.c_int32)]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("block_threads", C.c_int32), ("sweep_budget", C.c_int32),
+                ("max_chains", C.c_int32), ("hbm_budget_bytes", C.c_double)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("ticket", C.c_int64), ("element", C.c_int32), ("status", C.c_int32),
+                ("result", ChainResult * 3), ("arrival_s", C.c_double), ("admit_s", C.c_double),
+                ("finish_s", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("finished", C.c_int64),
+                ("launches", C.c_int64), ("waiting", C.c_int32), ("in_flight", C.c_int32),
+                ("scratch_bytes_in_use", C.c_double), ("device_ms", C.c_double)]
+
+
 _lib = None
 
 
@@ -132,6 +151,20 @@ def lib():
         L.pa_loglik_lengths.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), C.c_int32,
                                         P(C.c_double)]
         L.pa_loglik_lengths.restype = C.c_int
+        L.pa_serve_create.argtypes = [C.c_void_p, P(ServeConfig), P(C.c_void_p)]
+        L.pa_serve_create.restype = C.c_int
+        L.pa_serve_submit.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), P(C.c_int64)]
+        L.pa_serve_submit.restype = C.c_int
+        L.pa_serve_step.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32)]
+        L.pa_serve_step.restype = C.c_int
+        L.pa_serve_poll.argtypes = [C.c_void_p, P(ServeResult), C.c_int32, P(C.c_int32)]
+        L.pa_serve_poll.restype = C.c_int
+        L.pa_serve_get_stats.argtypes = [C.c_void_p, P(ServeStats)]
+        L.pa_serve_get_stats.restype = C.c_int
+        L.pa_serve_destroy.argtypes = [C.c_void_p]
+        L.pa_serve_destroy.restype = None
+        L.pa_serve_clock.argtypes = []
+        L.pa_serve_clock.restype = C.c_double
         L.pa_device_count.argtypes = []
         L.pa_device_count.restype = C.c_int32
         for f in ("pa_parse_mod", "pa_engine_create", "pa_elements_add", "pa_element_get", "pa_loglik",
@@ -363,3 +396,56 @@ def device_count() -> int:
 
 def rng_uniform(seed, c0, c1, c2, c3, lane):
     return lib().pa_rng_uniform(seed, c0, c1, c2, c3, lane)
+
+
+class Server:
+    """Continuous-batching request loop over an engine's encoded elements."""
+
+    def __init__(self, engine: "Engine", precision=PA_FP32, block_threads=0, sweep_budget=0, max_chains=0,
+                 hbm_budget_bytes=0.0):
+        self.engine = engine
+        cfg = ServeConfig(int(precision), int(block_threads), int(sweep_budget), int(max_chains),
+                          float(hbm_budget_bytes))
+        h = C.c_void_p()
+        engine._check(lib().pa_serve_create(engine._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def _check(self, st):
+        if st != PA_OK:
+            raise PaError(st, lib().pa_last_error(self.engine._h).decode())
+
+    def submit(self, ids, arrival_s=None):
+        ids = np.ascontiguousarray(ids, np.int32)
+        t = np.zeros(len(ids), np.int64)
+        arr = None if arrival_s is None else np.ascontiguousarray(arrival_s, np.float64)
+        self._check(lib().pa_serve_submit(self._h, _p(ids, C.c_int32), len(ids),
+                                          None if arr is None else _p(arr, C.c_double), _p(t, C.c_int64)))
+        return t
+
+    def step(self, wait=True) -> int:
+        n = C.c_int32()
+        self._check(lib().pa_serve_step(self._h, int(bool(wait)), C.byref(n)))
+        return n.value
+
+    def poll(self, max_results=4096):
+        buf = (ServeResult * max_results)()
+        n = C.c_int32()
+        self._check(lib().pa_serve_poll(self._h, buf, max_results, C.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def stats(self) -> ServeStats:
+        s = ServeStats()
+        self._check(lib().pa_serve_get_stats(self._h, C.byref(s)))
+        return s
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pa_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def serve_clock() -> float:
+    return lib().pa_serve_clock()

@@ -1,0 +1,101 @@
+"""Continuous-batching request loop: chains resumed across launches give the
+same results as one uninterrupted launch, and a chain's result is the same
+alone and in a mix of other requests arriving while it runs."""
+import numpy as np
+import pytest
+
+from phyloacc_amd import engine as E
+from phyloacc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("max_loglik", "max_m", "status", "final_nrate", "final_crate", "final_grate", "final_lrate",
+          "final_lrate2", "accept_n", "accept_c")
+
+
+def _setup(n, seed=3, config=2, **kw):
+    opts = dict(burnin=30, mcmc=90, adapt_freq=25)
+    opts.update(kw)
+    w = synth.make_workload(config, seed=seed, n_elements=n)
+    eng = E.Engine.from_workload(w, E.default_options(**opts))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    return eng, ids[st == E.PA_ELEM_OK]
+
+
+def _drain(srv, n_expected, max_steps=100000):
+    out = {}
+    steps = 0
+    while len(out) < n_expected:
+        srv.step(wait=True)
+        for r in srv.poll():
+            out[r.ticket] = r
+        steps += 1
+        assert steps < max_steps
+    return out
+
+
+def _as_tuple(r):
+    return tuple(getattr(r.result[k], f) for k in range(3) for f in FIELDS)
+
+
+@pytest.mark.parametrize("precision", [E.PA_FP64, E.PA_FP32])
+def test_resumed_chains_equal_one_launch(require_gpu, precision):
+    eng, ids = _setup(16)
+    ref = eng.run_chains(ids, precision=precision, block_threads=128)
+    srv = E.Server(eng, precision=precision, block_threads=128, sweep_budget=7)
+    tickets = srv.submit(ids)
+    got = _drain(srv, len(ids))
+    for k, t in enumerate(tickets):
+        r = got[t]
+        assert r.element == ids[k]
+        for ms in range(3):
+            for f in FIELDS:
+                assert getattr(r.result[ms], f) == ref[f][k, ms], (k, ms, f)
+    srv.close()
+
+
+def test_same_result_alone_and_in_a_mix(require_gpu):
+    eng, ids = _setup(24, seed=8)
+    x = int(ids[5])
+    alone = E.Server(eng, precision=E.PA_FP32, sweep_budget=11)
+    t = alone.submit([x])[0]
+    ra = _drain(alone, 1)[t]
+    alone.close()
+    mix = E.Server(eng, precision=E.PA_FP32, sweep_budget=11)
+    others = [int(i) for i in ids if i != x]
+    tick = list(mix.submit(others[:9]))
+    got = {}
+    for _ in range(3):                       # let the first requests get going
+        mix.step(wait=True)
+        got.update({r.ticket: r for r in mix.poll()})
+    tx = mix.submit([x])[0]
+    tick += [tx]
+    mix.step(wait=False)
+    tick += list(mix.submit(others[9:]))
+    got.update(_drain(mix, len(tick) - len(got)))
+    got.update({r.ticket: r for r in mix.poll()})
+    assert _as_tuple(got[tx]) == _as_tuple(ra)
+    st = mix.stats()
+    assert st.finished == len(tick) and st.in_flight == 0 and st.waiting == 0
+    mix.close()
+
+
+def test_hbm_budget_limits_chains_in_flight(require_gpu):
+    eng, ids = _setup(12, seed=4)
+    ref = eng.run_chains(ids, precision=E.PA_FP32, block_threads=128)
+    # room for about three of the largest elements' scratch at a time
+    biggest = max(eng.element(int(i))["P"] for i in ids)
+    ppad = (biggest + 63) // 64 * 64
+    per = 2 * (eng.N - eng.S) * ppad * 4 * 4 + 2 * (eng.N - eng.S) * ppad * 4 + 2 * 3 * eng.N * 16 * 8 + eng.S * 96
+    srv = E.Server(eng, precision=E.PA_FP32, block_threads=128, sweep_budget=25, hbm_budget_bytes=3.3 * per)
+    tickets = srv.submit(ids)
+    got = {}
+    peak = 0
+    while len(got) < len(ids):
+        srv.step(wait=True)
+        peak = max(peak, srv.stats().in_flight)
+        got.update({r.ticket: r for r in srv.poll()})
+    assert 1 <= peak <= 4
+    for k, t in enumerate(tickets):
+        assert got[t].result[1].max_loglik == ref["max_loglik"][k, 1]
+    srv.close()
