@@ -80,6 +80,7 @@ struct Ctx {
 template <typename R>
 using CP = LDS Ctx<R> *;
 using MP = LDS const ModelDev *;
+using AP = LDS const ChainArgs *;
 
 __host__ __device__ inline size_t al(size_t x, size_t a) { return (x + a - 1) & ~(a - 1); }
 
@@ -746,7 +747,7 @@ __device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
 // ---------------------------------------------------------- element setup
 
 template <typename R, int BS>
-__device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int eid, int rec)
+__device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int eid, int rec)
 {
     const int tid = threadIdx.x, N = M->N;
     const ElemDev E = A->elems[eid];
@@ -861,7 +862,7 @@ __device__ __noinline__ void monitor(MP M, CP<R> c, int m, int resZ, double *trd
 }
 
 template <typename R, int BS>
-__device__ __noinline__ void write_results(MP M, CP<R> c, const ChainArgs *A, int ms, int rec)
+__device__ __noinline__ void write_results(MP M, CP<R> c, AP A, int ms, int rec)
 {
     const int tid = threadIdx.x, N = M->N;
     if (tid == 0) {
@@ -917,7 +918,7 @@ __device__ __noinline__ void load_state(MP M, CP<R> c, const GBL ChainState *st)
 // *budget sweeps; returns false, with the state saved, when the budget ends
 // first.
 template <typename R, int BS>
-__device__ __forceinline__ bool run_model(MP M, CP<R> c, const ChainArgs *A, int ms, int rec, int m0, int *budget)
+__device__ __forceinline__ bool run_model(MP M, CP<R> c, AP A, int ms, int rec, int m0, int *budget)
 {
     const int tid = threadIdx.x;
     const int resZ = ms == 0 ? 0 : (ms == 1 ? 2 : 1);
@@ -969,7 +970,7 @@ __device__ __forceinline__ bool run_model(MP M, CP<R> c, const ChainArgs *A, int
 // The three model chains of one element (main.cpp:466-494), resumed from
 // the record's state in resumable runs.
 template <typename R, int BS>
-__device__ __noinline__ void run_element(MP M, CP<R> c, const ChainArgs *A, int rec)
+__device__ __noinline__ void run_element(MP M, CP<R> c, AP A, int rec)
 {
     int ms = 0, m0 = -1, budget = A->state ? A->sweep_budget : 0x7fffffff;
     if (A->state) {
@@ -991,64 +992,73 @@ __device__ __noinline__ void run_element(MP M, CP<R> c, const ChainArgs *A, int 
     }
 }
 
-__device__ __forceinline__ MP model_to_lds(const ModelDev &Mk, LDS ModelDev *Ms)
+// Kernel arguments are copied to LDS once per workgroup: the device
+// functions take LDS pointers to them, which keeps the argument blocks out of
+// the private (scratch) segment.
+template <typename T>
+__device__ __forceinline__ LDS const T *to_lds(const T &k, LDS T *dst)
 {
-    const uint32_t *src = (const uint32_t *)&Mk;
-    LDS uint32_t *dst = (LDS uint32_t *)Ms;
-    for (int i = threadIdx.x; i < (int)(sizeof(ModelDev) / 4); i += blockDim.x) dst[i] = src[i];
-    return (MP)Ms;
+    static_assert(sizeof(T) % 4 == 0, "word copy");
+    const uint32_t *src = (const uint32_t *)&k;
+    LDS uint32_t *d = (LDS uint32_t *)dst;
+    for (int i = threadIdx.x; i < (int)(sizeof(T) / 4); i += blockDim.x) d[i] = src[i];
+    return (LDS const T *)dst;
 }
 
 template <typename R, int BS>
-__global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs A)
+__global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs Ak)
 {
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ ModelDev Ms;
+    __shared__ ChainArgs As;
     __shared__ Ctx<R> Cs;
-    MP M = model_to_lds(Mk, (LDS ModelDev *)&Ms);
+    MP M = to_lds(Mk, (LDS ModelDev *)&Ms);
+    AP A = to_lds(Ak, (LDS ChainArgs *)&As);
     CP<R> c = (CP<R>)&Cs;
     __syncthreads();
     if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
     __syncthreads();
     for (;;) {
-        if (threadIdx.x == 0) c->work = atomicAdd(A.queue, 1);
+        if (threadIdx.x == 0) c->work = atomicAdd(A->queue, 1);
         __syncthreads();
         const int w = c->work;
         __syncthreads();
-        if (w >= A.n_work) break;
-        const int rec = A.rec ? A.rec[w] : w;
-        if (A.state && A.state[rec].ms >= 3) continue;
-        const unsigned long long b0 = A.state && A.state[rec].m >= 0 ? A.state[rec].bytes : 0ull;
-        bind_element<R, BS>(M, c, &A, A.work[w], rec);
-        run_element<R, BS>(M, c, &A, rec);
-        if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes - b0);
+        if (w >= A->n_work) break;
+        const int rec = A->rec ? A->rec[w] : w;
+        if (A->state && A->state[rec].ms >= 3) continue;
+        const unsigned long long b0 = A->state && A->state[rec].m >= 0 ? A->state[rec].bytes : 0ull;
+        bind_element<R, BS>(M, c, A, A->work[w], rec);
+        run_element<R, BS>(M, c, A, rec);
+        if (threadIdx.x == 0 && A->counters) atomicAdd(A->counters, c->bytes - b0);
     }
 }
 
 // Log-likelihood at fixed Z and rates (BPP_C::log_f_Xz, bpp_c2.cpp:392-425)
 template <typename R, int BS>
-__global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs A)
+__global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs Ak)
 {
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ ModelDev Ms;
+    __shared__ ChainArgs As;
     __shared__ Ctx<R> Cs;
-    MP M = model_to_lds(Mk, (LDS ModelDev *)&Ms);
+    MP M = to_lds(Mk, (LDS ModelDev *)&Ms);
+    AP A = to_lds(Ak, (LDS ChainArgs *)&As);
     CP<R> c = (CP<R>)&Cs;
     __syncthreads();
     if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
     __syncthreads();
     for (;;) {
-        if (threadIdx.x == 0) c->work = atomicAdd(A.queue, 1);
+        if (threadIdx.x == 0) c->work = atomicAdd(A->queue, 1);
         __syncthreads();
         const int w = c->work;
         __syncthreads();
-        if (w >= A.n_work) break;
-        bind_element<R, BS>(M, c, &A, A.work[w], w);
-        const double r1 = A.rates_in[2 * w], r2 = A.rates_in[2 * w + 1];
-        for (int i = threadIdx.x; i < M->N; i += BS) c->Z[i] = (int8_t)A.Zin[(size_t)w * M->N + i];
-        if (A.len_in) {
+        if (w >= A->n_work) break;
+        bind_element<R, BS>(M, c, A, A->work[w], w);
+        const double r1 = A->rates_in[2 * w], r2 = A->rates_in[2 * w + 1];
+        for (int i = threadIdx.x; i < M->N; i += BS) c->Z[i] = (int8_t)A->Zin[(size_t)w * M->N + i];
+        if (A->len_in) {
             // own-tree element: per-call lengths, every branch in the rate-1 class
-            for (int i = threadIdx.x; i < M->N; i += BS) c->dist2[i] = A.len_in[(size_t)w * M->N + i];
+            for (int i = threadIdx.x; i < M->N; i += BS) c->dist2[i] = A->len_in[(size_t)w * M->N + i];
             __syncthreads();
             for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, 0, c->nodes[q], 1.0, 0);
         }
@@ -1061,7 +1071,7 @@ __global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs A)
         refresh_PTc<R, BS>(c, -1);
         __syncthreads();
         const double l = prune<R, BS>(M, c, PR_FULL, 0);
-        if (threadIdx.x == 0) A.ll_out[w] = l;
+        if (threadIdx.x == 0) A->ll_out[w] = l;
         __syncthreads();
     }
 }
