@@ -22,6 +22,7 @@
 // allocation spanning the whole persistent loop.
 // No MFMA: a K=4 matvec per pattern is not a dense contraction.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 #include "pa_common.h"
 #include "pa_rng.h"
@@ -62,7 +63,7 @@ struct Ctx {
     GBL double *logt;      // [3][N][16]  log P_{j->i} at [i*4+j]
     GBL uint32_t *amb;     // [S][24]
     // LDS arrays
-    LDS int16_t *ch0, *ch1, *par, *nodes, *inodes, *up, *upoff, *down, *downoff, *tz;
+    LDS int16_t *ch0, *ch1, *par, *nodes, *inodes, *up, *upoff, *down, *downoff, *tz, *lvl, *kbf;
     LDS uint8_t *flags, *fixZ, *dirty, *cur, *changed;
     LDS int8_t *Z, *maxZ;
     LDS double *dist2, *E, *beta;
@@ -100,11 +101,29 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
         o += sizeof(T) * (size_t)(count);                                          \
     } while (0)
     PA_TAKE(dist2, double, N);
-    PA_TAKE(E, double, 3 * N);
-    PA_TAKE(beta, double, 3 * N);
     PA_TAKE(PTc, R, 16 * N);
-    PA_TAKE(PTp, R, 16 * N);
-    PA_TAKE(hist, uint32_t, 16 * N);
+    // Phases that never overlap share storage:
+    //   A: per-thread ancestral states (sampling) | emissions + Z messages;
+    //   B: per-branch transition counters (sampling, emissions) | proposal
+    //      matrices (rate updates).  The counters are cleared when a sweep's
+    //      sampling starts.
+    {
+        const size_t szA = std::max((size_t)(I > 0 ? I : 1) * BS, (size_t)6 * N * sizeof(double));
+        o = al(o, 32);
+        if constexpr (BIND) {
+            c->tgs = (LDS uint8_t *)(base + o);
+            c->E = (LDS double *)(base + o);
+            c->beta = (LDS double *)(base + o) + 3 * N;
+        }
+        o += szA;
+        const size_t szB = std::max((size_t)16 * N * sizeof(uint32_t), (size_t)16 * N * sizeof(R));
+        o = al(o, 32);
+        if constexpr (BIND) {
+            c->hist = (LDS uint32_t *)(base + o);
+            c->PTp = (LDS R *)(base + o);
+        }
+        o += szB;
+    }
     PA_TAKE(cnt9, uint32_t, 20);
     PA_TAKE(ch0, int16_t, N);
     PA_TAKE(ch1, int16_t, N);
@@ -116,6 +135,8 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
     PA_TAKE(down, int16_t, N);
     PA_TAKE(downoff, int16_t, N + 1);
     PA_TAKE(tz, int16_t, N);
+    PA_TAKE(lvl, int16_t, N);
+    PA_TAKE(kbf, int16_t, N + 1);
     PA_TAKE(flags, uint8_t, N);
     PA_TAKE(fixZ, uint8_t, N);
     PA_TAKE(dirty, uint8_t, N);
@@ -123,7 +144,6 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
     PA_TAKE(changed, uint8_t, N);
     PA_TAKE(Z, int8_t, N);
     PA_TAKE(maxZ, int8_t, N);
-    PA_TAKE(tgs, uint8_t, (size_t)(I > 0 ? I : 1) * BS);
 #undef PA_TAKE
     return al(o, 16);
 }
@@ -132,7 +152,7 @@ size_t chain_smem_bytes(int N, int S, int BS, int fp64)
 {
     return (fp64 ? smem_arrays<double, false>(N, S, BS, nullptr, nullptr)
                  : smem_arrays<float, false>(N, S, BS, nullptr, nullptr)) +
-           (fp64 ? sizeof(Ctx<double>) : sizeof(Ctx<float>)) + sizeof(ModelDev);
+           (fp64 ? sizeof(Ctx<double>) : sizeof(Ctx<float>)) + sizeof(ModelDev) + sizeof(ChainArgs);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -234,96 +254,202 @@ __device__ __forceinline__ void leaf_mul(const LDS R *PT, int code, R &a0, R &a1
     }
 }
 
+// One child's inputs at one pattern: a leaf's code, or an internal child's
+// scaled partial and cumulative exponent.
+template <typename R>
+struct ChildIn {
+    V4<R> v;
+    int x;
+};
+
+template <typename R>
+__device__ __forceinline__ ChildIn<R> child_load(bool leaf, bool miss, const GBL uint8_t *code_row,
+                                                 const GBL V4<R> *part_row, const GBL int32_t *exp_row, int p)
+{
+    // a missing child contributes nothing, but its gap state still decides
+    // whether the parent is all-gap at this pattern, so its code / exponent
+    // is read either way
+    ChildIn<R> in;
+    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
+    if (leaf) {
+        in.x = code_row[p];
+    } else {
+        in.x = exp_row[p];
+        if (!miss) in.v = ld4<R>(part_row + p);
+    }
+    return in;
+}
+
 // Multiplies one child's message P * L_child into (a0..a3).  A leaf reads the
 // row of its base from the LDS matrix; an internal child uses the register
 // copy m.  gap reports the child's GAPT state (Tg >= num_base).
 template <typename R>
-__device__ __forceinline__ void child_term(bool leaf, bool miss, const LDS R *src, const M16<R> &m,
-                                           const GBL uint8_t *code_row, const GBL V4<R> *part_row,
-                                           const GBL int32_t *exp_row, int p, R &a0, R &a1, R &a2, R &a3, int &e,
-                                           bool &gap)
+__device__ __forceinline__ void child_apply(bool leaf, bool miss, const LDS R *src, const M16<R> &m,
+                                            const ChildIn<R> &in, R &a0, R &a1, R &a2, R &a3, int &e, bool &gap)
 {
     if (leaf) {
-        const int code = code_row[p];
-        gap = (code == PA_GAP_CODE || code == PA_OTHER_CODE);
-        if (!miss && !gap) leaf_mul<R>(src, code, a0, a1, a2, a3);
+        gap = (in.x == PA_GAP_CODE || in.x == PA_OTHER_CODE);
+        if (!miss && !gap) leaf_mul<R>(src, in.x, a0, a1, a2, a3);
     } else {
-        const int ex = exp_row[p];
-        gap = (ex == GAPT);
+        gap = (in.x == GAPT);
         if (!miss && !gap) {
-            const V4<R> v = ld4<R>(part_row + p);
+            const V4<R> v = in.v;
             const R o0 = m.v[0] * v.x + m.v[4] * v.y + m.v[8] * v.z + m.v[12] * v.w;
             const R o1 = m.v[1] * v.x + m.v[5] * v.y + m.v[9] * v.z + m.v[13] * v.w;
             const R o2 = m.v[2] * v.x + m.v[6] * v.y + m.v[10] * v.z + m.v[14] * v.w;
             const R o3 = m.v[3] * v.x + m.v[7] * v.y + m.v[11] * v.z + m.v[15] * v.w;
             a0 *= o0; a1 *= o1; a2 *= o2; a3 *= o3;
-            e += ex;
+            e += in.x;
         }
     }
 }
 
-// One post-order pass over internal nodes (BPP_C::Update_Tg(tosample=false),
-// src/PhyloAcc-ST/bpp_c.cpp:644-671, and log_f_Xz, bpp_c.cpp:860-928).
-// FULL: every internal node of the tree (missing nodes only record their GAPT
-// flag) into the current buffer.  DIRTY: nodes with dirty[] set, in place.
-// PROP: dirty nodes into the other buffer, with PTp for branches whose Z is
-// cls.  Returns the element's log-likelihood (FP64), identical on every thread.
+// Values every lane of the workgroup holds alike (node ids, offsets read
+// from LDS) are moved to scalar registers, which frees vector registers and
+// makes the address arithmetic scalar.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <typename T>
+__device__ __forceinline__ T *uni_ptr(T *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (T *)(((uint64_t)hi << 32) | lo);
+}
+
+// Patterns handled per thread per round: all their loads are issued before
+// the first is used, so a round costs one memory latency.
+template <typename R>
+struct Unroll { static constexpr int U = sizeof(R) == 4 ? 4 : 2; };
+
+// Nodes a pass visits, in level order (height 1, 2, ...): all internal nodes
+// (FULL) or the non-missing dirty ones (DIRTY / PROP).  Wave 0 compacts the
+// height-ordered list with ballots; lvl[] receives the nodes and kbf[q] the
+// number kept before position q of the height-ordered list.
+template <typename R, int BS>
+__device__ __forceinline__ void visit_list(CP<R> c, int mode)
+{
+    const int tid = threadIdx.x, n_int = c->n_int;
+    if (tid < 64) {
+        int kept = 0;
+        for (int base = 0; base < n_int; base += 64) {
+            const int q = base + tid;
+            bool keep = false;
+            int s = 0;
+            if (q < n_int) {
+                s = c->up[q];
+                keep = mode == PR_FULL || (c->dirty[s] && !(c->flags[s] & NF_MISSING));
+            }
+            const unsigned long long bal = __ballot(keep);
+            const int before = kept + __popcll(bal & ((1ull << tid) - 1ull));
+            if (q < n_int) {
+                c->kbf[q] = (int16_t)before;
+                if (keep) c->lvl[before] = (int16_t)s;
+            }
+            kept += __popcll(bal);
+        }
+        if (tid == 0) c->kbf[n_int] = (int16_t)kept;
+    }
+    __syncthreads();
+}
+
+// One pruning pass (BPP_C::Update_Tg(tosample=false), src/PhyloAcc-ST/bpp_c.cpp:644-671,
+// and log_f_Xz, bpp_c.cpp:860-928).  FULL: every internal node of the tree
+// (missing nodes only record their GAPT flag) into the current buffer.
+// DIRTY: nodes with dirty[] set, in place.  PROP: dirty nodes into the other
+// buffer, with PTp for branches whose Z is cls.  Returns the element's
+// log-likelihood (FP64), identical on every thread.
+//
+// A thread owns the patterns p = tid (mod BS) of every node, so it only ever
+// reads partials it wrote itself and the pass needs no barrier.  Nodes of one
+// tree level are independent: a round takes U (node, pattern block) items of
+// the same level and issues all their loads before the first is used, so a
+// round costs one memory latency whether the level is wide (many nodes) or
+// the element is long (many pattern blocks).
 template <typename R, int BS>
 __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
 {
     const int tid = threadIdx.x;
-    const int P = c->P, Ppad = c->Ppad, S = c->S, n_int = c->n_int;
-    const size_t bufI = (size_t)c->I * Ppad;
-    GBL V4<R> *parts = c->parts;
-    GBL int32_t *exps = c->exps;
-    const GBL uint8_t *codes = c->codes;
-    for (int k = 0; k < n_int; k++) {
-        const int s = c->inodes[k];
-        const bool miss_s = c->flags[s] & NF_MISSING;
-        if (mode != PR_FULL && (!c->dirty[s] || miss_s)) continue;
-        const int wb = (mode == PR_PROP) ? 1 - c->cur[s] : c->cur[s];
-        const int x0 = c->ch0[s], x1 = c->ch1[s];
-        const bool leaf0 = x0 < S, leaf1 = x1 < S;
-        const bool miss0 = c->flags[x0] & NF_MISSING, miss1 = c->flags[x1] & NF_MISSING;
-        const LDS R *src0 = (mode == PR_PROP && c->Z[x0] == cls) ? c->PTp + 16 * x0 : c->PTc + 16 * x0;
-        const LDS R *src1 = (mode == PR_PROP && c->Z[x1] == cls) ? c->PTp + 16 * x1 : c->PTc + 16 * x1;
-        M16<R> m0, m1;
+    const int P = uni(c->P), Ppad = uni(c->Ppad), S = uni(c->S);
+    const size_t bufI = (size_t)uni(c->I) * Ppad;
+    GBL V4<R> *parts = uni_ptr(c->parts);
+    GBL int32_t *exps = uni_ptr(c->exps);
+    const GBL uint8_t *codes = uni_ptr(c->codes);
+    visit_list<R, BS>(c, mode);
+    const int Pb = uni((P + BS - 1) / BS);
+    constexpr int U = Unroll<R>::U;
+    unsigned long long bytes = 0;
+    for (int h = 0; h < c->n_up; h++) {
+        const int a = uni(c->kbf[h == 0 ? 0 : c->upoff[h - 1]]), b = uni(c->kbf[c->upoff[h]]);
+        const int items = (b - a) * Pb;
+        for (int i0 = 0; i0 < items; i0 += U) {
+            ChildIn<R> in0[U], in1[U];
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            m0.v[i] = leaf0 ? (R)0 : src0[i];
-            m1.v[i] = leaf1 ? (R)0 : src1[i];
-        }
-        const int rb0 = leaf0 ? 0 : ((mode == PR_PROP && c->dirty[x0]) ? 1 - c->cur[x0] : c->cur[x0]);
-        const int rb1 = leaf1 ? 0 : ((mode == PR_PROP && c->dirty[x1]) ? 1 - c->cur[x1] : c->cur[x1]);
-        const GBL uint8_t *cr0 = codes + (size_t)x0 * Ppad, *cr1 = codes + (size_t)x1 * Ppad;
-        const size_t o0 = leaf0 ? 0 : rb0 * bufI + (size_t)(x0 - S) * Ppad;
-        const size_t o1 = leaf1 ? 0 : rb1 * bufI + (size_t)(x1 - S) * Ppad;
-        const GBL V4<R> *pr0 = parts + o0, *pr1 = parts + o1;
-        const GBL int32_t *er0 = exps + o0, *er1 = exps + o1;
-        GBL V4<R> *dstp = parts + wb * bufI + (size_t)(s - S) * Ppad;
-        GBL int32_t *dste = exps + wb * bufI + (size_t)(s - S) * Ppad;
-        if (tid == 0)   // write partial + exponent; read each child's partial + exponent, or its leaf code
-            c->bytes += (unsigned long long)P * ((4 * sizeof(R) + 4) + (leaf0 ? 1 : 4 * sizeof(R) + 4) + (leaf1 ? 1 : 4 * sizeof(R) + 4));
-        for (int p = tid; p < P; p += BS) {
-            R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
-            int e = 0;
-            bool g0, g1;
-            child_term<R>(leaf0, miss0, src0, m0, cr0, pr0, er0, p, a0, a1, a2, a3, e, g0);
-            child_term<R>(leaf1, miss1, src1, m1, cr1, pr1, er1, p, a0, a1, a2, a3, e, g1);
-            if (g0 && g1) {
-                dste[p] = GAPT;
-            } else if (miss_s) {
-                dste[p] = 0;
-            } else {
-                const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
-                int kx = 0;
-                frexp(mx, &kx);
-                a0 = ldexp(a0, -kx); a1 = ldexp(a1, -kx); a2 = ldexp(a2, -kx); a3 = ldexp(a3, -kx);
-                st4<R>(dstp + p, a0, a1, a2, a3);
-                dste[p] = e + kx;
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u;
+                if (i >= items) break;
+                const int q = i / Pb, p = (i - q * Pb) * BS + tid;
+                const int s = uni(c->lvl[a + q]);
+                const int x0 = uni(c->ch0[s]), x1 = uni(c->ch1[s]);
+                const bool leaf0 = x0 < S, leaf1 = x1 < S;
+                const bool miss0 = uni(c->flags[x0]) & NF_MISSING, miss1 = uni(c->flags[x1]) & NF_MISSING;
+                const int rb0 = leaf0 ? 0 : ((mode == PR_PROP && c->dirty[x0]) ? 1 - c->cur[x0] : c->cur[x0]);
+                const int rb1 = leaf1 ? 0 : ((mode == PR_PROP && c->dirty[x1]) ? 1 - c->cur[x1] : c->cur[x1]);
+                const size_t o0 = leaf0 ? 0 : rb0 * bufI + (size_t)(x0 - S) * Ppad;
+                const size_t o1 = leaf1 ? 0 : rb1 * bufI + (size_t)(x1 - S) * Ppad;
+                if (p < P) {
+                    in0[u] = child_load<R>(leaf0, miss0, codes + (size_t)x0 * Ppad, parts + o0, exps + o0, p);
+                    in1[u] = child_load<R>(leaf1, miss1, codes + (size_t)x1 * Ppad, parts + o1, exps + o1, p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                // one item at a time: keeps the scheduler from hoisting every
+                // item's matrices into registers at once
+                __builtin_amdgcn_sched_barrier(0);
+                const int i = i0 + u;
+                if (i >= items) break;
+                const int q = i / Pb, p = (i - q * Pb) * BS + tid;
+                const int s = uni(c->lvl[a + q]);
+                const int x0 = uni(c->ch0[s]), x1 = uni(c->ch1[s]);
+                const bool leaf0 = x0 < S, leaf1 = x1 < S;
+                const bool miss_s = uni(c->flags[s]) & NF_MISSING;
+                const bool miss0 = uni(c->flags[x0]) & NF_MISSING, miss1 = uni(c->flags[x1]) & NF_MISSING;
+                if (tid == 0 && i - q * Pb == 0)   // write partial + exponent; read each child's partial + exponent, or its leaf code
+                    bytes += (unsigned long long)P * ((4 * sizeof(R) + 4) + (leaf0 ? 1 : 4 * sizeof(R) + 4) + (leaf1 ? 1 : 4 * sizeof(R) + 4));
+                if (p >= P) continue;
+                const LDS R *src0 = (mode == PR_PROP && c->Z[x0] == cls) ? c->PTp + 16 * x0 : c->PTc + 16 * x0;
+                const LDS R *src1 = (mode == PR_PROP && c->Z[x1] == cls) ? c->PTp + 16 * x1 : c->PTc + 16 * x1;
+                M16<R> m0, m1;
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    m0.v[k] = leaf0 ? (R)0 : src0[k];
+                    m1.v[k] = leaf1 ? (R)0 : src1[k];
+                }
+                const int wb = (mode == PR_PROP) ? 1 - c->cur[s] : c->cur[s];
+                GBL V4<R> *dstp = parts + wb * bufI + (size_t)(s - S) * Ppad;
+                GBL int32_t *dste = exps + wb * bufI + (size_t)(s - S) * Ppad;
+                R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
+                int e = 0;
+                bool g0, g1;
+                child_apply<R>(leaf0, miss0, src0, m0, in0[u], a0, a1, a2, a3, e, g0);
+                child_apply<R>(leaf1, miss1, src1, m1, in1[u], a0, a1, a2, a3, e, g1);
+                if (g0 && g1) {
+                    dste[p] = GAPT;
+                } else if (miss_s) {
+                    dste[p] = 0;
+                } else {
+                    const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
+                    int kx = 0;
+                    frexp(mx, &kx);
+                    a0 = ldexp(a0, -kx); a1 = ldexp(a1, -kx); a2 = ldexp(a2, -kx); a3 = ldexp(a3, -kx);
+                    st4<R>(dstp + p, a0, a1, a2, a3);
+                    dste[p] = e + kx;
+                }
             }
         }
     }
+    if (tid == 0) c->bytes += bytes;
     // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
     const int r = c->root;
     double acc = 0;
@@ -333,12 +459,25 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
         const GBL int32_t *re = exps + rb * bufI + (size_t)(r - S) * Ppad;
         const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
         if (tid == 0) c->bytes += (unsigned long long)P * (4 * sizeof(R) + 4 + 4);   // root partial, exponent, weight
-        for (int p = tid; p < P; p += BS) {
-            const int ex = re[p];
-            if (ex == GAPT) continue;
-            const V4<R> a = ld4<R>(rp + p);
-            const double L = p0 * (double)a.x + p1 * (double)a.y + p2 * (double)a.z + p3 * (double)a.w;
-            acc += (double)c->wts[p] * (log(L) + (double)ex * 0.69314718055994530942);
+        for (int pb = tid; pb < P; pb += U * BS) {
+            int ex[U], wt[U];
+            V4<R> av[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int p = pb + u * BS;
+                ex[u] = GAPT;
+                if (p < P) {
+                    ex[u] = re[p];
+                    wt[u] = c->wts[p];
+                    av[u] = ld4<R>(rp + p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (ex[u] == GAPT) continue;
+                const double L = p0 * (double)av[u].x + p1 * (double)av[u].y + p2 * (double)av[u].z + p3 * (double)av[u].w;
+                acc += (double)wt[u] * (log(L) + (double)ex[u] * 0.69314718055994530942);
+            }
         }
     }
     return block_sum<R>(acc, c);
@@ -353,15 +492,17 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
 template <typename R, int BS>
 __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c3)
 {
-    const int tid = threadIdx.x, S = c->S, n_int = c->n_int, GG = c->GG, Ppad = c->Ppad;
-    const size_t bufI = (size_t)c->I * Ppad;
-    const int r = c->root;
+    const int tid = threadIdx.x, S = uni(c->S), n_int = uni(c->n_int), GG = uni(c->GG), Ppad = uni(c->Ppad);
+    const size_t bufI = (size_t)uni(c->I) * Ppad;
+    const int r = uni(c->root);
     const uint32_t k0 = M->key0, k1 = M->key1;
     const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
-    const GBL V4<R> *parts = c->parts;
-    const GBL int32_t *exps = c->exps;
-    const GBL uint8_t *codes = c->codes;
+    const GBL V4<R> *parts = uni_ptr(c->parts);
+    const GBL int32_t *exps = uni_ptr(c->exps);
+    const GBL uint8_t *codes = uni_ptr(c->codes);
     LDS uint8_t *tgs = c->tgs;
+    for (int i = tid; i < 16 * c->N; i += BS) c->hist[i] = 0;   // shares storage with the proposal matrices
+    __syncthreads();
     if (tid == 0) {
         // per site: pattern index, root partial, each non-missing internal node's partial, each leaf code
         int nint = 0, nleaf = 0;
@@ -383,30 +524,64 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
             }
             tgs[(r - S) * BS + tid] = (uint8_t)categorical<4>(pr, u01(blk.x));
         }
+        // internal nodes in reverse post-order, K at a time: the partial loads
+        // of a chunk do not depend on the draws, so they are all issued first
+        constexpr int K = sizeof(R) == 4 ? 8 : 4;
         int t = 1;
-        for (int k = n_int - 2; k >= 0; k--, t++) {
-            if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
-            const int s = c->inodes[k];
-            if (c->flags[s] & NF_MISSING) continue;
-            const size_t off = c->cur[s] * bufI + (size_t)(s - S) * Ppad + p;
-            if (exps[off] == GAPT) continue;
-            const int tp = tgs[(c->par[s] - S) * BS + tid];
-            const V4<R> a = ld4<R>(parts + off);
-            const LDS R *PT = c->PTc + 16 * s;
-            double pr[4] = {(double)PT[0 * 4 + tp] * (double)a.x, (double)PT[1 * 4 + tp] * (double)a.y,
-                            (double)PT[2 * 4 + tp] * (double)a.z, (double)PT[3 * 4 + tp] * (double)a.w};
-            const int ts = categorical<4>(pr, u01(pick(blk, t & 3)));
-            tgs[(s - S) * BS + tid] = (uint8_t)ts;
-            lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
+        for (int kb = n_int - 2; kb >= 0; kb -= K) {
+            int ex[K], sn[K];
+            V4<R> av[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int k = kb - j;
+                ex[j] = GAPT;
+                sn[j] = 0;
+                if (k >= 0) {
+                    const int s = uni(c->inodes[k]);
+                    sn[j] = s;
+                    if (!(uni(c->flags[s]) & NF_MISSING)) {
+                        const size_t off = uni(c->cur[s]) * bufI + (size_t)(s - S) * Ppad + p;
+                        ex[j] = exps[off];
+                        av[j] = ld4<R>(parts + off);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++, t++) {
+                if (kb - j < 0) break;
+                if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
+                if (ex[j] == GAPT) continue;     // missing node, or an all-gap subtree at this site
+                const int s = sn[j];
+                const int tp = tgs[(c->par[s] - S) * BS + tid];
+                const V4<R> a = av[j];
+                const LDS R *PT = c->PTc + 16 * s;
+                double pr[4] = {(double)PT[0 * 4 + tp] * (double)a.x, (double)PT[1 * 4 + tp] * (double)a.y,
+                                (double)PT[2 * 4 + tp] * (double)a.z, (double)PT[3 * 4 + tp] * (double)a.w};
+                const int ts = categorical<4>(pr, u01(pick(blk, t & 3)));
+                tgs[(s - S) * BS + tid] = (uint8_t)ts;
+                lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
+            }
         }
-        for (int s = 0; s < S; s++) {
-            const uint8_t f = c->flags[s];
-            if (!(f & NF_IN_TREE) || (f & NF_MISSING)) continue;
-            const int code = codes[(size_t)s * Ppad + p];
-            if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
-            const int tp = tgs[(c->par[s] - S) * BS + tid];
-            if (code < 4) lds_add(&c->hist[s * 16 + tp * 4 + code], 1u);
-            else __hip_atomic_fetch_add(&c->amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // leaves, 16 codes loaded at a time
+        for (int sb = 0; sb < S; sb += 16) {
+            int cd[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int s = sb + j;
+                cd[j] = PA_GAP_CODE;
+                if (s < S) {
+                    const uint8_t f = c->flags[s];
+                    if ((f & NF_IN_TREE) && !(f & NF_MISSING)) cd[j] = codes[(size_t)s * Ppad + p];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int s = sb + j, code = cd[j];
+                if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
+                const int tp = tgs[(c->par[s] - S) * BS + tid];
+                if (code < 4) lds_add(&c->hist[s * 16 + tp * 4 + code], 1u);
+                else __hip_atomic_fetch_add(&c->amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
     }
 }
@@ -716,15 +891,17 @@ __device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int re
 template <typename R>
 __device__ __noinline__ void transition_draw(MP M, CP<R> c, int ms, int m)
 {
+    // counts n(child, parent) in cnt9[child * 3 + parent]; the fixZ-free
+    // counts subtract cnt9[9 + ...].  Each draw reads its counts from LDS right
+    // before it, so nothing but the results stays live across the draws.
     const Key key{M->key0, M->key1};
     const uint32_t c3 = tagword(ms, TAG_TRANS), c2 = (uint32_t)m;
-    double n[9], nf[9];
-    for (int i = 0; i < 9; i++) { n[i] = c->cnt9[i]; nf[i] = n[i] - c->cnt9[9 + i]; }
-    const double g = beta_draw(key, 0, 1, c2, c3, M->ha_g + n[1 * 3 + 0] + n[2 * 3 + 0], M->hb_g + n[0]);
-    const double l = beta_draw(key, 2, 3, c2, c3, M->ha_l + nf[2 * 3 + 1], M->hb_l + nf[1 * 3 + 1]);
-    const double l2 = M->ha_l2 == 0 ? 0.0 : beta_draw(key, 4, 5, c2, c3, M->ha_l2 + nf[1 * 3 + 2], M->hb_l2 + nf[2 * 3 + 2]);
-    c->g = g; c->l = l; c->l2 = l2;
-    set_TM(c->TMf, c->TMn, g, l, l2);
+    auto nf = [&](int i) { return (double)c->cnt9[i] - (double)c->cnt9[9 + i]; };
+    c->g = beta_draw(key, 0, 1, c2, c3, M->ha_g + (double)c->cnt9[1 * 3 + 0] + (double)c->cnt9[2 * 3 + 0],
+                     M->hb_g + (double)c->cnt9[0]);
+    c->l = beta_draw(key, 2, 3, c2, c3, M->ha_l + nf(2 * 3 + 1), M->hb_l + nf(1 * 3 + 1));
+    c->l2 = M->ha_l2 == 0 ? 0.0 : beta_draw(key, 4, 5, c2, c3, M->ha_l2 + nf(1 * 3 + 2), M->hb_l2 + nf(2 * 3 + 2));
+    set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
 }
 
 template <typename R, int BS>

@@ -59,3 +59,48 @@ def test_chain_fp64_matches_oracle_trajectory(require_gpu):
             np.testing.assert_allclose(res["max_loglik"][k, ms], r["max_loglik"], rtol=1e-9)
             assert res["max_m"][k, ms] == r["max_m"]
             np.testing.assert_array_equal(res["max_Z"][k, ms], r["max_Z"])
+
+
+def _with_missing(config, n, seed):
+    """Workload whose elements have missing species: both leaves of a cherry
+    (so their parent is missing too) and one more leaf keep only ~10% of
+    their bases; the rest are gaps."""
+    w = synth.make_workload(config, seed=seed, n_elements=n)
+    t = w.tree
+    cherry = next(v for v in range(t.S, t.N) if t.children[v][0] < t.S and t.children[v][1] < t.S)
+    a, b = t.children[cherry]
+    other = next(s for s in range(t.S) if s not in (a, b) and s not in w.outgroup)
+    rng = np.random.default_rng(seed)
+    al = w.align.copy()
+    for s in (a, b, other):
+        mask = rng.random(al.shape[1]) < 0.9
+        al[s, mask] = ord("-")
+    w.align = al
+    return w
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_missing_species_loglik_and_chain(require_gpu, config):
+    w = _with_missing(config, 6, 31)
+    kw = dict(burnin=15, mcmc=45, adapt_freq=20)
+    eng = E.Engine.from_workload(w, E.default_options(**kw))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    ids = ids[st == E.PA_ELEM_OK]
+    assert len(ids) > 0
+    assert any(eng.element(int(i))["missing"].sum() >= 3 for i in ids)
+    m = O.model_from_workload(w)
+    oo = O.default_options(**kw)
+    rng = np.random.default_rng(1)
+    Z = _random_Z(eng, ids, rng)
+    rates = np.stack([rng.uniform(0.1, 0.9, len(ids)), rng.uniform(1.2, 4.0, len(ids))], 1)
+    ref = np.array([O.Element(m, oo, w.align, w.starts[i], w.sizes[i]).loglik(Z[k], rates[k, 0], rates[k, 1])
+                    for k, i in enumerate(ids)])
+    np.testing.assert_allclose(eng.loglik(ids, Z, rates, E.PA_FP64), ref, rtol=1e-9)
+    np.testing.assert_allclose(eng.loglik(ids, Z, rates, E.PA_FP32), ref, rtol=1e-6)
+    res = eng.run_chains(ids, precision=E.PA_FP64, traces=True)
+    for k, i in enumerate(ids):
+        r = O.Element(m, oo, w.align, w.starts[i], w.sizes[i]).run(oo)
+        for ms in range(3):
+            np.testing.assert_array_equal(res["Z"][k, ms], r[ms]["Z"])
+            np.testing.assert_allclose(res["loglik"][k, ms], r[ms]["loglik"], rtol=1e-9)
+            assert res["max_m"][k, ms] == r[ms]["max_m"]
