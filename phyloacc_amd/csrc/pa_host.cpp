@@ -511,7 +511,7 @@ struct pa_engine {
     uint8_t *d_codes = nullptr;
     int32_t *d_w = nullptr, *d_site = nullptr, *d_tab = nullptr;
     int *d_queue = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr, stream2 = nullptr;   // stream2: second size class of a batched launch
     int n_cu = 0;
     size_t lds_per_cu = 0;
 };
@@ -635,6 +635,7 @@ static pa_status ensure_device(pa_engine *e)
     e->n_cu = prop.multiProcessorCount;
     e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
     HIPCHK(e, hipStreamCreate(&e->stream));
+    HIPCHK(e, hipStreamCreate(&e->stream2));
     HIPCHK(e, hipMalloc(&e->d_queue, 256));
     return PA_OK;
 }
@@ -655,6 +656,7 @@ extern "C" void pa_engine_destroy(pa_engine *e)
         free_arenas(e);
         (void)hipFree(e->d_queue);
         (void)hipStreamDestroy(e->stream);
+        (void)hipStreamDestroy(e->stream2);
     }
     delete e;
 }
@@ -819,65 +821,131 @@ struct DevBuf {
 
 }  // namespace
 
-// Common launch path for the chain and log-likelihood kernels.
+// Outputs of a launch that starts at work position i of the whole list.
+static ChainArgs shifted(const ChainArgs &A, int i, int N, int T)
+{
+    ChainArgs B = A;
+    B.work = A.work + i;
+    B.res_d = A.res_d ? A.res_d + (size_t)i * 24 : nullptr;
+    B.res_i = A.res_i ? A.res_i + (size_t)i * 12 : nullptr;
+    B.maxZ = A.maxZ ? A.maxZ + (size_t)i * 3 * N : nullptr;
+    B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * T : nullptr;
+    B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * T * N : nullptr;
+    B.Zin = A.Zin ? A.Zin + (size_t)i * N : nullptr;
+    B.rates_in = A.rates_in ? A.rates_in + (size_t)i * 2 : nullptr;
+    B.ll_out = A.ll_out ? A.ll_out + i : nullptr;
+    B.len_in = A.len_in ? A.len_in + (size_t)i * N : nullptr;
+    return B;
+}
+
+// Workgroups of block size BS that fit on one CU at once (LDS and waves).
+static int fit_per_cu(pa_engine *e, int N, int S, int BS, int fp64)
+{
+    const size_t smem = chain_smem_bytes(N, S, BS, fp64);
+    int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
+    return std::max(1, std::min(per_cu, 2048 / BS));
+}
+
+// Patterns above which a batched chain launch gives an element a 256-thread
+// workgroup; smaller elements get 64 threads.  Long elements would otherwise
+// set the end of the launch (the lengths are heavy-tailed).
+static constexpr int kLongP = 256;
+
+// Common launch path for the chain and log-likelihood kernels.  ids arrive
+// in schedule order (largest first for batched runs).  With bs_req == 0 a
+// batched chain run splits into two size classes launched side by side on
+// two streams: long elements on 256-thread workgroups, the rest on 64-thread
+// ones, each class given a share of the GPU's wave slots proportional to its
+// pattern count.
 static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, int kind, int mode, int bs_req,
                             ChainArgs &A, DevBuf *bufs, pa_run_stats *stats)
 {
-    const int S = e->tm.S, N = e->tm.N;
-    // ids arrive in schedule order (largest first for batched runs)
+    const int S = e->tm.S, N = e->tm.N, T = e->M.T;
     std::vector<int32_t> work(ids, ids + n);
-    int Pcap = 64;
+    if (bs_req != 0 && bs_req != 64 && bs_req != 128 && bs_req != 256)
+        return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
+    struct Cls { int first, count, BS, grid, Pcap; size_t slot; };
+    std::vector<Cls> cls;
     double meanP = 0;
-    for (int i = 0; i < n; i++) {
-        Pcap = std::max(Pcap, e->elems[work[i]].Ppad);
-        meanP += e->elems[work[i]].P;
-    }
+    for (int i = 0; i < n; i++) meanP += e->elems[work[i]].P;
     meanP /= std::max(1, n);
-    int BS = bs_req;
-    if (BS == 0) BS = (meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256);
-    if (BS != 64 && BS != 128 && BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
-    const size_t smem = chain_smem_bytes(N, S, BS, fp64);
-    if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
-    int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
-    per_cu = std::max(1, std::min(per_cu, 2048 / BS));
-    int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, e->n_cu * per_cu);
-    size_t off[5];
-    const size_t slot = slot_layout(N, S, Pcap, fp64, off);
-    HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
+    if (mode == PA_RUN_PER_ELEMENT) {
+        cls.push_back({0, n, bs_req ? bs_req : 64, 1, 64, 0});
+    } else if (bs_req != 0 || kind != 0) {
+        int BS = bs_req ? bs_req : ((meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256));
+        cls.push_back({0, n, BS, std::min(n, e->n_cu * fit_per_cu(e, N, S, BS, fp64)), 64, 0});
+    } else {
+        int nl = 0;
+        double wl = 0, wt = 0;
+        for (int i = 0; i < n; i++) {
+            const double w = e->elems[work[i]].P;
+            wt += w;
+            if (e->elems[work[i]].P > kLongP) { nl = i + 1; wl += w; }
+        }
+        const int ns = n - nl;
+        const int waves = e->n_cu * 8;   // wave slots at two waves per SIMD
+        if (nl > 0) {
+            int g = (int)std::lround(waves * (wl / wt) / 4.0);
+            if (ns == 0) g = e->n_cu * fit_per_cu(e, N, S, 256, fp64);
+            g = std::max(1, std::min({g, nl, e->n_cu * fit_per_cu(e, N, S, 256, fp64)}));
+            cls.push_back({0, nl, 256, g, 64, 0});
+        }
+        if (ns > 0) {
+            // both grids must be resident together, or the persistent small-element
+            // workgroups would hold the GPU until their queue drains: each long
+            // workgroup takes the place of four 64-thread ones
+            const int taken = nl > 0 ? 4 * cls[0].grid : 0;
+            const int g = std::max(1, e->n_cu * fit_per_cu(e, N, S, 64, fp64) - taken);
+            cls.push_back({nl, ns, 64, std::min(ns, g), 64, 0});
+        }
+    }
+    size_t total = 0, off[5];
+    for (Cls &k : cls) {
+        for (int i = k.first; i < k.first + k.count; i++) k.Pcap = std::max(k.Pcap, e->elems[work[i]].Ppad);
+        if (chain_smem_bytes(N, S, k.BS, fp64) > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+        k.slot = slot_layout(N, S, k.Pcap, fp64, off);
+        total += k.slot * k.grid;
+    }
+    HIPCHK(e, hipMalloc(&bufs[0].p, total));
     HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + 64));
     unsigned long long *d_cnt = (unsigned long long *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
     HIPCHK(e, hipMemsetAsync(d_cnt, 0, 16, e->stream));
     A.counters = d_cnt;
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, work.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
-    A.queue = e->d_queue;
-    A.scratch = (char *)bufs[0].p; A.slot_bytes = (int64_t)slot;
-    hipEvent_t t0, t1;
+    A.work = (const int32_t *)bufs[1].p;
+    hipEvent_t t0, t1, t2;
     HIPCHK(e, hipEventCreate(&t0));
     HIPCHK(e, hipEventCreate(&t1));
+    HIPCHK(e, hipEventCreate(&t2));
     HIPCHK(e, hipEventRecord(t0, e->stream));
     if (mode == PA_RUN_PER_ELEMENT) {
+        const size_t slot = slot_layout(N, S, cls[0].Pcap, fp64, off);
         for (int i = 0; i < n; i++) {
-            ChainArgs B = A;
-            B.work = (const int32_t *)bufs[1].p + i;
+            ChainArgs B = shifted(A, i, N, T);
             B.n_work = 1;
-            // per-element outputs shift with the work position
-            B.res_d = A.res_d ? A.res_d + (size_t)i * 24 : nullptr;
-            B.res_i = A.res_i ? A.res_i + (size_t)i * 12 : nullptr;
-            B.maxZ = A.maxZ ? A.maxZ + (size_t)i * 3 * N : nullptr;
-            B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * e->M.T : nullptr;
-            B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * e->M.T * N : nullptr;
-            B.Zin = A.Zin ? A.Zin + (size_t)i * N : nullptr;
-            B.rates_in = A.rates_in ? A.rates_in + (size_t)i * 2 : nullptr;
-            B.ll_out = A.ll_out ? A.ll_out + i : nullptr;
+            B.queue = e->d_queue;
+            B.scratch = (char *)bufs[0].p;
+            B.slot_bytes = (int64_t)slot;
             HIPCHK(e, launch_reset(e->d_queue, e->stream));
-            HIPCHK(e, launch_kernel(e->M, B, fp64, BS, 1, kind, e->stream));
+            HIPCHK(e, launch_kernel(e->M, B, fp64, cls[0].BS, 1, kind, e->stream));
         }
     } else {
-        A.work = (const int32_t *)bufs[1].p;
-        A.n_work = n;
-        HIPCHK(e, launch_reset(e->d_queue, e->stream));
-        HIPCHK(e, launch_kernel(e->M, A, fp64, BS, grid, kind, e->stream));
+        HIPCHK(e, hipStreamWaitEvent(e->stream2, t0, 0));
+        size_t base = 0;
+        for (size_t k = 0; k < cls.size(); k++) {
+            hipStream_t st = k == 0 ? e->stream : e->stream2;
+            ChainArgs B = shifted(A, cls[k].first, N, T);
+            B.n_work = cls[k].count;
+            B.queue = e->d_queue + 16 * k;
+            B.scratch = (char *)bufs[0].p + base;
+            B.slot_bytes = (int64_t)cls[k].slot;
+            base += cls[k].slot * cls[k].grid;
+            HIPCHK(e, launch_reset(B.queue, st));
+            HIPCHK(e, launch_kernel(e->M, B, fp64, cls[k].BS, cls[k].grid, kind, st));
+        }
+        HIPCHK(e, hipEventRecord(t2, e->stream2));
+        HIPCHK(e, hipStreamWaitEvent(e->stream, t2, 0));
     }
     HIPCHK(e, hipEventRecord(t1, e->stream));
     HIPCHK(e, hipEventSynchronize(t1));
@@ -888,13 +956,15 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
         HIPCHK(e, hipMemcpy(&cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
         stats->elapsed_ms = ms;
         stats->algorithmic_bytes = (double)cnt;
-        stats->grid = grid;
-        stats->block_threads = BS;
-        stats->lds_bytes = (int32_t)smem;
-        stats->n_launches = mode == PA_RUN_PER_ELEMENT ? n : 1;
+        stats->grid = 0;
+        for (const Cls &k : cls) stats->grid += k.grid;
+        stats->block_threads = cls.back().BS;
+        stats->lds_bytes = (int32_t)chain_smem_bytes(N, S, cls.back().BS, fp64);
+        stats->n_launches = mode == PA_RUN_PER_ELEMENT ? n : (int32_t)cls.size();
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    (void)hipEventDestroy(t2);
     return PA_OK;
 }
 

@@ -104,3 +104,21 @@ def test_missing_species_loglik_and_chain(require_gpu, config):
             np.testing.assert_array_equal(res["Z"][k, ms], r[ms]["Z"])
             np.testing.assert_allclose(res["loglik"][k, ms], r[ms]["loglik"], rtol=1e-9)
             assert res["max_m"][k, ms] == r[ms]["max_m"]
+
+
+def test_chain_fp64_two_size_classes(require_gpu):
+    """A batch mixing short and long elements runs as two concurrent size
+    classes (64- and 256-thread workgroups); both match the oracle."""
+    kw = dict(burnin=10, mcmc=30, adapt_freq=15)
+    w, eng, ids, st, m, oo = _setup(2, n=12, seed=17, **kw)
+    ids = ids[st == E.PA_ELEM_OK]
+    P = np.array([eng.element(int(i))["P"] for i in ids])
+    assert (P > 256).any() and (P <= 256).any()
+    res = eng.run_chains(ids, precision=E.PA_FP64, traces=True)
+    assert res["stats"]["n_launches"] == 2
+    for k, i in enumerate(ids):
+        ref = O.Element(m, oo, w.align, w.starts[i], w.sizes[i]).run(oo)
+        for ms in range(3):
+            np.testing.assert_array_equal(res["Z"][k, ms], ref[ms]["Z"])
+            np.testing.assert_allclose(res["loglik"][k, ms], ref[ms]["loglik"], rtol=1e-9)
+            assert res["max_m"][k, ms] == ref[ms]["max_m"]
