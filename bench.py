@@ -60,6 +60,10 @@ def main():
     ap.add_argument("--cpu-burnin", type=int, default=50)
     ap.add_argument("--cpu-mcmc", type=int, default=550)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default size classes")
+    ap.add_argument("--tune-sample", type=int, default=2048)
+    ap.add_argument("--no-agree", action="store_true", help="skip the FP32-vs-FP64 agreement sample")
+    ap.add_argument("--agree-sample", type=int, default=256)
     ap.add_argument("--arrival-rate", type=float, default=2000.0, help="config 4: requests per second")
     ap.add_argument("--chain", type=int, default=None, help="config 4: states per model chain (BURNIN = 1/4)")
     ap.add_argument("--sweep-budget", type=int, default=100, help="config 4: sweeps per chain per launch")
@@ -91,6 +95,11 @@ def main():
     T = args.burnin + args.mcmc
     sweeps_per_step = len(keep) * 3 * (T - 1)
 
+    tuning = None
+    if args.block == 0 and not args.no_tune:
+        tt = time.time()
+        tuning = eng.autotune(keep, precision=prec, sample=args.tune_sample, states=16)
+        log(f"autotune: {tuning} ({time.time() - tt:.1f}s)")
     for i in range(args.warmup):
         r = eng.run_chains(keep, precision=prec, block_threads=args.block, keep_traces=True)
         log(f"warmup {i}: {r['elapsed_ms'] / 1e3:.2f}s device")
@@ -109,9 +118,14 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "kernel": "chain_kernel", "algorithmic_bytes_per_launch": stats["algorithmic_bytes"],
-                "grid": stats["grid"], "block_threads": stats["block_threads"], "lds_bytes": stats["lds_bytes"]}
+                "grid": stats["grid"], "block_threads": stats["block_threads"], "lds_bytes": stats["lds_bytes"],
+                "launches": stats["n_launches"], "tuning": tuning}
 
     rng = np.random.default_rng(0)
+    agree = None
+    if not args.no_agree and prec == E.PA_FP32:
+        agree = fp32_vs_fp64(E, eng, keep, w, args, rng)
+        log(f"fp32 vs fp64: {agree}")
     plain = None
     if not args.no_plain:
         # plain FP64 path: per-element launches over an uncompressed copy of a stratified sample
@@ -173,8 +187,39 @@ def main():
         "cpu_baseline": cpu,
         "plain_gpu": plain,
         "speedup_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
+        "fp32_vs_fp64": agree,
     }
     print(json.dumps(out))
+
+
+def fp32_vs_fp64(E, eng, keep, w, args, rng):
+    """FP32 partials against the FP64 path on a length-stratified sample:
+    the same chains (same random streams) at both precisions.  Reports the
+    largest relative difference of the per-model maximum log-likelihood, of
+    the log-likelihood at a fixed (Z, r), the share of (element, model)
+    chains whose whole Z trajectory is identical, and the share whose per-
+    branch Z posterior modes (after burn-in) are identical."""
+    sizes = np.array([eng.element(int(i))["GG"] for i in keep])
+    idx = keep[stratified(sizes, min(args.agree_sample, len(keep)), rng)]
+    r32 = eng.run_chains(idx, precision=E.PA_FP32, traces=True)
+    r64 = eng.run_chains(idx, precision=E.PA_FP64, traces=True)
+    rel = np.abs(r32["max_loglik"] - r64["max_loglik"]) / np.abs(r64["max_loglik"])
+    same_traj = np.all(r32["Z"] == r64["Z"], axis=(2, 3))
+    b = args.burnin
+    def modes(Z):
+        cnt = np.stack([(Z[:, :, b:, :] == z).sum(axis=2) for z in (-1, 0, 1, 2)], axis=-1)
+        return cnt.argmax(-1)
+    same_mode = np.all(modes(r32["Z"]) == modes(r64["Z"]), axis=-1)
+    # fixed (Z, r): each element's M2 maximum state, rates at the chain's last values
+    Zf = r64["max_Z"][:, 2, :].astype(np.int32)
+    rates = np.stack([r64["final_crate"][:, 2], r64["final_nrate"][:, 2]], 1)
+    l32 = eng.loglik(idx, Zf, rates, E.PA_FP32)
+    l64 = eng.loglik(idx, Zf, rates, E.PA_FP64)
+    return {"elements": int(len(idx)), "chains": int(same_traj.size),
+            "max_rel_err_max_loglik": float(rel.max()),
+            "max_rel_err_loglik_fixed": float(np.max(np.abs(l32 - l64) / np.abs(l64))),
+            "identical_Z_trajectories": round(float(same_traj.mean()), 4),
+            "identical_Z_posterior_modes": round(float(same_mode.mean()), 4)}
 
 
 def serve(args, E, synth):

@@ -514,6 +514,7 @@ struct pa_engine {
     hipStream_t stream = nullptr, stream2 = nullptr;   // stream2: second size class of a batched launch
     int n_cu = 0;
     size_t lds_per_cu = 0;
+    pa_tuning tune = {64, 256, 256, 0.0};   // size classes of batched chain launches
 };
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
@@ -846,17 +847,14 @@ static int fit_per_cu(pa_engine *e, int N, int S, int BS, int fp64)
     return std::max(1, std::min(per_cu, 2048 / BS));
 }
 
-// Patterns above which a batched chain launch gives an element a 256-thread
-// workgroup; smaller elements get 64 threads.  Long elements would otherwise
-// set the end of the launch (the lengths are heavy-tailed).
-static constexpr int kLongP = 256;
-
 // Common launch path for the chain and log-likelihood kernels.  ids arrive
 // in schedule order (largest first for batched runs).  With bs_req == 0 a
 // batched chain run splits into two size classes launched side by side on
-// two streams: long elements on 256-thread workgroups, the rest on 64-thread
+// two streams: elements with more than tune.long_patterns patterns on
+// tune.long_threads-thread workgroups, the rest on tune.small_threads-thread
 // ones, each class given a share of the GPU's wave slots proportional to its
-// pattern count.
+// pattern count.  Long elements would otherwise set the end of the launch
+// (element lengths are heavy-tailed).
 static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, int kind, int mode, int bs_req,
                             ChainArgs &A, DevBuf *bufs, pa_run_stats *stats)
 {
@@ -875,28 +873,29 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
         int BS = bs_req ? bs_req : ((meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256));
         cls.push_back({0, n, BS, std::min(n, e->n_cu * fit_per_cu(e, N, S, BS, fp64)), 64, 0});
     } else {
+        const int bsl = e->tune.long_threads, bss = e->tune.small_threads;
         int nl = 0;
         double wl = 0, wt = 0;
         for (int i = 0; i < n; i++) {
             const double w = e->elems[work[i]].P;
             wt += w;
-            if (e->elems[work[i]].P > kLongP) { nl = i + 1; wl += w; }
+            if (e->elems[work[i]].P > e->tune.long_patterns) { nl = i + 1; wl += w; }
         }
         const int ns = n - nl;
         const int waves = e->n_cu * 8;   // wave slots at two waves per SIMD
         if (nl > 0) {
-            int g = (int)std::lround(waves * (wl / wt) / 4.0);
-            if (ns == 0) g = e->n_cu * fit_per_cu(e, N, S, 256, fp64);
-            g = std::max(1, std::min({g, nl, e->n_cu * fit_per_cu(e, N, S, 256, fp64)}));
-            cls.push_back({0, nl, 256, g, 64, 0});
+            const int full = e->n_cu * fit_per_cu(e, N, S, bsl, fp64);
+            int g = ns == 0 ? full : (int)std::lround(waves * (wl / wt) / (bsl / 64));
+            g = std::max(1, std::min({g, nl, full}));
+            cls.push_back({0, nl, bsl, g, 64, 0});
         }
         if (ns > 0) {
             // both grids must be resident together, or the persistent small-element
-            // workgroups would hold the GPU until their queue drains: each long
-            // workgroup takes the place of four 64-thread ones
-            const int taken = nl > 0 ? 4 * cls[0].grid : 0;
-            const int g = std::max(1, e->n_cu * fit_per_cu(e, N, S, 64, fp64) - taken);
-            cls.push_back({nl, ns, 64, std::min(ns, g), 64, 0});
+            // workgroups would hold the GPU until their queue drains: a long
+            // workgroup takes the place of bsl / bss small ones
+            const int taken = nl > 0 ? cls[0].grid * std::max(1, bsl / bss) : 0;
+            const int g = std::max(1, e->n_cu * fit_per_cu(e, N, S, bss, fp64) - taken);
+            cls.push_back({nl, ns, bss, std::min(ns, g), 64, 0});
         }
     }
     size_t total = 0, off[5];
@@ -975,6 +974,16 @@ static pa_status check_ids(pa_engine *e, const int32_t *ids, int32_t n)
         if (e->elems[ids[i]].filtered) return fail(e, PA_E_INPUT, "element " + std::to_string(ids[i]) + " is filtered");
     }
     return PA_OK;
+}
+
+static std::vector<int32_t> schedule(pa_engine *e, const int32_t *ids, int32_t n, int mode);
+
+// ids in launch order (longest first)
+static std::vector<int32_t> schedule_ids(pa_engine *e, const std::vector<int32_t> &ids)
+{
+    std::vector<int32_t> order = schedule(e, ids.data(), (int32_t)ids.size(), PA_RUN_BATCHED), out(ids.size());
+    for (size_t i = 0; i < ids.size(); i++) out[i] = ids[order[i]];
+    return out;
 }
 
 static std::vector<int32_t> schedule(pa_engine *e, const int32_t *ids, int32_t n, int mode)
@@ -1300,6 +1309,79 @@ extern "C" pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, c
     }
     *vb = VB;
     return PA_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+
+extern "C" pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t)
+{
+    if (!e || !t) return fail(e, PA_E_INPUT, "null argument");
+    auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
+    if (!ok(t->small_threads) || !ok(t->long_threads) || t->long_patterns < 0)
+        return fail(e, PA_E_INPUT, "tuning: block sizes must be 64, 128 or 256");
+    e->tune = *t;
+    return PA_OK;
+}
+
+extern "C" pa_status pa_get_tuning(pa_engine *e, pa_tuning *t)
+{
+    if (!e || !t) return fail(e, PA_E_INPUT, "null argument");
+    *t = e->tune;
+    return PA_OK;
+}
+
+// Times every candidate size-class configuration on a length-stratified
+// sample of the ids with short chains (the engine's options otherwise) and
+// keeps the fastest in element-MCMC iterations per second.
+extern "C" pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, int32_t precision, int32_t sample,
+                                 int32_t states, pa_tuning *out)
+{
+    if (!e || !ids || n <= 0) return fail(e, PA_E_INPUT, "null argument");
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    if (sample <= 0) sample = 2048;
+    if (states <= 1) states = 16;
+    // stratified sample by pattern count
+    std::vector<int32_t> order(ids, ids + n);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e->elems[a].P < e->elems[b].P; });
+    std::vector<int32_t> smp;
+    const int k = std::min(sample, n);
+    for (int i = 0; i < k; i++) smp.push_back(order[(size_t)((i + 0.5) * n / k)]);
+    std::vector<int32_t> sorted = schedule_ids(e, smp);
+    const ModelDev saved = e->M;
+    e->M.T = states;
+    e->M.burnin = states / 4;
+    e->M.mcmc = states - states / 4;
+    const pa_tuning keep = e->tune;
+    pa_tuning best = keep;
+    best.rate = -1;
+    const int smalls[2] = {64, 128}, longs[2] = {128, 256}, ths[3] = {128, 256, 512};
+    for (int a = 0; a < 2 && st == PA_OK; a++)
+        for (int b = 0; b < 2 && st == PA_OK; b++)
+            for (int c = 0; c < 3 && st == PA_OK; c++) {
+                if (longs[b] < smalls[a]) continue;
+                e->tune = {smalls[a], longs[b], ths[c], 0.0};
+                const int N = e->tm.N, m = (int)sorted.size();
+                DevBuf bufs[8];
+                pa_run_stats rs;
+                ChainArgs A;
+                memset(&A, 0, sizeof A);
+                if (hipMalloc(&bufs[3].p, (size_t)m * 3 * 8 * 8) != hipSuccess ||
+                    hipMalloc(&bufs[4].p, (size_t)m * 3 * 4 * 4) != hipSuccess ||
+                    hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) { st = fail(e, PA_E_OOM, "autotune buffers"); break; }
+                A.res_d = (double *)bufs[3].p;
+                A.res_i = (int32_t *)bufs[4].p;
+                A.maxZ = (int8_t *)bufs[5].p;
+                st = run_kernel(e, sorted.data(), m, precision == PA_FP64, 0, PA_RUN_BATCHED, 0, A, bufs, &rs);
+                if (st != PA_OK) break;
+                const double rate = (double)m * 3 * (states - 1) / (rs.elapsed_ms / 1e3);
+                if (rate > best.rate) { best = e->tune; best.rate = rate; }
+            }
+    e->M = saved;
+    e->tune = st == PA_OK ? best : keep;
+    if (out) *out = e->tune;
+    return st;
 }
 
 // ------------------------------------------------------------ serving loop

@@ -29,7 +29,7 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count",
            "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths",
            "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
-           "pa_serve_destroy", "pa_serve_clock"]
+           "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune"]
 
 
 class PaError(RuntimeError):
@@ -92,6 +92,11 @@ class RunConfig(C.Structure):
                 ("keep_traces", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("small_threads", C.c_int32), ("long_threads", C.c_int32), ("long_patterns", C.c_int32),
+                ("rate", C.c_double)]
+
+
 class ServeConfig(C.Structure):
     _fields_ = [("precision", C.c_int32), ("block_threads", C.c_int32), ("sweep_budget", C.c_int32),
                 ("max_chains", C.c_int32), ("hbm_budget_bytes", C.c_double)]
@@ -151,6 +156,12 @@ def lib():
         L.pa_loglik_lengths.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), C.c_int32,
                                         P(C.c_double)]
         L.pa_loglik_lengths.restype = C.c_int
+        L.pa_set_tuning.argtypes = [C.c_void_p, P(Tuning)]
+        L.pa_set_tuning.restype = C.c_int
+        L.pa_get_tuning.argtypes = [C.c_void_p, P(Tuning)]
+        L.pa_get_tuning.restype = C.c_int
+        L.pa_autotune.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(Tuning)]
+        L.pa_autotune.restype = C.c_int
         L.pa_serve_create.argtypes = [C.c_void_p, P(ServeConfig), P(C.c_void_p)]
         L.pa_serve_create.restype = C.c_int
         L.pa_serve_submit.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), P(C.c_int64)]
@@ -314,6 +325,22 @@ class Engine:
         d.update(children=ch.reshape(N, 2), parent=par, dist=dist, missing=mis, nodes=nodes[: info.n_nodes],
                  codes=codes.reshape(info.GG, S), site_pattern=site, weights=w[: info.P])
         return d
+
+    def autotune(self, ids, precision=PA_FP32, sample=2048, states=16):
+        """Time the size-class candidates on a sample; the best is kept for run_chains."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        t = Tuning()
+        self._check(lib().pa_autotune(self._h, _p(ids, C.c_int32), len(ids), int(precision), int(sample),
+                                      int(states), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+    def set_tuning(self, small_threads=64, long_threads=256, long_patterns=256):
+        self._check(lib().pa_set_tuning(self._h, C.byref(Tuning(small_threads, long_threads, long_patterns, 0.0))))
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().pa_get_tuning(self._h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
     def loglik(self, ids, Z, rates, precision=PA_FP64):
         ids = np.ascontiguousarray(ids, np.int32)
