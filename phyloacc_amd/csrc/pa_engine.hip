@@ -1195,12 +1195,13 @@ __global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs Ak)
     __syncthreads();
     if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
     __syncthreads();
+    const int n_work = A->n_work_dev ? *A->n_work_dev : A->n_work;
     for (;;) {
         if (threadIdx.x == 0) c->work = atomicAdd(A->queue, 1);
         __syncthreads();
         const int w = c->work;
         __syncthreads();
-        if (w >= A->n_work) break;
+        if (w >= n_work) break;
         const int rec = A->rec ? A->rec[w] : w;
         if (A->state && A->state[rec].ms >= 3) continue;
         const unsigned long long b0 = A->state && A->state[rec].m >= 0 ? A->state[rec].bytes : 0ull;
@@ -1255,26 +1256,46 @@ __global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs Ak)
 
 __global__ void reset_queue(int *q) { *q = 0; }
 
+__global__ void serve_admit(const EpochDesc *d, const int32_t *fresh, const int64_t *fresh_off, ChainState *state,
+                            int32_t *done, int64_t *slot_off, int *queue)
+{
+    const int nf = d->n_fresh;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        const int r = fresh[i];
+        state[r].ms = 0;
+        state[r].m = -1;
+        done[r] = 0;
+        slot_off[r] = fresh_off[i];
+    }
+    if (threadIdx.x == 0) *queue = 0;
+}
+
 template <typename R, int BS>
 static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, int kind, hipStream_t st)
 {
+    // dynamic LDS above 64 KB needs the per-kernel limit raised (large trees
+    // only); the call is never made while a stream is being captured otherwise
     const size_t sm = smem_arrays<R, false>(M.N, M.S, BS, nullptr, nullptr);
-    hipError_t err;
-    if (kind == 0) {
-        err = hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    const void *fn = kind == 0 ? (const void *)chain_kernel<R, BS> : (const void *)loglik_kernel<R, BS>;
+    if (sm > 65536) {
+        const hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (err != hipSuccess) return err;
-        hipLaunchKernelGGL((chain_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
-    } else {
-        err = hipFuncSetAttribute((const void *)loglik_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL((loglik_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
     }
+    if (kind == 0) hipLaunchKernelGGL((chain_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
+    else hipLaunchKernelGGL((loglik_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
     return hipGetLastError();
 }
 
 hipError_t launch_reset(int *q, hipStream_t st)
 {
     hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_admit(const EpochDesc *d, const int32_t *fresh, const int64_t *fresh_off, ChainState *state,
+                        int32_t *done, int64_t *slot_off, int *queue, hipStream_t st)
+{
+    hipLaunchKernelGGL(serve_admit, dim3(1), dim3(256), 0, st, d, fresh, fresh_off, state, done, slot_off, queue);
     return hipGetLastError();
 }
 

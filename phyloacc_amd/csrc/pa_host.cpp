@@ -515,6 +515,7 @@ struct pa_engine {
     int n_cu = 0;
     size_t lds_per_cu = 0;
     pa_tuning tune = {64, 256, 256, 0.0};   // size classes of batched chain launches
+    uint64_t arena_gen = 0;                  // bumped whenever the device arenas move
 };
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
@@ -810,6 +811,7 @@ static pa_status upload(pa_engine *e)
     HIPCHK(e, hipMemcpy(e->d_site, site.data(), site.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(e->d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
     e->dev_dirty = false;
+    e->arena_gen++;
     return PA_OK;
 }
 
@@ -1435,13 +1437,20 @@ struct pa_server {
     char *d_pool = nullptr;
     Pool pool;
     ChainState *d_state = nullptr;
-    int32_t *d_done = nullptr, *d_list = nullptr;   // done flags [max]; work ids + records [2*max]
-    int64_t *d_slot = nullptr;
+    int32_t *d_done = nullptr;                      // done flags [max]
+    int64_t *d_slot = nullptr;                      // scratch offset of each record [max]
     double *d_res_d = nullptr;
     int32_t *d_res_i = nullptr;
     int8_t *d_maxZ = nullptr;
-    int32_t *h_list = nullptr, *h_done = nullptr;  // pinned staging
-    int64_t *h_slot = nullptr;
+    // per-step upload, one copy: EpochDesc | work ids [max] | records [max] |
+    // fresh records [max] | fresh scratch offsets [max] (int64)
+    char *d_pack = nullptr, *h_pack = nullptr;
+    size_t pack_bytes = 0, off_work = 0, off_rec = 0, off_fresh = 0, off_foff = 0;
+    int32_t *h_done = nullptr;                      // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t graph_gen = ~0ull;
+    int grid = 0;
     std::vector<int32_t> free_recs;
     std::vector<Request> rec_req;                 // by record
     std::vector<int32_t> active;                  // records in flight
@@ -1460,9 +1469,11 @@ extern "C" double pa_serve_clock(void)
 
 static void serve_free(pa_server *s)
 {
-    (void)hipFree(s->d_pool); (void)hipFree(s->d_state); (void)hipFree(s->d_done); (void)hipFree(s->d_list);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    (void)hipFree(s->d_pool); (void)hipFree(s->d_state); (void)hipFree(s->d_done); (void)hipFree(s->d_pack);
     (void)hipFree(s->d_slot); (void)hipFree(s->d_res_d); (void)hipFree(s->d_res_i); (void)hipFree(s->d_maxZ);
-    (void)hipHostFree(s->h_list); (void)hipHostFree(s->h_done); (void)hipHostFree(s->h_slot);
+    (void)hipHostFree(s->h_pack); (void)hipHostFree(s->h_done);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
 }
@@ -1489,14 +1500,25 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     HIPCHK(e, hipMalloc(&s->d_pool, s->pool_bytes));
     HIPCHK(e, hipMalloc(&s->d_state, sizeof(ChainState) * M));
     HIPCHK(e, hipMalloc(&s->d_done, sizeof(int32_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_list, sizeof(int32_t) * 2 * M));
+    s->off_work = sizeof(EpochDesc);
+    s->off_rec = s->off_work + sizeof(int32_t) * M;
+    s->off_fresh = s->off_rec + sizeof(int32_t) * M;
+    s->off_foff = (s->off_fresh + sizeof(int32_t) * M + 7) & ~(size_t)7;
+    s->pack_bytes = s->off_foff + sizeof(int64_t) * M;
+    HIPCHK(e, hipMalloc(&s->d_pack, s->pack_bytes));
     HIPCHK(e, hipMalloc(&s->d_slot, sizeof(int64_t) * M));
     HIPCHK(e, hipMalloc(&s->d_res_d, sizeof(double) * 24 * M));
     HIPCHK(e, hipMalloc(&s->d_res_i, sizeof(int32_t) * 12 * M));
     HIPCHK(e, hipMalloc(&s->d_maxZ, (size_t)3 * N * M));
-    HIPCHK(e, hipHostMalloc(&s->h_list, sizeof(int32_t) * 2 * M));
+    HIPCHK(e, hipHostMalloc(&s->h_pack, s->pack_bytes));
     HIPCHK(e, hipHostMalloc(&s->h_done, sizeof(int32_t) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_slot, sizeof(int64_t) * M));
+    memset(s->h_done, 0, sizeof(int32_t) * M);
+    {
+        const size_t smem = chain_smem_bytes(N, e->tm.S, s->BS, s->fp64);
+        int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
+        per_cu = std::max(1, std::min(per_cu, 2048 / s->BS));
+        s->grid = std::min(M, e->n_cu * per_cu);
+    }
     HIPCHK(e, hipEventCreate(&s->ev0));
     HIPCHK(e, hipEventCreate(&s->ev1));
     s->pool.init(s->pool_bytes);
@@ -1575,6 +1597,49 @@ static pa_status serve_retire(pa_server *s, int32_t *n_finished)
     return PA_OK;
 }
 
+// Records the step's work as one graph: the packed upload, the admission
+// kernel (fresh records + work counter), the chain kernel over a fixed
+// grid (it reads the number of chains from the upload) and the copy of the
+// done flags back.  Re-recorded only when the device arenas move.
+static pa_status serve_capture(pa_server *s)
+{
+    pa_engine *e = s->e;
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    const EpochDesc *desc = (const EpochDesc *)s->d_pack;
+    ChainArgs A;
+    memset(&A, 0, sizeof A);
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+    A.queue = e->d_queue;
+    A.work = (const int32_t *)(s->d_pack + s->off_work);
+    A.rec = (const int32_t *)(s->d_pack + s->off_rec);
+    A.n_work_dev = &desc->n_work;
+    A.scratch = s->d_pool;
+    A.slot_off = s->d_slot;
+    A.state = s->d_state;
+    A.done = s->d_done;
+    A.sweep_budget = s->budget;
+    A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
+    HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t r = hipMemcpyAsync(s->d_pack, s->h_pack, s->pack_bytes, hipMemcpyHostToDevice, e->stream);
+    if (r == hipSuccess)
+        r = launch_admit(desc, (const int32_t *)(s->d_pack + s->off_fresh), (const int64_t *)(s->d_pack + s->off_foff),
+                         s->d_state, s->d_done, s->d_slot, e->d_queue, e->stream);
+    if (r == hipSuccess) r = launch_kernel(e->M, A, s->fp64, s->BS, s->grid, 0, e->stream);
+    if (r == hipSuccess)
+        r = hipMemcpyAsync(s->h_done, s->d_done, sizeof(int32_t) * s->max_chains, hipMemcpyDeviceToHost, e->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t r2 = hipStreamEndCapture(e->stream, &g);
+    if (r != hipSuccess || r2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(e, PA_E_DEVICE, std::string("serving graph capture: ") + hipGetErrorString(r != hipSuccess ? r : r2));
+    }
+    s->graph = g;
+    HIPCHK(e, hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    s->graph_gen = e->arena_gen;
+    return PA_OK;
+}
+
 // Admits waiting requests (FIFO) and launches the next sweep batch.
 static pa_status serve_launch(pa_server *s)
 {
@@ -1598,14 +1663,6 @@ static pa_status serve_launch(pa_server *s)
     if (s->active.empty()) return PA_OK;
     pa_status st = upload(e);
     if (st != PA_OK) return st;
-    // new chains start from initMCMC: header {ms = 0, m = -1}, done = 0
-    static const int32_t hdr[2] = {0, -1};
-    for (int r : fresh) {
-        HIPCHK(e, hipMemcpyAsync(&s->d_state[r], hdr, sizeof hdr, hipMemcpyHostToDevice, e->stream));
-        s->h_slot[r] = (int64_t)s->rec_req[r].off;
-        HIPCHK(e, hipMemcpyAsync(s->d_slot + r, s->h_slot + r, 8, hipMemcpyHostToDevice, e->stream));
-    }
-    if (!fresh.empty()) HIPCHK(e, hipMemsetAsync(s->d_done, 0, sizeof(int32_t) * s->max_chains, e->stream));
     // largest elements first
     std::sort(s->active.begin(), s->active.end(), [&](int a, int b) {
         const HostElem &x = e->elems[s->rec_req[a].eid], &y = e->elems[s->rec_req[b].eid];
@@ -1613,34 +1670,26 @@ static pa_status serve_launch(pa_server *s)
         return a < b;
     });
     const int n = (int)s->active.size();
+    EpochDesc *d = (EpochDesc *)s->h_pack;
+    d->n_work = n;
+    d->n_fresh = (int)fresh.size();
+    int32_t *work = (int32_t *)(s->h_pack + s->off_work), *rec = (int32_t *)(s->h_pack + s->off_rec);
+    int32_t *fr = (int32_t *)(s->h_pack + s->off_fresh);
+    int64_t *fo = (int64_t *)(s->h_pack + s->off_foff);
     for (int i = 0; i < n; i++) {
-        s->h_list[i] = s->rec_req[s->active[i]].eid;
-        s->h_list[s->max_chains + i] = s->active[i];
+        work[i] = s->rec_req[s->active[i]].eid;
+        rec[i] = s->active[i];
     }
-    HIPCHK(e, hipMemcpyAsync(s->d_list, s->h_list, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipMemcpyAsync(s->d_list + s->max_chains, s->h_list + s->max_chains, sizeof(int32_t) * n,
-                             hipMemcpyHostToDevice, e->stream));
-    ChainArgs A;
-    memset(&A, 0, sizeof A);
-    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
-    A.queue = e->d_queue;
-    A.work = s->d_list;
-    A.rec = s->d_list + s->max_chains;
-    A.n_work = n;
-    A.scratch = s->d_pool;
-    A.slot_off = s->d_slot;
-    A.state = s->d_state;
-    A.done = s->d_done;
-    A.sweep_budget = s->budget;
-    A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
-    const size_t smem = chain_smem_bytes(e->tm.N, e->tm.S, s->BS, s->fp64);
-    int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
-    per_cu = std::max(1, std::min(per_cu, 2048 / s->BS));
-    const int grid = std::min(n, e->n_cu * per_cu);
+    for (size_t i = 0; i < fresh.size(); i++) {
+        fr[i] = fresh[i];
+        fo[i] = (int64_t)s->rec_req[fresh[i]].off;
+        s->h_done[fresh[i]] = 0;
+    }
+    if (!s->gexec || s->graph_gen != e->arena_gen) {
+        if ((st = serve_capture(s)) != PA_OK) return st;
+    }
     HIPCHK(e, hipEventRecord(s->ev0, e->stream));
-    HIPCHK(e, launch_reset(e->d_queue, e->stream));
-    HIPCHK(e, launch_kernel(e->M, A, s->fp64, s->BS, grid, 0, e->stream));
-    HIPCHK(e, hipMemcpyAsync(s->h_done, s->d_done, sizeof(int32_t) * s->max_chains, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipGraphLaunch(s->gexec, e->stream));
     HIPCHK(e, hipEventRecord(s->ev1, e->stream));
     s->in_flight = true;
     s->st.launches++;

@@ -41,6 +41,7 @@ struct ChainArgs {
     ChainState *state;        // [records] resumable chains, or null (run to the end)
     int32_t *done;            // [records] set to 1 when all three models finished (resumable runs)
     int sweep_budget;         // sweeps per work item in this launch (resumable runs)
+    const int *n_work_dev;    // when set, the number of work items is read here at launch
     // outputs, indexed by record
     double *res_d;            // [n_work][3][8]
     int32_t *res_i;           // [n_work][3][4]
@@ -72,5 +73,16 @@ __host__ __device__ inline size_t slot_layout(int N, int S, int Ppad, int fp64, 
 }
 hipError_t launch_kernel(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, int kind, hipStream_t st);
 hipError_t launch_reset(int *q, hipStream_t st);
+
+// Serving-loop launch descriptor, written by the host before each launch.
+struct EpochDesc {
+    int n_work;               // chains to advance
+    int n_fresh;              // newly admitted records to initialise first
+    int pad[2];
+};
+// Initialises the fresh records (header {ms 0, m -1}, done 0, slot offset)
+// and resets the work counter.
+hipError_t launch_admit(const EpochDesc *d, const int32_t *fresh, const int64_t *fresh_off, ChainState *state,
+                        int32_t *done, int64_t *slot_off, int *queue, hipStream_t st);
 
 }  // namespace pa

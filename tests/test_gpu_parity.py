@@ -122,3 +122,42 @@ def test_chain_fp64_two_size_classes(require_gpu):
             np.testing.assert_array_equal(res["Z"][k, ms], ref[ms]["Z"])
             np.testing.assert_allclose(res["loglik"][k, ms], ref[ms]["loglik"], rtol=1e-9)
             assert res["max_m"][k, ms] == ref[ms]["max_m"]
+
+
+def _with_ambiguity(w, seed):
+    """Sprinkle IUPAC two-base codes and N into the alignment."""
+    rng = np.random.default_rng(seed)
+    al = w.align.copy()
+    mask = rng.random(al.shape) < 0.03
+    al[mask] = rng.choice(np.frombuffer(b"ryksmwn", np.uint8), size=int(mask.sum()))
+    w.align = al
+    return w
+
+
+@pytest.mark.parametrize("variant", [
+    dict(hyper_lrate2_a=0.0),                      # Dollo: no 1 -> 2 return after loss
+    dict(ropt=2),
+    dict(ropt=0),
+    dict(r1_shared_uniform=0),
+    dict(prune_tree=1, constomis=0.05),
+    dict(trim_gap_percent=0.3, init_crate=0.3, init_nrate=2.0),
+])
+def test_chain_fp64_option_variants(require_gpu, variant):
+    kw = dict(burnin=12, mcmc=36, adapt_freq=10, **variant)
+    w = _with_missing(1, 5, 41)
+    w = _with_ambiguity(w, 42)
+    eng = E.Engine.from_workload(w, E.default_options(**kw))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    ids = ids[st == E.PA_ELEM_OK]
+    assert len(ids) > 0
+    m = O.model_from_workload(w)
+    oo = O.default_options(**kw)
+    res = eng.run_chains(ids, precision=E.PA_FP64, traces=True)
+    for k, i in enumerate(ids):
+        r = O.Element(m, oo, w.align, w.starts[i], w.sizes[i]).run(oo)
+        for ms in range(3):
+            np.testing.assert_array_equal(res["Z"][k, ms], r[ms]["Z"])
+            np.testing.assert_allclose(res["loglik"][k, ms], r[ms]["loglik"], rtol=1e-9)
+            np.testing.assert_allclose(res["nrate"][k, ms], r[ms]["nrate"], rtol=1e-9)
+            np.testing.assert_allclose(res["crate"][k, ms], r[ms]["crate"], rtol=1e-9)
+            np.testing.assert_allclose(res["l2rate"][k, ms], r[ms]["l2rate"], rtol=1e-9)

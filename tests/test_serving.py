@@ -99,3 +99,32 @@ def test_hbm_budget_limits_chains_in_flight(require_gpu):
     for k, t in enumerate(tickets):
         assert got[t].result[1].max_loglik == ref["max_loglik"][k, 1]
     srv.close()
+
+
+def test_elements_added_while_serving(require_gpu):
+    """Elements encoded after the server started move the device arenas; the
+    recorded step is re-recorded and chains in flight carry on unchanged."""
+    w = synth.make_workload(2, seed=12, n_elements=14)
+    eng = E.Engine.from_workload(w, E.default_options(burnin=20, mcmc=40, adapt_freq=15))
+    ids_a, st_a = eng.add_elements(w.align, w.starts[:8], w.sizes[:8])
+    ref_eng = E.Engine.from_workload(w, E.default_options(burnin=20, mcmc=40, adapt_freq=15))
+    ref_ids, ref_st = ref_eng.add_elements(w.align, w.starts, w.sizes)
+    ref = ref_eng.run_chains(ref_ids[ref_st == E.PA_ELEM_OK], precision=E.PA_FP32, block_threads=128)
+    ok_ref = list(ref_ids[ref_st == E.PA_ELEM_OK])
+    srv = E.Server(eng, precision=E.PA_FP32, block_threads=128, sweep_budget=9)
+    t_a = list(srv.submit(ids_a[st_a == E.PA_ELEM_OK]))
+    got = {}
+    for _ in range(2):
+        srv.step(wait=True)
+        got.update({r.ticket: r for r in srv.poll()})
+    ids_b, st_b = eng.add_elements(w.align, w.starts[8:], w.sizes[8:])
+    t_b = list(srv.submit(ids_b[st_b == E.PA_ELEM_OK]))
+    tick = t_a + t_b
+    got.update(_drain(srv, len(tick) - len(got)))
+    for t in tick:
+        r = got[t]
+        k = ok_ref.index(r.element)       # engine ids follow workload order in both engines
+        for ms in range(3):
+            assert r.result[ms].max_loglik == ref["max_loglik"][k, ms]
+            assert r.result[ms].max_m == ref["max_m"][k, ms]
+    srv.close()
