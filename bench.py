@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--cpu-burnin", type=int, default=50)
     ap.add_argument("--cpu-mcmc", type=int, default=550)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--copy-probe", action="store_true", help="run a plain copy kernel of known size and exit")
     ap.add_argument("--no-tune", action="store_true", help="keep the default size classes")
     ap.add_argument("--tune-sample", type=int, default=2048)
     ap.add_argument("--no-agree", action="store_true", help="skip the FP32-vs-FP64 agreement sample")
@@ -75,11 +76,16 @@ def main():
                           "processes and are not part of this build", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.copy_probe:
+        return copy_probe(args)
+
     from phyloacc_amd import engine as E
     from phyloacc_amd import synth
 
     if args.config == 4:
         return serve(args, E, synth)
+    if args.config == 5:
+        return gene_trees(args, E, synth)
 
     t0 = time.time()
     w = synth.make_workload(args.config, seed=args.seed, n_elements=args.elements)
@@ -220,6 +226,75 @@ def fp32_vs_fp64(E, eng, keep, w, args, rng):
             "max_rel_err_loglik_fixed": float(np.max(np.abs(l32 - l64) / np.abs(l64))),
             "identical_Z_trajectories": round(float(same_traj.mean()), 4),
             "identical_Z_posterior_modes": round(float(same_mode.mean()), 4)}
+
+
+def copy_probe(args):
+    """A device-to-device copy of known size (a plain copy kernel), for
+    scaling the HBM byte counters of a profiler run: FETCH_SIZE and
+    WRITE_SIZE should each read this many bytes per dispatch."""
+    import torch
+    nbytes = 512 * 2 ** 20
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(max(1, args.steps)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_dispatch": nbytes, "dispatches": max(1, args.steps)}))
+
+
+def gene_trees(args, E, synth):
+    """Uneven-tree batched pruning: every element carries its own
+    multispecies-coalescent gene tree (120 species).  A step evaluates the
+    log-likelihood of every element at its gene-branch lengths in one launch.
+    Reports element log-likelihood evaluations per second, and the FP32 and
+    FP64 errors against the oracle's gene-tree pruning on a sample."""
+    from oracle import oracle_py as O
+    t0 = time.time()
+    n = args.elements or 20000
+    w = synth.make_gt_workload(seed=args.seed, n_elements=n)
+    log(f"gene-tree workload: {w.C} elements, S={w.tree.S}, {int(w.sizes.sum())} sites ({time.time() - t0:.1f}s)")
+    opts = E.default_options()
+    eng = E.Engine.from_workload(w, opts)
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    keep = st == E.PA_ELEM_OK
+    ids = ids[keep]
+    L = w.lengths()[keep]
+    prec = E.PA_FP32 if args.precision == "fp32" else E.PA_FP64
+    for _ in range(args.warmup):
+        eng.loglik_lengths(ids, L, prec)
+    times = []
+    for _ in range(args.steps):
+        tw = time.time()
+        eng.loglik_lengths(ids, L, prec)
+        times.append(time.time() - tw)
+    step = float(np.mean(times))
+    rng = np.random.default_rng(0)
+    smp = rng.choice(len(ids), size=min(48, len(ids)), replace=False)
+    m = O.parse_mod(w.mod_text)[0]
+    full = np.flatnonzero(keep)
+    ref = np.array([O.gt_loglik(m, w.tree.S, w.genes[full[i]].children, w.genes[full[i]].seg_gene,
+                                w.genes[full[i]].seg_species, w.genes[full[i]].seg_len * w.scale, w.Z[full[i]],
+                                w.r1[full[i]], w.r2[full[i]], w.align, w.starts[full[i]], w.sizes[full[i]],
+                                gap_prop=opts.gap_prop) for i in smp])
+    l32 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP32)
+    l64 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP64)
+    out = {
+        "metric": "gene-tree element log-likelihood evaluations/s (uneven per-element trees, batched)",
+        "value": round(len(ids) / step, 1), "unit": "element log-likelihoods/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 partials / fp64 sums" if prec == E.PA_FP32 else "fp64",
+        "data": "synthetic (seeded multispecies-coalescent gene trees)",
+        "config": {"workload": f"config5: S={w.tree.S} species tree in coalescent units, {len(ids)} elements with "
+                   "their own gene trees, lengths LogNormal(ln 300, 0.6)", "elements": int(len(ids)),
+                   "sites": int(w.sizes[keep].sum()), "parallelism": "1 GPU",
+                   "timing": "wall clock around the call, host-to-device transfer of the lengths included"},
+        "vs_oracle": {"sample": int(len(smp)),
+                      "fp64_max_rel_err": float(np.max(np.abs(l64 - ref) / np.abs(ref))),
+                      "fp32_max_rel_err": float(np.max(np.abs(l32 - ref) / np.abs(ref)))},
+    }
+    print(json.dumps(out))
 
 
 def serve(args, E, synth):

@@ -499,13 +499,15 @@ def simulate_gene_trees(children, lengths, pi, Q, sizes, gap_frac, rng, S):
     for q in range(2):
         np.put_along_axis(parent, children[:, S:, q].astype(np.int64), np.arange(S, N)[None, :].repeat(C, 0), 1)
     states = np.empty((N, G), np.uint8)
+    flat = states.reshape(-1)
     cpi = np.cumsum(pi)
     states[N - 1] = np.searchsorted(cpi, rng.random(G) * cpi[-1], side="right").clip(0, 3)
-    cols = np.arange(G)
+    cols = np.arange(G, dtype=np.int64)
     for v in range(N - 2, -1, -1):
         P = trans_batch(lam, V, Vi, lengths[:, v])
         cum = np.cumsum(P, axis=2).reshape(-1)
-        par = states[parent[elem, v], cols]
+        # each element's sites read one parent row: contiguous runs of the gather
+        par = np.take(flat, np.repeat(parent[:, v] * G, sizes) + cols)
         base = elem16 + par.astype(np.int64) * 4
         u = rng.random(G) * np.take(cum, base + 3)
         st = (u >= np.take(cum, base)).astype(np.uint8)
