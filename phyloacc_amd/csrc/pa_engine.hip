@@ -772,18 +772,31 @@ __device__ __noinline__ void rate_propose(MP M, CP<R> c, int ms, int m, int resZ
     if (neut) c->nrate = cur; else c->crate = cur;
 }
 
-// Metropolis decision (bpp_c.cpp:1046-1094), FP64.
+// The four density terms of the Metropolis ratio (bpp_c.cpp:1046-1047), one
+// per lane k = 0..3, into red[k]: prior at the proposal, prior at the
+// current value, then the proposal's Hastings terms (r2: reverse and forward
+// Gamma proposal densities; r1: log current, log proposal).
 template <typename R>
-__device__ __noinline__ void rate_decide(MP M, CP<R> c, bool neut, double old, double lnew)
+__device__ __noinline__ void decide_term(MP M, CP<R> c, bool neut, double old, int k)
 {
     const double pa = neut ? M->nprior_a : M->cprior_a, pb = neut ? M->nprior_b : M->cprior_b;
     const double prop = c->proposal;
-    double MH;
-    if (neut)
-        MH = lnew - c->loglik + log_gamma_pdf(prop, pa, pb) - log_gamma_pdf(old, pa, pb) +
-             log_gamma_pdf(old, prop / c->prop_n, c->prop_n) - log_gamma_pdf(prop, old / c->prop_n, c->prop_n);
-    else
-        MH = lnew - c->loglik + log_gamma_pdf(prop, pa, pb) - log_gamma_pdf(old, pa, pb) + log(old) - log(prop);
+    double v;
+    switch (k) {
+    case 0: v = log_gamma_pdf(prop, pa, pb); break;
+    case 1: v = log_gamma_pdf(old, pa, pb); break;
+    case 2: v = neut ? log_gamma_pdf(old, prop / c->prop_n, c->prop_n) : c_log(old); break;
+    default: v = neut ? log_gamma_pdf(prop, old / c->prop_n, c->prop_n) : c_log(prop); break;
+    }
+    c->red[k] = v;
+}
+
+// Metropolis decision (bpp_c.cpp:1046-1094), FP64, from the terms in red[0..3].
+template <typename R>
+__device__ __noinline__ void rate_decide(MP M, CP<R> c, bool neut, double old, double lnew)
+{
+    const double prop = c->proposal;
+    const double MH = lnew - c->loglik + c->red[0] - c->red[1] + c->red[2] - c->red[3];
     if (log(c->r) < MH) {
         c->accept = 1;
         c->loglik = lnew;
@@ -864,6 +877,8 @@ __device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int re
         build_prop_tables<R, BS>(M, c, cls, c->proposal);
         __syncthreads();
         const double lnew = prune<R, BS>(M, c, PR_PROP, cls);
+        if (tid < 4) decide_term<R>(M, c, neut, old, tid);
+        __syncthreads();
         if (tid == 0) rate_decide<R>(M, c, neut, old, lnew);
         __syncthreads();
         if (c->accept) {
@@ -887,21 +902,28 @@ __device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int re
     __syncthreads();
 }
 
-// Beta draws of BPP_C::sample_transition (src/PhyloAcc-ST/bpp_c.cpp:522-613)
+// Beta draws of BPP_C::sample_transition (src/PhyloAcc-ST/bpp_c.cpp:522-613).
+// Each Beta(a, b) is X / (X + Y) with X ~ Gamma(a), Y ~ Gamma(b) on their own
+// sub-streams (g: 0/1, l: 2/3, l2: 4/5), so the six Gamma draws are made by
+// six lanes at once; lane k leaves its draw in red[k].
 template <typename R>
-__device__ __noinline__ void transition_draw(MP M, CP<R> c, int ms, int m)
+__device__ __noinline__ void transition_gamma(MP M, CP<R> c, int ms, int m, int k)
 {
     // counts n(child, parent) in cnt9[child * 3 + parent]; the fixZ-free
-    // counts subtract cnt9[9 + ...].  Each draw reads its counts from LDS right
-    // before it, so nothing but the results stays live across the draws.
-    const Key key{M->key0, M->key1};
-    const uint32_t c3 = tagword(ms, TAG_TRANS), c2 = (uint32_t)m;
+    // counts subtract cnt9[9 + ...]
+    auto n = [&](int i) { return (double)c->cnt9[i]; };
     auto nf = [&](int i) { return (double)c->cnt9[i] - (double)c->cnt9[9 + i]; };
-    c->g = beta_draw(key, 0, 1, c2, c3, M->ha_g + (double)c->cnt9[1 * 3 + 0] + (double)c->cnt9[2 * 3 + 0],
-                     M->hb_g + (double)c->cnt9[0]);
-    c->l = beta_draw(key, 2, 3, c2, c3, M->ha_l + nf(2 * 3 + 1), M->hb_l + nf(1 * 3 + 1));
-    c->l2 = M->ha_l2 == 0 ? 0.0 : beta_draw(key, 4, 5, c2, c3, M->ha_l2 + nf(1 * 3 + 2), M->hb_l2 + nf(2 * 3 + 2));
-    set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
+    double shape;
+    switch (k) {
+    case 0: shape = M->ha_g + n(1 * 3 + 0) + n(2 * 3 + 0); break;
+    case 1: shape = M->hb_g + n(0); break;
+    case 2: shape = M->ha_l + nf(2 * 3 + 1); break;
+    case 3: shape = M->hb_l + nf(1 * 3 + 1); break;
+    case 4: shape = M->ha_l2 + nf(1 * 3 + 2); break;
+    default: shape = M->hb_l2 + nf(2 * 3 + 2); break;
+    }
+    Stream st{Key{M->key0, M->key1}, (uint32_t)k, (uint32_t)m, tagword(ms, TAG_TRANS), 0};
+    c->red[k] = gamma_draw(st, shape, 1.0);
 }
 
 template <typename R, int BS>
@@ -917,7 +939,14 @@ __device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
         if (c->fixZ[s]) lds_add(&c->cnt9[9 + zi * 3 + zp], 1u);
     }
     __syncthreads();
-    if (tid == 0) transition_draw<R>(M, c, ms, m);
+    if (tid < (M->ha_l2 == 0 ? 4 : 6)) transition_gamma<R>(M, c, ms, m, tid);
+    __syncthreads();
+    if (tid == 0) {
+        c->g = c->red[0] / (c->red[0] + c->red[1]);
+        c->l = c->red[2] / (c->red[2] + c->red[3]);
+        c->l2 = M->ha_l2 == 0 ? 0.0 : c->red[4] / (c->red[4] + c->red[5]);
+        set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
+    }
     __syncthreads();
 }
 
@@ -1022,10 +1051,18 @@ __device__ __noinline__ void init_model(MP M, CP<R> c, int resZ)
 
 // BPP_C::MonitorChain (src/PhyloAcc-ST/bpp_c.cpp:741-834), thread 0
 template <typename R>
+__device__ __noinline__ void monitor_term(MP M, CP<R> c, int k)
+{
+    // prior densities of the current rates (lane 0: r1, lane 1: r2) into red[8 + k]
+    c->red[8 + k] = k == 0 ? log_gamma_pdf(c->crate, M->cprior_a, M->cprior_b)
+                           : log_gamma_pdf(c->nrate, M->nprior_a, M->nprior_b);
+}
+
+template <typename R>
 __device__ __noinline__ void monitor(MP M, CP<R> c, int m, int resZ, double *trd, int T)
 {
-    double full = c->loglik + log_gamma_pdf(c->crate, M->cprior_a, M->cprior_b);
-    if (resZ != 0) full += log_gamma_pdf(c->nrate, M->nprior_a, M->nprior_b);
+    double full = c->loglik + c->red[8];
+    if (resZ != 0) full += c->red[9];
     if (trd) {
         trd[0 * T + m] = c->loglik; trd[1 * T + m] = full; trd[2 * T + m] = c->nrate;
         trd[3 * T + m] = c->crate; trd[4 * T + m] = c->g; trd[5 * T + m] = c->l; trd[6 * T + m] = c->l2;
@@ -1119,6 +1156,8 @@ __device__ __forceinline__ bool run_model(MP M, CP<R> c, AP A, int ms, int rec, 
         }
         if (trz)
             for (int i = tid; i < N; i += BS) trz[(size_t)m * N + i] = c->Z[i];
+        if (tid < (resZ != 0 ? 2 : 1)) monitor_term<R>(M, c, tid);
+        __syncthreads();
         if (tid == 0) monitor<R>(M, c, m, resZ, trd, T);
         if (m == T - 1) break;
         --*budget;

@@ -115,13 +115,6 @@ PA_COLD double gamma_draw(Stream &s, double a, double b)
     return d * b * boost;
 }
 
-PA_HD double beta_draw(Key k, uint32_t cx, uint32_t cy, uint32_t c2, uint32_t c3, double a, double b)
-{
-    Stream sx{k, cx, c2, c3, 0}, sy{k, cy, c2, c3, 0};
-    double x = gamma_draw(sx, a, 1.0), y = gamma_draw(sy, b, 1.0);
-    return x / (x + y);
-}
-
 // Inverse-CDF categorical over k non-negative weights; returns the last
 // positive-weight index when rounding puts the target past the end.
 template <int K>
