@@ -239,11 +239,16 @@ typedef struct {
     double hbm_budget_bytes;    /* scratch for in-flight chains; 0 = 4 GiB */
 } pa_serve_config;
 
+enum { PA_SERVE_CHAINS = 0, PA_SERVE_LOGLIK = 1 };
+
 typedef struct {
-    int64_t ticket;             /* from pa_serve_submit */
+    int64_t ticket;             /* from pa_serve_submit / pa_serve_submit_lengths */
     int32_t element;            /* element id */
     int32_t status;             /* PA_ELEM_OK / PA_ELEM_NUMERIC_FAIL */
+    int32_t kind;               /* PA_SERVE_CHAINS: result[]; PA_SERVE_LOGLIK: loglik */
+    int32_t pad;
     pa_chain_result result[3];  /* M0, M1, M2 */
+    double loglik;
     double arrival_s, admit_s, finish_s;   /* pa_serve_clock() times */
 } pa_serve_result;
 
@@ -257,6 +262,11 @@ pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **
 /* Queue n element ids (FIFO).  arrival_s: per-request arrival times on the
  * pa_serve_clock() scale, or NULL for "now".  tickets: [n] or NULL. */
 pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n, const double *arrival_s, int64_t *tickets);
+/* Log-likelihood requests for elements that carry their own tree (gene
+ * trees, pa_elements_add_trees) at per-request branch lengths [n*N]: each
+ * step prunes the waiting ones in one batched launch beside the chains. */
+pa_status pa_serve_submit_lengths(pa_server *s, const int32_t *ids, int32_t n, const double *lengths,
+                                  const double *arrival_s, int64_t *tickets);
 /* One turn of the request loop.  With wait = 0 it returns at once while the
  * current launch is still running (n_finished = 0). */
 pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finished);

@@ -1428,6 +1428,13 @@ struct Request {
     size_t off, bytes;
 };
 
+struct LRequest {
+    int64_t ticket;
+    int32_t eid;
+    double arrival;
+    std::vector<double> len;
+};
+
 }  // namespace
 
 struct pa_server {
@@ -1460,6 +1467,15 @@ struct pa_server {
     bool in_flight = false;
     int64_t next_ticket = 0;
     pa_serve_stats st{};
+    // log-likelihood requests (own-tree elements), one batched launch per step on stream2
+    std::deque<LRequest> lwait;
+    std::vector<LRequest> lbatch;                  // in flight, launch order
+    int32_t *d_lids = nullptr, *d_lZ = nullptr, *h_lids = nullptr;
+    double *d_lrates = nullptr, *d_llen = nullptr, *d_lout = nullptr, *h_llen = nullptr, *h_lout = nullptr;
+    char *d_lscratch = nullptr;
+    size_t lscratch_bytes = 0;
+    int lmax = 0;
+    hipEvent_t evl = nullptr;
 };
 
 extern "C" double pa_serve_clock(void)
@@ -1476,6 +1492,10 @@ static void serve_free(pa_server *s)
     (void)hipHostFree(s->h_pack); (void)hipHostFree(s->h_done);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->evl) (void)hipEventDestroy(s->evl);
+    (void)hipFree(s->d_lids); (void)hipFree(s->d_lZ); (void)hipFree(s->d_lrates); (void)hipFree(s->d_llen);
+    (void)hipFree(s->d_lout); (void)hipFree(s->d_lscratch);
+    (void)hipHostFree(s->h_lids); (void)hipHostFree(s->h_llen); (void)hipHostFree(s->h_lout);
 }
 
 extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **out)
@@ -1521,6 +1541,22 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     }
     HIPCHK(e, hipEventCreate(&s->ev0));
     HIPCHK(e, hipEventCreate(&s->ev1));
+    HIPCHK(e, hipEventCreate(&s->evl));
+    s->lmax = M;
+    HIPCHK(e, hipMalloc(&s->d_lids, sizeof(int32_t) * M));
+    HIPCHK(e, hipMalloc(&s->d_lZ, sizeof(int32_t) * (size_t)M * N));
+    HIPCHK(e, hipMalloc(&s->d_lrates, sizeof(double) * 2 * M));
+    HIPCHK(e, hipMalloc(&s->d_llen, sizeof(double) * (size_t)M * N));
+    HIPCHK(e, hipMalloc(&s->d_lout, sizeof(double) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_lids, sizeof(int32_t) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_llen, sizeof(double) * (size_t)M * N));
+    HIPCHK(e, hipHostMalloc(&s->h_lout, sizeof(double) * M));
+    {
+        // own-tree log-likelihoods: every branch in the rate-1 class
+        std::vector<double> ones((size_t)2 * M, 1.0);
+        HIPCHK(e, hipMemset(s->d_lZ, 0, sizeof(int32_t) * (size_t)M * N));
+        HIPCHK(e, hipMemcpy(s->d_lrates, ones.data(), ones.size() * 8, hipMemcpyHostToDevice));
+    }
     s->pool.init(s->pool_bytes);
     s->rec_req.resize(M);
     for (int r = M - 1; r >= 0; r--) s->free_recs.push_back(r);
@@ -1551,6 +1587,105 @@ extern "C" pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n
     return PA_OK;
 }
 
+extern "C" pa_status pa_serve_submit_lengths(pa_server *s, const int32_t *ids, int32_t n, const double *lengths,
+                                             const double *arrival_s, int64_t *tickets)
+{
+    if (!s || (n > 0 && (!ids || !lengths)) || n < 0) return PA_E_INPUT;
+    pa_engine *e = s->e;
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    const int N = e->tm.N;
+    const double now = pa_serve_clock();
+    for (int i = 0; i < n; i++) {
+        LRequest q;
+        q.ticket = s->next_ticket++;
+        q.eid = ids[i];
+        q.arrival = arrival_s ? arrival_s[i] : now;
+        q.len.assign(lengths + (size_t)i * N, lengths + (size_t)(i + 1) * N);
+        if (tickets) tickets[i] = q.ticket;
+        s->lwait.push_back(std::move(q));
+    }
+    s->st.submitted += n;
+    return PA_OK;
+}
+
+// Launches the waiting log-likelihood requests (up to lmax) on stream2.
+static pa_status serve_launch_loglik(pa_server *s)
+{
+    pa_engine *e = s->e;
+    if (s->lwait.empty()) return PA_OK;
+    pa_status st = upload(e);
+    if (st != PA_OK) return st;
+    const int N = e->tm.N, S = e->tm.S;
+    while (!s->lwait.empty() && (int)s->lbatch.size() < s->lmax) {
+        s->lbatch.push_back(std::move(s->lwait.front()));
+        s->lwait.pop_front();
+    }
+    std::stable_sort(s->lbatch.begin(), s->lbatch.end(),
+                     [&](const LRequest &a, const LRequest &b) { return e->elems[a.eid].P > e->elems[b.eid].P; });
+    const int n = (int)s->lbatch.size();
+    int Pcap = 64;
+    for (int i = 0; i < n; i++) {
+        s->h_lids[i] = s->lbatch[i].eid;
+        memcpy(s->h_llen + (size_t)i * N, s->lbatch[i].len.data(), sizeof(double) * N);
+        Pcap = std::max(Pcap, e->elems[s->lbatch[i].eid].Ppad);
+    }
+    const int BS = 64;
+    const int grid = std::min(n, e->n_cu * fit_per_cu(e, N, S, BS, s->fp64));
+    size_t off[5];
+    const size_t slot = slot_layout(N, S, Pcap, s->fp64, off);
+    if (slot * grid > s->lscratch_bytes) {
+        (void)hipFree(s->d_lscratch);
+        s->d_lscratch = nullptr;
+        s->lscratch_bytes = 0;
+        HIPCHK(e, hipMalloc(&s->d_lscratch, slot * grid));
+        s->lscratch_bytes = slot * grid;
+    }
+    HIPCHK(e, hipMemcpyAsync(s->d_lids, s->h_lids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream2));
+    HIPCHK(e, hipMemcpyAsync(s->d_llen, s->h_llen, sizeof(double) * (size_t)n * N, hipMemcpyHostToDevice, e->stream2));
+    ChainArgs A;
+    memset(&A, 0, sizeof A);
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+    A.queue = e->d_queue + 32;
+    A.work = s->d_lids;
+    A.n_work = n;
+    A.scratch = s->d_lscratch;
+    A.slot_bytes = (int64_t)slot;
+    A.Zin = s->d_lZ;
+    A.rates_in = s->d_lrates;
+    A.len_in = s->d_llen;
+    A.ll_out = s->d_lout;
+    HIPCHK(e, launch_reset(A.queue, e->stream2));
+    HIPCHK(e, launch_kernel(e->M, A, s->fp64, BS, grid, 1, e->stream2));
+    HIPCHK(e, hipMemcpyAsync(s->h_lout, s->d_lout, sizeof(double) * n, hipMemcpyDeviceToHost, e->stream2));
+    HIPCHK(e, hipEventRecord(s->evl, e->stream2));
+    return PA_OK;
+}
+
+static pa_status serve_retire_loglik(pa_server *s, int32_t *n_finished)
+{
+    pa_engine *e = s->e;
+    if (s->lbatch.empty()) return PA_OK;
+    HIPCHK(e, hipEventSynchronize(s->evl));
+    const double now = pa_serve_clock();
+    for (size_t i = 0; i < s->lbatch.size(); i++) {
+        pa_serve_result out{};
+        out.ticket = s->lbatch[i].ticket;
+        out.element = s->lbatch[i].eid;
+        out.kind = PA_SERVE_LOGLIK;
+        out.loglik = s->h_lout[i];
+        out.status = std::isnan(out.loglik) ? PA_ELEM_NUMERIC_FAIL : PA_ELEM_OK;
+        out.arrival_s = s->lbatch[i].arrival;
+        out.admit_s = now;
+        out.finish_s = now;
+        s->finished.push_back(out);
+        s->st.finished++;
+        (*n_finished)++;
+    }
+    s->lbatch.clear();
+    return PA_OK;
+}
+
 // Retires the chains of the finished launch.
 static pa_status serve_retire(pa_server *s, int32_t *n_finished)
 {
@@ -1574,6 +1709,7 @@ static pa_status serve_retire(pa_server *s, int32_t *n_finished)
         out.ticket = q.ticket;
         out.element = q.eid;
         out.status = PA_ELEM_OK;
+        out.kind = PA_SERVE_CHAINS;
         for (int k = 0; k < 3; k++) {
             const double *d = &rd[k * 8];
             const int32_t *x = &ri[k * 4];
@@ -1703,16 +1839,23 @@ extern "C" pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finish
     int32_t nf = 0;
     if (n_finished) *n_finished = 0;
     HIPCHK(e, hipSetDevice(e->device));
-    if (s->in_flight) {
-        if (!wait) {
+    if (!wait) {
+        if (s->in_flight) {
             const hipError_t q = hipEventQuery(s->ev1);
             if (q == hipErrorNotReady) return PA_OK;
             if (q != hipSuccess) return fail(e, PA_E_DEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
         }
-        pa_status st = serve_retire(s, &nf);
-        if (st != PA_OK) return st;
+        if (!s->lbatch.empty()) {
+            const hipError_t q = hipEventQuery(s->evl);
+            if (q == hipErrorNotReady) return PA_OK;
+            if (q != hipSuccess) return fail(e, PA_E_DEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
+        }
     }
-    pa_status st = serve_launch(s);
+    pa_status st = PA_OK;
+    if (s->in_flight && (st = serve_retire(s, &nf)) != PA_OK) return st;
+    if ((st = serve_retire_loglik(s, &nf)) != PA_OK) return st;
+    if ((st = serve_launch(s)) != PA_OK) return st;
+    if ((st = serve_launch_loglik(s)) != PA_OK) return st;
     if (n_finished) *n_finished = nf;
     return st;
 }
@@ -1733,8 +1876,8 @@ extern "C" pa_status pa_serve_get_stats(pa_server *s, pa_serve_stats *out)
 {
     if (!s || !out) return PA_E_INPUT;
     *out = s->st;
-    out->waiting = (int32_t)s->waiting.size();
-    out->in_flight = (int32_t)s->active.size();
+    out->waiting = (int32_t)(s->waiting.size() + s->lwait.size());
+    out->in_flight = (int32_t)(s->active.size() + s->lbatch.size());
     out->scratch_bytes_in_use = (double)s->pool.used;
     return PA_OK;
 }
@@ -1745,6 +1888,7 @@ extern "C" void pa_serve_destroy(pa_server *s)
     (void)hipSetDevice(s->e->device);
     if (s->in_flight) (void)hipEventSynchronize(s->ev1);
     (void)hipStreamSynchronize(s->e->stream);
+    (void)hipStreamSynchronize(s->e->stream2);
     serve_free(s);
     delete s;
 }

@@ -29,7 +29,8 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count",
            "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths",
            "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
-           "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune"]
+           "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
+           "pa_serve_submit_lengths"]
 
 
 class PaError(RuntimeError):
@@ -102,10 +103,13 @@ class ServeConfig(C.Structure):
                 ("max_chains", C.c_int32), ("hbm_budget_bytes", C.c_double)]
 
 
+PA_SERVE_CHAINS, PA_SERVE_LOGLIK = 0, 1
+
+
 class ServeResult(C.Structure):
-    _fields_ = [("ticket", C.c_int64), ("element", C.c_int32), ("status", C.c_int32),
-                ("result", ChainResult * 3), ("arrival_s", C.c_double), ("admit_s", C.c_double),
-                ("finish_s", C.c_double)]
+    _fields_ = [("ticket", C.c_int64), ("element", C.c_int32), ("status", C.c_int32), ("kind", C.c_int32),
+                ("pad", C.c_int32), ("result", ChainResult * 3), ("loglik", C.c_double),
+                ("arrival_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double)]
 
 
 class ServeStats(C.Structure):
@@ -166,6 +170,9 @@ def lib():
         L.pa_serve_create.restype = C.c_int
         L.pa_serve_submit.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), P(C.c_int64)]
         L.pa_serve_submit.restype = C.c_int
+        L.pa_serve_submit_lengths.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), P(C.c_double),
+                                              P(C.c_int64)]
+        L.pa_serve_submit_lengths.restype = C.c_int
         L.pa_serve_step.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32)]
         L.pa_serve_step.restype = C.c_int
         L.pa_serve_poll.argtypes = [C.c_void_p, P(ServeResult), C.c_int32, P(C.c_int32)]
@@ -447,6 +454,16 @@ class Server:
         arr = None if arrival_s is None else np.ascontiguousarray(arrival_s, np.float64)
         self._check(lib().pa_serve_submit(self._h, _p(ids, C.c_int32), len(ids),
                                           None if arr is None else _p(arr, C.c_double), _p(t, C.c_int64)))
+        return t
+
+    def submit_lengths(self, ids, lengths, arrival_s=None):
+        """Log-likelihood requests for own-tree elements at branch lengths [n, N]."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        L = np.ascontiguousarray(lengths, np.float64).reshape(len(ids), -1)
+        t = np.zeros(len(ids), np.int64)
+        arr = None if arrival_s is None else np.ascontiguousarray(arrival_s, np.float64)
+        self._check(lib().pa_serve_submit_lengths(self._h, _p(ids, C.c_int32), len(ids), _p(L, C.c_double),
+                                                  None if arr is None else _p(arr, C.c_double), _p(t, C.c_int64)))
         return t
 
     def step(self, wait=True) -> int:

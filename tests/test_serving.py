@@ -128,3 +128,27 @@ def test_elements_added_while_serving(require_gpu):
             assert r.result[ms].max_loglik == ref["max_loglik"][k, ms]
             assert r.result[ms].max_m == ref["max_m"][k, ms]
     srv.close()
+
+
+def test_gene_tree_requests_through_the_loop(require_gpu):
+    """Own-tree (gene-tree) log-likelihood requests are served beside chains
+    and equal the batched call."""
+    gw = synth.make_gt_workload(seed=6, n_elements=20, S=30)
+    eng = E.Engine.from_workload(gw, E.default_options(burnin=10, mcmc=30, adapt_freq=15))
+    gids, gst = eng.add_elements_with_trees(gw.align, gw.starts, gw.sizes, gw.children())
+    L = gw.lengths()
+    ref = eng.loglik_lengths(gids, L, E.PA_FP32)
+    cids, cst = eng.add_elements(gw.align, gw.starts[:6], gw.sizes[:6])   # species-tree chains alongside
+    srv = E.Server(eng, precision=E.PA_FP32, sweep_budget=10)
+    tc = list(srv.submit(cids[cst == E.PA_ELEM_OK]))
+    tl = list(srv.submit_lengths(gids[:10], L[:10]))
+    srv.step(wait=True)
+    tl += list(srv.submit_lengths(gids[10:], L[10:]))
+    got = _drain(srv, len(tc) + len(tl))
+    for k, t in enumerate(tl):
+        assert got[t].kind == E.PA_SERVE_LOGLIK
+        # same kernel; the batched call may pick another block size, which only
+        # reorders the FP64 sum over patterns
+        np.testing.assert_allclose(got[t].loglik, ref[k], rtol=1e-12)
+    assert all(got[t].kind == E.PA_SERVE_CHAINS for t in tc)
+    srv.close()
