@@ -82,7 +82,9 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
-            build()
+            # never built from here: a process that has touched the GPU must not
+            # start other programs, so the build belongs to __graft_entry__.build()
+            raise RuntimeError(f"{LIB} is missing: build it with make -C oracle")
         L = C.CDLL(LIB)
         P = C.POINTER
         L.or_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]

@@ -1,6 +1,8 @@
 // pa_cli.cpp -- drop-in command line for PhyloAcc-ST runs on the MI355X engine.
 //
 // Usage: phyloacc-st-gpu <param file> [--dry-run]
+// The program is pa_cli_main, exported by the engine library, so that a
+// host process can also run it in-process.
 //
 // Reads the reference's parameter file (src/PhyloAcc-ST/main.cpp:100-287),
 // the .mod tree (newick.cpp:431-517, through pa_parse_mod), the FASTA
@@ -63,10 +65,17 @@ std::vector<std::string> split(const std::string &s, char d)
 
 bool to_bool(const std::string &s) { return s == "true" || s == "1"; }
 
+// Input errors end the run with status 1 (the reference calls exit(1)); the
+// status travels as an exception to pa_cli_main so that an in-process caller
+// survives it.
+struct CliExit {
+    int code;
+};
+
 [[noreturn]] void die(const std::string &msg)
 {
     std::cerr << msg << std::endl;
-    exit(1);
+    throw CliExit{1};
 }
 
 // LoadParams (main.cpp:100-287): "KEY value" lines, '#' comments.
@@ -211,7 +220,24 @@ std::string fmt_g(double v, int prec)
 
 }  // namespace
 
-int main(int argc, char **argv)
+static int cli(int argc, char **argv);
+
+extern "C" int pa_cli_main(int argc, char **argv)
+{
+    try {
+        const int rc = cli(argc, argv);
+        std::cout.flush();
+        return rc;
+    } catch (const CliExit &x) {
+        std::cout.flush();
+        return x.code;
+    } catch (const std::exception &x) {
+        std::cerr << "error: " << x.what() << std::endl;
+        return 1;
+    }
+}
+
+static int cli(int argc, char **argv)
 {
     if (argc < 2) {
         std::cout << "Please specify the path of the parameter file." << std::endl;
@@ -310,6 +336,10 @@ int main(int argc, char **argv)
     m.conserve = cons.data(); m.n_conserve = (int32_t)cons.size();
     pa_engine *eng = nullptr;
     if (pa_engine_create(&m, &P.o, P.device, &eng) != PA_OK) die(std::string("engine: ") + pa_last_error(eng));
+    struct Guard {
+        pa_engine *e;
+        ~Guard() { pa_engine_destroy(e); }
+    } guard{eng};
     std::vector<int64_t> st_ids(ids.size());
     std::vector<int32_t> ln_ids(ids.size()), eid(ids.size()), est(ids.size());
     for (size_t i = 0; i < ids.size(); i++) { st_ids[i] = start[ids[i]]; ln_ids[i] = (int32_t)pf.elem_len[ids[i]]; }
@@ -324,7 +354,6 @@ int main(int argc, char **argv)
             std::cout << ids[i] << "\t" << pf.elem_names[ids[i]] << "\t" << (est[i] == PA_ELEM_FILTERED ? "filtered" : "ok")
                       << "\tGG=" << inf.GG << "\tP=" << inf.P << "\troot=" << inf.root << "\tnodes=" << inf.n_nodes << "\n";
         }
-        pa_engine_destroy(eng);
         return 0;
     }
 
@@ -412,6 +441,5 @@ int main(int argc, char **argv)
             }
         }
     }
-    pa_engine_destroy(eng);
     return 0;
 }

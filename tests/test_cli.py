@@ -49,10 +49,23 @@ def _write_inputs(tmp, w, extra="", shuffle_rows=True, ids=None):
 
 
 def _run(*args, check=True):
+    # CPU-only tests: the program runs as its own process
     r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=600)
     if check:
         assert r.returncode == 0, r.stdout + r.stderr
     return r
+
+
+def _run_in_process(*args):
+    """The same program through pa_cli_main in this process: a process that
+    has used the GPU must not start other programs."""
+    import ctypes as C
+    argv = [CLI.encode(), *[a.encode() for a in args]]
+    arr = (C.c_char_p * len(argv))(*argv)
+    fn = E.lib().pa_cli_main
+    fn.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+    fn.restype = C.c_int
+    return fn(len(argv), arr)
 
 
 def test_dry_run_matches_engine_encoding(tmp_path):
@@ -99,7 +112,7 @@ def _table(path):
 def test_full_run_outputs_match_oracle(require_gpu, tmp_path):
     w = synth.make_workload(1, seed=23, n_elements=5)
     p, out = _write_inputs(tmp_path, w, extra="BURNIN 20\nMCMC 60\nADAPT_FREQ 25\nPRECISION fp64\n")
-    _run(p)
+    assert _run_in_process(p) == 0
     m = O.model_from_workload(w)
     oo = O.default_options(burnin=20, mcmc=60, adapt_freq=25)
     ref = {c: O.Element(m, oo, w.align, w.starts[c], w.sizes[c]).run(oo) for c in range(w.C)}
