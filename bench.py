@@ -2,9 +2,9 @@
 
 Workload (default): 43 species, 50,000 elements with Pareto(1.5) lengths on
 [50, 10000] bp, 1-20% of branches accelerated per element (synthetic,
-seeded).  One step = the full three-model chain (M0, M1, M2; BURNIN 200 +
-MCMC 800 states each, the reference defaults) of every element in one
-persistent launch.  One element-MCMC iteration = one sweep m -> m+1 of one
+seeded).  One step = the full three-model chain (M0, M1, M2; BURNIN 50 +
+MCMC 150 states each by default) of every element in one persistent
+launch.  One element-MCMC iteration = one sweep m -> m+1 of one
 model chain of one element; value sums the three models.
 
 Also reported (same JSON line):
@@ -45,12 +45,12 @@ def stratified(sizes, k, rng):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--elements", type=int, default=None, help="scale the element count down")
-    ap.add_argument("--burnin", type=int, default=200)
-    ap.add_argument("--mcmc", type=int, default=800)
+    ap.add_argument("--burnin", type=int, default=50)
+    ap.add_argument("--mcmc", type=int, default=150)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-cpu", action="store_true")
@@ -61,8 +61,6 @@ def main():
     ap.add_argument("--cpu-mcmc", type=int, default=550)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true", help="run a plain copy kernel of known size and exit")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default size classes")
-    ap.add_argument("--tune-sample", type=int, default=2048)
     ap.add_argument("--no-agree", action="store_true", help="skip the FP32-vs-FP64 agreement sample")
     ap.add_argument("--agree-sample", type=int, default=256)
     ap.add_argument("--arrival-rate", type=float, default=2000.0, help="config 4: requests per second")
@@ -101,11 +99,6 @@ def main():
     T = args.burnin + args.mcmc
     sweeps_per_step = len(keep) * 3 * (T - 1)
 
-    tuning = None
-    if args.block == 0 and not args.no_tune:
-        tt = time.time()
-        tuning = eng.autotune(keep, precision=prec, sample=args.tune_sample, states=16)
-        log(f"autotune: {tuning} ({time.time() - tt:.1f}s)")
     for i in range(args.warmup):
         r = eng.run_chains(keep, precision=prec, block_threads=args.block, keep_traces=True)
         log(f"warmup {i}: {r['elapsed_ms'] / 1e3:.2f}s device")
@@ -125,7 +118,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "kernel": "chain_kernel", "algorithmic_bytes_per_launch": stats["algorithmic_bytes"],
                 "grid": stats["grid"], "block_threads": stats["block_threads"], "lds_bytes": stats["lds_bytes"],
-                "launches": stats["n_launches"], "tuning": tuning}
+                "launches": stats["n_launches"]}
 
     rng = np.random.default_rng(0)
     agree = None

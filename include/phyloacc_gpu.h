@@ -219,24 +219,26 @@ pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, const double
                            double *postZ);
 
 /* ------------------------------------------------------------------ serving
- * Continuous batching of element chains.  The reference has no serving mode:
- * its OpenMP loop (src/PhyloAcc-ST/main.cpp:452-500) runs a fixed id list to
+ * Request loop over element chains.  The reference has no serving mode: its
+ * OpenMP loop (src/PhyloAcc-ST/main.cpp:452-500) runs a fixed id list to
  * completion.  Here requests (already encoded element ids) arrive at any time;
- * each pa_serve_step retires the chains that finished, admits waiting
- * requests while a chain record and scratch for the element's partials fit
- * in the HBM budget, and launches one persistent kernel that advances every
- * admitted chain by sweep_budget sweeps.  Chain state stays on the device
- * between launches.  A chain's result does not depend on what else is in
- * flight: its random streams are keyed by SEED and the element, and the
- * launch boundaries fall on whole sweeps. */
+ * each pa_serve_step retires the launch that finished, admits waiting
+ * requests in arrival order while the launch's scratch fits in the HBM budget,
+ * and launches them: every admitted chain runs all three models to the end
+ * within that launch.  A chain's result does not depend on what else is in
+ * the launch: its random streams are keyed by SEED and the element, one
+ * workgroup runs it, and the block size is fixed per server.  Resumable
+ * chains (sweep_budget > 0: chain state kept on the device across launches)
+ * are switched off in this build; pa_serve_create rejects them with
+ * PA_E_STATE. */
 typedef struct pa_server pa_server;
 
 typedef struct {
     int32_t precision;          /* PA_FP64 / PA_FP32 */
     int32_t block_threads;      /* 64, 128 or 256; 0 = 128 */
-    int32_t sweep_budget;       /* sweeps per chain per launch; 0 = 100 */
+    int32_t sweep_budget;       /* must be 0: each chain runs to completion in its launch */
     int32_t max_chains;         /* chains in flight at most; 0 = 4096 */
-    double hbm_budget_bytes;    /* scratch for in-flight chains; 0 = 4 GiB */
+    double hbm_budget_bytes;    /* scratch of one launch; 0 = 4 GiB */
 } pa_serve_config;
 
 enum { PA_SERVE_CHAINS = 0, PA_SERVE_LOGLIK = 1 };
@@ -264,11 +266,12 @@ pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **
 pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n, const double *arrival_s, int64_t *tickets);
 /* Log-likelihood requests for elements that carry their own tree (gene
  * trees, pa_elements_add_trees) at per-request branch lengths [n*N]: each
- * step prunes the waiting ones in one batched launch beside the chains. */
+ * step prunes the admitted ones in one batched launch after the chains. */
 pa_status pa_serve_submit_lengths(pa_server *s, const int32_t *ids, int32_t n, const double *lengths,
                                   const double *arrival_s, int64_t *tickets);
 /* One turn of the request loop.  With wait = 0 it returns at once while the
- * current launch is still running (n_finished = 0). */
+ * current launch is still running (n_finished = 0); with wait = 1 it also
+ * waits for the launch it starts and retires it. */
 pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finished);
 /* Moves up to max finished results out of the server, oldest first. */
 pa_status pa_serve_poll(pa_server *s, pa_serve_result *out, int32_t max, int32_t *n);

@@ -1,0 +1,1042 @@
+// pa_chain.hip -- CDNA4 (gfx950) chain kernel for PhyloAcc-ST's per-element MCMC.
+//
+// The host launches chain_kernel from this file and the fixed-(Z, r)
+// log-likelihood kernel from pa_engine.hip.  The loglik_kernel below is
+// never launched; it stays because the compiler derives the code of the
+// functions it shares with chain_kernel (prune, bind_element, refresh_PTc,
+// init_model) from all their callers, and this file is kept compiling to
+// exactly the chain-kernel code that has run on the GPU.
+//
+// One workgroup owns one element at a time and runs its three model chains
+// (M0, M1, M2 = resZ 0, 2, 1; src/PhyloAcc-ST/main.cpp:466-494) start to
+// finish; workgroups pull elements from a device work queue (persistent
+// kernel).  Within a workgroup:
+//   * threads stride over site patterns (pruning) or sites (ancestral
+//     sampling); every thread keeps the same patterns through a whole
+//     post-order pass, so no barrier is needed between tree nodes;
+//   * conditional likelihoods are pattern-major in HBM scratch, two buffers
+//     per internal node (current / proposal) so an accepted Metropolis move
+//     flips a per-node bit instead of copying partials;
+//   * partials are linear-space with a power-of-two rescale per node; the
+//     cumulative exponent rides along as an int; GAPT marks all-gap subtrees
+//     (the reference's Tg >= num_base, src/PhyloAcc-ST/bpp_c.hpp:417-428);
+//   * per-branch 4x4 transition matrices of the current rate class live in
+//     LDS (read as broadcasts); FP64 is used for P(t) builds, log tables,
+//     emissions, the Z forward-backward, likelihood sums and MH ratios.
+// The chain context and the model constants live in LDS behind
+// address-space-3 pointers, and every sweep phase is a separate non-inlined
+// function, so each phase gets its own register allocation instead of one
+// allocation spanning the whole persistent loop.
+// No MFMA: a K=4 matvec per pattern is not a dense contraction.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pa_common.h"
+#include "pa_rng.h"
+#include "pa_chain.h"
+
+#define LDS __attribute__((address_space(3)))
+#define GBL __attribute__((address_space(1)))
+
+namespace pa {
+namespace chain {
+
+template <typename R>
+struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };
+
+// explicit member-wise loads/stores through address-space pointers
+template <typename R, typename P>
+__device__ __forceinline__ V4<R> ld4(const P *p)
+{
+    V4<R> v;
+    v.x = p->x; v.y = p->y; v.z = p->z; v.w = p->w;
+    return v;
+}
+template <typename R, typename P>
+__device__ __forceinline__ void st4(P *p, R a, R b, R c, R d)
+{
+    p->x = a; p->y = b; p->z = c; p->w = d;
+}
+
+template <typename R>
+struct Ctx {
+    // element
+    int S, N, I, GG, P, Ppad, root, n_nodes, n_int, n_up, n_down, has_amb;
+    const GBL uint8_t *codes;
+    const GBL int32_t *wts, *site;
+    // workgroup scratch in HBM
+    GBL V4<R> *parts;      // [2][I][Ppad]
+    GBL int32_t *exps;     // [2][I][Ppad]
+    GBL double *plin;      // [3][N][16]  P_{j->i} at [j*4+i]
+    GBL double *logt;      // [3][N][16]  log P_{j->i} at [i*4+j]
+    GBL uint32_t *amb;     // [S][24]
+    // LDS arrays
+    LDS int16_t *ch0, *ch1, *par, *nodes, *inodes, *up, *upoff, *down, *downoff, *tz;
+    LDS uint8_t *flags, *fixZ, *dirty, *cur, *changed;
+    LDS int8_t *Z, *maxZ;
+    LDS double *dist2, *E, *beta;
+    LDS R *PTc, *PTp;      // [N][16], PT[i*4+j] = P_{j->i}
+    LDS uint32_t *hist, *cnt9;
+    LDS uint8_t *tgs;
+    // chain scalars
+    double loglik, nrate, crate, g, l, l2, prop_n, prop_c, proposal, r, max_loglik;
+    double TMf[9], TMn[9];  // linear log_TM_Int for fixZ / free nodes, [i*3+j] = P(child i | parent j)
+    double red[16];
+    unsigned long long bytes;   // algorithmic HBM bytes of this element's passes (see prune / sample_sites)
+    int accept_n, accept_c, max_m, skip, accept, status, work, nch;
+};
+
+template <typename R>
+using CP = LDS Ctx<R> *;
+using MP = LDS const ModelDev *;
+
+__host__ __device__ inline size_t al(size_t x, size_t a) { return (x + a - 1) & ~(a - 1); }
+
+// Dynamic-LDS layout: returns the byte size; BIND also points the context's
+// array fields into the block.  The context may sit at LDS address 0, which
+// compares equal to a null pointer, so binding is selected at compile time and
+// never by testing the context pointer.
+template <typename R, bool BIND>
+__host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<R> c)
+{
+    const int I = N - S;
+    size_t o = 0;
+#define PA_TAKE(field, T, count)                                                   \
+    do {                                                                           \
+        o = al(o, 32);  /* rows of PT are read as 4-vectors */                      \
+        if constexpr (BIND) c->field = (LDS T *)(base + o);                        \
+        o += sizeof(T) * (size_t)(count);                                          \
+    } while (0)
+    PA_TAKE(dist2, double, N);
+    PA_TAKE(E, double, 3 * N);
+    PA_TAKE(beta, double, 3 * N);
+    PA_TAKE(PTc, R, 16 * N);
+    PA_TAKE(PTp, R, 16 * N);
+    PA_TAKE(hist, uint32_t, 16 * N);
+    PA_TAKE(cnt9, uint32_t, 20);
+    PA_TAKE(ch0, int16_t, N);
+    PA_TAKE(ch1, int16_t, N);
+    PA_TAKE(par, int16_t, N);
+    PA_TAKE(nodes, int16_t, N);
+    PA_TAKE(inodes, int16_t, N);
+    PA_TAKE(up, int16_t, N);
+    PA_TAKE(upoff, int16_t, N + 1);
+    PA_TAKE(down, int16_t, N);
+    PA_TAKE(downoff, int16_t, N + 1);
+    PA_TAKE(tz, int16_t, N);
+    PA_TAKE(flags, uint8_t, N);
+    PA_TAKE(fixZ, uint8_t, N);
+    PA_TAKE(dirty, uint8_t, N);
+    PA_TAKE(cur, uint8_t, N);
+    PA_TAKE(changed, uint8_t, N);
+    PA_TAKE(Z, int8_t, N);
+    PA_TAKE(maxZ, int8_t, N);
+    PA_TAKE(tgs, uint8_t, (size_t)(I > 0 ? I : 1) * BS);
+#undef PA_TAKE
+    return al(o, 16);
+}
+
+size_t chain_smem_bytes(int N, int S, int BS, int fp64)
+{
+    return (fp64 ? smem_arrays<double, false>(N, S, BS, nullptr, nullptr)
+                 : smem_arrays<float, false>(N, S, BS, nullptr, nullptr)) +
+           (fp64 ? sizeof(Ctx<double>) : sizeof(Ctx<float>)) + sizeof(ModelDev);
+}
+
+// Per-workgroup HBM scratch layout (bytes), shared with the host.
+size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off)
+{
+    int I = N - S;
+    size_t rs = fp64 ? 8 : 4, o = 0;
+    off[0] = o; o = al(o + (size_t)2 * I * Pcap * 4 * rs, 256);   // parts
+    off[1] = o; o = al(o + (size_t)2 * I * Pcap * 4, 256);        // exps
+    off[2] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // plin
+    off[3] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // logt
+    off[4] = o; o = al(o + (size_t)S * 24 * 4, 256);              // amb
+    return o;
+}
+
+// ------------------------------------------------------------------ helpers
+
+template <typename R>
+__device__ __forceinline__ double block_sum(double v, CP<R> c)
+{
+    // deterministic: fixed xor tree inside each wave, waves summed in order
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) c->red[w] = v;
+    __syncthreads();
+    double t = 0;
+    for (int i = 0; i < nw; i++) t += c->red[i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void lds_add(LDS uint32_t *p, uint32_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// P(t) = V diag(exp(lam t)) Vi, clamped at 0; out[j*4+i] = P_{j->i}
+template <typename OutPtr>
+__device__ __forceinline__ void ptrans(MP M, double t, OutPtr out)
+{
+    double e[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) e[k] = exp(M->lam[k] * t);
+#pragma unroll 1
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) s += M->V[j * 4 + k] * e[k] * M->Vi[k * 4 + i];
+            out[j * 4 + i] = s > 0 ? s : 0.0;
+        }
+}
+
+// class tables for branch s, class k (0 null, 1 conserved, 2 accelerated) at rate r.
+// mode 0: model start (dist2 > 1e-10 -> P(dist2*r), else identity; bpp_c.cpp:187-228)
+// mode 1: refresh after a rate change (dist2 < 1e-10 keeps the old table; bpp_c.cpp:1115-1131)
+template <typename R>
+__device__ __noinline__ void class_table(MP M, CP<R> c, int k, int s, double r, int mode)
+{
+    const double d = c->dist2[s];
+    GBL double *P = c->plin + ((size_t)k * c->N + s) * 16;
+    GBL double *L = c->logt + ((size_t)k * c->N + s) * 16;
+    if (mode == 1 && d < 1e-10) return;
+    if (d > 1e-10 || mode == 1) ptrans(M, d * r, P);
+    else
+        for (int i = 0; i < 16; i++) P[i] = (i % 5 == 0) ? 1.0 : 0.0;
+#pragma unroll 1
+    for (int q = 0; q < 16; q++) L[q] = log(P[(q & 3) * 4 + (q >> 2)]);
+}
+
+// PT of the current class of every branch (all branches, or those with Z == only_cls)
+template <typename R, int BS>
+__device__ __noinline__ void refresh_PTc(CP<R> c, int only_cls)
+{
+    for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
+        const int s = c->nodes[q];
+        const int z = c->Z[s];
+        if (only_cls >= 0 && z != only_cls) continue;
+        const int k = z == 2 ? 2 : (z == 0 ? 0 : 1);
+        const GBL double *P = c->plin + ((size_t)k * c->N + s) * 16;
+        LDS R *dst = c->PTc + 16 * s;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) dst[i * 4 + j] = (R)P[j * 4 + i];
+    }
+}
+
+// ----------------------------------------------------------------- pruning
+
+enum { PR_FULL = 0, PR_DIRTY = 1, PR_PROP = 2 };
+
+template <typename R>
+struct M16 { R v[16]; };
+
+template <typename R>
+__device__ __forceinline__ void leaf_mul(const LDS R *PT, int code, R &a0, R &a1, R &a2, R &a3)
+{
+    if (code < 4) {
+        const V4<R> row = ld4<R>((const LDS V4<R> *)(PT + code * 4));
+        a0 *= row.x; a1 *= row.y; a2 *= row.z; a3 *= row.w;
+    } else {
+        // ambiguity codes r y k m s w -> base pairs (a,g) (c,t) (g,t) (a,c) (c,g) (a,t)
+        const int b1 = (code == 6 || code == 9 || code == 11) ? 0 : (code == 7 || code == 10) ? 1 : 2;
+        const int b2 = (code == 6 || code == 10) ? 2 : (code == 9) ? 1 : 3;
+        const V4<R> r1 = ld4<R>((const LDS V4<R> *)(PT + b1 * 4));
+        const V4<R> r2 = ld4<R>((const LDS V4<R> *)(PT + b2 * 4));
+        a0 *= r1.x + r2.x; a1 *= r1.y + r2.y; a2 *= r1.z + r2.z; a3 *= r1.w + r2.w;
+    }
+}
+
+// Multiplies one child's message P * L_child into (a0..a3).  A leaf reads the
+// row of its base from the LDS matrix; an internal child uses the register
+// copy m.  gap reports the child's GAPT state (Tg >= num_base).
+template <typename R>
+__device__ __forceinline__ void child_term(bool leaf, bool miss, const LDS R *src, const M16<R> &m,
+                                           const GBL uint8_t *code_row, const GBL V4<R> *part_row,
+                                           const GBL int32_t *exp_row, int p, R &a0, R &a1, R &a2, R &a3, int &e,
+                                           bool &gap)
+{
+    if (leaf) {
+        const int code = code_row[p];
+        gap = (code == PA_GAP_CODE || code == PA_OTHER_CODE);
+        if (!miss && !gap) leaf_mul<R>(src, code, a0, a1, a2, a3);
+    } else {
+        const int ex = exp_row[p];
+        gap = (ex == GAPT);
+        if (!miss && !gap) {
+            const V4<R> v = ld4<R>(part_row + p);
+            const R o0 = m.v[0] * v.x + m.v[4] * v.y + m.v[8] * v.z + m.v[12] * v.w;
+            const R o1 = m.v[1] * v.x + m.v[5] * v.y + m.v[9] * v.z + m.v[13] * v.w;
+            const R o2 = m.v[2] * v.x + m.v[6] * v.y + m.v[10] * v.z + m.v[14] * v.w;
+            const R o3 = m.v[3] * v.x + m.v[7] * v.y + m.v[11] * v.z + m.v[15] * v.w;
+            a0 *= o0; a1 *= o1; a2 *= o2; a3 *= o3;
+            e += ex;
+        }
+    }
+}
+
+// One post-order pass over internal nodes (BPP_C::Update_Tg(tosample=false),
+// src/PhyloAcc-ST/bpp_c.cpp:644-671, and log_f_Xz, bpp_c.cpp:860-928).
+// FULL: every internal node of the tree (missing nodes only record their GAPT
+// flag) into the current buffer.  DIRTY: nodes with dirty[] set, in place.
+// PROP: dirty nodes into the other buffer, with PTp for branches whose Z is
+// cls.  Returns the element's log-likelihood (FP64), identical on every thread.
+template <typename R, int BS>
+__device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
+{
+    const int tid = threadIdx.x;
+    const int P = c->P, Ppad = c->Ppad, S = c->S, n_int = c->n_int;
+    const size_t bufI = (size_t)c->I * Ppad;
+    GBL V4<R> *parts = c->parts;
+    GBL int32_t *exps = c->exps;
+    const GBL uint8_t *codes = c->codes;
+    for (int k = 0; k < n_int; k++) {
+        const int s = c->inodes[k];
+        const bool miss_s = c->flags[s] & NF_MISSING;
+        if (mode != PR_FULL && (!c->dirty[s] || miss_s)) continue;
+        const int wb = (mode == PR_PROP) ? 1 - c->cur[s] : c->cur[s];
+        const int x0 = c->ch0[s], x1 = c->ch1[s];
+        const bool leaf0 = x0 < S, leaf1 = x1 < S;
+        const bool miss0 = c->flags[x0] & NF_MISSING, miss1 = c->flags[x1] & NF_MISSING;
+        const LDS R *src0 = (mode == PR_PROP && c->Z[x0] == cls) ? c->PTp + 16 * x0 : c->PTc + 16 * x0;
+        const LDS R *src1 = (mode == PR_PROP && c->Z[x1] == cls) ? c->PTp + 16 * x1 : c->PTc + 16 * x1;
+        M16<R> m0, m1;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            m0.v[i] = leaf0 ? (R)0 : src0[i];
+            m1.v[i] = leaf1 ? (R)0 : src1[i];
+        }
+        const int rb0 = leaf0 ? 0 : ((mode == PR_PROP && c->dirty[x0]) ? 1 - c->cur[x0] : c->cur[x0]);
+        const int rb1 = leaf1 ? 0 : ((mode == PR_PROP && c->dirty[x1]) ? 1 - c->cur[x1] : c->cur[x1]);
+        const GBL uint8_t *cr0 = codes + (size_t)x0 * Ppad, *cr1 = codes + (size_t)x1 * Ppad;
+        const size_t o0 = leaf0 ? 0 : rb0 * bufI + (size_t)(x0 - S) * Ppad;
+        const size_t o1 = leaf1 ? 0 : rb1 * bufI + (size_t)(x1 - S) * Ppad;
+        const GBL V4<R> *pr0 = parts + o0, *pr1 = parts + o1;
+        const GBL int32_t *er0 = exps + o0, *er1 = exps + o1;
+        GBL V4<R> *dstp = parts + wb * bufI + (size_t)(s - S) * Ppad;
+        GBL int32_t *dste = exps + wb * bufI + (size_t)(s - S) * Ppad;
+        if (tid == 0)   // write partial + exponent; read each child's partial + exponent, or its leaf code
+            c->bytes += (unsigned long long)P * ((4 * sizeof(R) + 4) + (leaf0 ? 1 : 4 * sizeof(R) + 4) + (leaf1 ? 1 : 4 * sizeof(R) + 4));
+        for (int p = tid; p < P; p += BS) {
+            R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
+            int e = 0;
+            bool g0, g1;
+            child_term<R>(leaf0, miss0, src0, m0, cr0, pr0, er0, p, a0, a1, a2, a3, e, g0);
+            child_term<R>(leaf1, miss1, src1, m1, cr1, pr1, er1, p, a0, a1, a2, a3, e, g1);
+            if (g0 && g1) {
+                dste[p] = GAPT;
+            } else if (miss_s) {
+                dste[p] = 0;
+            } else {
+                const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
+                int kx = 0;
+                frexp(mx, &kx);
+                a0 = ldexp(a0, -kx); a1 = ldexp(a1, -kx); a2 = ldexp(a2, -kx); a3 = ldexp(a3, -kx);
+                st4<R>(dstp + p, a0, a1, a2, a3);
+                dste[p] = e + kx;
+            }
+        }
+    }
+    // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
+    const int r = c->root;
+    double acc = 0;
+    if (r >= S) {
+        const int rb = (mode == PR_PROP && c->dirty[r]) ? 1 - c->cur[r] : c->cur[r];
+        const GBL V4<R> *rp = parts + rb * bufI + (size_t)(r - S) * Ppad;
+        const GBL int32_t *re = exps + rb * bufI + (size_t)(r - S) * Ppad;
+        const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
+        if (tid == 0) c->bytes += (unsigned long long)P * (4 * sizeof(R) + 4 + 4);   // root partial, exponent, weight
+        for (int p = tid; p < P; p += BS) {
+            const int ex = re[p];
+            if (ex == GAPT) continue;
+            const V4<R> a = ld4<R>(rp + p);
+            const double L = p0 * (double)a.x + p1 * (double)a.y + p2 * (double)a.z + p3 * (double)a.w;
+            acc += (double)c->wts[p] * (log(L) + (double)ex * 0.69314718055994530942);
+        }
+    }
+    return block_sum<R>(acc, c);
+}
+
+// ------------------------------------------------------ ancestral sampling
+
+// BPP_C::Update_Tg(tosample=true) (src/PhyloAcc-ST/bpp_c.cpp:672-733) for every
+// site, plus the per-branch transition counts that getEmission
+// (bpp_c2.cpp:355-390) sums over.  Uniform for site g, sampling position t
+// (0 = root, then internal nodes in reverse post-order): ANC stream, c1 = g.
+template <typename R, int BS>
+__device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c3)
+{
+    const int tid = threadIdx.x, S = c->S, n_int = c->n_int, GG = c->GG, Ppad = c->Ppad;
+    const size_t bufI = (size_t)c->I * Ppad;
+    const int r = c->root;
+    const uint32_t k0 = M->key0, k1 = M->key1;
+    const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
+    const GBL V4<R> *parts = c->parts;
+    const GBL int32_t *exps = c->exps;
+    const GBL uint8_t *codes = c->codes;
+    LDS uint8_t *tgs = c->tgs;
+    if (tid == 0) {
+        // per site: pattern index, root partial, each non-missing internal node's partial, each leaf code
+        int nint = 0, nleaf = 0;
+        for (int k = 0; k < n_int; k++) nint += !(c->flags[c->inodes[k]] & NF_MISSING);
+        for (int q = 0; q < S; q++) nleaf += (c->flags[q] & NF_IN_TREE) && !(c->flags[q] & NF_MISSING);
+        c->bytes += (unsigned long long)GG * (4 + (4 * sizeof(R) + 4) * nint + nleaf);
+    }
+    for (int g = tid; g < GG; g += BS) {
+        const int p = c->site[g];
+        U4 blk = philox(0, (uint32_t)g, c2, c3, k0, k1);
+        {
+            const size_t off = c->cur[r] * bufI + (size_t)(r - S) * Ppad + p;
+            double pr[4];
+            if (exps[off] == GAPT) {
+                pr[0] = p0; pr[1] = p1; pr[2] = p2; pr[3] = p3;
+            } else {
+                const V4<R> a = ld4<R>(parts + off);
+                pr[0] = p0 * (double)a.x; pr[1] = p1 * (double)a.y; pr[2] = p2 * (double)a.z; pr[3] = p3 * (double)a.w;
+            }
+            tgs[(r - S) * BS + tid] = (uint8_t)categorical<4>(pr, u01(blk.x));
+        }
+        int t = 1;
+        for (int k = n_int - 2; k >= 0; k--, t++) {
+            if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
+            const int s = c->inodes[k];
+            if (c->flags[s] & NF_MISSING) continue;
+            const size_t off = c->cur[s] * bufI + (size_t)(s - S) * Ppad + p;
+            if (exps[off] == GAPT) continue;
+            const int tp = tgs[(c->par[s] - S) * BS + tid];
+            const V4<R> a = ld4<R>(parts + off);
+            const LDS R *PT = c->PTc + 16 * s;
+            double pr[4] = {(double)PT[0 * 4 + tp] * (double)a.x, (double)PT[1 * 4 + tp] * (double)a.y,
+                            (double)PT[2 * 4 + tp] * (double)a.z, (double)PT[3 * 4 + tp] * (double)a.w};
+            const int ts = categorical<4>(pr, u01(pick(blk, t & 3)));
+            tgs[(s - S) * BS + tid] = (uint8_t)ts;
+            lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
+        }
+        for (int s = 0; s < S; s++) {
+            const uint8_t f = c->flags[s];
+            if (!(f & NF_IN_TREE) || (f & NF_MISSING)) continue;
+            const int code = codes[(size_t)s * Ppad + p];
+            if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
+            const int tp = tgs[(c->par[s] - S) * BS + tid];
+            if (code < 4) lds_add(&c->hist[s * 16 + tp * 4 + code], 1u);
+            else __hip_atomic_fetch_add(&c->amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+}
+
+// getEmission (src/PhyloAcc-ST/bpp_c2.cpp:355-390): E[s][k] from the counts.
+template <typename R, int BS>
+__device__ __noinline__ void emissions(CP<R> c)
+{
+    const int N = c->N;
+    for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
+        const int s = c->nodes[q];
+        if (c->flags[s] & NF_MISSING) continue;
+        LDS uint32_t *h = c->hist + s * 16;
+        uint32_t n[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) { n[i] = h[i]; h[i] = 0; }
+#pragma unroll 1
+        for (int k = 0; k < 3; k++) {
+            const GBL double *L = c->logt + ((size_t)k * N + s) * 16;
+            double e = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (n[j * 4 + i]) e += (double)n[j * 4 + i] * L[i * 4 + j];
+            c->E[s * 3 + k] = e;
+        }
+        if (c->has_amb && s < c->S) {
+            GBL uint32_t *a = c->amb + s * 24;
+            for (int k = 0; k < 3; k++) {
+                const GBL double *P = c->plin + ((size_t)k * N + s) * 16;
+                double e = 0;
+                for (int j = 0; j < 4; j++)
+                    for (int x = 0; x < 6; x++) {
+                        const uint32_t cnt = a[j * 6 + x];
+                        if (!cnt) continue;
+                        const int code = 6 + x;
+                        const int b1 = (code == 6 || code == 9 || code == 11) ? 0 : (code == 7 || code == 10) ? 1 : 2;
+                        const int b2 = (code == 6 || code == 10) ? 2 : (code == 9) ? 1 : 3;
+                        e += (double)cnt * log(P[j * 4 + b1] + P[j * 4 + b2]);
+                    }
+                c->E[s * 3 + k] += e;
+            }
+            for (int i = 0; i < 24; i++) a[i] = 0;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- Z update
+
+// BPP_C::Update_Z_subtree (src/PhyloAcc-ST/bpp_c.cpp:1182-1293) in linear
+// space, one tree level per step: upward messages by height, then top-down
+// draws by depth.  The draw for node s uses Z-stream slot tz[s].
+template <typename R, int BS>
+__device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
+{
+    const int tid = threadIdx.x;
+    const Key key{M->key0, M->key1};
+    for (int q = tid; q < c->n_nodes; q += BS) {
+        const int s = c->nodes[q];
+        double e0, e1, e2;
+        if (s == c->root) { e0 = e1 = e2 = 0; }
+        else if (c->flags[s] & NF_MISSING) { e0 = log(M->nconstomis); e1 = log(M->constomis); e2 = log(M->nconstomis); }
+        else { e0 = c->E[s * 3]; e1 = c->E[s * 3 + 1]; e2 = c->E[s * 3 + 2]; }
+        const bool fz = c->fixZ[s];
+        double mx = fmax(e0, e1);
+        if (!fz) mx = fmax(mx, e2);
+        c->beta[s * 3 + 0] = exp(e0 - mx);
+        c->beta[s * 3 + 1] = exp(e1 - mx);
+        c->beta[s * 3 + 2] = fz ? 0.0 : exp(e2 - mx);
+        c->changed[s] = 0;
+        c->dirty[s] = 0;
+    }
+    __syncthreads();
+    const int n_up = c->n_up, n_down = c->n_down;
+    for (int h = 0; h < n_up; h++) {
+        const int a = h == 0 ? 0 : c->upoff[h - 1], b = c->upoff[h];
+        for (int q = a + tid; q < b; q += BS) {
+            const int p = c->up[q];
+            double b0 = c->beta[p * 3], b1 = c->beta[p * 3 + 1], b2 = c->beta[p * 3 + 2];
+            for (int w = 0; w < 2; w++) {
+                const int x = w ? c->ch1[p] : c->ch0[p];
+                const LDS double *T = c->fixZ[x] ? c->TMf : c->TMn;
+                const double y0 = c->beta[x * 3], y1 = c->beta[x * 3 + 1], y2 = c->beta[x * 3 + 2];
+                b0 *= T[0] * y0 + T[3] * y1 + T[6] * y2;
+                b1 *= T[1] * y0 + T[4] * y1 + T[7] * y2;
+                b2 *= T[2] * y0 + T[5] * y1 + T[8] * y2;
+            }
+            const double mx = fmax(b0, fmax(b1, b2));
+            if (mx > 0) { b0 /= mx; b1 /= mx; b2 /= mx; }
+            c->beta[p * 3] = b0; c->beta[p * 3 + 1] = b1; c->beta[p * 3 + 2] = b2;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int r = c->root;
+        double pr[3] = {c->beta[r * 3] * 0.5, c->beta[r * 3 + 1] * 0.5, 0.0};
+        const int z = categorical<3>(pr, slot_uniform(key, 0, c2, c3, 0));
+        if (c->Z[r] != z) c->changed[r] = 1;
+        c->Z[r] = (int8_t)z;
+    }
+    __syncthreads();
+    for (int d = 0; d < n_down; d++) {
+        const int a = d == 0 ? 0 : c->downoff[d - 1], b = c->downoff[d];
+        for (int q = a + tid; q < b; q += BS) {
+            const int s = c->down[q];
+            const int zp = c->Z[c->par[s]];
+            const LDS double *T = c->fixZ[s] ? c->TMf : c->TMn;
+            double pr[3] = {T[0 * 3 + zp] * c->beta[s * 3], T[1 * 3 + zp] * c->beta[s * 3 + 1], T[2 * 3 + zp] * c->beta[s * 3 + 2]};
+            const int z = categorical<3>(pr, slot_uniform(key, 0, c2, c3, (uint32_t)c->tz[s]));
+            if (c->Z[s] != z) c->changed[s] = 1;
+            c->Z[s] = (int8_t)z;
+        }
+        __syncthreads();
+    }
+    // dirty = strict ancestors of changed nodes (getUpdateNode, bpp_c.cpp:620-642)
+    const int N = c->N;
+    for (int q = tid; q < c->n_nodes; q += BS) {
+        const int s = c->nodes[q];
+        if (!c->changed[s]) continue;
+        for (int j = c->par[s]; j != N; j = c->par[j]) c->dirty[j] = 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int any = 0;
+        for (int q = 0; q < c->n_nodes; q++) any |= c->changed[c->nodes[q]];
+        c->nch = any;
+    }
+}
+
+// ------------------------------------------------------------- rate update
+
+__device__ __forceinline__ void set_TM(LDS double *F, LDS double *T, double g, double l, double l2)
+{
+    // fixZ nodes (bpp_c.cpp:305-316, 577-585); column 2 keeps log 0 -> 1 (never used)
+    F[0] = 1 - g; F[3] = g; F[6] = 0;
+    F[1] = 0; F[4] = 1; F[7] = 0;
+    F[2] = 1; F[5] = 1; F[8] = 1;
+    T[0] = 1 - g; T[3] = g; T[6] = 0;
+    T[1] = 0; T[4] = 1 - l; T[7] = l;
+    T[2] = 0; T[5] = l2; T[8] = 1 - l2;
+}
+
+// Scalar head of BPP_C::sample_rate (src/PhyloAcc-ST/bpp_c.cpp:930-1041): prior draw
+// when no branch carries the class, else the proposal and its range checks.
+template <typename R>
+__device__ __noinline__ void rate_propose(MP M, CP<R> c, int ms, int m, int resZ, bool neut, double old)
+{
+    const Key key{M->key0, M->key1};
+    const uint32_t c3 = tagword(ms, neut ? TAG_R2 : TAG_R1), c2 = (uint32_t)m;
+    const double pa = neut ? M->nprior_a : M->cprior_a, pb = neut ? M->nprior_b : M->cprior_b;
+    double cur = old;
+    c->skip = 1;
+    c->accept = 0;
+    if (!c->dirty[c->root]) {
+        if (resZ != 0) {
+            Stream sp{key, 2, c2, c3, 0};
+            cur = gamma_draw(sp, pa, pb);
+            if (neut) {
+                if (M->ropt == 1 && cur < M->nlb) cur = old;
+                else if (M->ropt == 2 && cur < c->crate) cur = old;
+            } else {
+                if (M->ropt == 1 && cur > M->cub) cur = old;
+                else if (M->ropt == 2 && cur > c->nrate) cur = old;
+            }
+        }
+    } else {
+        const double r = slot_uniform(key, 0, c2, c3, 0);
+        double prop;
+        int skip = 0;
+        if (neut) {
+            Stream sg{key, 1, c2, c3, 0};
+            prop = gamma_draw(sg, cur / c->prop_n, c->prop_n);
+            if (M->ropt == 1 && prop < M->nlb) skip = 1;
+            else if (M->ropt == 2 && prop < c->crate) skip = 1;
+        } else {
+            const double ru = M->r1_shared ? r : slot_uniform(key, 1, c2, c3, 0);
+            const double u = ru * (c->prop_c - 1 / c->prop_c) + 1 / c->prop_c;
+            prop = cur * u;
+            if (M->ropt == 1 && prop > M->cub) skip = 1;
+            else if (M->ropt == 2 && prop > c->nrate) skip = 1;
+        }
+        if (!skip && (prop < 1e-12 || prop > 1e8)) skip = 1;
+        c->skip = skip;
+        c->proposal = prop;
+        c->r = r;
+    }
+    if (neut) c->nrate = cur; else c->crate = cur;
+}
+
+// Metropolis decision (bpp_c.cpp:1046-1094), FP64.
+template <typename R>
+__device__ __noinline__ void rate_decide(MP M, CP<R> c, bool neut, double old, double lnew)
+{
+    const double pa = neut ? M->nprior_a : M->cprior_a, pb = neut ? M->nprior_b : M->cprior_b;
+    const double prop = c->proposal;
+    double MH;
+    if (neut)
+        MH = lnew - c->loglik + log_gamma_pdf(prop, pa, pb) - log_gamma_pdf(old, pa, pb) +
+             log_gamma_pdf(old, prop / c->prop_n, c->prop_n) - log_gamma_pdf(prop, old / c->prop_n, c->prop_n);
+    else
+        MH = lnew - c->loglik + log_gamma_pdf(prop, pa, pb) - log_gamma_pdf(old, pa, pb) + log(old) - log(prop);
+    if (log(c->r) < MH) {
+        c->accept = 1;
+        c->loglik = lnew;
+        if (neut) c->nrate = prop; else c->crate = prop;
+    }
+}
+
+// Adaptive proposal scale (bpp_c.cpp:1144-1171).
+template <typename R>
+__device__ __noinline__ void rate_adapt(MP M, CP<R> c, int m, bool neut)
+{
+    if ((m + 1) % M->adapt_freq != 0) return;
+    double fac = 1 / sqrt((double)((m + 1) / M->adapt_freq));
+    if (fac > 0.5) fac = 0.5;
+    if (neut) {
+        const double ar = (double)c->accept_n / M->adapt_freq;
+        const double adj = ar > 0.44 ? exp(fac) : (ar < 0.23 ? exp(-fac) : 1);
+        c->prop_n = c->prop_n * adj;
+        if (c->prop_n > 1) c->prop_n = 1;
+        c->accept_n = 0;
+    } else {
+        const double ar = (double)c->accept_c / M->adapt_freq;
+        const double adj = ar > 0.44 ? exp(fac) : (ar < 0.23 ? exp(-fac) : 1);
+        c->prop_c = pow(c->prop_c, adj);
+        if (c->prop_c < 1.0001) c->prop_c = 1.0001;
+        c->accept_c = 0;
+    }
+}
+
+template <typename R, int BS>
+__device__ __noinline__ void build_prop_tables(MP M, CP<R> c, int cls, double prop)
+{
+    for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
+        const int s = c->nodes[q];
+        if (c->Z[s] != cls || (c->flags[s] & NF_MISSING)) continue;
+        const double t = c->dist2[s] * prop;
+        double e[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) e[k] = exp(M->lam[k] * t);
+        LDS R *dst = c->PTp + 16 * s;
+#pragma unroll 1
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                double v = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) v += M->V[j * 4 + k] * e[k] * M->Vi[k * 4 + i];
+                dst[i * 4 + j] = (R)(v > 0 ? v : 0.0);
+            }
+    }
+}
+
+template <typename R, int BS>
+__device__ __noinline__ void mark_class(CP<R> c, int cls)
+{
+    const int tid = threadIdx.x, N = c->N;
+    for (int q = tid; q < c->n_nodes; q += BS) c->dirty[c->nodes[q]] = 0;
+    __syncthreads();
+    for (int q = tid; q < c->n_nodes; q += BS) {
+        const int s = c->nodes[q];
+        if ((c->flags[s] & NF_MISSING) || c->Z[s] != cls) continue;
+        for (int j = c->par[s]; j != N; j = c->par[j]) c->dirty[j] = 1;
+    }
+}
+
+// BPP_C::getUpdateNode(neut) + BPP_C::sample_rate (src/PhyloAcc-ST/bpp_c.cpp:836-858, 930-1176)
+template <typename R, int BS>
+__device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int resZ, bool neut)
+{
+    const int tid = threadIdx.x;
+    const int cls = neut ? 2 : 1;
+    const double old = neut ? c->nrate : c->crate;
+    mark_class<R, BS>(c, cls);
+    __syncthreads();
+    if (tid == 0) rate_propose<R>(M, c, ms, m, resZ, neut, old);
+    __syncthreads();
+    if (!c->skip) {
+        build_prop_tables<R, BS>(M, c, cls, c->proposal);
+        __syncthreads();
+        const double lnew = prune<R, BS>(M, c, PR_PROP, cls);
+        if (tid == 0) rate_decide<R>(M, c, neut, old, lnew);
+        __syncthreads();
+        if (c->accept) {
+            for (int q = tid; q < c->n_int; q += BS) {
+                const int s = c->inodes[q];
+                if (c->dirty[s] && !(c->flags[s] & NF_MISSING)) c->cur[s] ^= 1;
+            }
+        }
+    }
+    const double now = neut ? c->nrate : c->crate;
+    if (now != old) {
+        for (int q = tid; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, cls, c->nodes[q], now, 1);
+        __syncthreads();
+        refresh_PTc<R, BS>(c, cls);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (now != old) { if (neut) c->accept_n += 1; else c->accept_c += 1; }
+        rate_adapt<R>(M, c, m, neut);
+    }
+    __syncthreads();
+}
+
+// Beta draws of BPP_C::sample_transition (src/PhyloAcc-ST/bpp_c.cpp:522-613)
+template <typename R>
+__device__ __noinline__ void transition_draw(MP M, CP<R> c, int ms, int m)
+{
+    const Key key{M->key0, M->key1};
+    const uint32_t c3 = tagword(ms, TAG_TRANS), c2 = (uint32_t)m;
+    double n[9], nf[9];
+    for (int i = 0; i < 9; i++) { n[i] = c->cnt9[i]; nf[i] = n[i] - c->cnt9[9 + i]; }
+    const double g = beta_draw(key, 0, 1, c2, c3, M->ha_g + n[1 * 3 + 0] + n[2 * 3 + 0], M->hb_g + n[0]);
+    const double l = beta_draw(key, 2, 3, c2, c3, M->ha_l + nf[2 * 3 + 1], M->hb_l + nf[1 * 3 + 1]);
+    const double l2 = M->ha_l2 == 0 ? 0.0 : beta_draw(key, 4, 5, c2, c3, M->ha_l2 + nf[1 * 3 + 2], M->hb_l2 + nf[2 * 3 + 2]);
+    c->g = g; c->l = l; c->l2 = l2;
+    set_TM(c->TMf, c->TMn, g, l, l2);
+}
+
+template <typename R, int BS>
+__device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
+{
+    const int tid = threadIdx.x;
+    if (tid < 18) c->cnt9[tid] = 0;
+    __syncthreads();
+    for (int q = tid; q < c->n_nodes - 1; q += BS) {
+        const int s = c->nodes[q];
+        const int zi = c->Z[s], zp = c->Z[c->par[s]];
+        lds_add(&c->cnt9[zi * 3 + zp], 1u);
+        if (c->fixZ[s]) lds_add(&c->cnt9[9 + zi * 3 + zp], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) transition_draw<R>(M, c, ms, m);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------- element setup
+
+template <typename R, int BS>
+__device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int eid)
+{
+    const int tid = threadIdx.x, N = M->N;
+    const ElemDev E = A->elems[eid];
+    if (tid == 0) {
+        c->S = M->S; c->N = N; c->I = M->I;
+        c->GG = E.GG; c->P = E.P; c->Ppad = E.Ppad; c->root = E.root;
+        c->n_nodes = E.n_nodes; c->n_int = E.n_int; c->n_up = E.n_up; c->n_down = E.n_down; c->has_amb = E.has_amb;
+        c->codes = (const GBL uint8_t *)(A->codes + E.off_codes);
+        c->wts = (const GBL int32_t *)(A->wts + E.off_w);
+        c->site = (const GBL int32_t *)(A->site + E.off_site);
+        char *base = A->scratch + (size_t)blockIdx.x * A->slot_bytes;
+        c->parts = (GBL V4<R> *)(base + A->off_parts);
+        c->exps = (GBL int32_t *)(base + A->off_exps);
+        c->plin = (GBL double *)(base + A->off_plin);
+        c->logt = (GBL double *)(base + A->off_logt);
+        c->amb = (GBL uint32_t *)(base + A->off_amb);
+        c->accept_n = 0;
+        c->accept_c = 0;
+        c->bytes = 0;
+    }
+    const int32_t *tab = A->tab + E.off_tab;
+    for (int i = tid; i < N; i += BS) {
+        c->ch0[i] = (int16_t)tab[tab_ch0(N) + i];
+        c->ch1[i] = (int16_t)tab[tab_ch1(N) + i];
+        c->par[i] = (int16_t)tab[tab_par(N) + i];
+        c->flags[i] = (uint8_t)tab[tab_flags(N) + i];
+        c->nodes[i] = (int16_t)tab[tab_nodes(N) + i];
+        c->inodes[i] = (int16_t)tab[tab_int(N) + i];
+        c->up[i] = (int16_t)tab[tab_up(N) + i];
+        c->down[i] = (int16_t)tab[tab_down(N) + i];
+        c->tz[i] = (int16_t)tab[tab_tz(N) + i];
+        c->dist2[i] = ((const double *)(tab + tab_dist(N)))[i];
+        for (int k = 0; k < 16; k++) c->hist[16 * i + k] = 0;
+        c->cur[i] = 0;
+        c->dirty[i] = 0;
+    }
+    for (int i = tid; i <= N; i += BS) {
+        c->upoff[i] = (int16_t)tab[tab_upoff(N) + i];
+        c->downoff[i] = (int16_t)tab[tab_downoff(N) + i];
+    }
+    __syncthreads();
+    for (int i = tid; i < M->S * 24; i += BS) c->amb[i] = 0;
+    // null-rate tables (rate 1), fixed for the element
+    for (int q = tid; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, 0, c->nodes[q], 1.0, 0);
+    __syncthreads();
+}
+
+// ------------------------------------------------------------- the chain
+
+template <typename R, int BS>
+__device__ __noinline__ void init_model(MP M, CP<R> c, int resZ)
+{
+    // BPP_C::initMCMC (src/PhyloAcc-ST/bpp_c.cpp:131-343), iter = 0
+    const int tid = threadIdx.x, N = M->N;
+    for (int i = tid; i < N; i += BS) {
+        c->Z[i] = -1;
+        c->maxZ[i] = 0;
+    }
+    __syncthreads();
+    const int root = c->root;
+    for (int q = tid; q < c->n_nodes; q += BS) {
+        const int s = c->nodes[q];
+        const uint8_t f = c->flags[s];
+        c->Z[s] = (s == root) ? 0 : 1;
+        int fz = (f & NF_UPPER) || s == root;
+        if (resZ == 2 && (f & NF_UPPER_CONS)) fz = 1;
+        if (resZ == 0 && s != root) fz = 1;
+        c->fixZ[s] = (uint8_t)fz;
+        c->cur[s] = 0;
+    }
+    if (tid == 0) {
+        c->prop_c = 1.2;
+        c->prop_n = (double)1 / c->GG * 10;
+        c->max_loglik = -INFINITY;
+        c->max_m = 0;
+        c->nrate = M->init_nrate;
+        c->crate = M->init_crate;
+        c->g = M->init_grate; c->l = M->init_lrate; c->l2 = M->ha_l2 == 0 ? 0.0 : M->init_lrate2;
+        set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
+        c->status = 0;
+    }
+    __syncthreads();
+    const double cr = c->crate, nr = c->nrate;
+    for (int q = tid; q < c->n_nodes - 1; q += BS) {
+        const int s = c->nodes[q];
+        class_table<R>(M, c, 1, s, cr, 0);
+        class_table<R>(M, c, 2, s, nr, 0);
+    }
+    __syncthreads();
+    refresh_PTc<R, BS>(c, -1);
+    __syncthreads();
+}
+
+// BPP_C::MonitorChain (src/PhyloAcc-ST/bpp_c.cpp:741-834), thread 0
+template <typename R>
+__device__ __noinline__ void monitor(MP M, CP<R> c, int m, int resZ, double *trd, int T)
+{
+    double full = c->loglik + log_gamma_pdf(c->crate, M->cprior_a, M->cprior_b);
+    if (resZ != 0) full += log_gamma_pdf(c->nrate, M->nprior_a, M->nprior_b);
+    if (trd) {
+        trd[0 * T + m] = c->loglik; trd[1 * T + m] = full; trd[2 * T + m] = c->nrate;
+        trd[3 * T + m] = c->crate; trd[4 * T + m] = c->g; trd[5 * T + m] = c->l; trd[6 * T + m] = c->l2;
+    }
+    if (m >= M->burnin && full > c->max_loglik) {
+        c->max_loglik = full;
+        c->max_m = m;
+        for (int i = 0; i < M->N; i++) c->maxZ[i] = c->Z[i];
+    }
+    if (isnan(c->loglik)) c->status = PA_STATUS_NUMERIC_FAIL;
+}
+
+template <typename R, int BS>
+__device__ __noinline__ void write_results(MP M, CP<R> c, const ChainArgs *A, int ms, int work)
+{
+    const int tid = threadIdx.x, N = M->N;
+    if (tid == 0) {
+        double *rd = A->res_d + ((size_t)work * 3 + ms) * 8;
+        int32_t *ri = A->res_i + ((size_t)work * 3 + ms) * 4;
+        rd[0] = c->max_loglik; rd[1] = c->nrate; rd[2] = c->crate; rd[3] = c->g; rd[4] = c->l; rd[5] = c->l2;
+        ri[0] = c->max_m; ri[1] = c->status; ri[2] = c->accept_n; ri[3] = c->accept_c;
+    }
+    int8_t *mz = A->maxZ + ((size_t)work * 3 + ms) * N;
+    for (int i = tid; i < N; i += BS) mz[i] = c->maxZ[i];
+}
+
+template <typename R, int BS>
+__device__ __forceinline__ void run_model(MP M, CP<R> c, const ChainArgs *A, int ms, int work)
+{
+    const int tid = threadIdx.x;
+    const int resZ = ms == 0 ? 0 : (ms == 1 ? 2 : 1);
+    const int T = M->T, N = M->N;
+    init_model<R, BS>(M, c, resZ);
+    {
+        const double l0 = prune<R, BS>(M, c, PR_FULL, 0);
+        if (tid == 0) c->loglik = l0;
+        __syncthreads();
+    }
+    double *trd = A->tr_d ? A->tr_d + ((size_t)work * 3 + ms) * 7 * T : nullptr;
+    int8_t *trz = A->tr_Z ? A->tr_Z + ((size_t)work * 3 + ms) * (size_t)T * N : nullptr;
+    for (int m = 0; m < T; m++) {
+        if (trz)
+            for (int i = tid; i < N; i += BS) trz[(size_t)m * N + i] = c->Z[i];
+        if (tid == 0) monitor<R>(M, c, m, resZ, trd, T);
+        if (m == T - 1) break;
+        sample_sites<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_ANC));
+        __syncthreads();
+        emissions<R, BS>(c);
+        __syncthreads();
+        update_Z<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_Z));
+        refresh_PTc<R, BS>(c, -1);
+        __syncthreads();
+        if (c->nch) {
+            const double l = prune<R, BS>(M, c, PR_DIRTY, 0);
+            if (tid == 0) c->loglik = l;
+            __syncthreads();
+        }
+        sample_rate<R, BS>(M, c, ms, m, resZ, true);
+        sample_rate<R, BS>(M, c, ms, m, resZ, false);
+        sample_transition<R, BS>(M, c, ms, m);
+    }
+    __syncthreads();
+    write_results<R, BS>(M, c, A, ms, work);
+    __syncthreads();
+}
+
+__device__ __forceinline__ MP model_to_lds(const ModelDev &Mk, LDS ModelDev *Ms)
+{
+    const uint32_t *src = (const uint32_t *)&Mk;
+    LDS uint32_t *dst = (LDS uint32_t *)Ms;
+    for (int i = threadIdx.x; i < (int)(sizeof(ModelDev) / 4); i += blockDim.x) dst[i] = src[i];
+    return (MP)Ms;
+}
+
+template <typename R, int BS>
+__global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs A)
+{
+    extern __shared__ __align__(16) char smem_raw[];
+    __shared__ ModelDev Ms;
+    __shared__ Ctx<R> Cs;
+    MP M = model_to_lds(Mk, (LDS ModelDev *)&Ms);
+    CP<R> c = (CP<R>)&Cs;
+    __syncthreads();
+    if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0) c->work = atomicAdd(A.queue, 1);
+        __syncthreads();
+        const int w = c->work;
+        __syncthreads();
+        if (w >= A.n_work) break;
+        bind_element<R, BS>(M, c, &A, A.work[w]);
+        for (int ms = 0; ms < 3; ms++) run_model<R, BS>(M, c, &A, ms, w);
+        if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes);
+    }
+}
+
+// Log-likelihood at fixed Z and rates (BPP_C::log_f_Xz, bpp_c2.cpp:392-425)
+template <typename R, int BS>
+__global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs A)
+{
+    extern __shared__ __align__(16) char smem_raw[];
+    __shared__ ModelDev Ms;
+    __shared__ Ctx<R> Cs;
+    MP M = model_to_lds(Mk, (LDS ModelDev *)&Ms);
+    CP<R> c = (CP<R>)&Cs;
+    __syncthreads();
+    if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0) c->work = atomicAdd(A.queue, 1);
+        __syncthreads();
+        const int w = c->work;
+        __syncthreads();
+        if (w >= A.n_work) break;
+        bind_element<R, BS>(M, c, &A, A.work[w]);
+        const double r1 = A.rates_in[2 * w], r2 = A.rates_in[2 * w + 1];
+        for (int i = threadIdx.x; i < M->N; i += BS) c->Z[i] = (int8_t)A.Zin[(size_t)w * M->N + i];
+        for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
+            const int s = c->nodes[q];
+            class_table<R>(M, c, 1, s, r1, 0);
+            class_table<R>(M, c, 2, s, r2, 0);
+        }
+        __syncthreads();
+        refresh_PTc<R, BS>(c, -1);
+        __syncthreads();
+        const double l = prune<R, BS>(M, c, PR_FULL, 0);
+        if (threadIdx.x == 0) A.ll_out[w] = l;
+        __syncthreads();
+    }
+}
+
+__global__ void reset_queue(int *q) { *q = 0; }
+
+template <typename R, int BS>
+static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, int kind, hipStream_t st)
+{
+    const size_t sm = smem_arrays<R, false>(M.N, M.S, BS, nullptr, nullptr);
+    hipError_t err;
+    if (kind == 0) {
+        err = hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL((chain_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
+    } else {
+        err = hipFuncSetAttribute((const void *)loglik_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL((loglik_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(int *q, hipStream_t st)
+{
+    hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_kernel(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, int kind, hipStream_t st)
+{
+    if (fp64) {
+        if (BS == 64) return launch_t<double, 64>(M, A, grid, kind, st);
+        if (BS == 128) return launch_t<double, 128>(M, A, grid, kind, st);
+        if (BS == 256) return launch_t<double, 256>(M, A, grid, kind, st);
+    } else {
+        if (BS == 64) return launch_t<float, 64>(M, A, grid, kind, st);
+        if (BS == 128) return launch_t<float, 128>(M, A, grid, kind, st);
+        if (BS == 256) return launch_t<float, 256>(M, A, grid, kind, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace chain
+}  // namespace pa

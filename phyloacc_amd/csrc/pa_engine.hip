@@ -483,473 +483,6 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
     return block_sum<R>(acc, c);
 }
 
-// ------------------------------------------------------ ancestral sampling
-
-// BPP_C::Update_Tg(tosample=true) (src/PhyloAcc-ST/bpp_c.cpp:672-733) for every
-// site, plus the per-branch transition counts that getEmission
-// (bpp_c2.cpp:355-390) sums over.  Uniform for site g, sampling position t
-// (0 = root, then internal nodes in reverse post-order): ANC stream, c1 = g.
-template <typename R, int BS>
-__device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c3)
-{
-    const int tid = threadIdx.x, S = uni(c->S), n_int = uni(c->n_int), GG = uni(c->GG), Ppad = uni(c->Ppad);
-    const size_t bufI = (size_t)uni(c->I) * Ppad;
-    const int r = uni(c->root);
-    const uint32_t k0 = M->key0, k1 = M->key1;
-    const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
-    const GBL V4<R> *parts = uni_ptr(c->parts);
-    const GBL int32_t *exps = uni_ptr(c->exps);
-    const GBL uint8_t *codes = uni_ptr(c->codes);
-    LDS uint8_t *tgs = c->tgs;
-    for (int i = tid; i < 16 * c->N; i += BS) c->hist[i] = 0;   // shares storage with the proposal matrices
-    __syncthreads();
-    if (tid == 0) {
-        // per site: pattern index, root partial, each non-missing internal node's partial, each leaf code
-        int nint = 0, nleaf = 0;
-        for (int k = 0; k < n_int; k++) nint += !(c->flags[c->inodes[k]] & NF_MISSING);
-        for (int q = 0; q < S; q++) nleaf += (c->flags[q] & NF_IN_TREE) && !(c->flags[q] & NF_MISSING);
-        c->bytes += (unsigned long long)GG * (4 + (4 * sizeof(R) + 4) * nint + nleaf);
-    }
-    for (int g = tid; g < GG; g += BS) {
-        const int p = c->site[g];
-        U4 blk = philox(0, (uint32_t)g, c2, c3, k0, k1);
-        {
-            const size_t off = c->cur[r] * bufI + (size_t)(r - S) * Ppad + p;
-            double pr[4];
-            if (exps[off] == GAPT) {
-                pr[0] = p0; pr[1] = p1; pr[2] = p2; pr[3] = p3;
-            } else {
-                const V4<R> a = ld4<R>(parts + off);
-                pr[0] = p0 * (double)a.x; pr[1] = p1 * (double)a.y; pr[2] = p2 * (double)a.z; pr[3] = p3 * (double)a.w;
-            }
-            tgs[(r - S) * BS + tid] = (uint8_t)categorical<4>(pr, u01(blk.x));
-        }
-        // internal nodes in reverse post-order, K at a time: the partial loads
-        // of a chunk do not depend on the draws, so they are all issued first
-        constexpr int K = sizeof(R) == 4 ? 8 : 4;
-        int t = 1;
-        for (int kb = n_int - 2; kb >= 0; kb -= K) {
-            int ex[K], sn[K];
-            V4<R> av[K];
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int k = kb - j;
-                ex[j] = GAPT;
-                sn[j] = 0;
-                if (k >= 0) {
-                    const int s = uni(c->inodes[k]);
-                    sn[j] = s;
-                    if (!(uni(c->flags[s]) & NF_MISSING)) {
-                        const size_t off = uni(c->cur[s]) * bufI + (size_t)(s - S) * Ppad + p;
-                        ex[j] = exps[off];
-                        av[j] = ld4<R>(parts + off);
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < K; j++, t++) {
-                if (kb - j < 0) break;
-                if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
-                if (ex[j] == GAPT) continue;     // missing node, or an all-gap subtree at this site
-                const int s = sn[j];
-                const int tp = tgs[(c->par[s] - S) * BS + tid];
-                const V4<R> a = av[j];
-                const LDS R *PT = c->PTc + 16 * s;
-                double pr[4] = {(double)PT[0 * 4 + tp] * (double)a.x, (double)PT[1 * 4 + tp] * (double)a.y,
-                                (double)PT[2 * 4 + tp] * (double)a.z, (double)PT[3 * 4 + tp] * (double)a.w};
-                const int ts = categorical<4>(pr, u01(pick(blk, t & 3)));
-                tgs[(s - S) * BS + tid] = (uint8_t)ts;
-                lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
-            }
-        }
-        // leaves, 16 codes loaded at a time
-        for (int sb = 0; sb < S; sb += 16) {
-            int cd[16];
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int s = sb + j;
-                cd[j] = PA_GAP_CODE;
-                if (s < S) {
-                    const uint8_t f = c->flags[s];
-                    if ((f & NF_IN_TREE) && !(f & NF_MISSING)) cd[j] = codes[(size_t)s * Ppad + p];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int s = sb + j, code = cd[j];
-                if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
-                const int tp = tgs[(c->par[s] - S) * BS + tid];
-                if (code < 4) lds_add(&c->hist[s * 16 + tp * 4 + code], 1u);
-                else __hip_atomic_fetch_add(&c->amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-    }
-}
-
-// getEmission (src/PhyloAcc-ST/bpp_c2.cpp:355-390): E[s][k] from the counts.
-template <typename R, int BS>
-__device__ __noinline__ void emissions(CP<R> c)
-{
-    const int N = c->N;
-    for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
-        const int s = c->nodes[q];
-        if (c->flags[s] & NF_MISSING) continue;
-        LDS uint32_t *h = c->hist + s * 16;
-        uint32_t n[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) { n[i] = h[i]; h[i] = 0; }
-#pragma unroll 1
-        for (int k = 0; k < 3; k++) {
-            const GBL double *L = c->logt + ((size_t)k * N + s) * 16;
-            double e = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    if (n[j * 4 + i]) e += (double)n[j * 4 + i] * L[i * 4 + j];
-            c->E[s * 3 + k] = e;
-        }
-        if (c->has_amb && s < c->S) {
-            GBL uint32_t *a = c->amb + s * 24;
-            for (int k = 0; k < 3; k++) {
-                const GBL double *P = c->plin + ((size_t)k * N + s) * 16;
-                double e = 0;
-                for (int j = 0; j < 4; j++)
-                    for (int x = 0; x < 6; x++) {
-                        const uint32_t cnt = a[j * 6 + x];
-                        if (!cnt) continue;
-                        const int code = 6 + x;
-                        const int b1 = (code == 6 || code == 9 || code == 11) ? 0 : (code == 7 || code == 10) ? 1 : 2;
-                        const int b2 = (code == 6 || code == 10) ? 2 : (code == 9) ? 1 : 3;
-                        e += (double)cnt * log(P[j * 4 + b1] + P[j * 4 + b2]);
-                    }
-                c->E[s * 3 + k] += e;
-            }
-            for (int i = 0; i < 24; i++) a[i] = 0;
-        }
-    }
-}
-
-// ----------------------------------------------------------------- Z update
-
-// BPP_C::Update_Z_subtree (src/PhyloAcc-ST/bpp_c.cpp:1182-1293) in linear
-// space, one tree level per step: upward messages by height, then top-down
-// draws by depth.  The draw for node s uses Z-stream slot tz[s].
-template <typename R, int BS>
-__device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
-{
-    const int tid = threadIdx.x;
-    const Key key{M->key0, M->key1};
-    for (int q = tid; q < c->n_nodes; q += BS) {
-        const int s = c->nodes[q];
-        double e0, e1, e2;
-        if (s == c->root) { e0 = e1 = e2 = 0; }
-        else if (c->flags[s] & NF_MISSING) { e0 = log(M->nconstomis); e1 = log(M->constomis); e2 = log(M->nconstomis); }
-        else { e0 = c->E[s * 3]; e1 = c->E[s * 3 + 1]; e2 = c->E[s * 3 + 2]; }
-        const bool fz = c->fixZ[s];
-        double mx = fmax(e0, e1);
-        if (!fz) mx = fmax(mx, e2);
-        c->beta[s * 3 + 0] = exp(e0 - mx);
-        c->beta[s * 3 + 1] = exp(e1 - mx);
-        c->beta[s * 3 + 2] = fz ? 0.0 : exp(e2 - mx);
-        c->changed[s] = 0;
-        c->dirty[s] = 0;
-    }
-    __syncthreads();
-    const int n_up = c->n_up, n_down = c->n_down;
-    for (int h = 0; h < n_up; h++) {
-        const int a = h == 0 ? 0 : c->upoff[h - 1], b = c->upoff[h];
-        for (int q = a + tid; q < b; q += BS) {
-            const int p = c->up[q];
-            double b0 = c->beta[p * 3], b1 = c->beta[p * 3 + 1], b2 = c->beta[p * 3 + 2];
-            for (int w = 0; w < 2; w++) {
-                const int x = w ? c->ch1[p] : c->ch0[p];
-                const LDS double *T = c->fixZ[x] ? c->TMf : c->TMn;
-                const double y0 = c->beta[x * 3], y1 = c->beta[x * 3 + 1], y2 = c->beta[x * 3 + 2];
-                b0 *= T[0] * y0 + T[3] * y1 + T[6] * y2;
-                b1 *= T[1] * y0 + T[4] * y1 + T[7] * y2;
-                b2 *= T[2] * y0 + T[5] * y1 + T[8] * y2;
-            }
-            const double mx = fmax(b0, fmax(b1, b2));
-            if (mx > 0) { b0 /= mx; b1 /= mx; b2 /= mx; }
-            c->beta[p * 3] = b0; c->beta[p * 3 + 1] = b1; c->beta[p * 3 + 2] = b2;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        const int r = c->root;
-        double pr[3] = {c->beta[r * 3] * 0.5, c->beta[r * 3 + 1] * 0.5, 0.0};
-        const int z = categorical<3>(pr, slot_uniform(key, 0, c2, c3, 0));
-        if (c->Z[r] != z) c->changed[r] = 1;
-        c->Z[r] = (int8_t)z;
-    }
-    __syncthreads();
-    for (int d = 0; d < n_down; d++) {
-        const int a = d == 0 ? 0 : c->downoff[d - 1], b = c->downoff[d];
-        for (int q = a + tid; q < b; q += BS) {
-            const int s = c->down[q];
-            const int zp = c->Z[c->par[s]];
-            const LDS double *T = c->fixZ[s] ? c->TMf : c->TMn;
-            double pr[3] = {T[0 * 3 + zp] * c->beta[s * 3], T[1 * 3 + zp] * c->beta[s * 3 + 1], T[2 * 3 + zp] * c->beta[s * 3 + 2]};
-            const int z = categorical<3>(pr, slot_uniform(key, 0, c2, c3, (uint32_t)c->tz[s]));
-            if (c->Z[s] != z) c->changed[s] = 1;
-            c->Z[s] = (int8_t)z;
-        }
-        __syncthreads();
-    }
-    // dirty = strict ancestors of changed nodes (getUpdateNode, bpp_c.cpp:620-642)
-    const int N = c->N;
-    for (int q = tid; q < c->n_nodes; q += BS) {
-        const int s = c->nodes[q];
-        if (!c->changed[s]) continue;
-        for (int j = c->par[s]; j != N; j = c->par[j]) c->dirty[j] = 1;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int any = 0;
-        for (int q = 0; q < c->n_nodes; q++) any |= c->changed[c->nodes[q]];
-        c->nch = any;
-    }
-}
-
-// ------------------------------------------------------------- rate update
-
-__device__ __forceinline__ void set_TM(LDS double *F, LDS double *T, double g, double l, double l2)
-{
-    // fixZ nodes (bpp_c.cpp:305-316, 577-585); column 2 keeps log 0 -> 1 (never used)
-    F[0] = 1 - g; F[3] = g; F[6] = 0;
-    F[1] = 0; F[4] = 1; F[7] = 0;
-    F[2] = 1; F[5] = 1; F[8] = 1;
-    T[0] = 1 - g; T[3] = g; T[6] = 0;
-    T[1] = 0; T[4] = 1 - l; T[7] = l;
-    T[2] = 0; T[5] = l2; T[8] = 1 - l2;
-}
-
-// Scalar head of BPP_C::sample_rate (src/PhyloAcc-ST/bpp_c.cpp:930-1041): prior draw
-// when no branch carries the class, else the proposal and its range checks.
-template <typename R>
-__device__ __noinline__ void rate_propose(MP M, CP<R> c, int ms, int m, int resZ, bool neut, double old)
-{
-    const Key key{M->key0, M->key1};
-    const uint32_t c3 = tagword(ms, neut ? TAG_R2 : TAG_R1), c2 = (uint32_t)m;
-    const double pa = neut ? M->nprior_a : M->cprior_a, pb = neut ? M->nprior_b : M->cprior_b;
-    double cur = old;
-    c->skip = 1;
-    c->accept = 0;
-    if (!c->dirty[c->root]) {
-        if (resZ != 0) {
-            Stream sp{key, 2, c2, c3, 0};
-            cur = gamma_draw(sp, pa, pb);
-            if (neut) {
-                if (M->ropt == 1 && cur < M->nlb) cur = old;
-                else if (M->ropt == 2 && cur < c->crate) cur = old;
-            } else {
-                if (M->ropt == 1 && cur > M->cub) cur = old;
-                else if (M->ropt == 2 && cur > c->nrate) cur = old;
-            }
-        }
-    } else {
-        const double r = slot_uniform(key, 0, c2, c3, 0);
-        double prop;
-        int skip = 0;
-        if (neut) {
-            Stream sg{key, 1, c2, c3, 0};
-            prop = gamma_draw(sg, cur / c->prop_n, c->prop_n);
-            if (M->ropt == 1 && prop < M->nlb) skip = 1;
-            else if (M->ropt == 2 && prop < c->crate) skip = 1;
-        } else {
-            const double ru = M->r1_shared ? r : slot_uniform(key, 1, c2, c3, 0);
-            const double u = ru * (c->prop_c - 1 / c->prop_c) + 1 / c->prop_c;
-            prop = cur * u;
-            if (M->ropt == 1 && prop > M->cub) skip = 1;
-            else if (M->ropt == 2 && prop > c->nrate) skip = 1;
-        }
-        if (!skip && (prop < 1e-12 || prop > 1e8)) skip = 1;
-        c->skip = skip;
-        c->proposal = prop;
-        c->r = r;
-    }
-    if (neut) c->nrate = cur; else c->crate = cur;
-}
-
-// The four density terms of the Metropolis ratio (bpp_c.cpp:1046-1047), one
-// per lane k = 0..3, into red[k]: prior at the proposal, prior at the
-// current value, then the proposal's Hastings terms (r2: reverse and forward
-// Gamma proposal densities; r1: log current, log proposal).
-template <typename R>
-__device__ __noinline__ void decide_term(MP M, CP<R> c, bool neut, double old, int k)
-{
-    const double pa = neut ? M->nprior_a : M->cprior_a, pb = neut ? M->nprior_b : M->cprior_b;
-    const double prop = c->proposal;
-    double v;
-    switch (k) {
-    case 0: v = log_gamma_pdf(prop, pa, pb); break;
-    case 1: v = log_gamma_pdf(old, pa, pb); break;
-    case 2: v = neut ? log_gamma_pdf(old, prop / c->prop_n, c->prop_n) : c_log(old); break;
-    default: v = neut ? log_gamma_pdf(prop, old / c->prop_n, c->prop_n) : c_log(prop); break;
-    }
-    c->red[k] = v;
-}
-
-// Metropolis decision (bpp_c.cpp:1046-1094), FP64, from the terms in red[0..3].
-template <typename R>
-__device__ __noinline__ void rate_decide(MP M, CP<R> c, bool neut, double old, double lnew)
-{
-    const double prop = c->proposal;
-    const double MH = lnew - c->loglik + c->red[0] - c->red[1] + c->red[2] - c->red[3];
-    if (log(c->r) < MH) {
-        c->accept = 1;
-        c->loglik = lnew;
-        if (neut) c->nrate = prop; else c->crate = prop;
-    }
-}
-
-// Adaptive proposal scale (bpp_c.cpp:1144-1171).
-template <typename R>
-__device__ __noinline__ void rate_adapt(MP M, CP<R> c, int m, bool neut)
-{
-    if ((m + 1) % M->adapt_freq != 0) return;
-    double fac = 1 / sqrt((double)((m + 1) / M->adapt_freq));
-    if (fac > 0.5) fac = 0.5;
-    if (neut) {
-        const double ar = (double)c->accept_n / M->adapt_freq;
-        const double adj = ar > 0.44 ? exp(fac) : (ar < 0.23 ? exp(-fac) : 1);
-        c->prop_n = c->prop_n * adj;
-        if (c->prop_n > 1) c->prop_n = 1;
-        c->accept_n = 0;
-    } else {
-        const double ar = (double)c->accept_c / M->adapt_freq;
-        const double adj = ar > 0.44 ? exp(fac) : (ar < 0.23 ? exp(-fac) : 1);
-        c->prop_c = pow(c->prop_c, adj);
-        if (c->prop_c < 1.0001) c->prop_c = 1.0001;
-        c->accept_c = 0;
-    }
-}
-
-template <typename R, int BS>
-__device__ __noinline__ void build_prop_tables(MP M, CP<R> c, int cls, double prop)
-{
-    for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
-        const int s = c->nodes[q];
-        if (c->Z[s] != cls || (c->flags[s] & NF_MISSING)) continue;
-        const double t = c->dist2[s] * prop;
-        double e[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) e[k] = exp(M->lam[k] * t);
-        LDS R *dst = c->PTp + 16 * s;
-#pragma unroll 1
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                double v = 0;
-#pragma unroll
-                for (int k = 0; k < 4; k++) v += M->V[j * 4 + k] * e[k] * M->Vi[k * 4 + i];
-                dst[i * 4 + j] = (R)(v > 0 ? v : 0.0);
-            }
-    }
-}
-
-template <typename R, int BS>
-__device__ __noinline__ void mark_class(CP<R> c, int cls)
-{
-    const int tid = threadIdx.x, N = c->N;
-    for (int q = tid; q < c->n_nodes; q += BS) c->dirty[c->nodes[q]] = 0;
-    __syncthreads();
-    for (int q = tid; q < c->n_nodes; q += BS) {
-        const int s = c->nodes[q];
-        if ((c->flags[s] & NF_MISSING) || c->Z[s] != cls) continue;
-        for (int j = c->par[s]; j != N; j = c->par[j]) c->dirty[j] = 1;
-    }
-}
-
-// BPP_C::getUpdateNode(neut) + BPP_C::sample_rate (src/PhyloAcc-ST/bpp_c.cpp:836-858, 930-1176)
-template <typename R, int BS>
-__device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int resZ, bool neut)
-{
-    const int tid = threadIdx.x;
-    const int cls = neut ? 2 : 1;
-    const double old = neut ? c->nrate : c->crate;
-    mark_class<R, BS>(c, cls);
-    __syncthreads();
-    if (tid == 0) rate_propose<R>(M, c, ms, m, resZ, neut, old);
-    __syncthreads();
-    if (!c->skip) {
-        build_prop_tables<R, BS>(M, c, cls, c->proposal);
-        __syncthreads();
-        const double lnew = prune<R, BS>(M, c, PR_PROP, cls);
-        if (tid < 4) decide_term<R>(M, c, neut, old, tid);
-        __syncthreads();
-        if (tid == 0) rate_decide<R>(M, c, neut, old, lnew);
-        __syncthreads();
-        if (c->accept) {
-            for (int q = tid; q < c->n_int; q += BS) {
-                const int s = c->inodes[q];
-                if (c->dirty[s] && !(c->flags[s] & NF_MISSING)) c->cur[s] ^= 1;
-            }
-        }
-    }
-    const double now = neut ? c->nrate : c->crate;
-    if (now != old) {
-        for (int q = tid; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, cls, c->nodes[q], now, 1);
-        __syncthreads();
-        refresh_PTc<R, BS>(c, cls);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        if (now != old) { if (neut) c->accept_n += 1; else c->accept_c += 1; }
-        rate_adapt<R>(M, c, m, neut);
-    }
-    __syncthreads();
-}
-
-// Beta draws of BPP_C::sample_transition (src/PhyloAcc-ST/bpp_c.cpp:522-613).
-// Each Beta(a, b) is X / (X + Y) with X ~ Gamma(a), Y ~ Gamma(b) on their own
-// sub-streams (g: 0/1, l: 2/3, l2: 4/5), so the six Gamma draws are made by
-// six lanes at once; lane k leaves its draw in red[k].
-template <typename R>
-__device__ __noinline__ void transition_gamma(MP M, CP<R> c, int ms, int m, int k)
-{
-    // counts n(child, parent) in cnt9[child * 3 + parent]; the fixZ-free
-    // counts subtract cnt9[9 + ...]
-    auto n = [&](int i) { return (double)c->cnt9[i]; };
-    auto nf = [&](int i) { return (double)c->cnt9[i] - (double)c->cnt9[9 + i]; };
-    double shape;
-    switch (k) {
-    case 0: shape = M->ha_g + n(1 * 3 + 0) + n(2 * 3 + 0); break;
-    case 1: shape = M->hb_g + n(0); break;
-    case 2: shape = M->ha_l + nf(2 * 3 + 1); break;
-    case 3: shape = M->hb_l + nf(1 * 3 + 1); break;
-    case 4: shape = M->ha_l2 + nf(1 * 3 + 2); break;
-    default: shape = M->hb_l2 + nf(2 * 3 + 2); break;
-    }
-    Stream st{Key{M->key0, M->key1}, (uint32_t)k, (uint32_t)m, tagword(ms, TAG_TRANS), 0};
-    c->red[k] = gamma_draw(st, shape, 1.0);
-}
-
-template <typename R, int BS>
-__device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
-{
-    const int tid = threadIdx.x;
-    if (tid < 18) c->cnt9[tid] = 0;
-    __syncthreads();
-    for (int q = tid; q < c->n_nodes - 1; q += BS) {
-        const int s = c->nodes[q];
-        const int zi = c->Z[s], zp = c->Z[c->par[s]];
-        lds_add(&c->cnt9[zi * 3 + zp], 1u);
-        if (c->fixZ[s]) lds_add(&c->cnt9[9 + zi * 3 + zp], 1u);
-    }
-    __syncthreads();
-    if (tid < (M->ha_l2 == 0 ? 4 : 6)) transition_gamma<R>(M, c, ms, m, tid);
-    __syncthreads();
-    if (tid == 0) {
-        c->g = c->red[0] / (c->red[0] + c->red[1]);
-        c->l = c->red[2] / (c->red[2] + c->red[3]);
-        c->l2 = M->ha_l2 == 0 ? 0.0 : c->red[4] / (c->red[4] + c->red[5]);
-        set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
-    }
-    __syncthreads();
-}
-
 // ---------------------------------------------------------- element setup
 
 template <typename R, int BS>
@@ -1003,211 +536,6 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int eid, int rec)
     __syncthreads();
 }
 
-// ------------------------------------------------------------- the chain
-
-template <typename R, int BS>
-__device__ __noinline__ void init_model(MP M, CP<R> c, int resZ)
-{
-    // BPP_C::initMCMC (src/PhyloAcc-ST/bpp_c.cpp:131-343), iter = 0
-    const int tid = threadIdx.x, N = M->N;
-    for (int i = tid; i < N; i += BS) {
-        c->Z[i] = -1;
-        c->maxZ[i] = 0;
-    }
-    __syncthreads();
-    const int root = c->root;
-    for (int q = tid; q < c->n_nodes; q += BS) {
-        const int s = c->nodes[q];
-        const uint8_t f = c->flags[s];
-        c->Z[s] = (s == root) ? 0 : 1;
-        int fz = (f & NF_UPPER) || s == root;
-        if (resZ == 2 && (f & NF_UPPER_CONS)) fz = 1;
-        if (resZ == 0 && s != root) fz = 1;
-        c->fixZ[s] = (uint8_t)fz;
-        c->cur[s] = 0;
-    }
-    if (tid == 0) {
-        c->prop_c = 1.2;
-        c->prop_n = (double)1 / c->GG * 10;
-        c->max_loglik = -INFINITY;
-        c->max_m = 0;
-        c->nrate = M->init_nrate;
-        c->crate = M->init_crate;
-        c->g = M->init_grate; c->l = M->init_lrate; c->l2 = M->ha_l2 == 0 ? 0.0 : M->init_lrate2;
-        set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
-        c->status = 0;
-    }
-    __syncthreads();
-    const double cr = c->crate, nr = c->nrate;
-    for (int q = tid; q < c->n_nodes - 1; q += BS) {
-        const int s = c->nodes[q];
-        class_table<R>(M, c, 1, s, cr, 0);
-        class_table<R>(M, c, 2, s, nr, 0);
-    }
-    __syncthreads();
-    refresh_PTc<R, BS>(c, -1);
-    __syncthreads();
-}
-
-// BPP_C::MonitorChain (src/PhyloAcc-ST/bpp_c.cpp:741-834), thread 0
-template <typename R>
-__device__ __noinline__ void monitor_term(MP M, CP<R> c, int k)
-{
-    // prior densities of the current rates (lane 0: r1, lane 1: r2) into red[8 + k]
-    c->red[8 + k] = k == 0 ? log_gamma_pdf(c->crate, M->cprior_a, M->cprior_b)
-                           : log_gamma_pdf(c->nrate, M->nprior_a, M->nprior_b);
-}
-
-template <typename R>
-__device__ __noinline__ void monitor(MP M, CP<R> c, int m, int resZ, double *trd, int T)
-{
-    double full = c->loglik + c->red[8];
-    if (resZ != 0) full += c->red[9];
-    if (trd) {
-        trd[0 * T + m] = c->loglik; trd[1 * T + m] = full; trd[2 * T + m] = c->nrate;
-        trd[3 * T + m] = c->crate; trd[4 * T + m] = c->g; trd[5 * T + m] = c->l; trd[6 * T + m] = c->l2;
-    }
-    if (m >= M->burnin && full > c->max_loglik) {
-        c->max_loglik = full;
-        c->max_m = m;
-        for (int i = 0; i < M->N; i++) c->maxZ[i] = c->Z[i];
-    }
-    if (isnan(c->loglik)) c->status = PA_STATUS_NUMERIC_FAIL;
-}
-
-template <typename R, int BS>
-__device__ __noinline__ void write_results(MP M, CP<R> c, AP A, int ms, int rec)
-{
-    const int tid = threadIdx.x, N = M->N;
-    if (tid == 0) {
-        double *rd = A->res_d + ((size_t)rec * 3 + ms) * 8;
-        int32_t *ri = A->res_i + ((size_t)rec * 3 + ms) * 4;
-        rd[0] = c->max_loglik; rd[1] = c->nrate; rd[2] = c->crate; rd[3] = c->g; rd[4] = c->l; rd[5] = c->l2;
-        ri[0] = c->max_m; ri[1] = c->status; ri[2] = c->accept_n; ri[3] = c->accept_c;
-    }
-    int8_t *mz = A->maxZ + ((size_t)rec * 3 + ms) * N;
-    for (int i = tid; i < N; i += BS) mz[i] = c->maxZ[i];
-}
-
-// Chain state across launches: everything a sweep hands to the next one
-// besides the partials and class tables, which stay in the scratch slot.
-template <typename R, int BS>
-__device__ __noinline__ void save_state(MP M, CP<R> c, GBL ChainState *st, int ms, int m)
-{
-    const int tid = threadIdx.x, N = M->N;
-    if (tid == 0) {
-        st->ms = ms; st->m = m;
-        st->accept_n = c->accept_n; st->accept_c = c->accept_c; st->max_m = c->max_m; st->status = c->status;
-        st->loglik = c->loglik; st->nrate = c->nrate; st->crate = c->crate;
-        st->g = c->g; st->l = c->l; st->l2 = c->l2;
-        st->prop_n = c->prop_n; st->prop_c = c->prop_c; st->max_loglik = c->max_loglik;
-        st->bytes = c->bytes;
-    }
-    for (int i = tid; i < N; i += BS) {
-        st->Z[i] = c->Z[i]; st->maxZ[i] = c->maxZ[i]; st->cur[i] = c->cur[i]; st->fixZ[i] = c->fixZ[i];
-    }
-}
-
-template <typename R, int BS>
-__device__ __noinline__ void load_state(MP M, CP<R> c, const GBL ChainState *st)
-{
-    const int tid = threadIdx.x, N = M->N;
-    if (tid == 0) {
-        c->accept_n = st->accept_n; c->accept_c = st->accept_c; c->max_m = st->max_m; c->status = st->status;
-        c->loglik = st->loglik; c->nrate = st->nrate; c->crate = st->crate;
-        c->g = st->g; c->l = st->l; c->l2 = st->l2;
-        c->prop_n = st->prop_n; c->prop_c = st->prop_c; c->max_loglik = st->max_loglik;
-        c->bytes = st->bytes;
-        set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
-    }
-    for (int i = tid; i < N; i += BS) {
-        c->Z[i] = st->Z[i]; c->maxZ[i] = st->maxZ[i]; c->cur[i] = st->cur[i]; c->fixZ[i] = st->fixZ[i];
-    }
-    __syncthreads();
-    refresh_PTc<R, BS>(c, -1);
-    __syncthreads();
-}
-
-// One model chain from iteration m0 (-1: from initMCMC).  Spends at most
-// *budget sweeps; returns false, with the state saved, when the budget ends
-// first.
-template <typename R, int BS>
-__device__ __forceinline__ bool run_model(MP M, CP<R> c, AP A, int ms, int rec, int m0, int *budget)
-{
-    const int tid = threadIdx.x;
-    const int resZ = ms == 0 ? 0 : (ms == 1 ? 2 : 1);
-    const int T = M->T, N = M->N;
-    GBL ChainState *st = A->state ? (GBL ChainState *)(A->state + rec) : nullptr;
-    if (m0 < 0) {
-        init_model<R, BS>(M, c, resZ);
-        const double l0 = prune<R, BS>(M, c, PR_FULL, 0);
-        if (tid == 0) c->loglik = l0;
-        __syncthreads();
-        m0 = 0;
-    } else {
-        load_state<R, BS>(M, c, st);
-    }
-    double *trd = A->tr_d ? A->tr_d + ((size_t)rec * 3 + ms) * 7 * T : nullptr;
-    int8_t *trz = A->tr_Z ? A->tr_Z + ((size_t)rec * 3 + ms) * (size_t)T * N : nullptr;
-    for (int m = m0; m < T; m++) {
-        if (*budget == 0) {
-            save_state<R, BS>(M, c, st, ms, m);
-            return false;
-        }
-        if (trz)
-            for (int i = tid; i < N; i += BS) trz[(size_t)m * N + i] = c->Z[i];
-        if (tid < (resZ != 0 ? 2 : 1)) monitor_term<R>(M, c, tid);
-        __syncthreads();
-        if (tid == 0) monitor<R>(M, c, m, resZ, trd, T);
-        if (m == T - 1) break;
-        --*budget;
-        sample_sites<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_ANC));
-        __syncthreads();
-        emissions<R, BS>(c);
-        __syncthreads();
-        update_Z<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_Z));
-        refresh_PTc<R, BS>(c, -1);
-        __syncthreads();
-        if (c->nch) {
-            const double l = prune<R, BS>(M, c, PR_DIRTY, 0);
-            if (tid == 0) c->loglik = l;
-            __syncthreads();
-        }
-        sample_rate<R, BS>(M, c, ms, m, resZ, true);
-        sample_rate<R, BS>(M, c, ms, m, resZ, false);
-        sample_transition<R, BS>(M, c, ms, m);
-    }
-    __syncthreads();
-    write_results<R, BS>(M, c, A, ms, rec);
-    __syncthreads();
-    return true;
-}
-
-// The three model chains of one element (main.cpp:466-494), resumed from
-// the record's state in resumable runs.
-template <typename R, int BS>
-__device__ __noinline__ void run_element(MP M, CP<R> c, AP A, int rec)
-{
-    int ms = 0, m0 = -1, budget = A->state ? A->sweep_budget : 0x7fffffff;
-    if (A->state) {
-        ms = A->state[rec].ms;
-        m0 = A->state[rec].m;
-    }
-    if (ms == 0 && m0 < 0 && threadIdx.x == 0) {
-        c->accept_n = 0;
-        c->accept_c = 0;
-        c->bytes = 0;
-    }
-    __syncthreads();
-    for (; ms < 3; ms++, m0 = -1)
-        if (!run_model<R, BS>(M, c, A, ms, rec, m0, &budget)) return;
-    if (A->state && threadIdx.x == 0) {
-        A->state[rec].ms = 3;
-        A->state[rec].bytes = c->bytes;
-        A->done[rec] = 1;
-    }
-}
-
 // Kernel arguments are copied to LDS once per workgroup: the device
 // functions take LDS pointers to them, which keeps the argument blocks out of
 // the private (scratch) segment.
@@ -1219,35 +547,6 @@ __device__ __forceinline__ LDS const T *to_lds(const T &k, LDS T *dst)
     LDS uint32_t *d = (LDS uint32_t *)dst;
     for (int i = threadIdx.x; i < (int)(sizeof(T) / 4); i += blockDim.x) d[i] = src[i];
     return (LDS const T *)dst;
-}
-
-template <typename R, int BS>
-__global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs Ak)
-{
-    extern __shared__ __align__(16) char smem_raw[];
-    __shared__ ModelDev Ms;
-    __shared__ ChainArgs As;
-    __shared__ Ctx<R> Cs;
-    MP M = to_lds(Mk, (LDS ModelDev *)&Ms);
-    AP A = to_lds(Ak, (LDS ChainArgs *)&As);
-    CP<R> c = (CP<R>)&Cs;
-    __syncthreads();
-    if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
-    __syncthreads();
-    const int n_work = A->n_work_dev ? *A->n_work_dev : A->n_work;
-    for (;;) {
-        if (threadIdx.x == 0) c->work = atomicAdd(A->queue, 1);
-        __syncthreads();
-        const int w = c->work;
-        __syncthreads();
-        if (w >= n_work) break;
-        const int rec = A->rec ? A->rec[w] : w;
-        if (A->state && A->state[rec].ms >= 3) continue;
-        const unsigned long long b0 = A->state && A->state[rec].m >= 0 ? A->state[rec].bytes : 0ull;
-        bind_element<R, BS>(M, c, A, A->work[w], rec);
-        run_element<R, BS>(M, c, A, rec);
-        if (threadIdx.x == 0 && A->counters) atomicAdd(A->counters, c->bytes - b0);
-    }
 }
 
 // Log-likelihood at fixed Z and rates (BPP_C::log_f_Xz, bpp_c2.cpp:392-425)
@@ -1295,33 +594,16 @@ __global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs Ak)
 
 __global__ void reset_queue(int *q) { *q = 0; }
 
-__global__ void serve_admit(const EpochDesc *d, const int32_t *fresh, const int64_t *fresh_off, ChainState *state,
-                            int32_t *done, int64_t *slot_off, int *queue)
-{
-    const int nf = d->n_fresh;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
-        const int r = fresh[i];
-        state[r].ms = 0;
-        state[r].m = -1;
-        done[r] = 0;
-        slot_off[r] = fresh_off[i];
-    }
-    if (threadIdx.x == 0) *queue = 0;
-}
-
 template <typename R, int BS>
-static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, int kind, hipStream_t st)
+static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, hipStream_t st)
 {
-    // dynamic LDS above 64 KB needs the per-kernel limit raised (large trees
-    // only); the call is never made while a stream is being captured otherwise
+    // dynamic LDS above 64 KB needs the per-kernel limit raised (large trees only)
     const size_t sm = smem_arrays<R, false>(M.N, M.S, BS, nullptr, nullptr);
-    const void *fn = kind == 0 ? (const void *)chain_kernel<R, BS> : (const void *)loglik_kernel<R, BS>;
     if (sm > 65536) {
-        const hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        const hipError_t err = hipFuncSetAttribute((const void *)loglik_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         if (err != hipSuccess) return err;
     }
-    if (kind == 0) hipLaunchKernelGGL((chain_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
-    else hipLaunchKernelGGL((loglik_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
+    hipLaunchKernelGGL((loglik_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
     return hipGetLastError();
 }
 
@@ -1331,23 +613,16 @@ hipError_t launch_reset(int *q, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_admit(const EpochDesc *d, const int32_t *fresh, const int64_t *fresh_off, ChainState *state,
-                        int32_t *done, int64_t *slot_off, int *queue, hipStream_t st)
-{
-    hipLaunchKernelGGL(serve_admit, dim3(1), dim3(256), 0, st, d, fresh, fresh_off, state, done, slot_off, queue);
-    return hipGetLastError();
-}
-
-hipError_t launch_kernel(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, int kind, hipStream_t st)
+hipError_t launch_loglik(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st)
 {
     if (fp64) {
-        if (BS == 64) return launch_t<double, 64>(M, A, grid, kind, st);
-        if (BS == 128) return launch_t<double, 128>(M, A, grid, kind, st);
-        if (BS == 256) return launch_t<double, 256>(M, A, grid, kind, st);
+        if (BS == 64) return launch_t<double, 64>(M, A, grid, st);
+        if (BS == 128) return launch_t<double, 128>(M, A, grid, st);
+        if (BS == 256) return launch_t<double, 256>(M, A, grid, st);
     } else {
-        if (BS == 64) return launch_t<float, 64>(M, A, grid, kind, st);
-        if (BS == 128) return launch_t<float, 128>(M, A, grid, kind, st);
-        if (BS == 256) return launch_t<float, 256>(M, A, grid, kind, st);
+        if (BS == 64) return launch_t<float, 64>(M, A, grid, st);
+        if (BS == 128) return launch_t<float, 128>(M, A, grid, st);
+        if (BS == 256) return launch_t<float, 256>(M, A, grid, st);
     }
     return hipErrorInvalidValue;
 }

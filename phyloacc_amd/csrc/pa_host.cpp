@@ -20,6 +20,7 @@
 #include "../../include/phyloacc_gpu.h"
 #include "pa_common.h"
 #include "pa_kernels.h"
+#include "pa_chain.h"
 #include "pa_rng.h"
 
 using namespace pa;
@@ -511,10 +512,9 @@ struct pa_engine {
     uint8_t *d_codes = nullptr;
     int32_t *d_w = nullptr, *d_site = nullptr, *d_tab = nullptr;
     int *d_queue = nullptr;
-    hipStream_t stream = nullptr, stream2 = nullptr;   // stream2: second size class of a batched launch
+    hipStream_t stream = nullptr;
     int n_cu = 0;
     size_t lds_per_cu = 0;
-    pa_tuning tune = {64, 256, 256, 0.0};   // size classes of batched chain launches
     uint64_t arena_gen = 0;                  // bumped whenever the device arenas move
 };
 
@@ -637,7 +637,6 @@ static pa_status ensure_device(pa_engine *e)
     e->n_cu = prop.multiProcessorCount;
     e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
     HIPCHK(e, hipStreamCreate(&e->stream));
-    HIPCHK(e, hipStreamCreate(&e->stream2));
     HIPCHK(e, hipMalloc(&e->d_queue, 256));
     return PA_OK;
 }
@@ -658,7 +657,6 @@ extern "C" void pa_engine_destroy(pa_engine *e)
         free_arenas(e);
         (void)hipFree(e->d_queue);
         (void)hipStreamDestroy(e->stream);
-        (void)hipStreamDestroy(e->stream2);
     }
     delete e;
 }
@@ -824,129 +822,101 @@ struct DevBuf {
 
 }  // namespace
 
-// Outputs of a launch that starts at work position i of the whole list.
-static ChainArgs shifted(const ChainArgs &A, int i, int N, int T)
+// Block size of a batched launch from the elements' mean pattern count.
+static int pick_block(pa_engine *e, const int32_t *work, int32_t n)
 {
-    ChainArgs B = A;
-    B.work = A.work + i;
-    B.res_d = A.res_d ? A.res_d + (size_t)i * 24 : nullptr;
-    B.res_i = A.res_i ? A.res_i + (size_t)i * 12 : nullptr;
-    B.maxZ = A.maxZ ? A.maxZ + (size_t)i * 3 * N : nullptr;
-    B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * T : nullptr;
-    B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * T * N : nullptr;
-    B.Zin = A.Zin ? A.Zin + (size_t)i * N : nullptr;
-    B.rates_in = A.rates_in ? A.rates_in + (size_t)i * 2 : nullptr;
-    B.ll_out = A.ll_out ? A.ll_out + i : nullptr;
-    B.len_in = A.len_in ? A.len_in + (size_t)i * N : nullptr;
-    return B;
-}
-
-// Workgroups of block size BS that fit on one CU at once (LDS and waves).
-static int fit_per_cu(pa_engine *e, int N, int S, int BS, int fp64)
-{
-    const size_t smem = chain_smem_bytes(N, S, BS, fp64);
-    int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
-    return std::max(1, std::min(per_cu, 2048 / BS));
-}
-
-// Common launch path for the chain and log-likelihood kernels.  ids arrive
-// in schedule order (largest first for batched runs).  With bs_req == 0 a
-// batched chain run splits into two size classes launched side by side on
-// two streams: elements with more than tune.long_patterns patterns on
-// tune.long_threads-thread workgroups, the rest on tune.small_threads-thread
-// ones, each class given a share of the GPU's wave slots proportional to its
-// pattern count.  Long elements would otherwise set the end of the launch
-// (element lengths are heavy-tailed).
-static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, int kind, int mode, int bs_req,
-                            ChainArgs &A, DevBuf *bufs, pa_run_stats *stats)
-{
-    const int S = e->tm.S, N = e->tm.N, T = e->M.T;
-    std::vector<int32_t> work(ids, ids + n);
-    if (bs_req != 0 && bs_req != 64 && bs_req != 128 && bs_req != 256)
-        return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
-    struct Cls { int first, count, BS, grid, Pcap; size_t slot; };
-    std::vector<Cls> cls;
     double meanP = 0;
     for (int i = 0; i < n; i++) meanP += e->elems[work[i]].P;
     meanP /= std::max(1, n);
-    if (mode == PA_RUN_PER_ELEMENT) {
-        cls.push_back({0, n, bs_req ? bs_req : 64, 1, 64, 0});
-    } else if (bs_req != 0 || kind != 0) {
-        int BS = bs_req ? bs_req : ((meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256));
-        cls.push_back({0, n, BS, std::min(n, e->n_cu * fit_per_cu(e, N, S, BS, fp64)), 64, 0});
-    } else {
-        const int bsl = e->tune.long_threads, bss = e->tune.small_threads;
-        int nl = 0;
-        double wl = 0, wt = 0;
-        for (int i = 0; i < n; i++) {
-            const double w = e->elems[work[i]].P;
-            wt += w;
-            if (e->elems[work[i]].P > e->tune.long_patterns) { nl = i + 1; wl += w; }
-        }
-        const int ns = n - nl;
-        const int waves = e->n_cu * 8;   // wave slots at two waves per SIMD
-        if (nl > 0) {
-            const int full = e->n_cu * fit_per_cu(e, N, S, bsl, fp64);
-            int g = ns == 0 ? full : (int)std::lround(waves * (wl / wt) / (bsl / 64));
-            g = std::max(1, std::min({g, nl, full}));
-            cls.push_back({0, nl, bsl, g, 64, 0});
-        }
-        if (ns > 0) {
-            // both grids must be resident together, or the persistent small-element
-            // workgroups would hold the GPU until their queue drains: a long
-            // workgroup takes the place of bsl / bss small ones
-            const int taken = nl > 0 ? cls[0].grid * std::max(1, bsl / bss) : 0;
-            const int g = std::max(1, e->n_cu * fit_per_cu(e, N, S, bss, fp64) - taken);
-            cls.push_back({nl, ns, bss, std::min(ns, g), 64, 0});
-        }
-    }
-    size_t total = 0, off[5];
-    for (Cls &k : cls) {
-        for (int i = k.first; i < k.first + k.count; i++) k.Pcap = std::max(k.Pcap, e->elems[work[i]].Ppad);
-        if (chain_smem_bytes(N, S, k.BS, fp64) > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
-        k.slot = slot_layout(N, S, k.Pcap, fp64, off);
-        total += k.slot * k.grid;
-    }
-    HIPCHK(e, hipMalloc(&bufs[0].p, total));
+    return (meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256);
+}
+
+// Workgroups of block size BS that fit on one CU at once (LDS and waves).
+static int fit_per_cu(pa_engine *e, size_t smem, int BS)
+{
+    const int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
+    return std::max(1, std::min(per_cu, 2048 / BS));
+}
+
+// Fixed-(Z, r) log-likelihood launch: one persistent launch over ids (in
+// schedule order, largest first), one scratch slot per workgroup.
+static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, ChainArgs &A, DevBuf *bufs)
+{
+    const int S = e->tm.S, N = e->tm.N;
+    const int BS = pick_block(e, ids, n);
+    const size_t smem = chain_smem_bytes(N, S, BS, fp64);
+    if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+    const int grid = std::min(n, e->n_cu * fit_per_cu(e, smem, BS));
+    int Pcap = 64;
+    for (int i = 0; i < n; i++) Pcap = std::max(Pcap, e->elems[ids[i]].Ppad);
+    size_t off[5];
+    const size_t slot = slot_layout(N, S, Pcap, fp64, off);
+    HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
+    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n));
+    HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+    A.work = (const int32_t *)bufs[1].p;
+    A.n_work = n;
+    A.queue = e->d_queue;
+    A.scratch = (char *)bufs[0].p;
+    A.slot_bytes = (int64_t)slot;
+    HIPCHK(e, launch_reset(A.queue, e->stream));
+    HIPCHK(e, launch_loglik(e->M, A, fp64, BS, grid, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return PA_OK;
+}
+
+// Chain launch (pa_chain.hip).  Batched: one persistent launch, block size
+// from the mean pattern count, workgroups pulling elements (largest first)
+// from the work counter.  Per element: one single-workgroup launch per
+// element, the plain per-element baseline.
+static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, int mode, int bs_req,
+                                  chain::ChainArgs &A, DevBuf *bufs, pa_run_stats *stats)
+{
+    const int S = e->tm.S, N = e->tm.N, T = e->M.T;
+    int BS = bs_req ? bs_req : pick_block(e, ids, n);
+    if (BS != 64 && BS != 128 && BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
+    const size_t smem = chain::chain_smem_bytes(N, S, BS, fp64);
+    if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+    const int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, e->n_cu * fit_per_cu(e, smem, BS));
+    int Pcap = 64;
+    for (int i = 0; i < n; i++) Pcap = std::max(Pcap, e->elems[ids[i]].Ppad);
+    size_t off[5];
+    const size_t slot = chain::slot_scratch_bytes(N, S, Pcap, fp64, off);
+    HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
     HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + 64));
     unsigned long long *d_cnt = (unsigned long long *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
     HIPCHK(e, hipMemsetAsync(d_cnt, 0, 16, e->stream));
+    HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
     A.counters = d_cnt;
-    HIPCHK(e, hipMemcpyAsync(bufs[1].p, work.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
-    A.work = (const int32_t *)bufs[1].p;
-    hipEvent_t t0, t1, t2;
+    A.queue = e->d_queue;
+    A.scratch = (char *)bufs[0].p;
+    A.slot_bytes = (int64_t)slot;
+    A.off_parts = off[0]; A.off_exps = off[1]; A.off_plin = off[2]; A.off_logt = off[3]; A.off_amb = off[4];
+    hipEvent_t t0, t1;
     HIPCHK(e, hipEventCreate(&t0));
     HIPCHK(e, hipEventCreate(&t1));
-    HIPCHK(e, hipEventCreate(&t2));
     HIPCHK(e, hipEventRecord(t0, e->stream));
     if (mode == PA_RUN_PER_ELEMENT) {
-        const size_t slot = slot_layout(N, S, cls[0].Pcap, fp64, off);
         for (int i = 0; i < n; i++) {
-            ChainArgs B = shifted(A, i, N, T);
+            chain::ChainArgs B = A;
+            B.work = (const int32_t *)bufs[1].p + i;
             B.n_work = 1;
-            B.queue = e->d_queue;
-            B.scratch = (char *)bufs[0].p;
-            B.slot_bytes = (int64_t)slot;
+            // per-element outputs shift with the work position
+            B.res_d = A.res_d ? A.res_d + (size_t)i * 24 : nullptr;
+            B.res_i = A.res_i ? A.res_i + (size_t)i * 12 : nullptr;
+            B.maxZ = A.maxZ ? A.maxZ + (size_t)i * 3 * N : nullptr;
+            B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * T : nullptr;
+            B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * T * N : nullptr;
             HIPCHK(e, launch_reset(e->d_queue, e->stream));
-            HIPCHK(e, launch_kernel(e->M, B, fp64, cls[0].BS, 1, kind, e->stream));
+            HIPCHK(e, chain::launch_kernel(e->M, B, fp64, BS, 1, 0, e->stream));
         }
     } else {
-        HIPCHK(e, hipStreamWaitEvent(e->stream2, t0, 0));
-        size_t base = 0;
-        for (size_t k = 0; k < cls.size(); k++) {
-            hipStream_t st = k == 0 ? e->stream : e->stream2;
-            ChainArgs B = shifted(A, cls[k].first, N, T);
-            B.n_work = cls[k].count;
-            B.queue = e->d_queue + 16 * k;
-            B.scratch = (char *)bufs[0].p + base;
-            B.slot_bytes = (int64_t)cls[k].slot;
-            base += cls[k].slot * cls[k].grid;
-            HIPCHK(e, launch_reset(B.queue, st));
-            HIPCHK(e, launch_kernel(e->M, B, fp64, cls[k].BS, cls[k].grid, kind, st));
-        }
-        HIPCHK(e, hipEventRecord(t2, e->stream2));
-        HIPCHK(e, hipStreamWaitEvent(e->stream, t2, 0));
+        A.work = (const int32_t *)bufs[1].p;
+        A.n_work = n;
+        HIPCHK(e, launch_reset(e->d_queue, e->stream));
+        HIPCHK(e, chain::launch_kernel(e->M, A, fp64, BS, grid, 0, e->stream));
     }
     HIPCHK(e, hipEventRecord(t1, e->stream));
     HIPCHK(e, hipEventSynchronize(t1));
@@ -957,15 +927,13 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
         HIPCHK(e, hipMemcpy(&cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
         stats->elapsed_ms = ms;
         stats->algorithmic_bytes = (double)cnt;
-        stats->grid = 0;
-        for (const Cls &k : cls) stats->grid += k.grid;
-        stats->block_threads = cls.back().BS;
-        stats->lds_bytes = (int32_t)chain_smem_bytes(N, S, cls.back().BS, fp64);
-        stats->n_launches = mode == PA_RUN_PER_ELEMENT ? n : (int32_t)cls.size();
+        stats->grid = grid;
+        stats->block_threads = BS;
+        stats->lds_bytes = (int32_t)smem;
+        stats->n_launches = mode == PA_RUN_PER_ELEMENT ? n : 1;
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    (void)hipEventDestroy(t2);
     return PA_OK;
 }
 
@@ -979,14 +947,6 @@ static pa_status check_ids(pa_engine *e, const int32_t *ids, int32_t n)
 }
 
 static std::vector<int32_t> schedule(pa_engine *e, const int32_t *ids, int32_t n, int mode);
-
-// ids in launch order (longest first)
-static std::vector<int32_t> schedule_ids(pa_engine *e, const std::vector<int32_t> &ids)
-{
-    std::vector<int32_t> order = schedule(e, ids.data(), (int32_t)ids.size(), PA_RUN_BATCHED), out(ids.size());
-    for (size_t i = 0; i < ids.size(); i++) out[i] = ids[order[i]];
-    return out;
-}
 
 static std::vector<int32_t> schedule(pa_engine *e, const int32_t *ids, int32_t n, int mode)
 {
@@ -1031,7 +991,7 @@ extern "C" pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, cons
     A.ll_out = (double *)bufs[5].p;
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
-    st = run_kernel(e, sorted.data(), n, precision == PA_FP64, 1, PA_RUN_BATCHED, 0, A, bufs, nullptr);
+    st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);
     if (st != PA_OK) return st;
     std::vector<double> ll(n);
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
@@ -1071,7 +1031,7 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     A.len_in = (const double *)bufs[6].p;
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
-    st = run_kernel(e, sorted.data(), n, precision == PA_FP64, 1, PA_RUN_BATCHED, 0, A, bufs, nullptr);
+    st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);
     if (st != PA_OK) return st;
     std::vector<double> ll(n);
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
@@ -1103,14 +1063,14 @@ extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, 
         HIPCHK(e, hipMalloc(&bufs[6].p, (size_t)n * 3 * 7 * T * 8));
         HIPCHK(e, hipMalloc(&bufs[7].p, (size_t)n * 3 * T * N));
     }
-    ChainArgs A;
+    chain::ChainArgs A;
     memset(&A, 0, sizeof A);
     A.res_d = (double *)bufs[3].p;
     A.res_i = (int32_t *)bufs[4].p;
     A.maxZ = (int8_t *)bufs[5].p;
     A.tr_d = keep ? (double *)bufs[6].p : nullptr;
     A.tr_Z = keep ? (int8_t *)bufs[7].p : nullptr;
-    st = run_kernel(e, sorted.data(), n, cfg->precision == PA_FP64, 0, cfg->mode, cfg->block_threads, A, bufs, stats);
+    st = run_chain_kernel(e, sorted.data(), n, cfg->precision == PA_FP64, cfg->mode, cfg->block_threads, A, bufs, stats);
     if (st != PA_OK) return st;
     std::vector<double> rd((size_t)n * 24);
     std::vector<int32_t> ri((size_t)n * 12);
@@ -1313,125 +1273,62 @@ extern "C" pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, c
     return PA_OK;
 }
 
-// ------------------------------------------------------------- autotuner
+// ------------------------------------------------ switched-off entry points
+//
+// Size-class launches (two block sizes on two streams), their autotuner and
+// the resumable serving loop ran on a chain kernel that did not finish on
+// the GPU; they are switched off until that kernel's failure is explained.
+// Each call below fails with PA_E_STATE and says so.
+
+static pa_status switched_off(pa_engine *e, const char *what)
+{
+    return fail(e, PA_E_STATE, std::string(what) + " is switched off in this build (batched chain launches use one block size on one stream)");
+}
 
 extern "C" pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t)
 {
     if (!e || !t) return fail(e, PA_E_INPUT, "null argument");
-    auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
-    if (!ok(t->small_threads) || !ok(t->long_threads) || t->long_patterns < 0)
-        return fail(e, PA_E_INPUT, "tuning: block sizes must be 64, 128 or 256");
-    e->tune = *t;
-    return PA_OK;
+    return switched_off(e, "size-class tuning");
 }
 
 extern "C" pa_status pa_get_tuning(pa_engine *e, pa_tuning *t)
 {
     if (!e || !t) return fail(e, PA_E_INPUT, "null argument");
-    *t = e->tune;
-    return PA_OK;
+    return switched_off(e, "size-class tuning");
 }
 
-// Times every candidate size-class configuration on a length-stratified
-// sample of the ids with short chains (the engine's options otherwise) and
-// keeps the fastest in element-MCMC iterations per second.
 extern "C" pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, int32_t precision, int32_t sample,
                                  int32_t states, pa_tuning *out)
 {
-    if (!e || !ids || n <= 0) return fail(e, PA_E_INPUT, "null argument");
-    pa_status st = check_ids(e, ids, n);
-    if (st != PA_OK) return st;
-    if ((st = upload(e)) != PA_OK) return st;
-    if (sample <= 0) sample = 2048;
-    if (states <= 1) states = 16;
-    // stratified sample by pattern count
-    std::vector<int32_t> order(ids, ids + n);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e->elems[a].P < e->elems[b].P; });
-    std::vector<int32_t> smp;
-    const int k = std::min(sample, n);
-    for (int i = 0; i < k; i++) smp.push_back(order[(size_t)((i + 0.5) * n / k)]);
-    std::vector<int32_t> sorted = schedule_ids(e, smp);
-    const ModelDev saved = e->M;
-    e->M.T = states;
-    e->M.burnin = states / 4;
-    e->M.mcmc = states - states / 4;
-    const pa_tuning keep = e->tune;
-    pa_tuning best = keep;
-    best.rate = -1;
-    const int smalls[2] = {64, 128}, longs[2] = {128, 256}, ths[3] = {128, 256, 512};
-    for (int a = 0; a < 2 && st == PA_OK; a++)
-        for (int b = 0; b < 2 && st == PA_OK; b++)
-            for (int c = 0; c < 3 && st == PA_OK; c++) {
-                if (longs[b] < smalls[a]) continue;
-                e->tune = {smalls[a], longs[b], ths[c], 0.0};
-                const int N = e->tm.N, m = (int)sorted.size();
-                DevBuf bufs[8];
-                pa_run_stats rs;
-                ChainArgs A;
-                memset(&A, 0, sizeof A);
-                if (hipMalloc(&bufs[3].p, (size_t)m * 3 * 8 * 8) != hipSuccess ||
-                    hipMalloc(&bufs[4].p, (size_t)m * 3 * 4 * 4) != hipSuccess ||
-                    hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) { st = fail(e, PA_E_OOM, "autotune buffers"); break; }
-                A.res_d = (double *)bufs[3].p;
-                A.res_i = (int32_t *)bufs[4].p;
-                A.maxZ = (int8_t *)bufs[5].p;
-                st = run_kernel(e, sorted.data(), m, precision == PA_FP64, 0, PA_RUN_BATCHED, 0, A, bufs, &rs);
-                if (st != PA_OK) break;
-                const double rate = (double)m * 3 * (states - 1) / (rs.elapsed_ms / 1e3);
-                if (rate > best.rate) { best = e->tune; best.rate = rate; }
-            }
-    e->M = saved;
-    e->tune = st == PA_OK ? best : keep;
-    if (out) *out = e->tune;
-    return st;
+    (void)ids; (void)n; (void)precision; (void)sample; (void)states; (void)out;
+    if (!e) return PA_E_INPUT;
+    return switched_off(e, "size-class autotuning");
 }
 
 // ------------------------------------------------------------ serving loop
+//
+// Batch-per-step request loop.  Each pa_serve_step retires the launch in
+// flight (if it finished), then admits waiting requests in arrival order
+// while one launch's scratch fits the HBM budget, and launches them: the
+// chain kernel over the admitted chain requests, then the log-likelihood
+// kernel over the admitted own-tree requests, on the engine's stream, with
+// the results copied into pinned host memory behind them.  Every admitted
+// chain runs to completion within its launch.  The block size is fixed per
+// server and each element's chain is run by one workgroup from its own
+// random streams, so a result does not depend on what else is in the batch.
 
 namespace {
-
-// First-fit allocator over one device scratch pool, with coalescing.
-struct Pool {
-    std::map<size_t, size_t> free_;   // offset -> length
-    size_t used = 0;
-    void init(size_t n) { free_.clear(); free_[0] = n; used = 0; }
-    bool take(size_t n, size_t *off)
-    {
-        for (auto it = free_.begin(); it != free_.end(); ++it)
-            if (it->second >= n) {
-                *off = it->first;
-                const size_t rest = it->second - n, at = it->first + n;
-                free_.erase(it);
-                if (rest) free_[at] = rest;
-                used += n;
-                return true;
-            }
-        return false;
-    }
-    void give(size_t off, size_t n)
-    {
-        used -= n;
-        auto nx = free_.lower_bound(off);
-        if (nx != free_.end() && off + n == nx->first) { n += nx->second; nx = free_.erase(nx); }
-        if (nx != free_.begin()) {
-            auto pv = std::prev(nx);
-            if (pv->first + pv->second == off) { pv->second += n; return; }
-        }
-        free_[off] = n;
-    }
-};
 
 struct Request {
     int64_t ticket;
     int32_t eid;
     double arrival, admit;
-    size_t off, bytes;
 };
 
 struct LRequest {
     int64_t ticket;
     int32_t eid;
-    double arrival;
+    double arrival, admit;
     std::vector<double> len;
 };
 
@@ -1439,43 +1336,25 @@ struct LRequest {
 
 struct pa_server {
     pa_engine *e = nullptr;
-    int fp64 = 0, BS = 128, budget = 100, max_chains = 4096;
-    size_t pool_bytes = 0;
-    char *d_pool = nullptr;
-    Pool pool;
-    ChainState *d_state = nullptr;
-    int32_t *d_done = nullptr;                      // done flags [max]
-    int64_t *d_slot = nullptr;                      // scratch offset of each record [max]
-    double *d_res_d = nullptr;
-    int32_t *d_res_i = nullptr;
-    int8_t *d_maxZ = nullptr;
-    // per-step upload, one copy: EpochDesc | work ids [max] | records [max] |
-    // fresh records [max] | fresh scratch offsets [max] (int64)
-    char *d_pack = nullptr, *h_pack = nullptr;
-    size_t pack_bytes = 0, off_work = 0, off_rec = 0, off_fresh = 0, off_foff = 0;
-    int32_t *h_done = nullptr;                      // pinned
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    uint64_t graph_gen = ~0ull;
-    int grid = 0;
-    std::vector<int32_t> free_recs;
-    std::vector<Request> rec_req;                 // by record
-    std::vector<int32_t> active;                  // records in flight
-    std::deque<Request> waiting;
-    std::deque<pa_serve_result> finished;
+    int fp64 = 0, BS = 128, max_chains = 4096;
+    size_t pool_bytes = 0, in_use = 0;
+    char *d_pool = nullptr;                         // scratch of the launch in flight
+    int32_t *d_ids = nullptr, *h_ids = nullptr;     // chain work list [max]
+    double *d_res_d = nullptr, *h_res_d = nullptr;  // [max][3][8]
+    int32_t *d_res_i = nullptr, *h_res_i = nullptr; // [max][3][4]
+    int8_t *d_maxZ = nullptr;                       // [max][3][N]
+    unsigned long long *d_cnt = nullptr;
+    int32_t *d_lids = nullptr, *h_lids = nullptr, *d_lZ = nullptr;
+    double *d_lrates = nullptr, *d_llen = nullptr, *h_llen = nullptr, *d_lout = nullptr, *h_lout = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool in_flight = false;
+    std::deque<Request> waiting;
+    std::deque<LRequest> lwait;
+    std::vector<Request> batch;                     // in flight, launch order
+    std::vector<LRequest> lbatch;
+    std::deque<pa_serve_result> finished;
     int64_t next_ticket = 0;
     pa_serve_stats st{};
-    // log-likelihood requests (own-tree elements), one batched launch per step on stream2
-    std::deque<LRequest> lwait;
-    std::vector<LRequest> lbatch;                  // in flight, launch order
-    int32_t *d_lids = nullptr, *d_lZ = nullptr, *h_lids = nullptr;
-    double *d_lrates = nullptr, *d_llen = nullptr, *d_lout = nullptr, *h_llen = nullptr, *h_lout = nullptr;
-    char *d_lscratch = nullptr;
-    size_t lscratch_bytes = 0;
-    int lmax = 0;
-    hipEvent_t evl = nullptr;
 };
 
 extern "C" double pa_serve_clock(void)
@@ -1485,23 +1364,22 @@ extern "C" double pa_serve_clock(void)
 
 static void serve_free(pa_server *s)
 {
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    (void)hipFree(s->d_pool); (void)hipFree(s->d_state); (void)hipFree(s->d_done); (void)hipFree(s->d_pack);
-    (void)hipFree(s->d_slot); (void)hipFree(s->d_res_d); (void)hipFree(s->d_res_i); (void)hipFree(s->d_maxZ);
-    (void)hipHostFree(s->h_pack); (void)hipHostFree(s->h_done);
+    (void)hipFree(s->d_pool); (void)hipFree(s->d_ids); (void)hipFree(s->d_res_d); (void)hipFree(s->d_res_i);
+    (void)hipFree(s->d_maxZ); (void)hipFree(s->d_cnt); (void)hipFree(s->d_lids); (void)hipFree(s->d_lZ);
+    (void)hipFree(s->d_lrates); (void)hipFree(s->d_llen); (void)hipFree(s->d_lout);
+    (void)hipHostFree(s->h_ids); (void)hipHostFree(s->h_res_d); (void)hipHostFree(s->h_res_i);
+    (void)hipHostFree(s->h_lids); (void)hipHostFree(s->h_llen); (void)hipHostFree(s->h_lout);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
-    if (s->evl) (void)hipEventDestroy(s->evl);
-    (void)hipFree(s->d_lids); (void)hipFree(s->d_lZ); (void)hipFree(s->d_lrates); (void)hipFree(s->d_llen);
-    (void)hipFree(s->d_lout); (void)hipFree(s->d_lscratch);
-    (void)hipHostFree(s->h_lids); (void)hipHostFree(s->h_llen); (void)hipHostFree(s->h_lout);
 }
 
 extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **out)
 {
+    if (out) *out = nullptr;
     if (!e || !cfg || !out) return fail(e, PA_E_INPUT, "null argument");
-    *out = nullptr;
+    if (cfg->sweep_budget != 0)
+        return fail(e, PA_E_STATE, "resumable chains (sweep_budget > 0) are switched off in this build; "
+                                   "use sweep_budget = 0 (each chain runs to completion in its launch)");
     pa_status st = ensure_device(e);
     if (st != PA_OK) return st;
     HIPCHK(e, hipSetDevice(e->device));
@@ -1510,56 +1388,38 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     s->fp64 = cfg->precision == PA_FP64;
     s->BS = cfg->block_threads ? cfg->block_threads : 128;
     if (s->BS != 64 && s->BS != 128 && s->BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
-    if (chain_smem_bytes(e->tm.N, e->tm.S, s->BS, s->fp64) > e->lds_per_cu)
+    const int N = e->tm.N, S = e->tm.S;
+    if (chain::chain_smem_bytes(N, S, s->BS, s->fp64) > e->lds_per_cu || chain_smem_bytes(N, S, s->BS, s->fp64) > e->lds_per_cu)
         return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
-    s->budget = cfg->sweep_budget > 0 ? cfg->sweep_budget : 100;
     s->max_chains = cfg->max_chains > 0 ? cfg->max_chains : 4096;
     s->pool_bytes = cfg->hbm_budget_bytes > 0 ? (size_t)cfg->hbm_budget_bytes : ((size_t)4 << 30);
-    const int M = s->max_chains, N = e->tm.N;
+    const size_t M = (size_t)s->max_chains;
     struct Alloc { pa_server *s; ~Alloc() { if (s) serve_free(s); } } guard{s.get()};
     HIPCHK(e, hipMalloc(&s->d_pool, s->pool_bytes));
-    HIPCHK(e, hipMalloc(&s->d_state, sizeof(ChainState) * M));
-    HIPCHK(e, hipMalloc(&s->d_done, sizeof(int32_t) * M));
-    s->off_work = sizeof(EpochDesc);
-    s->off_rec = s->off_work + sizeof(int32_t) * M;
-    s->off_fresh = s->off_rec + sizeof(int32_t) * M;
-    s->off_foff = (s->off_fresh + sizeof(int32_t) * M + 7) & ~(size_t)7;
-    s->pack_bytes = s->off_foff + sizeof(int64_t) * M;
-    HIPCHK(e, hipMalloc(&s->d_pack, s->pack_bytes));
-    HIPCHK(e, hipMalloc(&s->d_slot, sizeof(int64_t) * M));
+    HIPCHK(e, hipMalloc(&s->d_ids, sizeof(int32_t) * M));
     HIPCHK(e, hipMalloc(&s->d_res_d, sizeof(double) * 24 * M));
     HIPCHK(e, hipMalloc(&s->d_res_i, sizeof(int32_t) * 12 * M));
     HIPCHK(e, hipMalloc(&s->d_maxZ, (size_t)3 * N * M));
-    HIPCHK(e, hipHostMalloc(&s->h_pack, s->pack_bytes));
-    HIPCHK(e, hipHostMalloc(&s->h_done, sizeof(int32_t) * M));
-    memset(s->h_done, 0, sizeof(int32_t) * M);
+    HIPCHK(e, hipMalloc(&s->d_cnt, 16));
+    HIPCHK(e, hipHostMalloc(&s->h_ids, sizeof(int32_t) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_res_d, sizeof(double) * 24 * M));
+    HIPCHK(e, hipHostMalloc(&s->h_res_i, sizeof(int32_t) * 12 * M));
+    HIPCHK(e, hipMalloc(&s->d_lids, sizeof(int32_t) * M));
+    HIPCHK(e, hipMalloc(&s->d_lZ, sizeof(int32_t) * M * N));
+    HIPCHK(e, hipMalloc(&s->d_lrates, sizeof(double) * 2 * M));
+    HIPCHK(e, hipMalloc(&s->d_llen, sizeof(double) * M * N));
+    HIPCHK(e, hipMalloc(&s->d_lout, sizeof(double) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_lids, sizeof(int32_t) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_llen, sizeof(double) * M * N));
+    HIPCHK(e, hipHostMalloc(&s->h_lout, sizeof(double) * M));
     {
-        const size_t smem = chain_smem_bytes(N, e->tm.S, s->BS, s->fp64);
-        int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
-        per_cu = std::max(1, std::min(per_cu, 2048 / s->BS));
-        s->grid = std::min(M, e->n_cu * per_cu);
+        // own-tree log-likelihoods: Z = 0 and rates 1, every branch at its given length
+        std::vector<double> ones(2 * M, 1.0);
+        HIPCHK(e, hipMemset(s->d_lZ, 0, sizeof(int32_t) * M * N));
+        HIPCHK(e, hipMemcpy(s->d_lrates, ones.data(), ones.size() * 8, hipMemcpyHostToDevice));
     }
     HIPCHK(e, hipEventCreate(&s->ev0));
     HIPCHK(e, hipEventCreate(&s->ev1));
-    HIPCHK(e, hipEventCreate(&s->evl));
-    s->lmax = M;
-    HIPCHK(e, hipMalloc(&s->d_lids, sizeof(int32_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_lZ, sizeof(int32_t) * (size_t)M * N));
-    HIPCHK(e, hipMalloc(&s->d_lrates, sizeof(double) * 2 * M));
-    HIPCHK(e, hipMalloc(&s->d_llen, sizeof(double) * (size_t)M * N));
-    HIPCHK(e, hipMalloc(&s->d_lout, sizeof(double) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_lids, sizeof(int32_t) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_llen, sizeof(double) * (size_t)M * N));
-    HIPCHK(e, hipHostMalloc(&s->h_lout, sizeof(double) * M));
-    {
-        // own-tree log-likelihoods: every branch in the rate-1 class
-        std::vector<double> ones((size_t)2 * M, 1.0);
-        HIPCHK(e, hipMemset(s->d_lZ, 0, sizeof(int32_t) * (size_t)M * N));
-        HIPCHK(e, hipMemcpy(s->d_lrates, ones.data(), ones.size() * 8, hipMemcpyHostToDevice));
-    }
-    s->pool.init(s->pool_bytes);
-    s->rec_req.resize(M);
-    for (int r = M - 1; r >= 0; r--) s->free_recs.push_back(r);
     guard.s = nullptr;
     *out = s.release();
     return PA_OK;
@@ -1572,16 +1432,10 @@ extern "C" pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n
     pa_status st = check_ids(e, ids, n);
     if (st != PA_OK) return st;
     const double now = pa_serve_clock();
-    size_t off[5];
     for (int i = 0; i < n; i++) {
-        Request q{};
-        q.ticket = s->next_ticket++;
-        q.eid = ids[i];
-        q.arrival = arrival_s ? arrival_s[i] : now;
-        q.bytes = slot_layout(e->tm.N, e->tm.S, e->elems[ids[i]].Ppad, s->fp64, off);
-        if (q.bytes > s->pool_bytes) return fail(e, PA_E_INPUT, "element " + std::to_string(ids[i]) + " exceeds the HBM budget");
-        if (tickets) tickets[i] = q.ticket;
-        s->waiting.push_back(q);
+        const int64_t t = s->next_ticket++;
+        s->waiting.push_back({t, ids[i], arrival_s ? arrival_s[i] : now, 0.0});
+        if (tickets) tickets[i] = t;
     }
     s->st.submitted += n;
     return PA_OK;
@@ -1590,245 +1444,189 @@ extern "C" pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n
 extern "C" pa_status pa_serve_submit_lengths(pa_server *s, const int32_t *ids, int32_t n, const double *lengths,
                                              const double *arrival_s, int64_t *tickets)
 {
-    if (!s || (n > 0 && (!ids || !lengths)) || n < 0) return PA_E_INPUT;
+    if (!s || (!ids && n > 0) || (!lengths && n > 0) || n < 0) return PA_E_INPUT;
     pa_engine *e = s->e;
     pa_status st = check_ids(e, ids, n);
     if (st != PA_OK) return st;
     const int N = e->tm.N;
     const double now = pa_serve_clock();
     for (int i = 0; i < n; i++) {
-        LRequest q;
-        q.ticket = s->next_ticket++;
-        q.eid = ids[i];
-        q.arrival = arrival_s ? arrival_s[i] : now;
-        q.len.assign(lengths + (size_t)i * N, lengths + (size_t)(i + 1) * N);
-        if (tickets) tickets[i] = q.ticket;
-        s->lwait.push_back(std::move(q));
+        const int64_t t = s->next_ticket++;
+        s->lwait.push_back({t, ids[i], arrival_s ? arrival_s[i] : now, 0.0,
+                            std::vector<double>(lengths + (size_t)i * N, lengths + (size_t)(i + 1) * N)});
+        if (tickets) tickets[i] = t;
     }
     s->st.submitted += n;
     return PA_OK;
 }
 
-// Launches the waiting log-likelihood requests (up to lmax) on stream2.
-static pa_status serve_launch_loglik(pa_server *s)
+// Scratch of one launch of k workgroups over elements of up to Pcap padded patterns.
+static size_t chain_launch_scratch(pa_server *s, int Pcap, int k)
 {
-    pa_engine *e = s->e;
-    if (s->lwait.empty()) return PA_OK;
-    pa_status st = upload(e);
-    if (st != PA_OK) return st;
-    const int N = e->tm.N, S = e->tm.S;
-    while (!s->lwait.empty() && (int)s->lbatch.size() < s->lmax) {
-        s->lbatch.push_back(std::move(s->lwait.front()));
-        s->lwait.pop_front();
-    }
-    std::stable_sort(s->lbatch.begin(), s->lbatch.end(),
-                     [&](const LRequest &a, const LRequest &b) { return e->elems[a.eid].P > e->elems[b.eid].P; });
-    const int n = (int)s->lbatch.size();
-    int Pcap = 64;
-    for (int i = 0; i < n; i++) {
-        s->h_lids[i] = s->lbatch[i].eid;
-        memcpy(s->h_llen + (size_t)i * N, s->lbatch[i].len.data(), sizeof(double) * N);
-        Pcap = std::max(Pcap, e->elems[s->lbatch[i].eid].Ppad);
-    }
-    const int BS = 64;
-    const int grid = std::min(n, e->n_cu * fit_per_cu(e, N, S, BS, s->fp64));
     size_t off[5];
-    const size_t slot = slot_layout(N, S, Pcap, s->fp64, off);
-    if (slot * grid > s->lscratch_bytes) {
-        (void)hipFree(s->d_lscratch);
-        s->d_lscratch = nullptr;
-        s->lscratch_bytes = 0;
-        HIPCHK(e, hipMalloc(&s->d_lscratch, slot * grid));
-        s->lscratch_bytes = slot * grid;
-    }
-    HIPCHK(e, hipMemcpyAsync(s->d_lids, s->h_lids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream2));
-    HIPCHK(e, hipMemcpyAsync(s->d_llen, s->h_llen, sizeof(double) * (size_t)n * N, hipMemcpyHostToDevice, e->stream2));
-    ChainArgs A;
-    memset(&A, 0, sizeof A);
-    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
-    A.queue = e->d_queue + 32;
-    A.work = s->d_lids;
-    A.n_work = n;
-    A.scratch = s->d_lscratch;
-    A.slot_bytes = (int64_t)slot;
-    A.Zin = s->d_lZ;
-    A.rates_in = s->d_lrates;
-    A.len_in = s->d_llen;
-    A.ll_out = s->d_lout;
-    HIPCHK(e, launch_reset(A.queue, e->stream2));
-    HIPCHK(e, launch_kernel(e->M, A, s->fp64, BS, grid, 1, e->stream2));
-    HIPCHK(e, hipMemcpyAsync(s->h_lout, s->d_lout, sizeof(double) * n, hipMemcpyDeviceToHost, e->stream2));
-    HIPCHK(e, hipEventRecord(s->evl, e->stream2));
-    return PA_OK;
+    return chain::slot_scratch_bytes(s->e->tm.N, s->e->tm.S, Pcap, s->fp64, off) * (size_t)k;
 }
-
-static pa_status serve_retire_loglik(pa_server *s, int32_t *n_finished)
+static size_t loglik_launch_scratch(pa_server *s, int Pcap, int k)
 {
-    pa_engine *e = s->e;
-    if (s->lbatch.empty()) return PA_OK;
-    HIPCHK(e, hipEventSynchronize(s->evl));
-    const double now = pa_serve_clock();
-    for (size_t i = 0; i < s->lbatch.size(); i++) {
-        pa_serve_result out{};
-        out.ticket = s->lbatch[i].ticket;
-        out.element = s->lbatch[i].eid;
-        out.kind = PA_SERVE_LOGLIK;
-        out.loglik = s->h_lout[i];
-        out.status = std::isnan(out.loglik) ? PA_ELEM_NUMERIC_FAIL : PA_ELEM_OK;
-        out.arrival_s = s->lbatch[i].arrival;
-        out.admit_s = now;
-        out.finish_s = now;
-        s->finished.push_back(out);
-        s->st.finished++;
-        (*n_finished)++;
-    }
-    s->lbatch.clear();
-    return PA_OK;
+    size_t off[5];
+    return slot_layout(s->e->tm.N, s->e->tm.S, Pcap, s->fp64, off) * (size_t)k;
 }
 
-// Retires the chains of the finished launch.
-static pa_status serve_retire(pa_server *s, int32_t *n_finished)
-{
-    pa_engine *e = s->e;
-    const int N = e->tm.N;
-    HIPCHK(e, hipEventSynchronize(s->ev1));
-    float ms = 0;
-    HIPCHK(e, hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    s->st.device_ms += ms;
-    s->in_flight = false;
-    const double now = pa_serve_clock();
-    std::vector<int32_t> keep;
-    std::vector<double> rd(24);
-    std::vector<int32_t> ri(12);
-    for (int r : s->active) {
-        if (!s->h_done[r]) { keep.push_back(r); continue; }
-        HIPCHK(e, hipMemcpy(rd.data(), s->d_res_d + (size_t)r * 24, 24 * 8, hipMemcpyDeviceToHost));
-        HIPCHK(e, hipMemcpy(ri.data(), s->d_res_i + (size_t)r * 12, 12 * 4, hipMemcpyDeviceToHost));
-        const Request &q = s->rec_req[r];
-        pa_serve_result out{};
-        out.ticket = q.ticket;
-        out.element = q.eid;
-        out.status = PA_ELEM_OK;
-        out.kind = PA_SERVE_CHAINS;
-        for (int k = 0; k < 3; k++) {
-            const double *d = &rd[k * 8];
-            const int32_t *x = &ri[k * 4];
-            pa_chain_result &c = out.result[k];
-            c.max_loglik = d[0]; c.final_nrate = d[1]; c.final_crate = d[2];
-            c.final_grate = d[3]; c.final_lrate = d[4]; c.final_lrate2 = d[5];
-            c.max_m = x[0]; c.status = x[1]; c.accept_n = x[2]; c.accept_c = x[3];
-            if (c.status) out.status = PA_ELEM_NUMERIC_FAIL;
-        }
-        out.arrival_s = q.arrival;
-        out.admit_s = q.admit;
-        out.finish_s = now;
-        s->finished.push_back(out);
-        s->pool.give(q.off, q.bytes);
-        s->free_recs.push_back(r);
-        s->st.finished++;
-        (*n_finished)++;
-    }
-    (void)N;
-    s->active.swap(keep);
-    return PA_OK;
-}
-
-// Records the step's work as one graph: the packed upload, the admission
-// kernel (fresh records + work counter), the chain kernel over a fixed
-// grid (it reads the number of chains from the upload) and the copy of the
-// done flags back.  Re-recorded only when the device arenas move.
-static pa_status serve_capture(pa_server *s)
-{
-    pa_engine *e = s->e;
-    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
-    const EpochDesc *desc = (const EpochDesc *)s->d_pack;
-    ChainArgs A;
-    memset(&A, 0, sizeof A);
-    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
-    A.queue = e->d_queue;
-    A.work = (const int32_t *)(s->d_pack + s->off_work);
-    A.rec = (const int32_t *)(s->d_pack + s->off_rec);
-    A.n_work_dev = &desc->n_work;
-    A.scratch = s->d_pool;
-    A.slot_off = s->d_slot;
-    A.state = s->d_state;
-    A.done = s->d_done;
-    A.sweep_budget = s->budget;
-    A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
-    HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t r = hipMemcpyAsync(s->d_pack, s->h_pack, s->pack_bytes, hipMemcpyHostToDevice, e->stream);
-    if (r == hipSuccess)
-        r = launch_admit(desc, (const int32_t *)(s->d_pack + s->off_fresh), (const int64_t *)(s->d_pack + s->off_foff),
-                         s->d_state, s->d_done, s->d_slot, e->d_queue, e->stream);
-    if (r == hipSuccess) r = launch_kernel(e->M, A, s->fp64, s->BS, s->grid, 0, e->stream);
-    if (r == hipSuccess)
-        r = hipMemcpyAsync(s->h_done, s->d_done, sizeof(int32_t) * s->max_chains, hipMemcpyDeviceToHost, e->stream);
-    hipGraph_t g = nullptr;
-    const hipError_t r2 = hipStreamEndCapture(e->stream, &g);
-    if (r != hipSuccess || r2 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return fail(e, PA_E_DEVICE, std::string("serving graph capture: ") + hipGetErrorString(r != hipSuccess ? r : r2));
-    }
-    s->graph = g;
-    HIPCHK(e, hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
-    s->graph_gen = e->arena_gen;
-    return PA_OK;
-}
-
-// Admits waiting requests (FIFO) and launches the next sweep batch.
+// Admits waiting requests (arrival order) and launches them; no-op when
+// nothing waits.  Requests whose scratch alone exceeds the budget finish at
+// once with status PA_ELEM_NUMERIC_FAIL and an out-of-budget result.
 static pa_status serve_launch(pa_server *s)
 {
     pa_engine *e = s->e;
-    const double now = pa_serve_clock();
-    std::vector<int32_t> fresh;
-    while (!s->waiting.empty() && !s->free_recs.empty()) {
-        Request q = s->waiting.front();
-        size_t off;
-        if (!s->pool.take(q.bytes, &off)) break;   // wait for scratch to free up
-        s->waiting.pop_front();
-        const int r = s->free_recs.back();
-        s->free_recs.pop_back();
-        q.off = off;
-        q.admit = now;
-        s->rec_req[r] = q;
-        s->active.push_back(r);
-        fresh.push_back(r);
-        s->st.admitted++;
-    }
-    if (s->active.empty()) return PA_OK;
+    if (s->waiting.empty() && s->lwait.empty()) return PA_OK;
     pa_status st = upload(e);
     if (st != PA_OK) return st;
-    // largest elements first
-    std::sort(s->active.begin(), s->active.end(), [&](int a, int b) {
-        const HostElem &x = e->elems[s->rec_req[a].eid], &y = e->elems[s->rec_req[b].eid];
-        if (x.P != y.P) return x.P > y.P;
-        return a < b;
-    });
-    const int n = (int)s->active.size();
-    EpochDesc *d = (EpochDesc *)s->h_pack;
-    d->n_work = n;
-    d->n_fresh = (int)fresh.size();
-    int32_t *work = (int32_t *)(s->h_pack + s->off_work), *rec = (int32_t *)(s->h_pack + s->off_rec);
-    int32_t *fr = (int32_t *)(s->h_pack + s->off_fresh);
-    int64_t *fo = (int64_t *)(s->h_pack + s->off_foff);
-    for (int i = 0; i < n; i++) {
-        work[i] = s->rec_req[s->active[i]].eid;
-        rec[i] = s->active[i];
+    const int N = e->tm.N, S = e->tm.S;
+    const int gmax_c = e->n_cu * fit_per_cu(e, chain::chain_smem_bytes(N, S, s->BS, s->fp64), s->BS);
+    const int gmax_l = e->n_cu * fit_per_cu(e, chain_smem_bytes(N, S, s->BS, s->fp64), s->BS);
+    const double now = pa_serve_clock();
+    // chain requests
+    int PcapC = 64;
+    s->batch.clear();
+    while (!s->waiting.empty() && (int)s->batch.size() < s->max_chains) {
+        const Request &r = s->waiting.front();
+        const int pc = std::max(PcapC, e->elems[r.eid].Ppad);
+        const int k = (int)s->batch.size() + 1;
+        if (chain_launch_scratch(s, pc, std::min(k, gmax_c)) > s->pool_bytes) {
+            if (!s->batch.empty()) break;
+            // cannot run even alone
+            pa_serve_result out;
+            memset(&out, 0, sizeof out);
+            out.ticket = r.ticket; out.element = r.eid; out.status = PA_ELEM_NUMERIC_FAIL; out.kind = PA_SERVE_CHAINS;
+            out.arrival_s = r.arrival; out.admit_s = now; out.finish_s = now;
+            s->finished.push_back(out);
+            s->st.finished++;
+            s->waiting.pop_front();
+            continue;
+        }
+        PcapC = pc;
+        s->batch.push_back(r);
+        s->batch.back().admit = now;
+        s->waiting.pop_front();
     }
-    for (size_t i = 0; i < fresh.size(); i++) {
-        fr[i] = fresh[i];
-        fo[i] = (int64_t)s->rec_req[fresh[i]].off;
-        s->h_done[fresh[i]] = 0;
+    const int nc = (int)s->batch.size();
+    const int gc = std::min(nc, gmax_c);
+    const size_t scr_c = nc ? (size_t)((chain_launch_scratch(s, PcapC, gc) + 255) & ~(size_t)255) : 0;
+    // log-likelihood requests, in the pool after the chains' scratch
+    int PcapL = 64;
+    s->lbatch.clear();
+    while (!s->lwait.empty() && (int)s->lbatch.size() < s->max_chains) {
+        const LRequest &r = s->lwait.front();
+        const int pc = std::max(PcapL, e->elems[r.eid].Ppad);
+        const int k = (int)s->lbatch.size() + 1;
+        if (scr_c + loglik_launch_scratch(s, pc, std::min(k, gmax_l)) > s->pool_bytes) {
+            if (!s->lbatch.empty() || nc) break;
+            pa_serve_result out;
+            memset(&out, 0, sizeof out);
+            out.ticket = r.ticket; out.element = r.eid; out.status = PA_ELEM_NUMERIC_FAIL; out.kind = PA_SERVE_LOGLIK;
+            out.arrival_s = r.arrival; out.admit_s = now; out.finish_s = now;
+            s->finished.push_back(out);
+            s->st.finished++;
+            s->lwait.pop_front();
+            continue;
+        }
+        PcapL = pc;
+        s->lbatch.push_back(std::move(s->lwait.front()));
+        s->lbatch.back().admit = now;
+        s->lwait.pop_front();
     }
-    if (!s->gexec || s->graph_gen != e->arena_gen) {
-        if ((st = serve_capture(s)) != PA_OK) return st;
-    }
+    const int nl = (int)s->lbatch.size();
+    const int gl = std::min(nl, gmax_l);
+    if (nc == 0 && nl == 0) return PA_OK;
     HIPCHK(e, hipEventRecord(s->ev0, e->stream));
-    HIPCHK(e, hipGraphLaunch(s->gexec, e->stream));
+    if (nc) {
+        for (int i = 0; i < nc; i++) s->h_ids[i] = s->batch[i].eid;
+        HIPCHK(e, hipMemcpyAsync(s->d_ids, s->h_ids, sizeof(int32_t) * nc, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemsetAsync(s->d_cnt, 0, 16, e->stream));
+        chain::ChainArgs A;
+        memset(&A, 0, sizeof A);
+        size_t off[5];
+        const size_t slot = chain::slot_scratch_bytes(N, S, PcapC, s->fp64, off);
+        A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+        A.work = s->d_ids; A.n_work = nc; A.queue = e->d_queue;
+        A.scratch = s->d_pool; A.slot_bytes = (int64_t)slot;
+        A.off_parts = off[0]; A.off_exps = off[1]; A.off_plin = off[2]; A.off_logt = off[3]; A.off_amb = off[4];
+        A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
+        A.counters = s->d_cnt;
+        HIPCHK(e, chain::launch_reset(e->d_queue, e->stream));
+        HIPCHK(e, chain::launch_kernel(e->M, A, s->fp64, s->BS, gc, 0, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->h_res_d, s->d_res_d, sizeof(double) * 24 * nc, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->h_res_i, s->d_res_i, sizeof(int32_t) * 12 * nc, hipMemcpyDeviceToHost, e->stream));
+    }
+    if (nl) {
+        for (int i = 0; i < nl; i++) {
+            s->h_lids[i] = s->lbatch[i].eid;
+            memcpy(s->h_llen + (size_t)i * N, s->lbatch[i].len.data(), sizeof(double) * N);
+        }
+        HIPCHK(e, hipMemcpyAsync(s->d_lids, s->h_lids, sizeof(int32_t) * nl, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->d_llen, s->h_llen, sizeof(double) * (size_t)nl * N, hipMemcpyHostToDevice, e->stream));
+        ChainArgs A;
+        memset(&A, 0, sizeof A);
+        size_t off[5];
+        A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+        A.work = s->d_lids; A.n_work = nl; A.queue = e->d_queue;
+        A.scratch = s->d_pool + scr_c; A.slot_bytes = (int64_t)slot_layout(N, S, PcapL, s->fp64, off);
+        A.Zin = s->d_lZ; A.rates_in = s->d_lrates; A.ll_out = s->d_lout; A.len_in = s->d_llen;
+        HIPCHK(e, launch_reset(e->d_queue, e->stream));
+        HIPCHK(e, launch_loglik(e->M, A, s->fp64, s->BS, gl, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->h_lout, s->d_lout, sizeof(double) * nl, hipMemcpyDeviceToHost, e->stream));
+    }
     HIPCHK(e, hipEventRecord(s->ev1, e->stream));
     s->in_flight = true;
+    s->in_use = scr_c + (nl ? loglik_launch_scratch(s, PcapL, gl) : 0);
+    s->st.admitted += nc + nl;
     s->st.launches++;
+    return PA_OK;
+}
+
+static pa_status serve_retire(pa_server *s, int32_t *n_finished)
+{
+    pa_engine *e = s->e;
+    float ms = 0;
+    HIPCHK(e, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->st.device_ms += ms;
+    const double now = pa_serve_clock();
+    for (size_t i = 0; i < s->batch.size(); i++) {
+        const Request &r = s->batch[i];
+        pa_serve_result out;
+        memset(&out, 0, sizeof out);
+        out.ticket = r.ticket; out.element = r.eid; out.kind = PA_SERVE_CHAINS;
+        out.arrival_s = r.arrival; out.admit_s = r.admit; out.finish_s = now;
+        int bad = 0;
+        for (int k = 0; k < 3; k++) {
+            const double *d = s->h_res_d + (i * 3 + k) * 8;
+            const int32_t *q = s->h_res_i + (i * 3 + k) * 4;
+            pa_chain_result &c = out.result[k];
+            c.max_loglik = d[0]; c.final_nrate = d[1]; c.final_crate = d[2];
+            c.final_grate = d[3]; c.final_lrate = d[4]; c.final_lrate2 = d[5];
+            c.max_m = q[0]; c.status = q[1]; c.accept_n = q[2]; c.accept_c = q[3];
+            bad |= q[1] != 0;
+        }
+        out.status = bad ? PA_ELEM_NUMERIC_FAIL : PA_ELEM_OK;
+        s->finished.push_back(out);
+    }
+    for (size_t i = 0; i < s->lbatch.size(); i++) {
+        const LRequest &r = s->lbatch[i];
+        pa_serve_result out;
+        memset(&out, 0, sizeof out);
+        out.ticket = r.ticket; out.element = r.eid; out.kind = PA_SERVE_LOGLIK;
+        out.loglik = s->h_lout[i];
+        out.status = std::isfinite(out.loglik) ? PA_ELEM_OK : PA_ELEM_NUMERIC_FAIL;
+        out.arrival_s = r.arrival; out.admit_s = r.admit; out.finish_s = now;
+        s->finished.push_back(out);
+    }
+    const int n = (int)(s->batch.size() + s->lbatch.size());
+    s->st.finished += n;
+    if (n_finished) *n_finished += n;
+    s->batch.clear();
+    s->lbatch.clear();
+    s->in_flight = false;
+    s->in_use = 0;
     return PA_OK;
 }
 
@@ -1836,28 +1634,25 @@ extern "C" pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finish
 {
     if (!s) return PA_E_INPUT;
     pa_engine *e = s->e;
-    int32_t nf = 0;
     if (n_finished) *n_finished = 0;
     HIPCHK(e, hipSetDevice(e->device));
-    if (!wait) {
-        if (s->in_flight) {
+    pa_status st;
+    if (s->in_flight) {
+        if (!wait) {
             const hipError_t q = hipEventQuery(s->ev1);
             if (q == hipErrorNotReady) return PA_OK;
-            if (q != hipSuccess) return fail(e, PA_E_DEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
+            HIPCHK(e, q);
+        } else {
+            HIPCHK(e, hipEventSynchronize(s->ev1));
         }
-        if (!s->lbatch.empty()) {
-            const hipError_t q = hipEventQuery(s->evl);
-            if (q == hipErrorNotReady) return PA_OK;
-            if (q != hipSuccess) return fail(e, PA_E_DEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
-        }
+        if ((st = serve_retire(s, n_finished)) != PA_OK) return st;
     }
-    pa_status st = PA_OK;
-    if (s->in_flight && (st = serve_retire(s, &nf)) != PA_OK) return st;
-    if ((st = serve_retire_loglik(s, &nf)) != PA_OK) return st;
     if ((st = serve_launch(s)) != PA_OK) return st;
-    if ((st = serve_launch_loglik(s)) != PA_OK) return st;
-    if (n_finished) *n_finished = nf;
-    return st;
+    if (wait && s->in_flight) {
+        HIPCHK(e, hipEventSynchronize(s->ev1));
+        if ((st = serve_retire(s, n_finished)) != PA_OK) return st;
+    }
+    return PA_OK;
 }
 
 extern "C" pa_status pa_serve_poll(pa_server *s, pa_serve_result *out, int32_t max, int32_t *n)
@@ -1877,8 +1672,8 @@ extern "C" pa_status pa_serve_get_stats(pa_server *s, pa_serve_stats *out)
     if (!s || !out) return PA_E_INPUT;
     *out = s->st;
     out->waiting = (int32_t)(s->waiting.size() + s->lwait.size());
-    out->in_flight = (int32_t)(s->active.size() + s->lbatch.size());
-    out->scratch_bytes_in_use = (double)s->pool.used;
+    out->in_flight = (int32_t)(s->batch.size() + s->lbatch.size());
+    out->scratch_bytes_in_use = (double)s->in_use;
     return PA_OK;
 }
 
@@ -1888,7 +1683,6 @@ extern "C" void pa_serve_destroy(pa_server *s)
     (void)hipSetDevice(s->e->device);
     if (s->in_flight) (void)hipEventSynchronize(s->ev1);
     (void)hipStreamSynchronize(s->e->stream);
-    (void)hipStreamSynchronize(s->e->stream2);
     serve_free(s);
     delete s;
 }

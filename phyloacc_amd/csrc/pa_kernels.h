@@ -1,5 +1,5 @@
 // pa_kernels.h -- launch interface between the engine host code and the
-// HIP kernels in pa_engine.hip.
+// log-likelihood kernel in pa_engine.hip (the chain kernel: pa_chain.h).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -71,18 +71,7 @@ __host__ __device__ inline size_t slot_layout(int N, int S, int Ppad, int fp64, 
     off[4] = o; o = (o + (size_t)S * 24 * 4 + 255) & ~(size_t)255;              // amb
     return o;
 }
-hipError_t launch_kernel(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, int kind, hipStream_t st);
+hipError_t launch_loglik(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st);
 hipError_t launch_reset(int *q, hipStream_t st);
-
-// Serving-loop launch descriptor, written by the host before each launch.
-struct EpochDesc {
-    int n_work;               // chains to advance
-    int n_fresh;              // newly admitted records to initialise first
-    int pad[2];
-};
-// Initialises the fresh records (header {ms 0, m -1}, done 0, slot offset)
-// and resets the work counter.
-hipError_t launch_admit(const EpochDesc *d, const int32_t *fresh, const int64_t *fresh_off, ChainState *state,
-                        int32_t *done, int64_t *slot_off, int *queue, hipStream_t st);
 
 }  // namespace pa

@@ -77,42 +77,33 @@ struct Stream {
 
 #define PA_GAMMA_MAXIT 64
 
-// FP64 transcendental calls of the one-lane samplers, out of line on the
-// device: inlined, each brings its own temporaries into the caller's
-// register budget, and the caller's values live across the sampler calls are
-// allocated above that budget.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define PA_COLD __device__ __noinline__
-#else
-#define PA_COLD PA_HD
-#endif
-PA_COLD double c_log(double x) { return log(x); }
-PA_COLD double c_pow(double x, double y) { return pow(x, y); }
-PA_COLD double c_cos(double x) { return cos(x); }
-PA_COLD double c_lgamma(double x) { return lgamma(x); }
-
-// Out of line on the device: it runs on one lane, and inlined into its
-// callers its FP64 transcendental temporaries would set their register budget.
-PA_COLD double gamma_draw(Stream &s, double a, double b)
+PA_HD double gamma_draw(Stream &s, double a, double b)
 {
     double boost = 1.0;
     if (a < 1.0) {
         double u = s.next();
-        boost = c_pow(u, 1.0 / a);
+        boost = pow(u, 1.0 / a);
         a += 1.0;
     }
     double d = a - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
     for (int it = 0; it < PA_GAMMA_MAXIT; it++) {
         double u1 = s.next(), u2 = s.next();
-        double x = sqrt(-2.0 * c_log(u1)) * c_cos(6.283185307179586476925 * u2);
+        double x = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
         double v = 1.0 + c * x;
         if (v <= 0.0) continue;
         v = v * v * v;
         double u = s.next();
         if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v * b * boost;
-        if (c_log(u) < 0.5 * x * x + d * (1.0 - v + c_log(v))) return d * v * b * boost;
+        if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v * b * boost;
     }
     return d * b * boost;
+}
+
+PA_HD double beta_draw(Key k, uint32_t cx, uint32_t cy, uint32_t c2, uint32_t c3, double a, double b)
+{
+    Stream sx{k, cx, c2, c3, 0}, sy{k, cy, c2, c3, 0};
+    double x = gamma_draw(sx, a, 1.0), y = gamma_draw(sy, b, 1.0);
+    return x / (x + y);
 }
 
 // Inverse-CDF categorical over k non-negative weights; returns the last
@@ -141,10 +132,10 @@ PA_HD double log_gamma_pdf(double x, double a, double b)
 {
     if (x < 0) return -INFINITY;
     if (x == 0) {
-        if (a == 1) return -c_log(b);
+        if (a == 1) return -log(b);
         return a > 1 ? -INFINITY : INFINITY;
     }
-    return (a - 1) * c_log(x) - x / b - c_lgamma(a) - a * c_log(b);
+    return (a - 1) * log(x) - x / b - lgamma(a) - a * log(b);
 }
 
 }  // namespace pa

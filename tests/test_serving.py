@@ -1,13 +1,11 @@
-"""Continuous-batching request loop: chains resumed across launches give the
-same results as one uninterrupted launch, and a chain's result is the same
-alone and in a mix of other requests arriving while it runs."""
+"""Request loop: chains served in launches of whatever is waiting give the
+same results as one batched run, and a chain's result is the same alone and
+in a mix of other requests arriving while it waits."""
 import numpy as np
 import pytest
 
 from phyloacc_amd import engine as E
 from phyloacc_amd import synth
-
-pytestmark = pytest.mark.gpu
 
 FIELDS = ("max_loglik", "max_m", "status", "final_nrate", "final_crate", "final_grate", "final_lrate",
           "final_lrate2", "accept_n", "accept_c")
@@ -38,72 +36,95 @@ def _as_tuple(r):
     return tuple(getattr(r.result[k], f) for k in range(3) for f in FIELDS)
 
 
+def _slot_bytes(eng, P, fp64):
+    """HBM scratch of one chain workgroup for elements of up to P patterns
+    (padded to 64): partials and exponents (two buffers per internal node),
+    the three rate classes' P(t) and log P(t) tables, ambiguity counts."""
+    al = lambda x: (x + 255) // 256 * 256
+    ppad = (P + 63) // 64 * 64
+    I, N, S, rs = eng.N - eng.S, eng.N, eng.S, 8 if fp64 else 4
+    return al(2 * I * ppad * 4 * rs) + al(2 * I * ppad * 4) + 2 * al(3 * N * 16 * 8) + al(S * 24 * 4)
+
+
+def test_resumable_chains_are_switched_off():
+    eng, _ = _setup(2)
+    with pytest.raises(E.PaError) as ei:
+        E.Server(eng, sweep_budget=5)
+    assert ei.value.status == E.PA_E_STATE and "switched off" in str(ei.value)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", [E.PA_FP64, E.PA_FP32])
-def test_resumed_chains_equal_one_launch(require_gpu, precision):
+def test_served_chains_equal_one_batched_run(require_gpu, precision):
     eng, ids = _setup(16)
     ref = eng.run_chains(ids, precision=precision, block_threads=128)
-    srv = E.Server(eng, precision=precision, block_threads=128, sweep_budget=7)
+    srv = E.Server(eng, precision=precision, block_threads=128, max_chains=5)
     tickets = srv.submit(ids)
     got = _drain(srv, len(ids))
+    assert srv.stats().launches >= 4
     for k, t in enumerate(tickets):
         r = got[t]
-        assert r.element == ids[k]
+        assert r.element == ids[k] and r.kind == E.PA_SERVE_CHAINS
         for ms in range(3):
             for f in FIELDS:
                 assert getattr(r.result[ms], f) == ref[f][k, ms], (k, ms, f)
     srv.close()
 
 
+@pytest.mark.gpu
 def test_same_result_alone_and_in_a_mix(require_gpu):
     eng, ids = _setup(24, seed=8)
     x = int(ids[5])
-    alone = E.Server(eng, precision=E.PA_FP32, sweep_budget=11)
+    alone = E.Server(eng, precision=E.PA_FP32)
     t = alone.submit([x])[0]
     ra = _drain(alone, 1)[t]
     alone.close()
-    mix = E.Server(eng, precision=E.PA_FP32, sweep_budget=11)
+    mix = E.Server(eng, precision=E.PA_FP32, max_chains=7)
     others = [int(i) for i in ids if i != x]
     tick = list(mix.submit(others[:9]))
-    got = {}
-    for _ in range(3):                       # let the first requests get going
-        mix.step(wait=True)
-        got.update({r.ticket: r for r in mix.poll()})
+    mix.step(wait=False)                      # a launch in flight when x arrives
     tx = mix.submit([x])[0]
     tick += [tx]
-    mix.step(wait=False)
     tick += list(mix.submit(others[9:]))
-    got.update(_drain(mix, len(tick) - len(got)))
+    got = _drain(mix, len(tick))
     got.update({r.ticket: r for r in mix.poll()})
     assert _as_tuple(got[tx]) == _as_tuple(ra)
     st = mix.stats()
     assert st.finished == len(tick) and st.in_flight == 0 and st.waiting == 0
+    assert got[tx].arrival_s <= got[tx].admit_s <= got[tx].finish_s
     mix.close()
 
 
-def test_hbm_budget_limits_chains_in_flight(require_gpu):
+@pytest.mark.gpu
+def test_hbm_budget_limits_chains_per_launch(require_gpu):
     eng, ids = _setup(12, seed=4)
     ref = eng.run_chains(ids, precision=E.PA_FP32, block_threads=128)
-    # room for about three of the largest elements' scratch at a time
     biggest = max(eng.element(int(i))["P"] for i in ids)
-    ppad = (biggest + 63) // 64 * 64
-    per = 2 * (eng.N - eng.S) * ppad * 4 * 4 + 2 * (eng.N - eng.S) * ppad * 4 + 2 * 3 * eng.N * 16 * 8 + eng.S * 96
-    srv = E.Server(eng, precision=E.PA_FP32, block_threads=128, sweep_budget=25, hbm_budget_bytes=3.3 * per)
+    budget = 3.3 * _slot_bytes(eng, biggest, False)
+    srv = E.Server(eng, precision=E.PA_FP32, block_threads=128, hbm_budget_bytes=budget)
     tickets = srv.submit(ids)
     got = {}
     peak = 0
     while len(got) < len(ids):
+        srv.step(wait=False)
+        st = srv.stats()
+        assert st.scratch_bytes_in_use <= budget
+        peak = max(peak, st.in_flight)
         srv.step(wait=True)
-        peak = max(peak, srv.stats().in_flight)
         got.update({r.ticket: r for r in srv.poll()})
-    assert 1 <= peak <= 4
+    # one scratch slot per workgroup and one workgroup per chain (12 chains,
+    # far fewer than the GPU's resident workgroups): at most 3 per launch
+    assert 1 <= peak <= 3
+    assert srv.stats().launches >= 4
     for k, t in enumerate(tickets):
         assert got[t].result[1].max_loglik == ref["max_loglik"][k, 1]
     srv.close()
 
 
+@pytest.mark.gpu
 def test_elements_added_while_serving(require_gpu):
     """Elements encoded after the server started move the device arenas; the
-    recorded step is re-recorded and chains in flight carry on unchanged."""
+    next launch uploads them and earlier requests are unaffected."""
     w = synth.make_workload(2, seed=12, n_elements=14)
     eng = E.Engine.from_workload(w, E.default_options(burnin=20, mcmc=40, adapt_freq=15))
     ids_a, st_a = eng.add_elements(w.align, w.starts[:8], w.sizes[:8])
@@ -111,7 +132,7 @@ def test_elements_added_while_serving(require_gpu):
     ref_ids, ref_st = ref_eng.add_elements(w.align, w.starts, w.sizes)
     ref = ref_eng.run_chains(ref_ids[ref_st == E.PA_ELEM_OK], precision=E.PA_FP32, block_threads=128)
     ok_ref = list(ref_ids[ref_st == E.PA_ELEM_OK])
-    srv = E.Server(eng, precision=E.PA_FP32, block_threads=128, sweep_budget=9)
+    srv = E.Server(eng, precision=E.PA_FP32, block_threads=128, max_chains=3)
     t_a = list(srv.submit(ids_a[st_a == E.PA_ELEM_OK]))
     got = {}
     for _ in range(2):
@@ -130,6 +151,7 @@ def test_elements_added_while_serving(require_gpu):
     srv.close()
 
 
+@pytest.mark.gpu
 def test_gene_tree_requests_through_the_loop(require_gpu):
     """Own-tree (gene-tree) log-likelihood requests are served beside chains
     and equal the batched call."""
@@ -139,7 +161,7 @@ def test_gene_tree_requests_through_the_loop(require_gpu):
     L = gw.lengths()
     ref = eng.loglik_lengths(gids, L, E.PA_FP32)
     cids, cst = eng.add_elements(gw.align, gw.starts[:6], gw.sizes[:6])   # species-tree chains alongside
-    srv = E.Server(eng, precision=E.PA_FP32, sweep_budget=10)
+    srv = E.Server(eng, precision=E.PA_FP32)
     tc = list(srv.submit(cids[cst == E.PA_ELEM_OK]))
     tl = list(srv.submit_lengths(gids[:10], L[:10]))
     srv.step(wait=True)
