@@ -65,7 +65,6 @@ def main():
     ap.add_argument("--agree-sample", type=int, default=256)
     ap.add_argument("--arrival-rate", type=float, default=2000.0, help="config 4: requests per second")
     ap.add_argument("--chain", type=int, default=None, help="config 4: states per model chain (BURNIN = 1/4)")
-    ap.add_argument("--sweep-budget", type=int, default=100, help="config 4: sweeps per chain per launch")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0, help="config 4: scratch budget")
     args = ap.parse_args()
 
@@ -114,8 +113,9 @@ def main():
     step_ms = float(np.mean(ms))
     value = sweeps_per_step / (step_ms / 1e3)
     achieved = stats["algorithmic_bytes"] / (step_ms / 1e3) / 1e9
+    traffic, traffic_src = measured_traffic(args, w.C)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "chain_kernel", "algorithmic_bytes_per_launch": stats["algorithmic_bytes"],
                 "grid": stats["grid"], "block_threads": stats["block_threads"], "lds_bytes": stats["lds_bytes"],
                 "launches": stats["n_launches"]}
@@ -221,6 +221,25 @@ def fp32_vs_fp64(E, eng, keep, w, args, rng):
             "identical_Z_posterior_modes": round(float(same_mode.mean()), 4)}
 
 
+def measured_traffic(args, n_elements):
+    """HBM bytes per launch of this exact workload from a committed counter
+    summary under profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    probe), or None when no summary covers it."""
+    key = (f"config={args.config} elements={n_elements} seed={args.seed} burnin={args.burnin} "
+           f"mcmc={args.mcmc} precision={args.precision} block={args.block}")
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:") and f"(workload: {key})" in line:
+                    return float(line.split()[1]), "profiles/" + name
+    return None, None
+
+
 def copy_probe(args):
     """A device-to-device copy of known size (a plain copy kernel), for
     scaling the HBM byte counters of a profiler run: FETCH_SIZE and
@@ -291,10 +310,10 @@ def gene_trees(args, E, synth):
 
 
 def serve(args, E, synth):
-    """Request loop: config-2 elements arrive as a Poisson stream; the server
-    admits them by HBM budget, advances every chain in flight by a sweep
-    budget per launch and retires finished chains.  Reports throughput and
-    per-element completion latency (finish - arrival)."""
+    """Request loop: config-2 elements arrive as a Poisson stream; each step
+    of the server retires the launch that finished and launches every waiting
+    element that fits the HBM budget, running its three chains to the end.
+    Reports throughput and per-element completion latency (finish - arrival)."""
     rate = args.arrival_rate
     w = synth.make_workload(2, seed=args.seed, n_elements=args.elements)
     burnin, mcmc = (args.burnin, args.mcmc) if args.chain is None else (args.chain // 4, args.chain - args.chain // 4)
@@ -307,7 +326,7 @@ def serve(args, E, synth):
     rng = np.random.default_rng(args.seed + 1)
     order = rng.permutation(len(ids))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(ids)))
-    srv = E.Server(eng, precision=prec, block_threads=args.block or 128, sweep_budget=args.sweep_budget,
+    srv = E.Server(eng, precision=prec, block_threads=args.block or 128,
                    hbm_budget_bytes=args.hbm_budget_gib * 2 ** 30)
     # warm the kernel and the library once, outside the timed stream
     srv.submit(ids[order[:4]])
@@ -342,7 +361,7 @@ def serve(args, E, synth):
     bad = sum(1 for r in done if r.status != E.PA_ELEM_OK)
     srv.close()
     out = {
-        "metric": "element-MCMC iterations/s (serving loop, continuous batching)",
+        "metric": "element-MCMC iterations/s (serving loop, batch per step)",
         "value": round(sweeps / wall, 1),
         "unit": "element-MCMC iterations/s",
         "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 3),
@@ -355,7 +374,7 @@ def serve(args, E, synth):
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "max": round(float(lat.max()), 4), "mean": round(float(lat.mean()), 4)},
         "serving": {"arrival_rate_per_s": rate, "offered_s": round(float(arrivals[-1]), 3),
-                    "launches": int(s.launches - base.launches), "sweep_budget": args.sweep_budget,
+                    "launches": int(s.launches - base.launches),
                     "device_busy_frac": round((s.device_ms - base.device_ms) / 1e3 / wall, 4),
                     "numeric_failures": bad},
     }

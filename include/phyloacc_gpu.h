@@ -188,11 +188,10 @@ typedef struct {
 pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
                         pa_chain_result *results, pa_traces *traces, pa_run_stats *stats);
 
-/* Size classes of batched chain launches (no reference counterpart: the
- * reference runs one element per OpenMP thread).  Elements with more than
- * long_patterns site patterns run on long_threads-thread workgroups, the others
- * on small_threads-thread ones (64, 128 or 256), as two concurrent persistent
- * launches.  rate: the element-MCMC iterations/s the autotuner measured. */
+/* Size classes of batched chain launches (no reference counterpart).
+ * Switched off in this build: the three calls below return PA_E_STATE, and a
+ * batched launch uses one block size chosen from the batch's mean pattern
+ * count (or pa_run_config.block_threads). */
 typedef struct {
     int32_t small_threads, long_threads, long_patterns;
     double rate;
@@ -200,9 +199,6 @@ typedef struct {
 
 pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t);
 pa_status pa_get_tuning(pa_engine *e, pa_tuning *t);
-/* Times each candidate size-class configuration on a length-stratified sample
- * (`sample` elements, chains of `states` states) and keeps the fastest for
- * later pa_run_chains calls with block_threads = 0. */
 pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, int32_t precision, int32_t sample,
                       int32_t states, pa_tuning *out);
 

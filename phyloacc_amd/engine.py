@@ -334,7 +334,7 @@ class Engine:
         return d
 
     def autotune(self, ids, precision=PA_FP32, sample=2048, states=16):
-        """Time the size-class candidates on a sample; the best is kept for run_chains."""
+        """Size-class autotuning; switched off in this build (raises PaError, PA_E_STATE)."""
         ids = np.ascontiguousarray(ids, np.int32)
         t = Tuning()
         self._check(lib().pa_autotune(self._h, _p(ids, C.c_int32), len(ids), int(precision), int(sample),
@@ -433,7 +433,8 @@ def rng_uniform(seed, c0, c1, c2, c3, lane):
 
 
 class Server:
-    """Continuous-batching request loop over an engine's encoded elements."""
+    """Request loop over an engine's encoded elements: each step launches the
+    waiting requests that fit the HBM budget and runs their chains to the end."""
 
     def __init__(self, engine: "Engine", precision=PA_FP32, block_threads=0, sweep_budget=0, max_chains=0,
                  hbm_budget_bytes=0.0):

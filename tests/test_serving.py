@@ -112,10 +112,11 @@ def test_hbm_budget_limits_chains_per_launch(require_gpu):
         peak = max(peak, st.in_flight)
         srv.step(wait=True)
         got.update({r.ticket: r for r in srv.poll()})
-    # one scratch slot per workgroup and one workgroup per chain (12 chains,
-    # far fewer than the GPU's resident workgroups): at most 3 per launch
-    assert 1 <= peak <= 3
-    assert srv.stats().launches >= 4
+    # one scratch slot per workgroup, sized for the launch's longest element:
+    # the budget holds 3.3 slots of the longest element overall, so shorter
+    # elements share a launch in larger numbers, but never all twelve
+    assert 1 <= peak < len(ids)
+    assert srv.stats().launches >= 2
     for k, t in enumerate(tickets):
         assert got[t].result[1].max_loglik == ref["max_loglik"][k, 1]
     srv.close()
