@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--cpu-mcmc", type=int, default=550)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true", help="run a plain copy kernel of known size and exit")
+    ap.add_argument("--no-tune", action="store_true", help="skip the block-size autotuner")
     ap.add_argument("--no-agree", action="store_true", help="skip the FP32-vs-FP64 agreement sample")
     ap.add_argument("--agree-sample", type=int, default=256)
     ap.add_argument("--arrival-rate", type=float, default=2000.0, help="config 4: requests per second")
@@ -98,6 +99,11 @@ def main():
     T = args.burnin + args.mcmc
     sweeps_per_step = len(keep) * 3 * (T - 1)
 
+    tuning = None
+    if args.block == 0 and not args.no_tune:
+        tt = time.time()
+        tuning = eng.autotune(keep, precision=prec, states=8)
+        log(f"autotune: {tuning} ({time.time() - tt:.1f}s)")
     for i in range(args.warmup):
         r = eng.run_chains(keep, precision=prec, block_threads=args.block, keep_traces=True)
         log(f"warmup {i}: {r['elapsed_ms'] / 1e3:.2f}s device")
@@ -118,7 +124,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "chain_kernel", "algorithmic_bytes_per_launch": stats["algorithmic_bytes"],
                 "grid": stats["grid"], "block_threads": stats["block_threads"], "lds_bytes": stats["lds_bytes"],
-                "launches": stats["n_launches"]}
+                "launches": stats["n_launches"], "tuning": tuning}
 
     rng = np.random.default_rng(0)
     agree = None

@@ -188,10 +188,14 @@ typedef struct {
 pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
                         pa_chain_result *results, pa_traces *traces, pa_run_stats *stats);
 
-/* Size classes of batched chain launches (no reference counterpart).
- * Switched off in this build: the three calls below return PA_E_STATE, and a
- * batched launch uses one block size chosen from the batch's mean pattern
- * count (or pa_run_config.block_threads). */
+/* Block size of batched chain launches (no reference counterpart: the
+ * reference runs one element per OpenMP thread).  A batched launch uses one
+ * block size: small_threads == long_threads (64, 128 or 256), or both 0 for
+ * the default chosen from the batch's mean pattern count.  Split launches
+ * (small_threads != long_threads: two block sizes side by side, elements
+ * above long_patterns on the larger) are switched off in this build and
+ * rejected with PA_E_STATE.  rate: the element-MCMC iterations/s the
+ * autotuner measured. */
 typedef struct {
     int32_t small_threads, long_threads, long_patterns;
     double rate;
@@ -199,6 +203,10 @@ typedef struct {
 
 pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t);
 pa_status pa_get_tuning(pa_engine *e, pa_tuning *t);
+/* Times block sizes 64, 128 and 256 on the batch (or, with sample > 0, on
+ * `sample` of its elements evenly spaced by pattern count) with chains of
+ * `states` states (0: 8) and keeps the fastest for later pa_run_chains calls
+ * with block_threads = 0. */
 pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, int32_t precision, int32_t sample,
                       int32_t states, pa_tuning *out);
 

@@ -2,10 +2,10 @@
 //
 // The host launches chain_kernel from this file and the fixed-(Z, r)
 // log-likelihood kernel from pa_engine.hip.  The loglik_kernel below is
-// never launched; it stays because the compiler derives the code of the
-// functions it shares with chain_kernel (prune, bind_element, refresh_PTc,
-// init_model) from all their callers, and this file is kept compiling to
-// exactly the chain-kernel code that has run on the GPU.
+// never launched.  The compiler derives the code of the functions it shares
+// with chain_kernel (prune, bind_element, refresh_PTc, init_model) from all
+// their callers, and without it those functions compile differently (prune
+// spills to AGPRs), so it stays until that difference is measured.
 //
 // One workgroup owns one element at a time and runs its three model chains
 // (M0, M1, M2 = resZ 0, 2, 1; src/PhyloAcc-ST/main.cpp:466-494) start to
@@ -29,6 +29,7 @@
 // allocation spanning the whole persistent loop.
 // No MFMA: a K=4 matvec per pattern is not a dense contraction.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 #include "pa_common.h"
 #include "pa_rng.h"
@@ -69,11 +70,12 @@ struct Ctx {
     GBL double *plin;      // [3][N][16]  P_{j->i} at [j*4+i]
     GBL double *logt;      // [3][N][16]  log P_{j->i} at [i*4+j]
     GBL uint32_t *amb;     // [S][24]
+    const GBL double *dist2;   // [N] branch lengths (element node table)
     // LDS arrays
     LDS int16_t *ch0, *ch1, *par, *nodes, *inodes, *up, *upoff, *down, *downoff, *tz;
     LDS uint8_t *flags, *fixZ, *dirty, *cur, *changed;
     LDS int8_t *Z, *maxZ;
-    LDS double *dist2, *E, *beta;
+    LDS double *E, *beta;
     LDS R *PTc, *PTp;      // [N][16], PT[i*4+j] = P_{j->i}
     LDS uint32_t *hist, *cnt9;
     LDS uint8_t *tgs;
@@ -102,16 +104,34 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
     size_t o = 0;
 #define PA_TAKE(field, T, count)                                                   \
     do {                                                                           \
-        o = al(o, 32);  /* rows of PT are read as 4-vectors */                      \
+        o = al(o, sizeof(T) < 4 ? 4 : sizeof(T));                                  \
         if constexpr (BIND) c->field = (LDS T *)(base + o);                        \
         o += sizeof(T) * (size_t)(count);                                          \
     } while (0)
-    PA_TAKE(dist2, double, N);
-    PA_TAKE(E, double, 3 * N);
-    PA_TAKE(beta, double, 3 * N);
+    // matrices first: rows of PT are read as 4-vectors (32-byte aligned base)
     PA_TAKE(PTc, R, 16 * N);
-    PA_TAKE(PTp, R, 16 * N);
-    PA_TAKE(hist, uint32_t, 16 * N);
+    // Phases that never overlap share storage:
+    //   A: per-thread ancestral states (sample_sites) | emissions and Z
+    //      messages (emissions, update_Z);
+    //   B: per-branch transition counters (sample_sites, emissions) |
+    //      proposal matrices (rate updates).  sample_sites clears the
+    //      counters before it counts.
+    {
+        const size_t szA = std::max((size_t)(I > 0 ? I : 1) * BS, (size_t)6 * N * sizeof(double));
+        const size_t szB = std::max((size_t)16 * N * sizeof(uint32_t), (size_t)16 * N * sizeof(R));
+        o = al(o, 32);
+        if constexpr (BIND) {
+            c->PTp = (LDS R *)(base + o);
+            c->hist = (LDS uint32_t *)(base + o);
+        }
+        o += al(szB, 32);
+        if constexpr (BIND) {
+            c->E = (LDS double *)(base + o);
+            c->beta = (LDS double *)(base + o) + 3 * N;
+            c->tgs = (LDS uint8_t *)(base + o);
+        }
+        o += szA;
+    }
     PA_TAKE(cnt9, uint32_t, 20);
     PA_TAKE(ch0, int16_t, N);
     PA_TAKE(ch1, int16_t, N);
@@ -130,7 +150,6 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
     PA_TAKE(changed, uint8_t, N);
     PA_TAKE(Z, int8_t, N);
     PA_TAKE(maxZ, int8_t, N);
-    PA_TAKE(tgs, uint8_t, (size_t)(I > 0 ? I : 1) * BS);
 #undef PA_TAKE
     return al(o, 16);
 }
@@ -382,6 +401,8 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
     const GBL int32_t *exps = c->exps;
     const GBL uint8_t *codes = c->codes;
     LDS uint8_t *tgs = c->tgs;
+    for (int i = tid; i < 16 * c->N; i += BS) c->hist[i] = 0;   // shares storage with the proposal matrices
+    __syncthreads();
     if (tid == 0) {
         // per site: pattern index, root partial, each non-missing internal node's partial, each leaf code
         int nint = 0, nleaf = 0;
@@ -784,6 +805,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
         c->plin = (GBL double *)(base + A->off_plin);
         c->logt = (GBL double *)(base + A->off_logt);
         c->amb = (GBL uint32_t *)(base + A->off_amb);
+        c->dist2 = (const GBL double *)(A->tab + E.off_tab + tab_dist(N));
         c->accept_n = 0;
         c->accept_c = 0;
         c->bytes = 0;
@@ -799,7 +821,6 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
         c->up[i] = (int16_t)tab[tab_up(N) + i];
         c->down[i] = (int16_t)tab[tab_down(N) + i];
         c->tz[i] = (int16_t)tab[tab_tz(N) + i];
-        c->dist2[i] = ((const double *)(tab + tab_dist(N)))[i];
         for (int k = 0; k < 16; k++) c->hist[16 * i + k] = 0;
         c->cur[i] = 0;
         c->dirty[i] = 0;

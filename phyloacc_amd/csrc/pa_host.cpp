@@ -515,6 +515,7 @@ struct pa_engine {
     hipStream_t stream = nullptr;
     int n_cu = 0;
     size_t lds_per_cu = 0;
+    pa_tuning tune = {0, 0, 0, 0.0};        // block size of batched chain launches; 0: by mean pattern count
     uint64_t arena_gen = 0;                  // bumped whenever the device arenas move
 };
 
@@ -874,7 +875,7 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
                                   chain::ChainArgs &A, DevBuf *bufs, pa_run_stats *stats)
 {
     const int S = e->tm.S, N = e->tm.N, T = e->M.T;
-    int BS = bs_req ? bs_req : pick_block(e, ids, n);
+    int BS = bs_req ? bs_req : (e->tune.small_threads ? e->tune.small_threads : pick_block(e, ids, n));
     if (BS != 64 && BS != 128 && BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
     const size_t smem = chain::chain_smem_bytes(N, S, BS, fp64);
     if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
@@ -1273,36 +1274,82 @@ extern "C" pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, c
     return PA_OK;
 }
 
-// ------------------------------------------------ switched-off entry points
+// ------------------------------------------------------------- autotuner
 //
-// Size-class launches (two block sizes on two streams), their autotuner and
-// the resumable serving loop ran on a chain kernel that did not finish on
-// the GPU; they are switched off until that kernel's failure is explained.
-// Each call below fails with PA_E_STATE and says so.
-
-static pa_status switched_off(pa_engine *e, const char *what)
-{
-    return fail(e, PA_E_STATE, std::string(what) + " is switched off in this build (batched chain launches use one block size on one stream)");
-}
+// Batched chain launches use one block size.  pa_autotune times each
+// candidate on the batch itself with short chains and keeps the fastest for
+// later pa_run_chains calls with block_threads = 0.  Split launches (two
+// block sizes side by side on two streams, small_threads != long_threads)
+// are switched off in this build: their chain kernel did not finish on the
+// GPU and its failure is not explained yet.
 
 extern "C" pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t)
 {
     if (!e || !t) return fail(e, PA_E_INPUT, "null argument");
-    return switched_off(e, "size-class tuning");
+    if (t->small_threads == 0 && t->long_threads == 0) { e->tune = {0, 0, 0, 0.0}; return PA_OK; }
+    auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
+    if (!ok(t->small_threads) || !ok(t->long_threads) || t->long_patterns < 0)
+        return fail(e, PA_E_INPUT, "tuning: block sizes must be 64, 128 or 256");
+    if (t->small_threads != t->long_threads)
+        return fail(e, PA_E_STATE, "size classes (small_threads != long_threads) are switched off in this build");
+    e->tune = *t;
+    return PA_OK;
 }
 
 extern "C" pa_status pa_get_tuning(pa_engine *e, pa_tuning *t)
 {
     if (!e || !t) return fail(e, PA_E_INPUT, "null argument");
-    return switched_off(e, "size-class tuning");
+    *t = e->tune;
+    return PA_OK;
 }
 
 extern "C" pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, int32_t precision, int32_t sample,
                                  int32_t states, pa_tuning *out)
 {
-    (void)ids; (void)n; (void)precision; (void)sample; (void)states; (void)out;
-    if (!e) return PA_E_INPUT;
-    return switched_off(e, "size-class autotuning");
+    if (!e || !ids || n <= 0) return fail(e, PA_E_INPUT, "null argument");
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    if (states <= 1) states = 8;
+    // the batch itself (or an evenly spaced subset of it, by pattern count)
+    std::vector<int32_t> smp(ids, ids + n);
+    if (sample > 0 && sample < n) {
+        std::stable_sort(smp.begin(), smp.end(), [&](int a, int b) { return e->elems[a].P < e->elems[b].P; });
+        std::vector<int32_t> sub;
+        for (int i = 0; i < sample; i++) sub.push_back(smp[(size_t)((i + 0.5) * n / sample)]);
+        smp.swap(sub);
+    }
+    const std::vector<int32_t> order = schedule(e, smp.data(), (int32_t)smp.size(), PA_RUN_BATCHED);
+    std::vector<int32_t> sorted(smp.size());
+    for (size_t i = 0; i < smp.size(); i++) sorted[i] = smp[order[i]];
+    const int N = e->tm.N, m = (int)sorted.size();
+    const ModelDev saved = e->M;
+    e->M.T = states;
+    e->M.burnin = states / 4;
+    e->M.mcmc = states - states / 4;
+    pa_tuning best = {0, 0, 0, -1.0};
+    const int cand[3] = {64, 128, 256};
+    for (int a = 0; a < 3 && st == PA_OK; a++) {
+        DevBuf bufs[8];
+        pa_run_stats rs;
+        chain::ChainArgs A;
+        memset(&A, 0, sizeof A);
+        if (hipMalloc(&bufs[3].p, (size_t)m * 3 * 8 * 8) != hipSuccess ||
+            hipMalloc(&bufs[4].p, (size_t)m * 3 * 4 * 4) != hipSuccess ||
+            hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) { st = fail(e, PA_E_OOM, "autotune buffers"); break; }
+        A.res_d = (double *)bufs[3].p;
+        A.res_i = (int32_t *)bufs[4].p;
+        A.maxZ = (int8_t *)bufs[5].p;
+        st = run_chain_kernel(e, sorted.data(), m, precision == PA_FP64, PA_RUN_BATCHED, cand[a], A, bufs, &rs);
+        if (st != PA_OK) break;
+        const double rate = (double)m * 3 * (states - 1) / (rs.elapsed_ms / 1e3);
+        if (rate > best.rate) best = {cand[a], cand[a], 0, rate};
+    }
+    e->M = saved;
+    if (st != PA_OK) return st;
+    e->tune = best;
+    if (out) *out = best;
+    return PA_OK;
 }
 
 // ------------------------------------------------------------ serving loop

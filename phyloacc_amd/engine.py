@@ -333,15 +333,17 @@ class Engine:
                  codes=codes.reshape(info.GG, S), site_pattern=site, weights=w[: info.P])
         return d
 
-    def autotune(self, ids, precision=PA_FP32, sample=2048, states=16):
-        """Size-class autotuning; switched off in this build (raises PaError, PA_E_STATE)."""
+    def autotune(self, ids, precision=PA_FP32, sample=0, states=8):
+        """Time block sizes 64/128/256 on the batch (sample > 0: on that many
+        elements evenly spaced by pattern count) with short chains; the fastest
+        is kept for run_chains(block_threads=0)."""
         ids = np.ascontiguousarray(ids, np.int32)
         t = Tuning()
         self._check(lib().pa_autotune(self._h, _p(ids, C.c_int32), len(ids), int(precision), int(sample),
                                       int(states), C.byref(t)))
         return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
-    def set_tuning(self, small_threads=64, long_threads=256, long_patterns=256):
+    def set_tuning(self, small_threads=0, long_threads=0, long_patterns=0):
         self._check(lib().pa_set_tuning(self._h, C.byref(Tuning(small_threads, long_threads, long_patterns, 0.0))))
 
     def tuning(self):
