@@ -424,21 +424,44 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
             }
             tgs[(r - S) * BS + tid] = (uint8_t)categorical<4>(pr, u01(blk.x));
         }
+        // internal nodes in reverse post-order, K at a time: a chunk's partial
+        // loads do not depend on its draws, so they are all issued first
+        constexpr int K = sizeof(R) == 4 ? 8 : 4;
         int t = 1;
-        for (int k = n_int - 2; k >= 0; k--, t++) {
-            if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
-            const int s = c->inodes[k];
-            if (c->flags[s] & NF_MISSING) continue;
-            const size_t off = c->cur[s] * bufI + (size_t)(s - S) * Ppad + p;
-            if (exps[off] == GAPT) continue;
-            const int tp = tgs[(c->par[s] - S) * BS + tid];
-            const V4<R> a = ld4<R>(parts + off);
-            const LDS R *PT = c->PTc + 16 * s;
-            double pr[4] = {(double)PT[0 * 4 + tp] * (double)a.x, (double)PT[1 * 4 + tp] * (double)a.y,
-                            (double)PT[2 * 4 + tp] * (double)a.z, (double)PT[3 * 4 + tp] * (double)a.w};
-            const int ts = categorical<4>(pr, u01(pick(blk, t & 3)));
-            tgs[(s - S) * BS + tid] = (uint8_t)ts;
-            lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
+        for (int kb = n_int - 2; kb >= 0; kb -= K) {
+            int ex[K], sn[K];
+            V4<R> av[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int k = kb - j;
+                ex[j] = GAPT;
+                sn[j] = 0;
+                av[j].x = av[j].y = av[j].z = av[j].w = (R)0;
+                if (k >= 0) {
+                    const int s = c->inodes[k];
+                    sn[j] = s;
+                    if (!(c->flags[s] & NF_MISSING)) {
+                        const size_t off = c->cur[s] * bufI + (size_t)(s - S) * Ppad + p;
+                        ex[j] = exps[off];
+                        av[j] = ld4<R>(parts + off);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++, t++) {
+                if (kb - j < 0) break;
+                if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
+                if (ex[j] == GAPT) continue;     // missing node, or an all-gap subtree at this site
+                const int s = sn[j];
+                const int tp = tgs[(c->par[s] - S) * BS + tid];
+                const V4<R> a = av[j];
+                const LDS R *PT = c->PTc + 16 * s;
+                double pr[4] = {(double)PT[0 * 4 + tp] * (double)a.x, (double)PT[1 * 4 + tp] * (double)a.y,
+                                (double)PT[2 * 4 + tp] * (double)a.z, (double)PT[3 * 4 + tp] * (double)a.w};
+                const int ts = categorical<4>(pr, u01(pick(blk, t & 3)));
+                tgs[(s - S) * BS + tid] = (uint8_t)ts;
+                lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
+            }
         }
         for (int s = 0; s < S; s++) {
             const uint8_t f = c->flags[s];
