@@ -273,30 +273,52 @@ __device__ __forceinline__ void leaf_mul(const LDS R *PT, int code, R &a0, R &a1
     }
 }
 
+// One child's inputs at one pattern: a leaf's code, or an internal child's
+// scaled partial and cumulative exponent.
+template <typename R>
+struct ChildIn {
+    V4<R> v;
+    int x;
+};
+
+template <typename R>
+__device__ __forceinline__ ChildIn<R> child_load(bool leaf, bool miss, const GBL uint8_t *code_row,
+                                                 const GBL V4<R> *part_row, const GBL int32_t *exp_row, int p)
+{
+    // a missing child contributes nothing, but its gap state still decides
+    // whether the parent is all-gap at this pattern, so its code / exponent
+    // is read either way
+    ChildIn<R> in;
+    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
+    if (leaf) {
+        in.x = code_row[p];
+    } else {
+        in.x = exp_row[p];
+        if (!miss) in.v = ld4<R>(part_row + p);
+    }
+    return in;
+}
+
 // Multiplies one child's message P * L_child into (a0..a3).  A leaf reads the
 // row of its base from the LDS matrix; an internal child uses the register
 // copy m.  gap reports the child's GAPT state (Tg >= num_base).
 template <typename R>
-__device__ __forceinline__ void child_term(bool leaf, bool miss, const LDS R *src, const M16<R> &m,
-                                           const GBL uint8_t *code_row, const GBL V4<R> *part_row,
-                                           const GBL int32_t *exp_row, int p, R &a0, R &a1, R &a2, R &a3, int &e,
-                                           bool &gap)
+__device__ __forceinline__ void child_apply(bool leaf, bool miss, const LDS R *src, const M16<R> &m,
+                                            const ChildIn<R> &in, R &a0, R &a1, R &a2, R &a3, int &e, bool &gap)
 {
     if (leaf) {
-        const int code = code_row[p];
-        gap = (code == PA_GAP_CODE || code == PA_OTHER_CODE);
-        if (!miss && !gap) leaf_mul<R>(src, code, a0, a1, a2, a3);
+        gap = (in.x == PA_GAP_CODE || in.x == PA_OTHER_CODE);
+        if (!miss && !gap) leaf_mul<R>(src, in.x, a0, a1, a2, a3);
     } else {
-        const int ex = exp_row[p];
-        gap = (ex == GAPT);
+        gap = (in.x == GAPT);
         if (!miss && !gap) {
-            const V4<R> v = ld4<R>(part_row + p);
+            const V4<R> v = in.v;
             const R o0 = m.v[0] * v.x + m.v[4] * v.y + m.v[8] * v.z + m.v[12] * v.w;
             const R o1 = m.v[1] * v.x + m.v[5] * v.y + m.v[9] * v.z + m.v[13] * v.w;
             const R o2 = m.v[2] * v.x + m.v[6] * v.y + m.v[10] * v.z + m.v[14] * v.w;
             const R o3 = m.v[3] * v.x + m.v[7] * v.y + m.v[11] * v.z + m.v[15] * v.w;
             a0 *= o0; a1 *= o1; a2 *= o2; a3 *= o3;
-            e += ex;
+            e += in.x;
         }
     }
 }
@@ -344,11 +366,14 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
         if (tid == 0)   // write partial + exponent; read each child's partial + exponent, or its leaf code
             c->bytes += (unsigned long long)P * ((4 * sizeof(R) + 4) + (leaf0 ? 1 : 4 * sizeof(R) + 4) + (leaf1 ? 1 : 4 * sizeof(R) + 4));
         for (int p = tid; p < P; p += BS) {
+            // both children's loads are issued before either is used
+            const ChildIn<R> in0 = child_load<R>(leaf0, miss0, cr0, pr0, er0, p);
+            const ChildIn<R> in1 = child_load<R>(leaf1, miss1, cr1, pr1, er1, p);
             R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
             int e = 0;
             bool g0, g1;
-            child_term<R>(leaf0, miss0, src0, m0, cr0, pr0, er0, p, a0, a1, a2, a3, e, g0);
-            child_term<R>(leaf1, miss1, src1, m1, cr1, pr1, er1, p, a0, a1, a2, a3, e, g1);
+            child_apply<R>(leaf0, miss0, src0, m0, in0, a0, a1, a2, a3, e, g0);
+            child_apply<R>(leaf1, miss1, src1, m1, in1, a0, a1, a2, a3, e, g1);
             if (g0 && g1) {
                 dste[p] = GAPT;
             } else if (miss_s) {
