@@ -488,14 +488,26 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                 lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
             }
         }
-        for (int s = 0; s < S; s++) {
-            const uint8_t f = c->flags[s];
-            if (!(f & NF_IN_TREE) || (f & NF_MISSING)) continue;
-            const int code = codes[(size_t)s * Ppad + p];
-            if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
-            const int tp = tgs[(c->par[s] - S) * BS + tid];
-            if (code < 4) lds_add(&c->hist[s * 16 + tp * 4 + code], 1u);
-            else __hip_atomic_fetch_add(&c->amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // leaves, 16 codes loaded at a time
+        for (int sb = 0; sb < S; sb += 16) {
+            int cd[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int s = sb + j;
+                cd[j] = PA_GAP_CODE;
+                if (s < S) {
+                    const uint8_t f = c->flags[s];
+                    if ((f & NF_IN_TREE) && !(f & NF_MISSING)) cd[j] = codes[(size_t)s * Ppad + p];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int s = sb + j, code = cd[j];
+                if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
+                const int tp = tgs[(c->par[s] - S) * BS + tid];
+                if (code < 4) lds_add(&c->hist[s * 16 + tp * 4 + code], 1u);
+                else __hip_atomic_fetch_add(&c->amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
     }
 }
