@@ -348,11 +348,17 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
         const bool miss0 = c->flags[x0] & NF_MISSING, miss1 = c->flags[x1] & NF_MISSING;
         const LDS R *src0 = (mode == PR_PROP && c->Z[x0] == cls) ? c->PTp + 16 * x0 : c->PTc + 16 * x0;
         const LDS R *src1 = (mode == PR_PROP && c->Z[x1] == cls) ? c->PTp + 16 * x1 : c->PTc + 16 * x1;
+        // an internal child's matrix in registers, four 4-vector reads each
         M16<R> m0, m1;
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            m0.v[i] = leaf0 ? (R)0 : src0[i];
-            m1.v[i] = leaf1 ? (R)0 : src1[i];
+        for (int q = 0; q < 4; q++) {
+            V4<R> r0, r1;
+            r0.x = r0.y = r0.z = r0.w = (R)0;
+            r1 = r0;
+            if (!leaf0) r0 = ld4<R>((const LDS V4<R> *)(src0 + 4 * q));
+            if (!leaf1) r1 = ld4<R>((const LDS V4<R> *)(src1 + 4 * q));
+            m0.v[4 * q] = r0.x; m0.v[4 * q + 1] = r0.y; m0.v[4 * q + 2] = r0.z; m0.v[4 * q + 3] = r0.w;
+            m1.v[4 * q] = r1.x; m1.v[4 * q + 1] = r1.y; m1.v[4 * q + 2] = r1.z; m1.v[4 * q + 3] = r1.w;
         }
         const int rb0 = leaf0 ? 0 : ((mode == PR_PROP && c->dirty[x0]) ? 1 - c->cur[x0] : c->cur[x0]);
         const int rb1 = leaf1 ? 0 : ((mode == PR_PROP && c->dirty[x1]) ? 1 - c->cur[x1] : c->cur[x1]);
@@ -451,7 +457,7 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
         }
         // internal nodes in reverse post-order, K at a time: a chunk's partial
         // loads do not depend on its draws, so they are all issued first
-        constexpr int K = sizeof(R) == 4 ? 8 : 4;
+        constexpr int K = sizeof(R) == 4 ? 4 : 2;
         int t = 1;
         for (int kb = n_int - 2; kb >= 0; kb -= K) {
             int ex[K], sn[K];
@@ -488,11 +494,11 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                 lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
             }
         }
-        // leaves, 16 codes loaded at a time
-        for (int sb = 0; sb < S; sb += 16) {
-            int cd[16];
+        // leaves, 8 codes loaded at a time
+        for (int sb = 0; sb < S; sb += 8) {
+            int cd[8];
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
+            for (int j = 0; j < 8; j++) {
                 const int s = sb + j;
                 cd[j] = PA_GAP_CODE;
                 if (s < S) {
@@ -501,7 +507,7 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                 }
             }
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
+            for (int j = 0; j < 8; j++) {
                 const int s = sb + j, code = cd[j];
                 if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
                 const int tp = tgs[(c->par[s] - S) * BS + tid];
@@ -1023,7 +1029,7 @@ __device__ __forceinline__ MP model_to_lds(const ModelDev &Mk, LDS ModelDev *Ms)
 }
 
 template <typename R, int BS>
-__global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs A)
+__global__ void __launch_bounds__(BS, 4) chain_kernel(ModelDev Mk, ChainArgs A)
 {
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ ModelDev Ms;
@@ -1047,7 +1053,7 @@ __global__ void __launch_bounds__(BS) chain_kernel(ModelDev Mk, ChainArgs A)
 
 // Log-likelihood at fixed Z and rates (BPP_C::log_f_Xz, bpp_c2.cpp:392-425)
 template <typename R, int BS>
-__global__ void __launch_bounds__(BS) loglik_kernel(ModelDev Mk, ChainArgs A)
+__global__ void __launch_bounds__(BS, 4) loglik_kernel(ModelDev Mk, ChainArgs A)
 {
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ ModelDev Ms;
