@@ -27,10 +27,6 @@ struct ChainArgs {
     int8_t *maxZ;             // [n_work][3][N]
     double *tr_d;             // [n_work][3][7][T] or null
     int8_t *tr_Z;             // [n_work][3][T][N] or null
-    // fixed-(Z, r) log-likelihood (this file's loglik_kernel, not launched)
-    const int32_t *Zin;       // [n_work][N]
-    const double *rates_in;   // [n_work][2] (r1, r2)
-    double *ll_out;           // [n_work]
     unsigned long long *counters;   // [0]: algorithmic HBM bytes of the launch (or null)
 };
 
@@ -39,8 +35,7 @@ size_t chain_smem_bytes(int N, int S, int BS, int fp64);
 // Byte layout of one workgroup's HBM slot for elements of up to Pcap
 // (padded) patterns: off = {parts, exps, plin, logt, amb}; returns its size.
 size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off /*[5]*/);
-// kind 0: chain_kernel (the only kind the host launches)
-hipError_t launch_kernel(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, int kind, hipStream_t st);
+hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st);
 hipError_t launch_reset(int *q, hipStream_t st);
 
 }  // namespace chain

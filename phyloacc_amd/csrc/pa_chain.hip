@@ -1,11 +1,6 @@
 // pa_chain.hip -- CDNA4 (gfx950) chain kernel for PhyloAcc-ST's per-element MCMC.
 //
-// The host launches chain_kernel from this file and the fixed-(Z, r)
-// log-likelihood kernel from pa_engine.hip.  The loglik_kernel below is
-// never launched.  The compiler derives the code of the functions it shares
-// with chain_kernel (prune, bind_element, refresh_PTc, init_model) from all
-// their callers, and without it those functions compile differently (prune
-// spills to AGPRs), so it stays until that difference is measured.
+// The fixed-(Z, r) log-likelihood kernel is in pa_engine.hip.
 //
 // One workgroup owns one element at a time and runs its three model chains
 // (M0, M1, M2 = resZ 0, 2, 1; src/PhyloAcc-ST/main.cpp:466-494) start to
@@ -255,9 +250,6 @@ __device__ __noinline__ void refresh_PTc(CP<R> c, int only_cls)
 enum { PR_FULL = 0, PR_DIRTY = 1, PR_PROP = 2 };
 
 template <typename R>
-struct M16 { R v[16]; };
-
-template <typename R>
 __device__ __forceinline__ void leaf_mul(const LDS R *PT, int code, R &a0, R &a1, R &a2, R &a3)
 {
     if (code < 4) {
@@ -300,11 +292,11 @@ __device__ __forceinline__ ChildIn<R> child_load(bool leaf, bool miss, const GBL
 }
 
 // Multiplies one child's message P * L_child into (a0..a3).  A leaf reads the
-// row of its base from the LDS matrix; an internal child uses the register
-// copy m.  gap reports the child's GAPT state (Tg >= num_base).
+// row of its base from the LDS matrix; an internal child reads the whole
+// matrix as four 4-vectors.  gap reports the child's GAPT state (Tg >= num_base).
 template <typename R>
-__device__ __forceinline__ void child_apply(bool leaf, bool miss, const LDS R *src, const M16<R> &m,
-                                            const ChildIn<R> &in, R &a0, R &a1, R &a2, R &a3, int &e, bool &gap)
+__device__ __forceinline__ void child_apply(bool leaf, bool miss, const LDS R *src, const ChildIn<R> &in,
+                                            R &a0, R &a1, R &a2, R &a3, int &e, bool &gap)
 {
     if (leaf) {
         gap = (in.x == PA_GAP_CODE || in.x == PA_OTHER_CODE);
@@ -313,10 +305,12 @@ __device__ __forceinline__ void child_apply(bool leaf, bool miss, const LDS R *s
         gap = (in.x == GAPT);
         if (!miss && !gap) {
             const V4<R> v = in.v;
-            const R o0 = m.v[0] * v.x + m.v[4] * v.y + m.v[8] * v.z + m.v[12] * v.w;
-            const R o1 = m.v[1] * v.x + m.v[5] * v.y + m.v[9] * v.z + m.v[13] * v.w;
-            const R o2 = m.v[2] * v.x + m.v[6] * v.y + m.v[10] * v.z + m.v[14] * v.w;
-            const R o3 = m.v[3] * v.x + m.v[7] * v.y + m.v[11] * v.z + m.v[15] * v.w;
+            const V4<R> r0 = ld4<R>((const LDS V4<R> *)src), r1 = ld4<R>((const LDS V4<R> *)(src + 4));
+            const V4<R> r2 = ld4<R>((const LDS V4<R> *)(src + 8)), r3 = ld4<R>((const LDS V4<R> *)(src + 12));
+            const R o0 = r0.x * v.x + r1.x * v.y + r2.x * v.z + r3.x * v.w;
+            const R o1 = r0.y * v.x + r1.y * v.y + r2.y * v.z + r3.y * v.w;
+            const R o2 = r0.z * v.x + r1.z * v.y + r2.z * v.z + r3.z * v.w;
+            const R o3 = r0.w * v.x + r1.w * v.y + r2.w * v.z + r3.w * v.w;
             a0 *= o0; a1 *= o1; a2 *= o2; a3 *= o3;
             e += in.x;
         }
@@ -348,18 +342,6 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
         const bool miss0 = c->flags[x0] & NF_MISSING, miss1 = c->flags[x1] & NF_MISSING;
         const LDS R *src0 = (mode == PR_PROP && c->Z[x0] == cls) ? c->PTp + 16 * x0 : c->PTc + 16 * x0;
         const LDS R *src1 = (mode == PR_PROP && c->Z[x1] == cls) ? c->PTp + 16 * x1 : c->PTc + 16 * x1;
-        // an internal child's matrix in registers, four 4-vector reads each
-        M16<R> m0, m1;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            V4<R> r0, r1;
-            r0.x = r0.y = r0.z = r0.w = (R)0;
-            r1 = r0;
-            if (!leaf0) r0 = ld4<R>((const LDS V4<R> *)(src0 + 4 * q));
-            if (!leaf1) r1 = ld4<R>((const LDS V4<R> *)(src1 + 4 * q));
-            m0.v[4 * q] = r0.x; m0.v[4 * q + 1] = r0.y; m0.v[4 * q + 2] = r0.z; m0.v[4 * q + 3] = r0.w;
-            m1.v[4 * q] = r1.x; m1.v[4 * q + 1] = r1.y; m1.v[4 * q + 2] = r1.z; m1.v[4 * q + 3] = r1.w;
-        }
         const int rb0 = leaf0 ? 0 : ((mode == PR_PROP && c->dirty[x0]) ? 1 - c->cur[x0] : c->cur[x0]);
         const int rb1 = leaf1 ? 0 : ((mode == PR_PROP && c->dirty[x1]) ? 1 - c->cur[x1] : c->cur[x1]);
         const GBL uint8_t *cr0 = codes + (size_t)x0 * Ppad, *cr1 = codes + (size_t)x1 * Ppad;
@@ -378,8 +360,8 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
             R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
             int e = 0;
             bool g0, g1;
-            child_apply<R>(leaf0, miss0, src0, m0, in0, a0, a1, a2, a3, e, g0);
-            child_apply<R>(leaf1, miss1, src1, m1, in1, a0, a1, a2, a3, e, g1);
+            child_apply<R>(leaf0, miss0, src0, in0, a0, a1, a2, a3, e, g0);
+            child_apply<R>(leaf1, miss1, src1, in1, a0, a1, a2, a3, e, g1);
             if (g0 && g1) {
                 dste[p] = GAPT;
             } else if (miss_s) {
@@ -1051,57 +1033,15 @@ __global__ void __launch_bounds__(BS, 4) chain_kernel(ModelDev Mk, ChainArgs A)
     }
 }
 
-// Log-likelihood at fixed Z and rates (BPP_C::log_f_Xz, bpp_c2.cpp:392-425)
-template <typename R, int BS>
-__global__ void __launch_bounds__(BS, 4) loglik_kernel(ModelDev Mk, ChainArgs A)
-{
-    extern __shared__ __align__(16) char smem_raw[];
-    __shared__ ModelDev Ms;
-    __shared__ Ctx<R> Cs;
-    MP M = model_to_lds(Mk, (LDS ModelDev *)&Ms);
-    CP<R> c = (CP<R>)&Cs;
-    __syncthreads();
-    if (threadIdx.x == 0) smem_arrays<R, true>(M->N, M->S, BS, (LDS char *)smem_raw, c);
-    __syncthreads();
-    for (;;) {
-        if (threadIdx.x == 0) c->work = atomicAdd(A.queue, 1);
-        __syncthreads();
-        const int w = c->work;
-        __syncthreads();
-        if (w >= A.n_work) break;
-        bind_element<R, BS>(M, c, &A, A.work[w]);
-        const double r1 = A.rates_in[2 * w], r2 = A.rates_in[2 * w + 1];
-        for (int i = threadIdx.x; i < M->N; i += BS) c->Z[i] = (int8_t)A.Zin[(size_t)w * M->N + i];
-        for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
-            const int s = c->nodes[q];
-            class_table<R>(M, c, 1, s, r1, 0);
-            class_table<R>(M, c, 2, s, r2, 0);
-        }
-        __syncthreads();
-        refresh_PTc<R, BS>(c, -1);
-        __syncthreads();
-        const double l = prune<R, BS>(M, c, PR_FULL, 0);
-        if (threadIdx.x == 0) A.ll_out[w] = l;
-        __syncthreads();
-    }
-}
-
 __global__ void reset_queue(int *q) { *q = 0; }
 
 template <typename R, int BS>
-static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, int kind, hipStream_t st)
+static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, hipStream_t st)
 {
     const size_t sm = smem_arrays<R, false>(M.N, M.S, BS, nullptr, nullptr);
-    hipError_t err;
-    if (kind == 0) {
-        err = hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL((chain_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
-    } else {
-        err = hipFuncSetAttribute((const void *)loglik_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL((loglik_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
-    }
+    const hipError_t err = hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL((chain_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
     return hipGetLastError();
 }
 
@@ -1111,16 +1051,16 @@ hipError_t launch_reset(int *q, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_kernel(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, int kind, hipStream_t st)
+hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st)
 {
     if (fp64) {
-        if (BS == 64) return launch_t<double, 64>(M, A, grid, kind, st);
-        if (BS == 128) return launch_t<double, 128>(M, A, grid, kind, st);
-        if (BS == 256) return launch_t<double, 256>(M, A, grid, kind, st);
+        if (BS == 64) return launch_t<double, 64>(M, A, grid, st);
+        if (BS == 128) return launch_t<double, 128>(M, A, grid, st);
+        if (BS == 256) return launch_t<double, 256>(M, A, grid, st);
     } else {
-        if (BS == 64) return launch_t<float, 64>(M, A, grid, kind, st);
-        if (BS == 128) return launch_t<float, 128>(M, A, grid, kind, st);
-        if (BS == 256) return launch_t<float, 256>(M, A, grid, kind, st);
+        if (BS == 64) return launch_t<float, 64>(M, A, grid, st);
+        if (BS == 128) return launch_t<float, 128>(M, A, grid, st);
+        if (BS == 256) return launch_t<float, 256>(M, A, grid, st);
     }
     return hipErrorInvalidValue;
 }

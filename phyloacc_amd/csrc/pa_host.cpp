@@ -911,13 +911,13 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
             B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * T : nullptr;
             B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * T * N : nullptr;
             HIPCHK(e, launch_reset(e->d_queue, e->stream));
-            HIPCHK(e, chain::launch_kernel(e->M, B, fp64, BS, 1, 0, e->stream));
+            HIPCHK(e, chain::launch_chain(e->M, B, fp64, BS, 1, e->stream));
         }
     } else {
         A.work = (const int32_t *)bufs[1].p;
         A.n_work = n;
         HIPCHK(e, launch_reset(e->d_queue, e->stream));
-        HIPCHK(e, chain::launch_kernel(e->M, A, fp64, BS, grid, 0, e->stream));
+        HIPCHK(e, chain::launch_chain(e->M, A, fp64, BS, grid, e->stream));
     }
     HIPCHK(e, hipEventRecord(t1, e->stream));
     HIPCHK(e, hipEventSynchronize(t1));
@@ -1601,7 +1601,7 @@ static pa_status serve_launch(pa_server *s)
         A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
         A.counters = s->d_cnt;
         HIPCHK(e, chain::launch_reset(e->d_queue, e->stream));
-        HIPCHK(e, chain::launch_kernel(e->M, A, s->fp64, s->BS, gc, 0, e->stream));
+        HIPCHK(e, chain::launch_chain(e->M, A, s->fp64, s->BS, gc, e->stream));
         HIPCHK(e, hipMemcpyAsync(s->h_res_d, s->d_res_d, sizeof(double) * 24 * nc, hipMemcpyDeviceToHost, e->stream));
         HIPCHK(e, hipMemcpyAsync(s->h_res_i, s->d_res_i, sizeof(int32_t) * 12 * nc, hipMemcpyDeviceToHost, e->stream));
     }
