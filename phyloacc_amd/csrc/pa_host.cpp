@@ -841,21 +841,20 @@ static int fit_per_cu(pa_engine *e, size_t smem, int BS)
 
 // Fixed-(Z, r) log-likelihood launch: one persistent launch over ids (in
 // schedule order, largest first), one scratch slot per workgroup.
-static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, ChainArgs &A, DevBuf *bufs)
+static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp64, LoglikArgs &A, DevBuf *bufs)
 {
     const int S = e->tm.S, N = e->tm.N;
     const int BS = pick_block(e, ids, n);
-    const size_t smem = chain_smem_bytes(N, S, BS, fp64);
+    const size_t smem = loglik_smem_bytes(N, fp64);
     if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
     const int grid = std::min(n, e->n_cu * fit_per_cu(e, smem, BS));
     int Pcap = 64;
     for (int i = 0; i < n; i++) Pcap = std::max(Pcap, e->elems[ids[i]].Ppad);
-    size_t off[5];
-    const size_t slot = slot_layout(N, S, Pcap, fp64, off);
+    const size_t slot = loglik_slot_bytes(N, S, Pcap, fp64);
     HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
     HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n));
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
-    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
     A.work = (const int32_t *)bufs[1].p;
     A.n_work = n;
     A.queue = e->d_queue;
@@ -985,7 +984,7 @@ extern "C" pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, cons
     HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
     HIPCHK(e, hipMemcpy(bufs[3].p, Zw.data(), Zw.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(bufs[4].p, rw.data(), rw.size() * 8, hipMemcpyHostToDevice));
-    ChainArgs A;
+    LoglikArgs A;
     memset(&A, 0, sizeof A);
     A.Zin = (const int32_t *)bufs[3].p;
     A.rates_in = (const double *)bufs[4].p;
@@ -1013,21 +1012,14 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     if ((st = upload(e)) != PA_OK) return st;
     const int N = e->tm.N;
     std::vector<int32_t> order = schedule(e, ids, n, PA_RUN_BATCHED);
-    std::vector<int32_t> Zw((size_t)n * N, 0);
-    std::vector<double> lw((size_t)n * N), rw((size_t)n * 2, 1.0);
+    std::vector<double> lw((size_t)n * N);
     for (int i = 0; i < n; i++) memcpy(&lw[(size_t)i * N], lengths + (size_t)order[i] * N, sizeof(double) * N);
     DevBuf bufs[7];
-    HIPCHK(e, hipMalloc(&bufs[3].p, Zw.size() * 4));
-    HIPCHK(e, hipMalloc(&bufs[4].p, rw.size() * 8));
     HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
     HIPCHK(e, hipMalloc(&bufs[6].p, lw.size() * 8));
-    HIPCHK(e, hipMemcpy(bufs[3].p, Zw.data(), Zw.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(e, hipMemcpy(bufs[4].p, rw.data(), rw.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(bufs[6].p, lw.data(), lw.size() * 8, hipMemcpyHostToDevice));
-    ChainArgs A;
+    LoglikArgs A;
     memset(&A, 0, sizeof A);
-    A.Zin = (const int32_t *)bufs[3].p;
-    A.rates_in = (const double *)bufs[4].p;
     A.ll_out = (double *)bufs[5].p;
     A.len_in = (const double *)bufs[6].p;
     std::vector<int32_t> sorted(n);
@@ -1391,8 +1383,8 @@ struct pa_server {
     int32_t *d_res_i = nullptr, *h_res_i = nullptr; // [max][3][4]
     int8_t *d_maxZ = nullptr;                       // [max][3][N]
     unsigned long long *d_cnt = nullptr;
-    int32_t *d_lids = nullptr, *h_lids = nullptr, *d_lZ = nullptr;
-    double *d_lrates = nullptr, *d_llen = nullptr, *h_llen = nullptr, *d_lout = nullptr, *h_lout = nullptr;
+    int32_t *d_lids = nullptr, *h_lids = nullptr;
+    double *d_llen = nullptr, *h_llen = nullptr, *d_lout = nullptr, *h_lout = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool in_flight = false;
     std::deque<Request> waiting;
@@ -1412,8 +1404,8 @@ extern "C" double pa_serve_clock(void)
 static void serve_free(pa_server *s)
 {
     (void)hipFree(s->d_pool); (void)hipFree(s->d_ids); (void)hipFree(s->d_res_d); (void)hipFree(s->d_res_i);
-    (void)hipFree(s->d_maxZ); (void)hipFree(s->d_cnt); (void)hipFree(s->d_lids); (void)hipFree(s->d_lZ);
-    (void)hipFree(s->d_lrates); (void)hipFree(s->d_llen); (void)hipFree(s->d_lout);
+    (void)hipFree(s->d_maxZ); (void)hipFree(s->d_cnt); (void)hipFree(s->d_lids);
+    (void)hipFree(s->d_llen); (void)hipFree(s->d_lout);
     (void)hipHostFree(s->h_ids); (void)hipHostFree(s->h_res_d); (void)hipHostFree(s->h_res_i);
     (void)hipHostFree(s->h_lids); (void)hipHostFree(s->h_llen); (void)hipHostFree(s->h_lout);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -1436,7 +1428,7 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     s->BS = cfg->block_threads ? cfg->block_threads : 128;
     if (s->BS != 64 && s->BS != 128 && s->BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
     const int N = e->tm.N, S = e->tm.S;
-    if (chain::chain_smem_bytes(N, S, s->BS, s->fp64) > e->lds_per_cu || chain_smem_bytes(N, S, s->BS, s->fp64) > e->lds_per_cu)
+    if (chain::chain_smem_bytes(N, S, s->BS, s->fp64) > e->lds_per_cu || loglik_smem_bytes(N, s->fp64) > e->lds_per_cu)
         return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
     s->max_chains = cfg->max_chains > 0 ? cfg->max_chains : 4096;
     s->pool_bytes = cfg->hbm_budget_bytes > 0 ? (size_t)cfg->hbm_budget_bytes : ((size_t)4 << 30);
@@ -1452,19 +1444,11 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     HIPCHK(e, hipHostMalloc(&s->h_res_d, sizeof(double) * 24 * M));
     HIPCHK(e, hipHostMalloc(&s->h_res_i, sizeof(int32_t) * 12 * M));
     HIPCHK(e, hipMalloc(&s->d_lids, sizeof(int32_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_lZ, sizeof(int32_t) * M * N));
-    HIPCHK(e, hipMalloc(&s->d_lrates, sizeof(double) * 2 * M));
     HIPCHK(e, hipMalloc(&s->d_llen, sizeof(double) * M * N));
     HIPCHK(e, hipMalloc(&s->d_lout, sizeof(double) * M));
     HIPCHK(e, hipHostMalloc(&s->h_lids, sizeof(int32_t) * M));
     HIPCHK(e, hipHostMalloc(&s->h_llen, sizeof(double) * M * N));
     HIPCHK(e, hipHostMalloc(&s->h_lout, sizeof(double) * M));
-    {
-        // own-tree log-likelihoods: Z = 0 and rates 1, every branch at its given length
-        std::vector<double> ones(2 * M, 1.0);
-        HIPCHK(e, hipMemset(s->d_lZ, 0, sizeof(int32_t) * M * N));
-        HIPCHK(e, hipMemcpy(s->d_lrates, ones.data(), ones.size() * 8, hipMemcpyHostToDevice));
-    }
     HIPCHK(e, hipEventCreate(&s->ev0));
     HIPCHK(e, hipEventCreate(&s->ev1));
     guard.s = nullptr;
@@ -1515,8 +1499,7 @@ static size_t chain_launch_scratch(pa_server *s, int Pcap, int k)
 }
 static size_t loglik_launch_scratch(pa_server *s, int Pcap, int k)
 {
-    size_t off[5];
-    return slot_layout(s->e->tm.N, s->e->tm.S, Pcap, s->fp64, off) * (size_t)k;
+    return loglik_slot_bytes(s->e->tm.N, s->e->tm.S, Pcap, s->fp64) * (size_t)k;
 }
 
 // Admits waiting requests (arrival order) and launches them; no-op when
@@ -1530,7 +1513,7 @@ static pa_status serve_launch(pa_server *s)
     if (st != PA_OK) return st;
     const int N = e->tm.N, S = e->tm.S;
     const int gmax_c = e->n_cu * fit_per_cu(e, chain::chain_smem_bytes(N, S, s->BS, s->fp64), s->BS);
-    const int gmax_l = e->n_cu * fit_per_cu(e, chain_smem_bytes(N, S, s->BS, s->fp64), s->BS);
+    const int gmax_l = e->n_cu * fit_per_cu(e, loglik_smem_bytes(N, s->fp64), s->BS);
     const double now = pa_serve_clock();
     // chain requests
     int PcapC = 64;
@@ -1612,13 +1595,12 @@ static pa_status serve_launch(pa_server *s)
         }
         HIPCHK(e, hipMemcpyAsync(s->d_lids, s->h_lids, sizeof(int32_t) * nl, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(s->d_llen, s->h_llen, sizeof(double) * (size_t)nl * N, hipMemcpyHostToDevice, e->stream));
-        ChainArgs A;
+        LoglikArgs A;
         memset(&A, 0, sizeof A);
-        size_t off[5];
-        A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+        A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
         A.work = s->d_lids; A.n_work = nl; A.queue = e->d_queue;
-        A.scratch = s->d_pool + scr_c; A.slot_bytes = (int64_t)slot_layout(N, S, PcapL, s->fp64, off);
-        A.Zin = s->d_lZ; A.rates_in = s->d_lrates; A.ll_out = s->d_lout; A.len_in = s->d_llen;
+        A.scratch = s->d_pool + scr_c; A.slot_bytes = (int64_t)loglik_slot_bytes(N, S, PcapL, s->fp64);
+        A.ll_out = s->d_lout; A.len_in = s->d_llen;
         HIPCHK(e, launch_reset(e->d_queue, e->stream));
         HIPCHK(e, launch_loglik(e->M, A, s->fp64, s->BS, gl, e->stream));
         HIPCHK(e, hipMemcpyAsync(s->h_lout, s->d_lout, sizeof(double) * nl, hipMemcpyDeviceToHost, e->stream));

@@ -12,66 +12,32 @@
 
 namespace pa {
 
-// Chain state between launches of a resumable run.  The partials, exponents
-// and class tables stay in the record's scratch slot; everything else that a
-// sweep carries to the next one is here.
-struct ChainState {
-    int32_t ms;               // model slot being run (0..2), 3 when all three are done
-    int32_t m;                // next iteration of that model; -1: not initialised yet
-    int32_t accept_n, accept_c, max_m, status, pad0, pad1;
-    double loglik, nrate, crate, g, l, l2, prop_n, prop_c, max_loglik;
-    unsigned long long bytes;
-    int8_t Z[MAXN], maxZ[MAXN];
-    uint8_t cur[MAXN], fixZ[MAXN];
-};
-
-struct ChainArgs {
+// Arguments of the log-likelihood launch.  Work position w evaluates element
+// work[w] either at ancestral states Zin[w] and rates rates_in[w] over the
+// element's stored branch lengths, or, when len_in is set, at the per-call
+// lengths len_in[w] with every branch at rate 1 (Zin, rates_in unused).
+struct LoglikArgs {
     const ElemDev *elems;
     const int32_t *work;      // element ids in schedule order
     int n_work;
     int *queue;               // work counter, reset before each launch
     const uint8_t *codes;     // arenas
     const int32_t *wts;
-    const int32_t *site;
     const int32_t *tab;
-    char *scratch;            // scratch slots
-    int64_t slot_bytes;       // one slot per workgroup (when slot_off is null)
-    const int64_t *slot_off;  // [records] byte offset of each record's own slot, or null
-    const int32_t *rec;       // [n_work] record of each work position, or null (record = position)
-    ChainState *state;        // [records] resumable chains, or null (run to the end)
-    int32_t *done;            // [records] set to 1 when all three models finished (resumable runs)
-    int sweep_budget;         // sweeps per work item in this launch (resumable runs)
-    const int *n_work_dev;    // when set, the number of work items is read here at launch
-    // outputs, indexed by record
-    double *res_d;            // [n_work][3][8]
-    int32_t *res_i;           // [n_work][3][4]
-    int8_t *maxZ;             // [n_work][3][N]
-    double *tr_d;             // [n_work][3][7][T] or null
-    int8_t *tr_Z;             // [n_work][3][T][N] or null
-    // fixed-(Z, r) log-likelihood
-    const int32_t *Zin;       // [n_work][N]
-    const double *rates_in;   // [n_work][2] (r1, r2)
+    char *scratch;            // one slot per workgroup
+    int64_t slot_bytes;
+    const int32_t *Zin;       // [n_work][N] (when len_in is null)
+    const double *rates_in;   // [n_work][2] (r1, r2) (when len_in is null)
+    const double *len_in;     // [n_work][N] per-call branch lengths, or null
     double *ll_out;           // [n_work]
-    const double *len_in;     // [n_work][N] per-call branch lengths (own-tree elements) or null
-    unsigned long long *counters;   // [0]: algorithmic HBM bytes of the launch (or null)
 };
 
-size_t chain_smem_bytes(int N, int S, int BS, int fp64);
-// Byte layout of one scratch slot for an element with Ppad patterns:
-// off = {parts, exps, plin, logt, amb}; returns the slot size.
-__host__ __device__ inline size_t slot_layout(int N, int S, int Ppad, int fp64, size_t *off)
-{
-    const int I = N - S;
-    const size_t rs = fp64 ? 8 : 4;
-    size_t o = 0;
-    off[0] = o; o = (o + (size_t)2 * I * Ppad * 4 * rs + 255) & ~(size_t)255;   // parts
-    off[1] = o; o = (o + (size_t)2 * I * Ppad * 4 + 255) & ~(size_t)255;        // exps
-    off[2] = o; o = (o + (size_t)3 * N * 16 * 8 + 255) & ~(size_t)255;          // plin
-    off[3] = o; o = (o + (size_t)3 * N * 16 * 8 + 255) & ~(size_t)255;          // logt
-    off[4] = o; o = (o + (size_t)S * 24 * 4 + 255) & ~(size_t)255;              // amb
-    return o;
-}
-hipError_t launch_loglik(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st);
+// LDS bytes of one workgroup (dynamic arrays + context + argument blocks).
+size_t loglik_smem_bytes(int N, int fp64);
+// Bytes of one workgroup's HBM slot for elements of up to Pcap (padded)
+// patterns: the partials [I][Pcap] (4 values each) and their exponents.
+size_t loglik_slot_bytes(int N, int S, int Pcap, int fp64);
+hipError_t launch_loglik(const ModelDev &M, const LoglikArgs &A, int fp64, int BS, int grid, hipStream_t st);
 hipError_t launch_reset(int *q, hipStream_t st);
 
 }  // namespace pa
