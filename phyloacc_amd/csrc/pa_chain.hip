@@ -15,6 +15,11 @@
 //   * partials are linear-space with a power-of-two rescale per node; the
 //     cumulative exponent rides along as an int; GAPT marks all-gap subtrees
 //     (the reference's Tg >= num_base, src/PhyloAcc-ST/bpp_c.hpp:417-428);
+//   * a cherry (a non-root internal node whose children are both leaves) is
+//     never stored: its parent and the ancestral sampler recompute its
+//     scaled partial from the two leaf codes with the same operations, so
+//     results are unchanged and a pass saves the cherry's store, its reload
+//     and one dependent memory round trip;
 //   * per-branch 4x4 transition matrices of the current rate class live in
 //     LDS (read as broadcasts); FP64 is used for P(t) builds, log tables,
 //     emissions, the Z forward-backward, likelihood sums and MH ratios.
@@ -247,7 +252,29 @@ __device__ __noinline__ void refresh_PTc(CP<R> c, int only_cls)
 
 // ----------------------------------------------------------------- pruning
 
+// Values every lane of the workgroup holds alike (node ids, flags, offsets
+// read from LDS) are moved to scalar registers, which frees vector registers
+// and makes the address arithmetic scalar.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+// Global pointers are 64-bit, LDS pointers 32-bit.
+template <typename T>
+__device__ __forceinline__ T *uni_ptr(T *p)
+{
+    const uintptr_t v = (uintptr_t)p;
+    if constexpr (sizeof(T *) == 4) {
+        return (T *)(uintptr_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    } else {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+        return (T *)(((uint64_t)hi << 32) | lo);
+    }
+}
+
 enum { PR_FULL = 0, PR_DIRTY = 1, PR_PROP = 2 };
+
+// Node flag set when an element is bound (above the element table's flags):
+// the node is a cherry whose partial is recomputed where it is read.
+constexpr uint8_t NF_CHERRY = 0x80;
 
 template <typename R>
 __device__ __forceinline__ void leaf_mul(const LDS R *PT, int code, R &a0, R &a1, R &a2, R &a3)
@@ -291,6 +318,35 @@ __device__ __forceinline__ ChildIn<R> child_load(bool leaf, bool miss, const GBL
     return in;
 }
 
+// The scaled partial and exponent of cherry x at one pattern, from its leaf
+// codes cA, cB and their branch matrices: the operations the pass below
+// would apply to node x, in the same order, so the values are those it
+// would have stored.
+template <typename R>
+__device__ __forceinline__ ChildIn<R> cherry_in(const LDS R *srcA, const LDS R *srcB, bool missA, bool missB,
+                                                bool miss_x, int cA, int cB)
+{
+    ChildIn<R> in;
+    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
+    R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
+    const bool gA = (cA == PA_GAP_CODE || cA == PA_OTHER_CODE);
+    const bool gB = (cB == PA_GAP_CODE || cB == PA_OTHER_CODE);
+    if (!missA && !gA) leaf_mul<R>(srcA, cA, a0, a1, a2, a3);
+    if (!missB && !gB) leaf_mul<R>(srcB, cB, a0, a1, a2, a3);
+    if (gA && gB) {
+        in.x = GAPT;
+    } else if (miss_x) {
+        in.x = 0;
+    } else {
+        const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
+        int kx = 0;
+        frexp(mx, &kx);
+        in.v.x = ldexp(a0, -kx); in.v.y = ldexp(a1, -kx); in.v.z = ldexp(a2, -kx); in.v.w = ldexp(a3, -kx);
+        in.x = kx;
+    }
+    return in;
+}
+
 // Multiplies one child's message P * L_child into (a0..a3).  A leaf reads the
 // row of its base from the LDS matrix; an internal child reads the whole
 // matrix as four 4-vectors.  gap reports the child's GAPT state (Tg >= num_base).
@@ -317,65 +373,198 @@ __device__ __forceinline__ void child_apply(bool leaf, bool miss, const LDS R *s
     }
 }
 
-// One post-order pass over internal nodes (BPP_C::Update_Tg(tosample=false),
-// src/PhyloAcc-ST/bpp_c.cpp:644-671, and log_f_Xz, bpp_c.cpp:860-928).
-// FULL: every internal node of the tree (missing nodes only record their GAPT
-// flag) into the current buffer.  DIRTY: nodes with dirty[] set, in place.
-// PROP: dirty nodes into the other buffer, with PTp for branches whose Z is
-// cls.  Returns the element's log-likelihood (FP64), identical on every thread.
+// Where one child of a node comes from in a pass: a leaf's code row, an
+// internal child's partial / exponent rows (offsets into the slot arrays), or
+// a cherry's two leaf code rows; and the branch matrices that apply.  Every
+// field is the same on all lanes (scalar registers).
+enum { KD_LEAF = 0, KD_STORED = 1, KD_CHERRY = 2, KD_KIND = 3, KD_MISS = 4, KD_MISS_A = 8, KD_MISS_B = 16 };
+
+template <typename R>
+struct KidDesc {
+    int bits;             // kind (KD_KIND), the child missing, a cherry's leaves missing
+    int off, offB;        // code row / partial offset; cherry: code rows of its two leaves
+    const LDS R *src;     // the child's branch matrix
+    const LDS R *sA, *sB; // cherry: its leaves' branch matrices
+};
+
+template <typename R>
+struct NodeDesc {
+    int dst;              // 2 * (offset of the node's partial / exponent rows in the written buffer) + missing
+    KidDesc<R> k[2];
+};
+
+template <typename R>
+__device__ __forceinline__ const LDS R *pass_matrix(CP<R> c, const LDS R *PTc, const LDS R *PTp, int mode, int cls, int y)
+{
+    return uni(mode == PR_PROP && c->Z[y] == cls) ? PTp + 16 * y : PTc + 16 * y;
+}
+
+template <typename R>
+__device__ __forceinline__ KidDesc<R> kid_desc(CP<R> c, const LDS R *PTc, const LDS R *PTp, int mode, int cls,
+                                               int x, int S, int Ppad, int bufI)
+{
+    KidDesc<R> d;
+    const int f = uni(c->flags[x]);
+    d.src = pass_matrix(c, PTc, PTp, mode, cls, x);
+    d.offB = 0;
+    d.sA = d.sB = PTc;
+    const int miss = (f & NF_MISSING) ? KD_MISS : 0;
+    if (x < S) {
+        d.bits = KD_LEAF | miss;
+        d.off = x * Ppad;
+    } else if (f & NF_CHERRY) {
+        const int yA = uni(c->ch0[x]), yB = uni(c->ch1[x]);
+        d.bits = KD_CHERRY | miss | ((uni(c->flags[yA]) & NF_MISSING) ? KD_MISS_A : 0) |
+                 ((uni(c->flags[yB]) & NF_MISSING) ? KD_MISS_B : 0);
+        d.off = yA * Ppad;
+        d.offB = yB * Ppad;
+        d.sA = pass_matrix(c, PTc, PTp, mode, cls, yA);
+        d.sB = pass_matrix(c, PTc, PTp, mode, cls, yB);
+    } else {
+        const int rb = uni((mode == PR_PROP && c->dirty[x]) ? 1 - c->cur[x] : c->cur[x]);
+        d.bits = KD_STORED | miss;
+        d.off = rb * bufI + (x - S) * Ppad;
+    }
+    return d;
+}
+
+template <typename R>
+__device__ __forceinline__ ChildIn<R> kid_load(const KidDesc<R> &d, const GBL uint8_t *codes, const GBL V4<R> *parts,
+                                               const GBL int32_t *exps, int p)
+{
+    // a missing child contributes nothing, but its gap state still decides
+    // whether the parent is all-gap at this pattern, so its code / exponent
+    // is read either way; a cherry's two codes are packed into x
+    ChildIn<R> in;
+    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
+    const int kind = d.bits & KD_KIND;
+    if (kind == KD_LEAF) {
+        in.x = codes[d.off + p];
+    } else if (kind == KD_CHERRY) {
+        in.x = codes[d.off + p] | (codes[d.offB + p] << 8);
+    } else {
+        in.x = exps[d.off + p];
+        if (!(d.bits & KD_MISS)) in.v = ld4<R>(parts + d.off + p);
+    }
+    return in;
+}
+
+// Nodes pruned together per round: all their loads are issued before the
+// first is used.  Two in FP32 and one in FP64 fit the 128-register budget of
+// the chain kernel without spilling in the pattern loop (measured on config
+// 3: groups of 3 spill and ran 3 % slower than groups of 2).
+template <typename R>
+struct Group { static constexpr int U = sizeof(R) == 4 ? 2 : 1; };
+
+// One pruning pass (BPP_C::Update_Tg(tosample=false), src/PhyloAcc-ST/bpp_c.cpp:644-671,
+// and log_f_Xz, bpp_c.cpp:860-928).  FULL: every internal node of the tree
+// (missing nodes only record their GAPT flag) into the current buffer.
+// DIRTY: nodes with dirty[] set, in place.  PROP: dirty nodes into the other
+// buffer, with PTp for branches whose Z is cls.  Cherries are skipped: their
+// parent recomputes them (cherry_in) with the matrices the pass would have
+// used.  Returns the element's log-likelihood (FP64), identical on every
+// thread.
+//
+// Nodes are taken by tree level (the element's height-ordered list), so the
+// nodes of one round never depend on each other: a round takes up to U
+// visited nodes of one level and, for each pattern block, issues the loads
+// of all their children before computing any of them.  A pass then costs
+// about one memory round trip per round instead of one per node.  A thread
+// owns the patterns p = tid (mod BS) of every node, so it only reads partials
+// it wrote itself and the pass needs no barrier.
 template <typename R, int BS>
 __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
 {
     const int tid = threadIdx.x;
-    const int P = c->P, Ppad = c->Ppad, S = c->S, n_int = c->n_int;
-    const size_t bufI = (size_t)c->I * Ppad;
-    GBL V4<R> *parts = c->parts;
-    GBL int32_t *exps = c->exps;
-    const GBL uint8_t *codes = c->codes;
-    for (int k = 0; k < n_int; k++) {
-        const int s = c->inodes[k];
-        const bool miss_s = c->flags[s] & NF_MISSING;
-        if (mode != PR_FULL && (!c->dirty[s] || miss_s)) continue;
-        const int wb = (mode == PR_PROP) ? 1 - c->cur[s] : c->cur[s];
-        const int x0 = c->ch0[s], x1 = c->ch1[s];
-        const bool leaf0 = x0 < S, leaf1 = x1 < S;
-        const bool miss0 = c->flags[x0] & NF_MISSING, miss1 = c->flags[x1] & NF_MISSING;
-        const LDS R *src0 = (mode == PR_PROP && c->Z[x0] == cls) ? c->PTp + 16 * x0 : c->PTc + 16 * x0;
-        const LDS R *src1 = (mode == PR_PROP && c->Z[x1] == cls) ? c->PTp + 16 * x1 : c->PTc + 16 * x1;
-        const int rb0 = leaf0 ? 0 : ((mode == PR_PROP && c->dirty[x0]) ? 1 - c->cur[x0] : c->cur[x0]);
-        const int rb1 = leaf1 ? 0 : ((mode == PR_PROP && c->dirty[x1]) ? 1 - c->cur[x1] : c->cur[x1]);
-        const GBL uint8_t *cr0 = codes + (size_t)x0 * Ppad, *cr1 = codes + (size_t)x1 * Ppad;
-        const size_t o0 = leaf0 ? 0 : rb0 * bufI + (size_t)(x0 - S) * Ppad;
-        const size_t o1 = leaf1 ? 0 : rb1 * bufI + (size_t)(x1 - S) * Ppad;
-        const GBL V4<R> *pr0 = parts + o0, *pr1 = parts + o1;
-        const GBL int32_t *er0 = exps + o0, *er1 = exps + o1;
-        GBL V4<R> *dstp = parts + wb * bufI + (size_t)(s - S) * Ppad;
-        GBL int32_t *dste = exps + wb * bufI + (size_t)(s - S) * Ppad;
-        if (tid == 0)   // write partial + exponent; read each child's partial + exponent, or its leaf code
-            c->bytes += (unsigned long long)P * ((4 * sizeof(R) + 4) + (leaf0 ? 1 : 4 * sizeof(R) + 4) + (leaf1 ? 1 : 4 * sizeof(R) + 4));
-        for (int p = tid; p < P; p += BS) {
-            // both children's loads are issued before either is used
-            const ChildIn<R> in0 = child_load<R>(leaf0, miss0, cr0, pr0, er0, p);
-            const ChildIn<R> in1 = child_load<R>(leaf1, miss1, cr1, pr1, er1, p);
-            R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
-            int e = 0;
-            bool g0, g1;
-            child_apply<R>(leaf0, miss0, src0, in0, a0, a1, a2, a3, e, g0);
-            child_apply<R>(leaf1, miss1, src1, in1, a0, a1, a2, a3, e, g1);
-            if (g0 && g1) {
-                dste[p] = GAPT;
-            } else if (miss_s) {
-                dste[p] = 0;
-            } else {
-                const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
-                int kx = 0;
-                frexp(mx, &kx);
-                a0 = ldexp(a0, -kx); a1 = ldexp(a1, -kx); a2 = ldexp(a2, -kx); a3 = ldexp(a3, -kx);
-                st4<R>(dstp + p, a0, a1, a2, a3);
-                dste[p] = e + kx;
+    const int P = uni(c->P), Ppad = uni(c->Ppad), S = uni(c->S), n_up = uni(c->n_up);
+    const int bufI = uni(c->I) * Ppad;
+    GBL V4<R> *parts = uni_ptr(c->parts);
+    GBL int32_t *exps = uni_ptr(c->exps);
+    const GBL uint8_t *codes = uni_ptr(c->codes);
+    const LDS R *PTc = uni_ptr(c->PTc), *PTp = uni_ptr(c->PTp);
+    constexpr int U = Group<R>::U;
+    unsigned long long bytes = 0;
+    for (int h = 0; h < n_up; h++) {
+        int q = uni(h == 0 ? 0 : c->upoff[h - 1]);
+        const int qe = uni(c->upoff[h]);
+        while (q < qe) {
+            // the next (up to) U nodes of this level that the pass visits
+            int sel[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                sel[u] = -1;
+                while (q < qe) {
+                    const int s = uni(c->up[q]);
+                    q++;
+                    const int fs = uni(c->flags[s]);
+                    if (fs & NF_CHERRY) continue;
+                    if (mode != PR_FULL && (!uni(c->dirty[s]) || (fs & NF_MISSING))) continue;
+                    sel[u] = s;
+                    break;
+                }
+            }
+            if (sel[0] < 0) break;
+            NodeDesc<R> nd[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (sel[u] < 0) continue;
+                const int s = sel[u];
+                const int wb = uni((mode == PR_PROP) ? 1 - c->cur[s] : c->cur[s]);
+                nd[u].dst = 2 * (wb * bufI + (s - S) * Ppad) + ((uni(c->flags[s]) & NF_MISSING) ? 1 : 0);
+                nd[u].k[0] = kid_desc(c, PTc, PTp, mode, cls, uni(c->ch0[s]), S, Ppad, bufI);
+                nd[u].k[1] = kid_desc(c, PTc, PTp, mode, cls, uni(c->ch1[s]), S, Ppad, bufI);
+                // write partial + exponent; read each child's partial + exponent, its leaf code, or a cherry's two codes
+                bytes += (unsigned long long)P * (4 * sizeof(R) + 4);
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    bytes += (unsigned long long)P * ((nd[u].k[j].bits & KD_KIND) == KD_LEAF ? 1 : (nd[u].k[j].bits & KD_KIND) == KD_CHERRY ? 2 : 4 * sizeof(R) + 4);
+            }
+            for (int p = tid; p < P; p += BS) {
+                ChildIn<R> in[U][2];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (sel[u] < 0) continue;
+                    in[u][0] = kid_load<R>(nd[u].k[0], codes, parts, exps, p);
+                    in[u][1] = kid_load<R>(nd[u].k[1], codes, parts, exps, p);
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    // one node at a time: keeps the scheduler from hoisting
+                    // every node's matrices into registers at once
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (sel[u] < 0) continue;
+                    R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
+                    int e = 0;
+                    bool g[2];
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        const KidDesc<R> &d = nd[u].k[j];
+                        const int kind = d.bits & KD_KIND;
+                        const bool miss = d.bits & KD_MISS;
+                        ChildIn<R> cin = in[u][j];
+                        if (kind == KD_CHERRY)
+                            cin = cherry_in<R>(d.sA, d.sB, d.bits & KD_MISS_A, d.bits & KD_MISS_B, miss, cin.x & 255, cin.x >> 8);
+                        child_apply<R>(kind == KD_LEAF, miss, d.src, cin, a0, a1, a2, a3, e, g[j]);
+                    }
+                    const int dst = nd[u].dst >> 1;
+                    GBL int32_t *dste = exps + dst;
+                    if (g[0] && g[1]) {
+                        dste[p] = GAPT;
+                    } else if (nd[u].dst & 1) {
+                        dste[p] = 0;
+                    } else {
+                        const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
+                        int kx = 0;
+                        frexp(mx, &kx);
+                        a0 = ldexp(a0, -kx); a1 = ldexp(a1, -kx); a2 = ldexp(a2, -kx); a3 = ldexp(a3, -kx);
+                        st4<R>(parts + dst + p, a0, a1, a2, a3);
+                        dste[p] = e + kx;
+                    }
+                }
             }
         }
     }
+    if (tid == 0) c->bytes += bytes;
     // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
     const int r = c->root;
     double acc = 0;
@@ -417,11 +606,16 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
     for (int i = tid; i < 16 * c->N; i += BS) c->hist[i] = 0;   // shares storage with the proposal matrices
     __syncthreads();
     if (tid == 0) {
-        // per site: pattern index, root partial, each non-missing internal node's partial, each leaf code
-        int nint = 0, nleaf = 0;
-        for (int k = 0; k < n_int; k++) nint += !(c->flags[c->inodes[k]] & NF_MISSING);
+        // per site: pattern index, root partial, each non-missing internal node's partial
+        // (a cherry's two leaf codes), each leaf code
+        int nint = 0, nch = 0, nleaf = 0;
+        for (int k = 0; k < n_int; k++) {
+            const uint8_t f = c->flags[c->inodes[k]];
+            if (f & NF_MISSING) continue;
+            if (f & NF_CHERRY) nch++; else nint++;
+        }
         for (int q = 0; q < S; q++) nleaf += (c->flags[q] & NF_IN_TREE) && !(c->flags[q] & NF_MISSING);
-        c->bytes += (unsigned long long)GG * (4 + (4 * sizeof(R) + 4) * nint + nleaf);
+        c->bytes += (unsigned long long)GG * (4 + (4 * sizeof(R) + 4) * nint + 2 * nch + nleaf);
     }
     for (int g = tid; g < GG; g += BS) {
         const int p = c->site[g];
@@ -438,7 +632,9 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
             tgs[(r - S) * BS + tid] = (uint8_t)categorical<4>(pr, u01(blk.x));
         }
         // internal nodes in reverse post-order, K at a time: a chunk's partial
-        // loads do not depend on its draws, so they are all issued first
+        // loads do not depend on its draws, so they are all issued first.  A
+        // cherry's slot holds its two leaf codes (low and second byte) until
+        // its partial is recomputed.
         constexpr int K = sizeof(R) == 4 ? 4 : 2;
         int t = 1;
         for (int kb = n_int - 2; kb >= 0; kb -= K) {
@@ -452,8 +648,12 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                 av[j].x = av[j].y = av[j].z = av[j].w = (R)0;
                 if (k >= 0) {
                     const int s = c->inodes[k];
+                    const uint8_t f = c->flags[s];
                     sn[j] = s;
-                    if (!(c->flags[s] & NF_MISSING)) {
+                    if (f & NF_CHERRY) {
+                        if (!(f & NF_MISSING))
+                            ex[j] = codes[(size_t)c->ch0[s] * Ppad + p] | (codes[(size_t)c->ch1[s] * Ppad + p] << 8);
+                    } else if (!(f & NF_MISSING)) {
                         const size_t off = c->cur[s] * bufI + (size_t)(s - S) * Ppad + p;
                         ex[j] = exps[off];
                         av[j] = ld4<R>(parts + off);
@@ -464,8 +664,15 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
             for (int j = 0; j < K; j++, t++) {
                 if (kb - j < 0) break;
                 if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
-                if (ex[j] == GAPT) continue;     // missing node, or an all-gap subtree at this site
                 const int s = sn[j];
+                if ((c->flags[s] & NF_CHERRY) && ex[j] != GAPT) {
+                    const int yA = c->ch0[s], yB = c->ch1[s];
+                    const ChildIn<R> in = cherry_in<R>(c->PTc + 16 * yA, c->PTc + 16 * yB, c->flags[yA] & NF_MISSING,
+                                                       c->flags[yB] & NF_MISSING, false, ex[j] & 255, ex[j] >> 8);
+                    ex[j] = in.x;
+                    av[j] = in.v;
+                }
+                if (ex[j] == GAPT) continue;     // missing node, or an all-gap subtree at this site
                 const int tp = tgs[(c->par[s] - S) * BS + tid];
                 const V4<R> a = av[j];
                 const LDS R *PT = c->PTc + 16 * s;
@@ -879,6 +1086,11 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
     }
     __syncthreads();
     for (int i = tid; i < M->S * 24; i += BS) c->amb[i] = 0;
+    // cherries (both children leaves; never the root) are recomputed where read
+    for (int q = tid; q < c->n_int; q += BS) {
+        const int s = c->inodes[q];
+        if (s != c->root && c->ch0[s] < c->S && c->ch1[s] < c->S) c->flags[s] |= NF_CHERRY;
+    }
     // null-rate tables (rate 1), fixed for the element
     for (int q = tid; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, 0, c->nodes[q], 1.0, 0);
     __syncthreads();
