@@ -1008,19 +1008,32 @@ __device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int re
     __syncthreads();
 }
 
-// Beta draws of BPP_C::sample_transition (src/PhyloAcc-ST/bpp_c.cpp:522-613)
+// The three Beta draws of BPP_C::sample_transition (src/PhyloAcc-ST/bpp_c.cpp:522-613),
+// Beta(a, b) = X / (X + Y) with X ~ Gamma(a) on sub-stream 2k and Y ~ Gamma(b) on
+// sub-stream 2k + 1 (beta_draw).  The six gamma variates are independent, so lane j < 6
+// draws the one of sub-stream j into red[j]; the l2 pair is not drawn when its prior is off.
 template <typename R>
-__device__ __noinline__ void transition_draw(MP M, CP<R> c, int ms, int m)
+__device__ __noinline__ void transition_gamma(MP M, CP<R> c, int ms, int m, int j)
 {
     const Key key{M->key0, M->key1};
     const uint32_t c3 = tagword(ms, TAG_TRANS), c2 = (uint32_t)m;
     double n[9], nf[9];
     for (int i = 0; i < 9; i++) { n[i] = c->cnt9[i]; nf[i] = n[i] - c->cnt9[9 + i]; }
-    const double g = beta_draw(key, 0, 1, c2, c3, M->ha_g + n[1 * 3 + 0] + n[2 * 3 + 0], M->hb_g + n[0]);
-    const double l = beta_draw(key, 2, 3, c2, c3, M->ha_l + nf[2 * 3 + 1], M->hb_l + nf[1 * 3 + 1]);
-    const double l2 = M->ha_l2 == 0 ? 0.0 : beta_draw(key, 4, 5, c2, c3, M->ha_l2 + nf[1 * 3 + 2], M->hb_l2 + nf[2 * 3 + 2]);
-    c->g = g; c->l = l; c->l2 = l2;
-    set_TM(c->TMf, c->TMn, g, l, l2);
+    double shape;
+    switch (j) {
+    case 0: shape = M->ha_g + n[1 * 3 + 0] + n[2 * 3 + 0]; break;
+    case 1: shape = M->hb_g + n[0]; break;
+    case 2: shape = M->ha_l + nf[2 * 3 + 1]; break;
+    case 3: shape = M->hb_l + nf[1 * 3 + 1]; break;
+    case 4: shape = M->ha_l2 + nf[1 * 3 + 2]; break;
+    default: shape = M->hb_l2 + nf[2 * 3 + 2]; break;
+    }
+    double x = 0.0;
+    if (j < 4 || M->ha_l2 != 0) {
+        Stream st{key, (uint32_t)j, c2, c3, 0};
+        x = gamma_draw(st, shape, 1.0);
+    }
+    c->red[j] = x;
 }
 
 template <typename R, int BS>
@@ -1036,7 +1049,15 @@ __device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
         if (c->fixZ[s]) lds_add(&c->cnt9[9 + zi * 3 + zp], 1u);
     }
     __syncthreads();
-    if (tid == 0) transition_draw<R>(M, c, ms, m);
+    if (tid < 6) transition_gamma<R>(M, c, ms, m, tid);
+    __syncthreads();
+    if (tid == 0) {
+        const double g = c->red[0] / (c->red[0] + c->red[1]);
+        const double l = c->red[2] / (c->red[2] + c->red[3]);
+        const double l2 = M->ha_l2 == 0 ? 0.0 : c->red[4] / (c->red[4] + c->red[5]);
+        c->g = g; c->l = l; c->l2 = l2;
+        set_TM(c->TMf, c->TMn, g, l, l2);
+    }
     __syncthreads();
 }
 
