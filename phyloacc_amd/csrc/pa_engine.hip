@@ -15,6 +15,10 @@
 //     linear-space with a power-of-two rescale per node; the cumulative
 //     exponent rides along as an int; GAPT marks all-gap subtrees (the
 //     reference's Tg >= num_base, src/PhyloAcc-ST/bpp_c.hpp:417-428);
+//   * a cherry (a non-root internal node whose children are both leaves) is
+//     never stored: its parent recomputes its scaled partial from the two
+//     leaf codes with the same operations, so the first tree level costs no
+//     pass of its own;
 //   * the root sum over patterns is FP64 with a fixed reduction order.
 // No MFMA: a K=4 matvec per pattern is not a dense contraction.
 #include <hip/hip_runtime.h>
@@ -204,6 +208,39 @@ __device__ __forceinline__ ChildIn<R> child_load(bool leaf, bool miss, const GBL
     return in;
 }
 
+// Node flag set when an element is bound (above the element table's flags):
+// the node is a cherry whose partial is recomputed where it is read.
+constexpr uint8_t NF_CHERRY = 0x80;
+
+// The scaled partial and exponent of a cherry at one pattern, from its leaf
+// codes cA, cB and their branch matrices: the operations the pass would apply
+// to the cherry, in the same order, so the values are those it would have
+// stored.
+template <typename R>
+__device__ __forceinline__ ChildIn<R> cherry_in(const LDS R *srcA, const LDS R *srcB, bool missA, bool missB,
+                                                bool miss_x, int cA, int cB)
+{
+    ChildIn<R> in;
+    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
+    R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
+    const bool gA = (cA == PA_GAP_CODE || cA == PA_OTHER_CODE);
+    const bool gB = (cB == PA_GAP_CODE || cB == PA_OTHER_CODE);
+    if (!missA && !gA) leaf_mul<R>(srcA, cA, a0, a1, a2, a3);
+    if (!missB && !gB) leaf_mul<R>(srcB, cB, a0, a1, a2, a3);
+    if (gA && gB) {
+        in.x = GAPT;
+    } else if (miss_x) {
+        in.x = 0;
+    } else {
+        const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
+        int kx = 0;
+        frexp(mx, &kx);
+        in.v.x = ldexp(a0, -kx); in.v.y = ldexp(a1, -kx); in.v.z = ldexp(a2, -kx); in.v.w = ldexp(a3, -kx);
+        in.x = kx;
+    }
+    return in;
+}
+
 // Multiplies one child's message P * L_child into (a0..a3); PT is the
 // child's branch matrix in LDS.  gap reports the child's GAPT state
 // (Tg >= num_base).
@@ -270,7 +307,10 @@ __device__ __noinline__ double prune(MP M, CP<R> c)
     const GBL uint8_t *codes = uni_ptr(c->codes);
     const int Pb = uni((P + BS - 1) / BS);
     constexpr int U = Unroll<R>::U;
-    for (int h = 0; h < c->n_up; h++) {
+    const int n_up = uni(c->n_up);
+    // every node of the first level (height 1) is a cherry, unless the tree is
+    // one cherry: its root
+    for (int h = n_up > 1 ? 1 : 0; h < n_up; h++) {
         // level h: up[upoff[h-1] .. upoff[h]) (from 0 for the first level)
         const int a = uni(h == 0 ? 0 : c->upoff[h - 1]), b = uni(c->upoff[h]);
         const int items = (b - a) * Pb;
@@ -284,12 +324,20 @@ __device__ __noinline__ double prune(MP M, CP<R> c)
                 const int s = uni(c->up[a + q]);
                 const int x0 = uni(c->ch0[s]), x1 = uni(c->ch1[s]);
                 const bool leaf0 = x0 < S, leaf1 = x1 < S;
-                const bool miss0 = uni(c->flags[x0]) & NF_MISSING, miss1 = uni(c->flags[x1]) & NF_MISSING;
+                const int f0 = uni(c->flags[x0]), f1 = uni(c->flags[x1]);
+                const bool miss0 = f0 & NF_MISSING, miss1 = f1 & NF_MISSING;
                 const size_t o0 = leaf0 ? 0 : (size_t)(x0 - S) * Ppad;
                 const size_t o1 = leaf1 ? 0 : (size_t)(x1 - S) * Ppad;
                 if (p < P) {
-                    in0[u] = child_load<R>(leaf0, miss0, codes + (size_t)x0 * Ppad, parts + o0, exps + o0, p);
-                    in1[u] = child_load<R>(leaf1, miss1, codes + (size_t)x1 * Ppad, parts + o1, exps + o1, p);
+                    // a cherry child: its two leaf codes, packed
+                    if (f0 & NF_CHERRY)
+                        in0[u].x = codes[(size_t)uni(c->ch0[x0]) * Ppad + p] | (codes[(size_t)uni(c->ch1[x0]) * Ppad + p] << 8);
+                    else
+                        in0[u] = child_load<R>(leaf0, miss0, codes + (size_t)x0 * Ppad, parts + o0, exps + o0, p);
+                    if (f1 & NF_CHERRY)
+                        in1[u].x = codes[(size_t)uni(c->ch0[x1]) * Ppad + p] | (codes[(size_t)uni(c->ch1[x1]) * Ppad + p] << 8);
+                    else
+                        in1[u] = child_load<R>(leaf1, miss1, codes + (size_t)x1 * Ppad, parts + o1, exps + o1, p);
                 }
             }
 #pragma unroll
@@ -305,7 +353,18 @@ __device__ __noinline__ double prune(MP M, CP<R> c)
                 const int x0 = uni(c->ch0[s]), x1 = uni(c->ch1[s]);
                 const bool leaf0 = x0 < S, leaf1 = x1 < S;
                 const bool miss_s = uni(c->flags[s]) & NF_MISSING;
-                const bool miss0 = uni(c->flags[x0]) & NF_MISSING, miss1 = uni(c->flags[x1]) & NF_MISSING;
+                const int f0 = uni(c->flags[x0]), f1 = uni(c->flags[x1]);
+                const bool miss0 = f0 & NF_MISSING, miss1 = f1 & NF_MISSING;
+                if (f0 & NF_CHERRY) {
+                    const int yA = uni(c->ch0[x0]), yB = uni(c->ch1[x0]);
+                    in0[u] = cherry_in<R>(c->PT + 16 * yA, c->PT + 16 * yB, uni(c->flags[yA]) & NF_MISSING,
+                                          uni(c->flags[yB]) & NF_MISSING, miss0, in0[u].x & 255, in0[u].x >> 8);
+                }
+                if (f1 & NF_CHERRY) {
+                    const int yA = uni(c->ch0[x1]), yB = uni(c->ch1[x1]);
+                    in1[u] = cherry_in<R>(c->PT + 16 * yA, c->PT + 16 * yB, uni(c->flags[yA]) & NF_MISSING,
+                                          uni(c->flags[yB]) & NF_MISSING, miss1, in1[u].x & 255, in1[u].x >> 8);
+                }
                 GBL V4<R> *dstp = parts + (size_t)(s - S) * Ppad;
                 GBL int32_t *dste = exps + (size_t)(s - S) * Ppad;
                 R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
@@ -390,6 +449,9 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
     }
     for (int i = tid; i <= N; i += BS) c->upoff[i] = (int16_t)tab[tab_upoff(N) + i];
     __syncthreads();
+    // cherries (both children leaves; never the root) are recomputed where read
+    if (c->n_up > 1)
+        for (int q = tid; q < c->upoff[0]; q += BS) c->flags[c->up[q]] |= NF_CHERRY;
     const bool own = A->len_in != nullptr;
     const double r1 = own ? 1.0 : A->rates_in[2 * w], r2 = own ? 1.0 : A->rates_in[2 * w + 1];
     const double *dist = (const double *)(tab + tab_dist(N));
