@@ -517,6 +517,10 @@ struct pa_engine {
     size_t lds_per_cu = 0;
     pa_tuning tune = {0, 0, 0, 0.0};        // block size of batched chain launches; 0: by mean pattern count
     uint64_t arena_gen = 0;                  // bumped whenever the device arenas move
+    // log-likelihood calls: scratch slots and per-call branch lengths, kept
+    // between calls and grown when a call needs more (freed with the engine)
+    void *d_ll_scratch = nullptr, *d_ll_len = nullptr;
+    size_t ll_scratch_bytes = 0, ll_len_bytes = 0;
 };
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
@@ -657,6 +661,8 @@ extern "C" void pa_engine_destroy(pa_engine *e)
         (void)hipStreamSynchronize(e->stream);
         free_arenas(e);
         (void)hipFree(e->d_queue);
+        (void)hipFree(e->d_ll_scratch);
+        (void)hipFree(e->d_ll_len);
         (void)hipStreamDestroy(e->stream);
     }
     delete e;
@@ -851,14 +857,21 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     int Pcap = 64;
     for (int i = 0; i < n; i++) Pcap = std::max(Pcap, e->elems[ids[i]].Ppad);
     const size_t slot = loglik_slot_bytes(N, S, Pcap, fp64);
-    HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
+    // the previous call synchronised its stream, so its scratch is free to reuse
+    if (slot * grid > e->ll_scratch_bytes) {
+        (void)hipFree(e->d_ll_scratch);
+        e->d_ll_scratch = nullptr;
+        e->ll_scratch_bytes = 0;
+        HIPCHK(e, hipMalloc(&e->d_ll_scratch, slot * grid));
+        e->ll_scratch_bytes = slot * grid;
+    }
     HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n));
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
     A.work = (const int32_t *)bufs[1].p;
     A.n_work = n;
     A.queue = e->d_queue;
-    A.scratch = (char *)bufs[0].p;
+    A.scratch = (char *)e->d_ll_scratch;
     A.slot_bytes = (int64_t)slot;
     HIPCHK(e, launch_reset(A.queue, e->stream));
     HIPCHK(e, launch_loglik(e->M, A, fp64, BS, grid, e->stream));
@@ -1016,12 +1029,18 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     for (int i = 0; i < n; i++) memcpy(&lw[(size_t)i * N], lengths + (size_t)order[i] * N, sizeof(double) * N);
     DevBuf bufs[7];
     HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
-    HIPCHK(e, hipMalloc(&bufs[6].p, lw.size() * 8));
-    HIPCHK(e, hipMemcpy(bufs[6].p, lw.data(), lw.size() * 8, hipMemcpyHostToDevice));
+    if (lw.size() * 8 > e->ll_len_bytes) {
+        (void)hipFree(e->d_ll_len);
+        e->d_ll_len = nullptr;
+        e->ll_len_bytes = 0;
+        HIPCHK(e, hipMalloc(&e->d_ll_len, lw.size() * 8));
+        e->ll_len_bytes = lw.size() * 8;
+    }
+    HIPCHK(e, hipMemcpy(e->d_ll_len, lw.data(), lw.size() * 8, hipMemcpyHostToDevice));
     LoglikArgs A;
     memset(&A, 0, sizeof A);
     A.ll_out = (double *)bufs[5].p;
-    A.len_in = (const double *)bufs[6].p;
+    A.len_in = (const double *)e->d_ll_len;
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
     st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);
