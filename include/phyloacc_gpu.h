@@ -147,6 +147,13 @@ pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *
 pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
                             int32_t precision, double *out);
 
+/* pa_loglik and pa_loglik_lengths keep their device scratch (workgroup slots
+ * and per-call length buffer) for the next call when it is at most 2 GiB;
+ * this frees it now (no reference counterpart: the reference allocates per
+ * element on the host).  Call it before a large pa_run_chains or
+ * pa_serve_create when device memory is tight. */
+pa_status pa_release_scratch(pa_engine *e);
+
 /* Results of one model chain of one element (BPP_C::Gibbs bookkeeping,
  * src/PhyloAcc-ST/bpp_c.cpp:470-485). */
 typedef struct {
@@ -178,6 +185,7 @@ typedef struct {
     int32_t grid, block_threads;
     int32_t lds_bytes;          /* LDS per workgroup */
     int32_t n_launches;
+    double scratch_bytes;       /* device scratch of the launch: workgroup slots + long elements' own regions */
 } pa_run_stats;
 
 /* Runs the three model chains M0, M1, M2 (reference order resZ 0, 2, 1,

@@ -20,7 +20,12 @@ struct ChainArgs {
     const int32_t *tab;
     char *scratch;            // per-workgroup slots
     int64_t slot_bytes;
-    int64_t off_parts, off_exps, off_plin, off_logt, off_amb;
+    int64_t off_plin, off_logt, off_amb, off_parts;   // within a slot (slot_scratch_bytes)
+    // Elements too long for a slot's partial region: work position w keeps its
+    // partials at big + big_off[w] (chain_part_bytes of its own Ppad);
+    // big_off[w] < 0, or big_off null, means the workgroup's slot.
+    char *big;
+    const int64_t *big_off;
     // outputs, indexed by work position
     double *res_d;            // [n_work][3][8]
     int32_t *res_i;           // [n_work][3][4]
@@ -32,9 +37,16 @@ struct ChainArgs {
 
 // LDS bytes of one workgroup (dynamic arrays + context + model constants).
 size_t chain_smem_bytes(int N, int S, int BS, int fp64);
-// Byte layout of one workgroup's HBM slot for elements of up to Pcap
-// (padded) patterns: off = {parts, exps, plin, logt, amb}; returns its size.
-size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off /*[5]*/);
+// Dynamic LDS bytes of one workgroup (the launch's dynamic allocation).
+size_t chain_dyn_smem_bytes(int N, int S, int BS, int fp64);
+// Workgroups of the chain kernel resident on one CU (registers, LDS, launch bound).
+int chain_blocks_per_cu(int fp64, int BS, int N, int S);
+// Partials and exponents of one element of Ppad padded patterns: two buffers
+// per internal node; the exponents start at chain_exps_off.
+size_t chain_part_bytes(int N, int S, int Ppad, int fp64);
+// Byte layout of one workgroup's HBM slot whose partial region holds elements
+// of up to Pcap (padded) patterns: off = {plin, logt, amb, parts}; returns its size.
+size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off /*[4]*/);
 hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st);
 hipError_t launch_reset(int *q, hipStream_t st);
 

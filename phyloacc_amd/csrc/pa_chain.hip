@@ -81,7 +81,7 @@ struct Ctx {
     LDS uint8_t *tgs;
     // chain scalars
     double loglik, nrate, crate, g, l, l2, prop_n, prop_c, proposal, r, max_loglik;
-    double TMf[9], TMn[9];  // linear log_TM_Int for fixZ / free nodes, [i*3+j] = P(child i | parent j)
+    double TMf[9], TMn[9];  // log_TM_Int for fixZ / free nodes, [i*3+j] = log P(child i | parent j)
     double red[16];
     unsigned long long bytes;   // algorithmic HBM bytes of this element's passes (see prune / sample_sites)
     int accept_n, accept_c, max_m, skip, accept, status, work, nch;
@@ -154,6 +154,11 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
     return al(o, 16);
 }
 
+size_t chain_dyn_smem_bytes(int N, int S, int BS, int fp64)
+{
+    return fp64 ? smem_arrays<double, false>(N, S, BS, nullptr, nullptr) : smem_arrays<float, false>(N, S, BS, nullptr, nullptr);
+}
+
 size_t chain_smem_bytes(int N, int S, int BS, int fp64)
 {
     return (fp64 ? smem_arrays<double, false>(N, S, BS, nullptr, nullptr)
@@ -161,16 +166,27 @@ size_t chain_smem_bytes(int N, int S, int BS, int fp64)
            (fp64 ? sizeof(Ctx<double>) : sizeof(Ctx<float>)) + sizeof(ModelDev);
 }
 
-// Per-workgroup HBM scratch layout (bytes), shared with the host.
+// Byte offset of the exponents behind the partials of an element of Ppad
+// padded patterns ([2][I][Ppad] 4-vectors of R).
+__host__ __device__ inline size_t chain_exps_off(int N, int S, int Ppad, int rs)
+{
+    return al((size_t)2 * (N - S) * Ppad * 4 * rs, 256);
+}
+
+size_t chain_part_bytes(int N, int S, int Ppad, int fp64)
+{
+    return chain_exps_off(N, S, Ppad, fp64 ? 8 : 4) + al((size_t)2 * (N - S) * Ppad * 4, 256);
+}
+
+// Per-workgroup HBM scratch layout (bytes), shared with the host: the rate
+// class tables and ambiguity counters, then the partial region.
 size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off)
 {
-    int I = N - S;
-    size_t rs = fp64 ? 8 : 4, o = 0;
-    off[0] = o; o = al(o + (size_t)2 * I * Pcap * 4 * rs, 256);   // parts
-    off[1] = o; o = al(o + (size_t)2 * I * Pcap * 4, 256);        // exps
-    off[2] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // plin
-    off[3] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // logt
-    off[4] = o; o = al(o + (size_t)S * 24 * 4, 256);              // amb
+    size_t o = 0;
+    off[0] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // plin
+    off[1] = o; o = al(o + (size_t)3 * N * 16 * 8, 256);          // logt
+    off[2] = o; o = al(o + (size_t)S * 24 * 4, 256);              // amb
+    off[3] = o; o += chain_part_bytes(N, S, Pcap, fp64);          // parts, exps
     return o;
 }
 
@@ -753,9 +769,22 @@ __device__ __noinline__ void emissions(CP<R> c)
 
 // ----------------------------------------------------------------- Z update
 
-// BPP_C::Update_Z_subtree (src/PhyloAcc-ST/bpp_c.cpp:1182-1293) in linear
-// space, one tree level per step: upward messages by height, then top-down
-// draws by depth.  The draw for node s uses Z-stream slot tz[s].
+// log sum_i exp(T[i*3+j] + y[i]) for parent class j, with the column's own
+// maximum taken out first (log_multi2, src/PhyloAcc-ST/bpp.hpp:447-460);
+// -inf when every term is -inf.
+__device__ __forceinline__ double log_col(const LDS double *T, int j, double y0, double y1, double y2)
+{
+    const double a = T[j] + y0, b = T[3 + j] + y1, d = T[6 + j] + y2;
+    const double mx = fmax(a, fmax(b, d));
+    if (mx == -INFINITY) return -INFINITY;
+    return mx + log(exp(a - mx) + exp(b - mx) + exp(d - mx));
+}
+
+// BPP_C::Update_Z_subtree (src/PhyloAcc-ST/bpp_c.cpp:1182-1293) in log space,
+// one tree level per step: upward messages by height, then top-down draws by
+// depth.  The draw for node s uses Z-stream slot tz[s].  Log space matters:
+// over thousands of sites the classes' emissions differ by thousands of
+// nats, and products of linear messages would underflow to zero.
 template <typename R, int BS>
 __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
 {
@@ -767,12 +796,9 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
         if (s == c->root) { e0 = e1 = e2 = 0; }
         else if (c->flags[s] & NF_MISSING) { e0 = log(M->nconstomis); e1 = log(M->constomis); e2 = log(M->nconstomis); }
         else { e0 = c->E[s * 3]; e1 = c->E[s * 3 + 1]; e2 = c->E[s * 3 + 2]; }
-        const bool fz = c->fixZ[s];
-        double mx = fmax(e0, e1);
-        if (!fz) mx = fmax(mx, e2);
-        c->beta[s * 3 + 0] = exp(e0 - mx);
-        c->beta[s * 3 + 1] = exp(e1 - mx);
-        c->beta[s * 3 + 2] = fz ? 0.0 : exp(e2 - mx);
+        c->beta[s * 3 + 0] = e0;
+        c->beta[s * 3 + 1] = e1;
+        c->beta[s * 3 + 2] = c->fixZ[s] ? -INFINITY : e2;
         c->changed[s] = 0;
         c->dirty[s] = 0;
     }
@@ -787,19 +813,20 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
                 const int x = w ? c->ch1[p] : c->ch0[p];
                 const LDS double *T = c->fixZ[x] ? c->TMf : c->TMn;
                 const double y0 = c->beta[x * 3], y1 = c->beta[x * 3 + 1], y2 = c->beta[x * 3 + 2];
-                b0 *= T[0] * y0 + T[3] * y1 + T[6] * y2;
-                b1 *= T[1] * y0 + T[4] * y1 + T[7] * y2;
-                b2 *= T[2] * y0 + T[5] * y1 + T[8] * y2;
+                b0 += log_col(T, 0, y0, y1, y2);
+                b1 += log_col(T, 1, y0, y1, y2);
+                b2 += log_col(T, 2, y0, y1, y2);
             }
-            const double mx = fmax(b0, fmax(b1, b2));
-            if (mx > 0) { b0 /= mx; b1 /= mx; b2 /= mx; }
             c->beta[p * 3] = b0; c->beta[p * 3 + 1] = b1; c->beta[p * 3 + 2] = b2;
         }
         __syncthreads();
     }
     if (tid == 0) {
+        // root prior (1/2, 1/2, 0)
         const int r = c->root;
-        double pr[3] = {c->beta[r * 3] * 0.5, c->beta[r * 3 + 1] * 0.5, 0.0};
+        const double l0 = c->beta[r * 3] + log(0.5), l1 = c->beta[r * 3 + 1] + log(0.5);
+        const double mx = fmax(l0, l1);
+        double pr[3] = {exp(l0 - mx), exp(l1 - mx), 0.0};
         const int z = categorical<3>(pr, slot_uniform(key, 0, c2, c3, 0));
         if (c->Z[r] != z) c->changed[r] = 1;
         c->Z[r] = (int8_t)z;
@@ -811,7 +838,10 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
             const int s = c->down[q];
             const int zp = c->Z[c->par[s]];
             const LDS double *T = c->fixZ[s] ? c->TMf : c->TMn;
-            double pr[3] = {T[0 * 3 + zp] * c->beta[s * 3], T[1 * 3 + zp] * c->beta[s * 3 + 1], T[2 * 3 + zp] * c->beta[s * 3 + 2]};
+            const double y0 = T[0 * 3 + zp] + c->beta[s * 3], y1 = T[1 * 3 + zp] + c->beta[s * 3 + 1],
+                         y2 = T[2 * 3 + zp] + c->beta[s * 3 + 2];
+            const double mx = fmax(y0, fmax(y1, y2));
+            double pr[3] = {exp(y0 - mx), exp(y1 - mx), exp(y2 - mx)};
             const int z = categorical<3>(pr, slot_uniform(key, 0, c2, c3, (uint32_t)c->tz[s]));
             if (c->Z[s] != z) c->changed[s] = 1;
             c->Z[s] = (int8_t)z;
@@ -837,13 +867,17 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
 
 __device__ __forceinline__ void set_TM(LDS double *F, LDS double *T, double g, double l, double l2)
 {
-    // fixZ nodes (bpp_c.cpp:305-316, 577-585); column 2 keeps log 0 -> 1 (never used)
-    F[0] = 1 - g; F[3] = g; F[6] = 0;
-    F[1] = 0; F[4] = 1; F[7] = 0;
-    F[2] = 1; F[5] = 1; F[8] = 1;
-    T[0] = 1 - g; T[3] = g; T[6] = 0;
-    T[1] = 0; T[4] = 1 - l; T[7] = l;
-    T[2] = 0; T[5] = l2; T[8] = 1 - l2;
+    // log_TM_Int, [i*3+j] = log P(child i | parent j).  fixZ nodes
+    // (bpp_c.cpp:305-316, 577-585): column 2 is never set there and keeps its
+    // initial log 1 (a fixZ node's parent is never accelerated, so it is unused).
+    const double lg = log(g), l1g = log(1 - g);
+    F[0] = l1g; F[3] = lg; F[6] = -INFINITY;
+    F[1] = -INFINITY; F[4] = 0; F[7] = -INFINITY;
+    F[2] = 0; F[5] = 0; F[8] = 0;
+    T[0] = l1g; T[3] = lg; T[6] = -INFINITY;
+    const double y = 1 - l;   // the reference forms P(1 -> 2) as 1 - (1 - l)
+    T[1] = -INFINITY; T[4] = log(y); T[7] = log(1 - y);
+    T[2] = -INFINITY; T[5] = log(l2); T[8] = log(1 - l2);
 }
 
 // Scalar head of BPP_C::sample_rate (src/PhyloAcc-ST/bpp_c.cpp:930-1041): prior draw
@@ -1064,10 +1098,10 @@ __device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
 // ---------------------------------------------------------- element setup
 
 template <typename R, int BS>
-__device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int eid)
+__device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int work)
 {
     const int tid = threadIdx.x, N = M->N;
-    const ElemDev E = A->elems[eid];
+    const ElemDev E = A->elems[A->work[work]];
     if (tid == 0) {
         c->S = M->S; c->N = N; c->I = M->I;
         c->GG = E.GG; c->P = E.P; c->Ppad = E.Ppad; c->root = E.root;
@@ -1076,8 +1110,10 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
         c->wts = (const GBL int32_t *)(A->wts + E.off_w);
         c->site = (const GBL int32_t *)(A->site + E.off_site);
         char *base = A->scratch + (size_t)blockIdx.x * A->slot_bytes;
-        c->parts = (GBL V4<R> *)(base + A->off_parts);
-        c->exps = (GBL int32_t *)(base + A->off_exps);
+        const int64_t bo = A->big_off ? A->big_off[work] : -1;
+        char *pbase = bo >= 0 ? A->big + bo : base + A->off_parts;
+        c->parts = (GBL V4<R> *)pbase;
+        c->exps = (GBL int32_t *)(pbase + chain_exps_off(N, M->S, E.Ppad, sizeof(R)));
         c->plin = (GBL double *)(base + A->off_plin);
         c->logt = (GBL double *)(base + A->off_logt);
         c->amb = (GBL uint32_t *)(base + A->off_amb);
@@ -1260,7 +1296,7 @@ __global__ void __launch_bounds__(BS, 4) chain_kernel(ModelDev Mk, ChainArgs A)
         const int w = c->work;
         __syncthreads();
         if (w >= A.n_work) break;
-        bind_element<R, BS>(M, c, &A, A.work[w]);
+        bind_element<R, BS>(M, c, &A, w);
         for (int ms = 0; ms < 3; ms++) run_model<R, BS>(M, c, &A, ms, w);
         if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes);
     }
@@ -1282,6 +1318,31 @@ hipError_t launch_reset(int *q, hipStream_t st)
 {
     hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, q);
     return hipGetLastError();
+}
+
+template <typename R, int BS>
+static int blocks_t(int N, int S)
+{
+    const size_t sm = smem_arrays<R, false>(N, S, BS, nullptr, nullptr);
+    if (hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
+        return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<R, BS>, BS, sm) != hipSuccess) return 0;
+    return nb;
+}
+
+int chain_blocks_per_cu(int fp64, int BS, int N, int S)
+{
+    if (fp64) {
+        if (BS == 64) return blocks_t<double, 64>(N, S);
+        if (BS == 128) return blocks_t<double, 128>(N, S);
+        if (BS == 256) return blocks_t<double, 256>(N, S);
+    } else {
+        if (BS == 64) return blocks_t<float, 64>(N, S);
+        if (BS == 128) return blocks_t<float, 128>(N, S);
+        if (BS == 256) return blocks_t<float, 256>(N, S);
+    }
+    return 0;
 }
 
 hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st)

@@ -97,6 +97,11 @@ __host__ __device__ size_t smem_arrays(int N, LDS char *base, CP<R> c)
     return al(o, 16);
 }
 
+size_t loglik_dyn_smem_bytes(int N, int fp64)
+{
+    return fp64 ? smem_arrays<double, false>(N, nullptr, nullptr) : smem_arrays<float, false>(N, nullptr, nullptr);
+}
+
 size_t loglik_smem_bytes(int N, int fp64)
 {
     return (fp64 ? smem_arrays<double, false>(N, nullptr, nullptr) : smem_arrays<float, false>(N, nullptr, nullptr)) +
@@ -435,7 +440,8 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
         c->n_nodes = E.n_nodes; c->n_up = E.n_up;
         c->codes = (const GBL uint8_t *)(A->codes + E.off_codes);
         c->wts = (const GBL int32_t *)(A->wts + E.off_w);
-        char *base = A->scratch + (int64_t)blockIdx.x * A->slot_bytes;
+        const int64_t bo = A->big_off ? A->big_off[w] : -1;
+        char *base = bo >= 0 ? A->big + bo : A->scratch + (int64_t)blockIdx.x * A->slot_bytes;
         c->parts = (GBL V4<R> *)base;
         c->exps = (GBL int32_t *)(base + slot_exps_off(N, M->S, E.Ppad, sizeof(R) == 8));
     }
@@ -504,6 +510,13 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
         const int w = c->work;
         __syncthreads();
         if (w >= A->n_work) break;
+        // an id outside the arenas, or a filtered element (no patterns), scores NaN
+        const int id = A->work[w];
+        if (id < 0 || id >= A->n_elems || A->elems[id].P <= 0) {
+            if (threadIdx.x == 0) A->ll_out[w] = __builtin_nan("");
+            __syncthreads();
+            continue;
+        }
         bind_element<R, BS>(M, c, A, w);
         const double l = prune<R, BS>(M, c);
         if (threadIdx.x == 0) A->ll_out[w] = l;
@@ -530,6 +543,32 @@ hipError_t launch_reset(int *q, hipStream_t st)
 {
     hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, q);
     return hipGetLastError();
+}
+
+template <typename R, int BS>
+static int blocks_t(int N)
+{
+    const size_t sm = smem_arrays<R, false>(N, nullptr, nullptr);
+    if (sm > 65536 &&
+        hipFuncSetAttribute((const void *)loglik_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
+        return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, loglik_kernel<R, BS>, BS, sm) != hipSuccess) return 0;
+    return nb;
+}
+
+int loglik_blocks_per_cu(int fp64, int BS, int N)
+{
+    if (fp64) {
+        if (BS == 64) return blocks_t<double, 64>(N);
+        if (BS == 128) return blocks_t<double, 128>(N);
+        if (BS == 256) return blocks_t<double, 256>(N);
+    } else {
+        if (BS == 64) return blocks_t<float, 64>(N);
+        if (BS == 128) return blocks_t<float, 128>(N);
+        if (BS == 256) return blocks_t<float, 256>(N);
+    }
+    return 0;
 }
 
 hipError_t launch_loglik(const ModelDev &M, const LoglikArgs &A, int fp64, int BS, int grid, hipStream_t st)

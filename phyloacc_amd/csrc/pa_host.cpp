@@ -521,6 +521,7 @@ struct pa_engine {
     // between calls and grown when a call needs more (freed with the engine)
     void *d_ll_scratch = nullptr, *d_ll_len = nullptr;
     size_t ll_scratch_bytes = 0, ll_len_bytes = 0;
+    size_t ll_last_plan_bytes = 0;           // scratch the last likelihood launch used
 };
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
@@ -838,11 +839,77 @@ static int pick_block(pa_engine *e, const int32_t *work, int32_t n)
     return (meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256);
 }
 
-// Workgroups of block size BS that fit on one CU at once (LDS and waves).
-static int fit_per_cu(pa_engine *e, size_t smem, int BS)
+// Workgroups resident on one CU at once, from the runtime's occupancy query
+// of the kernel itself (registers under its launch bound, LDS, waves); the
+// LDS and wave count alone bound it when the query fails.
+static int fit_per_cu(pa_engine *e, size_t smem, int BS, int queried)
 {
     const int per_cu = (int)std::min<size_t>(8, e->lds_per_cu / smem);
-    return std::max(1, std::min(per_cu, 2048 / BS));
+    const int bound = std::max(1, std::min(per_cu, 2048 / BS));
+    return queried > 0 ? std::min(queried, bound) : bound;
+}
+static int chain_grid_max(pa_engine *e, int fp64, int BS)
+{
+    const int N = e->tm.N, S = e->tm.S;
+    return e->n_cu * fit_per_cu(e, chain::chain_smem_bytes(N, S, BS, fp64), BS, chain::chain_blocks_per_cu(fp64, BS, N, S));
+}
+static int loglik_grid_max(pa_engine *e, int fp64, int BS)
+{
+    const int N = e->tm.N;
+    return e->n_cu * fit_per_cu(e, loglik_smem_bytes(N, fp64), BS, loglik_blocks_per_cu(fp64, BS, N));
+}
+
+// Scratch of one persistent launch of `grid` workgroups over elements of
+// padded pattern counts ppad[] (work order).  Each workgroup's slot has a
+// fixed part and a partial region for elements of up to `cap` patterns; an
+// element above `cap` keeps its partials in a region of its own behind the
+// slots (big_off[w], bytes from the start of that area; -1: the slot).  The
+// cap minimises the total: the slots of a heavy-tailed batch are not all
+// sized for its longest element.
+namespace {
+struct ScratchPlan {
+    int cap = 64;
+    size_t slot = 0, slots = 0, big = 0, total = 0;
+    std::vector<int64_t> big_off;
+};
+}  // namespace
+
+template <typename PartBytes>
+static ScratchPlan plan_scratch(const std::vector<int> &ppad, int grid, size_t fixed, PartBytes part)
+{
+    std::vector<int> v(ppad);
+    std::sort(v.begin(), v.end(), std::greater<int>());
+    ScratchPlan best;
+    best.total = SIZE_MAX;
+    // candidate caps: every distinct padded count; above-cap elements are a prefix of v
+    size_t big = 0;
+    for (size_t j = 0; j < v.size(); j++) {
+        if (j > 0 && v[j] == v[j - 1]) { big += part(v[j]); continue; }
+        const size_t slot = (fixed + part(v[j]) + 255) & ~(size_t)255;
+        const size_t tot = slot * (size_t)grid + big;
+        if (tot < best.total) { best.cap = v[j]; best.slot = slot; best.slots = slot * (size_t)grid; best.big = big; best.total = tot; }
+        big += part(v[j]);
+    }
+    if (v.empty()) {
+        best.slot = (fixed + part(64) + 255) & ~(size_t)255;
+        best.slots = best.total = best.slot * (size_t)grid;
+    }
+    best.big_off.assign(ppad.size(), -1);
+    size_t o = 0;
+    for (size_t w = 0; w < ppad.size(); w++)
+        if (ppad[w] > best.cap) { best.big_off[w] = (int64_t)o; o += part(ppad[w]); }
+    return best;
+}
+
+// Retained likelihood scratch above this size is released when the call ends.
+static const size_t PA_RETAIN_MAX = (size_t)2 << 30;
+
+static void release_ll_buffers(pa_engine *e)
+{
+    (void)hipFree(e->d_ll_scratch);
+    (void)hipFree(e->d_ll_len);
+    e->d_ll_scratch = e->d_ll_len = nullptr;
+    e->ll_scratch_bytes = e->ll_len_bytes = 0;
 }
 
 // Fixed-(Z, r) log-likelihood launch: one persistent launch over ids (in
@@ -853,29 +920,35 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     const int BS = pick_block(e, ids, n);
     const size_t smem = loglik_smem_bytes(N, fp64);
     if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
-    const int grid = std::min(n, e->n_cu * fit_per_cu(e, smem, BS));
-    int Pcap = 64;
-    for (int i = 0; i < n; i++) Pcap = std::max(Pcap, e->elems[ids[i]].Ppad);
-    const size_t slot = loglik_slot_bytes(N, S, Pcap, fp64);
+    const int grid = std::min(n, loglik_grid_max(e, fp64, BS));
+    std::vector<int> ppad(n);
+    for (int i = 0; i < n; i++) ppad[i] = e->elems[ids[i]].Ppad;
+    const ScratchPlan pl = plan_scratch(ppad, grid, 0, [&](int p) { return loglik_slot_bytes(N, S, p, fp64); });
     // the previous call synchronised its stream, so its scratch is free to reuse
-    if (slot * grid > e->ll_scratch_bytes) {
+    if (pl.total > e->ll_scratch_bytes) {
         (void)hipFree(e->d_ll_scratch);
         e->d_ll_scratch = nullptr;
         e->ll_scratch_bytes = 0;
-        HIPCHK(e, hipMalloc(&e->d_ll_scratch, slot * grid));
-        e->ll_scratch_bytes = slot * grid;
+        HIPCHK(e, hipMalloc(&e->d_ll_scratch, pl.total));
+        e->ll_scratch_bytes = pl.total;
     }
-    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n));
+    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + sizeof(int64_t) * n + 16));
+    int64_t *d_bo = (int64_t *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(d_bo, pl.big_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, e->stream));
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
+    A.n_elems = (int)e->elems.size();
     A.work = (const int32_t *)bufs[1].p;
     A.n_work = n;
     A.queue = e->d_queue;
     A.scratch = (char *)e->d_ll_scratch;
-    A.slot_bytes = (int64_t)slot;
+    A.slot_bytes = (int64_t)pl.slot;
+    A.big = (char *)e->d_ll_scratch + pl.slots;
+    A.big_off = pl.big ? d_bo : nullptr;
     HIPCHK(e, launch_reset(A.queue, e->stream));
     HIPCHK(e, launch_loglik(e->M, A, fp64, BS, grid, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->ll_last_plan_bytes = pl.total;
     return PA_OK;
 }
 
@@ -891,22 +964,29 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
     if (BS != 64 && BS != 128 && BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
     const size_t smem = chain::chain_smem_bytes(N, S, BS, fp64);
     if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
-    const int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, e->n_cu * fit_per_cu(e, smem, BS));
-    int Pcap = 64;
-    for (int i = 0; i < n; i++) Pcap = std::max(Pcap, e->elems[ids[i]].Ppad);
-    size_t off[5];
-    const size_t slot = chain::slot_scratch_bytes(N, S, Pcap, fp64, off);
-    HIPCHK(e, hipMalloc(&bufs[0].p, slot * grid));
-    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + 64));
-    unsigned long long *d_cnt = (unsigned long long *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
+    const int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, chain_grid_max(e, fp64, BS));
+    std::vector<int> ppad(n);
+    for (int i = 0; i < n; i++) ppad[i] = e->elems[ids[i]].Ppad;
+    size_t off[4];
+    const size_t fixed = chain::slot_scratch_bytes(N, S, 0, fp64, off);
+    const ScratchPlan pl = plan_scratch(ppad, grid, fixed, [&](int p) { return chain::chain_part_bytes(N, S, p, fp64); });
+    chain::slot_scratch_bytes(N, S, pl.cap, fp64, off);
+    HIPCHK(e, hipMalloc(&bufs[0].p, pl.total));
+    const size_t ids_b = (sizeof(int32_t) * n + 15) & ~(size_t)15;
+    HIPCHK(e, hipMalloc(&bufs[1].p, ids_b + sizeof(int64_t) * n + 64));
+    int64_t *d_bo = (int64_t *)((char *)bufs[1].p + ids_b);
+    unsigned long long *d_cnt = (unsigned long long *)(d_bo + n);
     HIPCHK(e, hipMemsetAsync(d_cnt, 0, 16, e->stream));
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(d_bo, pl.big_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, e->stream));
     A.counters = d_cnt;
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
     A.queue = e->d_queue;
     A.scratch = (char *)bufs[0].p;
-    A.slot_bytes = (int64_t)slot;
-    A.off_parts = off[0]; A.off_exps = off[1]; A.off_plin = off[2]; A.off_logt = off[3]; A.off_amb = off[4];
+    A.slot_bytes = (int64_t)pl.slot;
+    A.off_plin = off[0]; A.off_logt = off[1]; A.off_amb = off[2]; A.off_parts = off[3];
+    A.big = (char *)bufs[0].p + pl.slots;
+    A.big_off = pl.big ? d_bo : nullptr;
     hipEvent_t t0, t1;
     HIPCHK(e, hipEventCreate(&t0));
     HIPCHK(e, hipEventCreate(&t1));
@@ -916,6 +996,7 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
             chain::ChainArgs B = A;
             B.work = (const int32_t *)bufs[1].p + i;
             B.n_work = 1;
+            B.big_off = A.big_off ? A.big_off + i : nullptr;
             // per-element outputs shift with the work position
             B.res_d = A.res_d ? A.res_d + (size_t)i * 24 : nullptr;
             B.res_i = A.res_i ? A.res_i + (size_t)i * 12 : nullptr;
@@ -944,6 +1025,7 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
         stats->block_threads = BS;
         stats->lds_bytes = (int32_t)smem;
         stats->n_launches = mode == PA_RUN_PER_ELEMENT ? n : 1;
+        stats->scratch_bytes = (double)pl.total;
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
@@ -1009,6 +1091,7 @@ extern "C" pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, cons
     std::vector<double> ll(n);
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) out[order[i]] = ll[i];
+    if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
     return PA_OK;
 }
 
@@ -1048,6 +1131,17 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     std::vector<double> ll(n);
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) out[order[i]] = ll[i];
+    if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
+    return PA_OK;
+}
+
+extern "C" pa_status pa_release_scratch(pa_engine *e)
+{
+    if (!e) return PA_E_INPUT;
+    if (!e->stream) return PA_OK;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    release_ll_buffers(e);
     return PA_OK;
 }
 
@@ -1403,6 +1497,8 @@ struct pa_server {
     int8_t *d_maxZ = nullptr;                       // [max][3][N]
     unsigned long long *d_cnt = nullptr;
     int32_t *d_lids = nullptr, *h_lids = nullptr;
+    int64_t *d_bo = nullptr, *h_bo = nullptr, *d_lbo = nullptr, *h_lbo = nullptr;   // long elements' own regions
+    std::vector<int64_t> big_off_c, big_off_l;
     double *d_llen = nullptr, *h_llen = nullptr, *d_lout = nullptr, *h_lout = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool in_flight = false;
@@ -1424,7 +1520,8 @@ static void serve_free(pa_server *s)
 {
     (void)hipFree(s->d_pool); (void)hipFree(s->d_ids); (void)hipFree(s->d_res_d); (void)hipFree(s->d_res_i);
     (void)hipFree(s->d_maxZ); (void)hipFree(s->d_cnt); (void)hipFree(s->d_lids);
-    (void)hipFree(s->d_llen); (void)hipFree(s->d_lout);
+    (void)hipFree(s->d_llen); (void)hipFree(s->d_lout); (void)hipFree(s->d_bo); (void)hipFree(s->d_lbo);
+    (void)hipHostFree(s->h_bo); (void)hipHostFree(s->h_lbo);
     (void)hipHostFree(s->h_ids); (void)hipHostFree(s->h_res_d); (void)hipHostFree(s->h_res_i);
     (void)hipHostFree(s->h_lids); (void)hipHostFree(s->h_llen); (void)hipHostFree(s->h_lout);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -1462,6 +1559,10 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     HIPCHK(e, hipHostMalloc(&s->h_ids, sizeof(int32_t) * M));
     HIPCHK(e, hipHostMalloc(&s->h_res_d, sizeof(double) * 24 * M));
     HIPCHK(e, hipHostMalloc(&s->h_res_i, sizeof(int32_t) * 12 * M));
+    HIPCHK(e, hipMalloc(&s->d_bo, sizeof(int64_t) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_bo, sizeof(int64_t) * M));
+    HIPCHK(e, hipMalloc(&s->d_lbo, sizeof(int64_t) * M));
+    HIPCHK(e, hipHostMalloc(&s->h_lbo, sizeof(int64_t) * M));
     HIPCHK(e, hipMalloc(&s->d_lids, sizeof(int32_t) * M));
     HIPCHK(e, hipMalloc(&s->d_llen, sizeof(double) * M * N));
     HIPCHK(e, hipMalloc(&s->d_lout, sizeof(double) * M));
@@ -1510,15 +1611,24 @@ extern "C" pa_status pa_serve_submit_lengths(pa_server *s, const int32_t *ids, i
     return PA_OK;
 }
 
-// Scratch of one launch of k workgroups over elements of up to Pcap padded patterns.
-static size_t chain_launch_scratch(pa_server *s, int Pcap, int k)
+// A server's slots hold elements of up to SERVE_SLOT_PATTERNS padded
+// patterns; longer ones get a partial region of their own in the launch's
+// scratch, so one long request does not size every slot.
+static const int SERVE_SLOT_PATTERNS = 512;
+
+static size_t chain_slot(pa_server *s)
 {
-    size_t off[5];
-    return chain::slot_scratch_bytes(s->e->tm.N, s->e->tm.S, Pcap, s->fp64, off) * (size_t)k;
+    size_t off[4];
+    return chain::slot_scratch_bytes(s->e->tm.N, s->e->tm.S, SERVE_SLOT_PATTERNS, s->fp64, off);
 }
-static size_t loglik_launch_scratch(pa_server *s, int Pcap, int k)
+static size_t chain_big(pa_server *s, int ppad)
 {
-    return loglik_slot_bytes(s->e->tm.N, s->e->tm.S, Pcap, s->fp64) * (size_t)k;
+    return ppad > SERVE_SLOT_PATTERNS ? chain::chain_part_bytes(s->e->tm.N, s->e->tm.S, ppad, s->fp64) : 0;
+}
+static size_t loglik_slot(pa_server *s) { return loglik_slot_bytes(s->e->tm.N, s->e->tm.S, SERVE_SLOT_PATTERNS, s->fp64); }
+static size_t loglik_big(pa_server *s, int ppad)
+{
+    return ppad > SERVE_SLOT_PATTERNS ? loglik_slot_bytes(s->e->tm.N, s->e->tm.S, ppad, s->fp64) : 0;
 }
 
 // Admits waiting requests (arrival order) and launches them; no-op when
@@ -1531,17 +1641,19 @@ static pa_status serve_launch(pa_server *s)
     pa_status st = upload(e);
     if (st != PA_OK) return st;
     const int N = e->tm.N, S = e->tm.S;
-    const int gmax_c = e->n_cu * fit_per_cu(e, chain::chain_smem_bytes(N, S, s->BS, s->fp64), s->BS);
-    const int gmax_l = e->n_cu * fit_per_cu(e, loglik_smem_bytes(N, s->fp64), s->BS);
+    const int gmax_c = chain_grid_max(e, s->fp64, s->BS);
+    const int gmax_l = loglik_grid_max(e, s->fp64, s->BS);
     const double now = pa_serve_clock();
-    // chain requests
-    int PcapC = 64;
+    // chain requests: slots for min(k, grid) workgroups plus the long elements' own regions
+    size_t bigC = 0;
     s->batch.clear();
+    std::vector<int64_t> &boC = s->big_off_c;
+    boC.clear();
     while (!s->waiting.empty() && (int)s->batch.size() < s->max_chains) {
         const Request &r = s->waiting.front();
-        const int pc = std::max(PcapC, e->elems[r.eid].Ppad);
+        const size_t b = chain_big(s, e->elems[r.eid].Ppad);
         const int k = (int)s->batch.size() + 1;
-        if (chain_launch_scratch(s, pc, std::min(k, gmax_c)) > s->pool_bytes) {
+        if (chain_slot(s) * (size_t)std::min(k, gmax_c) + bigC + b > s->pool_bytes) {
             if (!s->batch.empty()) break;
             // cannot run even alone
             pa_serve_result out;
@@ -1553,22 +1665,26 @@ static pa_status serve_launch(pa_server *s)
             s->waiting.pop_front();
             continue;
         }
-        PcapC = pc;
+        boC.push_back(b ? (int64_t)bigC : -1);
+        bigC += b;
         s->batch.push_back(r);
         s->batch.back().admit = now;
         s->waiting.pop_front();
     }
     const int nc = (int)s->batch.size();
     const int gc = std::min(nc, gmax_c);
-    const size_t scr_c = nc ? (size_t)((chain_launch_scratch(s, PcapC, gc) + 255) & ~(size_t)255) : 0;
+    const size_t slotsC = nc ? chain_slot(s) * (size_t)gc : 0;
+    const size_t scr_c = nc ? ((slotsC + bigC + 255) & ~(size_t)255) : 0;
     // log-likelihood requests, in the pool after the chains' scratch
-    int PcapL = 64;
+    size_t bigL = 0;
     s->lbatch.clear();
+    std::vector<int64_t> &boL = s->big_off_l;
+    boL.clear();
     while (!s->lwait.empty() && (int)s->lbatch.size() < s->max_chains) {
         const LRequest &r = s->lwait.front();
-        const int pc = std::max(PcapL, e->elems[r.eid].Ppad);
+        const size_t b = loglik_big(s, e->elems[r.eid].Ppad);
         const int k = (int)s->lbatch.size() + 1;
-        if (scr_c + loglik_launch_scratch(s, pc, std::min(k, gmax_l)) > s->pool_bytes) {
+        if (scr_c + loglik_slot(s) * (size_t)std::min(k, gmax_l) + bigL + b > s->pool_bytes) {
             if (!s->lbatch.empty() || nc) break;
             pa_serve_result out;
             memset(&out, 0, sizeof out);
@@ -1579,27 +1695,33 @@ static pa_status serve_launch(pa_server *s)
             s->lwait.pop_front();
             continue;
         }
-        PcapL = pc;
+        boL.push_back(b ? (int64_t)bigL : -1);
+        bigL += b;
         s->lbatch.push_back(std::move(s->lwait.front()));
         s->lbatch.back().admit = now;
         s->lwait.pop_front();
     }
     const int nl = (int)s->lbatch.size();
     const int gl = std::min(nl, gmax_l);
+    const size_t slotsL = nl ? loglik_slot(s) * (size_t)gl : 0;
     if (nc == 0 && nl == 0) return PA_OK;
     HIPCHK(e, hipEventRecord(s->ev0, e->stream));
     if (nc) {
         for (int i = 0; i < nc; i++) s->h_ids[i] = s->batch[i].eid;
+        memcpy(s->h_bo, boC.data(), sizeof(int64_t) * nc);
         HIPCHK(e, hipMemcpyAsync(s->d_ids, s->h_ids, sizeof(int32_t) * nc, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->d_bo, s->h_bo, sizeof(int64_t) * nc, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemsetAsync(s->d_cnt, 0, 16, e->stream));
         chain::ChainArgs A;
         memset(&A, 0, sizeof A);
-        size_t off[5];
-        const size_t slot = chain::slot_scratch_bytes(N, S, PcapC, s->fp64, off);
+        size_t off[4];
+        const size_t slot = chain::slot_scratch_bytes(N, S, SERVE_SLOT_PATTERNS, s->fp64, off);
         A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
         A.work = s->d_ids; A.n_work = nc; A.queue = e->d_queue;
         A.scratch = s->d_pool; A.slot_bytes = (int64_t)slot;
-        A.off_parts = off[0]; A.off_exps = off[1]; A.off_plin = off[2]; A.off_logt = off[3]; A.off_amb = off[4];
+        A.off_plin = off[0]; A.off_logt = off[1]; A.off_amb = off[2]; A.off_parts = off[3];
+        A.big = s->d_pool + slotsC;
+        A.big_off = bigC ? s->d_bo : nullptr;
         A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
         A.counters = s->d_cnt;
         HIPCHK(e, chain::launch_reset(e->d_queue, e->stream));
@@ -1612,13 +1734,18 @@ static pa_status serve_launch(pa_server *s)
             s->h_lids[i] = s->lbatch[i].eid;
             memcpy(s->h_llen + (size_t)i * N, s->lbatch[i].len.data(), sizeof(double) * N);
         }
+        memcpy(s->h_lbo, boL.data(), sizeof(int64_t) * nl);
         HIPCHK(e, hipMemcpyAsync(s->d_lids, s->h_lids, sizeof(int32_t) * nl, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->d_lbo, s->h_lbo, sizeof(int64_t) * nl, hipMemcpyHostToDevice, e->stream));
         HIPCHK(e, hipMemcpyAsync(s->d_llen, s->h_llen, sizeof(double) * (size_t)nl * N, hipMemcpyHostToDevice, e->stream));
         LoglikArgs A;
         memset(&A, 0, sizeof A);
         A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
+        A.n_elems = (int)e->elems.size();
         A.work = s->d_lids; A.n_work = nl; A.queue = e->d_queue;
-        A.scratch = s->d_pool + scr_c; A.slot_bytes = (int64_t)loglik_slot_bytes(N, S, PcapL, s->fp64);
+        A.scratch = s->d_pool + scr_c; A.slot_bytes = (int64_t)loglik_slot(s);
+        A.big = s->d_pool + scr_c + slotsL;
+        A.big_off = bigL ? s->d_lbo : nullptr;
         A.ll_out = s->d_lout; A.len_in = s->d_llen;
         HIPCHK(e, launch_reset(e->d_queue, e->stream));
         HIPCHK(e, launch_loglik(e->M, A, s->fp64, s->BS, gl, e->stream));
@@ -1626,7 +1753,7 @@ static pa_status serve_launch(pa_server *s)
     }
     HIPCHK(e, hipEventRecord(s->ev1, e->stream));
     s->in_flight = true;
-    s->in_use = scr_c + (nl ? loglik_launch_scratch(s, PcapL, gl) : 0);
+    s->in_use = scr_c + (nl ? slotsL + bigL : 0);
     s->st.admitted += nc + nl;
     s->st.launches++;
     return PA_OK;

@@ -24,8 +24,14 @@ struct LoglikArgs {
     const uint8_t *codes;     // arenas
     const int32_t *wts;
     const int32_t *tab;
+    int n_elems;              // elements in the arenas: ids outside [0, n_elems) score NaN
     char *scratch;            // one slot per workgroup
     int64_t slot_bytes;
+    // Elements too long for a slot: work position w keeps its partials at
+    // big + big_off[w] (loglik_slot_bytes of its own Ppad); big_off[w] < 0,
+    // or big_off null, means the workgroup's slot.
+    char *big;
+    const int64_t *big_off;
     const int32_t *Zin;       // [n_work][N] (when len_in is null)
     const double *rates_in;   // [n_work][2] (r1, r2) (when len_in is null)
     const double *len_in;     // [n_work][N] per-call branch lengths, or null
@@ -37,6 +43,10 @@ size_t loglik_smem_bytes(int N, int fp64);
 // Bytes of one workgroup's HBM slot for elements of up to Pcap (padded)
 // patterns: the partials [I][Pcap] (4 values each) and their exponents.
 size_t loglik_slot_bytes(int N, int S, int Pcap, int fp64);
+// Dynamic LDS bytes of one workgroup (the launch's dynamic allocation).
+size_t loglik_dyn_smem_bytes(int N, int fp64);
+// Workgroups of the likelihood kernel resident on one CU (registers, LDS, launch bound).
+int loglik_blocks_per_cu(int fp64, int BS, int N);
 hipError_t launch_loglik(const ModelDev &M, const LoglikArgs &A, int fp64, int BS, int grid, hipStream_t st);
 hipError_t launch_reset(int *q, hipStream_t st);
 

@@ -30,7 +30,7 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths",
            "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
            "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
-           "pa_serve_submit_lengths"]
+           "pa_serve_submit_lengths", "pa_release_scratch"]
 
 
 class PaError(RuntimeError):
@@ -85,7 +85,8 @@ class Traces(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("elapsed_ms", C.c_double), ("algorithmic_bytes", C.c_double), ("grid", C.c_int32),
-                ("block_threads", C.c_int32), ("lds_bytes", C.c_int32), ("n_launches", C.c_int32)]
+                ("block_threads", C.c_int32), ("lds_bytes", C.c_int32), ("n_launches", C.c_int32),
+                ("scratch_bytes", C.c_double)]
 
 
 class RunConfig(C.Structure):
@@ -183,6 +184,8 @@ def lib():
         L.pa_serve_destroy.restype = None
         L.pa_serve_clock.argtypes = []
         L.pa_serve_clock.restype = C.c_double
+        L.pa_release_scratch.argtypes = [C.c_void_p]
+        L.pa_release_scratch.restype = C.c_int
         L.pa_device_count.argtypes = []
         L.pa_device_count.restype = C.c_int32
         for f in ("pa_parse_mod", "pa_engine_create", "pa_elements_add", "pa_element_get", "pa_loglik",
@@ -308,6 +311,10 @@ class Engine:
         self._check(lib().pa_loglik_lengths(self._h, _p(ids, C.c_int32), len(ids), _p(L, C.c_double),
                                             int(precision), _p(out, C.c_double)))
         return out
+
+    def release_scratch(self):
+        """Free the device scratch the likelihood calls keep between calls."""
+        self._check(lib().pa_release_scratch(self._h))
 
     def element(self, eid: int):
         info = ElementInfo()

@@ -234,7 +234,7 @@ def simulate(tree, pi, Q, sizes, rates, gap_frac, rng):
     return np.ascontiguousarray(align)
 
 
-def make_workload(config: int, seed: int = 1, n_elements: int | None = None) -> Workload:
+def make_workload(config: int, seed: int = 1, n_elements: int | None = None, lengths=None) -> Workload:
     """Build the synthetic inputs of one benchmark configuration.
 
     1: S=10, Exp(0.1) branches, 200 elements U[100,500], 2% gap/N, types
@@ -244,6 +244,8 @@ def make_workload(config: int, seed: int = 1, n_elements: int | None = None) -> 
     3: S=43, 50000 elements Pareto(1.5) on [50, 10000]; 1-20% of branches
        accelerated per element, r1 ~ U[0.1,0.5], r2 ~ U[1.5,4]; 2% gaps.
     ``n_elements`` scales the element count down (tests) keeping the laws.
+    ``lengths`` replaces the length draws by the given element lengths (tests
+    that need chosen lengths under a configuration's other laws).
     """
     rng = np.random.default_rng(seed)
     if config == 1:
@@ -256,9 +258,13 @@ def make_workload(config: int, seed: int = 1, n_elements: int | None = None) -> 
         raise ValueError(config)
     if n_elements is not None:
         C = int(n_elements)
+    if lengths is not None:
+        C = len(lengths)
     tree = random_tree(S, mbl, rng)
     pi, Q = gtr(rng)
     sizes = _lengths(lk, C, rng)
+    if lengths is not None:
+        sizes = np.asarray(lengths, np.int64).copy()
     N = tree.N
     # groups: the smaller root clade is the outgroup; targets are tips of the other side
     r = N - 1
@@ -290,6 +296,14 @@ def make_workload(config: int, seed: int = 1, n_elements: int | None = None) -> 
     return Workload(f"config{config}", tree, pi, Q, mod_text(pi, Q, tree), align,
                     sizes.astype(np.int64), starts, [int(x) for x in outgroup],
                     [int(x) for x in conserve], target, rates, seed)
+
+
+def pareto_quantiles(k: int, alpha: float = 1.5, lo: float = 50.0, hi: float = 10000.0):
+    """k lengths at the mid-quantiles (i + 0.5) / k of the truncated Pareto
+    law of the 50,000-element configuration (inverse of its CDF)."""
+    u = (np.arange(k) + 0.5) / k
+    x = lo / (1.0 - u * (1.0 - (lo / hi) ** alpha)) ** (1.0 / alpha)
+    return np.clip(np.floor(x), 50, 10000).astype(np.int64)
 
 
 def small_workload(S=6, C=4, length=40, seed=3, gap=0.05, mean_bl=0.15):

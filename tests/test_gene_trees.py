@@ -139,3 +139,49 @@ def test_gene_tree_loglik_gpu_vs_oracle(require_gpu):
     np.testing.assert_allclose(got64, ref, rtol=1e-9)
     got32 = eng.loglik_lengths(ids, L, E.PA_FP32)
     np.testing.assert_allclose(got32, ref, rtol=1e-6)
+
+
+@pytest.mark.gpu
+def test_gene_tree_loglik_gpu_vs_oracle_120_species(require_gpu):
+    """The gene-tree configuration's own shape: 120-leaf gene trees (239
+    nodes, 119 internal), each element with its own multispecies-coalescent
+    topology."""
+    w = synth.make_gt_workload(seed=8, n_elements=16, S=120)
+    opts = E.default_options(min_len=1)
+    eng = E.Engine.from_workload(w, opts)
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    assert (st == E.PA_ELEM_OK).all()
+    assert len({tuple(map(tuple, g.children)) for g in w.genes}) == len(w.genes)   # shapes differ
+    m = O.parse_mod(w.mod_text)[0]
+    L = w.lengths()
+    ref = np.array([O.gt_loglik(m, w.tree.S, g.children, g.seg_gene, g.seg_species, g.seg_len * w.scale, w.Z[c],
+                                w.r1[c], w.r2[c], w.align, w.starts[c], w.sizes[c], gap_prop=opts.gap_prop)
+                    for c, g in enumerate(w.genes)])
+    got64 = eng.loglik_lengths(ids, L, E.PA_FP64)
+    got32 = eng.loglik_lengths(ids, L, E.PA_FP32)
+    print("S=120 gene trees rel err: fp64 %.2e fp32 %.2e"
+          % (np.max(np.abs(got64 - ref) / np.abs(ref)), np.max(np.abs(got32 - ref) / np.abs(ref))))
+    np.testing.assert_allclose(got64, ref, rtol=1e-9)
+    np.testing.assert_allclose(got32, ref, rtol=1e-6)
+
+
+@pytest.mark.gpu
+def test_gene_tree_scratch_regrows(require_gpu):
+    """One engine: a small call, then more and longer elements (the kept
+    scratch and length buffers grow), then the small call again; each against
+    the oracle.  Releasing the kept scratch leaves the next call unchanged."""
+    w = synth.make_gt_workload(seed=9, n_elements=24, S=24)
+    opts = E.default_options(min_len=1)
+    eng = E.Engine.from_workload(w, opts)
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    m = O.parse_mod(w.mod_text)[0]
+    L = w.lengths()
+    ref = np.array([O.gt_loglik(m, w.tree.S, g.children, g.seg_gene, g.seg_species, g.seg_len * w.scale, w.Z[c],
+                                w.r1[c], w.r2[c], w.align, w.starts[c], w.sizes[c], gap_prop=opts.gap_prop)
+                    for c, g in enumerate(w.genes)])
+    order = np.argsort(w.sizes)
+    small, big = order[:3], order
+    for sel, prec, tol in ((small, E.PA_FP32, 1e-6), (big, E.PA_FP64, 1e-9), (small, E.PA_FP32, 1e-6)):
+        np.testing.assert_allclose(eng.loglik_lengths(ids[sel], L[sel], prec), ref[sel], rtol=tol)
+    eng.release_scratch()
+    np.testing.assert_allclose(eng.loglik_lengths(ids[big], L[big], E.PA_FP32), ref[big], rtol=1e-6)

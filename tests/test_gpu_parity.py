@@ -28,9 +28,18 @@ def _random_Z(eng, ids, rng):
     return Z
 
 
+def _setup_amb(config, n, seed, **kw):
+    """As _setup, with IUPAC two-base codes and N sprinkled into the alignment."""
+    w = _with_ambiguity(synth.make_workload(config, seed=seed, n_elements=n), seed + 100)
+    eng = E.Engine.from_workload(w, E.default_options(**kw))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    return w, eng, ids, st, O.model_from_workload(w), O.default_options(**kw)
+
+
 @pytest.mark.parametrize("config", [1, 2])
 def test_loglik_fp64_and_fp32_vs_oracle(require_gpu, config):
-    w, eng, ids, st, m, oo = _setup(config, n=8, seed=5)
+    w, eng, ids, st, m, oo = _setup_amb(config, n=8, seed=5)
+    assert any(eng.element(int(i))["codes"].max() >= 6 for i in ids)   # ambiguity codes reach the kernel
     rng = np.random.default_rng(0)
     Z = _random_Z(eng, ids, rng)
     rates = np.stack([rng.uniform(0.1, 0.9, len(ids)), rng.uniform(1.2, 4.0, len(ids))], 1)
@@ -81,7 +90,7 @@ def _with_missing(config, n, seed):
 
 @pytest.mark.parametrize("config", [1, 2])
 def test_missing_species_loglik_and_chain(require_gpu, config):
-    w = _with_missing(config, 6, 31)
+    w = _with_ambiguity(_with_missing(config, 6, 31), 32)
     kw = dict(burnin=15, mcmc=45, adapt_freq=20)
     eng = E.Engine.from_workload(w, E.default_options(**kw))
     ids, st = eng.add_elements(w.align, w.starts, w.sizes)
@@ -163,3 +172,21 @@ def test_chain_fp64_option_variants(require_gpu, variant):
             np.testing.assert_allclose(res["nrate"][k, ms], r[ms]["nrate"], rtol=1e-9)
             np.testing.assert_allclose(res["crate"][k, ms], r[ms]["crate"], rtol=1e-9)
             np.testing.assert_allclose(res["l2rate"][k, ms], r[ms]["l2rate"], rtol=1e-9)
+
+
+def test_loglik_scratch_regrows(require_gpu):
+    """One engine: a small FP32 call, then an FP64 call over more and longer
+    elements (the kept scratch grows, long elements get regions of their
+    own), then the small call again; each against the oracle."""
+    w, eng, ids, st, m, oo = _setup(2, n=24, seed=19)
+    ids = ids[st == E.PA_ELEM_OK]
+    idx = np.flatnonzero(st == E.PA_ELEM_OK)
+    rng = np.random.default_rng(2)
+    Z = _random_Z(eng, ids, rng)
+    rates = np.stack([rng.uniform(0.1, 0.9, len(ids)), rng.uniform(1.2, 4.0, len(ids))], 1)
+    ref = np.array([O.Element(m, oo, w.align, w.starts[i], w.sizes[i]).loglik(Z[k], rates[k, 0], rates[k, 1])
+                    for k, i in enumerate(idx)])
+    order = np.argsort([eng.element(int(i))["P"] for i in ids])
+    small = order[:3]
+    for sel, prec, tol in ((small, E.PA_FP32, 1e-6), (order, E.PA_FP64, 1e-9), (small, E.PA_FP32, 1e-6)):
+        np.testing.assert_allclose(eng.loglik(ids[sel], Z[sel], rates[sel], prec), ref[sel], rtol=tol)

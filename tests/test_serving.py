@@ -4,6 +4,8 @@ in a mix of other requests arriving while it waits."""
 import numpy as np
 import pytest
 
+from chain_checks import oracle_chains
+from oracle import oracle_py as O
 from phyloacc_amd import engine as E
 from phyloacc_amd import synth
 
@@ -174,4 +176,37 @@ def test_gene_tree_requests_through_the_loop(require_gpu):
         # reorders the FP64 sum over patterns
         np.testing.assert_allclose(got[t].loglik, ref[k], rtol=1e-12)
     assert all(got[t].kind == E.PA_SERVE_CHAINS for t in tc)
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_served_fp64_chains_match_oracle(require_gpu):
+    """FP64 chains served while other requests are in flight (requests arrive
+    after a launch has started) reproduce the oracle's chains: maximum
+    log-likelihood, its state, the final rates and the adaptive-proposal
+    counters (src/PhyloAcc-ST/bpp_c.cpp:470-485)."""
+    kw = dict(burnin=10, mcmc=30, adapt_freq=12)
+    w = synth.make_workload(2, seed=21, n_elements=10)
+    eng = E.Engine.from_workload(w, E.default_options(**kw))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    ok = st == E.PA_ELEM_OK
+    ids, idx = ids[ok], np.flatnonzero(ok)
+    refs = oracle_chains(O.model_from_workload(w), O.default_options(**kw), w, idx)
+    srv = E.Server(eng, precision=E.PA_FP64, max_chains=4)
+    tick = list(srv.submit(ids[:4]))
+    srv.step(wait=False)                       # a launch in flight
+    tick += list(srv.submit(ids[4:]))          # arrivals while it runs
+    got = _drain(srv, len(tick))
+    for k, t in enumerate(tick):
+        r = got[t]
+        assert r.element == ids[k] and r.status == E.PA_ELEM_OK
+        for ms in range(3):
+            o = refs[k][ms]
+            c = r.result[ms]
+            assert abs(c.max_loglik - o["max_loglik"]) <= 1e-9 * abs(o["max_loglik"])
+            assert c.max_m == o["max_m"]
+            for f, tr in (("final_nrate", "nrate"), ("final_crate", "crate"), ("final_grate", "grate"),
+                          ("final_lrate", "lrate"), ("final_lrate2", "l2rate")):
+                assert abs(getattr(c, f) - o[tr][-1]) <= 1e-9 * abs(o[tr][-1]), (k, ms, f)
+            assert (c.accept_n, c.accept_c) == (o["accept_n"], o["accept_c"])
     srv.close()
