@@ -805,19 +805,17 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
     __syncthreads();
     const int n_up = c->n_up, n_down = c->n_down;
     for (int h = 0; h < n_up; h++) {
+        // one lane per (node, parent class): the classes' messages are independent
         const int a = h == 0 ? 0 : c->upoff[h - 1], b = c->upoff[h];
-        for (int q = a + tid; q < b; q += BS) {
-            const int p = c->up[q];
-            double b0 = c->beta[p * 3], b1 = c->beta[p * 3 + 1], b2 = c->beta[p * 3 + 2];
+        for (int q = 3 * a + tid; q < 3 * b; q += BS) {
+            const int p = c->up[q / 3], j = q - 3 * (q / 3);
+            double bj = c->beta[p * 3 + j];
             for (int w = 0; w < 2; w++) {
                 const int x = w ? c->ch1[p] : c->ch0[p];
                 const LDS double *T = c->fixZ[x] ? c->TMf : c->TMn;
-                const double y0 = c->beta[x * 3], y1 = c->beta[x * 3 + 1], y2 = c->beta[x * 3 + 2];
-                b0 += log_col(T, 0, y0, y1, y2);
-                b1 += log_col(T, 1, y0, y1, y2);
-                b2 += log_col(T, 2, y0, y1, y2);
+                bj += log_col(T, j, c->beta[x * 3], c->beta[x * 3 + 1], c->beta[x * 3 + 2]);
             }
-            c->beta[p * 3] = b0; c->beta[p * 3 + 1] = b1; c->beta[p * 3 + 2] = b2;
+            c->beta[p * 3 + j] = bj;
         }
         __syncthreads();
     }
