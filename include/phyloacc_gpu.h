@@ -154,6 +154,22 @@ pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const d
  * pa_serve_create when device memory is tight. */
 pa_status pa_release_scratch(pa_engine *e);
 
+/* pa_loglik without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  ids [n], Z [n*N], rates
+ * [n*2] and out [n] are DEVICE pointers, in the caller's order; the kernel
+ * scores an id outside the engine's elements, or a filtered one, as NaN.
+ * Scratch is the library's static device array of 256 MiB per GPU, every
+ * workgroup slot sized for the engine's longest element: an engine whose
+ * longest element needs more than the array holds is rejected with
+ * PA_E_INPUT.  The arenas must already be on the device (a pa_loglik or
+ * pa_run_chains call after the last pa_elements_add), else PA_E_STATE.  The
+ * launch is asynchronous on `stream` (a hipStream_t; NULL: the engine's
+ * stream): synchronise it before reading out.  All calls on one GPU share the
+ * array, so calls that overlap in time on one GPU race. */
+pa_status pa_loglik_noalloc(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *Z, const double *rates,
+                            int32_t precision, double *out, void *stream);
+
 /* Results of one model chain of one element (BPP_C::Gibbs bookkeeping,
  * src/PhyloAcc-ST/bpp_c.cpp:470-485). */
 typedef struct {

@@ -29,7 +29,33 @@
 #define LDS __attribute__((address_space(3)))
 #define GBL __attribute__((address_space(1)))
 
+#include <mutex>
+
 namespace pa {
+
+// Scratch of the allocation-free likelihood entry point: the work counter in
+// its first 256 bytes, then one slot per workgroup.  Every GPU the library
+// runs on holds its own copy until the process ends.  Host code must never
+// use the array's name as an address (on the host it is a stand-in symbol);
+// it uses the device address from noalloc_scratch().
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(int device, char **ptr)
+{
+    static std::mutex mu;
+    static char *addr[64] = {nullptr};
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        // the caller has made `device` current
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return err;
+        addr[device] = (char *)p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
 
 template <typename R>
 struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };

@@ -522,6 +522,11 @@ struct pa_engine {
     void *d_ll_scratch = nullptr, *d_ll_len = nullptr;
     size_t ll_scratch_bytes = 0, ll_len_bytes = 0;
     size_t ll_last_plan_bytes = 0;           // scratch the last likelihood launch used
+    // the allocation-free likelihood entry point: this device's copy of its
+    // scratch array, and the arenas' longest element and mean pattern count
+    char *d_noalloc = nullptr;
+    int max_ppad = 0;
+    double mean_p = 0;
 };
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
@@ -644,6 +649,7 @@ static pa_status ensure_device(pa_engine *e)
     e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
     HIPCHK(e, hipStreamCreate(&e->stream));
     HIPCHK(e, hipMalloc(&e->d_queue, 256));
+    HIPCHK(e, noalloc_scratch(e->device, &e->d_noalloc));
     return PA_OK;
 }
 
@@ -816,6 +822,12 @@ static pa_status upload(pa_engine *e)
     HIPCHK(e, hipMemcpy(e->d_w, w.data(), w.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(e->d_site, site.data(), site.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(e->d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    e->max_ppad = 0;
+    e->mean_p = 0;
+    size_t kept = 0;
+    for (const HostElem &E : e->elems)
+        if (!E.filtered) { e->max_ppad = std::max(e->max_ppad, E.Ppad); e->mean_p += E.P; kept++; }
+    if (kept) e->mean_p /= (double)kept;
     e->dev_dirty = false;
     e->arena_gen++;
     return PA_OK;
@@ -1132,6 +1144,51 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) out[order[i]] = ll[i];
     if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
+    return PA_OK;
+}
+
+// pa_loglik over device-resident inputs and outputs that allocates nothing:
+// the work counter and the workgroup slots live in the library's static
+// device scratch array (NOALLOC_SCRATCH_BYTES per GPU), every slot sized for
+// the longest element in the arenas, and the launch is asynchronous on the
+// caller's stream.  Calls that overlap on one GPU share that array and race.
+extern "C" pa_status pa_loglik_noalloc(pa_engine *e, const int32_t *d_ids, int32_t n, const int32_t *d_Z,
+                                       const double *d_rates, int32_t precision, double *d_out, void *stream)
+{
+    if (!e) return PA_E_INPUT;
+    if (n < 0 || (n > 0 && (!d_ids || !d_Z || !d_rates || !d_out))) return fail(e, PA_E_INPUT, "null argument");
+    if (precision != PA_FP64 && precision != PA_FP32) return fail(e, PA_E_INPUT, "precision must be PA_FP64 or PA_FP32");
+    if (!e->stream || e->dev_dirty || !e->d_noalloc)
+        return fail(e, PA_E_STATE, "the element arenas are not on the device: pa_loglik_noalloc uploads nothing, so "
+                                   "make one device call (pa_loglik, pa_run_chains) after the last pa_elements_add");
+    if (n == 0) return PA_OK;
+    HIPCHK(e, hipSetDevice(e->device));
+    const int S = e->tm.S, N = e->tm.N, fp64 = precision == PA_FP64;
+    const size_t smem = loglik_smem_bytes(N, fp64);
+    if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+    const int BS = (e->mean_p <= 96) ? 64 : (e->mean_p <= 320 ? 128 : 256);
+    const size_t slot = loglik_slot_bytes(N, S, std::max(64, e->max_ppad), fp64);
+    const size_t usable = NOALLOC_SCRATCH_BYTES - 256;
+    if (slot > usable)
+        return fail(e, PA_E_INPUT, "the longest element (" + std::to_string(e->max_ppad) + " padded patterns) needs " +
+                                       std::to_string(slot) + " B of scratch, over the " + std::to_string(usable) +
+                                       " B of the allocation-free entry point");
+    const int grid = (int)std::min<size_t>({(size_t)n, (size_t)loglik_grid_max(e, fp64, BS), usable / slot});
+    LoglikArgs A;
+    memset(&A, 0, sizeof A);
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
+    A.n_elems = (int)e->elems.size();
+    A.work = d_ids;
+    A.n_work = n;
+    A.queue = (int *)e->d_noalloc;
+    A.scratch = e->d_noalloc + 256;
+    A.slot_bytes = (int64_t)slot;
+    A.Zin = d_Z;
+    A.rates_in = d_rates;
+    A.ll_out = d_out;
+    hipStream_t st = stream ? (hipStream_t)stream : e->stream;
+    HIPCHK(e, launch_reset(A.queue, st));          // a fill kernel, not a memset: callers may capture the stream
+    HIPCHK(e, launch_loglik(e->M, A, fp64, BS, grid, st));
     return PA_OK;
 }
 

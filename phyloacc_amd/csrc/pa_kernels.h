@@ -48,6 +48,12 @@ size_t loglik_dyn_smem_bytes(int N, int fp64);
 // Workgroups of the likelihood kernel resident on one CU (registers, LDS, launch bound).
 int loglik_blocks_per_cu(int fp64, int BS, int N);
 hipError_t launch_loglik(const ModelDev &M, const LoglikArgs &A, int fp64, int BS, int grid, hipStream_t st);
+
+// Size of the device scratch array of the allocation-free likelihood entry
+// point, and the device address of the current device's copy (looked up once
+// per device with that device current).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+hipError_t noalloc_scratch(int device, char **ptr);
 hipError_t launch_reset(int *q, hipStream_t st);
 
 }  // namespace pa
