@@ -34,29 +34,10 @@
 #include "pa_common.h"
 #include "pa_rng.h"
 #include "pa_chain.h"
-
-#define LDS __attribute__((address_space(3)))
-#define GBL __attribute__((address_space(1)))
+#include "pa_device.h"
 
 namespace pa {
 namespace chain {
-
-template <typename R>
-struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };
-
-// explicit member-wise loads/stores through address-space pointers
-template <typename R, typename P>
-__device__ __forceinline__ V4<R> ld4(const P *p)
-{
-    V4<R> v;
-    v.x = p->x; v.y = p->y; v.z = p->z; v.w = p->w;
-    return v;
-}
-template <typename R, typename P>
-__device__ __forceinline__ void st4(P *p, R a, R b, R c, R d)
-{
-    p->x = a; p->y = b; p->z = c; p->w = d;
-}
 
 template <typename R>
 struct Ctx {
@@ -192,22 +173,6 @@ size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off)
 
 // ------------------------------------------------------------------ helpers
 
-template <typename R>
-__device__ __forceinline__ double block_sum(double v, CP<R> c)
-{
-    // deterministic: fixed xor tree inside each wave, waves summed in order
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) c->red[w] = v;
-    __syncthreads();
-    double t = 0;
-    for (int i = 0; i < nw; i++) t += c->red[i];
-    __syncthreads();
-    return t;
-}
-
 __device__ __forceinline__ void lds_add(LDS uint32_t *p, uint32_t v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -268,126 +233,7 @@ __device__ __noinline__ void refresh_PTc(CP<R> c, int only_cls)
 
 // ----------------------------------------------------------------- pruning
 
-// Values every lane of the workgroup holds alike (node ids, flags, offsets
-// read from LDS) are moved to scalar registers, which frees vector registers
-// and makes the address arithmetic scalar.
-__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
-// Global pointers are 64-bit, LDS pointers 32-bit.
-template <typename T>
-__device__ __forceinline__ T *uni_ptr(T *p)
-{
-    const uintptr_t v = (uintptr_t)p;
-    if constexpr (sizeof(T *) == 4) {
-        return (T *)(uintptr_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    } else {
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
-        return (T *)(((uint64_t)hi << 32) | lo);
-    }
-}
-
 enum { PR_FULL = 0, PR_DIRTY = 1, PR_PROP = 2 };
-
-// Node flag set when an element is bound (above the element table's flags):
-// the node is a cherry whose partial is recomputed where it is read.
-constexpr uint8_t NF_CHERRY = 0x80;
-
-template <typename R>
-__device__ __forceinline__ void leaf_mul(const LDS R *PT, int code, R &a0, R &a1, R &a2, R &a3)
-{
-    if (code < 4) {
-        const V4<R> row = ld4<R>((const LDS V4<R> *)(PT + code * 4));
-        a0 *= row.x; a1 *= row.y; a2 *= row.z; a3 *= row.w;
-    } else {
-        // ambiguity codes r y k m s w -> base pairs (a,g) (c,t) (g,t) (a,c) (c,g) (a,t)
-        const int b1 = (code == 6 || code == 9 || code == 11) ? 0 : (code == 7 || code == 10) ? 1 : 2;
-        const int b2 = (code == 6 || code == 10) ? 2 : (code == 9) ? 1 : 3;
-        const V4<R> r1 = ld4<R>((const LDS V4<R> *)(PT + b1 * 4));
-        const V4<R> r2 = ld4<R>((const LDS V4<R> *)(PT + b2 * 4));
-        a0 *= r1.x + r2.x; a1 *= r1.y + r2.y; a2 *= r1.z + r2.z; a3 *= r1.w + r2.w;
-    }
-}
-
-// One child's inputs at one pattern: a leaf's code, or an internal child's
-// scaled partial and cumulative exponent.
-template <typename R>
-struct ChildIn {
-    V4<R> v;
-    int x;
-};
-
-template <typename R>
-__device__ __forceinline__ ChildIn<R> child_load(bool leaf, bool miss, const GBL uint8_t *code_row,
-                                                 const GBL V4<R> *part_row, const GBL int32_t *exp_row, int p)
-{
-    // a missing child contributes nothing, but its gap state still decides
-    // whether the parent is all-gap at this pattern, so its code / exponent
-    // is read either way
-    ChildIn<R> in;
-    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
-    if (leaf) {
-        in.x = code_row[p];
-    } else {
-        in.x = exp_row[p];
-        if (!miss) in.v = ld4<R>(part_row + p);
-    }
-    return in;
-}
-
-// The scaled partial and exponent of cherry x at one pattern, from its leaf
-// codes cA, cB and their branch matrices: the operations the pass below
-// would apply to node x, in the same order, so the values are those it
-// would have stored.
-template <typename R>
-__device__ __forceinline__ ChildIn<R> cherry_in(const LDS R *srcA, const LDS R *srcB, bool missA, bool missB,
-                                                bool miss_x, int cA, int cB)
-{
-    ChildIn<R> in;
-    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
-    R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
-    const bool gA = (cA == PA_GAP_CODE || cA == PA_OTHER_CODE);
-    const bool gB = (cB == PA_GAP_CODE || cB == PA_OTHER_CODE);
-    if (!missA && !gA) leaf_mul<R>(srcA, cA, a0, a1, a2, a3);
-    if (!missB && !gB) leaf_mul<R>(srcB, cB, a0, a1, a2, a3);
-    if (gA && gB) {
-        in.x = GAPT;
-    } else if (miss_x) {
-        in.x = 0;
-    } else {
-        const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
-        int kx = 0;
-        frexp(mx, &kx);
-        in.v.x = ldexp(a0, -kx); in.v.y = ldexp(a1, -kx); in.v.z = ldexp(a2, -kx); in.v.w = ldexp(a3, -kx);
-        in.x = kx;
-    }
-    return in;
-}
-
-// Multiplies one child's message P * L_child into (a0..a3).  A leaf reads the
-// row of its base from the LDS matrix; an internal child reads the whole
-// matrix as four 4-vectors.  gap reports the child's GAPT state (Tg >= num_base).
-template <typename R>
-__device__ __forceinline__ void child_apply(bool leaf, bool miss, const LDS R *src, const ChildIn<R> &in,
-                                            R &a0, R &a1, R &a2, R &a3, int &e, bool &gap)
-{
-    if (leaf) {
-        gap = (in.x == PA_GAP_CODE || in.x == PA_OTHER_CODE);
-        if (!miss && !gap) leaf_mul<R>(src, in.x, a0, a1, a2, a3);
-    } else {
-        gap = (in.x == GAPT);
-        if (!miss && !gap) {
-            const V4<R> v = in.v;
-            const V4<R> r0 = ld4<R>((const LDS V4<R> *)src), r1 = ld4<R>((const LDS V4<R> *)(src + 4));
-            const V4<R> r2 = ld4<R>((const LDS V4<R> *)(src + 8)), r3 = ld4<R>((const LDS V4<R> *)(src + 12));
-            const R o0 = r0.x * v.x + r1.x * v.y + r2.x * v.z + r3.x * v.w;
-            const R o1 = r0.y * v.x + r1.y * v.y + r2.y * v.z + r3.y * v.w;
-            const R o2 = r0.z * v.x + r1.z * v.y + r2.z * v.z + r3.z * v.w;
-            const R o3 = r0.w * v.x + r1.w * v.y + r2.w * v.z + r3.w * v.w;
-            a0 *= o0; a1 *= o1; a2 *= o2; a3 *= o3;
-            e += in.x;
-        }
-    }
-}
 
 // Where one child of a node comes from in a pass: a leaf's code row, an
 // internal child's partial / exponent rows (offsets into the slot arrays), or
@@ -598,7 +444,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
             acc += (double)c->wts[p] * (log(L) + (double)ex * 0.69314718055994530942);
         }
     }
-    return block_sum<R>(acc, c);
+    return block_sum(acc, c->red);
 }
 
 // ------------------------------------------------------ ancestral sampling
