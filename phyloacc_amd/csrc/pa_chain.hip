@@ -53,8 +53,11 @@ struct Ctx {
     GBL uint32_t *amb;     // [S][24]
     const GBL double *dist2;   // [N] branch lengths (element node table)
     // LDS arrays
-    LDS int16_t *ch0, *ch1, *par, *nodes, *inodes, *up, *upoff, *down, *downoff, *tz;
+    LDS int16_t *par, *nodes, *inodes, *up, *upoff, *down, *downoff, *tz;
+    LDS uint32_t *desc;    // [I][4] static pruning descriptor of each internal node (node_desc)
+    LDS int32_t *plist;    // [I] nodes a pass visits, level order: s | level << 16
     LDS uint8_t *flags, *fixZ, *dirty, *cur, *changed;
+    LDS uint8_t *pb;       // [N] per-pass buffer / matrix bits (pass_bits); shares storage with changed
     LDS int8_t *Z, *maxZ;
     LDS double *E, *beta;
     LDS R *PTc, *PTp;      // [N][16], PT[i*4+j] = P_{j->i}
@@ -65,7 +68,7 @@ struct Ctx {
     double TMf[9], TMn[9];  // log_TM_Int for fixZ / free nodes, [i*3+j] = log P(child i | parent j)
     double red[16];
     unsigned long long bytes;   // algorithmic HBM bytes of this element's passes (see prune / sample_sites)
-    int accept_n, accept_c, max_m, skip, accept, status, work, nch;
+    int accept_n, accept_c, max_m, skip, accept, status, work, nch, n_vis;
 };
 
 template <typename R>
@@ -93,12 +96,13 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
     PA_TAKE(PTc, R, 16 * N);
     // Phases that never overlap share storage:
     //   A: per-thread ancestral states (sample_sites) | emissions and Z
-    //      messages (emissions, update_Z);
+    //      messages (emissions, update_Z) | Z-transition counts
+    //      (sample_transition);
     //   B: per-branch transition counters (sample_sites, emissions) |
     //      proposal matrices (rate updates).  sample_sites clears the
     //      counters before it counts.
     {
-        const size_t szA = std::max((size_t)(I > 0 ? I : 1) * BS, (size_t)6 * N * sizeof(double));
+        const size_t szA = std::max({(size_t)(I > 0 ? I : 1) * BS, (size_t)6 * N * sizeof(double), (size_t)20 * 4});
         const size_t szB = std::max((size_t)16 * N * sizeof(uint32_t), (size_t)16 * N * sizeof(R));
         o = al(o, 32);
         if constexpr (BIND) {
@@ -110,12 +114,14 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
             c->E = (LDS double *)(base + o);
             c->beta = (LDS double *)(base + o) + 3 * N;
             c->tgs = (LDS uint8_t *)(base + o);
+            c->cnt9 = (LDS uint32_t *)(base + o);
         }
         o += szA;
     }
-    PA_TAKE(cnt9, uint32_t, 20);
-    PA_TAKE(ch0, int16_t, N);
-    PA_TAKE(ch1, int16_t, N);
+    o = al(o, 16);      // descriptors are read as 16-byte vectors
+    if constexpr (BIND) c->desc = (LDS uint32_t *)(base + o);
+    o += (size_t)16 * (I > 0 ? I : 1);
+    PA_TAKE(plist, int32_t, I > 0 ? I : 1);
     PA_TAKE(par, int16_t, N);
     PA_TAKE(nodes, int16_t, N);
     PA_TAKE(inodes, int16_t, N);
@@ -129,6 +135,7 @@ __host__ __device__ size_t smem_arrays(int N, int S, int BS, LDS char *base, CP<
     PA_TAKE(dirty, uint8_t, N);
     PA_TAKE(cur, uint8_t, N);
     PA_TAKE(changed, uint8_t, N);
+    if constexpr (BIND) c->pb = c->changed;   // changed lives inside update_Z, pb inside prune
     PA_TAKE(Z, int8_t, N);
     PA_TAKE(maxZ, int8_t, N);
 #undef PA_TAKE
@@ -255,37 +262,40 @@ struct NodeDesc {
     KidDesc<R> k[2];
 };
 
-template <typename R>
-__device__ __forceinline__ const LDS R *pass_matrix(CP<R> c, const LDS R *PTc, const LDS R *PTp, int mode, int cls, int y)
-{
-    return uni(mode == PR_PROP && c->Z[y] == cls) ? PTp + 16 * y : PTc + 16 * y;
-}
+// Static pruning descriptor of an internal node, built when the element is
+// bound (node_desc): word 0 = its children x0 | x1 << 16; words 1 and 2 = the
+// two leaves of x0 (resp. x1) when that child is a cherry, yA | yB << 16;
+// word 3 = DB_* bits: each child's kind (KD_LEAF / KD_STORED / KD_CHERRY),
+// which children and cherry leaves are missing, and whether the node is.
+enum { DB_KIND1_SHIFT = 2, DB_MISS0 = 1 << 4, DB_MISS1 = 1 << 5, DB_MISS_A0 = 1 << 6, DB_MISS_B0 = 1 << 7,
+       DB_MISS_A1 = 1 << 8, DB_MISS_B1 = 1 << 9, DB_MISS = 1 << 10 };
+
+// Per-pass bits of node y (pass_bits): PB_BUF = the buffer holding y's
+// partial in this pass (read as a child, written as a visited node);
+// PB_PROP = y's branch takes the proposal matrix.
+enum { PB_BUF = 1, PB_PROP = 2 };
 
 template <typename R>
-__device__ __forceinline__ KidDesc<R> kid_desc(CP<R> c, const LDS R *PTc, const LDS R *PTp, int mode, int cls,
-                                               int x, int S, int Ppad, int bufI)
+__device__ __forceinline__ KidDesc<R> kid_from(int kind, int miss, int missA, int missB, int x, int px, int yA, int pA,
+                                               int yB, int pB, const LDS R *PTc, const LDS R *PTp, int S, int Ppad,
+                                               int bufI)
 {
     KidDesc<R> d;
-    const int f = uni(c->flags[x]);
-    d.src = pass_matrix(c, PTc, PTp, mode, cls, x);
+    d.src = ((px & PB_PROP) ? PTp : PTc) + 16 * x;
     d.offB = 0;
     d.sA = d.sB = PTc;
-    const int miss = (f & NF_MISSING) ? KD_MISS : 0;
-    if (x < S) {
-        d.bits = KD_LEAF | miss;
+    if (kind == KD_LEAF) {
+        d.bits = KD_LEAF | (miss ? KD_MISS : 0);
         d.off = x * Ppad;
-    } else if (f & NF_CHERRY) {
-        const int yA = uni(c->ch0[x]), yB = uni(c->ch1[x]);
-        d.bits = KD_CHERRY | miss | ((uni(c->flags[yA]) & NF_MISSING) ? KD_MISS_A : 0) |
-                 ((uni(c->flags[yB]) & NF_MISSING) ? KD_MISS_B : 0);
+    } else if (kind == KD_CHERRY) {
+        d.bits = KD_CHERRY | (miss ? KD_MISS : 0) | (missA ? KD_MISS_A : 0) | (missB ? KD_MISS_B : 0);
         d.off = yA * Ppad;
         d.offB = yB * Ppad;
-        d.sA = pass_matrix(c, PTc, PTp, mode, cls, yA);
-        d.sB = pass_matrix(c, PTc, PTp, mode, cls, yB);
+        d.sA = ((pA & PB_PROP) ? PTp : PTc) + 16 * yA;
+        d.sB = ((pB & PB_PROP) ? PTp : PTc) + 16 * yB;
     } else {
-        const int rb = uni((mode == PR_PROP && c->dirty[x]) ? 1 - c->cur[x] : c->cur[x]);
-        d.bits = KD_STORED | miss;
-        d.off = rb * bufI + (x - S) * Ppad;
+        d.bits = KD_STORED | (miss ? KD_MISS : 0);
+        d.off = (px & PB_BUF) * bufI + (x - S) * Ppad;
     }
     return d;
 }
@@ -333,7 +343,11 @@ struct Group { static constexpr int U = sizeof(R) == 4 ? 2 : 1; };
 // of all their children before computing any of them.  A pass then costs
 // about one memory round trip per round instead of one per node.  A thread
 // owns the patterns p = tid (mod BS) of every node, so it only reads partials
-// it wrote itself and the pass needs no barrier.
+// it wrote itself and the pass needs no barrier between rounds.  The pass
+// first lists the nodes it visits (one ballot per 64 nodes of a level) and
+// every node's buffer and matrix bits, so a round's scalar set-up is a few
+// independent LDS reads: its list entries, its nodes' static descriptors
+// (node_desc) and their children's bits.
 template <typename R, int BS>
 __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
 {
@@ -346,35 +360,70 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
     const LDS R *PTc = uni_ptr(c->PTc), *PTp = uni_ptr(c->PTp);
     constexpr int U = Group<R>::U;
     unsigned long long bytes = 0;
-    for (int h = 0; h < n_up; h++) {
-        int q = uni(h == 0 ? 0 : c->upoff[h - 1]);
-        const int qe = uni(c->upoff[h]);
-        while (q < qe) {
-            // the next (up to) U nodes of this level that the pass visits
-            int sel[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                sel[u] = -1;
-                while (q < qe) {
-                    const int s = uni(c->up[q]);
-                    q++;
-                    const int fs = uni(c->flags[s]);
-                    if (fs & NF_CHERRY) continue;
-                    if (mode != PR_FULL && (!uni(c->dirty[s]) || (fs & NF_MISSING))) continue;
-                    sel[u] = s;
-                    break;
+    // pass set-up: every node's buffer and matrix bits, and the list of
+    // nodes the pass visits in level order (one wave compacts it by ballot)
+    {
+        const bool prop = mode == PR_PROP;
+        for (int q = tid; q < c->n_nodes; q += BS) {
+            const int y = c->nodes[q];
+            const int cu = c->cur[y];
+            c->pb[y] = (uint8_t)(((prop && c->dirty[y]) ? 1 - cu : cu) | ((prop && c->Z[y] == cls) ? PB_PROP : 0));
+        }
+        if (tid < 64) {
+            int base = 0;
+            for (int h = 0; h < n_up; h++) {
+                const int qa = h == 0 ? 0 : c->upoff[h - 1], qb = c->upoff[h];
+                for (int q0 = qa; q0 < qb; q0 += 64) {
+                    const int q = q0 + tid;
+                    bool v = false;
+                    int y = 0;
+                    if (q < qb) {
+                        y = c->up[q];
+                        const int f = c->flags[y];
+                        v = !(f & NF_CHERRY) && (mode == PR_FULL || (c->dirty[y] && !(f & NF_MISSING)));
+                    }
+                    const unsigned long long bal = __ballot(v);
+                    if (v) c->plist[base + __popcll(bal & ((1ull << tid) - 1))] = y | (h << 16);
+                    base += __popcll(bal);
                 }
             }
-            if (sel[0] < 0) break;
+            if (tid == 0) c->n_vis = base;
+        }
+        __syncthreads();
+    }
+    const int n_vis = uni(c->n_vis);
+    const LDS uint8_t *pb = uni_ptr(c->pb);
+    const LDS uint32_t *desc = uni_ptr(c->desc);
+    for (int k = 0; k < n_vis;) {
+        {
+            // up to U consecutive visited nodes of one level
+            int ent[U], sel[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) ent[u] = (k + u < n_vis) ? uni(c->plist[k + u]) : -1;
+            int used = 1;
+            sel[0] = ent[0] & 0xffff;
+#pragma unroll
+            for (int u = 1; u < U; u++) {
+                sel[u] = -1;
+                if (used == u && ent[u] >= 0 && (ent[u] >> 16) == (ent[0] >> 16)) { sel[u] = ent[u] & 0xffff; used++; }
+            }
+            k += used;
             NodeDesc<R> nd[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (sel[u] < 0) continue;
                 const int s = sel[u];
-                const int wb = uni((mode == PR_PROP) ? 1 - c->cur[s] : c->cur[s]);
-                nd[u].dst = 2 * (wb * bufI + (s - S) * Ppad) + ((uni(c->flags[s]) & NF_MISSING) ? 1 : 0);
-                nd[u].k[0] = kid_desc(c, PTc, PTp, mode, cls, uni(c->ch0[s]), S, Ppad, bufI);
-                nd[u].k[1] = kid_desc(c, PTc, PTp, mode, cls, uni(c->ch1[s]), S, Ppad, bufI);
+                const LDS uint32_t *dd = desc + 4 * (s - S);
+                const uint32_t w0 = uni(dd[0]), w1 = uni(dd[1]), w2 = uni(dd[2]), w3 = uni(dd[3]);
+                const int x0 = w0 & 0xffff, x1 = w0 >> 16;
+                const int yA0 = w1 & 0xffff, yB0 = w1 >> 16, yA1 = w2 & 0xffff, yB1 = w2 >> 16;
+                const int ps = uni(pb[s]), p0 = uni(pb[x0]), p1 = uni(pb[x1]);
+                const int pA0 = uni(pb[yA0]), pB0 = uni(pb[yB0]), pA1 = uni(pb[yA1]), pB1 = uni(pb[yB1]);
+                nd[u].dst = 2 * ((ps & PB_BUF) * bufI + (s - S) * Ppad) + ((w3 & DB_MISS) ? 1 : 0);
+                nd[u].k[0] = kid_from<R>(w3 & 3, w3 & DB_MISS0, w3 & DB_MISS_A0, w3 & DB_MISS_B0, x0, p0, yA0, pA0, yB0,
+                                         pB0, PTc, PTp, S, Ppad, bufI);
+                nd[u].k[1] = kid_from<R>((w3 >> DB_KIND1_SHIFT) & 3, w3 & DB_MISS1, w3 & DB_MISS_A1, w3 & DB_MISS_B1, x1,
+                                         p1, yA1, pA1, yB1, pB1, PTc, PTp, S, Ppad, bufI);
                 // write partial + exponent; read each child's partial + exponent, its leaf code, or a cherry's two codes
                 bytes += (unsigned long long)P * (4 * sizeof(R) + 4);
 #pragma unroll
@@ -513,8 +562,9 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                     const uint8_t f = c->flags[s];
                     sn[j] = s;
                     if (f & NF_CHERRY) {
+                        const uint32_t kids = c->desc[4 * (s - S)];
                         if (!(f & NF_MISSING))
-                            ex[j] = codes[(size_t)c->ch0[s] * Ppad + p] | (codes[(size_t)c->ch1[s] * Ppad + p] << 8);
+                            ex[j] = codes[(size_t)(kids & 0xffff) * Ppad + p] | (codes[(size_t)(kids >> 16) * Ppad + p] << 8);
                     } else if (!(f & NF_MISSING)) {
                         const size_t off = c->cur[s] * bufI + (size_t)(s - S) * Ppad + p;
                         ex[j] = exps[off];
@@ -528,7 +578,8 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                 if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
                 const int s = sn[j];
                 if ((c->flags[s] & NF_CHERRY) && ex[j] != GAPT) {
-                    const int yA = c->ch0[s], yB = c->ch1[s];
+                    const uint32_t kids = c->desc[4 * (s - S)];
+                    const int yA = kids & 0xffff, yB = kids >> 16;
                     const ChildIn<R> in = cherry_in<R>(c->PTc + 16 * yA, c->PTc + 16 * yB, c->flags[yA] & NF_MISSING,
                                                        c->flags[yB] & NF_MISSING, false, ex[j] & 255, ex[j] >> 8);
                     ex[j] = in.x;
@@ -656,8 +707,9 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
         for (int q = 3 * a + tid; q < 3 * b; q += BS) {
             const int p = c->up[q / 3], j = q - 3 * (q / 3);
             double bj = c->beta[p * 3 + j];
+            const uint32_t kids = c->desc[4 * (p - c->S)];
             for (int w = 0; w < 2; w++) {
-                const int x = w ? c->ch1[p] : c->ch0[p];
+                const int x = w ? (int)(kids >> 16) : (int)(kids & 0xffff);
                 const LDS double *T = c->fixZ[x] ? c->TMf : c->TMn;
                 bj += log_col(T, j, c->beta[x * 3], c->beta[x * 3 + 1], c->beta[x * 3 + 2]);
             }
@@ -941,6 +993,44 @@ __device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
 
 // ---------------------------------------------------------- element setup
 
+// A cherry: a non-root internal node whose children are both leaves (tab: the
+// element's node table in global memory).
+__device__ __forceinline__ bool is_cherry(const int32_t *tab, int N, int S, int root, int x)
+{
+    return x >= S && x != root && tab[tab_ch0(N) + x] < S && tab[tab_ch1(N) + x] < S;
+}
+
+// The static pruning descriptor of internal node s (layout at DB_*): its
+// children, each child's kind and missing bit, a cherry child's two leaves
+// and their missing bits, and the node's own missing bit.
+__device__ __noinline__ void node_desc(LDS uint32_t *d, const int32_t *tab, int N, int S, int root, int s)
+{
+    const int32_t *ch0 = tab + tab_ch0(N), *ch1 = tab + tab_ch1(N), *fl = tab + tab_flags(N);
+    const int x[2] = {ch0[s], ch1[s]};
+    uint32_t w[2] = {0, 0};
+    uint32_t bits = (fl[s] & NF_MISSING) ? DB_MISS : 0;
+    for (int j = 0; j < 2; j++) {
+        const int xj = x[j];
+        if (xj < 0) continue;
+        int kind = KD_STORED;
+        if (xj < S) {
+            kind = KD_LEAF;
+        } else if (is_cherry(tab, N, S, root, xj)) {
+            kind = KD_CHERRY;
+            const int yA = ch0[xj], yB = ch1[xj];
+            w[j] = (uint32_t)yA | ((uint32_t)yB << 16);
+            if (fl[yA] & NF_MISSING) bits |= j ? DB_MISS_A1 : DB_MISS_A0;
+            if (fl[yB] & NF_MISSING) bits |= j ? DB_MISS_B1 : DB_MISS_B0;
+        }
+        bits |= (uint32_t)kind << (j ? DB_KIND1_SHIFT : 0);
+        if (fl[xj] & NF_MISSING) bits |= j ? DB_MISS1 : DB_MISS0;
+    }
+    d[0] = (uint32_t)(x[0] < 0 ? 0 : x[0]) | ((uint32_t)(x[1] < 0 ? 0 : x[1]) << 16);
+    d[1] = w[0];
+    d[2] = w[1];
+    d[3] = bits;
+}
+
 template <typename R, int BS>
 __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int work)
 {
@@ -968,8 +1058,6 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
     }
     const int32_t *tab = A->tab + E.off_tab;
     for (int i = tid; i < N; i += BS) {
-        c->ch0[i] = (int16_t)tab[tab_ch0(N) + i];
-        c->ch1[i] = (int16_t)tab[tab_ch1(N) + i];
         c->par[i] = (int16_t)tab[tab_par(N) + i];
         c->flags[i] = (uint8_t)tab[tab_flags(N) + i];
         c->nodes[i] = (int16_t)tab[tab_nodes(N) + i];
@@ -985,12 +1073,13 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
         c->upoff[i] = (int16_t)tab[tab_upoff(N) + i];
         c->downoff[i] = (int16_t)tab[tab_downoff(N) + i];
     }
+    for (int s = M->S + tid; s < N; s += BS) node_desc(c->desc + 4 * (s - M->S), tab, N, M->S, E.root, s);
     __syncthreads();
     for (int i = tid; i < M->S * 24; i += BS) c->amb[i] = 0;
     // cherries (both children leaves; never the root) are recomputed where read
     for (int q = tid; q < c->n_int; q += BS) {
         const int s = c->inodes[q];
-        if (s != c->root && c->ch0[s] < c->S && c->ch1[s] < c->S) c->flags[s] |= NF_CHERRY;
+        if (is_cherry(tab, N, M->S, E.root, s)) c->flags[s] |= NF_CHERRY;
     }
     // null-rate tables (rate 1), fixed for the element
     for (int q = tid; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, 0, c->nodes[q], 1.0, 0);
