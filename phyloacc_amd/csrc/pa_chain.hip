@@ -220,7 +220,10 @@ __device__ __noinline__ void class_table(MP M, CP<R> c, int k, int s, double r, 
     for (int q = 0; q < 16; q++) L[q] = log(P[(q & 3) * 4 + (q >> 2)]);
 }
 
-// PT of the current class of every branch (all branches, or those with Z == only_cls)
+// PT of the current class of every branch: all branches (only_cls = -1),
+// those with Z == only_cls, or (REFRESH_CHANGED) those whose Z the last Z
+// update changed
+constexpr int REFRESH_CHANGED = -2;
 template <typename R, int BS>
 __device__ __noinline__ void refresh_PTc(CP<R> c, int only_cls)
 {
@@ -228,6 +231,7 @@ __device__ __noinline__ void refresh_PTc(CP<R> c, int only_cls)
         const int s = c->nodes[q];
         const int z = c->Z[s];
         if (only_cls >= 0 && z != only_cls) continue;
+        if (only_cls == REFRESH_CHANGED && !c->changed[s]) continue;
         const int k = z == 2 ? 2 : (z == 0 ? 0 : 1);
         const GBL double *P = c->plin + ((size_t)k * c->N + s) * 16;
         LDS R *dst = c->PTc + 16 * s;
@@ -360,30 +364,37 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
     const LDS R *PTc = uni_ptr(c->PTc), *PTp = uni_ptr(c->PTp);
     constexpr int U = Group<R>::U;
     unsigned long long bytes = 0;
+    const LDS uint8_t *flags = uni_ptr(c->flags), *dirty = uni_ptr(c->dirty), *cur = uni_ptr(c->cur);
+    const LDS int32_t *plist = uni_ptr(c->plist);
     // pass set-up: every node's buffer and matrix bits, and the list of
     // nodes the pass visits in level order (one wave compacts it by ballot)
     {
         const bool prop = mode == PR_PROP;
-        for (int q = tid; q < c->n_nodes; q += BS) {
-            const int y = c->nodes[q];
-            const int cu = c->cur[y];
-            c->pb[y] = (uint8_t)(((prop && c->dirty[y]) ? 1 - cu : cu) | ((prop && c->Z[y] == cls) ? PB_PROP : 0));
+        const LDS int16_t *nodes = uni_ptr(c->nodes), *up = uni_ptr(c->up), *upoff = uni_ptr(c->upoff);
+        const LDS int8_t *Zs = uni_ptr(c->Z);
+        LDS uint8_t *pbw = uni_ptr(c->pb);
+        LDS int32_t *plw = uni_ptr(c->plist);
+        const int n_nodes = uni(c->n_nodes);
+        for (int q = tid; q < n_nodes; q += BS) {
+            const int y = nodes[q];
+            const int cu = cur[y];
+            pbw[y] = (uint8_t)(((prop && dirty[y]) ? 1 - cu : cu) | ((prop && Zs[y] == cls) ? PB_PROP : 0));
         }
         if (tid < 64) {
             int base = 0;
             for (int h = 0; h < n_up; h++) {
-                const int qa = h == 0 ? 0 : c->upoff[h - 1], qb = c->upoff[h];
+                const int qa = h == 0 ? 0 : upoff[h - 1], qb = upoff[h];
                 for (int q0 = qa; q0 < qb; q0 += 64) {
                     const int q = q0 + tid;
                     bool v = false;
                     int y = 0;
                     if (q < qb) {
-                        y = c->up[q];
-                        const int f = c->flags[y];
-                        v = !(f & NF_CHERRY) && (mode == PR_FULL || (c->dirty[y] && !(f & NF_MISSING)));
+                        y = up[q];
+                        const int f = flags[y];
+                        v = !(f & NF_CHERRY) && (mode == PR_FULL || (dirty[y] && !(f & NF_MISSING)));
                     }
                     const unsigned long long bal = __ballot(v);
-                    if (v) c->plist[base + __popcll(bal & ((1ull << tid) - 1))] = y | (h << 16);
+                    if (v) plw[base + __popcll(bal & ((1ull << tid) - 1))] = y | (h << 16);
                     base += __popcll(bal);
                 }
             }
@@ -399,7 +410,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
             // up to U consecutive visited nodes of one level
             int ent[U], sel[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) ent[u] = (k + u < n_vis) ? uni(c->plist[k + u]) : -1;
+            for (int u = 0; u < U; u++) ent[u] = (k + u < n_vis) ? uni(plist[k + u]) : -1;
             int used = 1;
             sel[0] = ent[0] & 0xffff;
 #pragma unroll
@@ -477,12 +488,13 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
     }
     if (tid == 0) c->bytes += bytes;
     // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
-    const int r = c->root;
+    const int r = uni(c->root);
     double acc = 0;
     if (r >= S) {
-        const int rb = (mode == PR_PROP && c->dirty[r]) ? 1 - c->cur[r] : c->cur[r];
+        const int rb = uni(pb[r]) & PB_BUF;
         const GBL V4<R> *rp = parts + rb * bufI + (size_t)(r - S) * Ppad;
         const GBL int32_t *re = exps + rb * bufI + (size_t)(r - S) * Ppad;
+        const GBL int32_t *wts = uni_ptr(c->wts);
         const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
         if (tid == 0) c->bytes += (unsigned long long)P * (4 * sizeof(R) + 4 + 4);   // root partial, exponent, weight
         for (int p = tid; p < P; p += BS) {
@@ -490,7 +502,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
             if (ex == GAPT) continue;
             const V4<R> a = ld4<R>(rp + p);
             const double L = p0 * (double)a.x + p1 * (double)a.y + p2 * (double)a.z + p3 * (double)a.w;
-            acc += (double)c->wts[p] * (log(L) + (double)ex * 0.69314718055994530942);
+            acc += (double)wts[p] * (log(L) + (double)ex * 0.69314718055994530942);
         }
     }
     return block_sum(acc, c->red);
@@ -505,34 +517,43 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
 template <typename R, int BS>
 __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c3)
 {
-    const int tid = threadIdx.x, S = c->S, n_int = c->n_int, GG = c->GG, Ppad = c->Ppad;
-    const size_t bufI = (size_t)c->I * Ppad;
-    const int r = c->root;
+    const int tid = threadIdx.x, S = uni(c->S), n_int = uni(c->n_int), GG = uni(c->GG), Ppad = uni(c->Ppad);
+    const size_t bufI = (size_t)uni(c->I) * Ppad;
+    const int r = uni(c->root);
     const uint32_t k0 = M->key0, k1 = M->key1;
     const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
-    const GBL V4<R> *parts = c->parts;
-    const GBL int32_t *exps = c->exps;
-    const GBL uint8_t *codes = c->codes;
-    LDS uint8_t *tgs = c->tgs;
-    for (int i = tid; i < 16 * c->N; i += BS) c->hist[i] = 0;   // shares storage with the proposal matrices
+    // the context's array pointers, read once: stores and atomics into LDS
+    // arrays would otherwise make every use reload them
+    const GBL V4<R> *parts = uni_ptr(c->parts);
+    const GBL int32_t *exps = uni_ptr(c->exps);
+    const GBL uint8_t *codes = uni_ptr(c->codes);
+    const GBL int32_t *site = uni_ptr(c->site);
+    GBL uint32_t *amb = uni_ptr(c->amb);
+    LDS uint8_t *tgs = uni_ptr(c->tgs);
+    LDS uint32_t *hist = uni_ptr(c->hist);
+    const LDS int16_t *inodes = uni_ptr(c->inodes), *par = uni_ptr(c->par);
+    const LDS uint8_t *flags = uni_ptr(c->flags), *cur = uni_ptr(c->cur);
+    const LDS uint32_t *desc = uni_ptr(c->desc);
+    const LDS R *PTc = uni_ptr(c->PTc);
+    for (int i = tid; i < 16 * c->N; i += BS) hist[i] = 0;   // shares storage with the proposal matrices
     __syncthreads();
     if (tid == 0) {
         // per site: pattern index, root partial, each non-missing internal node's partial
         // (a cherry's two leaf codes), each leaf code
         int nint = 0, nch = 0, nleaf = 0;
         for (int k = 0; k < n_int; k++) {
-            const uint8_t f = c->flags[c->inodes[k]];
+            const uint8_t f = flags[inodes[k]];
             if (f & NF_MISSING) continue;
             if (f & NF_CHERRY) nch++; else nint++;
         }
-        for (int q = 0; q < S; q++) nleaf += (c->flags[q] & NF_IN_TREE) && !(c->flags[q] & NF_MISSING);
+        for (int q = 0; q < S; q++) nleaf += (flags[q] & NF_IN_TREE) && !(flags[q] & NF_MISSING);
         c->bytes += (unsigned long long)GG * (4 + (4 * sizeof(R) + 4) * nint + 2 * nch + nleaf);
     }
     for (int g = tid; g < GG; g += BS) {
-        const int p = c->site[g];
+        const int p = site[g];
         U4 blk = philox(0, (uint32_t)g, c2, c3, k0, k1);
         {
-            const size_t off = c->cur[r] * bufI + (size_t)(r - S) * Ppad + p;
+            const size_t off = cur[r] * bufI + (size_t)(r - S) * Ppad + p;
             double pr[4];
             if (exps[off] == GAPT) {
                 pr[0] = p0; pr[1] = p1; pr[2] = p2; pr[3] = p3;
@@ -549,24 +570,30 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
         constexpr int K = sizeof(R) == 4 ? 4 : 2;
         int t = 1;
         for (int kb = n_int - 2; kb >= 0; kb -= K) {
-            int ex[K], sn[K];
+            // node ids, flags, buffers and cherry leaves are the same on every
+            // lane: they live in scalar registers
+            int ex[K], sn[K], fl[K], kid[K];
             V4<R> av[K];
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 const int k = kb - j;
                 ex[j] = GAPT;
                 sn[j] = 0;
+                fl[j] = NF_MISSING;
+                kid[j] = 0;
                 av[j].x = av[j].y = av[j].z = av[j].w = (R)0;
                 if (k >= 0) {
-                    const int s = c->inodes[k];
-                    const uint8_t f = c->flags[s];
+                    const int s = uni(inodes[k]);
+                    const int f = uni(flags[s]);
                     sn[j] = s;
+                    fl[j] = f;
                     if (f & NF_CHERRY) {
-                        const uint32_t kids = c->desc[4 * (s - S)];
+                        const int kids = uni((int)desc[4 * (s - S)]);
+                        kid[j] = kids;
                         if (!(f & NF_MISSING))
-                            ex[j] = codes[(size_t)(kids & 0xffff) * Ppad + p] | (codes[(size_t)(kids >> 16) * Ppad + p] << 8);
+                            ex[j] = codes[(size_t)(kids & 0xffff) * Ppad + p] | (codes[(size_t)((uint32_t)kids >> 16) * Ppad + p] << 8);
                     } else if (!(f & NF_MISSING)) {
-                        const size_t off = c->cur[s] * bufI + (size_t)(s - S) * Ppad + p;
+                        const size_t off = uni(cur[s]) * bufI + (size_t)(s - S) * Ppad + p;
                         ex[j] = exps[off];
                         av[j] = ld4<R>(parts + off);
                     }
@@ -577,23 +604,23 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                 if (kb - j < 0) break;
                 if ((t & 3) == 0) blk = philox((uint32_t)(t >> 2), (uint32_t)g, c2, c3, k0, k1);
                 const int s = sn[j];
-                if ((c->flags[s] & NF_CHERRY) && ex[j] != GAPT) {
-                    const uint32_t kids = c->desc[4 * (s - S)];
-                    const int yA = kids & 0xffff, yB = kids >> 16;
-                    const ChildIn<R> in = cherry_in<R>(c->PTc + 16 * yA, c->PTc + 16 * yB, c->flags[yA] & NF_MISSING,
-                                                       c->flags[yB] & NF_MISSING, false, ex[j] & 255, ex[j] >> 8);
+                if (fl[j] & NF_MISSING) continue;
+                if ((fl[j] & NF_CHERRY) && ex[j] != GAPT) {
+                    const int yA = kid[j] & 0xffff, yB = (int)((uint32_t)kid[j] >> 16);
+                    const ChildIn<R> in = cherry_in<R>(PTc + 16 * yA, PTc + 16 * yB, uni(flags[yA]) & NF_MISSING,
+                                                       uni(flags[yB]) & NF_MISSING, false, ex[j] & 255, ex[j] >> 8);
                     ex[j] = in.x;
                     av[j] = in.v;
                 }
                 if (ex[j] == GAPT) continue;     // missing node, or an all-gap subtree at this site
-                const int tp = tgs[(c->par[s] - S) * BS + tid];
+                const int tp = tgs[(uni(par[s]) - S) * BS + tid];
                 const V4<R> a = av[j];
-                const LDS R *PT = c->PTc + 16 * s;
+                const LDS R *PT = PTc + 16 * s;
                 double pr[4] = {(double)PT[0 * 4 + tp] * (double)a.x, (double)PT[1 * 4 + tp] * (double)a.y,
                                 (double)PT[2 * 4 + tp] * (double)a.z, (double)PT[3 * 4 + tp] * (double)a.w};
                 const int ts = categorical<4>(pr, u01(pick(blk, t & 3)));
                 tgs[(s - S) * BS + tid] = (uint8_t)ts;
-                lds_add(&c->hist[s * 16 + tp * 4 + ts], 1u);
+                lds_add(&hist[s * 16 + tp * 4 + ts], 1u);
             }
         }
         // leaves, 8 codes loaded at a time
@@ -604,7 +631,7 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
                 const int s = sb + j;
                 cd[j] = PA_GAP_CODE;
                 if (s < S) {
-                    const uint8_t f = c->flags[s];
+                    const int f = uni(flags[s]);
                     if ((f & NF_IN_TREE) && !(f & NF_MISSING)) cd[j] = codes[(size_t)s * Ppad + p];
                 }
             }
@@ -612,9 +639,9 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
             for (int j = 0; j < 8; j++) {
                 const int s = sb + j, code = cd[j];
                 if (code == PA_GAP_CODE || code == PA_OTHER_CODE) continue;
-                const int tp = tgs[(c->par[s] - S) * BS + tid];
-                if (code < 4) lds_add(&c->hist[s * 16 + tp * 4 + code], 1u);
-                else __hip_atomic_fetch_add(&c->amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const int tp = tgs[(uni(par[s]) - S) * BS + tid];
+                if (code < 4) lds_add(&hist[s * 16 + tp * 4 + code], 1u);
+                else __hip_atomic_fetch_add(&amb[s * 24 + tp * 6 + (code - 6)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
     }
@@ -699,6 +726,7 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
         c->changed[s] = 0;
         c->dirty[s] = 0;
     }
+    if (tid == 0) c->nch = 0;
     __syncthreads();
     const int n_up = c->n_up, n_down = c->n_down;
     for (int h = 0; h < n_up; h++) {
@@ -749,14 +777,10 @@ __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
     for (int q = tid; q < c->n_nodes; q += BS) {
         const int s = c->nodes[q];
         if (!c->changed[s]) continue;
+        c->nch = 1;     // every writer stores the same value
         for (int j = c->par[s]; j != N; j = c->par[j]) c->dirty[j] = 1;
     }
     __syncthreads();
-    if (tid == 0) {
-        int any = 0;
-        for (int q = 0; q < c->n_nodes; q++) any |= c->changed[c->nodes[q]];
-        c->nch = any;
-    }
 }
 
 // ------------------------------------------------------------- rate update
@@ -1188,7 +1212,7 @@ __device__ __forceinline__ void run_model(MP M, CP<R> c, const ChainArgs *A, int
         emissions<R, BS>(c);
         __syncthreads();
         update_Z<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_Z));
-        refresh_PTc<R, BS>(c, -1);
+        refresh_PTc<R, BS>(c, REFRESH_CHANGED);
         __syncthreads();
         if (c->nch) {
             const double l = prune<R, BS>(M, c, PR_DIRTY, 0);

@@ -106,8 +106,10 @@ PA_HD double beta_draw(Key k, uint32_t cx, uint32_t cy, uint32_t c2, uint32_t c3
     return x / (x + y);
 }
 
-// Inverse-CDF categorical over k non-negative weights; returns the last
-// positive-weight index when rounding puts the target past the end.
+// Inverse-CDF categorical over k non-negative weights: the first i with
+// u * total < p[0] + ... + p[i], or the last positive-weight index when
+// rounding puts the target past the end.  Written as selects, so the lanes
+// of a wave never branch apart.
 template <int K>
 PA_HD int categorical(const double *p, double u)
 {
@@ -116,15 +118,20 @@ PA_HD int categorical(const double *p, double u)
 #pragma unroll
     for (int i = 0; i < K; i++) {
         tot += p[i];
-        if (p[i] > 0) last = i;
+        last = p[i] > 0 ? i : last;
     }
-    double target = u * tot, cum = 0;
+    const double target = u * tot;
+    double cum[K > 1 ? K - 1 : 1];
+    double run = 0;
 #pragma unroll
     for (int i = 0; i < K - 1; i++) {
-        cum += p[i];
-        if (target < cum) return i;
+        run += p[i];
+        cum[i] = run;
     }
-    return last;
+    int res = last;
+#pragma unroll
+    for (int i = K - 2; i >= 0; i--) res = target < cum[i] ? i : res;
+    return res;
 }
 
 // log of the Gamma(shape a, scale b) density, closed form
