@@ -85,7 +85,15 @@ def lib():
             # never built from here: a process that has touched the GPU must not
             # start other programs, so the build belongs to __graft_entry__.build()
             raise RuntimeError(f"{LIB} is missing: build it with make -C oracle")
-        L = C.CDLL(LIB)
+        _lib = load(LIB)
+    return _lib
+
+
+def load(path):
+    """A ctypes view of an oracle library built at path (the oracle, or a
+    copy with a planted mistake that the oracle's own tests must catch)."""
+    if True:
+        L = C.CDLL(path)
         P = C.POINTER
         L.or_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         L.or_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
@@ -113,10 +121,14 @@ def lib():
         L.or_gt_loglik.argtypes = [P(Model), C.c_int, P(C.c_int), C.c_int, P(C.c_int), P(C.c_int), P(C.c_double),
                                    P(C.c_int), C.c_double, C.c_double, P(C.c_char_p), C.c_long, C.c_int, C.c_int, C.c_double]
         L.or_gt_loglik.restype = C.c_double
+        L.or_test_eval2.argtypes = [P(Model), P(Options), C.c_void_p, C.c_int, P(C.c_double), P(C.c_double),
+                                    P(C.c_double), P(C.c_byte), P(C.c_int)]
+        L.or_test_eval2.restype = C.c_double
+        L.or_test_sample_site.argtypes = [P(Model), C.c_void_p, P(C.c_int), C.c_double, C.c_double, C.c_uint64,
+                                          C.c_int, C.c_int, C.c_int, P(C.c_int)]
         L.or_run_batch.argtypes = [P(Model), P(Options), P(C.c_void_p), C.c_int, C.c_int, P(C.c_double)]
         L.or_run_batch.restype = C.c_long
-        _lib = L
-    return _lib
+    return L
 
 
 def _ptr(a, t):
@@ -192,6 +204,26 @@ class Element:
     def loglik(self, Z, r1, r2):
         Z = np.ascontiguousarray(Z, np.int32)
         return lib().or_loglik(C.byref(self.model), self.ptr, _ptr(Z, C.c_int), float(r1), float(r2))
+
+    def eval2(self, opts, ms, full, nrate, crate, Z, max_Z):
+        """Eval2 of model slot ms over the given traces (T = burnin + mcmc)."""
+        f = np.ascontiguousarray(full, np.float64)
+        n = np.ascontiguousarray(nrate, np.float64)
+        c = np.ascontiguousarray(crate, np.float64)
+        z = np.ascontiguousarray(Z, np.int8)
+        mz = np.ascontiguousarray(max_Z, np.int32)
+        return lib().or_test_eval2(C.byref(self.model), C.byref(opts), self.ptr, int(ms), _ptr(f, C.c_double),
+                                   _ptr(n, C.c_double), _ptr(c, C.c_double), _ptr(z, C.c_byte), _ptr(mz, C.c_int))
+
+    def sample_site(self, Z, r1, r2, seed, m_sweep, ms=0, site=0):
+        """The ancestral draw of one site at fixed Z and rates: every node's base."""
+        z = np.ascontiguousarray(Z, np.int32)
+        out = np.zeros(self.info()["N"], np.int32)
+        rc = lib().or_test_sample_site(C.byref(self.model), self.ptr, _ptr(z, C.c_int), float(r1), float(r2),
+                                       int(seed), int(m_sweep), int(ms), int(site), _ptr(out, C.c_int))
+        if rc != 0:
+            raise ValueError("or_test_sample_site failed")
+        return out
 
     def run(self, opts, traces=True):
         """Run M0, M1, M2 (reference run order resZ 0, 2, 1)."""

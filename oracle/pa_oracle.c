@@ -917,7 +917,10 @@ static double sample_rate(chain_t *c, double old_rate, int neut, const char *vis
             double MH;
             if (neut)
                 MH = loglik_new - *loglik_old + log_gamma_pdf(proposal, prior_a, prior_b) - log_gamma_pdf(cur, prior_a, prior_b)
-                     + log_gamma_pdf(cur, proposal / e->prop_n, e->prop_n) - log_gamma_pdf(proposal, cur / e->prop_n, e->prop_n);
+#ifndef OR_MUTANT_NO_HASTINGS   /* planted mistake for the oracle's own tests: the Hastings terms dropped */
+                     + log_gamma_pdf(cur, proposal / e->prop_n, e->prop_n) - log_gamma_pdf(proposal, cur / e->prop_n, e->prop_n)
+#endif
+                     ;
             else
                 MH = loglik_new - *loglik_old + log_gamma_pdf(proposal, prior_a, prior_b) - log_gamma_pdf(cur, prior_a, prior_b)
                      + log(cur) - log(proposal);
@@ -966,7 +969,11 @@ static void sample_transition(chain_t *c, double *gr, double *lr, double *lr2)
     double nZ[3][3] = {{0}};
     for (int k = 0; k < e->n_nodes - 1; k++) { int s = e->nodes[k]; nZ[e->Z[s]][e->Z[e->par2[s]]] += 1; }
     uint32_t c3 = tagword(c->ms, OR_TAG_TRANS), c2 = (uint32_t)c->m_cur;
+#ifndef OR_MUTANT_SWAP_G_COUNTS   /* planted mistake for the oracle's own tests: g's counts swapped */
     *gr = or_beta_stream(o->seed, 0, 1, c2, c3, o->hyper_grate_a + nZ[1][0] + nZ[2][0], o->hyper_grate_b + nZ[0][0]);
+#else
+    *gr = or_beta_stream(o->seed, 0, 1, c2, c3, o->hyper_grate_a + nZ[0][0], o->hyper_grate_b + nZ[1][0] + nZ[2][0]);
+#endif
     for (int k = 0; k < e->n_nodes - 1; k++) {
         int s = e->nodes[k];
         if (e->fixZ[s] == 1) nZ[e->Z[s]][e->Z[e->par2[s]]] -= 1;
@@ -1205,6 +1212,56 @@ int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, co
     update_Z_core(&c, changed);
     for (int s = 0; s < N; s++) Zout[s] = e->Z[s];
     free(e);
+    return 0;
+}
+
+/* Test hook: Eval2 (bpp_c2.cpp:14-353) over caller-supplied traces of model
+ * slot ms (0 = M0, 1 = M1, 2 = M2) of element e: full, nrate, crate [T] and
+ * Z [T*N] with T = burnin + mcmc, and the configuration max_Z [N].  The
+ * element's fixZ and caches are set up as initMCMC does for that model. */
+double or_test_eval2(const or_model *m, const or_options *o, or_element *e, int ms, const double *full,
+                     const double *nrate, const double *crate, const signed char *Z, const int *max_Z)
+{
+    if (e->filtered) return NAN;
+    static const int order[3] = {0, 2, 1};
+    int T = o->burnin + o->mcmc, N = e->N;
+    chain_t c; memset(&c, 0, sizeof c);
+    c.m = m; c.o = o; c.e = e; c.resZ = order[ms]; c.ms = ms; c.T = T;
+    for (int j = 0; j < 4; j++) c.logpi[j] = log(m->pi[j]);
+    double *buf = (double *)calloc((size_t)7 * T, sizeof(double));
+    c.tl = buf; c.tf = buf + T; c.tn = buf + 2 * T; c.tc = buf + 3 * T;
+    c.tg = buf + 4 * T; c.tlr = buf + 5 * T; c.tl2 = buf + 6 * T;
+    c.tZ = (signed char *)malloc((size_t)T * N);
+    init_mcmc(&c);
+    memcpy(c.tf, full, sizeof(double) * T);
+    memcpy(c.tn, nrate, sizeof(double) * T);
+    memcpy(c.tc, crate, sizeof(double) * T);
+    memcpy(c.tZ, Z, (size_t)T * N);
+    for (int s = 0; s < N; s++) c.Max_Z[s] = max_Z[s];
+    double vb = eval2(&c);
+    free(buf); free(c.tZ);
+    return vb;
+}
+
+/* Test hook: the ancestral draw of one site (BPP_C::Update_Tg(tosample=true),
+ * bpp_c.cpp:672-733) at fixed Z and rates (r1 conserved, r2 accelerated),
+ * with the uniforms of sweep m_sweep of model slot ms.  states: [N], the
+ * sampled base of every internal node (leaves keep their encoded Tg). */
+int or_test_sample_site(const or_model *m, or_element *e, const int *Z, double r1, double r2, uint64_t seed,
+                        int m_sweep, int ms, int site, int *states)
+{
+    if (e->filtered || site < 0 || site >= e->GG) return 1;
+    or_options o; memset(&o, 0, sizeof o); o.seed = seed;
+    model_caches(m, e, r2, r1);
+    for (int s = 0; s < e->N; s++) e->Z[s] = Z[s];
+    zero_internal(e);
+    chain_t c; memset(&c, 0, sizeof c);
+    c.m = m; c.o = &o; c.e = e; c.ms = ms; c.m_cur = m_sweep;
+    for (int j = 0; j < 4; j++) c.logpi[j] = log(m->pi[j]);
+    char visited[OR_MAXN] = {0};
+    prune_site(e, site, visited, c.logpi);
+    sample_site(&c, site);
+    for (int s = 0; s < e->N; s++) states[s] = TG(e, site, s);
     return 0;
 }
 

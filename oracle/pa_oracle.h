@@ -113,6 +113,14 @@ double or_gt_loglik(const or_model *m, int S, const int *gchildren, int K, const
 int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, const int *fixZ, const double *E,
                      double g, double l, double l2, uint64_t seed, int m, int ms, int *Zout);
 
+/* test hook: Eval2 over given traces of model slot ms (0 = M0, 1 = M1, 2 = M2) */
+double or_test_eval2(const or_model *m, const or_options *o, or_element *e, int ms, const double *full,
+                     const double *nrate, const double *crate, const signed char *Z, const int *max_Z);
+
+/* test hook: one site's ancestral draw at fixed Z and rates, uniforms of sweep m_sweep */
+int or_test_sample_site(const or_model *m, or_element *e, const int *Z, double r1, double r2, uint64_t seed,
+                        int m_sweep, int ms, int site, int *states);
+
 /* CPU baseline: many elements, OpenMP guided schedule; returns sweeps done */
 long or_run_batch(const or_model *m, const or_options *o, or_element **es, int n, int nthreads,
                   double *max_loglik /* [n*3] */);
