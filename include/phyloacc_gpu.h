@@ -31,6 +31,19 @@ enum { PA_FP64 = 0, PA_FP32 = 1 };                 /* partial-likelihood precisi
 enum { PA_RUN_BATCHED = 0, PA_RUN_PER_ELEMENT = 1 }; /* launch strategy */
 enum { PA_ELEM_OK = 0, PA_ELEM_FILTERED = 1, PA_ELEM_NUMERIC_FAIL = 2 };
 
+/* Per-chain failure and warning bits (pa_chain_result.flags).  The reference
+ * marks an element failed when its marginal likelihood is NaN
+ * (src/PhyloAcc-ST/bpp_c2.cpp:339-348) and warns on out-of-range rate
+ * proposals (bpp_c.cpp:1024-1041); here every chain records: */
+enum {
+    PA_FLAG_NAN = 1,            /* the log-likelihood was NaN at some state (status NUMERIC_FAIL) */
+    PA_FLAG_POS_INF = 2,        /* ... +inf at some state (status NUMERIC_FAIL) */
+    PA_FLAG_NEG_INF = 4,        /* ... -inf at some state: data impossible at that state's rates;
+                                   status NUMERIC_FAIL when the chain's maximum is not finite */
+    PA_FLAG_SAMPLER = 8,        /* a Gamma draw stopped at its rejection cap (status NUMERIC_FAIL) */
+    PA_FLAG_PROPOSAL_RANGE = 16 /* a rate proposal fell outside (1e-12, 1e8) and was skipped (warning) */
+};
+
 /* leaf codes produced by the encoder */
 enum { PA_CODE_A = 0, PA_CODE_C = 1, PA_CODE_G = 2, PA_CODE_T = 3, PA_CODE_GAP = 4,
        PA_CODE_OTHER = 5, PA_CODE_R = 6, PA_CODE_Y = 7, PA_CODE_K = 8, PA_CODE_M = 9,
@@ -178,6 +191,8 @@ typedef struct {
     int32_t status;             /* PA_ELEM_* */
     double final_nrate, final_crate, final_grate, final_lrate, final_lrate2;
     int32_t accept_n, accept_c; /* adaptive-MH counters at chain end */
+    int32_t flags;              /* PA_FLAG_* bits */
+    int32_t pad;
 } pa_chain_result;
 
 /* Optional device-to-host trace copy, per element and model, T = BURNIN+MCMC. */

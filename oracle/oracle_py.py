@@ -71,7 +71,7 @@ class ChainOut(C.Structure):
         ("postZ", C.POINTER(C.c_double)),
         ("max_loglik", C.c_double), ("max_m", C.c_int), ("max_Z", C.c_int * MAXN),
         ("vb", C.c_double), ("med", C.c_double * 5), ("failure", C.c_int),
-        ("accept_n", C.c_int), ("accept_c", C.c_int),
+        ("accept_n", C.c_int), ("accept_c", C.c_int), ("flags", C.c_int),
     ]
 
 
@@ -254,7 +254,7 @@ class Element:
             o = outs[k]
             d = dict(max_loglik=o.max_loglik, max_m=o.max_m, max_Z=np.array(o.max_Z[:N]),
                      vb=o.vb, med=np.array(o.med[:]), failure=o.failure,
-                     accept_n=o.accept_n, accept_c=o.accept_c)
+                     accept_n=o.accept_n, accept_c=o.accept_c, flags=o.flags)
             for f, a in keep[k].items():
                 d[f] = a.reshape(T, N) if f == "Z" else (a.reshape(N, 4) if f == "postZ" else a)
             res.append(d)

@@ -70,7 +70,10 @@ static inline double st_next(stream_t *s) { double u = slot_u(s->seed, s->c1, s-
 #define OR_GAMMA_MAXIT 64
 /* Marsaglia & Tsang (2000), shape a > 0, scale b; for a < 1 the boost
  * U^(1/a) uses draw 0 and the main loop starts at draw 1. */
-static double gamma_draw(stream_t *s, double a, double b)
+static double gamma_draw_c(stream_t *s, double a, double b, int *capped);
+static double gamma_draw(stream_t *s, double a, double b) { return gamma_draw_c(s, a, b, NULL); }
+/* capped (may be NULL): set to 1 when the rejection loop stops at its cap */
+static double gamma_draw_c(stream_t *s, double a, double b, int *capped)
 {
     double boost = 1.0;
     if (a < 1.0) { double u = st_next(s); boost = pow(u, 1.0 / a); a += 1.0; }
@@ -85,6 +88,7 @@ static double gamma_draw(stream_t *s, double a, double b)
         if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v * b * boost;
         if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v * b * boost;
     }
+    if (capped) *capped = 1;
     return d * b * boost;
 }
 
@@ -92,6 +96,14 @@ double or_gamma_stream(uint64_t seed, uint32_t c1, uint32_t c2, uint32_t c3, dou
 {
     stream_t s = {seed, c1, c2, c3, 0};
     return gamma_draw(&s, shape, scale);
+}
+
+static double beta_draw_c(uint64_t seed, uint32_t c1x, uint32_t c1y, uint32_t c2, uint32_t c3, double a, double b,
+                          int *capped)
+{
+    stream_t sx = {seed, c1x, c2, c3, 0}, sy = {seed, c1y, c2, c3, 0};
+    double x = gamma_draw_c(&sx, a, 1.0, capped), y = gamma_draw_c(&sy, b, 1.0, capped);
+    return x / (x + y);
 }
 
 double or_beta_stream(uint64_t seed, uint32_t c1x, uint32_t c1y, uint32_t c2, uint32_t c3, double a, double b)
@@ -632,6 +644,7 @@ typedef struct {
     double logpi[4];
     double *tl, *tf, *tn, *tc, *tg, *tlr, *tl2; signed char *tZ;
     double MaxLoglik; int Max_m; int Max_Z[OR_MAXN];
+    int flags;                             /* OR_FLAG_* bits (pa_oracle.h) */
 } chain_t;
 
 static void set_TM(or_element *e, int s, double g, double l, double l2, int init)
@@ -692,6 +705,8 @@ static void monitor(chain_t *c, int m, double *loglik)
         for (int g = 0; g < e->GG; g++) t += log_exp_sum(LAM(e, g, e->root), 4);
         c->tl[0] = t; *loglik = t;
     } else c->tl[m] = *loglik;
+    if (isnan(c->tl[m])) c->flags |= OR_FLAG_NAN;
+    else if (isinf(c->tl[m])) c->flags |= c->tl[m] > 0 ? OR_FLAG_POS_INF : OR_FLAG_NEG_INF;
     c->tf[m] = c->tl[m] + log_gamma_pdf(c->tc[m], o->cprior_a, o->cprior_b);
     if (c->resZ != 0) c->tf[m] += log_gamma_pdf(c->tn[m], o->nprior_a, o->nprior_b);
     if (m >= o->burnin && c->tf[m] > c->MaxLoglik) {
@@ -880,7 +895,9 @@ static double sample_rate(chain_t *c, double old_rate, int neut, const char *vis
     if (visited[e->root]) {
         if (c->resZ != 0) {
             stream_t sp = {o->seed, 2, c2, c3, 0};
-            cur = gamma_draw(&sp, prior_a, prior_b);
+            int capped = 0;
+            cur = gamma_draw_c(&sp, prior_a, prior_b, &capped);
+            if (capped) c->flags |= OR_FLAG_SAMPLER;
             if (neut) {
                 if (o->ropt == 1 && cur < o->nlb) cur = old_rate;
                 else if (o->ropt == 2 && cur < c->tc[m]) cur = old_rate;
@@ -894,7 +911,9 @@ static double sample_rate(chain_t *c, double old_rate, int neut, const char *vis
         double r = slot_u(o->seed, 0, c2, c3, 0), proposal;
         if (neut) {
             stream_t sg = {o->seed, 1, c2, c3, 0};
-            proposal = gamma_draw(&sg, cur / e->prop_n, e->prop_n);
+            int capped = 0;
+            proposal = gamma_draw_c(&sg, cur / e->prop_n, e->prop_n, &capped);
+            if (capped) c->flags |= OR_FLAG_SAMPLER;
             if (o->ropt == 1 && proposal < o->nlb) skip = 1;
             else if (o->ropt == 2 && proposal < c->tc[m]) skip = 1;
         } else {
@@ -904,7 +923,10 @@ static double sample_rate(chain_t *c, double old_rate, int neut, const char *vis
             if (o->ropt == 1 && proposal > o->cub) skip = 1;
             else if (o->ropt == 2 && proposal > c->tn[m + 1]) skip = 1;
         }
-        if (!skip && (proposal < 1e-12 || proposal > 1e8)) skip = 1;
+        if (!skip && (proposal < 1e-12 || proposal > 1e8)) {   /* the reference warns and skips (bpp_c.cpp:1024-1041) */
+            skip = 1;
+            c->flags |= OR_FLAG_PROPOSAL_RANGE;
+        }
         if (!skip) {
             size_t GG = e->GG;
             double *lt = (double *)calloc(GG * N * 4, sizeof(double));
@@ -969,18 +991,20 @@ static void sample_transition(chain_t *c, double *gr, double *lr, double *lr2)
     double nZ[3][3] = {{0}};
     for (int k = 0; k < e->n_nodes - 1; k++) { int s = e->nodes[k]; nZ[e->Z[s]][e->Z[e->par2[s]]] += 1; }
     uint32_t c3 = tagword(c->ms, OR_TAG_TRANS), c2 = (uint32_t)c->m_cur;
+    int capped = 0;
 #ifndef OR_MUTANT_SWAP_G_COUNTS   /* planted mistake for the oracle's own tests: g's counts swapped */
-    *gr = or_beta_stream(o->seed, 0, 1, c2, c3, o->hyper_grate_a + nZ[1][0] + nZ[2][0], o->hyper_grate_b + nZ[0][0]);
+    *gr = beta_draw_c(o->seed, 0, 1, c2, c3, o->hyper_grate_a + nZ[1][0] + nZ[2][0], o->hyper_grate_b + nZ[0][0], &capped);
 #else
-    *gr = or_beta_stream(o->seed, 0, 1, c2, c3, o->hyper_grate_a + nZ[0][0], o->hyper_grate_b + nZ[1][0] + nZ[2][0]);
+    *gr = beta_draw_c(o->seed, 0, 1, c2, c3, o->hyper_grate_a + nZ[0][0], o->hyper_grate_b + nZ[1][0] + nZ[2][0], &capped);
 #endif
     for (int k = 0; k < e->n_nodes - 1; k++) {
         int s = e->nodes[k];
         if (e->fixZ[s] == 1) nZ[e->Z[s]][e->Z[e->par2[s]]] -= 1;
     }
-    *lr = or_beta_stream(o->seed, 2, 3, c2, c3, o->hyper_lrate_a + nZ[2][1], o->hyper_lrate_b + nZ[1][1]);
+    *lr = beta_draw_c(o->seed, 2, 3, c2, c3, o->hyper_lrate_a + nZ[2][1], o->hyper_lrate_b + nZ[1][1], &capped);
     if (o->hyper_lrate2_a == 0) *lr2 = 0;
-    else *lr2 = or_beta_stream(o->seed, 4, 5, c2, c3, o->hyper_lrate2_a + nZ[1][2], o->hyper_lrate2_b + nZ[2][2]);
+    else *lr2 = beta_draw_c(o->seed, 4, 5, c2, c3, o->hyper_lrate2_a + nZ[1][2], o->hyper_lrate2_b + nZ[2][2], &capped);
+    if (capped) c->flags |= OR_FLAG_SAMPLER;
     for (int k = 0; k < e->n_nodes; k++) set_TM(e, e->nodes[k], *gr, *lr, *lr2, 0);
 }
 
@@ -1161,6 +1185,7 @@ int or_run_element(const or_model *m, const or_options *o, or_element *e, or_cha
         init_mcmc(&c);
         gibbs(&c);
         out->max_loglik = c.MaxLoglik; out->max_m = c.Max_m;
+        out->flags = c.flags;
         for (int s = 0; s < N; s++) out->max_Z[s] = c.Max_Z[s];
         out->vb = eval2(&c);
         out->failure = isnan(out->vb) ? 1 : 0;

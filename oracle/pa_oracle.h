@@ -21,6 +21,12 @@
 enum { OR_A = 0, OR_C = 1, OR_G = 2, OR_T = 3, OR_GAP = 4, OR_OTHER = 5,
        OR_R = 6, OR_Y = 7, OR_K = 8, OR_M = 9, OR_S = 10, OR_W = 11 };
 
+/* chain failure and warning bits (or_chain_out.flags): log-likelihood NaN,
+ * +inf or -inf at some state; a Gamma draw stopped at its rejection cap; a
+ * rate proposal outside (1e-12, 1e8) skipped (the reference's warning,
+ * src/PhyloAcc-ST/bpp_c.cpp:1024-1041) */
+enum { OR_FLAG_NAN = 1, OR_FLAG_POS_INF = 2, OR_FLAG_NEG_INF = 4, OR_FLAG_SAMPLER = 8, OR_FLAG_PROPOSAL_RANGE = 16 };
+
 /* RNG stream tags (bits 16..23 of counter word 3) */
 enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5 };
 
@@ -70,6 +76,7 @@ typedef struct {
     double med[5];                         /* Output_init medians: n, c, g, l, l2 */
     int failure;
     int accept_n, accept_c;                /* counters after the chain (carry over) */
+    int flags;                             /* OR_FLAG_* bits */
 } or_chain_out;
 
 #ifdef __cplusplus

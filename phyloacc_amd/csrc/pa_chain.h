@@ -9,6 +9,11 @@
 namespace pa {
 namespace chain {
 
+// Per (work position, model) result words: RES_D doubles (max_loglik, final
+// nrate, crate, g, l, l2) and RES_I ints (max_m, status, accept_n,
+// accept_c, flags).
+constexpr int RES_D = 8, RES_I = 8;
+
 struct ChainArgs {
     const ElemDev *elems;
     const int32_t *work;      // element ids in schedule order
@@ -27,8 +32,8 @@ struct ChainArgs {
     char *big;
     const int64_t *big_off;
     // outputs, indexed by work position
-    double *res_d;            // [n_work][3][8]
-    int32_t *res_i;           // [n_work][3][4]
+    double *res_d;            // [n_work][3][RES_D]
+    int32_t *res_i;           // [n_work][3][RES_I]
     int8_t *maxZ;             // [n_work][3][N]
     double *tr_d;             // [n_work][3][7][T] or null
     int8_t *tr_Z;             // [n_work][3][T][N] or null

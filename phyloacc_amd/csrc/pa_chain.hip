@@ -68,7 +68,7 @@ struct Ctx {
     double TMf[9], TMn[9];  // log_TM_Int for fixZ / free nodes, [i*3+j] = log P(child i | parent j)
     double red[16];
     unsigned long long bytes;   // algorithmic HBM bytes of this element's passes (see prune / sample_sites)
-    int accept_n, accept_c, max_m, skip, accept, status, work, nch, n_vis;
+    int accept_n, accept_c, max_m, skip, accept, fail, work, nch, n_vis;   // fail: PA_F_* bits
 };
 
 template <typename R>
@@ -199,7 +199,7 @@ __device__ __forceinline__ void ptrans(MP M, double t, OutPtr out)
             double s = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++) s += M->V[j * 4 + k] * e[k] * M->Vi[k * 4 + i];
-            out[j * 4 + i] = s > 0 ? s : 0.0;
+            out[j * 4 + i] = s < 0 ? 0.0 : s;
         }
 }
 
@@ -814,7 +814,9 @@ __device__ __noinline__ void rate_propose(MP M, CP<R> c, int ms, int m, int resZ
     if (!c->dirty[c->root]) {
         if (resZ != 0) {
             Stream sp{key, 2, c2, c3, 0};
-            cur = gamma_draw(sp, pa, pb);
+            int capped = 0;
+            cur = gamma_draw(sp, pa, pb, &capped);
+            if (capped) c->fail |= PA_F_SAMPLER;
             if (neut) {
                 if (M->ropt == 1 && cur < M->nlb) cur = old;
                 else if (M->ropt == 2 && cur < c->crate) cur = old;
@@ -829,7 +831,9 @@ __device__ __noinline__ void rate_propose(MP M, CP<R> c, int ms, int m, int resZ
         int skip = 0;
         if (neut) {
             Stream sg{key, 1, c2, c3, 0};
-            prop = gamma_draw(sg, cur / c->prop_n, c->prop_n);
+            int capped = 0;
+            prop = gamma_draw(sg, cur / c->prop_n, c->prop_n, &capped);
+            if (capped) c->fail |= PA_F_SAMPLER;
             if (M->ropt == 1 && prop < M->nlb) skip = 1;
             else if (M->ropt == 2 && prop < c->crate) skip = 1;
         } else {
@@ -839,7 +843,10 @@ __device__ __noinline__ void rate_propose(MP M, CP<R> c, int ms, int m, int resZ
             if (M->ropt == 1 && prop > M->cub) skip = 1;
             else if (M->ropt == 2 && prop > c->nrate) skip = 1;
         }
-        if (!skip && (prop < 1e-12 || prop > 1e8)) skip = 1;
+        if (!skip && (prop < 1e-12 || prop > 1e8)) {
+            skip = 1;
+            c->fail |= PA_F_PROPOSAL_RANGE;   // the reference's out-of-range warning (bpp_c.cpp:1024-1041)
+        }
         c->skip = skip;
         c->proposal = prop;
         c->r = r;
@@ -906,7 +913,7 @@ __device__ __noinline__ void build_prop_tables(MP M, CP<R> c, int cls, double pr
                 double v = 0;
 #pragma unroll
                 for (int k = 0; k < 4; k++) v += M->V[j * 4 + k] * e[k] * M->Vi[k * 4 + i];
-                dst[i * 4 + j] = (R)(v > 0 ? v : 0.0);
+                dst[i * 4 + j] = (R)(v < 0 ? 0.0 : v);
             }
     }
 }
@@ -985,7 +992,10 @@ __device__ __noinline__ void transition_gamma(MP M, CP<R> c, int ms, int m, int 
     double x = 0.0;
     if (j < 4 || M->ha_l2 != 0) {
         Stream st{key, (uint32_t)j, c2, c3, 0};
-        x = gamma_draw(st, shape, 1.0);
+        int capped = 0;
+        x = gamma_draw(st, shape, 1.0, &capped);
+        if (capped)   // up to six lanes draw at once
+            __hip_atomic_fetch_or(&c->fail, PA_F_SAMPLER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     c->red[j] = x;
 }
@@ -1142,7 +1152,7 @@ __device__ __noinline__ void init_model(MP M, CP<R> c, int resZ)
         c->crate = M->init_crate;
         c->g = M->init_grate; c->l = M->init_lrate; c->l2 = M->ha_l2 == 0 ? 0.0 : M->init_lrate2;
         set_TM(c->TMf, c->TMn, c->g, c->l, c->l2);
-        c->status = 0;
+        c->fail = 0;
     }
     __syncthreads();
     const double cr = c->crate, nr = c->nrate;
@@ -1171,7 +1181,8 @@ __device__ __noinline__ void monitor(MP M, CP<R> c, int m, int resZ, double *trd
         c->max_m = m;
         for (int i = 0; i < M->N; i++) c->maxZ[i] = c->Z[i];
     }
-    if (isnan(c->loglik)) c->status = PA_STATUS_NUMERIC_FAIL;
+    if (isnan(c->loglik)) c->fail |= PA_F_NAN;
+    else if (isinf(c->loglik)) c->fail |= c->loglik > 0 ? PA_F_POS_INF : PA_F_NEG_INF;
 }
 
 template <typename R, int BS>
@@ -1179,10 +1190,12 @@ __device__ __noinline__ void write_results(MP M, CP<R> c, const ChainArgs *A, in
 {
     const int tid = threadIdx.x, N = M->N;
     if (tid == 0) {
-        double *rd = A->res_d + ((size_t)work * 3 + ms) * 8;
-        int32_t *ri = A->res_i + ((size_t)work * 3 + ms) * 4;
+        double *rd = A->res_d + ((size_t)work * 3 + ms) * RES_D;
+        int32_t *ri = A->res_i + ((size_t)work * 3 + ms) * RES_I;
         rd[0] = c->max_loglik; rd[1] = c->nrate; rd[2] = c->crate; rd[3] = c->g; rd[4] = c->l; rd[5] = c->l2;
-        ri[0] = c->max_m; ri[1] = c->status; ri[2] = c->accept_n; ri[3] = c->accept_c;
+        const bool failed = (c->fail & (PA_F_NAN | PA_F_POS_INF | PA_F_SAMPLER)) || !isfinite(c->max_loglik);
+        ri[0] = c->max_m; ri[1] = failed ? PA_STATUS_NUMERIC_FAIL : 0; ri[2] = c->accept_n; ri[3] = c->accept_c;
+        ri[4] = c->fail;
     }
     int8_t *mz = A->maxZ + ((size_t)work * 3 + ms) * N;
     for (int i = tid; i < N; i += BS) mz[i] = c->maxZ[i];

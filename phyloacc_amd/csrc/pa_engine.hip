@@ -135,7 +135,7 @@ template <typename R>
 __device__ void branch_matrix(MP M, CP<R> c, int s, double d, double rate)
 {
     double P[16];
-    if (d > 1e-10) {
+    if (!(d <= 1e-10)) {          // a NaN length gives a NaN matrix, not the identity
         const double t = d * rate;
         double e[4];
 #pragma unroll
@@ -147,7 +147,7 @@ __device__ void branch_matrix(MP M, CP<R> c, int s, double d, double rate)
                 double v = 0;
 #pragma unroll
                 for (int k = 0; k < 4; k++) v += M->V[j * 4 + k] * e[k] * M->Vi[k * 4 + i];
-                P[j * 4 + i] = v > 0 ? v : 0.0;
+                P[j * 4 + i] = v < 0 ? 0.0 : v;
             }
     } else {
 #pragma unroll

@@ -25,6 +25,10 @@
 
 using namespace pa;
 
+static_assert(PA_F_NAN == PA_FLAG_NAN && PA_F_POS_INF == PA_FLAG_POS_INF && PA_F_NEG_INF == PA_FLAG_NEG_INF &&
+                  PA_F_SAMPLER == PA_FLAG_SAMPLER && PA_F_PROPOSAL_RANGE == PA_FLAG_PROPOSAL_RANGE,
+              "device failure bits match the C ABI");
+
 // ------------------------------------------------------------- .mod parse
 
 namespace {
@@ -144,6 +148,8 @@ extern "C" pa_status pa_parse_mod(const char *text, int32_t max_nodes, int32_t *
     Newick nw(tree);
     int root = nw.parse(0, end);
     if (nw.bad) return PA_E_INPUT;
+    for (auto &n : nw.nodes)
+        if (!std::isfinite(n.dist)) return PA_E_INPUT;   // lengths below 1e-10 (negative too) act as 0
     int S = 0;
     for (auto &n : nw.nodes) S += n.nk == 0;
     int N = 2 * S - 1;
@@ -1010,8 +1016,8 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
             B.n_work = 1;
             B.big_off = A.big_off ? A.big_off + i : nullptr;
             // per-element outputs shift with the work position
-            B.res_d = A.res_d ? A.res_d + (size_t)i * 24 : nullptr;
-            B.res_i = A.res_i ? A.res_i + (size_t)i * 12 : nullptr;
+            B.res_d = A.res_d ? A.res_d + (size_t)i * 3 * chain::RES_D : nullptr;
+            B.res_i = A.res_i ? A.res_i + (size_t)i * 3 * chain::RES_I : nullptr;
             B.maxZ = A.maxZ ? A.maxZ + (size_t)i * 3 * N : nullptr;
             B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * T : nullptr;
             B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * T * N : nullptr;
@@ -1219,8 +1225,8 @@ extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, 
                                     traces->lrate || traces->l2rate || traces->Z);
     const bool keep = cfg->keep_traces || want_tr;
     DevBuf bufs[8];
-    HIPCHK(e, hipMalloc(&bufs[3].p, (size_t)n * 3 * 8 * 8));
-    HIPCHK(e, hipMalloc(&bufs[4].p, (size_t)n * 3 * 4 * 4));
+    HIPCHK(e, hipMalloc(&bufs[3].p, (size_t)n * 3 * chain::RES_D * 8));
+    HIPCHK(e, hipMalloc(&bufs[4].p, (size_t)n * 3 * chain::RES_I * 4));
     HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 3 * N));
     if (keep) {
         HIPCHK(e, hipMalloc(&bufs[6].p, (size_t)n * 3 * 7 * T * 8));
@@ -1235,8 +1241,8 @@ extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, 
     A.tr_Z = keep ? (int8_t *)bufs[7].p : nullptr;
     st = run_chain_kernel(e, sorted.data(), n, cfg->precision == PA_FP64, cfg->mode, cfg->block_threads, A, bufs, stats);
     if (st != PA_OK) return st;
-    std::vector<double> rd((size_t)n * 24);
-    std::vector<int32_t> ri((size_t)n * 12);
+    std::vector<double> rd((size_t)n * 3 * chain::RES_D);
+    std::vector<int32_t> ri((size_t)n * 3 * chain::RES_I);
     std::vector<int8_t> mz((size_t)n * 3 * N);
     HIPCHK(e, hipMemcpy(rd.data(), bufs[3].p, rd.size() * 8, hipMemcpyDeviceToHost));
     HIPCHK(e, hipMemcpy(ri.data(), bufs[4].p, ri.size() * 4, hipMemcpyDeviceToHost));
@@ -1244,13 +1250,13 @@ extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, 
     for (int i = 0; i < n; i++) {
         const int dst = order[i];
         for (int k = 0; k < 3; k++) {
-            const double *d = &rd[((size_t)i * 3 + k) * 8];
-            const int32_t *q = &ri[((size_t)i * 3 + k) * 4];
+            const double *d = &rd[((size_t)i * 3 + k) * chain::RES_D];
+            const int32_t *q = &ri[((size_t)i * 3 + k) * chain::RES_I];
             if (results) {
                 pa_chain_result &r = results[(size_t)dst * 3 + k];
                 r.max_loglik = d[0]; r.final_nrate = d[1]; r.final_crate = d[2];
                 r.final_grate = d[3]; r.final_lrate = d[4]; r.final_lrate2 = d[5];
-                r.max_m = q[0]; r.status = q[1]; r.accept_n = q[2]; r.accept_c = q[3];
+                r.max_m = q[0]; r.status = q[1]; r.accept_n = q[2]; r.accept_c = q[3]; r.flags = q[4]; r.pad = 0;
             }
             if (traces && traces->max_Z) memcpy(traces->max_Z + ((size_t)dst * 3 + k) * N, &mz[((size_t)i * 3 + k) * N], N);
         }
@@ -1496,8 +1502,8 @@ extern "C" pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, in
         pa_run_stats rs;
         chain::ChainArgs A;
         memset(&A, 0, sizeof A);
-        if (hipMalloc(&bufs[3].p, (size_t)m * 3 * 8 * 8) != hipSuccess ||
-            hipMalloc(&bufs[4].p, (size_t)m * 3 * 4 * 4) != hipSuccess ||
+        if (hipMalloc(&bufs[3].p, (size_t)m * 3 * chain::RES_D * 8) != hipSuccess ||
+            hipMalloc(&bufs[4].p, (size_t)m * 3 * chain::RES_I * 4) != hipSuccess ||
             hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) { st = fail(e, PA_E_OOM, "autotune buffers"); break; }
         A.res_d = (double *)bufs[3].p;
         A.res_i = (int32_t *)bufs[4].p;
@@ -1549,8 +1555,8 @@ struct pa_server {
     size_t pool_bytes = 0, in_use = 0;
     char *d_pool = nullptr;                         // scratch of the launch in flight
     int32_t *d_ids = nullptr, *h_ids = nullptr;     // chain work list [max]
-    double *d_res_d = nullptr, *h_res_d = nullptr;  // [max][3][8]
-    int32_t *d_res_i = nullptr, *h_res_i = nullptr; // [max][3][4]
+    double *d_res_d = nullptr, *h_res_d = nullptr;  // [max][3][RES_D]
+    int32_t *d_res_i = nullptr, *h_res_i = nullptr; // [max][3][RES_I]
     int8_t *d_maxZ = nullptr;                       // [max][3][N]
     unsigned long long *d_cnt = nullptr;
     int32_t *d_lids = nullptr, *h_lids = nullptr;
@@ -1609,13 +1615,13 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     struct Alloc { pa_server *s; ~Alloc() { if (s) serve_free(s); } } guard{s.get()};
     HIPCHK(e, hipMalloc(&s->d_pool, s->pool_bytes));
     HIPCHK(e, hipMalloc(&s->d_ids, sizeof(int32_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_res_d, sizeof(double) * 24 * M));
-    HIPCHK(e, hipMalloc(&s->d_res_i, sizeof(int32_t) * 12 * M));
+    HIPCHK(e, hipMalloc(&s->d_res_d, sizeof(double) * 3 * chain::RES_D * M));
+    HIPCHK(e, hipMalloc(&s->d_res_i, sizeof(int32_t) * 3 * chain::RES_I * M));
     HIPCHK(e, hipMalloc(&s->d_maxZ, (size_t)3 * N * M));
     HIPCHK(e, hipMalloc(&s->d_cnt, 16));
     HIPCHK(e, hipHostMalloc(&s->h_ids, sizeof(int32_t) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_res_d, sizeof(double) * 24 * M));
-    HIPCHK(e, hipHostMalloc(&s->h_res_i, sizeof(int32_t) * 12 * M));
+    HIPCHK(e, hipHostMalloc(&s->h_res_d, sizeof(double) * 3 * chain::RES_D * M));
+    HIPCHK(e, hipHostMalloc(&s->h_res_i, sizeof(int32_t) * 3 * chain::RES_I * M));
     HIPCHK(e, hipMalloc(&s->d_bo, sizeof(int64_t) * M));
     HIPCHK(e, hipHostMalloc(&s->h_bo, sizeof(int64_t) * M));
     HIPCHK(e, hipMalloc(&s->d_lbo, sizeof(int64_t) * M));
@@ -1783,8 +1789,8 @@ static pa_status serve_launch(pa_server *s)
         A.counters = s->d_cnt;
         HIPCHK(e, chain::launch_reset(e->d_queue, e->stream));
         HIPCHK(e, chain::launch_chain(e->M, A, s->fp64, s->BS, gc, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->h_res_d, s->d_res_d, sizeof(double) * 24 * nc, hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->h_res_i, s->d_res_i, sizeof(int32_t) * 12 * nc, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->h_res_d, s->d_res_d, sizeof(double) * 3 * chain::RES_D * nc, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipMemcpyAsync(s->h_res_i, s->d_res_i, sizeof(int32_t) * 3 * chain::RES_I * nc, hipMemcpyDeviceToHost, e->stream));
     }
     if (nl) {
         for (int i = 0; i < nl; i++) {
@@ -1831,12 +1837,12 @@ static pa_status serve_retire(pa_server *s, int32_t *n_finished)
         out.arrival_s = r.arrival; out.admit_s = r.admit; out.finish_s = now;
         int bad = 0;
         for (int k = 0; k < 3; k++) {
-            const double *d = s->h_res_d + (i * 3 + k) * 8;
-            const int32_t *q = s->h_res_i + (i * 3 + k) * 4;
+            const double *d = s->h_res_d + (i * 3 + k) * chain::RES_D;
+            const int32_t *q = s->h_res_i + (i * 3 + k) * chain::RES_I;
             pa_chain_result &c = out.result[k];
             c.max_loglik = d[0]; c.final_nrate = d[1]; c.final_crate = d[2];
             c.final_grate = d[3]; c.final_lrate = d[4]; c.final_lrate2 = d[5];
-            c.max_m = q[0]; c.status = q[1]; c.accept_n = q[2]; c.accept_c = q[3];
+            c.max_m = q[0]; c.status = q[1]; c.accept_n = q[2]; c.accept_c = q[3]; c.flags = q[4]; c.pad = 0;
             bad |= q[1] != 0;
         }
         out.status = bad ? PA_ELEM_NUMERIC_FAIL : PA_ELEM_OK;

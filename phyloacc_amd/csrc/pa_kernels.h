@@ -9,6 +9,12 @@
 #define PA_GAP_CODE 4
 #define PA_OTHER_CODE 5
 #define PA_STATUS_NUMERIC_FAIL 2
+// chain failure and warning bits, as PA_FLAG_* in include/phyloacc_gpu.h
+#define PA_F_NAN 1
+#define PA_F_POS_INF 2
+#define PA_F_NEG_INF 4
+#define PA_F_SAMPLER 8
+#define PA_F_PROPOSAL_RANGE 16
 
 namespace pa {
 

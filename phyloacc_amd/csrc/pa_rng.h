@@ -77,7 +77,8 @@ struct Stream {
 
 #define PA_GAMMA_MAXIT 64
 
-PA_HD double gamma_draw(Stream &s, double a, double b)
+// capped (optional): set to 1 when the rejection loop stops at its cap
+PA_HD double gamma_draw(Stream &s, double a, double b, int *capped = nullptr)
 {
     double boost = 1.0;
     if (a < 1.0) {
@@ -96,6 +97,7 @@ PA_HD double gamma_draw(Stream &s, double a, double b)
         if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v * b * boost;
         if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v * b * boost;
     }
+    if (capped) *capped = 1;
     return d * b * boost;
 }
 

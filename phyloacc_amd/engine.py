@@ -24,6 +24,7 @@ PA_OK, PA_E_INPUT, PA_E_NUMERIC, PA_E_DEVICE, PA_E_OOM, PA_E_STATE = range(6)
 PA_FP64, PA_FP32 = 0, 1
 PA_RUN_BATCHED, PA_RUN_PER_ELEMENT = 0, 1
 PA_ELEM_OK, PA_ELEM_FILTERED, PA_ELEM_NUMERIC_FAIL = 0, 1, 2
+PA_FLAG_NAN, PA_FLAG_POS_INF, PA_FLAG_NEG_INF, PA_FLAG_SAMPLER, PA_FLAG_PROPOSAL_RANGE = 1, 2, 4, 8, 16
 
 EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_destroy", "pa_last_error",
            "pa_elements_add", "pa_element_get", "pa_loglik", "pa_run_chains", "pa_rng_uniform", "pa_device_count",
@@ -75,7 +76,7 @@ class ChainResult(C.Structure):
     _fields_ = [("max_loglik", C.c_double), ("max_m", C.c_int32), ("status", C.c_int32),
                 ("final_nrate", C.c_double), ("final_crate", C.c_double), ("final_grate", C.c_double),
                 ("final_lrate", C.c_double), ("final_lrate2", C.c_double),
-                ("accept_n", C.c_int32), ("accept_c", C.c_int32)]
+                ("accept_n", C.c_int32), ("accept_c", C.c_int32), ("flags", C.c_int32), ("pad", C.c_int32)]
 
 
 class Traces(C.Structure):
@@ -415,7 +416,7 @@ class Engine:
         for f, t in (("max_loglik", np.float64), ("max_m", np.int32), ("status", np.int32),
                      ("final_nrate", np.float64), ("final_crate", np.float64), ("final_grate", np.float64),
                      ("final_lrate", np.float64), ("final_lrate2", np.float64), ("accept_n", np.int32),
-                     ("accept_c", np.int32)):
+                     ("accept_c", np.int32), ("flags", np.int32)):
             out[f] = np.array([getattr(res[i], f) for i in range(3 * n)], t).reshape(n, 3)
         return out
 
