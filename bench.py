@@ -10,10 +10,24 @@ model chain of one element; value sums the three models.
 Also reported (same JSON line):
   roofline     - the chain kernel's counted algorithmic HBM bytes per launch
                  over its HIP-event duration, against 8 TB/s;
-  cpu_baseline - the CPU oracle (FP64, per site, OpenMP over elements) on a
-                 length-stratified sample of the same workload;
+  cpu_baseline - the CPU oracle (FP64, per site, OpenMP over elements, all
+                 the process's cores) on a length-stratified sample of the
+                 same workload;
   plain_gpu    - the plain FP64 path: one launch per element, one pattern
-                 per site, on a length-stratified sample.
+                 per site, on a length-stratified sample;
+  fp64_batched - the same workload and launch at FP64 partials;
+  reference_chain - 200 + 800-state chains (the reference's defaults, with
+                 the adaptive-proposal step at ADAPT_FREQ 500 inside the
+                 timed region) against 200-state chains on a length-stratified
+                 5,000-element subset: the per-sweep cost does not depend on
+                 the chain length, so the headline keeps 200-state chains;
+  config2      - the 5,000-element 43-species workload, batched FP32;
+  config4      - the continuous serving loop: config-2 elements arriving as a
+                 Poisson stream at 2,000/s with 2,000-state chains,
+                 throughput and p50/p99 completion latency;
+  config5      - 20,000 elements with their own 120-species gene trees:
+                 batched log-likelihood evaluations/s, FP32 and FP64 errors
+                 against the oracle on a sample.
 """
 from __future__ import annotations
 
@@ -67,6 +81,9 @@ def main():
     ap.add_argument("--arrival-rate", type=float, default=2000.0, help="config 4: requests per second")
     ap.add_argument("--chain", type=int, default=None, help="config 4: states per model chain (BURNIN = 1/4)")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0, help="config 4: scratch budget")
+    ap.add_argument("--lanes", type=int, default=0, help="config 4: launches in flight (0 = library default)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the fp64, reference-chain, config 2, 4 and 5 blocks of the default line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -81,9 +98,14 @@ def main():
     from phyloacc_amd import synth
 
     if args.config == 4:
-        return serve(args, E, synth)
+        print(json.dumps(serve(args, E, synth, args.elements, args.arrival_rate, args.chain, args.lanes)))
+        return
     if args.config == 5:
         return gene_trees(args, E, synth)
+    extra = args.config == 3 and not args.no_extra
+    # the gene-tree workload is simulated in forked workers, before this
+    # process touches the GPU
+    gt_w = make_gt(synth, args.seed, 20000) if extra else None
 
     t0 = time.time()
     w = synth.make_workload(args.config, seed=args.seed, n_elements=args.elements)
@@ -151,6 +173,17 @@ def main():
         peng.close()
         log(f"plain fp64 per-element: {prate:.1f} it/s")
 
+    blocks = {}
+    if extra:
+        blocks["fp64_batched"] = fp64_batched(E, eng, keep, sweeps_per_step, stats, value)
+        blocks["reference_chain"] = reference_chain(E, w, keep, eng, rng)
+        blocks["config2"] = config2_block(E, synth, args.seed)
+        blocks["config4"] = serve(args, E, synth, None, 2000.0, 2000, args.lanes)
+        blocks["config5"] = gene_trees_block(E, gt_w)
+        del gt_w
+        for k, v in blocks.items():
+            log(f"{k}: {v}")
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle_py as O
@@ -159,13 +192,15 @@ def main():
         oo = O.default_options(burnin=args.cpu_burnin, mcmc=args.cpu_mcmc)
         els = [O.Element(m, oo, w.align, w.starts[i], w.sizes[i]) for i in idx]
         els = [e for e in els if not e.filtered]
-        nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        nthr = args.cpu_threads or host["usable"]
         tc = time.time()
         n_sw, _ = O.run_batch(m, oo, els, nthr)
         dt = time.time() - tc
         mean_smp = float(np.mean([e.info()["GG"] for e in els]))
         crate = n_sw / dt * (mean_smp / float(np.mean(w.sizes)))
         cpu = {"value": round(crate, 2), "unit": "element-MCMC iterations/s", "cores": nthr, "kind": "port",
+               "nproc": host["nproc"], "cpu_model": host["model"], "core_limit": host["limit"],
                "sample": f"{len(els)} length-stratified elements, {args.cpu_burnin + args.cpu_mcmc}-state chains "
                f"(3 models), {dt:.1f}s; rate scaled by "
                "mean length (sample/population)"}
@@ -194,7 +229,80 @@ def main():
         "speedup_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
         "fp32_vs_fp64": agree,
     }
+    out.update(blocks)
     print(json.dumps(out))
+
+
+def host_cpu():
+    """Cores this process may use (its affinity set, capped by OMP_NUM_THREADS
+    when that is set: the host's CPU share), nproc and the CPU model."""
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    limit = os.environ.get("OMP_NUM_THREADS")
+    if limit and limit.isdigit() and int(limit) > 0:
+        usable = min(usable, int(limit))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"usable": usable, "nproc": os.cpu_count(), "model": model, "limit": limit}
+
+
+def make_gt(synth, seed, n):
+    t0 = time.time()
+    w = synth.make_gt_workload(seed=seed, n_elements=n, workers=host_cpu()["usable"])
+    log(f"gene-tree workload: {w.C} elements, S={w.tree.S}, {int(w.sizes.sum())} sites ({time.time() - t0:.1f}s)")
+    return w
+
+
+def fp64_batched(E, eng, keep, sweeps, stats32, value32):
+    """The headline workload and launch with FP64 partials (one step)."""
+    eng.run_chains(keep[:64], precision=E.PA_FP64)
+    r = eng.run_chains(keep, precision=E.PA_FP64, keep_traces=True)
+    v = sweeps / (r["elapsed_ms"] / 1e3)
+    return {"value": round(v, 1), "unit": "element-MCMC iterations/s", "ms_per_step": round(r["elapsed_ms"], 3),
+            "fp32_over_fp64": round(value32 / v, 3), "block_threads": r["stats"]["block_threads"],
+            "numeric_failures": int(np.sum(r["status"] != 0))}
+
+
+def reference_chain(E, w, keep, eng, rng):
+    """200 + 800-state chains with ADAPT_FREQ 500 (the reference's defaults)
+    against 200-state chains on the same length-stratified 5,000 elements."""
+    sizes = np.array([eng.element(int(i))["GG"] for i in keep])
+    sub = np.sort(keep[stratified(sizes, min(5000, len(keep)), rng)])
+    out = {"elements": int(len(sub))}
+    for burnin, mcmc in ((50, 150), (200, 800)):
+        opts = E.default_options(burnin=burnin, mcmc=mcmc)     # ADAPT_FREQ 500
+        e2 = E.Engine.from_workload(w, opts)
+        ids, st = e2.add_elements(w.align, w.starts[sub], w.sizes[sub])
+        ids = ids[st == E.PA_ELEM_OK]
+        e2.run_chains(ids[:64])
+        r = e2.run_chains(ids, keep_traces=True)
+        out[f"states_{burnin + mcmc}"] = round(len(ids) * 3 * (burnin + mcmc - 1) / (r["elapsed_ms"] / 1e3), 1)
+        e2.close()
+    out["ratio_1000_over_200"] = round(out["states_1000"] / out["states_200"], 4)
+    out["unit"] = "element-MCMC iterations/s"
+    return out
+
+
+def config2_block(E, synth, seed):
+    """The 5,000-element 43-species workload (LogNormal lengths), batched FP32
+    and FP64, 200-state chains."""
+    w = synth.make_workload(2, seed=seed)
+    eng = E.Engine.from_workload(w, E.default_options(burnin=50, mcmc=150))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    ids = ids[st == E.PA_ELEM_OK]
+    out = {"elements": int(len(ids)), "unit": "element-MCMC iterations/s"}
+    for name, prec in (("fp32", E.PA_FP32), ("fp64", E.PA_FP64)):
+        eng.run_chains(ids[:64], precision=prec)
+        r = eng.run_chains(ids, precision=prec, keep_traces=True)
+        out[name] = round(len(ids) * 3 * 199 / (r["elapsed_ms"] / 1e3), 1)
+    eng.close()
+    return out
 
 
 def fp32_vs_fp64(E, eng, keep, w, args, rng):
@@ -261,68 +369,15 @@ def copy_probe(args):
     print(json.dumps({"copy_probe_bytes_per_dispatch": nbytes, "dispatches": max(1, args.steps)}))
 
 
-def gene_trees(args, E, synth):
-    """Uneven-tree batched pruning: every element carries its own
-    multispecies-coalescent gene tree (120 species).  A step evaluates the
-    log-likelihood of every element at its gene-branch lengths in one launch.
-    Reports element log-likelihood evaluations per second, and the FP32 and
-    FP64 errors against the oracle's gene-tree pruning on a sample."""
-    from oracle import oracle_py as O
-    t0 = time.time()
-    n = args.elements or 20000
-    w = synth.make_gt_workload(seed=args.seed, n_elements=n)
-    log(f"gene-tree workload: {w.C} elements, S={w.tree.S}, {int(w.sizes.sum())} sites ({time.time() - t0:.1f}s)")
-    opts = E.default_options()
-    eng = E.Engine.from_workload(w, opts)
-    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
-    keep = st == E.PA_ELEM_OK
-    ids = ids[keep]
-    L = w.lengths()[keep]
-    prec = E.PA_FP32 if args.precision == "fp32" else E.PA_FP64
-    for _ in range(args.warmup):
-        eng.loglik_lengths(ids, L, prec)
-    times = []
-    for _ in range(args.steps):
-        tw = time.time()
-        eng.loglik_lengths(ids, L, prec)
-        times.append(time.time() - tw)
-    step = float(np.mean(times))
-    rng = np.random.default_rng(0)
-    smp = rng.choice(len(ids), size=min(48, len(ids)), replace=False)
-    m = O.parse_mod(w.mod_text)[0]
-    full = np.flatnonzero(keep)
-    ref = np.array([O.gt_loglik(m, w.tree.S, w.genes[full[i]].children, w.genes[full[i]].seg_gene,
-                                w.genes[full[i]].seg_species, w.genes[full[i]].seg_len * w.scale, w.Z[full[i]],
-                                w.r1[full[i]], w.r2[full[i]], w.align, w.starts[full[i]], w.sizes[full[i]],
-                                gap_prop=opts.gap_prop) for i in smp])
-    l32 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP32)
-    l64 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP64)
-    out = {
-        "metric": "gene-tree element log-likelihood evaluations/s (uneven per-element trees, batched)",
-        "value": round(len(ids) / step, 1), "unit": "element log-likelihoods/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 partials / fp64 sums" if prec == E.PA_FP32 else "fp64",
-        "data": "synthetic (seeded multispecies-coalescent gene trees)",
-        "config": {"workload": f"config5: S={w.tree.S} species tree in coalescent units, {len(ids)} elements with "
-                   "their own gene trees, lengths LogNormal(ln 300, 0.6)", "elements": int(len(ids)),
-                   "sites": int(w.sizes[keep].sum()), "parallelism": "1 GPU",
-                   "timing": "wall clock around the call, host-to-device transfer of the lengths included"},
-        "vs_oracle": {"sample": int(len(smp)),
-                      "fp64_max_rel_err": float(np.max(np.abs(l64 - ref) / np.abs(ref))),
-                      "fp32_max_rel_err": float(np.max(np.abs(l32 - ref) / np.abs(ref)))},
-    }
-    print(json.dumps(out))
-
-
-def serve(args, E, synth):
-    """Request loop: config-2 elements arrive as a Poisson stream; each step
-    of the server retires the launch that finished and launches every waiting
-    element that fits the HBM budget, running its three chains to the end.
-    Reports throughput and per-element completion latency (finish - arrival)."""
-    rate = args.arrival_rate
-    w = synth.make_workload(2, seed=args.seed, n_elements=args.elements)
-    burnin, mcmc = (args.burnin, args.mcmc) if args.chain is None else (args.chain // 4, args.chain - args.chain // 4)
+def serve(args, E, synth, n_elements, rate, chain, lanes):
+    """Continuous serving loop: config-2 elements arrive as a Poisson stream
+    at ``rate``/s, each running its three ``chain``-state model chains
+    (BURNIN = chain / 4).  The host loop submits arrivals as their time comes
+    and turns the server without waiting; every element retires as soon as
+    its own chains end.  Reports throughput and per-element completion
+    latency (finish - arrival)."""
+    w = synth.make_workload(2, seed=args.seed, n_elements=n_elements)
+    burnin, mcmc = (args.burnin, args.mcmc) if chain is None else (chain // 4, chain - chain // 4)
     opts = E.default_options(burnin=burnin, mcmc=mcmc)
     eng = E.Engine.from_workload(w, opts)
     ids, st = eng.add_elements(w.align, w.starts, w.sizes)
@@ -332,16 +387,16 @@ def serve(args, E, synth):
     rng = np.random.default_rng(args.seed + 1)
     order = rng.permutation(len(ids))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(ids)))
-    srv = E.Server(eng, precision=prec, block_threads=args.block or 128,
+    srv = E.Server(eng, precision=prec, block_threads=args.block or 128, lanes=lanes,
                    hbm_budget_bytes=args.hbm_budget_gib * 2 ** 30)
     # warm the kernel and the library once, outside the timed stream
     srv.submit(ids[order[:4]])
-    while srv.stats().finished < 4:
-        srv.step(wait=True)
+    srv.step(wait=True)
     srv.poll()
     base = srv.stats()
     done = []
     i = 0
+    peak_lanes = 0
     t0 = E.serve_clock()
     last_log = t0
     while len(done) < len(ids):
@@ -358,7 +413,11 @@ def serve(args, E, synth):
         if E.serve_clock() - last_log > 10:
             last_log = E.serve_clock()
             s = srv.stats()
-            log(f"serve: {len(done)}/{len(ids)} finished, {s.in_flight} in flight, {s.waiting} waiting")
+            peak_lanes = max(peak_lanes, s.lanes_busy)
+            log(f"serve: {len(done)}/{len(ids)} finished, {s.in_flight} in flight, {s.waiting} waiting, "
+                f"{s.lanes_busy} lanes busy")
+        else:
+            peak_lanes = max(peak_lanes, srv.stats().lanes_busy)
     t_end = max(r.finish_s for r in done)
     wall = t_end - t0
     lat = np.array([r.finish_s - r.arrival_s for r in done])
@@ -366,8 +425,9 @@ def serve(args, E, synth):
     sweeps = len(done) * 3 * (T - 1)
     bad = sum(1 for r in done if r.status != E.PA_ELEM_OK)
     srv.close()
-    out = {
-        "metric": "element-MCMC iterations/s (serving loop, batch per step)",
+    eng.close()
+    return {
+        "metric": "element-MCMC iterations/s (continuous serving loop)",
         "value": round(sweeps / wall, 1),
         "unit": "element-MCMC iterations/s",
         "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 3),
@@ -380,11 +440,72 @@ def serve(args, E, synth):
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "max": round(float(lat.max()), 4), "mean": round(float(lat.mean()), 4)},
         "serving": {"arrival_rate_per_s": rate, "offered_s": round(float(arrivals[-1]), 3),
-                    "launches": int(s.launches - base.launches),
+                    "launches": int(s.launches - base.launches), "peak_lanes_busy": int(peak_lanes),
                     "device_busy_frac": round((s.device_ms - base.device_ms) / 1e3 / wall, 4),
                     "numeric_failures": bad},
     }
+
+
+def gene_trees(args, E, synth):
+    """Config 5 alone: the gene-tree block as a JSON line of its own."""
+    from phyloacc_amd import synth as _s
+    w = make_gt(_s, args.seed, args.elements or 20000)
+    out = gene_trees_block(E, w, args.steps, args.warmup, args.precision)
+    out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None})
     print(json.dumps(out))
+
+
+def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32"):
+    """Uneven-tree batched pruning: every element carries its own
+    multispecies-coalescent gene tree (120 species).  A step evaluates the
+    log-likelihood of every element at its gene-branch lengths in one launch.
+    Reports element log-likelihood evaluations per second at both
+    precisions, and the FP32 and FP64 errors against the oracle's gene-tree
+    pruning on a sample."""
+    from oracle import oracle_py as O
+    opts = E.default_options()
+    eng = E.Engine.from_workload(w, opts)
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    keep = st == E.PA_ELEM_OK
+    ids = ids[keep]
+    L = w.lengths()[keep]
+    rates = {}
+    for name, prec in (("fp32", E.PA_FP32), ("fp64", E.PA_FP64)):
+        for _ in range(warmup):
+            eng.loglik_lengths(ids, L, prec)
+        times = []
+        for _ in range(steps):
+            tw = time.time()
+            eng.loglik_lengths(ids, L, prec)
+            times.append(time.time() - tw)
+        rates[name] = (len(ids) / float(np.mean(times)), float(np.mean(times)))
+    rng = np.random.default_rng(0)
+    smp = rng.choice(len(ids), size=min(48, len(ids)), replace=False)
+    m = O.parse_mod(w.mod_text)[0]
+    full = np.flatnonzero(keep)
+    ref = np.array([O.gt_loglik(m, w.tree.S, w.genes[full[i]].children, w.genes[full[i]].seg_gene,
+                                w.genes[full[i]].seg_species, w.genes[full[i]].seg_len * w.scale, w.Z[full[i]],
+                                w.r1[full[i]], w.r2[full[i]], w.align, w.starts[full[i]], w.sizes[full[i]],
+                                gap_prop=opts.gap_prop) for i in smp])
+    l32 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP32)
+    l64 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP64)
+    eng.close()
+    main = rates[precision]
+    return {
+        "metric": "gene-tree element log-likelihood evaluations/s (uneven per-element trees, batched)",
+        "value": round(main[0], 1), "unit": "element log-likelihoods/s", "ms_per_step": round(main[1] * 1e3, 3),
+        "fp64_value": round(rates["fp64"][0], 1),
+        "dtype": "fp32 partials / fp64 sums" if precision == "fp32" else "fp64",
+        "data": "synthetic (seeded multispecies-coalescent gene trees)",
+        "config": {"workload": f"config5: S={w.tree.S} species tree in coalescent units, {len(ids)} elements with "
+                   "their own gene trees, lengths LogNormal(ln 300, 0.6)", "elements": int(len(ids)),
+                   "sites": int(w.sizes[keep].sum()), "parallelism": "1 GPU",
+                   "timing": "wall clock around the call, host-to-device transfer of the lengths included"},
+        "vs_oracle": {"sample": int(len(smp)),
+                      "fp64_max_rel_err": float(np.max(np.abs(l64 - ref) / np.abs(ref))),
+                      "fp32_max_rel_err": float(np.max(np.abs(l32 - ref) / np.abs(ref)))},
+    }
 
 
 if __name__ == "__main__":

@@ -262,27 +262,32 @@ pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, const double
                            double *postZ);
 
 /* ------------------------------------------------------------------ serving
- * Request loop over element chains.  The reference has no serving mode: its
- * OpenMP loop (src/PhyloAcc-ST/main.cpp:452-500) runs a fixed id list to
- * completion.  Here requests (already encoded element ids) arrive at any time;
- * each pa_serve_step retires the launch that finished, admits waiting
- * requests in arrival order while the launch's scratch fits in the HBM budget,
- * and launches them: every admitted chain runs all three models to the end
- * within that launch.  A chain's result does not depend on what else is in
- * the launch: its random streams are keyed by SEED and the element, one
- * workgroup runs it, and the block size is fixed per server.  Resumable
- * chains (sweep_budget > 0: chain state kept on the device across launches)
- * are switched off in this build; pa_serve_create rejects them with
- * PA_E_STATE. */
+ * Continuous request loop over element chains.  The reference has no serving
+ * mode: its OpenMP loop (src/PhyloAcc-ST/main.cpp:452-500) runs a fixed id
+ * list to completion.  Here requests (already encoded element ids) arrive at
+ * any time and wait in arrival order.  Up to `lanes` launches are in flight
+ * at once, each on its own stream and holding a range of the server's HBM
+ * pool (hbm_budget_bytes) while it runs; a free lane takes the waiting
+ * requests that fit the largest free range, up to max_chains.  A lane is
+ * started when nothing is in flight, or when fewer chains are in flight than
+ * the chain kernel has resident workgroups on the device (retiring chains
+ * have freed slots).  Every element retires on its own: the kernel writes its
+ * three result rows into host-visible memory and then a completion flag, and
+ * each pa_serve_step collects the elements whose flag is set.  A chain's
+ * result does not depend on the lane or on what else is in flight: its
+ * random streams are keyed by SEED and the element, one workgroup runs it,
+ * and the block size is fixed per server. */
 typedef struct pa_server pa_server;
 
 typedef struct {
     int32_t precision;          /* PA_FP64 / PA_FP32 */
     int32_t block_threads;      /* 64, 128 or 256; 0 = 128 */
-    int32_t sweep_budget;       /* must be 0: each chain runs to completion in its launch */
+    int32_t lanes;              /* launches in flight at once, 1..PA_SERVE_MAX_LANES; 0 = PA_SERVE_DEFAULT_LANES */
     int32_t max_chains;         /* chains in flight at most; 0 = 4096 */
-    double hbm_budget_bytes;    /* scratch of one launch; 0 = 4 GiB */
+    double hbm_budget_bytes;    /* scratch pool shared by the lanes' launches; 0 = 4 GiB */
 } pa_serve_config;
+
+enum { PA_SERVE_MAX_LANES = 8, PA_SERVE_DEFAULT_LANES = 3 };
 
 enum { PA_SERVE_CHAINS = 0, PA_SERVE_LOGLIK = 1 };
 
@@ -299,8 +304,9 @@ typedef struct {
 
 typedef struct {
     int64_t submitted, admitted, finished, launches;
-    int32_t waiting, in_flight;
+    int32_t waiting, in_flight;            /* in_flight: admitted, not yet retired */
     double scratch_bytes_in_use, device_ms;
+    int32_t lanes_busy, pad;
 } pa_serve_stats;
 
 pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **out);
@@ -312,14 +318,15 @@ pa_status pa_serve_submit(pa_server *s, const int32_t *ids, int32_t n, const dou
  * step prunes the admitted ones in one batched launch after the chains. */
 pa_status pa_serve_submit_lengths(pa_server *s, const int32_t *ids, int32_t n, const double *lengths,
                                   const double *arrival_s, int64_t *tickets);
-/* One turn of the request loop.  With wait = 0 it returns at once while the
- * current launch is still running (n_finished = 0); with wait = 1 it also
- * waits for the launch it starts and retires it. */
+/* One turn of the request loop: collects the elements that finished, then
+ * starts lanes for waiting requests; with wait = 0 it returns at once, with
+ * wait = 1 it keeps turning until nothing is in flight (every admissible
+ * request has finished). */
 pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finished);
 /* Moves up to max finished results out of the server, oldest first. */
 pa_status pa_serve_poll(pa_server *s, pa_serve_result *out, int32_t max, int32_t *n);
 pa_status pa_serve_get_stats(pa_server *s, pa_serve_stats *out);
-/* Waits for the launch in flight, then frees the server. */
+/* Waits for the launches in flight, then frees the server. */
 void pa_serve_destroy(pa_server *s);
 /* Monotonic host clock in seconds. */
 double pa_serve_clock(void);

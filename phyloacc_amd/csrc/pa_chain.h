@@ -38,6 +38,10 @@ struct ChainArgs {
     double *tr_d;             // [n_work][3][7][T] or null
     int8_t *tr_Z;             // [n_work][3][T][N] or null
     unsigned long long *counters;   // [0]: algorithmic HBM bytes of the launch (or null)
+    // Per-element completion flags [n_work] in host-visible memory, or null:
+    // done[w] becomes 1 once work position w's three result rows are written
+    // (res_d and res_i then live in host-visible memory too).
+    int32_t *done;
 };
 
 // LDS bytes of one workgroup (dynamic arrays + context + model constants).

@@ -1269,6 +1269,12 @@ __global__ void __launch_bounds__(BS, 4) chain_kernel(ModelDev Mk, ChainArgs A)
         bind_element<R, BS>(M, c, &A, w);
         for (int ms = 0; ms < 3; ms++) run_model<R, BS>(M, c, &A, ms, w);
         if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes);
+        if (threadIdx.x == 0 && A.done) {
+            // thread 0 wrote the result rows: make them visible to the host
+            // before the flag that announces them
+            __threadfence_system();
+            __hip_atomic_store(A.done + w, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 

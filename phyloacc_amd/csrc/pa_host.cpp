@@ -790,7 +790,7 @@ static pa_status upload(pa_engine *e)
     if (st != PA_OK) return st;
     HIPCHK(e, hipSetDevice(e->device));
     if (!e->dev_dirty) return PA_OK;
-    HIPCHK(e, hipStreamSynchronize(e->stream));   // no launch may still read the old arenas
+    HIPCHK(e, hipDeviceSynchronize());   // no launch on any stream (serving lanes too) may still read the old arenas
     free_arenas(e);
     const int S = e->tm.S, N = e->tm.N;
     std::vector<ElemDev> d(e->elems.size());
@@ -1522,15 +1522,22 @@ extern "C" pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, in
 
 // ------------------------------------------------------------ serving loop
 //
-// Batch-per-step request loop.  Each pa_serve_step retires the launch in
-// flight (if it finished), then admits waiting requests in arrival order
-// while one launch's scratch fits the HBM budget, and launches them: the
-// chain kernel over the admitted chain requests, then the log-likelihood
-// kernel over the admitted own-tree requests, on the engine's stream, with
-// the results copied into pinned host memory behind them.  Every admitted
-// chain runs to completion within its launch.  The block size is fixed per
-// server and each element's chain is run by one workgroup from its own
-// random streams, so a result does not depend on what else is in the batch.
+// Continuous request loop.  Requests wait in arrival order.  Up to `lanes`
+// launches are in flight at once, each on a stream of its own with its own
+// work counter, and each holds a range of the server's HBM pool (first fit)
+// for its scratch while it runs.  A free lane takes the waiting requests that
+// fit the largest free range (up to max_chains) and launches the chain kernel
+// over them, then the log-likelihood kernel over the waiting own-tree
+// requests.  Elements retire one by one: the kernel writes an element's
+// result rows into host-visible memory and then its completion flag, and
+// every step collects the flags that are set, so a short chain does not wait
+// for the longest one of its launch.  A launch ends with its last element;
+// its pool range is then free for the next admission while the other lanes
+// keep running, so requests are admitted while earlier launches still occupy
+// the device and the hardware gives their workgroups the slots that the
+// retiring launches free.  Each element's chain is run by one workgroup from
+// its own random streams and the block size is fixed per server, so a result
+// does not depend on the lane or on what else is in flight.
 
 namespace {
 
@@ -1547,30 +1554,96 @@ struct LRequest {
     std::vector<double> len;
 };
 
+// First-fit allocator over [0, cap) in 256-byte units; free ranges coalesce.
+struct PoolMap {
+    size_t cap = 0, used = 0;
+    std::map<size_t, size_t> free_;   // offset -> length
+    static size_t round(size_t n) { return (n + 255) & ~(size_t)255; }
+    void init(size_t c)
+    {
+        cap = c & ~(size_t)255;
+        used = 0;
+        free_.clear();
+        if (cap) free_[0] = cap;
+    }
+    size_t largest() const
+    {
+        size_t m = 0;
+        for (const auto &f : free_) m = std::max(m, f.second);
+        return m;
+    }
+    bool alloc(size_t n, size_t *off)
+    {
+        n = round(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            const size_t o = it->first, rest = it->second - n;
+            free_.erase(it);
+            if (rest) free_[o + n] = rest;
+            *off = o;
+            used += n;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = round(n);
+        if (!n) return;
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+// One launch slot of the server: a stream, a work counter, work lists and
+// result rows for up to max_chains requests of each kind.
+struct Lane {
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int *d_queue = nullptr;
+    bool busy = false;
+    size_t pool_off = 0, pool_len = 0;
+    int32_t *d_ids = nullptr, *h_ids = nullptr;     // chain work list
+    int64_t *d_bo = nullptr, *h_bo = nullptr;       // long elements' own regions
+    double *h_res_d = nullptr, *v_res_d = nullptr;  // [max][3][RES_D], host-visible (h_) and its device view (v_)
+    int32_t *h_res_i = nullptr, *v_res_i = nullptr; // [max][3][RES_I]
+    int32_t *h_done = nullptr, *v_done = nullptr;   // [max] completion flags
+    int8_t *d_maxZ = nullptr;                       // [max][3][N]
+    int32_t *d_lids = nullptr, *h_lids = nullptr;
+    int64_t *d_lbo = nullptr, *h_lbo = nullptr;
+    double *d_llen = nullptr, *h_llen = nullptr, *d_lout = nullptr, *h_lout = nullptr;
+    std::vector<Request> batch;                     // launch order
+    std::vector<LRequest> lbatch;
+    std::vector<uint8_t> retired;
+    int n_retired = 0;
+};
+
 }  // namespace
 
 struct pa_server {
     pa_engine *e = nullptr;
     int fp64 = 0, BS = 128, max_chains = 4096;
-    size_t pool_bytes = 0, in_use = 0;
-    char *d_pool = nullptr;                         // scratch of the launch in flight
-    int32_t *d_ids = nullptr, *h_ids = nullptr;     // chain work list [max]
-    double *d_res_d = nullptr, *h_res_d = nullptr;  // [max][3][RES_D]
-    int32_t *d_res_i = nullptr, *h_res_i = nullptr; // [max][3][RES_I]
-    int8_t *d_maxZ = nullptr;                       // [max][3][N]
-    unsigned long long *d_cnt = nullptr;
-    int32_t *d_lids = nullptr, *h_lids = nullptr;
-    int64_t *d_bo = nullptr, *h_bo = nullptr, *d_lbo = nullptr, *h_lbo = nullptr;   // long elements' own regions
-    std::vector<int64_t> big_off_c, big_off_l;
-    double *d_llen = nullptr, *h_llen = nullptr, *d_lout = nullptr, *h_lout = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool in_flight = false;
+    size_t pool_bytes = 0;
+    char *d_pool = nullptr;
+    PoolMap pool;
+    std::vector<Lane> lanes;
     std::deque<Request> waiting;
     std::deque<LRequest> lwait;
-    std::vector<Request> batch;                     // in flight, launch order
-    std::vector<LRequest> lbatch;
     std::deque<pa_serve_result> finished;
     int64_t next_ticket = 0;
+    int in_flight = 0;                              // admitted, not yet retired
     pa_serve_stats st{};
 };
 
@@ -1579,25 +1652,61 @@ extern "C" double pa_serve_clock(void)
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+static void lane_free(Lane &L)
+{
+    if (L.st) (void)hipStreamSynchronize(L.st);
+    (void)hipFree(L.d_queue); (void)hipFree(L.d_ids); (void)hipFree(L.d_bo); (void)hipFree(L.d_maxZ);
+    (void)hipFree(L.d_lids); (void)hipFree(L.d_lbo); (void)hipFree(L.d_llen); (void)hipFree(L.d_lout);
+    (void)hipHostFree(L.h_ids); (void)hipHostFree(L.h_bo); (void)hipHostFree(L.h_res_d); (void)hipHostFree(L.h_res_i);
+    (void)hipHostFree(L.h_done); (void)hipHostFree(L.h_lids); (void)hipHostFree(L.h_lbo); (void)hipHostFree(L.h_llen);
+    (void)hipHostFree(L.h_lout);
+    if (L.ev0) (void)hipEventDestroy(L.ev0);
+    if (L.ev1) (void)hipEventDestroy(L.ev1);
+    if (L.st) (void)hipStreamDestroy(L.st);
+    L = Lane();
+}
+
 static void serve_free(pa_server *s)
 {
-    (void)hipFree(s->d_pool); (void)hipFree(s->d_ids); (void)hipFree(s->d_res_d); (void)hipFree(s->d_res_i);
-    (void)hipFree(s->d_maxZ); (void)hipFree(s->d_cnt); (void)hipFree(s->d_lids);
-    (void)hipFree(s->d_llen); (void)hipFree(s->d_lout); (void)hipFree(s->d_bo); (void)hipFree(s->d_lbo);
-    (void)hipHostFree(s->h_bo); (void)hipHostFree(s->h_lbo);
-    (void)hipHostFree(s->h_ids); (void)hipHostFree(s->h_res_d); (void)hipHostFree(s->h_res_i);
-    (void)hipHostFree(s->h_lids); (void)hipHostFree(s->h_llen); (void)hipHostFree(s->h_lout);
-    if (s->ev0) (void)hipEventDestroy(s->ev0);
-    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    for (Lane &L : s->lanes) lane_free(L);
+    (void)hipFree(s->d_pool);
+    s->d_pool = nullptr;
+}
+
+static pa_status lane_alloc(pa_engine *e, Lane &L, size_t M, int N)
+{
+    const unsigned vis = hipHostMallocMapped | hipHostMallocCoherent;
+    HIPCHK(e, hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+    HIPCHK(e, hipEventCreate(&L.ev0));
+    HIPCHK(e, hipEventCreate(&L.ev1));
+    HIPCHK(e, hipMalloc(&L.d_queue, 256));
+    HIPCHK(e, hipMalloc(&L.d_ids, sizeof(int32_t) * M));
+    HIPCHK(e, hipMalloc(&L.d_bo, sizeof(int64_t) * M));
+    HIPCHK(e, hipMalloc(&L.d_maxZ, (size_t)3 * N * M));
+    HIPCHK(e, hipMalloc(&L.d_lids, sizeof(int32_t) * M));
+    HIPCHK(e, hipMalloc(&L.d_lbo, sizeof(int64_t) * M));
+    HIPCHK(e, hipMalloc(&L.d_llen, sizeof(double) * M * N));
+    HIPCHK(e, hipMalloc(&L.d_lout, sizeof(double) * M));
+    HIPCHK(e, hipHostMalloc(&L.h_ids, sizeof(int32_t) * M));
+    HIPCHK(e, hipHostMalloc(&L.h_bo, sizeof(int64_t) * M));
+    HIPCHK(e, hipHostMalloc(&L.h_res_d, sizeof(double) * 3 * chain::RES_D * M, vis));
+    HIPCHK(e, hipHostMalloc(&L.h_res_i, sizeof(int32_t) * 3 * chain::RES_I * M, vis));
+    HIPCHK(e, hipHostMalloc(&L.h_done, sizeof(int32_t) * M, vis));
+    HIPCHK(e, hipHostGetDevicePointer((void **)&L.v_res_d, L.h_res_d, 0));
+    HIPCHK(e, hipHostGetDevicePointer((void **)&L.v_res_i, L.h_res_i, 0));
+    HIPCHK(e, hipHostGetDevicePointer((void **)&L.v_done, L.h_done, 0));
+    HIPCHK(e, hipHostMalloc(&L.h_lids, sizeof(int32_t) * M));
+    HIPCHK(e, hipHostMalloc(&L.h_lbo, sizeof(int64_t) * M));
+    HIPCHK(e, hipHostMalloc(&L.h_llen, sizeof(double) * M * N));
+    HIPCHK(e, hipHostMalloc(&L.h_lout, sizeof(double) * M));
+    return PA_OK;
 }
 
 extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, pa_server **out)
 {
     if (out) *out = nullptr;
     if (!e || !cfg || !out) return fail(e, PA_E_INPUT, "null argument");
-    if (cfg->sweep_budget != 0)
-        return fail(e, PA_E_STATE, "resumable chains (sweep_budget > 0) are switched off in this build; "
-                                   "use sweep_budget = 0 (each chain runs to completion in its launch)");
+    if (cfg->lanes < 0 || cfg->lanes > PA_SERVE_MAX_LANES) return fail(e, PA_E_INPUT, "lanes must be 0..PA_SERVE_MAX_LANES");
     pa_status st = ensure_device(e);
     if (st != PA_OK) return st;
     HIPCHK(e, hipSetDevice(e->device));
@@ -1614,26 +1723,10 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     const size_t M = (size_t)s->max_chains;
     struct Alloc { pa_server *s; ~Alloc() { if (s) serve_free(s); } } guard{s.get()};
     HIPCHK(e, hipMalloc(&s->d_pool, s->pool_bytes));
-    HIPCHK(e, hipMalloc(&s->d_ids, sizeof(int32_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_res_d, sizeof(double) * 3 * chain::RES_D * M));
-    HIPCHK(e, hipMalloc(&s->d_res_i, sizeof(int32_t) * 3 * chain::RES_I * M));
-    HIPCHK(e, hipMalloc(&s->d_maxZ, (size_t)3 * N * M));
-    HIPCHK(e, hipMalloc(&s->d_cnt, 16));
-    HIPCHK(e, hipHostMalloc(&s->h_ids, sizeof(int32_t) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_res_d, sizeof(double) * 3 * chain::RES_D * M));
-    HIPCHK(e, hipHostMalloc(&s->h_res_i, sizeof(int32_t) * 3 * chain::RES_I * M));
-    HIPCHK(e, hipMalloc(&s->d_bo, sizeof(int64_t) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_bo, sizeof(int64_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_lbo, sizeof(int64_t) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_lbo, sizeof(int64_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_lids, sizeof(int32_t) * M));
-    HIPCHK(e, hipMalloc(&s->d_llen, sizeof(double) * M * N));
-    HIPCHK(e, hipMalloc(&s->d_lout, sizeof(double) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_lids, sizeof(int32_t) * M));
-    HIPCHK(e, hipHostMalloc(&s->h_llen, sizeof(double) * M * N));
-    HIPCHK(e, hipHostMalloc(&s->h_lout, sizeof(double) * M));
-    HIPCHK(e, hipEventCreate(&s->ev0));
-    HIPCHK(e, hipEventCreate(&s->ev1));
+    s->pool.init(s->pool_bytes);
+    s->lanes.resize(cfg->lanes ? cfg->lanes : PA_SERVE_DEFAULT_LANES);
+    for (Lane &L : s->lanes)
+        if ((st = lane_alloc(e, L, M, N)) != PA_OK) return st;
     guard.s = nullptr;
     *out = s.release();
     return PA_OK;
@@ -1694,178 +1787,238 @@ static size_t loglik_big(pa_server *s, int ppad)
     return ppad > SERVE_SLOT_PATTERNS ? loglik_slot_bytes(s->e->tm.N, s->e->tm.S, ppad, s->fp64) : 0;
 }
 
-// Admits waiting requests (arrival order) and launches them; no-op when
-// nothing waits.  Requests whose scratch alone exceeds the budget finish at
-// once with status PA_ELEM_NUMERIC_FAIL and an out-of-budget result.
-static pa_status serve_launch(pa_server *s)
+static void finish_unfit(pa_server *s, int64_t ticket, int32_t eid, int kind, double arrival, double now)
+{
+    pa_serve_result out;
+    memset(&out, 0, sizeof out);
+    out.ticket = ticket; out.element = eid; out.status = PA_ELEM_NUMERIC_FAIL; out.kind = kind;
+    out.arrival_s = arrival; out.admit_s = now; out.finish_s = now;
+    s->finished.push_back(out);
+    s->st.finished++;
+}
+
+// Admits waiting requests (arrival order) into free lane L within the pool's
+// largest free range and launches them.  Requests whose scratch alone exceeds
+// the whole pool finish at once with status PA_ELEM_NUMERIC_FAIL.
+static pa_status lane_launch(pa_server *s, Lane &L, bool others_busy)
 {
     pa_engine *e = s->e;
-    if (s->waiting.empty() && s->lwait.empty()) return PA_OK;
-    pa_status st = upload(e);
-    if (st != PA_OK) return st;
     const int N = e->tm.N, S = e->tm.S;
     const int gmax_c = chain_grid_max(e, s->fp64, s->BS);
     const int gmax_l = loglik_grid_max(e, s->fp64, s->BS);
+    const size_t room = s->pool.largest();
     const double now = pa_serve_clock();
     // chain requests: slots for min(k, grid) workgroups plus the long elements' own regions
     size_t bigC = 0;
-    s->batch.clear();
-    std::vector<int64_t> &boC = s->big_off_c;
-    boC.clear();
-    while (!s->waiting.empty() && (int)s->batch.size() < s->max_chains) {
+    L.batch.clear();
+    std::vector<int64_t> boC;
+    while (!s->waiting.empty() && (int)L.batch.size() < s->max_chains) {
         const Request &r = s->waiting.front();
         const size_t b = chain_big(s, e->elems[r.eid].Ppad);
-        const int k = (int)s->batch.size() + 1;
-        if (chain_slot(s) * (size_t)std::min(k, gmax_c) + bigC + b > s->pool_bytes) {
-            if (!s->batch.empty()) break;
-            // cannot run even alone
-            pa_serve_result out;
-            memset(&out, 0, sizeof out);
-            out.ticket = r.ticket; out.element = r.eid; out.status = PA_ELEM_NUMERIC_FAIL; out.kind = PA_SERVE_CHAINS;
-            out.arrival_s = r.arrival; out.admit_s = now; out.finish_s = now;
-            s->finished.push_back(out);
-            s->st.finished++;
+        const int k = (int)L.batch.size() + 1;
+        const size_t need = PoolMap::round(chain_slot(s) * (size_t)std::min(k, gmax_c) + bigC + b);
+        if (need > room) {
+            if (!L.batch.empty() || others_busy || need <= s->pool.cap) break;   // waits for room
+            finish_unfit(s, r.ticket, r.eid, PA_SERVE_CHAINS, r.arrival, now);    // cannot run even alone
             s->waiting.pop_front();
             continue;
         }
         boC.push_back(b ? (int64_t)bigC : -1);
         bigC += b;
-        s->batch.push_back(r);
-        s->batch.back().admit = now;
+        L.batch.push_back(r);
+        L.batch.back().admit = now;
         s->waiting.pop_front();
     }
-    const int nc = (int)s->batch.size();
+    const int nc = (int)L.batch.size();
     const int gc = std::min(nc, gmax_c);
     const size_t slotsC = nc ? chain_slot(s) * (size_t)gc : 0;
-    const size_t scr_c = nc ? ((slotsC + bigC + 255) & ~(size_t)255) : 0;
-    // log-likelihood requests, in the pool after the chains' scratch
+    const size_t scr_c = nc ? PoolMap::round(slotsC + bigC) : 0;
+    // log-likelihood requests, after the chains' scratch
     size_t bigL = 0;
-    s->lbatch.clear();
-    std::vector<int64_t> &boL = s->big_off_l;
-    boL.clear();
-    while (!s->lwait.empty() && (int)s->lbatch.size() < s->max_chains) {
+    L.lbatch.clear();
+    std::vector<int64_t> boL;
+    while (!s->lwait.empty() && (int)L.lbatch.size() < s->max_chains) {
         const LRequest &r = s->lwait.front();
         const size_t b = loglik_big(s, e->elems[r.eid].Ppad);
-        const int k = (int)s->lbatch.size() + 1;
-        if (scr_c + loglik_slot(s) * (size_t)std::min(k, gmax_l) + bigL + b > s->pool_bytes) {
-            if (!s->lbatch.empty() || nc) break;
-            pa_serve_result out;
-            memset(&out, 0, sizeof out);
-            out.ticket = r.ticket; out.element = r.eid; out.status = PA_ELEM_NUMERIC_FAIL; out.kind = PA_SERVE_LOGLIK;
-            out.arrival_s = r.arrival; out.admit_s = now; out.finish_s = now;
-            s->finished.push_back(out);
-            s->st.finished++;
+        const int k = (int)L.lbatch.size() + 1;
+        const size_t alone = PoolMap::round(loglik_slot(s) + b);
+        if (scr_c + PoolMap::round(loglik_slot(s) * (size_t)std::min(k, gmax_l) + bigL + b) > room) {
+            if (!L.lbatch.empty() || nc || others_busy || alone <= s->pool.cap) break;
+            finish_unfit(s, r.ticket, r.eid, PA_SERVE_LOGLIK, r.arrival, now);
             s->lwait.pop_front();
             continue;
         }
         boL.push_back(b ? (int64_t)bigL : -1);
         bigL += b;
-        s->lbatch.push_back(std::move(s->lwait.front()));
-        s->lbatch.back().admit = now;
+        L.lbatch.push_back(std::move(s->lwait.front()));
+        L.lbatch.back().admit = now;
         s->lwait.pop_front();
     }
-    const int nl = (int)s->lbatch.size();
+    const int nl = (int)L.lbatch.size();
     const int gl = std::min(nl, gmax_l);
     const size_t slotsL = nl ? loglik_slot(s) * (size_t)gl : 0;
     if (nc == 0 && nl == 0) return PA_OK;
-    HIPCHK(e, hipEventRecord(s->ev0, e->stream));
+    L.pool_len = scr_c + (nl ? PoolMap::round(slotsL + bigL) : 0);
+    if (!s->pool.alloc(L.pool_len, &L.pool_off))   // cannot happen: the batch was sized to the largest range
+        return fail(e, PA_E_STATE, "serving pool allocation failed");
+    char *base = s->d_pool + L.pool_off;
+    L.retired.assign(nc, 0);
+    L.n_retired = 0;
+    HIPCHK(e, hipEventRecord(L.ev0, L.st));
     if (nc) {
-        for (int i = 0; i < nc; i++) s->h_ids[i] = s->batch[i].eid;
-        memcpy(s->h_bo, boC.data(), sizeof(int64_t) * nc);
-        HIPCHK(e, hipMemcpyAsync(s->d_ids, s->h_ids, sizeof(int32_t) * nc, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->d_bo, s->h_bo, sizeof(int64_t) * nc, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, hipMemsetAsync(s->d_cnt, 0, 16, e->stream));
+        for (int i = 0; i < nc; i++) L.h_ids[i] = L.batch[i].eid;
+        memcpy(L.h_bo, boC.data(), sizeof(int64_t) * nc);
+        memset(L.h_done, 0, sizeof(int32_t) * nc);   // the host owns the flags until the launch below
+        HIPCHK(e, hipMemcpyAsync(L.d_ids, L.h_ids, sizeof(int32_t) * nc, hipMemcpyHostToDevice, L.st));
+        HIPCHK(e, hipMemcpyAsync(L.d_bo, L.h_bo, sizeof(int64_t) * nc, hipMemcpyHostToDevice, L.st));
         chain::ChainArgs A;
         memset(&A, 0, sizeof A);
         size_t off[4];
         const size_t slot = chain::slot_scratch_bytes(N, S, SERVE_SLOT_PATTERNS, s->fp64, off);
         A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
-        A.work = s->d_ids; A.n_work = nc; A.queue = e->d_queue;
-        A.scratch = s->d_pool; A.slot_bytes = (int64_t)slot;
+        A.work = L.d_ids; A.n_work = nc; A.queue = L.d_queue;
+        A.scratch = base; A.slot_bytes = (int64_t)slot;
         A.off_plin = off[0]; A.off_logt = off[1]; A.off_amb = off[2]; A.off_parts = off[3];
-        A.big = s->d_pool + slotsC;
-        A.big_off = bigC ? s->d_bo : nullptr;
-        A.res_d = s->d_res_d; A.res_i = s->d_res_i; A.maxZ = s->d_maxZ;
-        A.counters = s->d_cnt;
-        HIPCHK(e, chain::launch_reset(e->d_queue, e->stream));
-        HIPCHK(e, chain::launch_chain(e->M, A, s->fp64, s->BS, gc, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->h_res_d, s->d_res_d, sizeof(double) * 3 * chain::RES_D * nc, hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->h_res_i, s->d_res_i, sizeof(int32_t) * 3 * chain::RES_I * nc, hipMemcpyDeviceToHost, e->stream));
+        A.big = base + slotsC;
+        A.big_off = bigC ? L.d_bo : nullptr;
+        A.res_d = L.v_res_d; A.res_i = L.v_res_i; A.maxZ = L.d_maxZ;
+        A.done = L.v_done;
+        HIPCHK(e, chain::launch_reset(L.d_queue, L.st));
+        HIPCHK(e, chain::launch_chain(e->M, A, s->fp64, s->BS, gc, L.st));
     }
     if (nl) {
         for (int i = 0; i < nl; i++) {
-            s->h_lids[i] = s->lbatch[i].eid;
-            memcpy(s->h_llen + (size_t)i * N, s->lbatch[i].len.data(), sizeof(double) * N);
+            L.h_lids[i] = L.lbatch[i].eid;
+            memcpy(L.h_llen + (size_t)i * N, L.lbatch[i].len.data(), sizeof(double) * N);
         }
-        memcpy(s->h_lbo, boL.data(), sizeof(int64_t) * nl);
-        HIPCHK(e, hipMemcpyAsync(s->d_lids, s->h_lids, sizeof(int32_t) * nl, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->d_lbo, s->h_lbo, sizeof(int64_t) * nl, hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->d_llen, s->h_llen, sizeof(double) * (size_t)nl * N, hipMemcpyHostToDevice, e->stream));
+        memcpy(L.h_lbo, boL.data(), sizeof(int64_t) * nl);
+        HIPCHK(e, hipMemcpyAsync(L.d_lids, L.h_lids, sizeof(int32_t) * nl, hipMemcpyHostToDevice, L.st));
+        HIPCHK(e, hipMemcpyAsync(L.d_lbo, L.h_lbo, sizeof(int64_t) * nl, hipMemcpyHostToDevice, L.st));
+        HIPCHK(e, hipMemcpyAsync(L.d_llen, L.h_llen, sizeof(double) * (size_t)nl * N, hipMemcpyHostToDevice, L.st));
         LoglikArgs A;
         memset(&A, 0, sizeof A);
         A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
         A.n_elems = (int)e->elems.size();
-        A.work = s->d_lids; A.n_work = nl; A.queue = e->d_queue;
-        A.scratch = s->d_pool + scr_c; A.slot_bytes = (int64_t)loglik_slot(s);
-        A.big = s->d_pool + scr_c + slotsL;
-        A.big_off = bigL ? s->d_lbo : nullptr;
-        A.ll_out = s->d_lout; A.len_in = s->d_llen;
-        HIPCHK(e, launch_reset(e->d_queue, e->stream));
-        HIPCHK(e, launch_loglik(e->M, A, s->fp64, s->BS, gl, e->stream));
-        HIPCHK(e, hipMemcpyAsync(s->h_lout, s->d_lout, sizeof(double) * nl, hipMemcpyDeviceToHost, e->stream));
+        A.work = L.d_lids; A.n_work = nl; A.queue = L.d_queue;
+        A.scratch = base + scr_c; A.slot_bytes = (int64_t)loglik_slot(s);
+        A.big = base + scr_c + slotsL;
+        A.big_off = bigL ? L.d_lbo : nullptr;
+        A.ll_out = L.d_lout; A.len_in = L.d_llen;
+        HIPCHK(e, launch_reset(L.d_queue, L.st));
+        HIPCHK(e, launch_loglik(e->M, A, s->fp64, s->BS, gl, L.st));
+        HIPCHK(e, hipMemcpyAsync(L.h_lout, L.d_lout, sizeof(double) * nl, hipMemcpyDeviceToHost, L.st));
     }
-    HIPCHK(e, hipEventRecord(s->ev1, e->stream));
-    s->in_flight = true;
-    s->in_use = scr_c + (nl ? slotsL + bigL : 0);
+    HIPCHK(e, hipEventRecord(L.ev1, L.st));
+    L.busy = true;
+    s->in_flight += nc + nl;
     s->st.admitted += nc + nl;
     s->st.launches++;
     return PA_OK;
 }
 
-static pa_status serve_retire(pa_server *s, int32_t *n_finished)
+// Collects the elements of busy lanes whose completion flag is set; a lane
+// whose launch has ended retires its log-likelihood requests and returns its
+// pool range.  block: wait for every busy lane's launch to end.
+static pa_status serve_collect(pa_server *s, bool block, int32_t *n_finished)
 {
     pa_engine *e = s->e;
-    float ms = 0;
-    HIPCHK(e, hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    s->st.device_ms += ms;
-    const double now = pa_serve_clock();
-    for (size_t i = 0; i < s->batch.size(); i++) {
-        const Request &r = s->batch[i];
-        pa_serve_result out;
-        memset(&out, 0, sizeof out);
-        out.ticket = r.ticket; out.element = r.eid; out.kind = PA_SERVE_CHAINS;
-        out.arrival_s = r.arrival; out.admit_s = r.admit; out.finish_s = now;
-        int bad = 0;
-        for (int k = 0; k < 3; k++) {
-            const double *d = s->h_res_d + (i * 3 + k) * chain::RES_D;
-            const int32_t *q = s->h_res_i + (i * 3 + k) * chain::RES_I;
-            pa_chain_result &c = out.result[k];
-            c.max_loglik = d[0]; c.final_nrate = d[1]; c.final_crate = d[2];
-            c.final_grate = d[3]; c.final_lrate = d[4]; c.final_lrate2 = d[5];
-            c.max_m = q[0]; c.status = q[1]; c.accept_n = q[2]; c.accept_c = q[3]; c.flags = q[4]; c.pad = 0;
-            bad |= q[1] != 0;
+    for (Lane &L : s->lanes) {
+        if (!L.busy) continue;
+        bool ended = false;
+        if (block) {
+            HIPCHK(e, hipEventSynchronize(L.ev1));
+            ended = true;
+        } else {
+            const hipError_t q = hipEventQuery(L.ev1);
+            if (q != hipErrorNotReady) { HIPCHK(e, q); ended = true; }
         }
-        out.status = bad ? PA_ELEM_NUMERIC_FAIL : PA_ELEM_OK;
-        s->finished.push_back(out);
+        const double now = pa_serve_clock();
+        const int nc = (int)L.batch.size();
+        for (int i = 0; i < nc; i++) {
+            if (L.retired[i] || !__atomic_load_n(L.h_done + i, __ATOMIC_ACQUIRE)) continue;
+            const Request &r = L.batch[i];
+            pa_serve_result out;
+            memset(&out, 0, sizeof out);
+            out.ticket = r.ticket; out.element = r.eid; out.kind = PA_SERVE_CHAINS;
+            out.arrival_s = r.arrival; out.admit_s = r.admit; out.finish_s = now;
+            int bad = 0;
+            for (int k = 0; k < 3; k++) {
+                const double *d = L.h_res_d + ((size_t)i * 3 + k) * chain::RES_D;
+                const int32_t *q = L.h_res_i + ((size_t)i * 3 + k) * chain::RES_I;
+                pa_chain_result &c = out.result[k];
+                c.max_loglik = d[0]; c.final_nrate = d[1]; c.final_crate = d[2];
+                c.final_grate = d[3]; c.final_lrate = d[4]; c.final_lrate2 = d[5];
+                c.max_m = q[0]; c.status = q[1]; c.accept_n = q[2]; c.accept_c = q[3]; c.flags = q[4]; c.pad = 0;
+                bad |= q[1] != 0;
+            }
+            out.status = bad ? PA_ELEM_NUMERIC_FAIL : PA_ELEM_OK;
+            s->finished.push_back(out);
+            L.retired[i] = 1;
+            L.n_retired++;
+            s->in_flight--;
+            s->st.finished++;
+            if (n_finished) ++*n_finished;
+        }
+        if (!ended) continue;
+        if (L.n_retired != nc) return fail(e, PA_E_DEVICE, "a serving launch ended with elements not marked complete");
+        for (size_t i = 0; i < L.lbatch.size(); i++) {
+            const LRequest &r = L.lbatch[i];
+            pa_serve_result out;
+            memset(&out, 0, sizeof out);
+            out.ticket = r.ticket; out.element = r.eid; out.kind = PA_SERVE_LOGLIK;
+            out.loglik = L.h_lout[i];
+            out.status = std::isfinite(out.loglik) ? PA_ELEM_OK : PA_ELEM_NUMERIC_FAIL;
+            out.arrival_s = r.arrival; out.admit_s = r.admit; out.finish_s = now;
+            s->finished.push_back(out);
+            s->in_flight--;
+            s->st.finished++;
+            if (n_finished) ++*n_finished;
+        }
+        float ms = 0;
+        HIPCHK(e, hipEventElapsedTime(&ms, L.ev0, L.ev1));
+        s->st.device_ms += ms;
+        s->pool.release(L.pool_off, L.pool_len);
+        L.pool_off = L.pool_len = 0;
+        L.batch.clear();
+        L.lbatch.clear();
+        L.busy = false;
     }
-    for (size_t i = 0; i < s->lbatch.size(); i++) {
-        const LRequest &r = s->lbatch[i];
-        pa_serve_result out;
-        memset(&out, 0, sizeof out);
-        out.ticket = r.ticket; out.element = r.eid; out.kind = PA_SERVE_LOGLIK;
-        out.loglik = s->h_lout[i];
-        out.status = std::isfinite(out.loglik) ? PA_ELEM_OK : PA_ELEM_NUMERIC_FAIL;
-        out.arrival_s = r.arrival; out.admit_s = r.admit; out.finish_s = now;
-        s->finished.push_back(out);
-    }
-    const int n = (int)(s->batch.size() + s->lbatch.size());
-    s->st.finished += n;
-    if (n_finished) *n_finished += n;
-    s->batch.clear();
-    s->lbatch.clear();
-    s->in_flight = false;
-    s->in_use = 0;
     return PA_OK;
+}
+
+// Fills free lanes while requests wait.  A lane is started when no launch is
+// in flight, or when fewer elements are in flight than the chain kernel has
+// resident workgroups, i.e. when retiring elements have freed workgroup
+// slots; the arenas are uploaded first if elements were added (after every
+// launch that reads the old ones has ended).
+static pa_status serve_admit(pa_server *s)
+{
+    pa_engine *e = s->e;
+    if (s->waiting.empty() && s->lwait.empty()) return PA_OK;
+    pa_status st;
+    if (e->dev_dirty) {
+        if ((st = serve_collect(s, true, nullptr)) != PA_OK) return st;   // no lane may read the old arenas
+        if ((st = upload(e)) != PA_OK) return st;
+    }
+    const int gmax_c = chain_grid_max(e, s->fp64, s->BS);
+    for (Lane &L : s->lanes) {
+        if (L.busy) continue;
+        if (s->waiting.empty() && s->lwait.empty()) break;
+        bool busy = false;
+        for (const Lane &o : s->lanes) busy |= o.busy;
+        if (busy && s->in_flight >= gmax_c) break;
+        const int64_t before = s->st.launches;
+        if ((st = lane_launch(s, L, busy)) != PA_OK) return st;
+        if (s->st.launches == before) break;   // nothing fitted the free pool
+    }
+    return PA_OK;
+}
+
+static bool serve_busy(const pa_server *s)
+{
+    for (const Lane &L : s->lanes)
+        if (L.busy) return true;
+    return false;
 }
 
 extern "C" pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finished)
@@ -1875,20 +2028,14 @@ extern "C" pa_status pa_serve_step(pa_server *s, int32_t wait, int32_t *n_finish
     if (n_finished) *n_finished = 0;
     HIPCHK(e, hipSetDevice(e->device));
     pa_status st;
-    if (s->in_flight) {
-        if (!wait) {
-            const hipError_t q = hipEventQuery(s->ev1);
-            if (q == hipErrorNotReady) return PA_OK;
-            HIPCHK(e, q);
-        } else {
-            HIPCHK(e, hipEventSynchronize(s->ev1));
-        }
-        if ((st = serve_retire(s, n_finished)) != PA_OK) return st;
-    }
-    if ((st = serve_launch(s)) != PA_OK) return st;
-    if (wait && s->in_flight) {
-        HIPCHK(e, hipEventSynchronize(s->ev1));
-        if ((st = serve_retire(s, n_finished)) != PA_OK) return st;
+    if ((st = serve_collect(s, false, n_finished)) != PA_OK) return st;
+    if ((st = serve_admit(s)) != PA_OK) return st;
+    if (!wait) return PA_OK;
+    // wait = 1: run until nothing is in flight and nothing admissible waits
+    while (serve_busy(s)) {
+        if ((st = serve_collect(s, false, n_finished)) != PA_OK) return st;
+        if ((st = serve_admit(s)) != PA_OK) return st;
+        if (serve_busy(s)) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
     return PA_OK;
 }
@@ -1910,8 +2057,11 @@ extern "C" pa_status pa_serve_get_stats(pa_server *s, pa_serve_stats *out)
     if (!s || !out) return PA_E_INPUT;
     *out = s->st;
     out->waiting = (int32_t)(s->waiting.size() + s->lwait.size());
-    out->in_flight = (int32_t)(s->batch.size() + s->lbatch.size());
-    out->scratch_bytes_in_use = (double)s->in_use;
+    out->in_flight = s->in_flight;
+    out->scratch_bytes_in_use = (double)s->pool.used;
+    int busy = 0;
+    for (const Lane &L : s->lanes) busy += L.busy;
+    out->lanes_busy = busy;
     return PA_OK;
 }
 
@@ -1919,8 +2069,6 @@ extern "C" void pa_serve_destroy(pa_server *s)
 {
     if (!s) return;
     (void)hipSetDevice(s->e->device);
-    if (s->in_flight) (void)hipEventSynchronize(s->ev1);
-    (void)hipStreamSynchronize(s->e->stream);
     serve_free(s);
     delete s;
 }

@@ -101,8 +101,11 @@ class Tuning(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("block_threads", C.c_int32), ("sweep_budget", C.c_int32),
+    _fields_ = [("precision", C.c_int32), ("block_threads", C.c_int32), ("lanes", C.c_int32),
                 ("max_chains", C.c_int32), ("hbm_budget_bytes", C.c_double)]
+
+
+PA_SERVE_MAX_LANES, PA_SERVE_DEFAULT_LANES = 8, 3
 
 
 PA_SERVE_CHAINS, PA_SERVE_LOGLIK = 0, 1
@@ -117,7 +120,8 @@ class ServeResult(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("finished", C.c_int64),
                 ("launches", C.c_int64), ("waiting", C.c_int32), ("in_flight", C.c_int32),
-                ("scratch_bytes_in_use", C.c_double), ("device_ms", C.c_double)]
+                ("scratch_bytes_in_use", C.c_double), ("device_ms", C.c_double),
+                ("lanes_busy", C.c_int32), ("pad", C.c_int32)]
 
 
 _lib = None
@@ -443,13 +447,14 @@ def rng_uniform(seed, c0, c1, c2, c3, lane):
 
 
 class Server:
-    """Request loop over an engine's encoded elements: each step launches the
-    waiting requests that fit the HBM budget and runs their chains to the end."""
+    """Continuous request loop over an engine's encoded elements: up to
+    ``lanes`` launches in flight, each holding part of the HBM pool; every
+    element retires as soon as its own chains end."""
 
-    def __init__(self, engine: "Engine", precision=PA_FP32, block_threads=0, sweep_budget=0, max_chains=0,
+    def __init__(self, engine: "Engine", precision=PA_FP32, block_threads=0, lanes=0, max_chains=0,
                  hbm_budget_bytes=0.0):
         self.engine = engine
-        cfg = ServeConfig(int(precision), int(block_threads), int(sweep_budget), int(max_chains),
+        cfg = ServeConfig(int(precision), int(block_threads), int(lanes), int(max_chains),
                           float(hbm_budget_bytes))
         h = C.c_void_p()
         engine._check(lib().pa_serve_create(engine._h, C.byref(cfg), C.byref(h)))

@@ -536,20 +536,37 @@ def simulate_gene_trees(children, lengths, pi, Q, sizes, gap_frac, rng, S):
     return np.ascontiguousarray(align)
 
 
+def _gt_genes(args):
+    tree, h, seed, lo, hi = args
+    return [msc_gene_tree(tree, h, np.random.default_rng([seed, 1, c])) for c in range(lo, hi)]
+
+
+def _gt_align(args):
+    children, lengths, pi, Q, sizes, gap, seed, k, S = args
+    return simulate_gene_trees(children, lengths, pi, Q, sizes, gap, np.random.default_rng([seed, 2, k]), S)
+
+
+GT_CHUNK = 500     # elements per independently seeded simulation chunk
+
+
 def make_gt_workload(seed: int = 5, n_elements: int = 20000, S: int = 120, mean_bl: float = 0.5,
-                     scale: float = 0.1, gap: float = 0.02) -> GTWorkload:
+                     scale: float = 0.1, gap: float = 0.02, workers: int = 1) -> GTWorkload:
     """Gene-tree workload: an ultrametric species tree in coalescent units whose
     branch lengths average ``mean_bl``; every element draws its own gene tree
     from the multispecies coalescent; lengths LogNormal(ln 300, 0.6) clipped to
     [50, 3000]; 1-20% of species branches accelerated (r2 ~ U[1.5, 4]), the
-    rest conserved (r1 ~ U[0.1, 0.5]); ``scale`` substitutions per unit."""
+    rest conserved (r1 ~ U[0.1, 0.5]); ``scale`` substitutions per unit.
+
+    Element c's gene tree comes from its own stream (seed, 1, c) and the
+    sequences of chunk k (elements k*GT_CHUNK ...) from stream (seed, 2, k),
+    so the workload is the same for any number of ``workers`` (forked
+    processes; use them only before the process has touched the GPU)."""
     rng = np.random.default_rng(seed)
     tree, h = ultrametric_tree(S, 1.0, rng, mean_bl=mean_bl)
     N = tree.N
     pi, Q = gtr(rng)
     C = int(n_elements)
     sizes = np.clip(np.rint(rng.lognormal(np.log(300), 0.6, C)), 50, 3000).astype(np.int64)
-    genes = [msc_gene_tree(tree, h, rng) for _ in range(C)]
     r1 = rng.uniform(0.1, 0.5, size=C)
     r2 = rng.uniform(1.5, 4.0, size=C)
     frac = rng.uniform(0.01, 0.20, size=C)
@@ -557,7 +574,22 @@ def make_gt_workload(seed: int = 5, n_elements: int = 20000, S: int = 120, mean_
     rank = np.argsort(np.argsort(rng.random((C, N - 1)), axis=1), axis=1)
     Z = np.ones((C, N), np.int32)
     Z[:, : N - 1] = np.where(rank < nacc[:, None], 2, 1)
-    w = GTWorkload(f"gt{S}", tree, h, pi, Q, mod_text(pi, Q, tree), None, sizes,
-                   np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64), genes, Z, r1, r2, scale, seed)
-    w.align = simulate_gene_trees(w.children(), w.lengths(), pi, Q, sizes, gap, rng, S)
+    bounds = [(lo, min(C, lo + GT_CHUNK)) for lo in range(0, C, GT_CHUNK)]
+    pool = None
+    if workers > 1 and len(bounds) > 1:
+        import multiprocessing as mp
+        pool = mp.get_context("fork").Pool(min(workers, len(bounds)))
+    try:
+        run = pool.map if pool else (lambda f, xs: [f(x) for x in xs])
+        genes = [g for part in run(_gt_genes, [(tree, h, seed, lo, hi) for lo, hi in bounds]) for g in part]
+        w = GTWorkload(f"gt{S}", tree, h, pi, Q, mod_text(pi, Q, tree), None, sizes,
+                       np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64), genes, Z, r1, r2, scale, seed)
+        ch, L = w.children(), w.lengths()
+        parts = run(_gt_align, [(ch[lo:hi], L[lo:hi], pi, Q, sizes[lo:hi], gap, seed, k, S)
+                                for k, (lo, hi) in enumerate(bounds)])
+    finally:
+        if pool:
+            pool.close()
+            pool.join()
+    w.align = np.ascontiguousarray(np.concatenate(parts, axis=1))
     return w

@@ -1,6 +1,7 @@
-"""Request loop: chains served in launches of whatever is waiting give the
-same results as one batched run, and a chain's result is the same alone and
-in a mix of other requests arriving while it waits."""
+"""Continuous request loop: chains served through lanes of concurrent
+launches give the same results as one batched run, a chain's result is the
+same alone and in a mix of other requests arriving while others run, and
+elements retire one by one rather than with their launch."""
 import numpy as np
 import pytest
 
@@ -48,19 +49,20 @@ def _slot_bytes(eng, P, fp64):
     return al(2 * I * ppad * 4 * rs) + al(2 * I * ppad * 4) + 2 * al(3 * N * 16 * 8) + al(S * 24 * 4)
 
 
-def test_resumable_chains_are_switched_off():
+def test_lane_count_is_checked():
     eng, _ = _setup(2)
-    with pytest.raises(E.PaError) as ei:
-        E.Server(eng, sweep_budget=5)
-    assert ei.value.status == E.PA_E_STATE and "switched off" in str(ei.value)
+    for bad in (-1, E.PA_SERVE_MAX_LANES + 1):
+        with pytest.raises(E.PaError) as ei:
+            E.Server(eng, lanes=bad)
+        assert ei.value.status == E.PA_E_INPUT and "lanes" in str(ei.value)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [E.PA_FP64, E.PA_FP32])
-def test_served_chains_equal_one_batched_run(require_gpu, precision):
+@pytest.mark.parametrize("precision,lanes", [(E.PA_FP64, 1), (E.PA_FP64, 3), (E.PA_FP32, 3)])
+def test_served_chains_equal_one_batched_run(require_gpu, precision, lanes):
     eng, ids = _setup(16)
     ref = eng.run_chains(ids, precision=precision, block_threads=128)
-    srv = E.Server(eng, precision=precision, block_threads=128, max_chains=5)
+    srv = E.Server(eng, precision=precision, block_threads=128, max_chains=5, lanes=lanes)
     tickets = srv.submit(ids)
     got = _drain(srv, len(ids))
     assert srv.stats().launches >= 4
@@ -74,14 +76,15 @@ def test_served_chains_equal_one_batched_run(require_gpu, precision):
 
 
 @pytest.mark.gpu
-def test_same_result_alone_and_in_a_mix(require_gpu):
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_same_result_alone_and_in_a_mix(require_gpu, lanes):
     eng, ids = _setup(24, seed=8)
     x = int(ids[5])
-    alone = E.Server(eng, precision=E.PA_FP32)
+    alone = E.Server(eng, precision=E.PA_FP32, lanes=1)
     t = alone.submit([x])[0]
     ra = _drain(alone, 1)[t]
     alone.close()
-    mix = E.Server(eng, precision=E.PA_FP32, max_chains=7)
+    mix = E.Server(eng, precision=E.PA_FP32, max_chains=7, lanes=lanes)
     others = [int(i) for i in ids if i != x]
     tick = list(mix.submit(others[:9]))
     mix.step(wait=False)                      # a launch in flight when x arrives
@@ -209,4 +212,39 @@ def test_served_fp64_chains_match_oracle(require_gpu):
                           ("final_lrate", "lrate"), ("final_lrate2", "l2rate")):
                 assert abs(getattr(c, f) - o[tr][-1]) <= 1e-9 * abs(o[tr][-1]), (k, ms, f)
             assert (c.accept_n, c.accept_c) == (o["accept_n"], o["accept_c"])
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_lanes_overlap_and_elements_retire_one_by_one(require_gpu):
+    """Requests that arrive while a launch runs start in a second lane at once
+    (two launches in flight), and a short element of a launch retires before
+    its launch's longest element has finished."""
+    w = synth.make_workload(3, seed=3, lengths=[60, 70, 80, 9958])
+    eng = E.Engine.from_workload(w, E.default_options(burnin=40, mcmc=160, adapt_freq=50))
+    ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+    ids = ids[st == E.PA_ELEM_OK]
+    ref = eng.run_chains(ids, precision=E.PA_FP32, block_threads=128)
+    srv = E.Server(eng, precision=E.PA_FP32, block_threads=128, lanes=2)
+    t_first = list(srv.submit(ids[[0, 3]]))        # a short and the long element in one launch
+    srv.step(wait=False)
+    t_second = list(srv.submit(ids[[1, 2]]))       # arrive while that launch runs
+    srv.step(wait=False)
+    assert srv.stats().lanes_busy == 2 and srv.stats().launches == 2
+    got = {}
+    while len(got) < 4:
+        srv.step(wait=False)
+        for r in srv.poll():
+            got[r.ticket] = r
+    tick = t_first + t_second
+    order = [0, 3, 1, 2]
+    for t, k in zip(tick, order):
+        for ms in range(3):
+            for f in FIELDS:
+                assert getattr(got[t].result[ms], f) == ref[f][k, ms], (k, ms, f)
+    # the three short elements (two of them admitted later) finish before the long one
+    assert max(got[t].finish_s for t in (t_first[0], *t_second)) < got[t_first[1]].finish_s
+    srv.step(wait=True)                            # the lanes end once their launches have
+    st = srv.stats()
+    assert st.in_flight == 0 and st.lanes_busy == 0 and st.scratch_bytes_in_use == 0
     srv.close()
