@@ -160,6 +160,20 @@ pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *
 pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
                             int32_t precision, double *out);
 
+/* Per-element log-likelihood under the gBGC substitution model, on the device.
+ * Replaces the pruning of PhyloAcc-ST-GBGC over log_cache_TM_{null,cons,neut}
+ * and their _B twins (src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:1563-1621), whose
+ * rate matrices are BPP_C::getRate(submat, selection, bias) (bpp_c.cpp:13-29)
+ * for the six classes of bpp_c.cpp:191-197 and whose transitions are
+ * getTransition (bpp_c.cpp:1405-1427) over the element's branch lengths.
+ * Branch s takes class Z (0 background: selection 0; 1 conserved: sel[0];
+ * 2 accelerated: sel[1]) with bias sel[2] when B[s] != 0, else bias 0.
+ * Z, B: [n*N] int32; sel: [n*3] (conserved selection, accelerated selection,
+ * gBGC bias).  A class whose Q(s, b) has no time-reversible eigensystem
+ * gives NaN.  out: [n]. */
+pa_status pa_loglik_gbgc(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *Z, const int32_t *B,
+                         const double *sel, int32_t precision, double *out);
+
 /* pa_loglik and pa_loglik_lengths keep their device scratch (workgroup slots
  * and per-call length buffer) for the next call when it is at most 2 GiB;
  * this frees it now (no reference counterpart: the reference allocates per

@@ -78,6 +78,14 @@ class ChainOut(C.Structure):
 _lib = None
 
 
+def gbgc_rate(Q0, s, b):
+    """Q(s, b) of the gBGC model from Q0 [4, 4]."""
+    Q0 = np.ascontiguousarray(Q0, np.float64).reshape(16)
+    Q = np.zeros(16)
+    lib().or_gbgc_rate(_ptr(Q0, C.c_double), float(s), float(b), _ptr(Q, C.c_double))
+    return Q.reshape(4, 4)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -115,6 +123,10 @@ def load(path):
                                      P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_uint8)]
         L.or_loglik.argtypes = [P(Model), C.c_void_p, P(C.c_int), C.c_double, C.c_double]
         L.or_loglik.restype = C.c_double
+        L.or_gbgc_rate.argtypes = [P(C.c_double), C.c_double, C.c_double, P(C.c_double)]
+        L.or_gbgc_rate.restype = None
+        L.or_loglik_gbgc.argtypes = [P(Model), C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_double)]
+        L.or_loglik_gbgc.restype = C.c_double
         L.or_run_element.argtypes = [P(Model), P(Options), C.c_void_p, P(ChainOut)]
         L.or_test_update_Z.argtypes = [C.c_int, P(C.c_int), C.c_int, P(C.c_int), P(C.c_int), P(C.c_double),
                                        C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, C.c_int, P(C.c_int)]
@@ -200,6 +212,13 @@ class Element:
         return dict(children=ch.reshape(N, 2), parent=par, dist=d, missing=mis,
                     nodes=nodes[: inf["n_nodes"]], internal=internal[: inf["n_internal"]],
                     Tg=Tg[: GG * N].reshape(GG, N), codes=codes[: GG * S].reshape(GG, S), **inf)
+
+    def loglik_gbgc(self, Z, B, sel):
+        Z = np.ascontiguousarray(Z, np.int32)
+        B = np.ascontiguousarray(B, np.int32)
+        sel = np.ascontiguousarray(sel, np.float64)
+        return lib().or_loglik_gbgc(C.byref(self.model), self.ptr, _ptr(Z, C.c_int), _ptr(B, C.c_int),
+                                    _ptr(sel, C.c_double))
 
     def loglik(self, Z, r1, r2):
         Z = np.ascontiguousarray(Z, np.int32)

@@ -314,6 +314,16 @@ void or_expm(const double *Q, double t, double *P)
 
 /* log((V e^{L t} V^-1)^T), bpp.hpp:370-397 and bpp_c.cpp:187-228:
  * logPt[i*4+j] = log P_{j->i}(t).  Used for t > threshold only by callers. */
+static void log_trans_q(const double *Q, double t, double *logPt)
+{
+    double P[16];
+    or_expm(Q, t, P);
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) {
+        double v = P[j * 4 + i]; if (v < 0) v = 0;
+        logPt[i * 4 + j] = log(v);
+    }
+}
+
 void or_log_trans(const or_model *m, double t, double *logPt)
 {
     double P[16];
@@ -625,6 +635,60 @@ double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, doub
     if (e->filtered) return NAN;
     model_caches(m, e, r2, r1);
     for (int s = 0; s < e->N; s++) e->Z[s] = Z[s];
+    zero_internal(e);
+    double logpi[4]; for (int j = 0; j < 4; j++) logpi[j] = log(m->pi[j]);
+    char visited[OR_MAXN] = {0};
+    double res = 0;
+    for (int g = 0; g < e->GG; g++) {
+        prune_site(e, g, visited, logpi);
+        res += log_exp_sum(LAM(e, g, e->root), 4);
+    }
+    return res;
+}
+
+/* --------------------------------------------------------------- gBGC -- */
+
+/* BPP_C::getRate, src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:13-29: B holds +b on
+ * A->C, A->G, T->C, T->G and -b on C->A, C->T, G->A, G->T (bases A, C, G, T),
+ * s is added to every entry, each entry x becomes x / (1 - e^-x) (1 at 0),
+ * Q0 is scaled entrywise and the row sums are taken off the diagonal. */
+void or_gbgc_rate(const double *Q0, double s, double b, double *Q)
+{
+    if (s == 0 && b == 0) { memcpy(Q, Q0, 16 * sizeof(double)); return; }
+    double B[16] = {0};
+    B[0 * 4 + 1] = b; B[0 * 4 + 2] = b; B[3 * 4 + 1] = b; B[3 * 4 + 2] = b;
+    B[1 * 4 + 0] = -b; B[1 * 4 + 3] = -b; B[2 * 4 + 0] = -b; B[2 * 4 + 3] = -b;
+    for (int i = 0; i < 16; i++) B[i] += s;
+    for (int i = 0; i < 16; i++) B[i] = B[i] == 0 ? 1.0 : B[i] / (1 - exp(-B[i]));
+    for (int i = 0; i < 16; i++) Q[i] = Q0[i] * B[i];
+    for (int i = 0; i < 4; i++) {
+        double c = 0;
+        for (int j = 0; j < 4; j++) c += Q[i * 4 + j];
+        Q[i * 4 + i] -= c;
+    }
+}
+
+/* Pruning under the gBGC model: the six rate matrices of BPP_C::Gibbs
+ * (src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:191-197: background, conserved and
+ * accelerated selection, each without and with the bias) and every
+ * non-missing branch's log transition at its length (init_cache_TM,
+ * bpp_c.cpp:1563-1621, computed for every length), then the same recursion
+ * and site sum as or_loglik.  sel = {conserved selection, accelerated
+ * selection, bias}; B[s] != 0 selects the biased class of branch s. */
+double or_loglik_gbgc(const or_model *m, or_element *e, const int *Z, const int *B, const double *sel)
+{
+    if (e->filtered) return NAN;
+    double Qk[6][16];
+    for (int k = 0; k < 6; k++)
+        or_gbgc_rate(m->Q, k % 3 == 0 ? 0.0 : (k % 3 == 1 ? sel[0] : sel[1]), k >= 3 ? sel[2] : 0.0, Qk[k]);
+    for (int s = 0; s < e->N; s++) e->Z[s] = Z[s];
+    for (int k = 0; k < e->n_nodes - 1; k++) {
+        int s = e->nodes[k];
+        if (e->missing[s]) continue;
+        int cls = (Z[s] == 2 ? 2 : (Z[s] == 0 ? 0 : 1)) + (B[s] ? 3 : 0);
+        double *dst = (double *)Lclass(e, s);   /* the table prune_site reads for branch s */
+        log_trans_q(Qk[cls], e->dist2[s], dst);
+    }
     zero_internal(e);
     double logpi[4]; for (int j = 0; j < 4; j++) logpi[j] = log(m->pi[j]);
     char visited[OR_MAXN] = {0};

@@ -107,6 +107,10 @@ void or_elem_arrays(const or_element *e, int *children2 /*[N*2]*/, int *parent2,
 
 /* log-likelihood at fixed Z, rates (bpp_c2.cpp:392-425 form) */
 double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, double r2);
+/* gBGC model (PhyloAcc-ST-GBGC): Q(s, b) from Q0, and the log-likelihood with
+ * per-branch classes (Z, B) and sel = {conserved s, accelerated s, bias}. */
+void or_gbgc_rate(const double *Q0, double s, double b, double *Q);
+double or_loglik_gbgc(const or_model *m, or_element *e, const int *Z, const int *B, const double *sel);
 
 /* full three-model chain of one element, outs[0..2] = M0, M1, M2 (run order resZ 0, 2, 1) */
 int or_run_element(const or_model *m, const or_options *o, or_element *e, or_chain_out *outs);

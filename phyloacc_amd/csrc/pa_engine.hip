@@ -26,6 +26,7 @@
 #include "pa_common.h"
 #include "pa_kernels.h"
 #include "pa_device.h"
+#include "pa_eig.h"
 
 #include <mutex>
 
@@ -130,23 +131,24 @@ size_t loglik_slot_bytes(int N, int S, int Pcap, int fp64)
 
 // Transition matrix of branch s into PT: P(d * rate) for length d > 1e-10,
 // else the identity (BPP_C::initMats, src/PhyloAcc-ST/bpp_c.cpp:187-228).
-// P(t) = V diag(exp(lam t)) Vi, clamped at 0, built in FP64.
-template <typename R>
-__device__ void branch_matrix(MP M, CP<R> c, int s, double d, double rate)
+// P(t) = V diag(exp(lam t)) Vi, clamped at 0, built in FP64, from the
+// model's eigensystem or a gBGC class's (lam, V, Vi in LDS or global memory).
+template <typename R, typename PE>
+__device__ void branch_matrix_from(PE lam, PE V, PE Vi, CP<R> c, int s, double d, double rate)
 {
     double P[16];
     if (!(d <= 1e-10)) {          // a NaN length gives a NaN matrix, not the identity
         const double t = d * rate;
         double e[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) e[k] = exp(M->lam[k] * t);
+        for (int k = 0; k < 4; k++) e[k] = exp(lam[k] * t);
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 double v = 0;
 #pragma unroll
-                for (int k = 0; k < 4; k++) v += M->V[j * 4 + k] * e[k] * M->Vi[k * 4 + i];
+                for (int k = 0; k < 4; k++) v += V[j * 4 + k] * e[k] * Vi[k * 4 + i];
                 P[j * 4 + i] = v < 0 ? 0.0 : v;
             }
     } else {
@@ -158,6 +160,53 @@ __device__ void branch_matrix(MP M, CP<R> c, int s, double d, double rate)
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) dst[i * 4 + j] = (R)P[j * 4 + i];
+}
+
+template <typename R>
+__device__ void branch_matrix(MP M, CP<R> c, int s, double d, double rate)
+{
+    branch_matrix_from<R>(M->lam, M->V, M->Vi, c, s, d, rate);
+}
+
+// gBGC classes (BPP_C::init_cache_TM, src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:
+// 1563-1621, with the rate matrices of Gibbs, bpp_c.cpp:191-197): one thread
+// per (work position, class).
+struct GbgcEigArgs {
+    double Q0[16];
+    const double *sel;
+    int n;
+    double *eig;
+};
+
+__global__ void gbgc_eig_kernel(GbgcEigArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n * GB_CLASSES) return;
+    const int w = t / GB_CLASSES, k = t % GB_CLASSES;
+    const double *sel = a.sel + (size_t)w * 3;
+    const double s = k % 3 == 0 ? 0.0 : (k % 3 == 1 ? sel[0] : sel[1]);
+    const double b = k >= 3 ? sel[2] : 0.0;
+    double Q[16], lam[4], V[16], Vi[16];
+    gbgc_rate(a.Q0, s, b, Q);
+    const int st = reversible_eig4(Q, lam, V, Vi);
+    double *out = a.eig + (size_t)t * GB_EIG_WORDS;
+    for (int i = 0; i < 4; i++) out[i] = st == EIG_OK ? lam[i] : NAN;
+    for (int i = 0; i < 16; i++) {
+        out[4 + i] = V[i];
+        out[20 + i] = Vi[i];
+    }
+}
+
+hipError_t launch_gbgc_eig(const double *Q0, const double *sel, int n, double *eig, hipStream_t st)
+{
+    GbgcEigArgs a;
+    for (int i = 0; i < 16; i++) a.Q0[i] = Q0[i];
+    a.sel = sel;
+    a.n = n;
+    a.eig = eig;
+    const int threads = n * GB_CLASSES;
+    hipLaunchKernelGGL(gbgc_eig_kernel, dim3((threads + 127) / 128), dim3(128), 0, st, a);
+    return hipGetLastError();
 }
 
 // ----------------------------------------------------------------- pruning
@@ -335,13 +384,18 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
     // cherries (both children leaves; never the root) are recomputed where read
     if (c->n_up > 1)
         for (int q = tid; q < c->upoff[0]; q += BS) c->flags[c->up[q]] |= NF_CHERRY;
-    const bool own = A->len_in != nullptr;
-    const double r1 = own ? 1.0 : A->rates_in[2 * w], r2 = own ? 1.0 : A->rates_in[2 * w + 1];
+    const bool own = A->len_in != nullptr, gb = A->gb_eig != nullptr;
+    const double r1 = own || gb ? 1.0 : A->rates_in[2 * w], r2 = own || gb ? 1.0 : A->rates_in[2 * w + 1];
     const double *dist = (const double *)(tab + tab_dist(N));
     for (int q = tid; q < c->n_nodes - 1; q += BS) {
         const int s = c->nodes[q];
         if (own) {
             branch_matrix<R>(M, c, s, A->len_in[(size_t)w * N + s], 1.0);
+        } else if (gb) {
+            const int z = A->Zin[(size_t)w * N + s];
+            const int k = (z == 2 ? 2 : (z == 0 ? 0 : 1)) + (A->Bin[(size_t)w * N + s] != 0 ? 3 : 0);
+            const double *E = A->gb_eig + ((size_t)w * GB_CLASSES + k) * GB_EIG_WORDS;
+            branch_matrix_from<R>(E, E + 4, E + 20, c, s, dist[s], 1.0);
         } else {
             const int z = A->Zin[(size_t)w * N + s];
             branch_matrix<R>(M, c, s, dist[s], z == 2 ? r2 : (z == 0 ? 1.0 : r1));

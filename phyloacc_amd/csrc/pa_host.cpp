@@ -22,6 +22,7 @@
 #include "pa_kernels.h"
 #include "pa_chain.h"
 #include "pa_rng.h"
+#include "pa_eig.h"
 
 using namespace pa;
 
@@ -192,98 +193,16 @@ extern "C" pa_status pa_parse_mod(const char *text, int32_t max_nodes, int32_t *
 
 namespace {
 
-// Stationary distribution of Q: pi Q = 0, sum pi = 1 (Gaussian elimination).
-bool stationary(const double *Q, double *pi)
-{
-    double A[4][5];
-    for (int i = 0; i < 3; i++) {
-        for (int j = 0; j < 4; j++) A[i][j] = Q[j * 4 + i];
-        A[i][4] = 0;
-    }
-    for (int j = 0; j < 4; j++) A[3][j] = 1;
-    A[3][4] = 1;
-    for (int c = 0; c < 4; c++) {
-        int piv = c;
-        for (int r = c + 1; r < 4; r++) if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
-        if (fabs(A[piv][c]) < 1e-300) return false;
-        for (int k = 0; k < 5; k++) std::swap(A[c][k], A[piv][k]);
-        for (int r = 0; r < 4; r++) {
-            if (r == c) continue;
-            double f = A[r][c] / A[c][c];
-            for (int k = c; k < 5; k++) A[r][k] -= f * A[c][k];
-        }
-    }
-    for (int i = 0; i < 4; i++) pi[i] = A[i][4] / A[i][i];
-    for (int i = 0; i < 4; i++) if (!(pi[i] > 0)) return false;
-    return true;
-}
-
-// Cyclic Jacobi on a symmetric 4x4: A = U diag(w) U^T.
-void jacobi4(double A[4][4], double w[4], double U[4][4])
-{
-    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) U[i][j] = i == j;
-    for (int sweep = 0; sweep < 64; sweep++) {
-        double off = 0;
-        for (int p = 0; p < 4; p++) for (int q = p + 1; q < 4; q++) off += A[p][q] * A[p][q];
-        if (off < 1e-300) break;
-        for (int p = 0; p < 4; p++)
-            for (int q = p + 1; q < 4; q++) {
-                if (A[p][q] == 0) continue;
-                double th = (A[q][q] - A[p][p]) / (2 * A[p][q]);
-                double t = (th >= 0 ? 1 : -1) / (fabs(th) + sqrt(th * th + 1));
-                double c = 1 / sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < 4; k++) {
-                    double akp = A[k][p], akq = A[k][q];
-                    A[k][p] = c * akp - s * akq;
-                    A[k][q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < 4; k++) {
-                    double apk = A[p][k], aqk = A[q][k];
-                    A[p][k] = c * apk - s * aqk;
-                    A[q][k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < 4; k++) {
-                    double ukp = U[k][p], ukq = U[k][q];
-                    U[k][p] = c * ukp - s * ukq;
-                    U[k][q] = s * ukp + c * ukq;
-                }
-            }
-    }
-    for (int i = 0; i < 4; i++) w[i] = A[i][i];
-}
-
 // Q = V diag(lam) Vi for a time-reversible Q (BPP ctor eig_gen + inv,
 // src/PhyloAcc-ST/bpp.hpp:357-362), via the symmetric form D^1/2 Q D^-1/2.
 bool eigensystem(const double *Q, double *lam, double *V, double *Vi, std::string &err)
 {
-    double pq[4];
-    if (!stationary(Q, pq)) { err = "RATE_MAT has no positive stationary distribution"; return false; }
-    double A[4][4], amax = 0, asym = 0;
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) {
-            A[i][j] = sqrt(pq[i]) * Q[i * 4 + j] / sqrt(pq[j]);
-            amax = std::max(amax, fabs(A[i][j]));
-        }
-    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) asym = std::max(asym, fabs(A[i][j] - A[j][i]));
-    if (asym > 1e-9 * amax) { err = "RATE_MAT is not time-reversible"; return false; }
-    for (int i = 0; i < 4; i++) for (int j = i + 1; j < 4; j++) A[i][j] = A[j][i] = 0.5 * (A[i][j] + A[j][i]);
-    double U[4][4];
-    jacobi4(A, lam, U);
-    for (int i = 0; i < 4; i++)
-        for (int k = 0; k < 4; k++) {
-            V[i * 4 + k] = U[i][k] / sqrt(pq[i]);
-            Vi[k * 4 + i] = U[i][k] * sqrt(pq[i]);
-        }
-    double rec = 0, qmax = 0;
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) {
-            double s = 0;
-            for (int k = 0; k < 4; k++) s += V[i * 4 + k] * lam[k] * Vi[k * 4 + j];
-            rec = std::max(rec, fabs(s - Q[i * 4 + j]));
-            qmax = std::max(qmax, fabs(Q[i * 4 + j]));
-        }
-    if (rec > 1e-10 * qmax) { err = "eigendecomposition of RATE_MAT failed"; return false; }
-    return true;
+    switch (reversible_eig4(Q, lam, V, Vi)) {
+    case EIG_OK: return true;
+    case EIG_NO_STATIONARY: err = "RATE_MAT has no positive stationary distribution"; return false;
+    case EIG_NOT_REVERSIBLE: err = "RATE_MAT is not time-reversible"; return false;
+    default: err = "eigendecomposition of RATE_MAT failed"; return false;
+    }
 }
 
 }  // namespace
@@ -509,6 +428,7 @@ struct pa_engine {
     int device = 0;
     pa_options opt;
     ModelDev M;
+    double Q0[16];                  // RATE_MAT (the gBGC classes are built from it)
     TreeModel tm;
     std::string err;
     std::vector<HostElem> elems;
@@ -634,6 +554,7 @@ extern "C" pa_status pa_engine_create(const pa_model *model, const pa_options *o
     M.r1_shared = o.r1_shared_uniform;
     std::string err;
     if (!eigensystem(model->Q, M.lam, M.V, M.Vi, err)) { e->err = err; *out = e; return PA_E_NUMERIC; }
+    memcpy(e->Q0, model->Q, sizeof e->Q0);
     e->device = device;
     *out = e;
     return PA_OK;
@@ -1101,6 +1022,59 @@ extern "C" pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, cons
     memset(&A, 0, sizeof A);
     A.Zin = (const int32_t *)bufs[3].p;
     A.rates_in = (const double *)bufs[4].p;
+    A.ll_out = (double *)bufs[5].p;
+    std::vector<int32_t> sorted(n);
+    for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
+    st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);
+    if (st != PA_OK) return st;
+    std::vector<double> ll(n);
+    HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) out[order[i]] = ll[i];
+    if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
+    return PA_OK;
+}
+
+// Log-likelihood under the gBGC substitution model (PhyloAcc-ST-GBGC): every
+// branch's P(t) comes from Q(s, b) of its class -- background (Z = 0),
+// conserved (Z = 1) or accelerated (Z = 2) selection, with or without the
+// GC-biased conversion bias when B = 1 -- over the element's own branch
+// lengths (BPP_C::getRate / getTransition / init_cache_TM,
+// src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:13-29,1405-1427,1563-1621; the rate
+// matrices of bpp_c.cpp:191-197).  The six class eigensystems of each element
+// are built on the device, then the likelihood kernel prunes as pa_loglik.
+extern "C" pa_status pa_loglik_gbgc(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *Z, const int32_t *B,
+                                    const double *sel, int32_t precision, double *out)
+{
+    if (!e || !ids || !Z || !B || !sel || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    if (n == 0) return PA_OK;
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    const int N = e->tm.N;
+    std::vector<int32_t> order = schedule(e, ids, n, PA_RUN_BATCHED);
+    std::vector<int32_t> Zw((size_t)n * N), Bw((size_t)n * N);
+    std::vector<double> sw((size_t)n * 3);
+    for (int i = 0; i < n; i++) {
+        memcpy(&Zw[(size_t)i * N], Z + (size_t)order[i] * N, sizeof(int32_t) * N);
+        memcpy(&Bw[(size_t)i * N], B + (size_t)order[i] * N, sizeof(int32_t) * N);
+        memcpy(&sw[(size_t)i * 3], sel + (size_t)order[i] * 3, sizeof(double) * 3);
+    }
+    DevBuf bufs[8];
+    HIPCHK(e, hipMalloc(&bufs[3].p, Zw.size() * 4));
+    HIPCHK(e, hipMalloc(&bufs[4].p, sw.size() * 8));
+    HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
+    HIPCHK(e, hipMalloc(&bufs[6].p, Bw.size() * 4));
+    HIPCHK(e, hipMalloc(&bufs[7].p, (size_t)n * GB_CLASSES * GB_EIG_WORDS * 8));
+    HIPCHK(e, hipMemcpy(bufs[3].p, Zw.data(), Zw.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(bufs[4].p, sw.data(), sw.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(bufs[6].p, Bw.data(), Bw.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, launch_gbgc_eig(e->Q0, (const double *)bufs[4].p, n, (double *)bufs[7].p, e->stream));
+    LoglikArgs A;
+    memset(&A, 0, sizeof A);
+    A.Zin = (const int32_t *)bufs[3].p;
+    A.Bin = (const int32_t *)bufs[6].p;
+    A.gb_eig = (const double *)bufs[7].p;
     A.ll_out = (double *)bufs[5].p;
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];

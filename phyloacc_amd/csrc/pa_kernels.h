@@ -42,7 +42,22 @@ struct LoglikArgs {
     const double *rates_in;   // [n_work][2] (r1, r2) (when len_in is null)
     const double *len_in;     // [n_work][N] per-call branch lengths, or null
     double *ll_out;           // [n_work]
+    // gBGC substitution model, or null: branch s of work position w takes
+    // P(dist) of class k = c(Z) + 3 * (Bin[w][s] != 0), c(0) = 0 (background),
+    // c(2) = 2 (accelerated), else 1 (conserved), whose eigensystem is
+    // gb_eig[w][k] = {lam[4], V[16], Vi[16]} (gbgc_eig_kernel).
+    const double *gb_eig;     // [n_work][6][GB_EIG_WORDS]
+    const int32_t *Bin;       // [n_work][N]
 };
+
+constexpr int GB_CLASSES = 6, GB_EIG_WORDS = 36;
+
+// Eigensystems of the six gBGC classes of every work position: class k has
+// selection 0, sel[w][0] (conserved) or sel[w][1] (accelerated) for k % 3 =
+// 0, 1, 2, and bias sel[w][2] when k >= 3, else 0 (Q from Q0 by gbgc_rate).
+// A class whose Q is not time-reversible gets NaN eigenvalues.
+hipError_t launch_gbgc_eig(const double *Q0 /*[16], host*/, const double *sel /*[n][3], device*/, int n,
+                           double *eig /*[n][6][36], device*/, hipStream_t st);
 
 // LDS bytes of one workgroup (dynamic arrays + context + argument blocks).
 size_t loglik_smem_bytes(int N, int fp64);

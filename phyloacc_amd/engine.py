@@ -31,7 +31,7 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths",
            "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
            "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
-           "pa_serve_submit_lengths", "pa_release_scratch"]
+           "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc"]
 
 
 class PaError(RuntimeError):
@@ -166,6 +166,9 @@ def lib():
         L.pa_loglik_lengths.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), C.c_int32,
                                         P(C.c_double)]
         L.pa_loglik_lengths.restype = C.c_int
+        L.pa_loglik_gbgc.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double),
+                                     C.c_int32, P(C.c_double)]
+        L.pa_loglik_gbgc.restype = C.c_int
         L.pa_set_tuning.argtypes = [C.c_void_p, P(Tuning)]
         L.pa_set_tuning.restype = C.c_int
         L.pa_get_tuning.argtypes = [C.c_void_p, P(Tuning)]
@@ -315,6 +318,18 @@ class Engine:
         out = np.zeros(len(ids), np.float64)
         self._check(lib().pa_loglik_lengths(self._h, _p(ids, C.c_int32), len(ids), _p(L, C.c_double),
                                             int(precision), _p(out, C.c_double)))
+        return out
+
+    def loglik_gbgc(self, ids, Z, B, sel, precision=PA_FP64):
+        """gBGC model: Z, B [n, N]; sel [n, 3] = (conserved selection,
+        accelerated selection, gBGC bias)."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        Z = np.ascontiguousarray(Z, np.int32).reshape(len(ids), self.N)
+        B = np.ascontiguousarray(B, np.int32).reshape(len(ids), self.N)
+        sel = np.ascontiguousarray(sel, np.float64).reshape(len(ids), 3)
+        out = np.zeros(len(ids), np.float64)
+        self._check(lib().pa_loglik_gbgc(self._h, _p(ids, C.c_int32), len(ids), _p(Z, C.c_int32), _p(B, C.c_int32),
+                                         _p(sel, C.c_double), int(precision), _p(out, C.c_double)))
         return out
 
     def release_scratch(self):
