@@ -241,25 +241,47 @@ typedef struct {
 pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
                         pa_chain_result *results, pa_traces *traces, pa_run_stats *stats);
 
-/* Block size of batched chain launches (no reference counterpart: the
- * reference runs one element per OpenMP thread).  A batched launch uses one
- * block size: small_threads == long_threads (64, 128 or 256), or both 0 for
- * the default chosen from the batch's mean pattern count.  Split launches
- * (small_threads != long_threads: two block sizes side by side, elements
- * above long_patterns on the larger) are switched off in this build and
- * rejected with PA_E_STATE.  rate: the element-MCMC iterations/s the
- * autotuner measured. */
+/* A prepared batched chain run, captured once as a HIP graph (no reference
+ * counterpart: the reference loops over elements, src/PhyloAcc-ST/main.cpp:452-500).
+ * pa_batch_create schedules the ids, plans and allocates the device scratch
+ * and result buffers, and captures the launch sequence -- a kernel clearing
+ * the counters, the persistent chain kernel, the copies of the result rows
+ * into pinned host memory -- on the engine stream.  pa_batch_run replays the
+ * graph and writes results ([n*3], like pa_run_chains) and stats; the chains
+ * draw from the same streams, so every replay equals pa_run_chains on the
+ * same ids.  cfg->mode must be PA_RUN_BATCHED; with cfg->keep_traces the
+ * traces are written on the device (as pa_run_chains does) and not returned.  After pa_elements_add the
+ * arenas move and pa_batch_run returns PA_E_STATE (prepare the batch again).
+ * pa_engine_destroy frees the engine's remaining batches; do not call
+ * pa_batch_destroy on them afterwards. */
+typedef struct pa_batch pa_batch;
+pa_status pa_batch_create(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg, pa_batch **out);
+pa_status pa_batch_run(pa_batch *b, pa_chain_result *results, pa_run_stats *stats);
+void pa_batch_destroy(pa_batch *b);
+
+/* Launch shape of batched chain launches (no reference counterpart: the
+ * reference runs one element per OpenMP thread): the block size (64, 128 or
+ * 256 threads, or 0 for the default chosen from the batch's mean pattern
+ * count) and the pattern-loop shape 0..PA_N_SHAPES-1, a pair (U nodes
+ * pruned per round, K nodes per ancestral-sampling chunk): FP32 (2, 4),
+ * (1, 4), (2, 8); FP64 (1, 2), (2, 2), (1, 4).  Neither changes any result
+ * (each chain is one workgroup's, and the shapes only regroup independent
+ * loads).  round_nodes / chunk_nodes report U and K of the shape at the
+ * precision the autotuner ran; rate: the element-MCMC iterations/s it
+ * measured. */
+enum { PA_N_SHAPES = 3 };
 typedef struct {
-    int32_t small_threads, long_threads, long_patterns;
+    int32_t block_threads, shape;
+    int32_t round_nodes, chunk_nodes;
     double rate;
 } pa_tuning;
 
 pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t);
 pa_status pa_get_tuning(pa_engine *e, pa_tuning *t);
-/* Times block sizes 64, 128 and 256 on the batch (or, with sample > 0, on
- * `sample` of its elements evenly spaced by pattern count) with chains of
- * `states` states (0: 8) and keeps the fastest for later pa_run_chains calls
- * with block_threads = 0. */
+/* Times every block size (64, 128, 256) and shape on the batch (or, with
+ * sample > 0, on `sample` of its elements evenly spaced by pattern count)
+ * with chains of `states` states (0: 8) and keeps the fastest pair for later
+ * pa_run_chains / pa_batch_create calls with block_threads = 0. */
 pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, int32_t precision, int32_t sample,
                       int32_t states, pa_tuning *out);
 

@@ -49,15 +49,21 @@ size_t chain_smem_bytes(int N, int S, int BS, int fp64);
 // Dynamic LDS bytes of one workgroup (the launch's dynamic allocation).
 size_t chain_dyn_smem_bytes(int N, int S, int BS, int fp64);
 // Workgroups of the chain kernel resident on one CU (registers, LDS, launch bound).
-int chain_blocks_per_cu(int fp64, int BS, int N, int S);
+int chain_blocks_per_cu(int fp64, int BS, int N, int S, int shape = 0);
+// Pattern-loop shapes of the chain kernel (nodes per pruning round, nodes
+// per sampling chunk), 0 = the default; see Shape in pa_chain.hip.
+constexpr int N_SHAPES = 3;
 // Partials and exponents of one element of Ppad padded patterns: two buffers
 // per internal node; the exponents start at chain_exps_off.
 size_t chain_part_bytes(int N, int S, int Ppad, int fp64);
 // Byte layout of one workgroup's HBM slot whose partial region holds elements
 // of up to Pcap (padded) patterns: off = {plin, logt, amb, parts}; returns its size.
 size_t slot_scratch_bytes(int N, int S, int Pcap, int fp64, size_t *off /*[4]*/);
-hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st);
+hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st,
+                        int shape = 0);
 hipError_t launch_reset(int *q, hipStream_t st);
+// Zeroes `words` 32-bit words at p with a kernel (graph-capturable).
+hipError_t launch_clear(void *p, int words, hipStream_t st);
 
 }  // namespace chain
 }  // namespace pa

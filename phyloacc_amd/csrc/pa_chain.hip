@@ -29,6 +29,7 @@
 // allocation spanning the whole persistent loop.
 // No MFMA: a K=4 matvec per pattern is not a dense contraction.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <algorithm>
 #include <stdint.h>
 #include "pa_common.h"
@@ -224,7 +225,7 @@ __device__ __noinline__ void class_table(MP M, CP<R> c, int k, int s, double r, 
 // those with Z == only_cls, or (REFRESH_CHANGED) those whose Z the last Z
 // update changed
 constexpr int REFRESH_CHANGED = -2;
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void refresh_PTc(CP<R> c, int only_cls)
 {
     for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
@@ -325,12 +326,21 @@ __device__ __forceinline__ ChildIn<R> kid_load(const KidDesc<R> &d, const GBL ui
     return in;
 }
 
-// Nodes pruned together per round: all their loads are issued before the
-// first is used.  Two in FP32 and one in FP64 fit the 128-register budget of
-// the chain kernel without spilling in the pattern loop (measured on config
-// 3: groups of 3 spill and ran 3 % slower than groups of 2).
-template <typename R>
-struct Group { static constexpr int U = sizeof(R) == 4 ? 2 : 1; };
+// Pattern-loop shapes the autotuner picks among.  U: visited nodes pruned
+// together per round (all their loads are issued before the first is used);
+// K: internal nodes whose partials one sampling chunk loads before drawing.
+// Shape 0 is the default: two nodes in FP32 and one in FP64 fit the
+// 128-register budget of the chain kernel without spilling in the pattern
+// loop (groups of 3 spill and ran 3 % slower on the 50,000-element
+// 43-species workload).
+template <typename R, int SH>
+struct Shape;
+template <> struct Shape<float, 0> { static constexpr int U = 2, K = 4; };
+template <> struct Shape<float, 1> { static constexpr int U = 1, K = 4; };
+template <> struct Shape<float, 2> { static constexpr int U = 2, K = 8; };
+template <> struct Shape<double, 0> { static constexpr int U = 1, K = 2; };
+template <> struct Shape<double, 1> { static constexpr int U = 2, K = 2; };
+template <> struct Shape<double, 2> { static constexpr int U = 1, K = 4; };
 
 // One pruning pass (BPP_C::Update_Tg(tosample=false), src/PhyloAcc-ST/bpp_c.cpp:644-671,
 // and log_f_Xz, bpp_c.cpp:860-928).  FULL: every internal node of the tree
@@ -352,7 +362,7 @@ struct Group { static constexpr int U = sizeof(R) == 4 ? 2 : 1; };
 // every node's buffer and matrix bits, so a round's scalar set-up is a few
 // independent LDS reads: its list entries, its nodes' static descriptors
 // (node_desc) and their children's bits.
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
 {
     const int tid = threadIdx.x;
@@ -362,7 +372,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
     GBL int32_t *exps = uni_ptr(c->exps);
     const GBL uint8_t *codes = uni_ptr(c->codes);
     const LDS R *PTc = uni_ptr(c->PTc), *PTp = uni_ptr(c->PTp);
-    constexpr int U = Group<R>::U;
+    constexpr int U = Shape<R, SH>::U;
     unsigned long long bytes = 0;
     const LDS uint8_t *flags = uni_ptr(c->flags), *dirty = uni_ptr(c->dirty), *cur = uni_ptr(c->cur);
     const LDS int32_t *plist = uni_ptr(c->plist);
@@ -514,7 +524,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, int mode, int cls)
 // site, plus the per-branch transition counts that getEmission
 // (bpp_c2.cpp:355-390) sums over.  Uniform for site g, sampling position t
 // (0 = root, then internal nodes in reverse post-order): ANC stream, c1 = g.
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c3)
 {
     const int tid = threadIdx.x, S = uni(c->S), n_int = uni(c->n_int), GG = uni(c->GG), Ppad = uni(c->Ppad);
@@ -567,7 +577,7 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
         // loads do not depend on its draws, so they are all issued first.  A
         // cherry's slot holds its two leaf codes (low and second byte) until
         // its partial is recomputed.
-        constexpr int K = sizeof(R) == 4 ? 4 : 2;
+        constexpr int K = Shape<R, SH>::K;
         int t = 1;
         for (int kb = n_int - 2; kb >= 0; kb -= K) {
             // node ids, flags, buffers and cherry leaves are the same on every
@@ -648,7 +658,7 @@ __device__ __noinline__ void sample_sites(MP M, CP<R> c, uint32_t c2, uint32_t c
 }
 
 // getEmission (src/PhyloAcc-ST/bpp_c2.cpp:355-390): E[s][k] from the counts.
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void emissions(CP<R> c)
 {
     const int N = c->N;
@@ -709,7 +719,7 @@ __device__ __forceinline__ double log_col(const LDS double *T, int j, double y0,
 // depth.  The draw for node s uses Z-stream slot tz[s].  Log space matters:
 // over thousands of sites the classes' emissions differ by thousands of
 // nats, and products of linear messages would underflow to zero.
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void update_Z(MP M, CP<R> c, uint32_t c2, uint32_t c3)
 {
     const int tid = threadIdx.x;
@@ -895,7 +905,7 @@ __device__ __noinline__ void rate_adapt(MP M, CP<R> c, int m, bool neut)
     }
 }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void build_prop_tables(MP M, CP<R> c, int cls, double prop)
 {
     for (int q = threadIdx.x; q < c->n_nodes - 1; q += BS) {
@@ -918,7 +928,7 @@ __device__ __noinline__ void build_prop_tables(MP M, CP<R> c, int cls, double pr
     }
 }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void mark_class(CP<R> c, int cls)
 {
     const int tid = threadIdx.x, N = c->N;
@@ -932,20 +942,20 @@ __device__ __noinline__ void mark_class(CP<R> c, int cls)
 }
 
 // BPP_C::getUpdateNode(neut) + BPP_C::sample_rate (src/PhyloAcc-ST/bpp_c.cpp:836-858, 930-1176)
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int resZ, bool neut)
 {
     const int tid = threadIdx.x;
     const int cls = neut ? 2 : 1;
     const double old = neut ? c->nrate : c->crate;
-    mark_class<R, BS>(c, cls);
+    mark_class<R, BS, SH>(c, cls);
     __syncthreads();
     if (tid == 0) rate_propose<R>(M, c, ms, m, resZ, neut, old);
     __syncthreads();
     if (!c->skip) {
-        build_prop_tables<R, BS>(M, c, cls, c->proposal);
+        build_prop_tables<R, BS, SH>(M, c, cls, c->proposal);
         __syncthreads();
-        const double lnew = prune<R, BS>(M, c, PR_PROP, cls);
+        const double lnew = prune<R, BS, SH>(M, c, PR_PROP, cls);
         if (tid == 0) rate_decide<R>(M, c, neut, old, lnew);
         __syncthreads();
         if (c->accept) {
@@ -959,7 +969,7 @@ __device__ __forceinline__ void sample_rate(MP M, CP<R> c, int ms, int m, int re
     if (now != old) {
         for (int q = tid; q < c->n_nodes - 1; q += BS) class_table<R>(M, c, cls, c->nodes[q], now, 1);
         __syncthreads();
-        refresh_PTc<R, BS>(c, cls);
+        refresh_PTc<R, BS, SH>(c, cls);
     }
     __syncthreads();
     if (tid == 0) {
@@ -1000,7 +1010,7 @@ __device__ __noinline__ void transition_gamma(MP M, CP<R> c, int ms, int m, int 
     c->red[j] = x;
 }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __forceinline__ void sample_transition(MP M, CP<R> c, int ms, int m)
 {
     const int tid = threadIdx.x;
@@ -1065,7 +1075,7 @@ __device__ __noinline__ void node_desc(LDS uint32_t *d, const int32_t *tab, int 
     d[3] = bits;
 }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int work)
 {
     const int tid = threadIdx.x, N = M->N;
@@ -1122,7 +1132,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, const ChainArgs *A, int
 
 // ------------------------------------------------------------- the chain
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void init_model(MP M, CP<R> c, int resZ)
 {
     // BPP_C::initMCMC (src/PhyloAcc-ST/bpp_c.cpp:131-343), iter = 0
@@ -1162,7 +1172,7 @@ __device__ __noinline__ void init_model(MP M, CP<R> c, int resZ)
         class_table<R>(M, c, 2, s, nr, 0);
     }
     __syncthreads();
-    refresh_PTc<R, BS>(c, -1);
+    refresh_PTc<R, BS, SH>(c, -1);
     __syncthreads();
 }
 
@@ -1185,7 +1195,7 @@ __device__ __noinline__ void monitor(MP M, CP<R> c, int m, int resZ, double *trd
     else if (isinf(c->loglik)) c->fail |= c->loglik > 0 ? PA_F_POS_INF : PA_F_NEG_INF;
 }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __noinline__ void write_results(MP M, CP<R> c, const ChainArgs *A, int ms, int work)
 {
     const int tid = threadIdx.x, N = M->N;
@@ -1201,15 +1211,15 @@ __device__ __noinline__ void write_results(MP M, CP<R> c, const ChainArgs *A, in
     for (int i = tid; i < N; i += BS) mz[i] = c->maxZ[i];
 }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __device__ __forceinline__ void run_model(MP M, CP<R> c, const ChainArgs *A, int ms, int work)
 {
     const int tid = threadIdx.x;
     const int resZ = ms == 0 ? 0 : (ms == 1 ? 2 : 1);
     const int T = M->T, N = M->N;
-    init_model<R, BS>(M, c, resZ);
+    init_model<R, BS, SH>(M, c, resZ);
     {
-        const double l0 = prune<R, BS>(M, c, PR_FULL, 0);
+        const double l0 = prune<R, BS, SH>(M, c, PR_FULL, 0);
         if (tid == 0) c->loglik = l0;
         __syncthreads();
     }
@@ -1220,24 +1230,24 @@ __device__ __forceinline__ void run_model(MP M, CP<R> c, const ChainArgs *A, int
             for (int i = tid; i < N; i += BS) trz[(size_t)m * N + i] = c->Z[i];
         if (tid == 0) monitor<R>(M, c, m, resZ, trd, T);
         if (m == T - 1) break;
-        sample_sites<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_ANC));
+        sample_sites<R, BS, SH>(M, c, (uint32_t)m, tagword(ms, TAG_ANC));
         __syncthreads();
-        emissions<R, BS>(c);
+        emissions<R, BS, SH>(c);
         __syncthreads();
-        update_Z<R, BS>(M, c, (uint32_t)m, tagword(ms, TAG_Z));
-        refresh_PTc<R, BS>(c, REFRESH_CHANGED);
+        update_Z<R, BS, SH>(M, c, (uint32_t)m, tagword(ms, TAG_Z));
+        refresh_PTc<R, BS, SH>(c, REFRESH_CHANGED);
         __syncthreads();
         if (c->nch) {
-            const double l = prune<R, BS>(M, c, PR_DIRTY, 0);
+            const double l = prune<R, BS, SH>(M, c, PR_DIRTY, 0);
             if (tid == 0) c->loglik = l;
             __syncthreads();
         }
-        sample_rate<R, BS>(M, c, ms, m, resZ, true);
-        sample_rate<R, BS>(M, c, ms, m, resZ, false);
-        sample_transition<R, BS>(M, c, ms, m);
+        sample_rate<R, BS, SH>(M, c, ms, m, resZ, true);
+        sample_rate<R, BS, SH>(M, c, ms, m, resZ, false);
+        sample_transition<R, BS, SH>(M, c, ms, m);
     }
     __syncthreads();
-    write_results<R, BS>(M, c, A, ms, work);
+    write_results<R, BS, SH>(M, c, A, ms, work);
     __syncthreads();
 }
 
@@ -1249,7 +1259,7 @@ __device__ __forceinline__ MP model_to_lds(const ModelDev &Mk, LDS ModelDev *Ms)
     return (MP)Ms;
 }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 __global__ void __launch_bounds__(BS, 4) chain_kernel(ModelDev Mk, ChainArgs A)
 {
     extern __shared__ __align__(16) char smem_raw[];
@@ -1266,8 +1276,8 @@ __global__ void __launch_bounds__(BS, 4) chain_kernel(ModelDev Mk, ChainArgs A)
         const int w = c->work;
         __syncthreads();
         if (w >= A.n_work) break;
-        bind_element<R, BS>(M, c, &A, w);
-        for (int ms = 0; ms < 3; ms++) run_model<R, BS>(M, c, &A, ms, w);
+        bind_element<R, BS, SH>(M, c, &A, w);
+        for (int ms = 0; ms < 3; ms++) run_model<R, BS, SH>(M, c, &A, ms, w);
         if (threadIdx.x == 0 && A.counters) atomicAdd(A.counters, c->bytes);
         if (threadIdx.x == 0 && A.done) {
             // thread 0 wrote the result rows: make them visible to the host
@@ -1280,13 +1290,13 @@ __global__ void __launch_bounds__(BS, 4) chain_kernel(ModelDev Mk, ChainArgs A)
 
 __global__ void reset_queue(int *q) { *q = 0; }
 
-template <typename R, int BS>
+template <typename R, int BS, int SH>
 static hipError_t launch_t(const ModelDev &M, const ChainArgs &A, int grid, hipStream_t st)
 {
     const size_t sm = smem_arrays<R, false>(M.N, M.S, BS, nullptr, nullptr);
-    const hipError_t err = hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    const hipError_t err = hipFuncSetAttribute((const void *)chain_kernel<R, BS, SH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((chain_kernel<R, BS>), dim3(grid), dim3(BS), sm, st, M, A);
+    hipLaunchKernelGGL((chain_kernel<R, BS, SH>), dim3(grid), dim3(BS), sm, st, M, A);
     return hipGetLastError();
 }
 
@@ -1296,43 +1306,75 @@ hipError_t launch_reset(int *q, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename R, int BS>
+// Zeroes n 32-bit words: a kernel, so that it can be captured in a HIP graph
+// where a memset node would not be used.
+__global__ void clear_words(uint32_t *p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_clear(void *p, int words, hipStream_t st)
+{
+    hipLaunchKernelGGL(clear_words, dim3(1), dim3(64), 0, st, (uint32_t *)p, words);
+    return hipGetLastError();
+}
+
+template <typename R, int BS, int SH>
 static int blocks_t(int N, int S)
 {
     const size_t sm = smem_arrays<R, false>(N, S, BS, nullptr, nullptr);
-    if (hipFuncSetAttribute((const void *)chain_kernel<R, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
+    if (hipFuncSetAttribute((const void *)chain_kernel<R, BS, SH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
         return 0;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<R, BS>, BS, sm) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<R, BS, SH>, BS, sm) != hipSuccess) return 0;
     return nb;
 }
 
-int chain_blocks_per_cu(int fp64, int BS, int N, int S)
+// Runtime (precision, block size, shape) -> template instance.
+template <typename R, int SH, typename F>
+static auto by_block(int BS, F &&f) -> decltype(f(std::integral_constant<int, 64>()))
 {
-    if (fp64) {
-        if (BS == 64) return blocks_t<double, 64>(N, S);
-        if (BS == 128) return blocks_t<double, 128>(N, S);
-        if (BS == 256) return blocks_t<double, 256>(N, S);
-    } else {
-        if (BS == 64) return blocks_t<float, 64>(N, S);
-        if (BS == 128) return blocks_t<float, 128>(N, S);
-        if (BS == 256) return blocks_t<float, 256>(N, S);
-    }
-    return 0;
+    if (BS == 64) return f(std::integral_constant<int, 64>());
+    if (BS == 128) return f(std::integral_constant<int, 128>());
+    return f(std::integral_constant<int, 256>());
 }
 
-hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st)
+template <typename R, typename F>
+static auto by_shape(int shape, F &&f) -> decltype(f(std::integral_constant<int, 0>()))
 {
-    if (fp64) {
-        if (BS == 64) return launch_t<double, 64>(M, A, grid, st);
-        if (BS == 128) return launch_t<double, 128>(M, A, grid, st);
-        if (BS == 256) return launch_t<double, 256>(M, A, grid, st);
-    } else {
-        if (BS == 64) return launch_t<float, 64>(M, A, grid, st);
-        if (BS == 128) return launch_t<float, 128>(M, A, grid, st);
-        if (BS == 256) return launch_t<float, 256>(M, A, grid, st);
-    }
-    return hipErrorInvalidValue;
+    if (shape == 1) return f(std::integral_constant<int, 1>());
+    if (shape == 2) return f(std::integral_constant<int, 2>());
+    return f(std::integral_constant<int, 0>());
+}
+
+template <typename R>
+static int blocks_r(int BS, int shape, int N, int S)
+{
+    return by_shape<R>(shape, [&](auto sh) {
+        return by_block<R, decltype(sh)::value>(BS, [&](auto bs) { return blocks_t<R, decltype(bs)::value, decltype(sh)::value>(N, S); });
+    });
+}
+
+template <typename R>
+static hipError_t launch_r(const ModelDev &M, const ChainArgs &A, int BS, int shape, int grid, hipStream_t st)
+{
+    return by_shape<R>(shape, [&](auto sh) {
+        return by_block<R, decltype(sh)::value>(BS, [&](auto bs) {
+            return launch_t<R, decltype(bs)::value, decltype(sh)::value>(M, A, grid, st);
+        });
+    });
+}
+
+int chain_blocks_per_cu(int fp64, int BS, int N, int S, int shape)
+{
+    if ((BS != 64 && BS != 128 && BS != 256) || shape < 0 || shape >= N_SHAPES) return 0;
+    return fp64 ? blocks_r<double>(BS, shape, N, S) : blocks_r<float>(BS, shape, N, S);
+}
+
+hipError_t launch_chain(const ModelDev &M, const ChainArgs &A, int fp64, int BS, int grid, hipStream_t st, int shape)
+{
+    if ((BS != 64 && BS != 128 && BS != 256) || shape < 0 || shape >= N_SHAPES) return hipErrorInvalidValue;
+    return fp64 ? launch_r<double>(M, A, BS, shape, grid, st) : launch_r<float>(M, A, BS, shape, grid, st);
 }
 
 }  // namespace chain

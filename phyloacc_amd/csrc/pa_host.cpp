@@ -433,6 +433,8 @@ struct pa_engine {
     std::string err;
     std::vector<HostElem> elems;
     bool dev_dirty = true;
+    uint64_t arena_epoch = 0;       // counts uploads: prepared batches hold pointers into one generation
+    std::vector<struct pa_batch *> batches;   // prepared batches, freed with the engine
     // device arenas
     ElemDev *d_elems = nullptr;
     uint8_t *d_codes = nullptr;
@@ -441,7 +443,7 @@ struct pa_engine {
     hipStream_t stream = nullptr;
     int n_cu = 0;
     size_t lds_per_cu = 0;
-    pa_tuning tune = {0, 0, 0, 0.0};        // block size of batched chain launches; 0: by mean pattern count
+    pa_tuning tune = {0, 0, 0, 0, 0.0};     // batched chain launches: block size (0: by mean pattern count), shape
     uint64_t arena_gen = 0;                  // bumped whenever the device arenas move
     // log-likelihood calls: scratch slots and per-call branch lengths, kept
     // between calls and grown when a call needs more (freed with the engine)
@@ -587,9 +589,12 @@ static void free_arenas(pa_engine *e)
     e->d_elems = nullptr; e->d_codes = nullptr; e->d_w = e->d_site = e->d_tab = nullptr;
 }
 
+static void batch_free(pa_batch *b);
+
 extern "C" void pa_engine_destroy(pa_engine *e)
 {
     if (!e) return;
+    while (!e->batches.empty()) batch_free(e->batches.back());
     if (e->stream) {
         (void)hipSetDevice(e->device);
         (void)hipStreamSynchronize(e->stream);
@@ -756,6 +761,7 @@ static pa_status upload(pa_engine *e)
         if (!E.filtered) { e->max_ppad = std::max(e->max_ppad, E.Ppad); e->mean_p += E.P; kept++; }
     if (kept) e->mean_p /= (double)kept;
     e->dev_dirty = false;
+    e->arena_epoch++;
     e->arena_gen++;
     return PA_OK;
 }
@@ -787,10 +793,10 @@ static int fit_per_cu(pa_engine *e, size_t smem, int BS, int queried)
     const int bound = std::max(1, std::min(per_cu, 2048 / BS));
     return queried > 0 ? std::min(queried, bound) : bound;
 }
-static int chain_grid_max(pa_engine *e, int fp64, int BS)
+static int chain_grid_max(pa_engine *e, int fp64, int BS, int shape)
 {
     const int N = e->tm.N, S = e->tm.S;
-    return e->n_cu * fit_per_cu(e, chain::chain_smem_bytes(N, S, BS, fp64), BS, chain::chain_blocks_per_cu(fp64, BS, N, S));
+    return e->n_cu * fit_per_cu(e, chain::chain_smem_bytes(N, S, BS, fp64), BS, chain::chain_blocks_per_cu(fp64, BS, N, S, shape));
 }
 static int loglik_grid_max(pa_engine *e, int fp64, int BS)
 {
@@ -899,11 +905,12 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
                                   chain::ChainArgs &A, DevBuf *bufs, pa_run_stats *stats)
 {
     const int S = e->tm.S, N = e->tm.N, T = e->M.T;
-    int BS = bs_req ? bs_req : (e->tune.small_threads ? e->tune.small_threads : pick_block(e, ids, n));
+    int BS = bs_req ? bs_req : (e->tune.block_threads ? e->tune.block_threads : pick_block(e, ids, n));
     if (BS != 64 && BS != 128 && BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
     const size_t smem = chain::chain_smem_bytes(N, S, BS, fp64);
     if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
-    const int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, chain_grid_max(e, fp64, BS));
+    const int shape = e->tune.shape;
+    const int grid = mode == PA_RUN_PER_ELEMENT ? 1 : std::min(n, chain_grid_max(e, fp64, BS, shape));
     std::vector<int> ppad(n);
     for (int i = 0; i < n; i++) ppad[i] = e->elems[ids[i]].Ppad;
     size_t off[4];
@@ -943,13 +950,13 @@ static pa_status run_chain_kernel(pa_engine *e, const int32_t *ids, int32_t n, i
             B.tr_d = A.tr_d ? A.tr_d + (size_t)i * 3 * 7 * T : nullptr;
             B.tr_Z = A.tr_Z ? A.tr_Z + (size_t)i * 3 * T * N : nullptr;
             HIPCHK(e, launch_reset(e->d_queue, e->stream));
-            HIPCHK(e, chain::launch_chain(e->M, B, fp64, BS, 1, e->stream));
+            HIPCHK(e, chain::launch_chain(e->M, B, fp64, BS, 1, e->stream, shape));
         }
     } else {
         A.work = (const int32_t *)bufs[1].p;
         A.n_work = n;
         HIPCHK(e, launch_reset(e->d_queue, e->stream));
-        HIPCHK(e, chain::launch_chain(e->M, A, fp64, BS, grid, e->stream));
+        HIPCHK(e, chain::launch_chain(e->M, A, fp64, BS, grid, e->stream, shape));
     }
     HIPCHK(e, hipEventRecord(t1, e->stream));
     HIPCHK(e, hipEventSynchronize(t1));
@@ -1253,6 +1260,186 @@ extern "C" pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, 
     return PA_OK;
 }
 
+// ---------------------------------------------------- captured chain batches
+//
+// A prepared batched chain run: the schedule, the scratch plan and every
+// device buffer are set up once, and the launch sequence -- a kernel that
+// clears the work and byte counters, the persistent chain kernel, and the
+// copies of the result rows into pinned host memory -- is captured once as a
+// HIP graph on the engine's stream.  pa_batch_run replays it.  The chains
+// draw from the same counter-based streams every time, so each replay gives
+// pa_run_chains's results for the same ids and options.
+
+struct pa_batch {
+    pa_engine *e = nullptr;
+    int n = 0, fp64 = 0, BS = 0, shape = 0, grid = 0, lds = 0;
+    uint64_t epoch = 0;
+    size_t scratch_bytes = 0;
+    std::vector<int32_t> order;
+    DevBuf scratch, meta, res_d, res_i, maxZ, tr_d, tr_Z;
+    double *h_res_d = nullptr;
+    int32_t *h_res_i = nullptr;
+    unsigned long long *h_cnt = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    ~pa_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        (void)hipHostFree(h_res_d);
+        (void)hipHostFree(h_res_i);
+        (void)hipHostFree(h_cnt);
+    }
+};
+
+// Enqueues the batch's launch sequence on the engine stream (under capture).
+static hipError_t batch_sequence(pa_batch *b, const chain::ChainArgs &A, void *clear_at, int clear_words)
+{
+    pa_engine *e = b->e;
+    const size_t n = (size_t)b->n;
+    hipError_t err;
+    if ((err = chain::launch_clear(clear_at, clear_words, e->stream)) != hipSuccess) return err;
+    if ((err = chain::launch_chain(e->M, A, b->fp64, b->BS, b->grid, e->stream, b->shape)) != hipSuccess) return err;
+    if ((err = hipMemcpyAsync(b->h_res_d, b->res_d.p, n * 3 * chain::RES_D * 8, hipMemcpyDeviceToHost, e->stream)) != hipSuccess)
+        return err;
+    if ((err = hipMemcpyAsync(b->h_res_i, b->res_i.p, n * 3 * chain::RES_I * 4, hipMemcpyDeviceToHost, e->stream)) != hipSuccess)
+        return err;
+    return hipMemcpyAsync(b->h_cnt, A.counters, 8, hipMemcpyDeviceToHost, e->stream);
+}
+
+extern "C" pa_status pa_batch_create(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg, pa_batch **out)
+{
+    if (out) *out = nullptr;
+    if (!e || !ids || !cfg || !out || n <= 0) return fail(e, PA_E_INPUT, "null argument or empty batch");
+    if (cfg->mode != PA_RUN_BATCHED) return fail(e, PA_E_INPUT, "a prepared batch is one batched launch (PA_RUN_BATCHED)");
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    std::unique_ptr<pa_batch> b(new pa_batch());
+    b->e = e;
+    b->n = n;
+    b->fp64 = cfg->precision == PA_FP64;
+    b->epoch = e->arena_epoch;
+    b->order = schedule(e, ids, n, PA_RUN_BATCHED);
+    std::vector<int32_t> sorted(n);
+    for (int i = 0; i < n; i++) sorted[i] = ids[b->order[i]];
+    const int S = e->tm.S, N = e->tm.N;
+    int BS = cfg->block_threads ? cfg->block_threads : (e->tune.block_threads ? e->tune.block_threads : pick_block(e, sorted.data(), n));
+    if (BS != 64 && BS != 128 && BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
+    b->BS = BS;
+    b->lds = (int)chain::chain_smem_bytes(N, S, BS, b->fp64);
+    if ((size_t)b->lds > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+    b->shape = e->tune.shape;
+    b->grid = std::min(n, chain_grid_max(e, b->fp64, BS, b->shape));
+    std::vector<int> ppad(n);
+    for (int i = 0; i < n; i++) ppad[i] = e->elems[sorted[i]].Ppad;
+    size_t off[4];
+    const size_t fixed = chain::slot_scratch_bytes(N, S, 0, b->fp64, off);
+    const ScratchPlan pl = plan_scratch(ppad, b->grid, fixed, [&](int p) { return chain::chain_part_bytes(N, S, p, b->fp64); });
+    chain::slot_scratch_bytes(N, S, pl.cap, b->fp64, off);
+    b->scratch_bytes = pl.total;
+    HIPCHK(e, hipMalloc(&b->scratch.p, pl.total));
+    const size_t ids_b = ((size_t)4 * n + 15) & ~(size_t)15;
+    HIPCHK(e, hipMalloc(&b->meta.p, ids_b + (size_t)8 * n + 64));
+    int64_t *d_bo = (int64_t *)((char *)b->meta.p + ids_b);
+    unsigned long long *d_cnt = (unsigned long long *)(d_bo + n);   // [0]: bytes, [1]: spare
+    int *d_queue = (int *)(d_cnt + 2);
+    HIPCHK(e, hipMemcpy(b->meta.p, sorted.data(), (size_t)4 * n, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d_bo, pl.big_off.data(), (size_t)8 * n, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMalloc(&b->res_d.p, (size_t)n * 3 * chain::RES_D * 8));
+    HIPCHK(e, hipMalloc(&b->res_i.p, (size_t)n * 3 * chain::RES_I * 4));
+    HIPCHK(e, hipMalloc(&b->maxZ.p, (size_t)n * 3 * N));
+    const int T = e->M.T;
+    if (cfg->keep_traces) {        // traces stay on the device, as the reference keeps them in memory
+        HIPCHK(e, hipMalloc(&b->tr_d.p, (size_t)n * 3 * 7 * T * 8));
+        HIPCHK(e, hipMalloc(&b->tr_Z.p, (size_t)n * 3 * T * N));
+    }
+    HIPCHK(e, hipHostMalloc(&b->h_res_d, (size_t)n * 3 * chain::RES_D * 8));
+    HIPCHK(e, hipHostMalloc(&b->h_res_i, (size_t)n * 3 * chain::RES_I * 4));
+    HIPCHK(e, hipHostMalloc(&b->h_cnt, 16));
+    HIPCHK(e, hipEventCreate(&b->t0));
+    HIPCHK(e, hipEventCreate(&b->t1));
+    chain::ChainArgs A;
+    memset(&A, 0, sizeof A);
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.site = e->d_site; A.tab = e->d_tab;
+    A.work = (const int32_t *)b->meta.p; A.n_work = n; A.queue = d_queue;
+    A.scratch = (char *)b->scratch.p; A.slot_bytes = (int64_t)pl.slot;
+    A.off_plin = off[0]; A.off_logt = off[1]; A.off_amb = off[2]; A.off_parts = off[3];
+    A.big = (char *)b->scratch.p + pl.slots;
+    A.big_off = pl.big ? d_bo : nullptr;
+    A.res_d = (double *)b->res_d.p; A.res_i = (int32_t *)b->res_i.p; A.maxZ = (int8_t *)b->maxZ.p;
+    A.tr_d = (double *)b->tr_d.p; A.tr_Z = (int8_t *)b->tr_Z.p;
+    A.counters = d_cnt;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+    const hipError_t qerr = batch_sequence(b.get(), A, d_cnt, 6);   // bytes, spare, queue (+pad)
+    hipGraph_t g = nullptr;
+    const hipError_t cerr = hipStreamEndCapture(e->stream, &g);
+    if (qerr != hipSuccess || cerr != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(e, PA_E_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(qerr != hipSuccess ? qerr : cerr));
+    }
+    b->graph = g;
+    HIPCHK(e, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    e->batches.push_back(b.get());
+    *out = b.release();
+    return PA_OK;
+}
+
+extern "C" pa_status pa_batch_run(pa_batch *b, pa_chain_result *results, pa_run_stats *stats)
+{
+    if (!b) return PA_E_INPUT;
+    pa_engine *e = b->e;
+    if (e->dev_dirty || e->arena_epoch != b->epoch)
+        return fail(e, PA_E_STATE, "elements were added after the batch was prepared; prepare it again");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipEventRecord(b->t0, e->stream));
+    HIPCHK(e, hipGraphLaunch(b->exec, e->stream));
+    HIPCHK(e, hipEventRecord(b->t1, e->stream));
+    HIPCHK(e, hipEventSynchronize(b->t1));
+    if (results)
+        for (int i = 0; i < b->n; i++)
+            for (int k = 0; k < 3; k++) {
+                const double *d = b->h_res_d + ((size_t)i * 3 + k) * chain::RES_D;
+                const int32_t *q = b->h_res_i + ((size_t)i * 3 + k) * chain::RES_I;
+                pa_chain_result &r = results[(size_t)b->order[i] * 3 + k];
+                r.max_loglik = d[0]; r.final_nrate = d[1]; r.final_crate = d[2];
+                r.final_grate = d[3]; r.final_lrate = d[4]; r.final_lrate2 = d[5];
+                r.max_m = q[0]; r.status = q[1]; r.accept_n = q[2]; r.accept_c = q[3]; r.flags = q[4]; r.pad = 0;
+            }
+    if (stats) {
+        float ms = 0;
+        HIPCHK(e, hipEventElapsedTime(&ms, b->t0, b->t1));
+        stats->elapsed_ms = ms;
+        stats->algorithmic_bytes = (double)*b->h_cnt;
+        stats->grid = b->grid;
+        stats->block_threads = b->BS;
+        stats->lds_bytes = b->lds;
+        stats->n_launches = 1;
+        stats->scratch_bytes = (double)b->scratch_bytes;
+    }
+    return PA_OK;
+}
+
+static void batch_free(pa_batch *b)
+{
+    pa_engine *e = b->e;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    e->batches.erase(std::remove(e->batches.begin(), e->batches.end(), b), e->batches.end());
+    delete b;
+}
+
+// Batches still alive when their engine is destroyed are freed with it.
+extern "C" void pa_batch_destroy(pa_batch *b)
+{
+    if (b) batch_free(b);
+}
+
 // ------------------------------------------------------- chain summaries
 
 namespace {
@@ -1418,23 +1605,25 @@ extern "C" pa_status pa_chain_summary(pa_engine *e, int32_t id, int32_t model, c
 
 // ------------------------------------------------------------- autotuner
 //
-// Batched chain launches use one block size.  pa_autotune times each
-// candidate on the batch itself with short chains and keeps the fastest for
-// later pa_run_chains calls with block_threads = 0.  Split launches (two
-// block sizes side by side on two streams, small_threads != long_threads)
-// are switched off in this build: their chain kernel did not finish on the
-// GPU and its failure is not explained yet.
+// Batched chain launches use one block size and one pattern-loop shape.
+// pa_autotune times each pair on the batch itself with short chains and
+// keeps the fastest for later calls with block_threads = 0.
+
+static void shape_dims(int fp64, int shape, int32_t *U, int32_t *K)
+{
+    static const int u32[3] = {2, 1, 2}, k32[3] = {4, 4, 8}, u64[3] = {1, 2, 1}, k64[3] = {2, 2, 4};
+    *U = fp64 ? u64[shape] : u32[shape];
+    *K = fp64 ? k64[shape] : k32[shape];
+}
 
 extern "C" pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t)
 {
     if (!e || !t) return fail(e, PA_E_INPUT, "null argument");
-    if (t->small_threads == 0 && t->long_threads == 0) { e->tune = {0, 0, 0, 0.0}; return PA_OK; }
-    auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
-    if (!ok(t->small_threads) || !ok(t->long_threads) || t->long_patterns < 0)
-        return fail(e, PA_E_INPUT, "tuning: block sizes must be 64, 128 or 256");
-    if (t->small_threads != t->long_threads)
-        return fail(e, PA_E_STATE, "size classes (small_threads != long_threads) are switched off in this build");
+    if (t->block_threads != 0 && t->block_threads != 64 && t->block_threads != 128 && t->block_threads != 256)
+        return fail(e, PA_E_INPUT, "tuning: block_threads must be 0, 64, 128 or 256");
+    if (t->shape < 0 || t->shape >= chain::N_SHAPES) return fail(e, PA_E_INPUT, "tuning: shape out of range");
     e->tune = *t;
+    shape_dims(0, t->shape, &e->tune.round_nodes, &e->tune.chunk_nodes);
     return PA_OK;
 }
 
@@ -1469,24 +1658,31 @@ extern "C" pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, in
     e->M.T = states;
     e->M.burnin = states / 4;
     e->M.mcmc = states - states / 4;
-    pa_tuning best = {0, 0, 0, -1.0};
+    pa_tuning best = {0, 0, 0, 0, -1.0};
     const int cand[3] = {64, 128, 256};
-    for (int a = 0; a < 3 && st == PA_OK; a++) {
-        DevBuf bufs[8];
-        pa_run_stats rs;
-        chain::ChainArgs A;
-        memset(&A, 0, sizeof A);
-        if (hipMalloc(&bufs[3].p, (size_t)m * 3 * chain::RES_D * 8) != hipSuccess ||
-            hipMalloc(&bufs[4].p, (size_t)m * 3 * chain::RES_I * 4) != hipSuccess ||
-            hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) { st = fail(e, PA_E_OOM, "autotune buffers"); break; }
-        A.res_d = (double *)bufs[3].p;
-        A.res_i = (int32_t *)bufs[4].p;
-        A.maxZ = (int8_t *)bufs[5].p;
-        st = run_chain_kernel(e, sorted.data(), m, precision == PA_FP64, PA_RUN_BATCHED, cand[a], A, bufs, &rs);
-        if (st != PA_OK) break;
-        const double rate = (double)m * 3 * (states - 1) / (rs.elapsed_ms / 1e3);
-        if (rate > best.rate) best = {cand[a], cand[a], 0, rate};
-    }
+    const pa_tuning keep = e->tune;
+    for (int a = 0; a < 3 && st == PA_OK; a++)
+        for (int sh = 0; sh < chain::N_SHAPES && st == PA_OK; sh++) {
+            DevBuf bufs[8];
+            pa_run_stats rs;
+            chain::ChainArgs A;
+            memset(&A, 0, sizeof A);
+            if (hipMalloc(&bufs[3].p, (size_t)m * 3 * chain::RES_D * 8) != hipSuccess ||
+                hipMalloc(&bufs[4].p, (size_t)m * 3 * chain::RES_I * 4) != hipSuccess ||
+                hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) { st = fail(e, PA_E_OOM, "autotune buffers"); break; }
+            A.res_d = (double *)bufs[3].p;
+            A.res_i = (int32_t *)bufs[4].p;
+            A.maxZ = (int8_t *)bufs[5].p;
+            e->tune.shape = sh;
+            st = run_chain_kernel(e, sorted.data(), m, precision == PA_FP64, PA_RUN_BATCHED, cand[a], A, bufs, &rs);
+            if (st != PA_OK) break;
+            const double rate = (double)m * 3 * (states - 1) / (rs.elapsed_ms / 1e3);
+            if (rate > best.rate) {
+                best = {cand[a], sh, 0, 0, rate};
+                shape_dims(precision == PA_FP64, sh, &best.round_nodes, &best.chunk_nodes);
+            }
+        }
+    e->tune = keep;
     e->M = saved;
     if (st != PA_OK) return st;
     e->tune = best;
@@ -1608,7 +1804,7 @@ struct Lane {
 
 struct pa_server {
     pa_engine *e = nullptr;
-    int fp64 = 0, BS = 128, max_chains = 4096;
+    int fp64 = 0, BS = 128, shape = 0, max_chains = 4096;
     size_t pool_bytes = 0;
     char *d_pool = nullptr;
     PoolMap pool;
@@ -1688,6 +1884,7 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     s->e = e;
     s->fp64 = cfg->precision == PA_FP64;
     s->BS = cfg->block_threads ? cfg->block_threads : 128;
+    s->shape = e->tune.shape;
     if (s->BS != 64 && s->BS != 128 && s->BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
     const int N = e->tm.N, S = e->tm.S;
     if (chain::chain_smem_bytes(N, S, s->BS, s->fp64) > e->lds_per_cu || loglik_smem_bytes(N, s->fp64) > e->lds_per_cu)
@@ -1778,7 +1975,7 @@ static pa_status lane_launch(pa_server *s, Lane &L, bool others_busy)
 {
     pa_engine *e = s->e;
     const int N = e->tm.N, S = e->tm.S;
-    const int gmax_c = chain_grid_max(e, s->fp64, s->BS);
+    const int gmax_c = chain_grid_max(e, s->fp64, s->BS, s->shape);
     const int gmax_l = loglik_grid_max(e, s->fp64, s->BS);
     const size_t room = s->pool.largest();
     const double now = pa_serve_clock();
@@ -1858,7 +2055,7 @@ static pa_status lane_launch(pa_server *s, Lane &L, bool others_busy)
         A.res_d = L.v_res_d; A.res_i = L.v_res_i; A.maxZ = L.d_maxZ;
         A.done = L.v_done;
         HIPCHK(e, chain::launch_reset(L.d_queue, L.st));
-        HIPCHK(e, chain::launch_chain(e->M, A, s->fp64, s->BS, gc, L.st));
+        HIPCHK(e, chain::launch_chain(e->M, A, s->fp64, s->BS, gc, L.st, s->shape));
     }
     if (nl) {
         for (int i = 0; i < nl; i++) {
@@ -1974,7 +2171,7 @@ static pa_status serve_admit(pa_server *s)
         if ((st = serve_collect(s, true, nullptr)) != PA_OK) return st;   // no lane may read the old arenas
         if ((st = upload(e)) != PA_OK) return st;
     }
-    const int gmax_c = chain_grid_max(e, s->fp64, s->BS);
+    const int gmax_c = chain_grid_max(e, s->fp64, s->BS, s->shape);
     for (Lane &L : s->lanes) {
         if (L.busy) continue;
         if (s->waiting.empty() && s->lwait.empty()) break;

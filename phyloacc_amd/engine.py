@@ -31,7 +31,8 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_chain_summary", "pa_elements_add_trees", "pa_gene_lengths", "pa_loglik_lengths",
            "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
            "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
-           "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc"]
+           "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc",
+           "pa_batch_create", "pa_batch_run", "pa_batch_destroy"]
 
 
 class PaError(RuntimeError):
@@ -96,7 +97,7 @@ class RunConfig(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("small_threads", C.c_int32), ("long_threads", C.c_int32), ("long_patterns", C.c_int32),
+    _fields_ = [("block_threads", C.c_int32), ("shape", C.c_int32), ("round_nodes", C.c_int32), ("chunk_nodes", C.c_int32),
                 ("rate", C.c_double)]
 
 
@@ -169,6 +170,12 @@ def lib():
         L.pa_loglik_gbgc.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double),
                                      C.c_int32, P(C.c_double)]
         L.pa_loglik_gbgc.restype = C.c_int
+        L.pa_batch_create.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(C.c_void_p)]
+        L.pa_batch_create.restype = C.c_int
+        L.pa_batch_run.argtypes = [C.c_void_p, P(ChainResult), P(RunStats)]
+        L.pa_batch_run.restype = C.c_int
+        L.pa_batch_destroy.argtypes = [C.c_void_p]
+        L.pa_batch_destroy.restype = None
         L.pa_set_tuning.argtypes = [C.c_void_p, P(Tuning)]
         L.pa_set_tuning.restype = C.c_int
         L.pa_get_tuning.argtypes = [C.c_void_p, P(Tuning)]
@@ -361,17 +368,18 @@ class Engine:
         return d
 
     def autotune(self, ids, precision=PA_FP32, sample=0, states=8):
-        """Time block sizes 64/128/256 on the batch (sample > 0: on that many
-        elements evenly spaced by pattern count) with short chains; the fastest
-        is kept for run_chains(block_threads=0)."""
+        """Time every block size (64/128/256) and pattern-loop shape on the
+        batch (sample > 0: on that many elements evenly spaced by pattern
+        count) with short chains; the fastest pair is kept for
+        run_chains(block_threads=0) and Batch(block_threads=0)."""
         ids = np.ascontiguousarray(ids, np.int32)
         t = Tuning()
         self._check(lib().pa_autotune(self._h, _p(ids, C.c_int32), len(ids), int(precision), int(sample),
                                       int(states), C.byref(t)))
         return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
-    def set_tuning(self, small_threads=0, long_threads=0, long_patterns=0):
-        self._check(lib().pa_set_tuning(self._h, C.byref(Tuning(small_threads, long_threads, long_patterns, 0.0))))
+    def set_tuning(self, block_threads=0, shape=0):
+        self._check(lib().pa_set_tuning(self._h, C.byref(Tuning(block_threads, shape, 0, 0, 0.0))))
 
     def tuning(self):
         t = Tuning()
@@ -438,6 +446,45 @@ class Engine:
                      ("accept_c", np.int32), ("flags", np.int32)):
             out[f] = np.array([getattr(res[i], f) for i in range(3 * n)], t).reshape(n, 3)
         return out
+
+
+RESULT_FIELDS = (("max_loglik", np.float64), ("max_m", np.int32), ("status", np.int32),
+                 ("final_nrate", np.float64), ("final_crate", np.float64), ("final_grate", np.float64),
+                 ("final_lrate", np.float64), ("final_lrate2", np.float64), ("accept_n", np.int32),
+                 ("accept_c", np.int32), ("flags", np.int32))
+
+
+class Batch:
+    """A batched chain run prepared once and captured as a HIP graph;
+    ``run()`` replays it (same results as ``Engine.run_chains`` on the ids)."""
+
+    def __init__(self, engine: "Engine", ids, precision=PA_FP32, block_threads=0, keep_traces=False):
+        self.engine = engine
+        self.ids = np.ascontiguousarray(ids, np.int32)
+        cfg = RunConfig(int(precision), int(PA_RUN_BATCHED), int(block_threads), int(bool(keep_traces)))
+        h = C.c_void_p()
+        engine._check(lib().pa_batch_create(engine._h, _p(self.ids, C.c_int32), len(self.ids), C.byref(cfg),
+                                            C.byref(h)))
+        self._h = h
+
+    def run(self):
+        n = len(self.ids)
+        res = (ChainResult * (3 * n))()
+        stats = RunStats()
+        self.engine._check(lib().pa_batch_run(self._h, res, C.byref(stats)))
+        out = {"elapsed_ms": stats.elapsed_ms, "stats": {f: getattr(stats, f) for f, _ in RunStats._fields_}}
+        for f, t in RESULT_FIELDS:
+            out[f] = np.array([getattr(res[i], f) for i in range(3 * n)], t).reshape(n, 3)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            if getattr(self.engine, "_h", None):    # a closed engine has freed its batches
+                lib().pa_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def gene_lengths(N, seg_gene, seg_species, seg_len, Z, r1, r2):
