@@ -488,6 +488,14 @@ def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32"):
             eng.loglik_lengths(ids, L, prec)
             times.append(time.time() - tw)
         rates[name] = (len(ids) / float(np.mean(times)), float(np.mean(times)))
+        if name == precision:
+            ks = eng.loglik_stats()     # the last timed call's kernel
+            ach = ks["algorithmic_bytes"] / (ks["elapsed_ms"] / 1e3) / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "loglik_kernel",
+                    "algorithmic_bytes_per_launch": ks["algorithmic_bytes"],
+                    "kernel_ms": round(ks["elapsed_ms"], 3), "grid": ks["grid"], "block_threads": ks["block_threads"],
+                    "lds_bytes": ks["lds_bytes"]}
     rng = np.random.default_rng(0)
     smp = rng.choice(len(ids), size=min(48, len(ids)), replace=False)
     m = O.parse_mod(w.mod_text)[0]
@@ -513,6 +521,7 @@ def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32"):
         "vs_oracle": {"sample": int(len(smp)),
                       "fp64_max_rel_err": float(np.max(np.abs(l64 - ref) / np.abs(ref))),
                       "fp32_max_rel_err": float(np.max(np.abs(l32 - ref) / np.abs(ref)))},
+        "roofline": roof,
     }
 
 

@@ -241,6 +241,12 @@ typedef struct {
 pa_status pa_run_chains(pa_engine *e, const int32_t *ids, int32_t n, const pa_run_config *cfg,
                         pa_chain_result *results, pa_traces *traces, pa_run_stats *stats);
 
+/* Launch statistics of the last pa_loglik / pa_loglik_lengths / pa_loglik_gbgc
+ * call (no reference counterpart): the likelihood kernel's HIP-event duration
+ * and its counted algorithmic HBM bytes (partials, exponents, leaf codes,
+ * weights), grid, block size, LDS and scratch bytes. */
+pa_status pa_loglik_last_stats(pa_engine *e, pa_run_stats *out);
+
 /* A prepared batched chain run, captured once as a HIP graph (no reference
  * counterpart: the reference loops over elements, src/PhyloAcc-ST/main.cpp:452-500).
  * pa_batch_create schedules the ids, plans and allocates the device scratch

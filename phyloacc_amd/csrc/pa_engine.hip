@@ -229,7 +229,7 @@ struct Unroll { static constexpr int U = sizeof(R) == 4 ? 4 : 2; };
 // round costs one memory latency whether the level is wide (many nodes) or
 // the element is long (many pattern blocks).
 template <typename R, int BS>
-__device__ __noinline__ double prune(MP M, CP<R> c)
+__device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
 {
     const int tid = threadIdx.x;
     const int P = uni(c->P), Ppad = uni(c->Ppad), S = uni(c->S);
@@ -245,6 +245,19 @@ __device__ __noinline__ double prune(MP M, CP<R> c)
         // level h: up[upoff[h-1] .. upoff[h]) (from 0 for the first level)
         const int a = uni(h == 0 ? 0 : c->upoff[h - 1]), b = uni(c->upoff[h]);
         const int items = (b - a) * Pb;
+        if (bytes) {
+            // write partial + exponent; read each child's partial + exponent,
+            // its leaf code, or a cherry's two leaf codes
+            for (int q = a; q < b; q++) {
+                const int s = uni(c->up[q]);
+                unsigned long long per = 4 * sizeof(R) + 4;
+                for (int j = 0; j < 2; j++) {
+                    const int x = uni(j ? c->ch1[s] : c->ch0[s]);
+                    per += x < S ? 1 : ((uni(c->flags[x]) & NF_CHERRY) ? 2 : 4 * sizeof(R) + 4);
+                }
+                *bytes += (unsigned long long)P * per;
+            }
+        }
         for (int i0 = 0; i0 < items; i0 += U) {
             ChildIn<R> in0[U], in1[U];
 #pragma unroll
@@ -449,8 +462,13 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
             continue;
         }
         bind_element<R, BS>(M, c, A, w);
-        const double l = prune<R, BS>(M, c);
-        if (threadIdx.x == 0) A->ll_out[w] = l;
+        unsigned long long bytes = 0;
+        const double l = prune<R, BS>(M, c, A->counters && threadIdx.x == 0 ? &bytes : nullptr);
+        if (threadIdx.x == 0) {
+            A->ll_out[w] = l;
+            // root partial + exponent + pattern weight
+            if (A->counters) atomicAdd(A->counters, bytes + (unsigned long long)c->P * (4 * sizeof(R) + 8));
+        }
         __syncthreads();
     }
 }

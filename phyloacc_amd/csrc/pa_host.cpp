@@ -435,6 +435,8 @@ struct pa_engine {
     bool dev_dirty = true;
     uint64_t arena_epoch = 0;       // counts uploads: prepared batches hold pointers into one generation
     std::vector<struct pa_batch *> batches;   // prepared batches, freed with the engine
+    hipEvent_t ll_t0 = nullptr, ll_t1 = nullptr;   // around the last log-likelihood launch
+    pa_run_stats ll_stats{};
     // device arenas
     ElemDev *d_elems = nullptr;
     uint8_t *d_codes = nullptr;
@@ -578,6 +580,8 @@ static pa_status ensure_device(pa_engine *e)
     e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
     HIPCHK(e, hipStreamCreate(&e->stream));
     HIPCHK(e, hipMalloc(&e->d_queue, 256));
+    HIPCHK(e, hipEventCreate(&e->ll_t0));
+    HIPCHK(e, hipEventCreate(&e->ll_t1));
     HIPCHK(e, noalloc_scratch(e->device, &e->d_noalloc));
     return PA_OK;
 }
@@ -602,6 +606,8 @@ extern "C" void pa_engine_destroy(pa_engine *e)
         (void)hipFree(e->d_queue);
         (void)hipFree(e->d_ll_scratch);
         (void)hipFree(e->d_ll_len);
+        if (e->ll_t0) (void)hipEventDestroy(e->ll_t0);
+        if (e->ll_t1) (void)hipEventDestroy(e->ll_t1);
         (void)hipStreamDestroy(e->stream);
     }
     delete e;
@@ -877,10 +883,13 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
         HIPCHK(e, hipMalloc(&e->d_ll_scratch, pl.total));
         e->ll_scratch_bytes = pl.total;
     }
-    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + sizeof(int64_t) * n + 16));
+    HIPCHK(e, hipMalloc(&bufs[1].p, sizeof(int32_t) * n + sizeof(int64_t) * n + 32));
     int64_t *d_bo = (int64_t *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
+    unsigned long long *d_cnt = (unsigned long long *)(d_bo + n);
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipMemcpyAsync(d_bo, pl.big_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemsetAsync(d_cnt, 0, 8, e->stream));
+    A.counters = d_cnt;
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
     A.n_elems = (int)e->elems.size();
     A.work = (const int32_t *)bufs[1].p;
@@ -891,9 +900,22 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     A.big = (char *)e->d_ll_scratch + pl.slots;
     A.big_off = pl.big ? d_bo : nullptr;
     HIPCHK(e, launch_reset(A.queue, e->stream));
+    HIPCHK(e, hipEventRecord(e->ll_t0, e->stream));
     HIPCHK(e, launch_loglik(e->M, A, fp64, BS, grid, e->stream));
+    HIPCHK(e, hipEventRecord(e->ll_t1, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->ll_last_plan_bytes = pl.total;
+    float ms = 0;
+    unsigned long long cnt = 0;
+    HIPCHK(e, hipEventElapsedTime(&ms, e->ll_t0, e->ll_t1));
+    HIPCHK(e, hipMemcpy(&cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+    e->ll_stats.elapsed_ms = ms;
+    e->ll_stats.algorithmic_bytes = (double)cnt;
+    e->ll_stats.grid = grid;
+    e->ll_stats.block_threads = BS;
+    e->ll_stats.lds_bytes = (int32_t)smem;
+    e->ll_stats.n_launches = 1;
+    e->ll_stats.scratch_bytes = (double)pl.total;
     return PA_OK;
 }
 
@@ -1091,6 +1113,16 @@ extern "C" pa_status pa_loglik_gbgc(pa_engine *e, const int32_t *ids, int32_t n,
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) out[order[i]] = ll[i];
     if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
+    return PA_OK;
+}
+
+// Launch statistics of the last pa_loglik / pa_loglik_lengths /
+// pa_loglik_gbgc call: the likelihood kernel's HIP-event duration and the
+// partial, code and weight bytes its passes had to move (counted on device).
+extern "C" pa_status pa_loglik_last_stats(pa_engine *e, pa_run_stats *out)
+{
+    if (!e || !out) return fail(e, PA_E_INPUT, "null argument");
+    *out = e->ll_stats;
     return PA_OK;
 }
 

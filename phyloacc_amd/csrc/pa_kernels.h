@@ -48,6 +48,7 @@ struct LoglikArgs {
     // gb_eig[w][k] = {lam[4], V[16], Vi[16]} (gbgc_eig_kernel).
     const double *gb_eig;     // [n_work][6][GB_EIG_WORDS]
     const int32_t *Bin;       // [n_work][N]
+    unsigned long long *counters;   // [0] += algorithmic HBM bytes of the launch, or null
 };
 
 constexpr int GB_CLASSES = 6, GB_EIG_WORDS = 36;

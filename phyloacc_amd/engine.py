@@ -32,7 +32,7 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
            "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
            "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc",
-           "pa_batch_create", "pa_batch_run", "pa_batch_destroy"]
+           "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats"]
 
 
 class PaError(RuntimeError):
@@ -170,6 +170,8 @@ def lib():
         L.pa_loglik_gbgc.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double),
                                      C.c_int32, P(C.c_double)]
         L.pa_loglik_gbgc.restype = C.c_int
+        L.pa_loglik_last_stats.argtypes = [C.c_void_p, P(RunStats)]
+        L.pa_loglik_last_stats.restype = C.c_int
         L.pa_batch_create.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(C.c_void_p)]
         L.pa_batch_create.restype = C.c_int
         L.pa_batch_run.argtypes = [C.c_void_p, P(ChainResult), P(RunStats)]
@@ -338,6 +340,12 @@ class Engine:
         self._check(lib().pa_loglik_gbgc(self._h, _p(ids, C.c_int32), len(ids), _p(Z, C.c_int32), _p(B, C.c_int32),
                                          _p(sel, C.c_double), int(precision), _p(out, C.c_double)))
         return out
+
+    def loglik_stats(self):
+        """Kernel time and counted bytes of the last likelihood call."""
+        st = RunStats()
+        self._check(lib().pa_loglik_last_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in RunStats._fields_}
 
     def release_scratch(self):
         """Free the device scratch the likelihood calls keep between calls."""

@@ -165,6 +165,43 @@ def test_gene_tree_loglik_gpu_vs_oracle_120_species(require_gpu):
     np.testing.assert_allclose(got32, ref, rtol=1e-6)
 
 
+def _counted_bytes(eng, ids, word):
+    """Bytes a full pruning pass must move, from each element's tree: for
+    every internal node that is not a cherry (a node whose two children are
+    leaves is recomputed by its parent), one partial + exponent written and,
+    per child, a partial + exponent, a leaf code or a cherry's two codes
+    read; at the root, its partial + exponent and the pattern weight."""
+    total = 0
+    for i in ids:
+        d = eng.element(int(i))
+        ch, S, P, root = d["children"], eng.S, d["P"], d["root"]
+        internal = [v for v in d["nodes"] if v >= S]
+        cherry = {v for v in internal if ch[v][0] < S and ch[v][1] < S and v != root and len(internal) > 1}
+        for v in internal:
+            if v in cherry:
+                continue
+            per = word + 4
+            for x in ch[v]:
+                per += 1 if x < S else (2 if x in cherry else word + 4)
+            total += P * per
+        total += P * (word + 8)
+    return total
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,word", [(E.PA_FP32, 16), (E.PA_FP64, 32)])
+def test_loglik_counted_bytes(require_gpu, precision, word):
+    """The likelihood kernel's on-device byte count (the numerator of its
+    roofline figure) equals the count from the elements' trees."""
+    w = synth.make_gt_workload(seed=10, n_elements=12, S=30)
+    eng = E.Engine.from_workload(w, E.default_options(min_len=1))
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    eng.loglik_lengths(ids, w.lengths(), precision)
+    stats = eng.loglik_stats()
+    assert stats["algorithmic_bytes"] == _counted_bytes(eng, ids, word)
+    assert stats["elapsed_ms"] > 0 and stats["n_launches"] == 1
+
+
 @pytest.mark.gpu
 def test_gene_tree_scratch_regrows(require_gpu):
     """One engine: a small call, then more and longer elements (the kept
