@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0, help="config 4: scratch budget")
     ap.add_argument("--lanes", type=int, default=0, help="config 4: launches in flight (0 = library default)")
     ap.add_argument("--no-graph", action="store_true", help="time direct launches instead of the captured graph")
+    ap.add_argument("--table", action="store_true",
+                    help="per-configuration figures (batched FP32/FP64, plain FP64, CPU, agreement) as one JSON line")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the fp64, reference-chain, config 2, 4 and 5 blocks of the default line")
     args = ap.parse_args()
@@ -98,6 +100,8 @@ def main():
     from phyloacc_amd import engine as E
     from phyloacc_amd import synth
 
+    if args.table:
+        return table(args, E, synth)
     if args.config == 4:
         print(json.dumps(serve(args, E, synth, args.elements, args.arrival_rate, args.chain, args.lanes)))
         return
@@ -311,6 +315,120 @@ def config2_block(E, synth, seed):
         out[name] = round(len(ids) * 3 * 199 / (r["elapsed_ms"] / 1e3), 1)
     eng.close()
     return out
+
+
+def table(args, E, synth):
+    """Per-configuration figures: for configs 1, 2 and 3 the batched FP32 and
+    FP64 chain rates over the whole workload, the plain FP64 per-element
+    rate and the CPU oracle's rate (all the process's cores) on
+    length-stratified samples, and FP32-against-FP64 agreement on a sample;
+    for config 5 the gene-tree likelihood rates (batched FP32 and FP64, one
+    launch per element in FP64, the oracle on the CPU) and the errors against
+    the oracle.  Config 1 runs the reference's default chain (BURNIN 200 +
+    MCMC 800); configs 2 and 3 the headline's 200-state chain."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle_py as O
+    host = host_cpu()
+    nthr = args.cpu_threads or host["usable"]
+    gt_w = make_gt(synth, args.seed, 20000)
+    rng = np.random.default_rng(0)
+    out = {"host": host, "cpu_threads": nthr, "configs": {}}
+    for cfg, (burnin, mcmc), cpu_n in ((1, (200, 800), None), (2, (50, 150), 48), (3, (50, 150), 48)):
+        t0 = time.time()
+        w = synth.make_workload(cfg, seed=args.seed)
+        eng = E.Engine.from_workload(w, E.default_options(burnin=burnin, mcmc=mcmc))
+        ids, st = eng.add_elements(w.align, w.starts, w.sizes)
+        keep = ids[st == E.PA_ELEM_OK]
+        T = burnin + mcmc
+        sweeps = len(keep) * 3 * (T - 1)
+        tune = eng.autotune(keep, precision=E.PA_FP32, states=8)
+        row = {"elements": int(len(keep)), "states": T, "tuning": tune}
+        for name, prec in (("batched_fp32", E.PA_FP32), ("batched_fp64", E.PA_FP64)):
+            eng.run_chains(keep[:64], precision=prec)
+            r = eng.run_chains(keep, precision=prec, keep_traces=True)
+            row[name] = round(sweeps / (r["elapsed_ms"] / 1e3), 1)
+        # agreement: the same chains at both precisions on a stratified sample
+        sizes = np.array([eng.element(int(i))["GG"] for i in keep])
+        idx = keep[stratified(sizes, min(256, len(keep)), rng)]
+        r32 = eng.run_chains(idx, precision=E.PA_FP32, traces=True)
+        r64 = eng.run_chains(idx, precision=E.PA_FP64, traces=True)
+        b = burnin
+        def modes(Z):
+            cnt = np.stack([(Z[:, :, b:, :] == z).sum(axis=2) for z in (-1, 0, 1, 2)], axis=-1)
+            return cnt.argmax(-1)
+        row["fp32_vs_fp64"] = {
+            "chains": int(3 * len(idx)),
+            "max_rel_err_max_loglik": float(np.max(np.abs(r32["max_loglik"] - r64["max_loglik"]) / np.abs(r64["max_loglik"]))),
+            "identical_Z_trajectories": round(float(np.all(r32["Z"] == r64["Z"], axis=(2, 3)).mean()), 4),
+            "identical_Z_posterior_modes": round(float(np.all(modes(r32["Z"]) == modes(r64["Z"]), axis=-1).mean()), 4)}
+        eng.close()
+        # plain FP64: one launch per element over uncompressed sites, 24 stratified elements
+        pidx = stratified(w.sizes, min(24, w.C), rng)
+        po = E.default_options(burnin=burnin, mcmc=mcmc, compress_patterns=0)
+        peng = E.Engine.from_workload(w, po)
+        pids, pst = peng.add_elements(w.align, w.starts[pidx], w.sizes[pidx])
+        pids = pids[pst == E.PA_ELEM_OK]
+        peng.run_chains(pids[:2], precision=E.PA_FP64, mode=E.PA_RUN_PER_ELEMENT)
+        rp = peng.run_chains(pids, precision=E.PA_FP64, mode=E.PA_RUN_PER_ELEMENT)
+        scale = float(np.mean(w.sizes[pidx][pst == E.PA_ELEM_OK])) / float(np.mean(w.sizes))
+        row["plain_fp64"] = round(len(pids) * 3 * (T - 1) / (rp["elapsed_ms"] / 1e3) * scale, 1)
+        peng.close()
+        # CPU oracle: config 1 in full, the others on a stratified sample
+        cidx = np.arange(w.C) if cpu_n is None else stratified(w.sizes, cpu_n, rng)
+        m = O.model_from_workload(w)
+        oo = O.default_options(burnin=burnin if cpu_n is None else 50, mcmc=mcmc if cpu_n is None else 550)
+        els = [O.Element(m, oo, w.align, w.starts[i], w.sizes[i]) for i in cidx]
+        els = [e for e in els if not e.filtered]
+        tc = time.time()
+        n_sw, _ = O.run_batch(m, oo, els, nthr)
+        dt = time.time() - tc
+        scale = float(np.mean([e.info()["GG"] for e in els])) / float(np.mean(w.sizes))
+        row["cpu"] = {"value": round(n_sw / dt * scale, 2), "cores": nthr, "elements": len(els),
+                      "states": int(oo.burnin + oo.mcmc), "seconds": round(dt, 1)}
+        row["speedup_batched_fp32_vs_plain"] = round(row["batched_fp32"] / row["plain_fp64"], 1)
+        row["wall_s"] = round(time.time() - t0, 1)
+        out["configs"][str(cfg)] = row
+        log(f"config {cfg}: {row}")
+    # config 5: gene trees
+    w = gt_w
+    eng = E.Engine.from_workload(w, E.default_options())
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    keep = st == E.PA_ELEM_OK
+    ids = ids[keep]
+    L = w.lengths()[keep]
+    row = {"elements": int(len(ids)), "unit": "element log-likelihoods/s"}
+    for name, prec in (("batched_fp32", E.PA_FP32), ("batched_fp64", E.PA_FP64)):
+        eng.loglik_lengths(ids, L, prec)
+        eng.loglik_lengths(ids, L, prec)
+        row[name + "_kernel"] = round(len(ids) / (eng.loglik_stats()["elapsed_ms"] / 1e3), 1)
+    full = np.flatnonzero(keep)
+    pidx = stratified(w.sizes[keep], 24, rng)
+    t1 = time.time()
+    for i in pidx:
+        eng.loglik_lengths(ids[i:i + 1], L[i:i + 1], E.PA_FP64)
+    row["plain_fp64_per_element_calls"] = round(len(pidx) / (time.time() - t1), 1)
+    m = O.parse_mod(w.mod_text)[0]
+    gp = E.default_options().gap_prop
+    smp = stratified(w.sizes[keep], 200, rng)
+
+    def ora(i):
+        c = full[i]
+        g = w.genes[c]
+        return O.gt_loglik(m, w.tree.S, g.children, g.seg_gene, g.seg_species, g.seg_len * w.scale, w.Z[c], w.r1[c],
+                           w.r2[c], w.align, w.starts[c], w.sizes[c], gap_prop=gp)
+    tc = time.time()
+    with ThreadPoolExecutor(nthr) as ex:
+        ref = np.array(list(ex.map(ora, smp)))
+    dt = time.time() - tc
+    row["cpu"] = {"value": round(len(smp) / dt, 1), "cores": nthr, "elements": int(len(smp)), "seconds": round(dt, 2)}
+    l32 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP32)
+    l64 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP64)
+    row["fp64_max_rel_err_vs_oracle"] = float(np.max(np.abs(l64 - ref) / np.abs(ref)))
+    row["fp32_max_rel_err_vs_oracle"] = float(np.max(np.abs(l32 - ref) / np.abs(ref)))
+    eng.close()
+    out["configs"]["5"] = row
+    log(f"config 5: {row}")
+    print(json.dumps(out))
 
 
 def fp32_vs_fp64(E, eng, keep, w, args, rng):
