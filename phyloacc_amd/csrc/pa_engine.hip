@@ -403,7 +403,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
     for (int q = tid; q < c->n_nodes - 1; q += BS) {
         const int s = c->nodes[q];
         if (own) {
-            branch_matrix<R>(M, c, s, A->len_in[(size_t)w * N + s], 1.0);
+            branch_matrix<R>(M, c, s, A->len_in[(size_t)(A->len_row ? A->len_row[w] : w) * N + s], 1.0);
         } else if (gb) {
             const int z = A->Zin[(size_t)w * N + s];
             const int k = (z == 2 ? 2 : (z == 0 ? 0 : 1)) + (A->Bin[(size_t)w * N + s] != 0 ? 3 : 0);

@@ -1139,22 +1139,25 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     if ((st = upload(e)) != PA_OK) return st;
     const int N = e->tm.N;
     std::vector<int32_t> order = schedule(e, ids, n, PA_RUN_BATCHED);
-    std::vector<double> lw((size_t)n * N);
-    for (int i = 0; i < n; i++) memcpy(&lw[(size_t)i * N], lengths + (size_t)order[i] * N, sizeof(double) * N);
+    // the lengths go up in the caller's row order; work position w reads row order[w]
+    const size_t lbytes = (size_t)n * N * 8;
     DevBuf bufs[7];
     HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
-    if (lw.size() * 8 > e->ll_len_bytes) {
+    HIPCHK(e, hipMalloc(&bufs[6].p, (size_t)n * 4));
+    if (lbytes > e->ll_len_bytes) {
         (void)hipFree(e->d_ll_len);
         e->d_ll_len = nullptr;
         e->ll_len_bytes = 0;
-        HIPCHK(e, hipMalloc(&e->d_ll_len, lw.size() * 8));
-        e->ll_len_bytes = lw.size() * 8;
+        HIPCHK(e, hipMalloc(&e->d_ll_len, lbytes));
+        e->ll_len_bytes = lbytes;
     }
-    HIPCHK(e, hipMemcpy(e->d_ll_len, lw.data(), lw.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(e->d_ll_len, lengths, lbytes, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(bufs[6].p, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
     LoglikArgs A;
     memset(&A, 0, sizeof A);
     A.ll_out = (double *)bufs[5].p;
     A.len_in = (const double *)e->d_ll_len;
+    A.len_row = (const int32_t *)bufs[6].p;
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
     st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);

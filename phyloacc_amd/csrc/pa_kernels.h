@@ -21,7 +21,8 @@ namespace pa {
 // Arguments of the log-likelihood launch.  Work position w evaluates element
 // work[w] either at ancestral states Zin[w] and rates rates_in[w] over the
 // element's stored branch lengths, or, when len_in is set, at the per-call
-// lengths len_in[w] with every branch at rate 1 (Zin, rates_in unused).
+// lengths (row len_row[w], or w, of len_in) with every branch at rate 1 (Zin,
+// rates_in unused).
 struct LoglikArgs {
     const ElemDev *elems;
     const int32_t *work;      // element ids in schedule order
@@ -40,7 +41,8 @@ struct LoglikArgs {
     const int64_t *big_off;
     const int32_t *Zin;       // [n_work][N] (when len_in is null)
     const double *rates_in;   // [n_work][2] (r1, r2) (when len_in is null)
-    const double *len_in;     // [n_work][N] per-call branch lengths, or null
+    const double *len_in;     // [rows][N] per-call branch lengths, or null
+    const int32_t *len_row;   // [n_work] row of len_in for each work position, or null (row w)
     double *ll_out;           // [n_work]
     // gBGC substitution model, or null: branch s of work position w takes
     // P(dist) of class k = c(Z) + 3 * (Bin[w][s] != 0), c(0) = 0 (background),
