@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0, help="config 4: scratch budget")
     ap.add_argument("--lanes", type=int, default=0, help="config 4: launches in flight (0 = library default)")
     ap.add_argument("--no-graph", action="store_true", help="time direct launches instead of the captured graph")
+    ap.add_argument("--gt-cache", default=None,
+                    help="config 5: load the gene-tree workload from this file, or simulate and save it there")
     ap.add_argument("--table", action="store_true",
                     help="per-configuration figures (batched FP32/FP64, plain FP64, CPU, agreement) as one JSON line")
     ap.add_argument("--no-extra", action="store_true",
@@ -110,7 +112,7 @@ def main():
     extra = args.config == 3 and not args.no_extra
     # the gene-tree workload is simulated in forked workers, before this
     # process touches the GPU
-    gt_w = make_gt(synth, args.seed, 20000) if extra else None
+    gt_w = make_gt(synth, args.seed, 20000, args.gt_cache) if extra else None
 
     t0 = time.time()
     w = synth.make_workload(args.config, seed=args.seed, n_elements=args.elements)
@@ -191,7 +193,8 @@ def main():
         blocks["reference_chain"] = reference_chain(E, w, keep, eng, rng)
         blocks["config2"] = config2_block(E, synth, args.seed)
         blocks["config4"] = serve(args, E, synth, None, 2000.0, 2000, args.lanes)
-        blocks["config5"] = gene_trees_block(E, gt_w)
+        blocks["config5"] = gene_trees_block(
+            E, gt_w, traffic=measured_traffic(args, None, f"config=5 elements={gt_w.C} seed={args.seed} precision=fp32")[0])
         del gt_w
         for k, v in blocks.items():
             log(f"{k}: {v}")
@@ -264,9 +267,21 @@ def host_cpu():
     return {"usable": usable, "nproc": os.cpu_count(), "model": model, "limit": limit}
 
 
-def make_gt(synth, seed, n):
+def make_gt(synth, seed, n, cache=None):
+    """The gene-tree workload, simulated in forked workers (so before the
+    process touches the GPU), or loaded from `cache`, a file this function
+    wrote for the same seed and size."""
+    import pickle
     t0 = time.time()
-    w = synth.make_gt_workload(seed=seed, n_elements=n, workers=host_cpu()["usable"])
+    if cache and os.path.exists(cache):
+        with open(cache, "rb") as f:
+            w = pickle.load(f)
+        assert w.seed == seed and w.C == n, "gene-tree cache holds another workload"
+    else:
+        w = synth.make_gt_workload(seed=seed, n_elements=n, workers=host_cpu()["usable"])
+        if cache:
+            with open(cache, "wb") as f:
+                pickle.dump(w, f)
     log(f"gene-tree workload: {w.C} elements, S={w.tree.S}, {int(w.sizes.sum())} sites ({time.time() - t0:.1f}s)")
     return w
 
@@ -462,12 +477,12 @@ def fp32_vs_fp64(E, eng, keep, w, args, rng):
             "identical_Z_posterior_modes": round(float(same_mode.mean()), 4)}
 
 
-def measured_traffic(args, n_elements):
+def measured_traffic(args, n_elements, key=None):
     """HBM bytes per launch of this exact workload from a committed counter
     summary under profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the copy
     probe), or None when no summary covers it."""
-    key = (f"config={args.config} elements={n_elements} seed={args.seed} burnin={args.burnin} "
-           f"mcmc={args.mcmc} precision={args.precision} block={args.block}")
+    key = key or (f"config={args.config} elements={n_elements} seed={args.seed} burnin={args.burnin} "
+                  f"mcmc={args.mcmc} precision={args.precision} block={args.block}")
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None, None
@@ -576,14 +591,15 @@ def serve(args, E, synth, n_elements, rate, chain, lanes):
 def gene_trees(args, E, synth):
     """Config 5 alone: the gene-tree block as a JSON line of its own."""
     from phyloacc_amd import synth as _s
-    w = make_gt(_s, args.seed, args.elements or 20000)
-    out = gene_trees_block(E, w, args.steps, args.warmup, args.precision)
+    w = make_gt(_s, args.seed, args.elements or 20000, args.gt_cache)
+    tr, _ = measured_traffic(args, None, f"config=5 elements={w.C} seed={args.seed} precision={args.precision}")
+    out = gene_trees_block(E, w, args.steps, args.warmup, args.precision, tr, oracle=not args.no_cpu)
     out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None})
     print(json.dumps(out))
 
 
-def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32"):
+def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32", traffic=None, oracle=True):
     """Uneven-tree batched pruning: every element carries its own
     multispecies-coalescent gene tree (120 species).  A step evaluates the
     log-likelihood of every element at its gene-branch lengths in one launch.
@@ -611,20 +627,24 @@ def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32"):
             ks = eng.loglik_stats()     # the last timed call's kernel
             ach = ks["algorithmic_bytes"] / (ks["elapsed_ms"] / 1e3) / 1e9
             roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "loglik_kernel",
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": "loglik_kernel",
                     "algorithmic_bytes_per_launch": ks["algorithmic_bytes"],
                     "kernel_ms": round(ks["elapsed_ms"], 3), "grid": ks["grid"], "block_threads": ks["block_threads"],
                     "lds_bytes": ks["lds_bytes"]}
-    rng = np.random.default_rng(0)
-    smp = rng.choice(len(ids), size=min(48, len(ids)), replace=False)
-    m = O.parse_mod(w.mod_text)[0]
-    full = np.flatnonzero(keep)
-    ref = np.array([O.gt_loglik(m, w.tree.S, w.genes[full[i]].children, w.genes[full[i]].seg_gene,
-                                w.genes[full[i]].seg_species, w.genes[full[i]].seg_len * w.scale, w.Z[full[i]],
-                                w.r1[full[i]], w.r2[full[i]], w.align, w.starts[full[i]], w.sizes[full[i]],
-                                gap_prop=opts.gap_prop) for i in smp])
-    l32 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP32)
-    l64 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP64)
+    vs = None
+    if oracle:
+        rng = np.random.default_rng(0)
+        smp = rng.choice(len(ids), size=min(48, len(ids)), replace=False)
+        m = O.parse_mod(w.mod_text)[0]
+        full = np.flatnonzero(keep)
+        ref = np.array([O.gt_loglik(m, w.tree.S, w.genes[full[i]].children, w.genes[full[i]].seg_gene,
+                                    w.genes[full[i]].seg_species, w.genes[full[i]].seg_len * w.scale, w.Z[full[i]],
+                                    w.r1[full[i]], w.r2[full[i]], w.align, w.starts[full[i]], w.sizes[full[i]],
+                                    gap_prop=opts.gap_prop) for i in smp])
+        l32 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP32)
+        l64 = eng.loglik_lengths(ids[smp], L[smp], E.PA_FP64)
+        vs = {"sample": int(len(smp)), "fp64_max_rel_err": float(np.max(np.abs(l64 - ref) / np.abs(ref))),
+              "fp32_max_rel_err": float(np.max(np.abs(l32 - ref) / np.abs(ref)))}
     eng.close()
     main = rates[precision]
     return {
@@ -637,9 +657,7 @@ def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32"):
                    "their own gene trees, lengths LogNormal(ln 300, 0.6)", "elements": int(len(ids)),
                    "sites": int(w.sizes[keep].sum()), "parallelism": "1 GPU",
                    "timing": "wall clock around the call, host-to-device transfer of the lengths included"},
-        "vs_oracle": {"sample": int(len(smp)),
-                      "fp64_max_rel_err": float(np.max(np.abs(l64 - ref) / np.abs(ref))),
-                      "fp32_max_rel_err": float(np.max(np.abs(l32 - ref) / np.abs(ref)))},
+        "vs_oracle": vs,
         "roofline": roof,
     }
 

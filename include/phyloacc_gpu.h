@@ -160,6 +160,30 @@ pa_status pa_loglik(pa_engine *e, const int32_t *ids, int32_t n, const int32_t *
 pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
                             int32_t precision, double *out);
 
+/* Gene-tree log-likelihoods as pa_loglik_lengths, each element with its own
+ * strand-symmetric base frequencies pi = (a, 1/2 - a, 1/2 - a, a), piA[n] = a
+ * in (0, 1/2): the element's Q is BPP_C::getQmat of pi and the model's
+ * exchangeabilities (src/PhyloAcc-GT/bpp_c.cpp:160-185; exchangeabilities as
+ * in src/PhyloAcc-GT/bpp.hpp:346-353), rebuilt and eigendecomposed on the
+ * device, and the root sum uses pi. */
+pa_status pa_loglik_lengths_pi(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                               const double *piA, int32_t precision, double *out);
+
+/* One BPP_C::sample_pi step (src/PhyloAcc-GT/bpp_c.cpp:1508-1597) for n
+ * gene-tree elements at their lengths [n*N], sweep m, over the full element
+ * (no Wang-Landau split).  piA[n] and loglik[n] hold each element's current
+ * a and log-likelihood and are updated in place; accepted[n] is 1 when the
+ * element's step was accepted (an unchanged proposal counts as accepted, as
+ * in the reference).  resample = 0: a uniform step of half-width pi_delta
+ * clamped to [0.05, 0.45] with the reference's boundary terms and a
+ * Metropolis decision under the Beta(prior_a, prior_b) prior on 2a;
+ * resample = 1: a = Beta(prior_a + #A + #T, prior_b + #C + #G) / 2 over the
+ * element's leaf bases, always taken (bpp_c.cpp:1520-1521,1557-1566).  The
+ * proposals' likelihoods are one batched launch. */
+pa_status pa_sample_pi(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths, int32_t m,
+                       double pi_delta, double prior_a, double prior_b, int32_t resample, int32_t precision,
+                       double *piA, double *loglik, int32_t *accepted);
+
 /* Per-element log-likelihood under the gBGC substitution model, on the device.
  * Replaces the pruning of PhyloAcc-ST-GBGC over log_cache_TM_{null,cons,neut}
  * and their _B twins (src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:1563-1621), whose

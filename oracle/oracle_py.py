@@ -133,6 +133,11 @@ def load(path):
         L.or_gt_loglik.argtypes = [P(Model), C.c_int, P(C.c_int), C.c_int, P(C.c_int), P(C.c_int), P(C.c_double),
                                    P(C.c_int), C.c_double, C.c_double, P(C.c_char_p), C.c_long, C.c_int, C.c_int, C.c_double]
         L.or_gt_loglik.restype = C.c_double
+        L.or_gt_loglik_pi.argtypes = [P(Model), C.c_double] + L.or_gt_loglik.argtypes[1:]
+        L.or_gt_loglik_pi.restype = C.c_double
+        L.or_gt_sample_pi.argtypes = [P(Model), C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      P(C.c_double), P(C.c_double)] + L.or_gt_loglik.argtypes[1:]
+        L.or_gt_sample_pi.restype = C.c_int
         L.or_test_eval2.argtypes = [P(Model), P(Options), C.c_void_p, C.c_int, P(C.c_double), P(C.c_double),
                                     P(C.c_double), P(C.c_byte), P(C.c_int)]
         L.or_test_eval2.restype = C.c_double
@@ -285,6 +290,35 @@ def run_batch(model, opts, elements, nthreads=0):
     ml = np.zeros(3 * len(elements), np.float64)
     n = lib().or_run_batch(C.byref(model), C.byref(opts), arr, len(elements), nthreads, _ptr(ml, C.c_double))
     return n, ml.reshape(-1, 3)
+
+
+def _gt_args(S, gchildren, seg_gene, seg_sp, seg_len, Z, r1, r2, align, start, length, gapchar, gap_prop):
+    rows_keep = [np.ascontiguousarray(align[s]).tobytes() for s in range(align.shape[0])]
+    rows = (C.c_char_p * len(rows_keep))(*rows_keep)
+    keep = [rows_keep, np.ascontiguousarray(gchildren, np.int32).reshape(-1), np.ascontiguousarray(seg_gene, np.int32),
+            np.ascontiguousarray(seg_sp, np.int32), np.ascontiguousarray(seg_len, np.float64),
+            np.ascontiguousarray(Z, np.int32)]
+    _, ch, sg, ss, sl, z = keep
+    args = (int(S), _ptr(ch, C.c_int), len(sg), _ptr(sg, C.c_int), _ptr(ss, C.c_int), _ptr(sl, C.c_double),
+            _ptr(z, C.c_int), float(r1), float(r2), rows, int(start), int(length), ord(gapchar), float(gap_prop))
+    return args, keep
+
+
+def gt_loglik_pi(model, a, S, gchildren, seg_gene, seg_sp, seg_len, Z, r1, r2, align, start, length, gapchar="-",
+                 gap_prop=0.8):
+    """or_gt_loglik with the element's base frequencies (a, 1/2 - a, 1/2 - a, a)."""
+    args, keep = _gt_args(S, gchildren, seg_gene, seg_sp, seg_len, Z, r1, r2, align, start, length, gapchar, gap_prop)
+    return lib().or_gt_loglik_pi(C.byref(model), float(a), *args)
+
+
+def gt_sample_pi(model, seed, sweep, pi_delta, prior, resample, a, loglik, S, gchildren, seg_gene, seg_sp, seg_len,
+                 Z, r1, r2, align, start, length, gapchar="-", gap_prop=0.8):
+    """One sample_pi step; returns (a, loglik, accepted)."""
+    args, keep = _gt_args(S, gchildren, seg_gene, seg_sp, seg_len, Z, r1, r2, align, start, length, gapchar, gap_prop)
+    pa, ll = C.c_double(a), C.c_double(loglik)
+    acc = lib().or_gt_sample_pi(C.byref(model), int(seed), int(sweep), float(pi_delta), float(prior[0]),
+                                float(prior[1]), int(bool(resample)), C.byref(pa), C.byref(ll), *args)
+    return pa.value, ll.value, acc
 
 
 def gt_loglik(model, S, gchildren, seg_gene, seg_sp, seg_len, Z, r1, r2, align, start, length, gapchar="-",

@@ -646,6 +646,92 @@ double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, doub
     return res;
 }
 
+/* ------------------------------------------------- GT base frequencies -- */
+
+/* BPP_C::getQmat (src/PhyloAcc-GT/bpp_c.cpp:160-185) at pi = (a, 1/2 - a,
+ * 1/2 - a, a), with BPP's exchangeabilities from the model's RATE_MAT and
+ * BACKGROUND (src/PhyloAcc-GT/bpp.hpp:346-353: inst(3) = inst(0), inst(4) =
+ * Q(1,2) / pi(2), inst(5) = inst(1)). */
+static void gt_model_at(const or_model *m, double a, or_model *out)
+{
+    double inst[6];
+    inst[0] = m->Q[0 * 4 + 1] / m->pi[1];
+    inst[1] = m->Q[0 * 4 + 2] / m->pi[2];
+    inst[2] = m->Q[0 * 4 + 3] / m->pi[3];
+    inst[3] = inst[0];
+    inst[4] = m->Q[1 * 4 + 2] / m->pi[2];
+    inst[5] = inst[1];
+    *out = *m;
+    double *pi = out->pi, *Q = out->Q;
+    pi[0] = pi[3] = a;
+    pi[1] = pi[2] = 0.5 - a;
+    Q[0 * 4 + 0] = -(inst[0] * pi[1] + inst[1] * pi[2] + inst[2] * pi[3]);
+    Q[0 * 4 + 1] = inst[0] * pi[1]; Q[0 * 4 + 2] = inst[1] * pi[2]; Q[0 * 4 + 3] = inst[2] * pi[3];
+    Q[1 * 4 + 0] = inst[0] * pi[0];
+    Q[1 * 4 + 1] = -(inst[0] * pi[0] + inst[3] * pi[2] + inst[4] * pi[3]);
+    Q[1 * 4 + 2] = inst[3] * pi[2]; Q[1 * 4 + 3] = inst[4] * pi[3];
+    Q[2 * 4 + 0] = inst[1] * pi[0]; Q[2 * 4 + 1] = inst[3] * pi[1];
+    Q[2 * 4 + 2] = -(inst[1] * pi[0] + inst[3] * pi[1] + inst[5] * pi[3]);
+    Q[2 * 4 + 3] = inst[5] * pi[3];
+    Q[3 * 4 + 0] = inst[2] * pi[0]; Q[3 * 4 + 1] = inst[4] * pi[1]; Q[3 * 4 + 2] = inst[5] * pi[2];
+    Q[3 * 4 + 3] = -(inst[2] * pi[0] + inst[4] * pi[1] + inst[5] * pi[2]);
+}
+
+double or_gt_loglik_pi(const or_model *m, double a, int S, const int *gchildren, int K, const int *seg_gene,
+                       const int *seg_sp, const double *seg_len, const int *Z, double r1, double r2,
+                       const char *const *rows, long start, int GG, int gapchar, double gap_prop)
+{
+    or_model mm;
+    gt_model_at(m, a, &mm);
+    return or_gt_loglik(&mm, S, gchildren, K, seg_gene, seg_sp, seg_len, Z, r1, r2, rows, start, GG, gapchar, gap_prop);
+}
+
+static double log_beta_pdf(double x, double a, double b)
+{
+    return lgamma(a + b) - lgamma(a) - lgamma(b) + (a - 1) * log(x) + (b - 1) * log1p(-x);
+}
+
+/* BPP_C::sample_pi, src/PhyloAcc-GT/bpp_c.cpp:1508-1597, for one element at
+ * sweep `sweep` over its full length.  *piA and *loglik are the current a and
+ * log-likelihood and are updated; returns 1 when accepted (an unchanged
+ * proposal counts as accepted).  Draws: tag PI; the step's uniform on
+ * sub-stream 0, the decision's on 1, the resample Beta's variates on 2, 3. */
+int or_gt_sample_pi(const or_model *m, uint64_t seed, int sweep, double pi_delta, double prior_a, double prior_b,
+                    int resample, double *piA, double *loglik, int S, const int *gchildren, int K,
+                    const int *seg_gene, const int *seg_sp, const double *seg_len, const int *Z, double r1,
+                    double r2, const char *const *rows, long start, int GG, int gapchar, double gap_prop)
+{
+    uint32_t c3 = tagword(0, OR_TAG_PI), c2 = (uint32_t)sweep;
+    double a = *piA, t, log_p_prop = 0, log_p_cur = 0;
+    if (resample) {
+        /* obsACGT: a, c, g, t of the leaves over the element (Gibbs, bpp_c.cpp:388-394) */
+        double cnt[4] = {0, 0, 0, 0};
+        for (int s = 0; s < S; s++)
+            for (int g = 0; g < GG; g++) {
+                int ch = tolower((unsigned char)rows[s][start + g]);
+                int b = ch == 'a' ? 0 : ch == 'c' ? 1 : ch == 'g' ? 2 : ch == 't' ? 3 : -1;
+                if (b >= 0) cnt[b] += 1;
+            }
+        t = beta_draw_c(seed, 2, 3, c2, c3, prior_a + cnt[0] + cnt[3], prior_b + cnt[1] + cnt[2], NULL) / 2;
+    } else {
+        t = a + (2.0 * slot_u(seed, 0, c2, c3, 0) - 1) * pi_delta;
+        log_p_prop = -log(2.0 * pi_delta);
+        if (t < 0.05) { log_p_prop += log(0.05 - (a - pi_delta)); t = 0.05; }
+        else if (t > 0.45) { log_p_prop += log((a + pi_delta) - 0.45); t = 0.45; }
+        log_p_cur = -log(2.0 * pi_delta);
+        if (a == 0.05 && (t - pi_delta) < 0.05) log_p_cur += log(0.05 - (t - pi_delta));
+        else if (a == 0.45 && (t + pi_delta) > 0.45) log_p_cur += log(t + pi_delta - 0.45);
+    }
+    if (t == a) return 1;
+    double lnew = or_gt_loglik_pi(m, t, S, gchildren, K, seg_gene, seg_sp, seg_len, Z, r1, r2, rows, start, GG,
+                                  gapchar, gap_prop);
+    if (resample) { *piA = t; *loglik = lnew; return 1; }
+    double ratio = lnew - *loglik + log_beta_pdf(2 * t, prior_a, prior_b) - log_beta_pdf(2 * a, prior_a, prior_b)
+                   + log_p_cur - log_p_prop;
+    if (log(slot_u(seed, 1, c2, c3, 0)) < ratio) { *piA = t; *loglik = lnew; return 1; }
+    return 0;
+}
+
 /* --------------------------------------------------------------- gBGC -- */
 
 /* BPP_C::getRate, src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:13-29: B holds +b on

@@ -28,7 +28,7 @@ enum { OR_A = 0, OR_C = 1, OR_G = 2, OR_T = 3, OR_GAP = 4, OR_OTHER = 5,
 enum { OR_FLAG_NAN = 1, OR_FLAG_POS_INF = 2, OR_FLAG_NEG_INF = 4, OR_FLAG_SAMPLER = 8, OR_FLAG_PROPOSAL_RANGE = 16 };
 
 /* RNG stream tags (bits 16..23 of counter word 3) */
-enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5 };
+enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5, OR_TAG_PI = 6 };
 
 typedef struct {
     int burnin, mcmc, adapt_freq;          /* BURNIN, MCMC, ADAPT_FREQ */
@@ -107,6 +107,16 @@ void or_elem_arrays(const or_element *e, int *children2 /*[N*2]*/, int *parent2,
 
 /* log-likelihood at fixed Z, rates (bpp_c2.cpp:392-425 form) */
 double or_loglik(const or_model *m, or_element *e, const int *Z, double r1, double r2);
+/* PhyloAcc-GT base frequencies: the gene-tree log-likelihood at pi = (a,
+ * 1/2 - a, 1/2 - a, a) and one sample_pi step (src/PhyloAcc-GT/bpp_c.cpp:
+ * 1508-1597); arguments after the first ones as or_gt_loglik. */
+double or_gt_loglik_pi(const or_model *m, double a, int S, const int *gchildren, int K, const int *seg_gene,
+                       const int *seg_sp, const double *seg_len, const int *Z, double r1, double r2,
+                       const char *const *rows, long start, int GG, int gapchar, double gap_prop);
+int or_gt_sample_pi(const or_model *m, uint64_t seed, int sweep, double pi_delta, double prior_a, double prior_b,
+                    int resample, double *piA, double *loglik, int S, const int *gchildren, int K,
+                    const int *seg_gene, const int *seg_sp, const double *seg_len, const int *Z, double r1,
+                    double r2, const char *const *rows, long start, int GG, int gapchar, double gap_prop);
 /* gBGC model (PhyloAcc-ST-GBGC): Q(s, b) from Q0, and the log-likelihood with
  * per-branch classes (Z, B) and sel = {conserved s, accelerated s, bias}. */
 void or_gbgc_rate(const double *Q0, double s, double b, double *Q);

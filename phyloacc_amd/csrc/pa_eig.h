@@ -1,6 +1,8 @@
-// pa_eig.h -- eigensystem of a time-reversible 4x4 rate matrix and the
-// gBGC rate matrix, compiled for the host (the model's Q at engine creation)
-// and for the device (per-element gBGC classes), so both use one solver.
+// pa_eig.h -- eigensystem of a time-reversible 4x4 rate matrix, the gBGC
+// rate matrix and PhyloAcc-GT's strand-symmetric rate matrix, compiled for
+// the host (the model's Q at engine creation) and for the device
+// (per-element gBGC classes and per-element base frequencies), so both use
+// one solver.
 //
 // Q = V diag(lam) Vi is found through the symmetric form
 // D^1/2 Q D^-1/2 (D = diag of Q's stationary distribution) and cyclic
@@ -134,6 +136,27 @@ PA_HD inline void gbgc_rate(const double *Q0, double s, double b, double *Q)
             row += Q[i * 4 + j];
         }
         Q[i * 4 + i] -= row;
+    }
+}
+
+// PhyloAcc-GT's strand-symmetric stationary distribution and rate matrix
+// (BPP_C::getQmat, src/PhyloAcc-GT/bpp_c.cpp:160-185, with pi set as in
+// BPP_C::sample_pi, bpp_c.cpp:1545-1546): pi = (a, 1/2 - a, 1/2 - a, a) and
+// Q[i][j] = inst[k(i, j)] * pi[j] off the diagonal, k over the pairs AC, AG,
+// AT, CG, CT, GT; rows sum to 0.
+PA_HD inline void gt_strand_q(const double *inst, double a, double *Q, double *pi)
+{
+    pi[0] = pi[3] = a;
+    pi[1] = pi[2] = 0.5 - a;
+    const int pair[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+    for (int i = 0; i < 4; i++) {
+        double row = 0;
+        for (int j = 0; j < 4; j++) {
+            if (i == j) continue;
+            Q[i * 4 + j] = inst[pair[i][j]] * pi[j];
+            row += Q[i * 4 + j];
+        }
+        Q[i * 4 + i] = -row;
     }
 }
 

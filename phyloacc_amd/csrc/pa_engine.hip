@@ -70,6 +70,7 @@ struct Ctx {
     LDS uint8_t *flags;
     LDS R *PT;             // [N][16], PT[i*4+j] = P_{j->i}
     double red[16];
+    double pi[4];          // base frequencies of the root sum (the model's, or the element's own)
     int work;
 };
 
@@ -195,6 +196,44 @@ __global__ void gbgc_eig_kernel(GbgcEigArgs a)
         out[4 + i] = V[i];
         out[20 + i] = Vi[i];
     }
+}
+
+// Per-element strand-symmetric base frequencies (BPP_C::sample_pi and
+// getQmat, src/PhyloAcc-GT/bpp_c.cpp:1545-1548,160-185): one thread per work
+// position.
+struct GtEigArgs {
+    double inst[6];
+    const double *piA;
+    int n;
+    double *eig, *pi;
+};
+
+__global__ void gt_eig_kernel(GtEigArgs a)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.n) return;
+    double Q[16], pi[4], lam[4], V[16], Vi[16];
+    gt_strand_q(a.inst, a.piA[w], Q, pi);
+    const int st = reversible_eig4(Q, lam, V, Vi);
+    double *out = a.eig + (size_t)w * GB_EIG_WORDS;
+    for (int i = 0; i < 4; i++) out[i] = st == EIG_OK ? lam[i] : NAN;
+    for (int i = 0; i < 16; i++) {
+        out[4 + i] = V[i];
+        out[20 + i] = Vi[i];
+    }
+    for (int i = 0; i < 4; i++) a.pi[(size_t)w * 4 + i] = pi[i];
+}
+
+hipError_t launch_gt_eig(const double *inst, const double *piA, int n, double *eig, double *pi, hipStream_t st)
+{
+    GtEigArgs a;
+    for (int i = 0; i < 6; i++) a.inst[i] = inst[i];
+    a.piA = piA;
+    a.n = n;
+    a.eig = eig;
+    a.pi = pi;
+    hipLaunchKernelGGL(gt_eig_kernel, dim3((n + 127) / 128), dim3(128), 0, st, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_gbgc_eig(const double *Q0, const double *sel, int n, double *eig, hipStream_t st)
@@ -337,7 +376,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
     if (r >= S) {
         const GBL V4<R> *rp = parts + (size_t)(r - S) * Ppad;
         const GBL int32_t *re = exps + (size_t)(r - S) * Ppad;
-        const double p0 = M->pi[0], p1 = M->pi[1], p2 = M->pi[2], p3 = M->pi[3];
+        const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
         for (int pb = tid; pb < P; pb += U * BS) {
             int ex[U], wt[U];
             V4<R> av[U];
@@ -376,6 +415,8 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
     if (tid == 0) {
         c->S = M->S; c->N = N;
         c->P = E.P; c->Ppad = E.Ppad; c->root = E.root;
+        const double *pw = A->el_pi ? A->el_pi + (size_t)w * 4 : nullptr;
+        for (int j = 0; j < 4; j++) c->pi[j] = pw ? pw[j] : M->pi[j];
         c->n_nodes = E.n_nodes; c->n_up = E.n_up;
         c->codes = (const GBL uint8_t *)(A->codes + E.off_codes);
         c->wts = (const GBL int32_t *)(A->wts + E.off_w);
@@ -403,7 +444,13 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
     for (int q = tid; q < c->n_nodes - 1; q += BS) {
         const int s = c->nodes[q];
         if (own) {
-            branch_matrix<R>(M, c, s, A->len_in[(size_t)(A->len_row ? A->len_row[w] : w) * N + s], 1.0);
+            const double len = A->len_in[(size_t)(A->len_row ? A->len_row[w] : w) * N + s];
+            if (A->el_eig) {
+                const double *E = A->el_eig + (size_t)w * GB_EIG_WORDS;
+                branch_matrix_from<R>(E, E + 4, E + 20, c, s, len, 1.0);
+            } else {
+                branch_matrix<R>(M, c, s, len, 1.0);
+            }
         } else if (gb) {
             const int z = A->Zin[(size_t)w * N + s];
             const int k = (z == 2 ? 2 : (z == 0 ? 0 : 1)) + (A->Bin[(size_t)w * N + s] != 0 ? 3 : 0);

@@ -1129,8 +1129,10 @@ extern "C" pa_status pa_loglik_last_stats(pa_engine *e, pa_run_stats *out)
 // Log-likelihood with per-call branch lengths (every branch at rate 1):
 // the pruning of an element's own tree, e.g. a gene tree whose lengths come
 // from pa_gene_lengths.
-extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
-                                       int32_t precision, double *out)
+// Own-tree log-likelihoods at per-call lengths, with the model's base
+// frequencies or, when piA is given, each element's strand-symmetric ones.
+static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                                     const double *piA, int32_t precision, double *out)
 {
     if (!e || !ids || !lengths || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
     if (n == 0) return PA_OK;
@@ -1141,7 +1143,7 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     std::vector<int32_t> order = schedule(e, ids, n, PA_RUN_BATCHED);
     // the lengths go up in the caller's row order; work position w reads row order[w]
     const size_t lbytes = (size_t)n * N * 8;
-    DevBuf bufs[7];
+    DevBuf bufs[10];
     HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
     HIPCHK(e, hipMalloc(&bufs[6].p, (size_t)n * 4));
     if (lbytes > e->ll_len_bytes) {
@@ -1158,6 +1160,28 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     A.ll_out = (double *)bufs[5].p;
     A.len_in = (const double *)e->d_ll_len;
     A.len_row = (const int32_t *)bufs[6].p;
+    if (piA) {
+        // BPP's exchangeabilities from RATE_MAT and BACKGROUND, with the
+        // reference's assignment (src/PhyloAcc-GT/bpp.hpp:346-353: CG takes
+        // AC's value, CT takes C->G / pi_G, GT takes AG's value)
+        const double *Q = e->Q0, *p0 = e->M.pi;
+        double inst[6];
+        inst[0] = Q[0 * 4 + 1] / p0[1];
+        inst[1] = Q[0 * 4 + 2] / p0[2];
+        inst[2] = Q[0 * 4 + 3] / p0[3];
+        inst[3] = inst[0];
+        inst[4] = Q[1 * 4 + 2] / p0[2];
+        inst[5] = inst[1];
+        std::vector<double> pw(n);
+        for (int i = 0; i < n; i++) pw[i] = piA[order[i]];
+        HIPCHK(e, hipMalloc(&bufs[7].p, (size_t)n * 8));
+        HIPCHK(e, hipMalloc(&bufs[8].p, (size_t)n * GB_EIG_WORDS * 8));
+        HIPCHK(e, hipMalloc(&bufs[9].p, (size_t)n * 4 * 8));
+        HIPCHK(e, hipMemcpy(bufs[7].p, pw.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        HIPCHK(e, launch_gt_eig(inst, (const double *)bufs[7].p, n, (double *)bufs[8].p, (double *)bufs[9].p, e->stream));
+        A.el_eig = (const double *)bufs[8].p;
+        A.el_pi = (const double *)bufs[9].p;
+    }
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
     st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);
@@ -1166,6 +1190,110 @@ extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) out[order[i]] = ll[i];
     if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
+    return PA_OK;
+}
+
+extern "C" pa_status pa_loglik_lengths(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                                       int32_t precision, double *out)
+{
+    return loglik_lengths_impl(e, ids, n, lengths, nullptr, precision, out);
+}
+
+// PhyloAcc-GT's model with each element's own strand-symmetric base
+// frequencies pi = (a, 1/2 - a, 1/2 - a, a) (BPP_C::sample_pi and getQmat,
+// src/PhyloAcc-GT/bpp_c.cpp:1545-1548,160-185): the element's Q is rebuilt from
+// the exchangeabilities on the device, then pruned as pa_loglik_lengths.
+extern "C" pa_status pa_loglik_lengths_pi(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                                          const double *piA, int32_t precision, double *out)
+{
+    if (!piA && n > 0) return fail(e, PA_E_INPUT, "null argument");
+    for (int i = 0; i < n; i++)
+        if (!(piA[i] > 0 && piA[i] < 0.5)) return fail(e, PA_E_INPUT, "piA must lie in (0, 1/2)");
+    return loglik_lengths_impl(e, ids, n, lengths, piA, precision, out);
+}
+
+// One BPP_C::sample_pi step (src/PhyloAcc-GT/bpp_c.cpp:1508-1597) for a batch
+// of gene-tree elements, at the full element length (no Wang-Landau split).
+// Per element, on the host: the proposal -- a uniform step of half-width
+// pi_delta clamped to [0.05, 0.45] with the reference's boundary terms, or,
+// with resample set, a draw from Beta(prior_a + #A + #T, prior_b + #C + #G) / 2
+// over the element's leaf bases; then the batched likelihood of every
+// proposal in one launch; then the Metropolis decision with the Beta prior on
+// 2a.  An unchanged proposal counts as accepted, as in the reference.
+// Draws: stream tag PI, sweep m; the step's uniform on sub-stream 0, the
+// decision's on 1, the Beta's gamma variates on 2 and 3.
+extern "C" pa_status pa_sample_pi(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths, int32_t m,
+                                  double pi_delta, double prior_a, double prior_b, int32_t resample, int32_t precision,
+                                  double *piA, double *loglik, int32_t *accepted)
+{
+    if (!e || !ids || !lengths || !piA || !loglik || !accepted || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    if (!(pi_delta > 0) || !(prior_a > 0) || !(prior_b > 0)) return fail(e, PA_E_INPUT, "pi_delta and the prior must be positive");
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    const Key key{e->M.key0, e->M.key1};
+    const uint32_t c3 = tagword(0, TAG_PI), c2 = (uint32_t)m;
+    std::vector<double> prop(n), lq(n, 0.0);     // lq: log q(cur | prop) - log q(prop | cur)
+    std::vector<int32_t> run;
+    for (int i = 0; i < n; i++) {
+        const double a = piA[i];
+        double t;
+        if (resample) {
+            const HostElem &E = e->elems[ids[i]];
+            double cnt[4] = {0, 0, 0, 0};
+            for (int s = 0; s < e->tm.S; s++)
+                for (int p = 0; p < E.P; p++) {
+                    const int code = E.codes[(size_t)s * E.Ppad + p];
+                    if (code < 4) cnt[code] += E.w[p];
+                }
+            t = beta_draw(key, 2, 3, c2, c3, prior_a + cnt[0] + cnt[3], prior_b + cnt[1] + cnt[2]) / 2;
+        } else {
+            t = a + (2.0 * slot_uniform(key, 0, c2, c3, 0) - 1) * pi_delta;
+            double lp_prop = -log(2.0 * pi_delta);
+            if (t < 0.05) {
+                lp_prop += log(0.05 - (a - pi_delta));
+                t = 0.05;
+            } else if (t > 0.45) {
+                lp_prop += log((a + pi_delta) - 0.45);
+                t = 0.45;
+            }
+            double lp_cur = -log(2.0 * pi_delta);
+            if (a == 0.05 && (t - pi_delta) < 0.05) lp_cur += log(0.05 - (t - pi_delta));
+            else if (a == 0.45 && (t + pi_delta) > 0.45) lp_cur += log(t + pi_delta - 0.45);
+            lq[i] = lp_cur - lp_prop;
+        }
+        prop[i] = t;
+        accepted[i] = 1;
+        if (t != a) run.push_back(i);
+    }
+    if (run.empty()) return PA_OK;
+    const int N = e->tm.N, k = (int)run.size();
+    std::vector<int32_t> rid(k);
+    std::vector<double> rl((size_t)k * N), rp(k), lnew(k);
+    for (int j = 0; j < k; j++) {
+        rid[j] = ids[run[j]];
+        rp[j] = prop[run[j]];
+        memcpy(&rl[(size_t)j * N], lengths + (size_t)run[j] * N, sizeof(double) * N);
+    }
+    if ((st = loglik_lengths_impl(e, rid.data(), k, rl.data(), rp.data(), precision, lnew.data())) != PA_OK) return st;
+    auto log_beta_pdf = [](double x, double a, double b) {
+        return lgamma(a + b) - lgamma(a) - lgamma(b) + (a - 1) * log(x) + (b - 1) * log1p(-x);
+    };
+    for (int j = 0; j < k; j++) {
+        const int i = run[j];
+        if (resample) {
+            piA[i] = prop[i];
+            loglik[i] = lnew[j];
+            continue;
+        }
+        const double ratio = lnew[j] - loglik[i] + log_beta_pdf(2 * prop[i], prior_a, prior_b) -
+                             log_beta_pdf(2 * piA[i], prior_a, prior_b) + lq[i];
+        if (log(slot_uniform(key, 1, c2, c3, 0)) < ratio) {
+            piA[i] = prop[i];
+            loglik[i] = lnew[j];
+        } else {
+            accepted[i] = 0;
+        }
+    }
     return PA_OK;
 }
 

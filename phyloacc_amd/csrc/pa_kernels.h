@@ -51,6 +51,12 @@ struct LoglikArgs {
     const double *gb_eig;     // [n_work][6][GB_EIG_WORDS]
     const int32_t *Bin;       // [n_work][N]
     unsigned long long *counters;   // [0] += algorithmic HBM bytes of the launch, or null
+    // Per-element substitution model for the own-tree path, or null: work
+    // position w prunes with the eigensystem el_eig[w] = {lam[4], V[16],
+    // Vi[16]} and sums its root with the base frequencies el_pi[w][4]
+    // (gt_eig_kernel).
+    const double *el_eig;     // [n_work][GB_EIG_WORDS]
+    const double *el_pi;      // [n_work][4]
 };
 
 constexpr int GB_CLASSES = 6, GB_EIG_WORDS = 36;
@@ -61,6 +67,13 @@ constexpr int GB_CLASSES = 6, GB_EIG_WORDS = 36;
 // A class whose Q is not time-reversible gets NaN eigenvalues.
 hipError_t launch_gbgc_eig(const double *Q0 /*[16], host*/, const double *sel /*[n][3], device*/, int n,
                            double *eig /*[n][6][36], device*/, hipStream_t st);
+
+// Eigensystem and base frequencies of PhyloAcc-GT's strand-symmetric model
+// for every work position: pi = (a, 1/2 - a, 1/2 - a, a) with a = piA[w],
+// Q from the six exchangeabilities inst (gt_strand_q).  A non-reversible
+// result gets NaN eigenvalues.
+hipError_t launch_gt_eig(const double *inst /*[6], host*/, const double *piA /*[n], device*/, int n,
+                         double *eig /*[n][36], device*/, double *pi /*[n][4], device*/, hipStream_t st);
 
 // LDS bytes of one workgroup (dynamic arrays + context + argument blocks).
 size_t loglik_smem_bytes(int N, int fp64);

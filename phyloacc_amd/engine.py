@@ -32,7 +32,8 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_serve_create", "pa_serve_submit", "pa_serve_step", "pa_serve_poll", "pa_serve_get_stats",
            "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
            "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc",
-           "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats"]
+           "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats",
+           "pa_loglik_lengths_pi", "pa_sample_pi"]
 
 
 class PaError(RuntimeError):
@@ -170,6 +171,13 @@ def lib():
         L.pa_loglik_gbgc.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double),
                                      C.c_int32, P(C.c_double)]
         L.pa_loglik_gbgc.restype = C.c_int
+        L.pa_loglik_lengths_pi.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), P(C.c_double),
+                                           C.c_int32, P(C.c_double)]
+        L.pa_loglik_lengths_pi.restype = C.c_int
+        L.pa_sample_pi.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), C.c_int32, C.c_double,
+                                   C.c_double, C.c_double, C.c_int32, C.c_int32, P(C.c_double), P(C.c_double),
+                                   P(C.c_int32)]
+        L.pa_sample_pi.restype = C.c_int
         L.pa_loglik_last_stats.argtypes = [C.c_void_p, P(RunStats)]
         L.pa_loglik_last_stats.restype = C.c_int
         L.pa_batch_create.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(C.c_void_p)]
@@ -340,6 +348,30 @@ class Engine:
         self._check(lib().pa_loglik_gbgc(self._h, _p(ids, C.c_int32), len(ids), _p(Z, C.c_int32), _p(B, C.c_int32),
                                          _p(sel, C.c_double), int(precision), _p(out, C.c_double)))
         return out
+
+    def loglik_lengths_pi(self, ids, lengths, piA, precision=PA_FP64):
+        """Gene-tree log-likelihoods with per-element strand-symmetric base
+        frequencies (a, 1/2 - a, 1/2 - a, a), piA[n] = a."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        L = np.ascontiguousarray(lengths, np.float64).reshape(len(ids), self.N)
+        a = np.ascontiguousarray(piA, np.float64)
+        out = np.zeros(len(ids), np.float64)
+        self._check(lib().pa_loglik_lengths_pi(self._h, _p(ids, C.c_int32), len(ids), _p(L, C.c_double),
+                                               _p(a, C.c_double), int(precision), _p(out, C.c_double)))
+        return out
+
+    def sample_pi(self, ids, lengths, m, piA, loglik, pi_delta=0.1, prior=(1.0, 1.0), resample=False,
+                  precision=PA_FP64):
+        """One batched sample_pi step; returns (piA, loglik, accepted) after it."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        L = np.ascontiguousarray(lengths, np.float64).reshape(len(ids), self.N)
+        a = np.array(piA, np.float64)
+        ll = np.array(loglik, np.float64)
+        acc = np.zeros(len(ids), np.int32)
+        self._check(lib().pa_sample_pi(self._h, _p(ids, C.c_int32), len(ids), _p(L, C.c_double), int(m),
+                                       float(pi_delta), float(prior[0]), float(prior[1]), int(bool(resample)),
+                                       int(precision), _p(a, C.c_double), _p(ll, C.c_double), _p(acc, C.c_int32)))
+        return a, ll, acc
 
     def loglik_stats(self):
         """Kernel time and counted bytes of the last likelihood call."""
