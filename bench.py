@@ -529,7 +529,8 @@ def serve(args, E, synth, n_elements, rate, chain, lanes):
     rng = np.random.default_rng(args.seed + 1)
     order = rng.permutation(len(ids))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(ids)))
-    srv = E.Server(eng, precision=prec, block_threads=args.block or 128, lanes=lanes,
+    tuning = None if args.block or args.no_tune else eng.autotune(ids, precision=prec, states=8)
+    srv = E.Server(eng, precision=prec, block_threads=args.block, lanes=lanes,
                    hbm_budget_bytes=args.hbm_budget_gib * 2 ** 30)
     # warm the kernel and the library once, outside the timed stream
     srv.submit(ids[order[:4]])
@@ -583,6 +584,7 @@ def serve(args, E, synth, n_elements, rate, chain, lanes):
                       "max": round(float(lat.max()), 4), "mean": round(float(lat.mean()), 4)},
         "serving": {"arrival_rate_per_s": rate, "offered_s": round(float(arrivals[-1]), 3),
                     "launches": int(s.launches - base.launches), "peak_lanes_busy": int(peak_lanes),
+                    "tuning": tuning,
                     "device_busy_frac": round((s.device_ms - base.device_ms) / 1e3 / wall, 4),
                     "numeric_failures": bad},
     }

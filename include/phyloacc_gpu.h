@@ -347,7 +347,7 @@ typedef struct pa_server pa_server;
 
 typedef struct {
     int32_t precision;          /* PA_FP64 / PA_FP32 */
-    int32_t block_threads;      /* 64, 128 or 256; 0 = 128 */
+    int32_t block_threads;      /* 64, 128 or 256; 0 = the engine's tuned block size, else one chosen from the mean pattern count */
     int32_t lanes;              /* launches in flight at once, 1..PA_SERVE_MAX_LANES; 0 = PA_SERVE_DEFAULT_LANES */
     int32_t max_chains;         /* chains in flight at most; 0 = 4096 */
     double hbm_budget_bytes;    /* scratch pool shared by the lanes' launches; 0 = 4 GiB */

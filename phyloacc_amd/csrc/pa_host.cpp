@@ -433,7 +433,6 @@ struct pa_engine {
     std::string err;
     std::vector<HostElem> elems;
     bool dev_dirty = true;
-    uint64_t arena_epoch = 0;       // counts uploads: prepared batches hold pointers into one generation
     std::vector<struct pa_batch *> batches;   // prepared batches, freed with the engine
     hipEvent_t ll_t0 = nullptr, ll_t1 = nullptr;   // around the last log-likelihood launch
     pa_run_stats ll_stats{};
@@ -446,7 +445,7 @@ struct pa_engine {
     int n_cu = 0;
     size_t lds_per_cu = 0;
     pa_tuning tune = {0, 0, 0, 0, 0.0};     // batched chain launches: block size (0: by mean pattern count), shape
-    uint64_t arena_gen = 0;                  // bumped whenever the device arenas move
+    uint64_t arena_gen = 0;                  // bumped whenever the device arenas move (prepared batches check it)
     // log-likelihood calls: scratch slots and per-call branch lengths, kept
     // between calls and grown when a call needs more (freed with the engine)
     void *d_ll_scratch = nullptr, *d_ll_len = nullptr;
@@ -767,7 +766,6 @@ static pa_status upload(pa_engine *e)
         if (!E.filtered) { e->max_ppad = std::max(e->max_ppad, E.Ppad); e->mean_p += E.P; kept++; }
     if (kept) e->mean_p /= (double)kept;
     e->dev_dirty = false;
-    e->arena_epoch++;
     e->arena_gen++;
     return PA_OK;
 }
@@ -782,12 +780,19 @@ struct DevBuf {
 }  // namespace
 
 // Block size of a batched launch from the elements' mean pattern count.
+// Default block size (no tuning): by the mean pattern count (measured on the
+// 43-species workloads: 128 threads at a mean of 136 patterns, 256 at 299),
+// and 256 when the batch has fewer elements than four per CU, where more
+// threads per element is all the device can add.
+static int block_for_mean(double meanP) { return meanP <= 96 ? 64 : (meanP <= 200 ? 128 : 256); }
+
 static int pick_block(pa_engine *e, const int32_t *work, int32_t n)
 {
     double meanP = 0;
     for (int i = 0; i < n; i++) meanP += e->elems[work[i]].P;
     meanP /= std::max(1, n);
-    return (meanP <= 96) ? 64 : (meanP <= 320 ? 128 : 256);
+    if (e->n_cu > 0 && n < 4 * e->n_cu) return 256;
+    return block_for_mean(meanP);
 }
 
 // Workgroups resident on one CU at once, from the runtime's occupancy query
@@ -1486,7 +1491,7 @@ extern "C" pa_status pa_batch_create(pa_engine *e, const int32_t *ids, int32_t n
     b->e = e;
     b->n = n;
     b->fp64 = cfg->precision == PA_FP64;
-    b->epoch = e->arena_epoch;
+    b->epoch = e->arena_gen;
     b->order = schedule(e, ids, n, PA_RUN_BATCHED);
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[b->order[i]];
@@ -1557,7 +1562,7 @@ extern "C" pa_status pa_batch_run(pa_batch *b, pa_chain_result *results, pa_run_
 {
     if (!b) return PA_E_INPUT;
     pa_engine *e = b->e;
-    if (e->dev_dirty || e->arena_epoch != b->epoch)
+    if (e->dev_dirty || e->arena_gen != b->epoch)
         return fail(e, PA_E_STATE, "elements were added after the batch was prepared; prepare it again");
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipEventRecord(b->t0, e->stream));
@@ -2046,7 +2051,9 @@ extern "C" pa_status pa_serve_create(pa_engine *e, const pa_serve_config *cfg, p
     std::unique_ptr<pa_server> s(new pa_server());
     s->e = e;
     s->fp64 = cfg->precision == PA_FP64;
-    s->BS = cfg->block_threads ? cfg->block_threads : 128;
+    if ((st = upload(e)) != PA_OK) return st;
+    s->BS = cfg->block_threads ? cfg->block_threads
+                               : (e->tune.block_threads ? e->tune.block_threads : block_for_mean(e->mean_p));
     s->shape = e->tune.shape;
     if (s->BS != 64 && s->BS != 128 && s->BS != 256) return fail(e, PA_E_INPUT, "block_threads must be 64, 128 or 256");
     const int N = e->tm.N, S = e->tm.S;
