@@ -184,6 +184,25 @@ pa_status pa_sample_pi(pa_engine *e, const int32_t *ids, int32_t n, const double
                        double pi_delta, double prior_a, double prior_b, int32_t resample, int32_t precision,
                        double *piA, double *loglik, int32_t *accepted);
 
+/* Gene-tree log-likelihoods over two prefixes of each element's sites: out0
+ * sums its first lens0[i] sites, out1 its first lens1[i] (0 <= lens1 <= lens0
+ * <= the element's sites), the two log_lik entries of the Wang-Landau blocks
+ * of PhyloAcc-GT's Gibbs (src/PhyloAcc-GT/bpp_c.cpp:359-380,436-443); one
+ * launch prunes each element once.  piA: NULL for the model's base
+ * frequencies, or per element as in pa_loglik_lengths_pi. */
+pa_status pa_loglik_lengths_split(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                                  const double *piA, const int32_t *lens0, const int32_t *lens1,
+                                  int32_t precision, double *out0, double *out1);
+
+/* One Wang-Landau indicator step (src/PhyloAcc-GT/bpp_c.cpp:399-440) for n
+ * elements at sweep m, on the host.  loglik, logpz: [n*2] (the two prefixes'
+ * log-likelihoods and missing-pattern terms); state: [n*7] = two log
+ * normalising constants, two momenta, the learning rate, two visit counts,
+ * set from loglik + logpz at m = 0 and updated in place; indicator: [n], the
+ * distribution (0 or 1) the next sweep targets. */
+pa_status pa_wl_step(pa_engine *e, int32_t n, int32_t m, const double *loglik, const double *logpz,
+                     double *state, int32_t *indicator);
+
 /* Per-element log-likelihood under the gBGC substitution model, on the device.
  * Replaces the pruning of PhyloAcc-ST-GBGC over log_cache_TM_{null,cons,neut}
  * and their _B twins (src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:1563-1621), whose

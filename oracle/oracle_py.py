@@ -138,6 +138,8 @@ def load(path):
         L.or_gt_sample_pi.argtypes = [P(Model), C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                       P(C.c_double), P(C.c_double)] + L.or_gt_loglik.argtypes[1:]
         L.or_gt_sample_pi.restype = C.c_int
+        L.or_wl_step.argtypes = [C.c_uint64, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.or_wl_step.restype = C.c_int
         L.or_test_eval2.argtypes = [P(Model), P(Options), C.c_void_p, C.c_int, P(C.c_double), P(C.c_double),
                                     P(C.c_double), P(C.c_byte), P(C.c_int)]
         L.or_test_eval2.restype = C.c_double
@@ -319,6 +321,14 @@ def gt_sample_pi(model, seed, sweep, pi_delta, prior, resample, a, loglik, S, gc
     acc = lib().or_gt_sample_pi(C.byref(model), int(seed), int(sweep), float(pi_delta), float(prior[0]),
                                 float(prior[1]), int(bool(resample)), C.byref(pa), C.byref(ll), *args)
     return pa.value, ll.value, acc
+
+
+def wl_step(seed, m, loglik, logpz, state):
+    """One Wang-Landau indicator step; state [7] updated in place."""
+    ll = np.ascontiguousarray(loglik, np.float64)
+    lz = np.ascontiguousarray(logpz, np.float64)
+    assert state.dtype == np.float64 and state.flags.c_contiguous
+    return lib().or_wl_step(int(seed), int(m), _ptr(ll, C.c_double), _ptr(lz, C.c_double), _ptr(state, C.c_double))
 
 
 def gt_loglik(model, S, gchildren, seg_gene, seg_sp, seg_len, Z, r1, r2, align, start, length, gapchar="-",

@@ -732,6 +732,37 @@ int or_gt_sample_pi(const or_model *m, uint64_t seed, int sweep, double pi_delta
     return 0;
 }
 
+/* The Wang-Landau indicator step of PhyloAcc-GT's Gibbs (src/PhyloAcc-GT/
+ * bpp_c.cpp:399-440) for one element at sweep m.  st[7]: log normalising
+ * constants (2), momenta (2), learning rate, visit counts (2); set at m = 0
+ * as bpp_c.cpp:423-427 does.  Returns the indicator (tag WL, sub-stream 0). */
+int or_wl_step(uint64_t seed, int m, const double *loglik, const double *logpz, double *st)
+{
+    if (m == 0) {
+        st[0] = loglik[0] + logpz[0]; st[1] = loglik[1] + logpz[1];
+        st[2] = st[3] = 0; st[4] = 1; st[5] = st[6] = 0;
+    }
+    double y[2] = {loglik[0] + logpz[0] - st[0], loglik[1] + logpz[1] - st[1]}, prob[2] = {0, 0};
+    double x = log_exp_sum(y, 2);                 /* BPP::log_sample_norm, GT bpp.hpp:497-503 */
+    if (x != INFINITY) { prob[0] = exp(y[0] - x); prob[1] = exp(y[1] - x); }
+    int ind = categorical(prob, 2, slot_u(seed, 0, (uint32_t)m, tagword(0, OR_TAG_WL), 0));
+    const double beta = 0.9;
+    st[2] = beta * st[2] - st[4] * prob[0];
+    st[3] = beta * st[3] - st[4] * prob[1];
+    st[5 + ind] += 1;
+    st[0] -= st[2];
+    st[1] -= st[3];
+    double z = log_exp_sum(st, 2);
+    st[0] -= z;
+    st[1] -= z;
+    double rr = st[5] / st[6];
+    if (rr > 2.0 / 3.0 && rr < 3.0 / 2.0) {
+        st[4] = 1 / (1 + 1 / st[4]);
+        st[5] = st[6] = 0;
+    }
+    return ind;
+}
+
 /* --------------------------------------------------------------- gBGC -- */
 
 /* BPP_C::getRate, src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:13-29: B holds +b on

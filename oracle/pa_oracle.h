@@ -28,7 +28,7 @@ enum { OR_A = 0, OR_C = 1, OR_G = 2, OR_T = 3, OR_GAP = 4, OR_OTHER = 5,
 enum { OR_FLAG_NAN = 1, OR_FLAG_POS_INF = 2, OR_FLAG_NEG_INF = 4, OR_FLAG_SAMPLER = 8, OR_FLAG_PROPOSAL_RANGE = 16 };
 
 /* RNG stream tags (bits 16..23 of counter word 3) */
-enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5, OR_TAG_PI = 6 };
+enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5, OR_TAG_PI = 6, OR_TAG_WL = 7 };
 
 typedef struct {
     int burnin, mcmc, adapt_freq;          /* BURNIN, MCMC, ADAPT_FREQ */
@@ -117,6 +117,8 @@ int or_gt_sample_pi(const or_model *m, uint64_t seed, int sweep, double pi_delta
                     int resample, double *piA, double *loglik, int S, const int *gchildren, int K,
                     const int *seg_gene, const int *seg_sp, const double *seg_len, const int *Z, double r1,
                     double r2, const char *const *rows, long start, int GG, int gapchar, double gap_prop);
+/* One Wang-Landau indicator step of PhyloAcc-GT's Gibbs; st[7] as pa_wl_step. */
+int or_wl_step(uint64_t seed, int m, const double *loglik, const double *logpz, double *st);
 /* gBGC model (PhyloAcc-ST-GBGC): Q(s, b) from Q0, and the log-likelihood with
  * per-branch classes (Z, B) and sel = {conserved s, accelerated s, bias}. */
 void or_gbgc_rate(const double *Q0, double s, double b, double *Q);

@@ -62,6 +62,8 @@ struct Ctx {
     int S, N, P, Ppad, root, n_nodes, n_up;
     const GBL uint8_t *codes;
     const GBL int32_t *wts;
+    const GBL int32_t *wts1;   // weights of a second root sum (a second site prefix), or null
+    double sum1;               // that sum
     // workgroup slot in HBM
     GBL V4<R> *parts;      // [I][Ppad]
     GBL int32_t *exps;     // [I][Ppad]
@@ -372,13 +374,14 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
     }
     // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
     const int r = c->root;
-    double acc = 0;
+    const GBL int32_t *wts1 = uni_ptr(c->wts1);
+    double acc = 0, acc1 = 0;
     if (r >= S) {
         const GBL V4<R> *rp = parts + (size_t)(r - S) * Ppad;
         const GBL int32_t *re = exps + (size_t)(r - S) * Ppad;
         const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
         for (int pb = tid; pb < P; pb += U * BS) {
-            int ex[U], wt[U];
+            int ex[U], wt[U], wt1[U];
             V4<R> av[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -387,6 +390,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
                 if (p < P) {
                     ex[u] = re[p];
                     wt[u] = c->wts[p];
+                    wt1[u] = wts1 ? wts1[p] : 0;
                     av[u] = ld4<R>(rp + p);
                 }
             }
@@ -394,9 +398,15 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
             for (int u = 0; u < U; u++) {
                 if (ex[u] == GAPT) continue;
                 const double L = p0 * (double)av[u].x + p1 * (double)av[u].y + p2 * (double)av[u].z + p3 * (double)av[u].w;
-                acc += (double)wt[u] * (log(L) + (double)ex[u] * 0.69314718055994530942);
+                const double t = log(L) + (double)ex[u] * 0.69314718055994530942;
+                acc += (double)wt[u] * t;
+                acc1 += (double)wt1[u] * t;
             }
         }
+    }
+    if (wts1) {
+        const double s1 = block_sum(acc1, c->red);
+        if (threadIdx.x == 0) c->sum1 = s1;
     }
     return block_sum(acc, c->red);
 }
@@ -419,7 +429,13 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
         for (int j = 0; j < 4; j++) c->pi[j] = pw ? pw[j] : M->pi[j];
         c->n_nodes = E.n_nodes; c->n_up = E.n_up;
         c->codes = (const GBL uint8_t *)(A->codes + E.off_codes);
-        c->wts = (const GBL int32_t *)(A->wts + E.off_w);
+        if (A->wpre) {
+            c->wts = (const GBL int32_t *)(A->wpre + (size_t)w * 2 * A->wpre_stride);
+            c->wts1 = c->wts + A->wpre_stride;
+        } else {
+            c->wts = (const GBL int32_t *)(A->wts + E.off_w);
+            c->wts1 = nullptr;
+        }
         const int64_t bo = A->big_off ? A->big_off[w] : -1;
         char *base = bo >= 0 ? A->big + bo : A->scratch + (int64_t)blockIdx.x * A->slot_bytes;
         c->parts = (GBL V4<R> *)base;
@@ -513,6 +529,7 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
         const double l = prune<R, BS>(M, c, A->counters && threadIdx.x == 0 ? &bytes : nullptr);
         if (threadIdx.x == 0) {
             A->ll_out[w] = l;
+            if (A->wpre) A->ll_out2[w] = c->sum1;
             // root partial + exponent + pattern weight
             if (A->counters) atomicAdd(A->counters, bytes + (unsigned long long)c->P * (4 * sizeof(R) + 8));
         }

@@ -1137,7 +1137,9 @@ extern "C" pa_status pa_loglik_last_stats(pa_engine *e, pa_run_stats *out)
 // Own-tree log-likelihoods at per-call lengths, with the model's base
 // frequencies or, when piA is given, each element's strand-symmetric ones.
 static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
-                                     const double *piA, int32_t precision, double *out)
+                                     const double *piA, int32_t precision, double *out,
+                                     const int32_t *lens0 = nullptr, const int32_t *lens1 = nullptr,
+                                     double *out1 = nullptr)
 {
     if (!e || !ids || !lengths || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
     if (n == 0) return PA_OK;
@@ -1189,12 +1191,111 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
     }
     std::vector<int32_t> sorted(n);
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
+    DevBuf pre[2];
+    if (lens0) {
+        // per work position, the pattern weights of its first lens0 and lens1 sites
+        int stride = 1;
+        for (int i = 0; i < n; i++) stride = std::max(stride, e->elems[sorted[i]].Ppad);
+        std::vector<int32_t> wp((size_t)n * 2 * stride, 0);
+        for (int i = 0; i < n; i++) {
+            const HostElem &E = e->elems[sorted[i]];
+            const int L0 = lens0[order[i]], L1 = lens1[order[i]];
+            int32_t *w0 = &wp[(size_t)i * 2 * stride], *w1 = w0 + stride;
+            for (int g = 0; g < L0; g++) {
+                w0[E.site[g]]++;
+                if (g < L1) w1[E.site[g]]++;
+            }
+        }
+        HIPCHK(e, hipMalloc(&pre[0].p, wp.size() * 4));
+        HIPCHK(e, hipMalloc(&pre[1].p, (size_t)n * 8));
+        HIPCHK(e, hipMemcpy(pre[0].p, wp.data(), wp.size() * 4, hipMemcpyHostToDevice));
+        A.wpre = (const int32_t *)pre[0].p;
+        A.wpre_stride = stride;
+        A.ll_out2 = (double *)pre[1].p;
+    }
     st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);
     if (st != PA_OK) return st;
     std::vector<double> ll(n);
     HIPCHK(e, hipMemcpy(ll.data(), bufs[5].p, (size_t)n * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) out[order[i]] = ll[i];
+    if (lens0) {
+        HIPCHK(e, hipMemcpy(ll.data(), pre[1].p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) out1[order[i]] = ll[i];
+    }
     if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
+    return PA_OK;
+}
+
+// The gene-tree log-likelihood over two prefixes of each element's sites,
+// the first lens0[i] and the first lens1[i] (Wang-Landau blocks of
+// PhyloAcc-GT's Gibbs, src/PhyloAcc-GT/bpp_c.cpp:359-380,436-443: log_lik[0]
+// sums sites g < lens[0], log_lik[1] sites g < lens[1]), one launch: the
+// patterns are pruned once and summed with two weight vectors.  piA may be
+// null (the model's base frequencies) or per element as in
+// pa_loglik_lengths_pi.
+extern "C" pa_status pa_loglik_lengths_split(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
+                                             const double *piA, const int32_t *lens0, const int32_t *lens1,
+                                             int32_t precision, double *out0, double *out1)
+{
+    if (!e || !ids || !lengths || !lens0 || !lens1 || !out0 || !out1 || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    for (int i = 0; i < n; i++) {
+        const int GG = e->elems[ids[i]].GG;
+        if (lens1[i] < 0 || lens1[i] > lens0[i] || lens0[i] > GG)
+            return fail(e, PA_E_INPUT, "prefix lengths must satisfy 0 <= lens1 <= lens0 <= the element's sites");
+        if (piA && !(piA[i] > 0 && piA[i] < 0.5)) return fail(e, PA_E_INPUT, "piA must lie in (0, 1/2)");
+    }
+    return loglik_lengths_impl(e, ids, n, lengths, piA, precision, out0, lens0, lens1, out1);
+}
+
+// One Wang-Landau indicator step of PhyloAcc-GT's Gibbs (src/PhyloAcc-GT/
+// bpp_c.cpp:399-440) for n elements at sweep m.  state[i*7 ..]: the two log
+// normalising constants, the two momenta, the learning rate and the two
+// visit counts; at m = 0 the constants start at loglik + logpz, the momenta
+// at 0, the rate at 1 and the counts at 0 (bpp_c.cpp:423-427).  The
+// indicator is drawn from normalise(exp(loglik + logpz - constants)) (one
+// uniform, tag WL, sweep m), then momentum = 0.9 momentum - rate * prob,
+// constants -= momentum and renormalised, and the rate shrinks to
+// 1 / (1 + 1 / rate) (counts reset) once the visit ratio is within
+// (2/3, 3/2).
+extern "C" pa_status pa_wl_step(pa_engine *e, int32_t n, int32_t m, const double *loglik, const double *logpz,
+                                double *state, int32_t *indicator)
+{
+    if (!e || n < 0 || (n > 0 && (!loglik || !logpz || !state || !indicator))) return fail(e, PA_E_INPUT, "null argument");
+    const Key key{e->M.key0, e->M.key1};
+    const double u = slot_uniform(key, 0, (uint32_t)m, tagword(0, TAG_WL), 0);
+    for (int i = 0; i < n; i++) {
+        double *s = state + (size_t)i * 7;
+        const double *ll = loglik + 2 * i, *lz = logpz + 2 * i;
+        if (m == 0) {
+            s[0] = ll[0] + lz[0]; s[1] = ll[1] + lz[1];
+            s[2] = s[3] = 0.0;
+            s[4] = 1.0;
+            s[5] = s[6] = 0.0;
+        }
+        double y[2] = {ll[0] + lz[0] - s[0], ll[1] + lz[1] - s[1]};
+        const double mx = std::max(y[0], y[1]);
+        const double x = mx + log(exp(y[0] - mx) + exp(y[1] - mx));
+        double prob[2] = {0.0, 0.0};
+        if (x != INFINITY) { prob[0] = exp(y[0] - x); prob[1] = exp(y[1] - x); }
+        const int ind = categorical<2>(prob, u);
+        indicator[i] = ind;
+        s[2] = 0.9 * s[2] - s[4] * prob[0];
+        s[3] = 0.9 * s[3] - s[4] * prob[1];
+        s[5 + ind] += 1;
+        s[0] -= s[2];
+        s[1] -= s[3];
+        const double mz = std::max(s[0], s[1]);
+        const double z = mz + log(exp(s[0] - mz) + exp(s[1] - mz));
+        s[0] -= z;
+        s[1] -= z;
+        const double rr = s[5] / s[6];
+        if (rr > 2.0 / 3.0 && rr < 3.0 / 2.0) {
+            s[4] = 1 / (1 + 1 / s[4]);
+            s[5] = s[6] = 0.0;
+        }
+    }
     return PA_OK;
 }
 

@@ -57,6 +57,12 @@ struct LoglikArgs {
     // (gt_eig_kernel).
     const double *el_eig;     // [n_work][GB_EIG_WORDS]
     const double *el_pi;      // [n_work][4]
+    // Two site prefixes per work position, or null: the root sums use the
+    // pattern weights wpre[w][0][.] (into ll_out) and wpre[w][1][.] (into
+    // ll_out2) instead of the element's weights, each row wpre_stride long.
+    const int32_t *wpre;      // [n_work][2][wpre_stride]
+    int wpre_stride;
+    double *ll_out2;          // [n_work]
 };
 
 constexpr int GB_CLASSES = 6, GB_EIG_WORDS = 36;

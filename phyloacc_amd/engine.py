@@ -33,7 +33,7 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
            "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc",
            "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats",
-           "pa_loglik_lengths_pi", "pa_sample_pi"]
+           "pa_loglik_lengths_pi", "pa_sample_pi", "pa_loglik_lengths_split", "pa_wl_step"]
 
 
 class PaError(RuntimeError):
@@ -178,6 +178,12 @@ def lib():
                                    C.c_double, C.c_double, C.c_int32, C.c_int32, P(C.c_double), P(C.c_double),
                                    P(C.c_int32)]
         L.pa_sample_pi.restype = C.c_int
+        L.pa_loglik_lengths_split.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(C.c_double), P(C.c_double),
+                                              P(C.c_int32), P(C.c_int32), C.c_int32, P(C.c_double), P(C.c_double)]
+        L.pa_loglik_lengths_split.restype = C.c_int
+        L.pa_wl_step.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
+                                 P(C.c_int32)]
+        L.pa_wl_step.restype = C.c_int
         L.pa_loglik_last_stats.argtypes = [C.c_void_p, P(RunStats)]
         L.pa_loglik_last_stats.restype = C.c_int
         L.pa_batch_create.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(C.c_void_p)]
@@ -372,6 +378,33 @@ class Engine:
                                        float(pi_delta), float(prior[0]), float(prior[1]), int(bool(resample)),
                                        int(precision), _p(a, C.c_double), _p(ll, C.c_double), _p(acc, C.c_int32)))
         return a, ll, acc
+
+    def loglik_lengths_split(self, ids, lengths, lens0, lens1, piA=None, precision=PA_FP64):
+        """Gene-tree log-likelihoods over each element's first lens0 and first
+        lens1 sites; returns (out0, out1)."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        L = np.ascontiguousarray(lengths, np.float64).reshape(len(ids), self.N)
+        l0 = np.ascontiguousarray(lens0, np.int32)
+        l1 = np.ascontiguousarray(lens1, np.int32)
+        a = None if piA is None else np.ascontiguousarray(piA, np.float64)
+        o0 = np.zeros(len(ids), np.float64)
+        o1 = np.zeros(len(ids), np.float64)
+        self._check(lib().pa_loglik_lengths_split(self._h, _p(ids, C.c_int32), len(ids), _p(L, C.c_double),
+                                                  None if a is None else _p(a, C.c_double), _p(l0, C.c_int32),
+                                                  _p(l1, C.c_int32), int(precision), _p(o0, C.c_double),
+                                                  _p(o1, C.c_double)))
+        return o0, o1
+
+    def wl_step(self, m, loglik, logpz, state):
+        """One Wang-Landau indicator step; loglik, logpz [n, 2], state [n, 7]
+        (updated in place).  Returns the indicators."""
+        ll = np.ascontiguousarray(loglik, np.float64).reshape(-1, 2)
+        lz = np.ascontiguousarray(logpz, np.float64).reshape(-1, 2)
+        assert state.dtype == np.float64 and state.flags.c_contiguous and state.shape == (len(ll), 7)
+        ind = np.zeros(len(ll), np.int32)
+        self._check(lib().pa_wl_step(self._h, len(ll), int(m), _p(ll, C.c_double), _p(lz, C.c_double),
+                                     _p(state, C.c_double), _p(ind, C.c_int32)))
+        return ind
 
     def loglik_stats(self):
         """Kernel time and counted bytes of the last likelihood call."""
