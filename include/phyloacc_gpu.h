@@ -203,6 +203,76 @@ pa_status pa_loglik_lengths_split(pa_engine *e, const int32_t *ids, int32_t n, c
 pa_status pa_wl_step(pa_engine *e, int32_t n, int32_t m, const double *loglik, const double *logpz,
                      double *state, int32_t *indicator);
 
+/* ---- PhyloAcc-GT gene trees inside the species tree ----
+ * The species tree of PhyloAcc-GT (BPP::InitPhyloTree, src/PhyloAcc-GT/
+ * bpp.cpp:40-84, with the population sizes of src/PhyloAcc-GT/main.cpp:
+ * 379-413): parent [N] (N at the root), node heights [N] (only differences
+ * matter) and theta [N] (internal species; the coalescence rate of k lineages
+ * in species s is k(k-1)/theta_s).  A gene tree (GTree, src/PhyloAcc-GT/
+ * genetree.hpp:16-127) is passed as children [N*2] (-1 at leaves; leaf i is
+ * species i), parent [N] (N at the gene root) and height [N].  Whatever GTree
+ * caches per species (parent_gene2, temp_coal) follows from these: a gene
+ * node lies in the lowest species on the climb from its first leaf whose
+ * parent species is not lower than the node (GTree::findSp, genetree.cpp:
+ * 2521-2534), and a gene lineage crosses every species from its node's up to
+ * its parent's. */
+typedef struct {
+    int32_t S, N;
+    const int32_t *parent;
+    const double *height;
+    const double *theta;
+} pa_species_tree;
+
+/* GTree::priorTree (src/PhyloAcc-GT/genetree.cpp:560-584): the multispecies-
+ * coalescent log density of the gene tree's coalescence times.  Host only. */
+pa_status pa_gt_prior(const pa_species_tree *sp, const int32_t *children, const int32_t *parent,
+                      const double *height, double *out);
+
+/* GTree::get_logpZ1 (src/PhyloAcc-GT/genetree.cpp:1156-1190): sum of
+ * log P(missing | Z) over the leaves and over the gene nodes born in internal
+ * species, P(missing | Z = 1) = cons_to_mis, P(missing | Z != 1) =
+ * ncons_to_mis; a gene node is missing when both children are.  leaf_missing
+ * [S], Z [N].  Host only. */
+pa_status pa_gt_logpz(const pa_species_tree *sp, const int32_t *children, const int32_t *parent,
+                      const double *height, const int32_t *leaf_missing, const int32_t *Z, double cons_to_mis,
+                      double ncons_to_mis, double *out);
+
+/* Substitution length of every gene branch (lengths [N], 0 at the root): its
+ * pieces in the species it crosses at each species' rate (Z 2: r2
+ * accelerated, Z 1: r1 conserved, else 1); pieces shorter than 1e-8 add
+ * nothing.  The one-matrix-per-piece composition of GTree::Update_Tg
+ * (src/PhyloAcc-GT/genetree.cpp:1330-1426) is P of the summed length.  These
+ * are the lengths pa_loglik_lengths takes.  Host only. */
+pa_status pa_gt_lengths(const pa_species_tree *sp, const int32_t *children, const int32_t *parent,
+                        const double *height, const int32_t *Z, double r1, double r2, double *lengths);
+
+/* One GTree::Sample_BranchLen move (src/PhyloAcc-GT/genetree.cpp:1658-1965)
+ * for each of n gene-tree elements, the proposals' likelihoods in one launch.
+ * Element i moves gene node gnode[i] of its gene tree (gchildren [n*N*2],
+ * gparent [n*N], gheight [n*N]; the tree must be the element's stored tree,
+ * pa_elements_add_trees): the height is drawn uniformly in the window
+ * h +- delta * theta / 2 of the node's species, clipped below at the bottom
+ * of the lowest species both children's lineages cross or at the taller
+ * child, above at the parent gene node when that is born in the same
+ * species, else at the top of the species (the gene root: 10 theta_root / 2,
+ * or h + 5 theta_root / 2 when higher); then Metropolis-Hastings on the
+ * prefix log-likelihood of indicator[i] (sites g < lens[2i + indicator[i]];
+ * loglik[2i + 1] sums g < lens[2i + 1] as well), the missing-gene term when
+ * logpz[2i + indicator[i]] != 0, the coalescent prior and the clipped window's
+ * proposal ratio.  Z [n*N] species classes, rates [n*2] (r1 conserved, r2
+ * accelerated), piA [n] or NULL as pa_loglik_lengths_pi.  On acceptance
+ * gheight, loglik [n*2], logpz [n*2] (both entries, or entry 0 while
+ * logpz[2i + 1] == 0) and prior [n] take the proposal's values and
+ * accepted[i] = 1.  An empty window draws nothing and is not accepted.
+ * Draws: tag GT_BRANCH, sub-stream slot, sweep m; the height's uniform
+ * (floored at 1e-5) at 0, the decision's at 1. */
+pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *sp, const int32_t *ids, int32_t n,
+                           const int32_t *gchildren, const int32_t *gparent, double *gheight, const int32_t *Z,
+                           const double *rates, const double *piA, const int32_t *gnode, const int32_t *indicator,
+                           const int32_t *lens, double delta, double cons_to_mis, double ncons_to_mis, int32_t m,
+                           int32_t slot, int32_t precision, double *loglik, double *logpz, double *prior,
+                           int32_t *accepted);
+
 /* Per-element log-likelihood under the gBGC substitution model, on the device.
  * Replaces the pruning of PhyloAcc-ST-GBGC over log_cache_TM_{null,cons,neut}
  * and their _B twins (src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:1563-1621), whose

@@ -86,6 +86,101 @@ def gbgc_rate(Q0, s, b):
     return Q.reshape(4, 4)
 
 
+class SpTree(C.Structure):
+    _fields_ = [("S", C.c_int), ("N", C.c_int), ("parent", C.POINTER(C.c_int)),
+                ("height", C.POINTER(C.c_double)), ("theta", C.POINTER(C.c_double))]
+
+
+def sptree(S, parent, height, theta):
+    """An or_sptree over numpy copies that live as long as the struct."""
+    keep = [np.ascontiguousarray(parent, np.int32), np.ascontiguousarray(height, np.float64),
+            np.ascontiguousarray(theta, np.float64)]
+    t = SpTree(int(S), len(keep[0]), _ptr(keep[0], C.c_int), _ptr(keep[1], C.c_double), _ptr(keep[2], C.c_double))
+    t._keep = keep
+    return t
+
+
+class GeneTree:
+    """A gene tree as topology + heights (GTree): children [N, 2] (-1 at
+    leaves), parent [N] (N at the root), heights [N], missing [N]."""
+
+    def __init__(self, children, parent, heights, missing=None):
+        self.ch = np.ascontiguousarray(np.asarray(children, np.int32).reshape(-1))
+        self.par = np.ascontiguousarray(parent, np.int32).copy()
+        self.h = np.ascontiguousarray(heights, np.float64).copy()
+        N = len(self.par)
+        self.miss = np.zeros(N, np.int32) if missing is None else np.ascontiguousarray(missing, np.int32).copy()
+
+    @property
+    def root(self):
+        return int(np.flatnonzero(self.par == len(self.par))[0])
+
+    def args(self):
+        return _ptr(self.ch, C.c_int), _ptr(self.par, C.c_int), _ptr(self.h, C.c_double)
+
+
+def gt_species(sp, gt, v):
+    return lib().or_gt_species(C.byref(sp), _ptr(gt.ch, C.c_int), _ptr(gt.h, C.c_double), int(v))
+
+
+def gt_prior(sp, gt):
+    return lib().or_gt_prior(C.byref(sp), *gt.args(), gt.root)
+
+
+def gt_logpz(sp, gt, Z, c2m, n2m):
+    z = np.ascontiguousarray(Z, np.int32)
+    return lib().or_gt_logpz(C.byref(sp), _ptr(gt.ch, C.c_int), _ptr(gt.h, C.c_double), _ptr(gt.miss, C.c_int),
+                             _ptr(z, C.c_int), float(c2m), float(n2m))
+
+
+def gt_pieces(sp, gt):
+    N = len(gt.par)
+    cap = N * (N + 1)
+    g = np.zeros(cap, np.int32)
+    s = np.zeros(cap, np.int32)
+    ln = np.zeros(cap)
+    K = lib().or_gt_pieces(C.byref(sp), *gt.args(), gt.root, _ptr(g, C.c_int), _ptr(s, C.c_int), _ptr(ln, C.c_double))
+    return g[:K], s[:K], ln[:K]
+
+
+def _rows(align):
+    keep = [np.ascontiguousarray(align[s]).tobytes() for s in range(align.shape[0])]
+    return (C.c_char_p * len(keep))(*keep), keep
+
+
+def gt_site_logliks(model, sp, gt, Z, r1, r2, align, start, length, piA=0.0, gapchar="-", gap_prop=0.8):
+    """Per-site log-likelihoods of the gene tree at its heights (pieces at the
+    species branches' rates), over sites [start, start + length)."""
+    g, s, ln = gt_pieces(sp, gt)
+    rows, keep = _rows(align)
+    z = np.ascontiguousarray(Z, np.int32)
+    out = np.zeros(int(length))
+    m = model
+    if piA > 0:
+        raise ValueError("per-element pi: use gt_branch_len")
+    lib().or_gt_site_logliks(C.byref(m), int(sp.S), _ptr(gt.ch, C.c_int), gt.root, len(g), _ptr(g, C.c_int),
+                             _ptr(s, C.c_int), _ptr(ln, C.c_double), _ptr(z, C.c_int), float(r1), float(r2), rows,
+                             int(start), int(length), ord(gapchar), float(gap_prop), _ptr(out, C.c_double))
+    return out
+
+
+def gt_branch_len(model, sp, gt, Z, c_rate, n_rate, c2m, n2m, gnode, delta, indicator, lens, loglik, logpz, prior,
+                  seed, m, slot, align, start, length, piA=0.0, gapchar="-", gap_prop=0.8):
+    """One Sample_BranchLen move; gt.h, loglik [2], logpz [2] are updated in
+    place.  Returns (code, prior): code 1 accepted, 0 not, -1 error."""
+    rows, keep = _rows(align)
+    z = np.ascontiguousarray(Z, np.int32)
+    ln = np.ascontiguousarray(lens, np.int32)
+    assert loglik.dtype == np.float64 and logpz.dtype == np.float64
+    pr = C.c_double(prior)
+    code = lib().or_gt_branch_len(C.byref(model), float(piA), C.byref(sp), *gt.args(), _ptr(gt.miss, C.c_int),
+                                  gt.root, _ptr(z, C.c_int), float(c_rate), float(n_rate), float(c2m), float(n2m),
+                                  int(gnode), float(delta), int(indicator), _ptr(ln, C.c_int),
+                                  _ptr(loglik, C.c_double), _ptr(logpz, C.c_double), C.byref(pr), int(seed), int(m),
+                                  int(slot), rows, int(start), int(length), ord(gapchar), float(gap_prop))
+    return code, pr.value
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -147,6 +242,20 @@ def load(path):
                                           C.c_int, C.c_int, C.c_int, P(C.c_int)]
         L.or_run_batch.argtypes = [P(Model), P(Options), P(C.c_void_p), C.c_int, C.c_int, P(C.c_double)]
         L.or_run_batch.restype = C.c_long
+        PI, PD = P(C.c_int), P(C.c_double)
+        L.or_gt_species.argtypes = [P(SpTree), PI, PD, C.c_int]
+        L.or_gt_prior.argtypes = [P(SpTree), PI, PI, PD, C.c_int]
+        L.or_gt_prior.restype = C.c_double
+        L.or_gt_logpz.argtypes = [P(SpTree), PI, PD, PI, PI, C.c_double, C.c_double]
+        L.or_gt_logpz.restype = C.c_double
+        L.or_gt_pieces.argtypes = [P(SpTree), PI, PI, PD, C.c_int, PI, PI, PD]
+        L.or_gt_site_logliks.argtypes = [P(Model), C.c_int, PI, C.c_int, C.c_int, PI, PI, PD, PI, C.c_double,
+                                         C.c_double, P(C.c_char_p), C.c_long, C.c_int, C.c_int, C.c_double, PD]
+        L.or_gt_site_logliks.restype = None
+        L.or_gt_branch_len.argtypes = [P(Model), C.c_double, P(SpTree), PI, PI, PD, PI, C.c_int, PI, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, PI, PD, PD,
+                                       PD, C.c_uint64, C.c_int, C.c_int, P(C.c_char_p), C.c_long, C.c_int, C.c_int,
+                                       C.c_double]
     return L
 
 

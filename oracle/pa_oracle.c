@@ -1535,3 +1535,270 @@ double or_gt_loglik(const or_model *m, int S, const int *gchildren, int K, const
     free(L); free(lam); free(gap); free(miss);
     return total;
 }
+
+/* ------------------------------------------- GT gene tree in the species tree -- */
+
+/* Per-site gene-tree log-likelihoods (the summands of or_gt_loglik) for a
+ * gene tree in any numbering: gchildren [N*2] (-1 at leaves), root groot.
+ * Missing genes and gaps as or_gt_loglik, over the element's GG sites. */
+void or_gt_site_logliks(const or_model *m, int S, const int *gchildren, int groot, int K, const int *seg_gene,
+                        const int *seg_sp, const double *seg_len, const int *Z, double r1, double r2,
+                        const char *const *rows, long start, int GG, int gapchar, double gap_prop, double *site)
+{
+    int N = 2 * S - 1;
+    double logpi[4];
+    for (int j = 0; j < 4; j++) logpi[j] = log(m->pi[j]);
+    double (*L)[16] = (double (*)[16])malloc(sizeof(double) * 16 * (K > 0 ? K : 1));
+    for (int k = 0; k < K; k++) {
+        int z = Z[seg_sp[k]];
+        double rate = z == 2 ? r2 : (z == 1 ? r1 : 1.0);
+        if (seg_len[k] < 1e-8) log_identity(L[k]);
+        else or_log_trans(m, seg_len[k] * rate, L[k]);
+    }
+    /* post-order of the gene tree from its root */
+    int order[OR_MAXN], no = 0, stk[OR_MAXN], vis[OR_MAXN], ns = 0;
+    stk[ns] = groot; vis[ns++] = 0;
+    while (ns > 0) {
+        int v = stk[ns - 1];
+        if (v >= S && vis[ns - 1] == 0) {
+            vis[ns - 1] = 1;
+            stk[ns] = gchildren[2 * v + 1]; vis[ns++] = 0;
+            stk[ns] = gchildren[2 * v]; vis[ns++] = 0;
+        } else {
+            order[no++] = v;
+            ns--;
+        }
+    }
+    double (*lam)[4] = (double (*)[4])malloc(sizeof(double) * 4 * N);
+    int *gap = (int *)malloc(sizeof(int) * N), *miss = (int *)calloc(N, sizeof(int));
+    for (int s = 0; s < S; s++) {
+        int ng = 0;
+        for (int g = 0; g < GG; g++) ng += tolower((unsigned char)rows[s][start + g]) == gapchar;
+        miss[s] = ng > gap_prop * GG;
+    }
+    for (int q = 0; q < no; q++) {
+        int s = order[q];
+        if (s >= S) miss[s] = miss[gchildren[2 * s]] && miss[gchildren[2 * s + 1]];
+    }
+    for (int g = 0; g < GG; g++) {
+        for (int s = 0; s < S; s++) {
+            int Tg; unsigned char code;
+            encode_char(tolower((unsigned char)rows[s][start + g]), gapchar, lam[s], &Tg, &code);
+            gap[s] = Tg >= 4;
+        }
+        for (int q = 0; q < no; q++) {
+            int s = order[q];
+            if (s < S) continue;
+            for (int j = 0; j < 4; j++) lam[s][j] = 0;
+            int a = gchildren[2 * s], b = gchildren[2 * s + 1];
+            gap[s] = gap[a] && gap[b];
+            for (int c = 0; c < 2; c++) {
+                int x = c ? b : a;
+                if (gap[x] || miss[x]) continue;
+                double msg[4], tmp[4];
+                memcpy(msg, lam[x], sizeof msg);
+                for (int k = 0; k < K; k++) {
+                    if (seg_gene[k] != x) continue;
+                    log_multi(L[k], msg, tmp);
+                    memcpy(msg, tmp, sizeof msg);
+                }
+                for (int j = 0; j < 4; j++) lam[s][j] += msg[j];
+            }
+        }
+        double y[4];
+        for (int j = 0; j < 4; j++) y[j] = lam[groot][j] + logpi[j];
+        site[g] = log_exp_sum(y, 4);
+    }
+    free(L); free(lam); free(gap); free(miss);
+}
+
+/* GTree::findSp (src/PhyloAcc-GT/genetree.cpp:2521-2534): from gene node v
+ * down its first children to a leaf, whose species is the leaf id; then up
+ * the species tree while the parent species is lower than v. */
+int or_gt_species(const or_sptree *sp, const int *gch, const double *gh, int v)
+{
+    int s = v;
+    while (gch[2 * s] != -1) s = gch[2 * s];
+    int p = sp->parent[s];
+    while (p < sp->N && sp->height[p] < gh[v]) { s = p; p = sp->parent[s]; }
+    return s;
+}
+
+/* Is gene lineage v (from v up to its parent gene node, or up through the
+ * root species for the gene root) in species branch s?  The lineage runs
+ * through the species from findSp(v) up to findSp(parent), both included
+ * (GTree::parent_gene2[s] holds exactly these lineages as keys). */
+static int gt_present(const or_sptree *sp, const int *gch, const int *gpar, const double *gh, int v, int s)
+{
+    int a = or_gt_species(sp, gch, gh, v);
+    int top = gpar[v] < sp->N ? or_gt_species(sp, gch, gh, gpar[v]) : sp->N - 1;
+    for (;;) {
+        if (a == s) return 1;
+        if (a == top || a >= sp->N - 1) return 0;
+        a = sp->parent[a];
+    }
+}
+
+/* GTree::priorTree (genetree.cpp:560-584): for every internal species s, nn
+ * lineages entering from below coalesce at the gene nodes born in s in
+ * height order, each at rate nn(nn-1)/theta_s with density 2/theta_s; the
+ * lineages left at the top survive to the parent species. */
+double or_gt_prior(const or_sptree *sp, const int *gch, const int *gpar, const double *gh, int groot)
+{
+    int S = sp->S, N = sp->N;
+    double logP = 0;
+    for (int s = S; s < N; s++) {
+        int nn = 0, nb = 0;
+        double born[OR_MAXN];
+        for (int v = 0; v < N; v++) {
+            if (v >= S && gch[2 * v] == -1) continue;
+            if (v != groot && gpar[v] < 0) continue;
+            if (!gt_present(sp, gch, gpar, gh, v, s)) continue;
+            if (v >= S && or_gt_species(sp, gch, gh, v) == s) born[nb++] = gh[v];
+            else nn++;
+        }
+        if (nn == 0) continue;
+        for (int i = 1; i < nb; i++)           /* insertion sort: ascending heights */
+            for (int j = i; j > 0 && born[j - 1] > born[j]; j--) { double t = born[j]; born[j] = born[j - 1]; born[j - 1] = t; }
+        double cur = sp->height[s];
+        for (int i = 0; i < nb; i++) {
+            logP -= (born[i] - cur) * (nn - 1) * nn / sp->theta[s];
+            logP += log(2.0) - log(sp->theta[s]);
+            nn--;
+            cur = born[i];
+        }
+        if (nn > 1 && sp->parent[s] != N)
+            logP -= (sp->height[sp->parent[s]] - cur) * (nn - 1) * nn / sp->theta[s];
+    }
+    return logP;
+}
+
+/* GTree::get_logpZ1 (genetree.cpp:1156-1190): every tip, then every gene node
+ * born in an internal species s, scores log P(missing | Z_s) with
+ * P(missing | Z = 1) = consToMis and P(missing | Z != 1) = nconsToMis. */
+double or_gt_logpz(const or_sptree *sp, const int *gch, const double *gh, const int *gmiss, const int *Z,
+                   double c2m, double n2m)
+{
+    int S = sp->S, N = sp->N;
+    double lp = 0;
+    for (int s = 0; s < S; s++)
+        lp += gmiss[s] ? (Z[s] == 1 ? log(c2m) : log(n2m)) : (Z[s] == 1 ? log(1 - c2m) : log(1 - n2m));
+    for (int v = S; v < N; v++) {
+        if (gch[2 * v] == -1) continue;
+        int s = or_gt_species(sp, gch, gh, v);
+        if (s < S || gh[v] < sp->height[s]) continue;
+        lp += gmiss[v] ? (Z[s] == 1 ? log(c2m) : log(n2m)) : (Z[s] == 1 ? log(1 - c2m) : log(1 - n2m));
+    }
+    return lp;
+}
+
+/* The pieces every gene branch spends in species branches: lineage v from
+ * max(h(v), bottom of s) to min(h(parent), top of s) in each species s on its
+ * path (GTree::Update_Tg / Update_Lambda compose one transition per piece).
+ * Returns K; the arrays hold up to N * (depth + 1) pieces. */
+int or_gt_pieces(const or_sptree *sp, const int *gch, const int *gpar, const double *gh, int groot, int *seg_gene,
+                 int *seg_sp, double *seg_len)
+{
+    int S = sp->S, N = sp->N, K = 0;
+    for (int v = 0; v < N; v++) {
+        if (v == groot || gpar[v] < 0 || gpar[v] >= N) continue;
+        if (v >= S && gch[2 * v] == -1) continue;
+        int s = or_gt_species(sp, gch, gh, v), top = or_gt_species(sp, gch, gh, gpar[v]);
+        double lo = gh[v];
+        for (;;) {
+            double bottom = sp->height[s] > lo ? sp->height[s] : lo;
+            double hi = (s == top || sp->parent[s] >= N) ? gh[gpar[v]] : sp->height[sp->parent[s]];
+            seg_gene[K] = v; seg_sp[K] = s; seg_len[K] = hi - bottom; K++;
+            if (s == top || sp->parent[s] >= N) break;
+            s = sp->parent[s];
+        }
+    }
+    return K;
+}
+
+/* GTree::Sample_BranchLen (genetree.cpp:1658-1965) on a gene tree held as
+ * topology + heights: gene node gnode's height moves uniformly in a window of
+ * half-width delta * theta_ss / 2 (ss: gnode's species) clipped to
+ * [max(bottom of css, h(cnode)), upper1], where cnode is the taller child, css
+ * the lowest species on cnode's path that the sibling's lineage also reaches,
+ * and upper1 the parent gene node when that coalesces in ss, else the top of
+ * ss (the gene root: 10 theta_root / 2, or h(root) + 5 theta_root / 2 when
+ * that is lower).  Metropolis-Hastings on the prefix log-likelihood of the
+ * WL indicator, the missing-gene term (when logpz[indicator] != 0), the
+ * multispecies-coalescent prior and the clipped window's proposal ratio.
+ * Draws: tag GT_BRANCH, sub-stream slot, sweep m: the height's uniform
+ * (floored at 1e-5) at 0, the decision's at 1; none when the window is empty.
+ * gh, loglik [2], logpz [2] and *prior are updated on acceptance.  Returns 1
+ * accepted, 0 rejected or empty window, -1 when no species holds both
+ * children's lineages (the reference throws). */
+int or_gt_branch_len(const or_model *m, double piA, const or_sptree *sp, const int *gch, const int *gpar, double *gh,
+                     const int *gmiss, int groot, const int *Z, double c_rate, double n_rate, double c2m, double n2m,
+                     int gnode, double delta, int indicator, const int *lens, double *loglik, double *logpz,
+                     double *prior, uint64_t seed, int m_sweep, int slot, const char *const *rows, long start, int GG,
+                     int gapchar, double gap_prop)
+{
+    int N = sp->N;
+    const double *H = sp->height;
+    int c0 = gch[2 * gnode], c1 = gch[2 * gnode + 1], cnode, sib;
+    if (gh[gnode] - gh[c0] < gh[gnode] - gh[c1]) { cnode = c0; sib = c1; }
+    else { cnode = c1; sib = c0; }
+    int ss1 = or_gt_species(sp, gch, gh, cnode), ss = ss1, css = ss1, spp = ss, found = 0;
+    while (spp < N && H[spp] < gh[gnode]) {
+        ss = spp;
+        spp = sp->parent[ss];
+        if (!found && gt_present(sp, gch, gpar, gh, sib, ss)) { css = ss; found = 1; }
+    }
+    if (!found) return -1;
+    int pg = gpar[gnode];
+    double upper1;
+    if (pg < N && or_gt_species(sp, gch, gh, pg) == ss) upper1 = gh[pg];
+    else upper1 = spp < N ? H[spp] : INFINITY;
+    if (gnode == groot) {
+        upper1 = 10 * sp->theta[N - 1] / 2;
+        if (upper1 < gh[groot]) upper1 = gh[groot] + 5 * sp->theta[N - 1] / 2;
+    }
+    double lower1 = H[css] > gh[cnode] ? H[css] : gh[cnode];
+    double half = delta * sp->theta[ss] / 2;
+    double upper = fmin(upper1, gh[gnode] + half), lower = fmax(lower1, gh[gnode] - half);
+    double difL = upper - lower;
+    if (difL == 0) return 0;
+    uint32_t c3 = tagword(0, OR_TAG_GT_BRANCH), c2 = (uint32_t)m_sweep;
+    double u = slot_u(seed, (uint32_t)slot, c2, c3, 0);
+    double prop = fmax(u, 1e-5) * difL + lower;
+    double r = slot_u(seed, (uint32_t)slot, c2, c3, 1);
+    double old = gh[gnode];
+    gh[gnode] = prop;
+    /* the proposal's prefix log-likelihoods (GG sites, Wang-Landau lens) */
+    int cap = N * (N + 1);
+    int *sg = (int *)malloc(sizeof(int) * cap), *ssp = (int *)malloc(sizeof(int) * cap);
+    double *sl = (double *)malloc(sizeof(double) * cap), *site = (double *)malloc(sizeof(double) * (GG > 0 ? GG : 1));
+    int K = or_gt_pieces(sp, gch, gpar, gh, groot, sg, ssp, sl);
+    or_model mm;
+    if (piA > 0) gt_model_at(m, piA, &mm); else mm = *m;
+    or_gt_site_logliks(&mm, sp->S, gch, groot, K, sg, ssp, sl, Z, c_rate, n_rate, rows, start, GG, gapchar, gap_prop,
+                       site);
+    double lnew[2] = {0, 0};
+    for (int g = 0; g < lens[indicator]; g++) {
+        lnew[0] += site[g];
+        if (g < lens[1]) lnew[1] += site[g];
+    }
+    free(sg); free(ssp); free(sl); free(site);
+    double lpz_new = or_gt_logpz(sp, gch, gh, gmiss, Z, c2m, n2m);
+    double cur_den = -log(upper - lower);
+    double lower2 = fmax(lower1, prop - half), upper2 = fmin(upper1, prop + half);
+    double pro_den = -log(upper2 - lower2);
+    double d = lnew[indicator] - loglik[indicator] + pro_den - cur_den;
+    if (logpz[indicator] != 0) d += lpz_new - logpz[indicator];
+    double prior_new = or_gt_prior(sp, gch, gpar, gh, groot);
+    d += prior_new - *prior;
+    if (log(r) < d) {
+        loglik[0] = lnew[0];
+        loglik[1] = lnew[1];
+        if (logpz[1] == 0) logpz[0] = lpz_new;
+        else logpz[0] = logpz[1] = lpz_new;
+        *prior = prior_new;
+        return 1;
+    }
+    gh[gnode] = old;
+    return 0;
+}

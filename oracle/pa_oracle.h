@@ -28,7 +28,8 @@ enum { OR_A = 0, OR_C = 1, OR_G = 2, OR_T = 3, OR_GAP = 4, OR_OTHER = 5,
 enum { OR_FLAG_NAN = 1, OR_FLAG_POS_INF = 2, OR_FLAG_NEG_INF = 4, OR_FLAG_SAMPLER = 8, OR_FLAG_PROPOSAL_RANGE = 16 };
 
 /* RNG stream tags (bits 16..23 of counter word 3) */
-enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5, OR_TAG_PI = 6, OR_TAG_WL = 7 };
+enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5, OR_TAG_PI = 6, OR_TAG_WL = 7,
+       OR_TAG_GT_BRANCH = 8, OR_TAG_GT_TREE = 9 };
 
 typedef struct {
     int burnin, mcmc, adapt_freq;          /* BURNIN, MCMC, ADAPT_FREQ */
@@ -131,6 +132,29 @@ int or_run_element(const or_model *m, const or_options *o, or_element *e, or_cha
 double or_gt_loglik(const or_model *m, int S, const int *gchildren, int K, const int *seg_gene, const int *seg_sp,
                     const double *seg_len, const int *Z, double r1, double r2, const char *const *rows, long start,
                     int GG, int gapchar, double gap_prop);
+
+/* PhyloAcc-GT's species tree (BPP::InitPhyloTree, src/PhyloAcc-GT/bpp.cpp:40-84):
+ * parent [N] (N at the root), node heights [N], population sizes theta [N]. */
+typedef struct { int S, N; const int *parent; const double *height; const double *theta; } or_sptree;
+/* A gene tree (GTree) as topology + heights: gch [N*2] (-1 at leaves; leaf i
+ * is species i), gpar [N] (N at the root), gh [N], gmiss [N] (missing_gene).
+ * Everything GTree caches per species (parent_gene2, temp_coal) follows from
+ * these and the species tree. */
+int or_gt_species(const or_sptree *sp, const int *gch, const double *gh, int v);
+double or_gt_prior(const or_sptree *sp, const int *gch, const int *gpar, const double *gh, int groot);
+double or_gt_logpz(const or_sptree *sp, const int *gch, const double *gh, const int *gmiss, const int *Z,
+                   double c2m, double n2m);
+int or_gt_pieces(const or_sptree *sp, const int *gch, const int *gpar, const double *gh, int groot, int *seg_gene,
+                 int *seg_sp, double *seg_len);
+void or_gt_site_logliks(const or_model *m, int S, const int *gchildren, int groot, int K, const int *seg_gene,
+                        const int *seg_sp, const double *seg_len, const int *Z, double r1, double r2,
+                        const char *const *rows, long start, int GG, int gapchar, double gap_prop, double *site);
+/* One GTree::Sample_BranchLen move; returns 1 accepted, 0 not, -1 error. */
+int or_gt_branch_len(const or_model *m, double piA, const or_sptree *sp, const int *gch, const int *gpar, double *gh,
+                     const int *gmiss, int groot, const int *Z, double c_rate, double n_rate, double c2m, double n2m,
+                     int gnode, double delta, int indicator, const int *lens, double *loglik, double *logpz,
+                     double *prior, uint64_t seed, int m_sweep, int slot, const char *const *rows, long start, int GG,
+                     int gapchar, double gap_prop);
 
 /* test hook: one Z forward-backward draw on a bare tree */
 int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, const int *fixZ, const double *E,

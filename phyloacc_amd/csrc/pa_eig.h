@@ -15,13 +15,13 @@
 
 namespace pa {
 
-#define PA_HD __host__ __device__
+#define PA_EIG_HD __host__ __device__ inline
 
 enum { EIG_OK = 0, EIG_NO_STATIONARY = 1, EIG_NOT_REVERSIBLE = 2, EIG_INACCURATE = 3 };
 
 // Stationary distribution of Q: pi Q = 0, sum pi = 1 (Gaussian elimination
 // with partial pivoting); false unless every pi_i > 0.
-PA_HD inline bool stationary4(const double *Q, double *pi)
+PA_EIG_HD bool stationary4(const double *Q, double *pi)
 {
     double A[4][5];
     for (int i = 0; i < 3; i++) {
@@ -47,7 +47,7 @@ PA_HD inline bool stationary4(const double *Q, double *pi)
 }
 
 // Cyclic Jacobi on a symmetric 4x4: A = U diag(w) U^T.
-PA_HD inline void jacobi4(double A[4][4], double w[4], double U[4][4])
+PA_EIG_HD void jacobi4(double A[4][4], double w[4], double U[4][4])
 {
     for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) U[i][j] = i == j;
     for (int sweep = 0; sweep < 64; sweep++) {
@@ -82,7 +82,7 @@ PA_HD inline void jacobi4(double A[4][4], double w[4], double U[4][4])
 
 // Q = V diag(lam) Vi (row-major, P(t)[i][j] = sum_k V[i][k] e^{lam_k t}
 // Vi[k][j] = P_{i->j}) for a time-reversible Q; EIG_* status.
-PA_HD inline int reversible_eig4(const double *Q, double *lam, double *V, double *Vi)
+PA_EIG_HD int reversible_eig4(const double *Q, double *lam, double *V, double *Vi)
 {
     double pq[4];
     if (!stationary4(Q, pq)) return EIG_NO_STATIONARY;
@@ -118,9 +118,9 @@ PA_HD inline int reversible_eig4(const double *Q, double *lam, double *V, double
 // x / (1 - e^-x) (1 at x = 0) with x = s + b for A/T -> C/G, s - b for
 // C/G -> A/T and s otherwise; the diagonal makes rows sum to 0.  s = b = 0
 // returns Q0 unchanged.
-PA_HD inline double gbgc_fix(double x) { return x == 0 ? 1.0 : x / (1 - exp(-x)); }
+PA_EIG_HD double gbgc_fix(double x) { return x == 0 ? 1.0 : x / (1 - exp(-x)); }
 
-PA_HD inline void gbgc_rate(const double *Q0, double s, double b, double *Q)
+PA_EIG_HD void gbgc_rate(const double *Q0, double s, double b, double *Q)
 {
     if (s == 0 && b == 0) {
         for (int i = 0; i < 16; i++) Q[i] = Q0[i];
@@ -144,7 +144,7 @@ PA_HD inline void gbgc_rate(const double *Q0, double s, double b, double *Q)
 // BPP_C::sample_pi, bpp_c.cpp:1545-1546): pi = (a, 1/2 - a, 1/2 - a, a) and
 // Q[i][j] = inst[k(i, j)] * pi[j] off the diagonal, k over the pairs AC, AG,
 // AT, CG, CT, GT; rows sum to 0.
-PA_HD inline void gt_strand_q(const double *inst, double a, double *Q, double *pi)
+PA_EIG_HD void gt_strand_q(const double *inst, double a, double *Q, double *pi)
 {
     pi[0] = pi[3] = a;
     pi[1] = pi[2] = 0.5 - a;

@@ -23,6 +23,7 @@
 #include "pa_chain.h"
 #include "pa_rng.h"
 #include "pa_eig.h"
+#include "pa_internal.h"
 
 using namespace pa;
 
@@ -713,6 +714,36 @@ extern "C" pa_status pa_element_get(const pa_engine *e, int32_t id, pa_element_i
     (void)S;
     return PA_OK;
 }
+
+namespace pa {
+
+void engine_view(const pa_engine *e, EngineView *v)
+{
+    v->S = e->tm.S; v->N = e->tm.N;
+    v->key0 = e->M.key0; v->key1 = e->M.key1;
+}
+
+pa_status engine_fail(pa_engine *e, pa_status st, const std::string &msg) { return fail(e, st, msg); }
+
+pa_status element_gene_tree(pa_engine *e, int32_t id, uint8_t *leaf_missing, const int32_t *children)
+{
+    if (id < 0 || id >= (int)e->elems.size()) return fail(e, PA_E_INPUT, "element id out of range");
+    const HostElem &E = e->elems[id];
+    if (E.filtered) return fail(e, PA_E_INPUT, "element " + std::to_string(id) + " is filtered");
+    const int N = e->tm.N, S = e->tm.S;
+    const int32_t *T = E.tab.data();
+    for (int s = 0; s < S; s++) leaf_missing[s] = (T[tab_flags(N) + s] & NF_MISSING) ? 1 : 0;
+    if (children)
+        for (int v = S; v < N; v++) {
+            const int a = T[tab_ch0(N) + v], b = T[tab_ch1(N) + v], x = children[2 * v], y = children[2 * v + 1];
+            if (!((a == x && b == y) || (a == y && b == x)))
+                return fail(e, PA_E_INPUT, "element " + std::to_string(id) + ": gene node " + std::to_string(v) +
+                                               " has other children than the element's stored tree");
+        }
+    return PA_OK;
+}
+
+}  // namespace pa
 
 // Upload every element's encoding into flat device arenas.
 static pa_status upload(pa_engine *e)

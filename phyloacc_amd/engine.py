@@ -33,7 +33,22 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_serve_destroy", "pa_serve_clock", "pa_set_tuning", "pa_get_tuning", "pa_autotune",
            "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc",
            "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats",
-           "pa_loglik_lengths_pi", "pa_sample_pi", "pa_loglik_lengths_split", "pa_wl_step"]
+           "pa_loglik_lengths_pi", "pa_sample_pi", "pa_loglik_lengths_split", "pa_wl_step",
+           "pa_gt_prior", "pa_gt_logpz", "pa_gt_lengths", "pa_gt_branch_len"]
+
+
+class SpeciesTree(C.Structure):
+    """pa_species_tree over numpy copies kept alive by the struct."""
+    _fields_ = [("S", C.c_int32), ("N", C.c_int32), ("parent", C.POINTER(C.c_int32)),
+                ("height", C.POINTER(C.c_double)), ("theta", C.POINTER(C.c_double))]
+
+    @classmethod
+    def make(cls, S, parent, height, theta):
+        keep = [np.ascontiguousarray(parent, np.int32), np.ascontiguousarray(height, np.float64),
+                np.ascontiguousarray(theta, np.float64)]
+        t = cls(int(S), len(keep[0]), _p(keep[0], C.c_int32), _p(keep[1], C.c_double), _p(keep[2], C.c_double))
+        t._keep = keep
+        return t
 
 
 class PaError(RuntimeError):
@@ -184,6 +199,17 @@ def lib():
         L.pa_wl_step.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                  P(C.c_int32)]
         L.pa_wl_step.restype = C.c_int
+        PI, PD = P(C.c_int32), P(C.c_double)
+        L.pa_gt_prior.argtypes = [P(SpeciesTree), PI, PI, PD, PD]
+        L.pa_gt_prior.restype = C.c_int
+        L.pa_gt_logpz.argtypes = [P(SpeciesTree), PI, PI, PD, PI, PI, C.c_double, C.c_double, PD]
+        L.pa_gt_logpz.restype = C.c_int
+        L.pa_gt_lengths.argtypes = [P(SpeciesTree), PI, PI, PD, PI, C.c_double, C.c_double, PD]
+        L.pa_gt_lengths.restype = C.c_int
+        L.pa_gt_branch_len.argtypes = [C.c_void_p, P(SpeciesTree), PI, C.c_int32, PI, PI, PD, PI, PD, PD, PI, PI, PI,
+                                       C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, PD, PD,
+                                       PD, PI]
+        L.pa_gt_branch_len.restype = C.c_int
         L.pa_loglik_last_stats.argtypes = [C.c_void_p, P(RunStats)]
         L.pa_loglik_last_stats.restype = C.c_int
         L.pa_batch_create.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(C.c_void_p)]
@@ -406,6 +432,33 @@ class Engine:
                                      _p(state, C.c_double), _p(ind, C.c_int32)))
         return ind
 
+    def gt_branch_len(self, sp, ids, children, parent, heights, Z, rates, gnode, indicator, lens, loglik, logpz,
+                      prior, m, slot, delta=1.0, cons_to_mis=0.01, ncons_to_mis=0.5, piA=None, precision=PA_FP64):
+        """One batched Sample_BranchLen move.  heights [n, N], loglik, logpz
+        [n, 2] and prior [n] (float64, C order) are updated in place; returns
+        accepted [n]."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        n = len(ids)
+        ch = np.ascontiguousarray(children, np.int32).reshape(n, self.N, 2)
+        par = np.ascontiguousarray(parent, np.int32).reshape(n, self.N)
+        for a in (heights, loglik, logpz, prior):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        assert heights.shape == (n, self.N) and loglik.shape == (n, 2) and logpz.shape == (n, 2) and prior.shape == (n,)
+        z = np.ascontiguousarray(Z, np.int32).reshape(n, self.N)
+        r = np.ascontiguousarray(rates, np.float64).reshape(n, 2)
+        g = np.ascontiguousarray(np.broadcast_to(gnode, (n,)), np.int32)
+        ind = np.ascontiguousarray(np.broadcast_to(indicator, (n,)), np.int32)
+        ln = np.ascontiguousarray(np.broadcast_to(lens, (n, 2)), np.int32)
+        a = None if piA is None else np.ascontiguousarray(piA, np.float64)
+        acc = np.zeros(n, np.int32)
+        self._check(lib().pa_gt_branch_len(
+            self._h, C.byref(sp), _p(ids, C.c_int32), n, _p(ch, C.c_int32), _p(par, C.c_int32),
+            _p(heights, C.c_double), _p(z, C.c_int32), _p(r, C.c_double), None if a is None else _p(a, C.c_double),
+            _p(g, C.c_int32), _p(ind, C.c_int32), _p(ln, C.c_int32), float(delta), float(cons_to_mis),
+            float(ncons_to_mis), int(m), int(slot), int(precision), _p(loglik, C.c_double), _p(logpz, C.c_double),
+            _p(prior, C.c_double), _p(acc, C.c_int32)))
+        return acc
+
     def loglik_stats(self):
         """Kernel time and counted bytes of the last likelihood call."""
         st = RunStats()
@@ -570,6 +623,41 @@ def gene_lengths(N, seg_gene, seg_species, seg_len, Z, r1, r2):
                                _p(z, C.c_int32), float(r1), float(r2), _p(out, C.c_double))
     if st != PA_OK:
         raise PaError(st, "pa_gene_lengths failed")
+    return out
+
+
+def _gt_call(name, sp, children, parent, heights, *rest):
+    ch = np.ascontiguousarray(children, np.int32).reshape(-1)
+    par = np.ascontiguousarray(parent, np.int32)
+    h = np.ascontiguousarray(heights, np.float64)
+    st = getattr(lib(), name)(C.byref(sp), _p(ch, C.c_int32), _p(par, C.c_int32), _p(h, C.c_double), *rest)
+    if st != PA_OK:
+        raise PaError(st, f"{name} failed")
+
+
+def gt_prior(sp, children, parent, heights):
+    """GTree::priorTree of one gene tree (host)."""
+    out = C.c_double()
+    _gt_call("pa_gt_prior", sp, children, parent, heights, C.byref(out))
+    return out.value
+
+
+def gt_logpz(sp, children, parent, heights, leaf_missing, Z, cons_to_mis, ncons_to_mis):
+    """GTree::get_logpZ1 of one gene tree (host)."""
+    lm = np.ascontiguousarray(leaf_missing, np.int32)
+    z = np.ascontiguousarray(Z, np.int32)
+    out = C.c_double()
+    _gt_call("pa_gt_logpz", sp, children, parent, heights, _p(lm, C.c_int32), _p(z, C.c_int32),
+             float(cons_to_mis), float(ncons_to_mis), C.byref(out))
+    return out.value
+
+
+def gt_lengths(sp, children, parent, heights, Z, r1, r2):
+    """Substitution length of every gene branch of one gene tree (host)."""
+    z = np.ascontiguousarray(Z, np.int32)
+    out = np.zeros(sp.N, np.float64)
+    _gt_call("pa_gt_lengths", sp, children, parent, heights, _p(z, C.c_int32), float(r1), float(r2),
+             _p(out, C.c_double))
     return out
 
 
