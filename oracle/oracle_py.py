@@ -181,6 +181,48 @@ def gt_branch_len(model, sp, gt, Z, c_rate, n_rate, c2m, n2m, gnode, delta, indi
     return code, pr.value
 
 
+def gt_sample_history(model, sp, gt, Z, c_rate, n_rate, seed, m, length, align, start, size, piA=0.0,
+                      gapchar="-", gap_prop=0.8):
+    """Update_Tg(tosample=true): int8 [N, length] history of the first
+    `length` sites of an element of `size` sites."""
+    rows, keep = _rows(align)
+    z = np.ascontiguousarray(Z, np.int32)
+    out = np.zeros((len(gt.par), int(length)), np.int8)
+    lib().or_gt_sample_history(C.byref(model), float(piA), C.byref(sp), *gt.args(), gt.root, _ptr(z, C.c_int),
+                               float(c_rate), float(n_rate), int(seed), int(m), int(length), rows, int(start),
+                               int(size), ord(gapchar), float(gap_prop), _ptr(out, C.c_int8))
+    return out
+
+
+def gt_sample_tree(model, sp, gt, Z, c_rate, n_rate, c2m, n2m, branch, indicator, lens, Tg, loglik, logpz, seed, m,
+                   slot, align, start, size, root_ingrp=None, piA=0.0, gapchar="-", gap_prop=0.8, exact=False,
+                   scores=None):
+    """One Sample_tree2 move on gt (updated in place when accepted); Tg is
+    the history of the first lens[indicator] sites.  Returns 1 or 0.  exact:
+    the test hook's complete Hastings ratio; scores: a list that receives
+    [(candidate, score), ...]."""
+    rows, keep = _rows(align)
+    N = len(gt.par)
+    cn = np.zeros(N + 2, np.int32)
+    cs = np.zeros(N + 2)
+    nc = C.c_int(0)
+    z = np.ascontiguousarray(Z, np.int32)
+    ln = np.ascontiguousarray(lens, np.int32)
+    T = np.ascontiguousarray(Tg, np.int8)
+    assert T.shape == (len(gt.par), int(ln[indicator]))
+    root = C.c_int(gt.root)
+    ri = sp.N - 1 if root_ingrp is None else int(root_ingrp)
+    code = lib().or_test_sample_tree(C.byref(model), float(piA), C.byref(sp), *gt.args(), C.byref(root),
+                                     _ptr(z, C.c_int), float(c_rate), float(n_rate), float(c2m), float(n2m), ri,
+                                     int(branch), int(indicator), _ptr(ln, C.c_int), _ptr(T, C.c_int8),
+                                     _ptr(loglik, C.c_double), _ptr(logpz, C.c_double), int(seed), int(m), int(slot),
+                                     rows, int(start), int(size), ord(gapchar), float(gap_prop), int(bool(exact)),
+                                     _ptr(cn, C.c_int), _ptr(cs, C.c_double), C.byref(nc))
+    if scores is not None:
+        scores[:] = list(zip(cn[:nc.value].tolist(), cs[:nc.value].tolist()))
+    return code
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -252,6 +294,15 @@ def load(path):
         L.or_gt_site_logliks.argtypes = [P(Model), C.c_int, PI, C.c_int, C.c_int, PI, PI, PD, PI, C.c_double,
                                          C.c_double, P(C.c_char_p), C.c_long, C.c_int, C.c_int, C.c_double, PD]
         L.or_gt_site_logliks.restype = None
+        L.or_gt_sample_history.argtypes = [P(Model), C.c_double, P(SpTree), PI, PI, PD, C.c_int, PI, C.c_double,
+                                           C.c_double, C.c_uint64, C.c_int, C.c_int, P(C.c_char_p), C.c_long, C.c_int,
+                                           C.c_int, C.c_double, P(C.c_int8)]
+        L.or_gt_sample_history.restype = None
+        L.or_gt_sample_tree.argtypes = [P(Model), C.c_double, P(SpTree), PI, PI, PD, PI, PI, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, PI, P(C.c_int8), PD, PD,
+                                        C.c_uint64, C.c_int, C.c_int, P(C.c_char_p), C.c_long, C.c_int, C.c_int,
+                                        C.c_double]
+        L.or_test_sample_tree.argtypes = L.or_gt_sample_tree.argtypes + [C.c_int, PI, PD, PI]
         L.or_gt_branch_len.argtypes = [P(Model), C.c_double, P(SpTree), PI, PI, PD, PI, C.c_int, PI, C.c_double,
                                        C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, PI, PD, PD,
                                        PD, C.c_uint64, C.c_int, C.c_int, P(C.c_char_p), C.c_long, C.c_int, C.c_int,

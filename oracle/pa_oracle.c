@@ -1802,3 +1802,415 @@ int or_gt_branch_len(const or_model *m, double piA, const or_sptree *sp, const i
     gh[gnode] = old;
     return 0;
 }
+
+/* Shared set-up of one gene tree for per-site work: the post-order from the
+ * root, each piece's log transition (species rate applied) and the missing
+ * genes (more than gap_prop of the GG sites are gaps; an internal gene when
+ * both children are). */
+typedef struct {
+    int S, N, groot, no, K;
+    int order[OR_MAXN], miss[OR_MAXN];
+    int *sg, *ssp;
+    double *sl, (*L)[16];
+    double logpi[4];
+} gt_ctx;
+
+static void gt_ctx_init(gt_ctx *c, const or_model *m, const or_sptree *sp, const int *gch, const int *gpar,
+                        const double *gh, int groot, const int *Z, double c_rate, double n_rate,
+                        const char *const *rows, long start, int GG, int gapchar, double gap_prop)
+{
+    int N = sp->N, S = sp->S, cap = N * (N + 1);
+    c->S = S; c->N = N; c->groot = groot;
+    c->sg = (int *)malloc(sizeof(int) * cap);
+    c->ssp = (int *)malloc(sizeof(int) * cap);
+    c->sl = (double *)malloc(sizeof(double) * cap);
+    c->K = or_gt_pieces(sp, gch, gpar, gh, groot, c->sg, c->ssp, c->sl);
+    c->L = (double (*)[16])malloc(sizeof(double) * 16 * (c->K > 0 ? c->K : 1));
+    for (int k = 0; k < c->K; k++) {
+        int z = Z[c->ssp[k]];
+        double rate = z == 2 ? n_rate : (z == 1 ? c_rate : 1.0);
+        if (c->sl[k] < 1e-8) log_identity(c->L[k]);
+        else or_log_trans(m, c->sl[k] * rate, c->L[k]);
+    }
+    for (int j = 0; j < 4; j++) c->logpi[j] = log(m->pi[j]);
+    int stk[OR_MAXN], vis[OR_MAXN], ns = 0;
+    c->no = 0;
+    stk[ns] = groot; vis[ns++] = 0;
+    while (ns > 0) {
+        int v = stk[ns - 1];
+        if (v >= S && vis[ns - 1] == 0) {
+            vis[ns - 1] = 1;
+            stk[ns] = gch[2 * v + 1]; vis[ns++] = 0;
+            stk[ns] = gch[2 * v]; vis[ns++] = 0;
+        } else {
+            c->order[c->no++] = v;
+            ns--;
+        }
+    }
+    for (int s = 0; s < N; s++) c->miss[s] = 0;
+    for (int s = 0; s < S; s++) {
+        int ng = 0;
+        for (int g = 0; g < GG; g++) ng += tolower((unsigned char)rows[s][start + g]) == gapchar;
+        c->miss[s] = ng > gap_prop * GG;
+    }
+    for (int q = 0; q < c->no; q++) {
+        int s = c->order[q];
+        if (s >= S) c->miss[s] = c->miss[gch[2 * s]] && c->miss[gch[2 * s + 1]];
+    }
+}
+
+static void gt_ctx_free(gt_ctx *c) { free(c->sg); free(c->ssp); free(c->sl); free(c->L); }
+
+/* Site g's log partials lam[v] of every node on the tree and gap[v] (the
+ * subtree holds no base), messages composed piece by piece (Update_Tg with
+ * tosample = false); tg[v] of the leaves: 0-3 a/c/g/t, -1 ambiguous, >= 4 no
+ * base (the GT encoding, src/PhyloAcc-GT/bpp_c.hpp:340-384). */
+static void gt_site_partials(const gt_ctx *c, const int *gch, const char *const *rows, long start, int g, int gapchar,
+                             double (*lam)[4], int *gap, int *tg)
+{
+    for (int s = 0; s < c->S; s++) {
+        unsigned char code;
+        encode_char(tolower((unsigned char)rows[s][start + g]), gapchar, lam[s], &tg[s], &code);
+        gap[s] = tg[s] >= 4;
+    }
+    for (int q = 0; q < c->no; q++) {
+        int s = c->order[q];
+        if (s < c->S) continue;
+        for (int j = 0; j < 4; j++) lam[s][j] = 0;
+        int a = gch[2 * s], b = gch[2 * s + 1];
+        gap[s] = gap[a] && gap[b];
+        for (int k2 = 0; k2 < 2; k2++) {
+            int x = k2 ? b : a;
+            if (gap[x] || c->miss[x]) continue;
+            double msg[4], tmp[4];
+            memcpy(msg, lam[x], sizeof msg);
+            for (int k = 0; k < c->K; k++) {
+                if (c->sg[k] != x) continue;
+                log_multi(c->L[k], msg, tmp);
+                memcpy(msg, tmp, sizeof msg);
+            }
+            for (int j = 0; j < 4; j++) lam[s][j] += msg[j];
+        }
+    }
+}
+
+/* GTree::Update_Tg with tosample = true (src/PhyloAcc-GT/genetree.cpp:
+ * 1416-1532): for sites g < len, the gene root's base from pi * exp(lam_root),
+ * then down the tree each gene node's base from P(parent base -> .) *
+ * exp(lam) over its whole branch (the reference steps through the species
+ * boundaries; the composed draw has the same law).  Gene nodes that are
+ * missing genes or hold no base at the site get 4; leaves keep their
+ * encoding (0-3, -1 ambiguous, 4 no base).  Tg [N * len], node-major.
+ * Draws: tag GT_HIST, sub-stream g, sweep m, slot = node id. */
+void or_gt_sample_history(const or_model *m, double piA, const or_sptree *sp, const int *gch, const int *gpar,
+                          const double *gh, int groot, const int *Z, double c_rate, double n_rate, uint64_t seed,
+                          int m_sweep, int len, const char *const *rows, long start, int GG, int gapchar,
+                          double gap_prop, signed char *Tg)
+{
+    or_model mm;
+    if (piA > 0) gt_model_at(m, piA, &mm); else mm = *m;
+    gt_ctx c;
+    gt_ctx_init(&c, &mm, sp, gch, gpar, gh, groot, Z, c_rate, n_rate, rows, start, GG, gapchar, gap_prop);
+    int N = sp->N;
+    double (*lam)[4] = (double (*)[4])malloc(sizeof(double) * 4 * N);
+    double (*P)[16] = (double (*)[16])malloc(sizeof(double) * 16 * N);
+    int gap[OR_MAXN], tg[OR_MAXN];
+    /* each branch's transition over its whole substitution length */
+    for (int v = 0; v < N; v++) {
+        double len_v = 0;
+        for (int k = 0; k < c.K; k++)
+            if (c.sg[k] == v && c.sl[k] >= 1e-8) {
+                int z = Z[c.ssp[k]];
+                len_v += c.sl[k] * (z == 2 ? n_rate : (z == 1 ? c_rate : 1.0));
+            }
+        double LP[16];
+        if (len_v < 1e-8) log_identity(LP); else or_log_trans(&mm, len_v, LP);
+        for (int q = 0; q < 16; q++) P[v][q] = exp(LP[q]);
+    }
+    uint32_t c3 = tagword(0, OR_TAG_GT_HIST), c2 = (uint32_t)m_sweep;
+    for (int g = 0; g < len; g++) {
+        gt_site_partials(&c, gch, rows, start, g, gapchar, lam, gap, tg);
+        for (int v = 0; v < N; v++) Tg[(size_t)v * len + g] = 4;
+        for (int s = 0; s < c.S; s++) Tg[(size_t)s * len + g] = (signed char)(tg[s] >= 4 ? 4 : tg[s]);
+        /* pre-order: the reverse of the post-order */
+        for (int q = c.no - 1; q >= 0; q--) {
+            int v = c.order[q];
+            if (v < c.S) continue;
+            double p[4], mx = NEG_INF;
+            if (v == groot) {
+                for (int j = 0; j < 4; j++) { p[j] = lam[v][j] + c.logpi[j]; if (p[j] > mx) mx = p[j]; }
+            } else {
+                int x = Tg[(size_t)gpar[v] * len + g];
+                if (c.miss[v] || gap[v] || x < 0 || x > 3) continue;
+                /* log P(x -> j): row x of exp(log transposed) = column x */
+                for (int j = 0; j < 4; j++) {
+                    double pj = P[v][j * 4 + x];
+                    p[j] = pj > 0 ? log(pj) + lam[v][j] : NEG_INF;
+                    if (p[j] > mx) mx = p[j];
+                }
+            }
+            for (int j = 0; j < 4; j++) p[j] = exp(p[j] - mx);
+            Tg[(size_t)v * len + g] = (signed char)categorical(p, 4, slot_u(seed, (uint32_t)g, c2, c3, (uint32_t)v));
+        }
+    }
+    free(lam); free(P);
+    gt_ctx_free(&c);
+}
+
+/* Candidate scores of Sample_tree2: log P(branch's subtree | the new parent
+ * sits on candidate t at substitution distance len) summed over the sites
+ * g < len_sites where branch holds a base.  kind 0: t's sampled base x
+ * (log of row x of P(len) times branch's partial), or the pi-average of that
+ * message when t has no base; kind 1: the pi-average of the message (the
+ * root candidate); kind 2: the root candidate's first-option score
+ * log sum_i pi_i P(len)(b -> i), b = branch's own sampled base
+ * (genetree.cpp:2221-2232), where an ambiguous leaf contributes nothing. */
+static double gt_cand_score(const gt_ctx *c, const or_model *mm, const int *gch, const signed char *Tg, int len_sites,
+                            const char *const *rows, long start, int gapchar, int branch, int t, int kind, double len,
+                            double (*lam)[4], int *gap, int *tg, const double (*lam_branch)[4])
+{
+    double LP[16];
+    if (len < 1e-8) log_identity(LP); else or_log_trans(mm, len, LP);
+    double tot = 0;
+    (void)gch; (void)rows; (void)start; (void)gapchar; (void)lam; (void)gap; (void)tg;
+    for (int g = 0; g < len_sites; g++) {
+        int xb = Tg[(size_t)branch * len_sites + g];
+        if (xb >= 4) continue;
+        if (kind == 2) {
+            if (xb < 0) continue;
+            double y[4];
+            for (int i = 0; i < 4; i++) y[i] = LP[i * 4 + xb] + c->logpi[i];
+            tot += log_exp_sum(y, 4);
+            continue;
+        }
+        double M[4];
+        log_multi(LP, lam_branch[g], M);
+        int xt = kind == 0 ? Tg[(size_t)t * len_sites + g] : 4;
+        if (kind == 0 && !c->miss[t] && xt >= 0 && xt < 4) tot += M[xt];
+        else {
+            double y[4];
+            for (int j = 0; j < 4; j++) y[j] = M[j] + c->logpi[j];
+            tot += log_exp_sum(y, 4);
+        }
+    }
+    return tot;
+}
+
+int or_test_sample_tree(const or_model *m, double piA, const or_sptree *sp, int *gch, int *gpar, double *gh,
+                        int *groot_io, const int *Z, double c_rate, double n_rate, double c2m, double n2m,
+                        int root_ingrp, int branch, int indicator, const int *lens, const signed char *Tg,
+                        double *loglik, double *logpz, uint64_t seed, int m_sweep, int slot, const char *const *rows,
+                        long start, int GG, int gapchar, double gap_prop, int exact, int *cand_node,
+                        double *cand_score, int *n_cand);
+
+/* GTree::Sample_tree2 with Remove_branch and Graft (src/PhyloAcc-GT/
+ * genetree.cpp:2138-2518, 737-923) on a gene tree held as topology +
+ * heights, the unconstrained candidate set (BPP::cons_sample false: no
+ * deep-coalescence species and the default branch-sampling threshold).
+ *   Prune gene lineage `branch` with its parent bp: the sibling takes bp's
+ *   place under bpp.  Walk up the species from branch's species ss: every
+ *   gene node born in a species on the way (above branch in ss) is a
+ *   candidate new grandparent, at substitution distance cur_len + rate *
+ *   (h(t) - max(bottom, h(branch))), cur_len summing the species crossed
+ *   below; in the species root a candidate above the gene root (N) at the
+ *   fixed height curN = 10 theta_root / 2 (h(root) + 5 theta_root / 2 when
+ *   the root is higher); stop after root_ingrp once a candidate exists.
+ *   Draw a candidate with probability proportional to exp(score), then
+ *   the new sibling (one of its two children, or the gene root for N) and
+ *   bp's new height uniformly between where both lineages first share a
+ *   species and the candidate (curN rule for N).  Graft bp there, then
+ *   Metropolis-Hastings on the prefix likelihood, missing-gene term, the
+ *   coalescent prior, the two uniform heights' densities and the candidate
+ *   probabilities of the new and the old grandparent.
+ * Tg [N * Tlen] is the history from the last GTree::Update_Tg draw, Tlen =
+ * lens[indicator].  Draws: tag GT_TREE, sub-stream slot, sweep m: candidate
+ * 0, child 1 (u < 1/2: the first), height 2 (floored at 1e-5), decision 3.
+ * Returns 1 accepted (tree, loglik, logpz updated), 0 rejected (tree
+ * restored); an old grandparent that is not a candidate (possible only
+ * with root_ingrp below the species root) rejects the move. */
+int or_gt_sample_tree(const or_model *m, double piA, const or_sptree *sp, int *gch, int *gpar, double *gh,
+                      int *groot_io, const int *Z, double c_rate, double n_rate, double c2m, double n2m, int root_ingrp,
+                      int branch, int indicator, const int *lens, const signed char *Tg, double *loglik,
+                      double *logpz, uint64_t seed, int m_sweep, int slot, const char *const *rows, long start, int GG,
+                      int gapchar, double gap_prop)
+{
+    return or_test_sample_tree(m, piA, sp, gch, gpar, gh, groot_io, Z, c_rate, n_rate, c2m, n2m, root_ingrp, branch,
+                               indicator, lens, Tg, loglik, logpz, seed, m_sweep, slot, rows, start, GG, gapchar,
+                               gap_prop, 0, NULL, NULL, NULL);
+}
+
+/* Test hook: or_gt_sample_tree with, when exact is set, the probabilities of
+ * the new sibling's choice (1/2 among a candidate's two children, 1 for the
+ * root candidate) added to the Hastings ratio for both directions -- the
+ * reference leaves them out, so only with them is the move reversible with
+ * respect to the coalescent prior.  cand_node / cand_score (size N + 2, may
+ * be NULL) receive the candidates and their scores; the count is returned in
+ * *n_cand. */
+int or_test_sample_tree(const or_model *m, double piA, const or_sptree *sp, int *gch, int *gpar, double *gh,
+                        int *groot_io, const int *Z, double c_rate, double n_rate, double c2m, double n2m,
+                        int root_ingrp, int branch, int indicator, const int *lens, const signed char *Tg,
+                        double *loglik, double *logpz, uint64_t seed, int m_sweep, int slot, const char *const *rows,
+                        long start, int GG, int gapchar, double gap_prop, int exact, int *cand_node,
+                        double *cand_score, int *n_cand)
+{
+    int S = sp->S, N = sp->N, groot = *groot_io;
+    const double *H = sp->height;
+    or_model mm;
+    if (piA > 0) gt_model_at(m, piA, &mm); else mm = *m;
+    int save_ch[2 * OR_MAXN], save_par[OR_MAXN];
+    double save_h[OR_MAXN];
+    memcpy(save_ch, gch, sizeof(int) * 2 * N);
+    memcpy(save_par, gpar, sizeof(int) * N);
+    memcpy(save_h, gh, sizeof(double) * N);
+    double curG = or_gt_prior(sp, gch, gpar, gh, groot);
+    int ss = or_gt_species(sp, gch, gh, branch);
+    int bp = gpar[branch], cb = gch[2 * bp] == branch ? 0 : 1, bs = gch[2 * bp + 1 - cb], bpp = gpar[bp];
+    double thr = sp->theta[N - 1];
+    double curN = 10 * thr / 2;
+    if (curN < gh[groot]) curN = gh[groot] + 5 * thr / 2;
+    double curlow = 0;
+    for (int x = ss; x < N; x = sp->parent[x])
+        if (gt_present(sp, gch, gpar, gh, bs, x)) { curlow = fmax(H[x], fmax(gh[branch], gh[bs])); break; }
+    double curupp = bpp < N ? gh[bpp] : curN;
+    double cur_unif = -log(curupp - curlow);
+    /* the branch's partials over the Tlen sites (current tree) */
+    int Tlen = lens[indicator];
+    gt_ctx c0;
+    gt_ctx_init(&c0, &mm, sp, gch, gpar, gh, groot, Z, c_rate, n_rate, rows, start, GG, gapchar, gap_prop);
+    double (*lam)[4] = (double (*)[4])malloc(sizeof(double) * 4 * N);
+    double (*lb)[4] = (double (*)[4])malloc(sizeof(double) * 4 * (Tlen > 0 ? Tlen : 1));
+    int gap[OR_MAXN], tg[OR_MAXN];
+    for (int g = 0; g < Tlen; g++) {
+        gt_site_partials(&c0, gch, rows, start, g, gapchar, lam, gap, tg);
+        memcpy(lb[g], lam[branch], sizeof lb[g]);
+    }
+    /* Remove_branch */
+    int cbp = bpp < N ? (gch[2 * bpp] == bp ? 0 : 1) : -1;
+    gpar[bs] = bpp;
+    if (bpp < N) gch[2 * bpp + cbp] = bs; else groot = bs;
+    gpar[bp] = -1;
+    /* candidates on the pruned tree */
+    int cn[OR_MAXN + 2], csp[OR_MAXN + 2], ck[OR_MAXN + 2], nc = 0;
+    double cl[OR_MAXN + 2], cs[OR_MAXN + 2];
+    double cur_len = 0;
+    for (int x = ss; x < N;) {
+        double rate = Z[x] == 2 ? n_rate : (Z[x] == 1 ? c_rate : 1.0);
+        double base = fmax(H[x], gh[branch]);
+        int born[OR_MAXN], nb = 0;
+        for (int v = S; v < N; v++) {
+            if (v == bp || gch[2 * v] == -1) continue;
+            if (v != groot && gpar[v] < 0) continue;
+            if (or_gt_species(sp, gch, gh, v) == x) born[nb++] = v;
+        }
+        for (int i = 1; i < nb; i++)
+            for (int j = i; j > 0 && gh[born[j - 1]] > gh[born[j]]; j--) { int t = born[j]; born[j] = born[j - 1]; born[j - 1] = t; }
+        int up = sp->parent[x];
+        if (nb == 0) {
+            if (up == N) {
+                cn[nc] = N; csp[nc] = x; cl[nc] = rate * (curN - base) + cur_len;
+                ck[nc] = nc == 0 ? -1 : 2;
+                nc++;
+                break;
+            }
+            cur_len += (H[up] - base) * rate;
+            x = up;
+            continue;
+        }
+        for (int i = 0; i < nb; i++) {
+            int t = born[i];
+            if (x == ss && gh[t] <= gh[branch]) continue;
+            cn[nc] = t; csp[nc] = x; cl[nc] = rate * (gh[t] - base) + cur_len; ck[nc] = 0;
+            nc++;
+        }
+        if (up == N) {
+            double ht = nc > 0 ? gh[cn[nc - 1]] : base;
+            cn[nc] = N; csp[nc] = x; cl[nc] = cur_len + rate * (curN - ht); ck[nc] = 1;
+            nc++;
+        }
+        if (up < N) cur_len += (H[up] - base) * rate;
+        if (x == root_ingrp && nc > 0) break;
+        x = up;
+    }
+    for (int i = 0; i < nc; i++)
+        cs[i] = ck[i] < 0 ? 0.0 : gt_cand_score(&c0, &mm, gch, Tg, Tlen, rows, start, gapchar, branch, cn[i], ck[i],
+                                                cl[i], lam, gap, tg, (const double (*)[4])lb);
+    if (cand_node) for (int i = 0; i < nc; i++) { cand_node[i] = cn[i]; cand_score[i] = cs[i]; }
+    if (n_cand) *n_cand = nc;
+    double sp_p[OR_MAXN + 2];
+    uint32_t c3 = tagword(0, OR_TAG_GT_TREE), c2 = (uint32_t)m_sweep;
+    int get = 0;
+    if (nc == 1) sp_p[0] = 1;
+    else {
+        double mx = NEG_INF, tot = 0;
+        for (int i = 0; i < nc; i++) if (cs[i] > mx) mx = cs[i];
+        for (int i = 0; i < nc; i++) { sp_p[i] = exp(cs[i] - mx); tot += sp_p[i]; }
+        for (int i = 0; i < nc; i++) sp_p[i] /= tot;
+        get = categorical(sp_p, nc, slot_u(seed, (uint32_t)slot, c2, c3, 0));
+    }
+    int nbpp = cn[get], nsib;
+    double upp;
+    if (nbpp == N) {
+        nsib = groot;
+        upp = 10 * thr / 2 > gh[nsib] ? 10 * thr / 2 : gh[nsib] + 5 * thr / 2;
+    } else {
+        int a = slot_u(seed, (uint32_t)slot, c2, c3, 1) < 0.5 ? 0 : 1;
+        nsib = gch[2 * nbpp + a];
+        upp = gh[nbpp];
+    }
+    double low = 0;
+    for (int x = ss; x != sp->parent[csp[get]]; x = sp->parent[x])
+        if (gt_present(sp, gch, gpar, gh, nsib, x)) { low = fmax(gh[branch], fmax(gh[nsib], H[x])); break; }
+    double u = slot_u(seed, (uint32_t)slot, c2, c3, 2);
+    if (u < 1e-5) u = 1e-5;
+    double coal = u * (upp - low) + low;
+    double new_unif = -log(upp - low);
+    /* Graft */
+    int it2p = gpar[nsib], cs_sib = it2p < N ? (gch[2 * it2p] == nsib ? 0 : 1) : -1;
+    gh[bp] = coal;
+    gpar[bp] = it2p;
+    gpar[nsib] = bp;
+    gch[2 * bp + cb] = branch;
+    gch[2 * bp + 1 - cb] = nsib;
+    if (it2p < N) gch[2 * it2p + cs_sib] = bp; else groot = bp;
+    /* MH */
+    double newG = or_gt_prior(sp, gch, gpar, gh, groot);
+    double r = slot_u(seed, (uint32_t)slot, c2, c3, 3);
+    gt_ctx c1;
+    gt_ctx_init(&c1, &mm, sp, gch, gpar, gh, groot, Z, c_rate, n_rate, rows, start, GG, gapchar, gap_prop);
+    double lnew[2] = {0, 0};
+    for (int g = 0; g < Tlen; g++) {
+        gt_site_partials(&c1, gch, rows, start, g, gapchar, lam, gap, tg);
+        double y[4];
+        for (int j = 0; j < 4; j++) y[j] = lam[groot][j] + c1.logpi[j];
+        double t = log_exp_sum(y, 4);
+        lnew[0] += t;
+        if (g < lens[1]) lnew[1] += t;
+    }
+    int gm[OR_MAXN];
+    for (int v = 0; v < N; v++) gm[v] = c1.miss[v];
+    double lpz_new = or_gt_logpz(sp, gch, gh, gm, Z, c2m, n2m);
+    gt_ctx_free(&c0); gt_ctx_free(&c1);
+    free(lam); free(lb);
+    int pos = -1;
+    for (int i = 0; i < nc; i++) if (cn[i] == bpp) { pos = i; break; }
+    int ok = 0;
+    if (pos >= 0) {
+        double d = lnew[indicator] - loglik[indicator];
+        if (logpz[indicator] != 0) d += lpz_new - logpz[indicator];
+        d += newG - curG + cur_unif - new_unif + log(sp_p[pos]) - log(sp_p[get]);
+        if (exact) d += (bpp < N ? log(0.5) : 0.0) - (nbpp < N ? log(0.5) : 0.0);
+        ok = log(r) < d;
+    }
+    if (ok) {
+        loglik[0] = lnew[0];
+        loglik[1] = lnew[1];
+        if (logpz[1] == 0) logpz[0] = lpz_new;
+        else logpz[0] = logpz[1] = lpz_new;
+        *groot_io = groot;
+        return 1;
+    }
+    memcpy(gch, save_ch, sizeof(int) * 2 * N);
+    memcpy(gpar, save_par, sizeof(int) * N);
+    memcpy(gh, save_h, sizeof(double) * N);
+    return 0;
+}

@@ -29,7 +29,7 @@ enum { OR_FLAG_NAN = 1, OR_FLAG_POS_INF = 2, OR_FLAG_NEG_INF = 4, OR_FLAG_SAMPLE
 
 /* RNG stream tags (bits 16..23 of counter word 3) */
 enum { OR_TAG_ANC = 1, OR_TAG_Z = 2, OR_TAG_R2 = 3, OR_TAG_R1 = 4, OR_TAG_TRANS = 5, OR_TAG_PI = 6, OR_TAG_WL = 7,
-       OR_TAG_GT_BRANCH = 8, OR_TAG_GT_TREE = 9 };
+       OR_TAG_GT_BRANCH = 8, OR_TAG_GT_TREE = 9, OR_TAG_GT_HIST = 10 };
 
 typedef struct {
     int burnin, mcmc, adapt_freq;          /* BURNIN, MCMC, ADAPT_FREQ */
@@ -155,6 +155,26 @@ int or_gt_branch_len(const or_model *m, double piA, const or_sptree *sp, const i
                      int gnode, double delta, int indicator, const int *lens, double *loglik, double *logpz,
                      double *prior, uint64_t seed, int m_sweep, int slot, const char *const *rows, long start, int GG,
                      int gapchar, double gap_prop);
+
+/* GTree::Update_Tg(tosample = true): the history Tg [N * len] of sites g < len. */
+void or_gt_sample_history(const or_model *m, double piA, const or_sptree *sp, const int *gch, const int *gpar,
+                          const double *gh, int groot, const int *Z, double c_rate, double n_rate, uint64_t seed,
+                          int m_sweep, int len, const char *const *rows, long start, int GG, int gapchar,
+                          double gap_prop, signed char *Tg);
+/* One GTree::Sample_tree2 move; returns 1 accepted, 0 rejected. */
+int or_gt_sample_tree(const or_model *m, double piA, const or_sptree *sp, int *gch, int *gpar, double *gh,
+                      int *groot_io, const int *Z, double c_rate, double n_rate, double c2m, double n2m, int root_ingrp,
+                      int branch, int indicator, const int *lens, const signed char *Tg, double *loglik,
+                      double *logpz, uint64_t seed, int m_sweep, int slot, const char *const *rows, long start, int GG,
+                      int gapchar, double gap_prop);
+
+/* test hook: or_gt_sample_tree with the complete Hastings ratio (exact) and the candidates' scores */
+int or_test_sample_tree(const or_model *m, double piA, const or_sptree *sp, int *gch, int *gpar, double *gh,
+                        int *groot_io, const int *Z, double c_rate, double n_rate, double c2m, double n2m,
+                        int root_ingrp, int branch, int indicator, const int *lens, const signed char *Tg,
+                        double *loglik, double *logpz, uint64_t seed, int m_sweep, int slot, const char *const *rows,
+                        long start, int GG, int gapchar, double gap_prop, int exact, int *cand_node,
+                        double *cand_score, int *n_cand);
 
 /* test hook: one Z forward-backward draw on a bare tree */
 int or_test_update_Z(int N, const int *nodes, int n_nodes, const int *parent, const int *fixZ, const double *E,

@@ -214,3 +214,191 @@ def test_height_move_known_answer():
                                    np.zeros(2), np.zeros(2), p0, SEED, m, 5, align, 0, 1)
         assert code == int(np.log(r) < d), m
         assert gt.h[2] == pytest.approx(prop if code else h0, abs=1e-15)
+
+
+def _tree_chain(sp, gt, steps, exact, align=None, lens=(0, 0), Tg_fn=None, seed=SEED):
+    S, N = sp.S, len(gt.par)
+    model, no_align = _no_data(S)
+    align = no_align if align is None else align
+    Z = np.ones(N, np.int32)
+    ll, lz = np.zeros(2), np.zeros(2)
+    if lens[0] > 0:
+        site = O.gt_site_logliks(model, sp, gt, Z, 0.5, 2.0, align, 0, align.shape[1])
+        ll[:] = site[:lens[0]].sum(), site[:lens[1]].sum()
+    acc = 0
+    states = []
+    for m in range(steps):
+        b = [v for v in range(N) if v != gt.root][m % (N - 1)]
+        Tg = np.zeros((N, lens[0]), np.int8) if Tg_fn is None else Tg_fn(model, gt, m)
+        acc += O.gt_sample_tree(model, sp, gt, Z, 0.5, 2.0, 0.01, 0.5, b, 0, lens, Tg, ll, lz, seed, m, 0, align, 0,
+                                align.shape[1], exact=exact)
+        if lens[0] > 0:
+            site = O.gt_site_logliks(model, sp, gt, Z, 0.5, 2.0, align, 0, align.shape[1])
+            assert ll == pytest.approx([site[:lens[0]].sum(), site[:lens[1]].sum()], rel=1e-12)
+        states.append((gt.ch.copy(), gt.par.copy(), gt.h.copy()))
+    return acc / steps, states
+
+
+def _three_summary(states, tau2):
+    conc = inbr = 0
+    ages = []
+    for ch, par, h in states:
+        root = int(np.flatnonzero(par == 5)[0])
+        inner = 7 - root                               # the internal node that is not the root
+        pair = tuple(sorted(ch[2 * inner:2 * inner + 2]))
+        conc += pair == (0, 1)
+        inbr += pair == (0, 1) and h[inner] < tau2
+        ages.append(h[root] - max(h[inner], tau2))
+    n = len(states)
+    return conc / n, inbr / n, np.array(ages)
+
+
+def test_tree_move_with_complete_hastings_samples_the_prior():
+    """With no data and the sibling choice's probability in the Hastings
+    ratio, the topology move alone must reproduce the coalescent: the
+    concordance probability, the chance that 0 and 1 meet inside their
+    ancestral branch, and the root's exponential age."""
+    sp, tau1, tau2, th3, th4 = _three_species()
+    gt = _gene3((0, 1), 2, 0.07, 0.3)
+    acc, states = _tree_chain(sp, gt, 60000, exact=True)
+    assert 0.1 < acc < 0.6
+    conc, inbr, ages = _three_summary(states[1000:], tau2)
+    T = tau2 - tau1
+    assert conc == pytest.approx(1 - 2 / 3 * np.exp(-2 * T / th3), abs=0.02)
+    assert inbr == pytest.approx(1 - np.exp(-2 * T / th3), abs=0.02)
+    assert stats.kstest(ages[::20], "expon", args=(0, th4 / 2)).pvalue > 1e-3
+
+
+def test_tree_move_as_the_reference_writes_it_leaves_out_the_sibling_choice():
+    """The reference's Hastings ratio omits the 1/2 of choosing the new
+    sibling among a candidate's children, so its chain favours regrafting
+    above the gene root: it spends clearly less time in the concordant
+    topology than the coalescent says (the restatement keeps that)."""
+    sp, tau1, tau2, th3, th4 = _three_species()
+    gt = _gene3((0, 1), 2, 0.07, 0.3)
+    _, states = _tree_chain(sp, gt, 40000, exact=False)
+    conc, inbr, _ = _three_summary(states[1000:], tau2)
+    assert conc < 1 - 2 / 3 * np.exp(-2 * (tau2 - tau1) / th3) - 0.025
+
+
+def test_tree_move_keeps_a_valid_embedding():
+    """With data: every state is a binary gene tree over all leaves whose
+    nodes sit above the speciation of the species holding all their leaves
+    and above their children."""
+    rng = np.random.default_rng(5)
+    tree, h = synth.ultrametric_tree(6, 0.4, rng)
+    sp = _sp_from_synth(tree, h * 0.2, theta=0.4)
+    g = synth.msc_gene_tree(tree, h * 0.2, np.random.default_rng(2))
+    gt = _gene_from_synth(g, tree.N)
+    align = np.frombuffer(b"acgtacgtta" + b"acgtacgata" + b"acctacgtta" + b"tcgtacgtta" + b"acgaacgtta" + b"acgtacttta", np.uint8).reshape(6, 10).copy()
+    _, states = _tree_chain(sp, gt, 400, exact=False, align=align, lens=(10, 5),
+                            Tg_fn=lambda model, gt, m: O.gt_sample_history(model, sp, gt, np.ones(tree.N, np.int32),
+                                                                           0.5, 2.0, SEED, m, 10, align, 0, 10))
+    changed = 0
+    for ch, par, hh in states:
+        changed += not np.array_equal(ch, states[0][0])
+        root = int(np.flatnonzero(par == tree.N)[0])
+        seen, stack = [], [root]
+        while stack:
+            v = stack.pop()
+            seen.append(v)
+            if v >= tree.S:
+                a, b = ch[2 * v], ch[2 * v + 1]
+                assert par[a] == v and par[b] == v and hh[a] < hh[v] and hh[b] < hh[v]
+                stack += [a, b]
+        assert sorted(seen) == list(range(tree.N))
+        leaves = {}
+        for v in reversed(seen):
+            leaves[v] = {v} if v < tree.S else leaves[ch[2 * v]] | leaves[ch[2 * v + 1]]
+        for v in range(tree.S, tree.N):
+            x = min(leaves[v])
+            while not leaves[v] <= set(_clade(tree, x)):
+                x = tree.parent[x]
+            assert hh[v] > h[x] * 0.2 - 1e-12
+    assert changed > 0
+
+
+def _clade(tree, x):
+    if x < tree.S:
+        return [x]
+    return _clade(tree, tree.children[x][0]) + _clade(tree, tree.children[x][1])
+
+
+def test_candidate_scores_known_answer():
+    """Branch = leaf 0 of the concordant three-species tree, one site: the
+    candidates are the gene root (its sampled base 2) at distance
+    (H3 - 0) r0 + (H4 - H3) r3 + (h(root) - H4) r4 and the root candidate at
+    ... + (curN - h(root)) r4; their scores are log of row x of P(len) times
+    the leaf's indicator, and the pi-average of that message."""
+    import scipy.linalg
+    sp, tau1, tau2, th3, th4 = _three_species()
+    gt = _gene3((0, 1), 2, 0.07, 0.3)
+    model, _ = _no_data(3)
+    align = np.frombuffer(b"c" + b"a" + b"g", np.uint8).reshape(3, 1).copy()
+    Z = np.array([2, 1, 0, 1, 2], np.int32)
+    cr, nr = 0.4, 2.5
+    rate = {0: 1.0, 1: cr, 2: nr}
+    Tg = np.array([[1], [0], [2], [3], [2]], np.int8)     # leaves' codes; node 3 -> 3, root 4 -> 2
+    got = []
+    O.gt_sample_tree(model, sp, gt, Z, cr, nr, 0.01, 0.5, 0, 0, [1, 1], Tg, np.zeros(2), np.zeros(2), SEED, 0, 0,
+                     align, 0, 1, scores=got)
+    Q = np.array([model.Q[i] for i in range(16)]).reshape(4, 4)
+    pi = np.array([model.pi[i] for i in range(4)])
+    cur = tau1 * rate[Z[0]] + (tau2 - tau1) * rate[Z[3]]
+    curN = 10 * th4 / 2
+    l_root = cur + (0.3 - tau2) * rate[Z[4]]
+    l_N = cur + (curN - 0.3) * rate[Z[4]]
+    Lb = np.array([0, 1.0, 0, 0])                          # leaf 0 holds c
+    msg = lambda l: scipy.linalg.expm(Q * l) @ Lb
+    want = [(4, np.log(msg(l_root)[2])), (5, np.log(pi @ msg(l_N)))]
+    assert [c for c, _ in got] == [c for c, _ in want]
+    np.testing.assert_allclose([s for _, s in got], [s for _, s in want], rtol=1e-10)
+
+
+def test_history_draws_follow_the_posterior():
+    """Update_Tg's draws at one site of a four-leaf gene tree against the
+    exact posterior of the internal bases by enumeration (chi-square over
+    many sweeps' independent uniforms)."""
+    import scipy.linalg
+    rng = np.random.default_rng(3)
+    tree, h = synth.ultrametric_tree(4, 0.3, rng)
+    sp = _sp_from_synth(tree, h * 0.5, theta=0.6)
+    g = synth.msc_gene_tree(tree, h * 0.5, np.random.default_rng(8))
+    gt = _gene_from_synth(g, tree.N)
+    model, _ = _no_data(4)
+    align = np.frombuffer(b"acgg", np.uint8).reshape(4, 1).copy()
+    Z = np.array([0, 1, 2, 1, 2, 1, 1], np.int32)
+    cr, nr = 0.5, 3.0
+    Q = np.array([model.Q[i] for i in range(16)]).reshape(4, 4)
+    pi = np.array([model.pi[i] for i in range(4)])
+    ln = np.zeros(tree.N)
+    sg, ss, sl = O.gt_pieces(sp, gt)
+    for a, s_, l in zip(sg, ss, sl):
+        ln[a] += l * {0: 1.0, 1: cr, 2: nr}[int(Z[s_])]
+    P = {v: scipy.linalg.expm(Q * ln[v]) for v in range(tree.N)}
+    obs = {0: 0, 1: 1, 2: 2, 3: 2}
+    internal = list(range(tree.S, tree.N))
+    root = gt.root
+    import itertools
+    post = {}
+    for assign in itertools.product(range(4), repeat=len(internal)):
+        x = dict(obs)
+        x.update(zip(internal, assign))
+        p = pi[x[root]]
+        for v in range(tree.N):
+            if v != root:
+                p *= P[v][x[gt.par[v]], x[v]]
+        post[assign] = p
+    tot = sum(post.values())
+    counts = {}
+    n = 6000
+    for m in range(n):
+        T = O.gt_sample_history(model, sp, gt, Z, cr, nr, SEED, m, 1, align, 0, 1)
+        assert list(T[:4, 0]) == [0, 1, 2, 2]
+        key = tuple(int(T[v, 0]) for v in internal)
+        counts[key] = counts.get(key, 0) + 1
+    keys = [k for k in post if post[k] / tot * n >= 5]
+    f_obs = np.array([counts.get(k, 0) for k in keys], float)
+    f_exp = np.array([post[k] / tot * n for k in keys])
+    f_exp *= f_obs.sum() / f_exp.sum()
+    assert stats.chisquare(f_obs, f_exp).pvalue > 1e-3
