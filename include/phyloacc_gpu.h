@@ -273,6 +273,53 @@ pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *sp, const int32_
                            int32_t slot, int32_t precision, double *loglik, double *logpz, double *prior,
                            int32_t *accepted);
 
+/* GTree::Update_Tg with tosample = true (src/PhyloAcc-GT/genetree.cpp:
+ * 1416-1532) for n gene-tree elements in one launch: after the pruning pass
+ * at the gene trees' heights (gchildren, gparent, gheight as
+ * pa_gt_branch_len; Z, rates, piA too), sites g < len[i] get the base of
+ * every gene node in Tg[(i * N + v) * stride + g]: leaves their data (0-3
+ * a/c/g/t, -1 an ambiguity code, 4 no base), the gene root drawn from
+ * pi * L_root, every other internal gene node from P(parent's base -> .) * L
+ * over its whole branch (the law of the reference's steps through species
+ * boundaries); 4 for a missing gene or a subtree with no base at the site.
+ * Draws: tag GT_HIST, sub-stream g, sweep m, slot = node id. */
+pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *sp, const int32_t *ids, int32_t n,
+                               const int32_t *gchildren, const int32_t *gparent, const double *gheight,
+                               const int32_t *Z, const double *rates, const double *piA, const int32_t *len,
+                               int32_t m, int32_t precision, int8_t *Tg, int32_t stride);
+
+/* One GTree::Sample_tree2 move (src/PhyloAcc-GT/genetree.cpp:2138-2518, with
+ * Remove_branch :737-808 and Graft :810-923) for each of n gene-tree elements
+ * (distinct ids), the unconstrained candidate set (no deep-coalescence
+ * species, the default branch-sampling threshold).  Gene lineage branch[i]
+ * is pruned with its parent bp; walking up the species from branch's, every
+ * gene node born above branch is a candidate new grandparent at substitution
+ * distance (rates of the species crossed), with a candidate above the gene
+ * root at 10 theta_root / 2 (or h(root) + 5 theta_root / 2) in the species
+ * root; the walk stops after species root_ingrp once a candidate exists.
+ * Each candidate is scored over the sites g < lens[2i + indicator[i]] with
+ * the history Tg (as pa_gt_sample_history writes it, same stride) and
+ * branch's partial, all in one launch; a candidate is drawn with probability
+ * proportional to exp(score), then the new sibling (one of its two
+ * children: u < 1/2 the first) and bp's height uniformly from where both
+ * lineages first share a species up to the candidate.  A second launch
+ * scores the regrafted trees; Metropolis-Hastings on the prefix likelihood,
+ * the missing-gene term (when logpz[2i + indicator[i]] != 0), the
+ * coalescent prior, the two uniform heights' densities and the candidate
+ * probabilities of the new and the old grandparent (the reference's ratio,
+ * which leaves out the 1/2 of the sibling choice).  An old grandparent that
+ * is no candidate rejects the move.  On acceptance gchildren, gparent and
+ * gheight take the new tree, which also becomes the element's stored tree,
+ * and loglik / logpz update as in pa_gt_branch_len.  Draws: tag GT_TREE,
+ * sub-stream slot, sweep m: candidate 0, sibling 1, height 2 (floored at
+ * 1e-5), decision 3. */
+pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *sp, const int32_t *ids, int32_t n,
+                            int32_t *gchildren, int32_t *gparent, double *gheight, const int32_t *Z,
+                            const double *rates, const double *piA, const int32_t *branch, const int32_t *indicator,
+                            const int32_t *lens, const int8_t *Tg, int32_t stride, double cons_to_mis,
+                            double ncons_to_mis, int32_t root_ingrp, int32_t m, int32_t slot, int32_t precision,
+                            double *loglik, double *logpz, int32_t *accepted);
+
 /* Per-element log-likelihood under the gBGC substitution model, on the device.
  * Replaces the pruning of PhyloAcc-ST-GBGC over log_cache_TM_{null,cons,neut}
  * and their _B twins (src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:1563-1621), whose

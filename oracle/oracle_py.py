@@ -105,7 +105,7 @@ class GeneTree:
     leaves), parent [N] (N at the root), heights [N], missing [N]."""
 
     def __init__(self, children, parent, heights, missing=None):
-        self.ch = np.ascontiguousarray(np.asarray(children, np.int32).reshape(-1))
+        self.ch = np.array(children, np.int32).reshape(-1)
         self.par = np.ascontiguousarray(parent, np.int32).copy()
         self.h = np.ascontiguousarray(heights, np.float64).copy()
         N = len(self.par)

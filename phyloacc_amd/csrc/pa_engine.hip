@@ -27,6 +27,7 @@
 #include "pa_kernels.h"
 #include "pa_device.h"
 #include "pa_eig.h"
+#include "pa_rng.h"
 
 #include <mutex>
 
@@ -73,6 +74,8 @@ struct Ctx {
     LDS R *PT;             // [N][16], PT[i*4+j] = P_{j->i}
     double red[16];
     double pi[4];          // base frequencies of the root sum (the model's, or the element's own)
+    double Pc[16];         // a candidate's transition matrix, Pc[a*4+b] = P_{a->b} (gene-tree moves)
+    const GBL int32_t *site;   // the element's site -> pattern map
     int work;
 };
 
@@ -411,6 +414,164 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
     return block_sum(acc, c->red);
 }
 
+// ------------------------------------------- gene-tree moves (PhyloAcc-GT)
+
+// Node v's scaled partial and exponent at pattern p (GAPT: no base below);
+// a cherry's is recomputed from its leaf codes, as the pruning pass does.
+template <typename R>
+__device__ inline ChildIn<R> node_in(CP<R> c, int v, int p)
+{
+    const int S = c->S, Ppad = c->Ppad;
+    if (c->flags[v] & NF_CHERRY) {
+        const int yA = c->ch0[v], yB = c->ch1[v];
+        return cherry_in<R>(c->PT + 16 * yA, c->PT + 16 * yB, c->flags[yA] & NF_MISSING, c->flags[yB] & NF_MISSING,
+                            c->flags[v] & NF_MISSING, c->codes[(size_t)yA * Ppad + p], c->codes[(size_t)yB * Ppad + p]);
+    }
+    ChildIn<R> in;
+    in.x = c->exps[(size_t)(v - S) * Ppad + p];
+    in.v = ld4<R>(c->parts + (size_t)(v - S) * Ppad + p);
+    return in;
+}
+
+// A leaf code as a history base: 0-3, -1 for an ambiguity code, 4 for a gap
+// or another character (src/PhyloAcc-GT/bpp_c.hpp:340-384).
+__device__ inline int leaf_base(int code)
+{
+    return code < 4 ? code : (code == PA_GAP_CODE || code == PA_OTHER_CODE ? 4 : -1);
+}
+
+// GTree::Update_Tg with tosample = true (src/PhyloAcc-GT/genetree.cpp:
+// 1416-1532) after the pruning pass: one thread per site draws the gene
+// root's base from pi * L_root, then every internal gene node's from
+// P(parent base -> .) * L over its whole branch (the law of the reference's
+// species-by-species steps), pre-order, and writes every node's base.
+template <typename R, int BS>
+__device__ __noinline__ void gt_history(MP M, CP<R> c, AP A, int row)
+{
+    const int N = c->N, S = c->S, Ppad = c->Ppad, root = c->root, n_nodes = c->n_nodes;
+    const int len = A->hist_len[row], stride = A->hist_stride;
+    GBL int8_t *out = (GBL int8_t *)A->hist + (size_t)row * N * stride;
+    const GBL int32_t *site = c->site;
+    const Key key{M->key0, M->key1};
+    const uint32_t c2 = (uint32_t)A->hist_sweep, c3 = tagword(0, TAG_GT_HIST);
+    for (int g = threadIdx.x; g < len; g += BS) {
+        const int p = site[g];
+        for (int v = 0; v < S; v++) out[(size_t)v * stride + g] = (int8_t)leaf_base(c->codes[(size_t)v * Ppad + p]);
+        {
+            const ChildIn<R> in = node_in<R>(c, root, p);
+            const bool none = in.x == GAPT || (c->flags[root] & NF_MISSING);
+            double w[4] = {c->pi[0], c->pi[1], c->pi[2], c->pi[3]};
+            if (!none) { w[0] *= (double)in.v.x; w[1] *= (double)in.v.y; w[2] *= (double)in.v.z; w[3] *= (double)in.v.w; }
+            out[(size_t)root * stride + g] = (int8_t)categorical<4>(w, slot_uniform(key, (uint32_t)g, c2, c3, (uint32_t)root));
+        }
+        for (int q = n_nodes - 1; q >= 0; q--) {
+            const int u = c->nodes[q];
+            if (u < S) continue;
+            const int xu = out[(size_t)u * stride + g];
+            for (int k = 0; k < 2; k++) {
+                const int v = k ? c->ch1[u] : c->ch0[u];
+                if (v < S) continue;
+                int res = 4;
+                if (!(c->flags[v] & NF_MISSING) && xu >= 0 && xu < 4) {
+                    const ChildIn<R> in = node_in<R>(c, v, p);
+                    if (in.x != GAPT) {
+                        const LDS R *PT = c->PT + 16 * v;      // PT[i*4+j] = P_{j->i}
+                        double w[4] = {(double)PT[0 * 4 + xu] * (double)in.v.x, (double)PT[1 * 4 + xu] * (double)in.v.y,
+                                       (double)PT[2 * 4 + xu] * (double)in.v.z, (double)PT[3 * 4 + xu] * (double)in.v.w};
+                        res = categorical<4>(w, slot_uniform(key, (uint32_t)g, c2, c3, (uint32_t)v));
+                    }
+                }
+                out[(size_t)v * stride + g] = (int8_t)res;
+            }
+        }
+    }
+}
+
+// P(d) = V e^{lam d} Vi in FP64 into Pc[a*4+b] = P_{a->b}, clamped at 0; the
+// identity below 1e-8 (BPP::getlogTM_len, src/PhyloAcc-GT/bpp.cpp:238-252).
+template <typename PE>
+__device__ inline void full_transition(PE lam, PE V, PE Vi, double d, LDS double *Pc)
+{
+    for (int a = 0; a < 4; a++)
+        for (int j = 0; j < 4; j++) {
+            double v = 0;
+            for (int k = 0; k < 4; k++) v += V[a * 4 + k] * exp(lam[k] * d) * Vi[k * 4 + j];
+            Pc[a * 4 + j] = d < 1e-8 ? (a == j ? 1.0 : 0.0) : (v < 0 ? 0.0 : v);
+        }
+}
+
+// Candidate scores of GTree::Sample_tree2 (src/PhyloAcc-GT/genetree.cpp:
+// 2190-2380) after the pruning pass of the current tree: for each candidate
+// new grandparent of branch b at substitution distance d, the sum over sites
+// where b holds a base of: kind 0, log (P(d) L_b)[x_t] at the candidate's
+// sampled base x_t (the pi-average of P(d) L_b when it has none); kind 1, the
+// pi-average; kind 2, log sum_i pi_i P(d)_{x_b -> i} at b's own base.
+template <typename R, int BS>
+__device__ __noinline__ void gt_candidates(MP M, CP<R> c, AP A, int w, int row)
+{
+    const int N = c->N, S = c->S, Ppad = c->Ppad, stride = A->hist_stride, cm = A->cand_max;
+    const GBL int8_t *tg = (const GBL int8_t *)A->cand_tg + (size_t)row * N * stride;
+    const int b = A->cand_branch[row], nc = A->cand_n[row], len = A->cand_sites[row];
+    const GBL int32_t *site = c->site;
+    const double *E = A->el_eig ? A->el_eig + (size_t)w * GB_EIG_WORDS : nullptr;
+    const double lp[4] = {log(c->pi[0]), log(c->pi[1]), log(c->pi[2]), log(c->pi[3])};
+    for (int q = 0; q < nc; q++) {
+        const int t = A->cand_node[(size_t)row * cm + q], kind = A->cand_kind[(size_t)row * cm + q];
+        if (threadIdx.x == 0) {
+            const double d = A->cand_len[(size_t)row * cm + q];
+            if (E) full_transition(E, E + 4, E + 20, d, c->Pc);
+            else full_transition(M->lam, M->V, M->Vi, d, c->Pc);
+        }
+        __syncthreads();
+        double acc = 0;
+        for (int g = threadIdx.x; g < len; g += BS) {
+            const int xb = tg[(size_t)b * stride + g];
+            if (xb >= 4) continue;
+            if (kind == 2) {
+                if (xb < 0) continue;
+                double sum = 0;
+                for (int i = 0; i < 4; i++) sum += c->pi[i] * c->Pc[xb * 4 + i];
+                acc += log(sum);
+                continue;
+            }
+            const int p = site[g];
+            double L[4];
+            int e = 0;
+            if (b < S) {
+                const int code = c->codes[(size_t)b * Ppad + p];
+                for (int j = 0; j < 4; j++) L[j] = 0;
+                if (code < 4) L[code] = 1;
+                else {
+                    const int b1 = (code == 6 || code == 9 || code == 11) ? 0 : (code == 7 || code == 10) ? 1 : 2;
+                    const int b2 = (code == 6 || code == 10) ? 2 : (code == 9) ? 1 : 3;
+                    L[b1] = 1; L[b2] = 1;
+                }
+            } else {
+                const ChildIn<R> in = node_in<R>(c, b, p);
+                L[0] = in.v.x; L[1] = in.v.y; L[2] = in.v.z; L[3] = in.v.w;
+                e = in.x;
+            }
+            double Mv[4];
+            for (int a = 0; a < 4; a++)
+                Mv[a] = log(c->Pc[a * 4 + 0] * L[0] + c->Pc[a * 4 + 1] * L[1] + c->Pc[a * 4 + 2] * L[2] +
+                            c->Pc[a * 4 + 3] * L[3]) + (double)e * 0.69314718055994530942;
+            const int xt = kind == 0 ? tg[(size_t)t * stride + g] : 4;
+            if (kind == 0 && !(c->flags[t] & NF_MISSING) && xt >= 0 && xt < 4) {
+                acc += Mv[xt];
+            } else {
+                double mx = -INFINITY;
+                for (int a = 0; a < 4; a++) mx = fmax(mx, Mv[a] + lp[a]);
+                double sum = 0;
+                for (int a = 0; a < 4; a++) sum += exp(Mv[a] + lp[a] - mx);
+                acc += mx == -INFINITY ? -INFINITY : log(sum) + mx;
+            }
+        }
+        const double tot = block_sum(acc, c->red);
+        if (threadIdx.x == 0) A->cand_score[(size_t)row * cm + q] = tot;
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------- element setup
 
 // Binds work position w: the element's node table into LDS, its slot, and
@@ -422,12 +583,16 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
 {
     const int tid = threadIdx.x, N = M->N;
     const ElemDev E = A->elems[A->work[w]];
+    const int row = A->len_row ? A->len_row[w] : w;
+    const int32_t *hdr = A->hdr_call ? A->hdr_call + (size_t)row * 4 : nullptr;
     if (tid == 0) {
         c->S = M->S; c->N = N;
-        c->P = E.P; c->Ppad = E.Ppad; c->root = E.root;
+        c->P = E.P; c->Ppad = E.Ppad; c->root = hdr ? hdr[0] : E.root;
+        c->site = A->site ? (const GBL int32_t *)(A->site + E.off_site) : nullptr;
         const double *pw = A->el_pi ? A->el_pi + (size_t)w * 4 : nullptr;
         for (int j = 0; j < 4; j++) c->pi[j] = pw ? pw[j] : M->pi[j];
-        c->n_nodes = E.n_nodes; c->n_up = E.n_up;
+        c->n_nodes = hdr ? hdr[1] : E.n_nodes;
+        c->n_up = hdr ? hdr[2] : E.n_up;
         c->codes = (const GBL uint8_t *)(A->codes + E.off_codes);
         if (A->wpre) {
             c->wts = (const GBL int32_t *)(A->wpre + (size_t)w * 2 * A->wpre_stride);
@@ -441,7 +606,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
         c->parts = (GBL V4<R> *)base;
         c->exps = (GBL int32_t *)(base + slot_exps_off(N, M->S, E.Ppad, sizeof(R) == 8));
     }
-    const int32_t *tab = A->tab + E.off_tab;
+    const int32_t *tab = A->tab_call ? A->tab_call + (size_t)row * tab_words(N) : A->tab + E.off_tab;
     for (int i = tid; i < N; i += BS) {
         c->ch0[i] = (int16_t)tab[tab_ch0(N) + i];
         c->ch1[i] = (int16_t)tab[tab_ch1(N) + i];
@@ -527,6 +692,11 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
         bind_element<R, BS>(M, c, A, w);
         unsigned long long bytes = 0;
         const double l = prune<R, BS>(M, c, A->counters && threadIdx.x == 0 ? &bytes : nullptr);
+        if (A->hist || A->cand_score) {
+            const int row = A->len_row ? A->len_row[w] : w;
+            if (A->hist) gt_history<R, BS>(M, c, A, row);
+            if (A->cand_score) gt_candidates<R, BS>(M, c, A, w, row);
+        }
         if (threadIdx.x == 0) {
             A->ll_out[w] = l;
             if (A->wpre) A->ll_out2[w] = c->sum1;

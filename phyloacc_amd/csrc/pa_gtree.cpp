@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/phyloacc_gpu.h"
+#include "pa_common.h"
 #include "pa_internal.h"
 #include "pa_rng.h"
 
@@ -360,6 +361,336 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
             else logpz[2 * i] = logpz[2 * i + 1] = lz;
             prior[i] = pr;
         }
+    }
+    return PA_OK;
+}
+
+namespace {
+
+// Inverse-CDF draw over k non-negative weights: the first i with u * total <
+// p[0] + ... + p[i], or the last positive-weight index when rounding puts
+// the target past the end.
+int categorical_n(const double *p, int k, double u)
+{
+    double tot = 0;
+    int last = 0;
+    for (int i = 0; i < k; i++) {
+        tot += p[i];
+        if (p[i] > 0) last = i;
+    }
+    const double target = u * tot;
+    double cum = 0;
+    for (int i = 0; i < k - 1; i++) {
+        cum += p[i];
+        if (target < cum) return i;
+    }
+    return last;
+}
+
+// Lineage v of a tree whose node species are `where`: from where[v] up to the
+// species of its parent gene node (pn), or to the species root when pn is N.
+bool lineage_crosses(const Sp &s, const std::vector<int> &where, int v, int pn, int x)
+{
+    const int top = pn < s.N ? where[pn] : s.N - 1;
+    for (int y = where[v];; y = s.up[y]) {
+        if (y == x) return true;
+        if (y == top || s.up[y] >= s.N) return false;
+    }
+}
+
+struct TreeMove {
+    // candidates: node (N for above the gene root), kind (0 node, 1 root pi-average, 2 root first option,
+    // 3 the first candidate when it is the root: score 0), substitution distance
+    std::vector<int> cnode, ckind, csp;
+    std::vector<double> clen;
+    int bp = -1, cb = 0, bs = -1, bpp = -1, ss = -1;
+    double curG = 0, curN = 0, cur_unif = 0;
+    std::vector<int32_t> ch2, par2;        // the pruned tree
+    int root2 = -1;
+};
+
+// GTree::Sample_tree2's removal and candidate walk (src/PhyloAcc-GT/
+// genetree.cpp:2147-2380, Remove_branch :737-808), unconstrained candidates.
+void tree_candidates(const Embed &g, int branch, int root_ingrp, const int32_t *Z, double r1, double r2, TreeMove &t)
+{
+    const Sp &s = *g.sp;
+    const int N = s.N;
+    t.curG = prior_of(g);
+    t.ss = g.where[branch];
+    t.bp = g.par[branch];
+    t.cb = g.ch[2 * t.bp] == branch ? 0 : 1;
+    t.bs = g.ch[2 * t.bp + 1 - t.cb];
+    t.bpp = g.par[t.bp];
+    const double thr = s.theta[N - 1];
+    t.curN = 10 * thr / 2;
+    if (t.curN < g.h[g.root]) t.curN = g.h[g.root] + 5 * thr / 2;
+    double lowc = 0;
+    for (int x = t.ss; x < N; x = s.up[x])
+        if (g.crosses(t.bs, x)) { lowc = std::max(s.H[x], std::max(g.h[branch], g.h[t.bs])); break; }
+    t.cur_unif = -std::log((t.bpp < N ? g.h[t.bpp] : t.curN) - lowc);
+    // Remove_branch: the sibling takes bp's place
+    t.ch2.assign(g.ch, g.ch + 2 * N);
+    t.par2.assign(g.par, g.par + N);
+    t.par2[t.bs] = t.bpp;
+    t.root2 = g.root;
+    if (t.bpp < N) t.ch2[2 * t.bpp + (g.ch[2 * t.bpp] == t.bp ? 0 : 1)] = t.bs;
+    else t.root2 = t.bs;
+    t.par2[t.bp] = -1;
+    // gene nodes born in each species, by height (bp is gone; branch's own
+    // subtree stays, below branch, and is passed over)
+    std::vector<std::vector<int>> born(N);
+    for (int v : g.order)
+        if (g.internal(v) && v != t.bp) born[g.where[v]].push_back(v);
+    for (auto &b : born) std::sort(b.begin(), b.end(), [&](int a, int c) { return g.h[a] < g.h[c]; });
+    auto rate = [&](int x) { return Z[x] == 2 ? r2 : (Z[x] == 1 ? r1 : 1.0); };
+    auto push = [&](int node, int kind, int x, double len) {
+        t.cnode.push_back(node); t.ckind.push_back(kind); t.csp.push_back(x); t.clen.push_back(len);
+    };
+    double cur = 0;
+    for (int x = t.ss; x < N;) {
+        const double base = std::max(s.H[x], g.h[branch]);
+        const int up = s.up[x];
+        if (born[x].empty()) {
+            if (up == N) {
+                push(N, t.cnode.empty() ? 3 : 2, x, rate(x) * (t.curN - base) + cur);
+                break;
+            }
+            cur += (s.H[up] - base) * rate(x);
+            x = up;
+            continue;
+        }
+        for (int v : born[x]) {
+            if (x == t.ss && g.h[v] <= g.h[branch]) continue;
+            push(v, 0, x, rate(x) * (g.h[v] - base) + cur);
+        }
+        if (up == N) {
+            const double ht = t.cnode.empty() ? base : g.h[t.cnode.back()];
+            push(N, 1, x, cur + rate(x) * (t.curN - ht));
+        }
+        if (up < N) cur += (s.H[up] - base) * rate(x);
+        if (x == root_ingrp && !t.cnode.empty()) break;
+        x = up;
+    }
+}
+
+}  // namespace
+
+// Batched GTree::Update_Tg with tosample = true.
+extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t, const int32_t *ids, int32_t n,
+                                          const int32_t *gchildren, const int32_t *gparent, const double *gheight,
+                                          const int32_t *Z, const double *rates, const double *piA,
+                                          const int32_t *len, int32_t m, int32_t precision, int8_t *Tg,
+                                          int32_t stride)
+{
+    if (!e) return PA_E_INPUT;
+    if (n < 0 || (n > 0 && (!ids || !gchildren || !gparent || !gheight || !Z || !rates || !len || !Tg)))
+        return engine_fail(e, PA_E_INPUT, "null argument");
+    if (stride <= 0) return engine_fail(e, PA_E_INPUT, "stride must be positive");
+    Sp s;
+    pa_status st = check_sp(e, t, s);
+    if (st != PA_OK) return st;
+    EngineView ev;
+    engine_view(e, &ev);
+    if (s.S != ev.S || s.N != ev.N) return engine_fail(e, PA_E_INPUT, "species tree size differs from the model's");
+    if (n == 0) return PA_OK;
+    const int N = s.N;
+    std::vector<double> lens((size_t)n * N);
+    for (int i = 0; i < n; i++) {
+        uint8_t lm[512];
+        if ((st = element_gene_tree(e, ids[i], lm, gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
+        Embed g;
+        if (!g.build(s, gchildren + (size_t)i * 2 * N, gparent + (size_t)i * N, gheight + (size_t)i * N))
+            return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
+        lengths_of(g, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &lens[(size_t)i * N]);
+    }
+    GtExtra x;
+    x.hist = Tg;
+    x.hist_len = len;
+    x.hist_stride = stride;
+    x.hist_sweep = m;
+    std::vector<double> o(n), o1(n);
+    return gt_launch(e, ids, n, lens.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x);
+}
+
+// Batched GTree::Sample_tree2 (src/PhyloAcc-GT/genetree.cpp:2138-2518).
+extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, const int32_t *ids, int32_t n,
+                                       int32_t *gchildren, int32_t *gparent, double *gheight, const int32_t *Z,
+                                       const double *rates, const double *piA, const int32_t *branch,
+                                       const int32_t *indicator, const int32_t *lens, const int8_t *Tg,
+                                       int32_t stride, double cons_to_mis, double ncons_to_mis, int32_t root_ingrp,
+                                       int32_t m, int32_t slot, int32_t precision, double *loglik, double *logpz,
+                                       int32_t *accepted)
+{
+    if (!e) return PA_E_INPUT;
+    if (n < 0 || (n > 0 && (!ids || !gchildren || !gparent || !gheight || !Z || !rates || !branch || !indicator ||
+                            !lens || !Tg || !loglik || !logpz || !accepted)))
+        return engine_fail(e, PA_E_INPUT, "null argument");
+    Sp s;
+    pa_status st = check_sp(e, t, s);
+    if (st != PA_OK) return st;
+    EngineView ev;
+    engine_view(e, &ev);
+    if (s.S != ev.S || s.N != ev.N) return engine_fail(e, PA_E_INPUT, "species tree size differs from the model's");
+    if (root_ingrp < s.S || root_ingrp >= s.N) return engine_fail(e, PA_E_INPUT, "root_ingrp must be an internal species");
+    if (n == 0) return PA_OK;
+    {
+        std::vector<int32_t> u(ids, ids + n);
+        std::sort(u.begin(), u.end());
+        if (std::adjacent_find(u.begin(), u.end()) != u.end())
+            return engine_fail(e, PA_E_INPUT, "an element appears twice in one tree move");
+    }
+    const int N = s.N;
+    const Key key{ev.key0, ev.key1};
+    const uint32_t c3 = tagword(0, TAG_GT_TREE), c2 = (uint32_t)m;
+    std::vector<Embed> emb(n);
+    std::vector<TreeMove> mv(n);
+    std::vector<std::vector<uint8_t>> miss0(n);
+    int cmax = 1;
+    std::vector<double> cur_len((size_t)n * N);
+    for (int i = 0; i < n; i++) {
+        int32_t *ch = gchildren + (size_t)i * 2 * N, *pa = gparent + (size_t)i * N;
+        uint8_t lm[512];
+        if ((st = element_gene_tree(e, ids[i], lm, ch)) != PA_OK) return st;
+        if (!emb[i].build(s, ch, pa, gheight + (size_t)i * N))
+            return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
+        const int b = branch[i], ind = indicator[i];
+        if (b < 0 || b >= N || b == emb[i].root || emb[i].where[b] < 0)
+            return engine_fail(e, PA_E_INPUT, "branch must be a gene node other than the root");
+        if (ind != 0 && ind != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
+        if (lens[2 * i + ind] < 0 || lens[2 * i + 1] < 0 || lens[2 * i + ind] > stride)
+            return engine_fail(e, PA_E_INPUT, "prefix length negative or beyond the history's stride");
+        missing_of(emb[i], lm, miss0[i]);
+        tree_candidates(emb[i], b, root_ingrp, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], mv[i]);
+        cmax = std::max(cmax, (int)mv[i].cnode.size());
+        lengths_of(emb[i], Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &cur_len[(size_t)i * N]);
+    }
+    // launch 1: the current trees' partials score every candidate
+    std::vector<int32_t> cb(n), cn(n), cnode((size_t)n * cmax, 0), ckind((size_t)n * cmax, 1), csites(n);
+    std::vector<double> clen((size_t)n * cmax, 0.0), cscore((size_t)n * cmax, 0.0);
+    for (int i = 0; i < n; i++) {
+        cb[i] = branch[i];
+        cn[i] = (int)mv[i].cnode.size();
+        csites[i] = lens[2 * i + indicator[i]];
+        for (int q = 0; q < cn[i]; q++) {
+            cnode[(size_t)i * cmax + q] = mv[i].cnode[q] < N ? mv[i].cnode[q] : 0;
+            ckind[(size_t)i * cmax + q] = mv[i].ckind[q] == 3 ? 1 : mv[i].ckind[q];
+            clen[(size_t)i * cmax + q] = mv[i].clen[q];
+        }
+    }
+    {
+        GtExtra x;
+        x.hist_stride = stride;
+        x.cand_tg = Tg;
+        x.cand_branch = cb.data(); x.cand_n = cn.data(); x.cand_node = cnode.data(); x.cand_kind = ckind.data();
+        x.cand_sites = csites.data(); x.cand_len = clen.data(); x.cand_max = cmax; x.cand_score = cscore.data();
+        std::vector<double> o(n), o1(n);
+        if ((st = gt_launch(e, ids, n, cur_len.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x)) != PA_OK)
+            return st;
+    }
+    // draw each element's regraft; build the proposed trees
+    std::vector<std::vector<int32_t>> nch(n), npar(n);
+    std::vector<std::vector<double>> nh(n), sprob(n);
+    std::vector<int> get(n), pos(n);
+    std::vector<double> new_unif(n), rdraw(n);
+    std::vector<int32_t> tabs((size_t)n * tab_words(N)), hdrs((size_t)n * 4), l0(n), l1(n);
+    std::vector<double> nlen((size_t)n * N), rpi;
+    for (int i = 0; i < n; i++) {
+        TreeMove &t = mv[i];
+        const Embed &g = emb[i];
+        const int nc = (int)t.cnode.size();
+        std::vector<double> &p = sprob[i];
+        p.assign(nc, 1.0);
+        get[i] = 0;
+        if (nc > 1) {
+            double mx = -INFINITY, tot = 0;
+            std::vector<double> sc(nc);
+            for (int q = 0; q < nc; q++) {
+                sc[q] = t.ckind[q] == 3 ? 0.0 : cscore[(size_t)i * cmax + q];
+                mx = std::max(mx, sc[q]);
+            }
+            for (int q = 0; q < nc; q++) { p[q] = std::exp(sc[q] - mx); tot += p[q]; }
+            for (int q = 0; q < nc; q++) p[q] /= tot;
+            get[i] = categorical_n(p.data(), nc, slot_uniform(key, (uint32_t)slot, c2, c3, 0));
+        }
+        pos[i] = -1;
+        for (int q = 0; q < nc; q++) if (t.cnode[q] == (t.bpp < N ? t.bpp : N)) { pos[i] = q; break; }
+        const int nb = t.cnode[get[i]];
+        const double thr = s.theta[N - 1];
+        int nsib;
+        double upp;
+        if (nb == N) {
+            nsib = t.root2;
+            upp = 10 * thr / 2 > g.h[nsib] ? 10 * thr / 2 : g.h[nsib] + 5 * thr / 2;
+        } else {
+            nsib = t.ch2[2 * nb + (slot_uniform(key, (uint32_t)slot, c2, c3, 1) < 0.5 ? 0 : 1)];
+            upp = g.h[nb];
+        }
+        double low = 0;
+        const int stop = s.up[t.csp[get[i]]];
+        for (int x = t.ss; x != stop; x = s.up[x])
+            if (lineage_crosses(s, g.where, nsib, t.par2[nsib], x)) {
+                low = std::max(g.h[branch[i]], std::max(g.h[nsib], s.H[x]));
+                break;
+            }
+        const double u = std::max(slot_uniform(key, (uint32_t)slot, c2, c3, 2), 1e-5);
+        new_unif[i] = -std::log(upp - low);
+        rdraw[i] = slot_uniform(key, (uint32_t)slot, c2, c3, 3);
+        // Graft: bp above nsib, branch keeps its side of bp
+        nch[i] = t.ch2;
+        npar[i] = t.par2;
+        nh[i].assign(g.h, g.h + N);
+        const int bp = t.bp, it2p = npar[i][nsib];
+        nh[i][bp] = u * (upp - low) + low;
+        npar[i][bp] = it2p;
+        npar[i][nsib] = bp;
+        nch[i][2 * bp + t.cb] = branch[i];
+        nch[i][2 * bp + 1 - t.cb] = nsib;
+        if (it2p < N) nch[i][2 * it2p + (t.ch2[2 * it2p] == nsib ? 0 : 1)] = bp;
+        if ((st = gene_table(e, ids[i], nch[i].data(), &tabs[(size_t)i * tab_words(N)], &hdrs[(size_t)i * 4])) != PA_OK)
+            return st;
+        Embed g2;
+        if (!g2.build(s, nch[i].data(), npar[i].data(), nh[i].data()))
+            return engine_fail(e, PA_E_INPUT, "regrafted gene tree is not valid");
+        lengths_of(g2, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &nlen[(size_t)i * N]);
+        const int ind = indicator[i];
+        l0[i] = lens[2 * i + ind];
+        l1[i] = std::min(lens[2 * i + 1], lens[2 * i + ind]);
+    }
+    // launch 2: the proposed trees' prefix likelihoods
+    std::vector<double> o0(n), o1(n);
+    {
+        GtExtra x;
+        x.tabs = tabs.data();
+        x.hdrs = hdrs.data();
+        if ((st = gt_launch(e, ids, n, nlen.data(), piA, l0.data(), l1.data(), precision, o0.data(), o1.data(), x)) != PA_OK)
+            return st;
+    }
+    for (int i = 0; i < n; i++) {
+        accepted[i] = 0;
+        if (pos[i] < 0) continue;
+        Embed g2;
+        g2.build(s, nch[i].data(), npar[i].data(), nh[i].data());
+        uint8_t lm[512];
+        if ((st = element_gene_tree(e, ids[i], lm, nullptr)) != PA_OK) return st;
+        std::vector<uint8_t> miss;
+        missing_of(g2, lm, miss);
+        const double lz = logpz_of(g2, miss, Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
+        const int ind = indicator[i];
+        const double lnew[2] = {o0[i], o1[i]};
+        double d = lnew[ind] - loglik[2 * i + ind];
+        if (logpz[2 * i + ind] != 0) d += lz - logpz[2 * i + ind];
+        d += prior_of(g2) - mv[i].curG + mv[i].cur_unif - new_unif[i] + std::log(sprob[i][pos[i]]) -
+             std::log(sprob[i][get[i]]);
+        if (!(std::log(rdraw[i]) < d)) continue;
+        accepted[i] = 1;
+        memcpy(gchildren + (size_t)i * 2 * N, nch[i].data(), sizeof(int32_t) * 2 * N);
+        memcpy(gparent + (size_t)i * N, npar[i].data(), sizeof(int32_t) * N);
+        memcpy(gheight + (size_t)i * N, nh[i].data(), sizeof(double) * N);
+        loglik[2 * i] = lnew[0];
+        loglik[2 * i + 1] = lnew[1];
+        if (logpz[2 * i + 1] == 0) logpz[2 * i] = lz;
+        else logpz[2 * i] = logpz[2 * i + 1] = lz;
+        if ((st = element_set_tree(e, ids[i], nch[i].data())) != PA_OK) return st;
     }
     return PA_OK;
 }

@@ -272,6 +272,73 @@ struct Collapse {
     }
 };
 
+// Node table of one element's tree (the layout of pa_common.h): children
+// and parents (the root's parent N), flags, the post-order of the nodes
+// under root (BPP_C::getSubtree, src/PhyloAcc-ST/bpp_c.cpp:44-60), the
+// internal nodes by height level, the non-root nodes by depth level, the
+// Z-draw slots and the branch lengths.  own: a gene tree (no outgroup flags).
+void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std::vector<int> &par2,
+                     const std::vector<double> &dist2, const std::vector<uint8_t> &missing, bool own, int root,
+                     HostElem &E)
+{
+    const int S = tm.S, N = tm.N;
+    // post-order of the element's tree (BPP_C::getSubtree, bpp_c.cpp:44-60)
+    std::vector<int> nodes;
+    {
+        std::vector<std::pair<int, int>> st{{root, 0}};
+        while (!st.empty()) {
+            auto &[v, k] = st.back();
+            if (v >= S && k < 2) { int x = ch2[2 * v + k]; k++; if (x != -1) st.push_back({x, 0}); continue; }
+            nodes.push_back(v);
+            st.pop_back();
+        }
+    }
+    std::vector<int> internal;
+    for (int v : nodes) if (v >= S) internal.push_back(v);
+    // levels for the Z forward-backward
+    std::vector<int> height(N, 0), depth(N, 0);
+    int hmax = 0, dmax = 0;
+    for (int v : nodes)
+        if (v >= S) { height[v] = 1 + std::max(height[ch2[2 * v]], height[ch2[2 * v + 1]]); hmax = std::max(hmax, height[v]); }
+    for (int k = (int)nodes.size() - 1; k >= 0; k--) {
+        int v = nodes[k];
+        depth[v] = v == root ? 0 : depth[par2[v]] + 1;
+        dmax = std::max(dmax, depth[v]);
+    }
+    E.tab.assign(tab_words(N), 0);
+    int32_t *T = E.tab.data();
+    for (int i = 0; i < N; i++) {
+        T[tab_ch0(N) + i] = ch2[2 * i];
+        T[tab_ch1(N) + i] = ch2[2 * i + 1];
+        T[tab_par(N) + i] = par2[i];
+        T[tab_flags(N) + i] = (missing[i] ? NF_MISSING : 0) | (!own && tm.upper[i] ? NF_UPPER : 0) |
+                              (!own && tm.upper_cons[i] ? NF_UPPER_CONS : 0) | (i < S ? NF_LEAF : 0);
+    }
+    for (int v : nodes) T[tab_flags(N) + v] |= NF_IN_TREE;
+    for (size_t k = 0; k < nodes.size(); k++) {
+        T[tab_nodes(N) + k] = nodes[k];
+        T[tab_tz(N) + nodes[k]] = (int)nodes.size() - 1 - (int)k;
+    }
+    for (size_t k = 0; k < internal.size(); k++) T[tab_int(N) + k] = internal[k];
+    int q = 0;
+    for (int h = 1; h <= hmax; h++) {
+        for (int v : internal) if (height[v] == h) T[tab_up(N) + q++] = v;
+        T[tab_upoff(N) + h - 1] = q;
+    }
+    q = 0;
+    for (int d = 1; d <= dmax; d++) {
+        for (int k = (int)nodes.size() - 1; k >= 0; k--) if (depth[nodes[k]] == d) T[tab_down(N) + q++] = nodes[k];
+        T[tab_downoff(N) + d - 1] = q;
+    }
+    double *D = (double *)(T + tab_dist(N));
+    for (int i = 0; i < N; i++) D[i] = dist2[i];
+    E.n_up = hmax;
+    E.n_down = dmax;
+    E.root = root;
+    E.nodes = nodes;
+    E.internal = internal;
+}
+
 void encode_element(const TreeModel &tm, const pa_options &o, const char *const *rows, int64_t start, int len,
                     HostElem &E, const int32_t *own_children = nullptr)
 {
@@ -332,61 +399,7 @@ void encode_element(const TreeModel &tm, const pa_options &o, const char *const 
     col.run(N - 1, -1);
     par2[root] = N;
     if (root < S) { E.filtered = 1; return; }   // nothing left to prune over
-    // post-order of the element's tree (BPP_C::getSubtree, bpp_c.cpp:44-60)
-    std::vector<int> nodes;
-    {
-        std::vector<std::pair<int, int>> st{{root, 0}};
-        while (!st.empty()) {
-            auto &[v, k] = st.back();
-            if (v >= S && k < 2) { int x = ch2[2 * v + k]; k++; if (x != -1) st.push_back({x, 0}); continue; }
-            nodes.push_back(v);
-            st.pop_back();
-        }
-    }
-    std::vector<int> internal;
-    for (int v : nodes) if (v >= S) internal.push_back(v);
-    // levels for the Z forward-backward
-    std::vector<int> height(N, 0), depth(N, 0);
-    int hmax = 0, dmax = 0;
-    for (int v : nodes)
-        if (v >= S) { height[v] = 1 + std::max(height[ch2[2 * v]], height[ch2[2 * v + 1]]); hmax = std::max(hmax, height[v]); }
-    for (int k = (int)nodes.size() - 1; k >= 0; k--) {
-        int v = nodes[k];
-        depth[v] = v == root ? 0 : depth[par2[v]] + 1;
-        dmax = std::max(dmax, depth[v]);
-    }
-    E.tab.assign(tab_words(N), 0);
-    int32_t *T = E.tab.data();
-    for (int i = 0; i < N; i++) {
-        T[tab_ch0(N) + i] = ch2[2 * i];
-        T[tab_ch1(N) + i] = ch2[2 * i + 1];
-        T[tab_par(N) + i] = par2[i];
-        T[tab_flags(N) + i] = (missing[i] ? NF_MISSING : 0) | (!own_children && tm.upper[i] ? NF_UPPER : 0) |
-                              (!own_children && tm.upper_cons[i] ? NF_UPPER_CONS : 0) | (i < S ? NF_LEAF : 0);
-    }
-    for (int v : nodes) T[tab_flags(N) + v] |= NF_IN_TREE;
-    for (size_t k = 0; k < nodes.size(); k++) {
-        T[tab_nodes(N) + k] = nodes[k];
-        T[tab_tz(N) + nodes[k]] = (int)nodes.size() - 1 - (int)k;
-    }
-    for (size_t k = 0; k < internal.size(); k++) T[tab_int(N) + k] = internal[k];
-    int q = 0;
-    for (int h = 1; h <= hmax; h++) {
-        for (int v : internal) if (height[v] == h) T[tab_up(N) + q++] = v;
-        T[tab_upoff(N) + h - 1] = q;
-    }
-    q = 0;
-    for (int d = 1; d <= dmax; d++) {
-        for (int k = (int)nodes.size() - 1; k >= 0; k--) if (depth[nodes[k]] == d) T[tab_down(N) + q++] = nodes[k];
-        T[tab_downoff(N) + d - 1] = q;
-    }
-    double *D = (double *)(T + tab_dist(N));
-    for (int i = 0; i < N; i++) D[i] = dist2[i];
-    E.n_up = hmax;
-    E.n_down = dmax;
-    E.root = root;
-    E.nodes = nodes;
-    E.internal = internal;
+    fill_tree_table(tm, ch2, par2, dist2, missing, own_children != nullptr, root, E);
     // pattern compression (first-appearance order) or one pattern per site
     E.site.resize(GG);
     std::vector<int> first;
@@ -457,6 +470,7 @@ struct pa_engine {
     char *d_noalloc = nullptr;
     int max_ppad = 0;
     double mean_p = 0;
+    std::vector<ElemDev> dev_elems;          // host copy of the uploaded descriptors (tree updates patch both)
 };
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
@@ -796,6 +810,7 @@ static pa_status upload(pa_engine *e)
     for (const HostElem &E : e->elems)
         if (!E.filtered) { e->max_ppad = std::max(e->max_ppad, E.Ppad); e->mean_p += E.P; kept++; }
     if (kept) e->mean_p /= (double)kept;
+    e->dev_elems = std::move(d);
     e->dev_dirty = false;
     e->arena_gen++;
     return PA_OK;
@@ -1170,7 +1185,7 @@ extern "C" pa_status pa_loglik_last_stats(pa_engine *e, pa_run_stats *out)
 static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
                                      const double *piA, int32_t precision, double *out,
                                      const int32_t *lens0 = nullptr, const int32_t *lens1 = nullptr,
-                                     double *out1 = nullptr)
+                                     double *out1 = nullptr, const GtExtra *gx = nullptr)
 {
     if (!e || !ids || !lengths || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
     if (n == 0) return PA_OK;
@@ -1244,6 +1259,50 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
         A.wpre_stride = stride;
         A.ll_out2 = (double *)pre[1].p;
     }
+    // gene-tree moves: per-row trees, history draws, candidate scores (rows in the caller's order)
+    DevBuf gb[12];
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) -> hipError_t {
+        hipError_t r = hipMalloc(&b.p, std::max<size_t>(bytes, 8));
+        if (r == hipSuccess && bytes) r = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+        return r;
+    };
+    if (gx) {
+        A.site = e->d_site;
+        if (gx->tabs) {
+            HIPCHK(e, up(gb[0], gx->tabs, (size_t)n * tab_words(N) * 4));
+            HIPCHK(e, up(gb[1], gx->hdrs, (size_t)n * 4 * 4));
+            A.tab_call = (const int32_t *)gb[0].p;
+            A.hdr_call = (const int32_t *)gb[1].p;
+        }
+        A.hist_stride = gx->hist_stride;
+        if (gx->hist) {
+            HIPCHK(e, hipMalloc(&gb[2].p, std::max<size_t>((size_t)n * N * gx->hist_stride, 8)));
+            HIPCHK(e, up(gb[3], gx->hist_len, (size_t)n * 4));
+            A.hist = (int8_t *)gb[2].p;
+            A.hist_len = (const int32_t *)gb[3].p;
+            A.hist_sweep = gx->hist_sweep;
+        }
+        if (gx->cand_score) {
+            const size_t rows = (size_t)n * gx->cand_max;
+            HIPCHK(e, up(gb[4], gx->cand_tg, (size_t)n * N * gx->hist_stride));
+            HIPCHK(e, up(gb[5], gx->cand_branch, (size_t)n * 4));
+            HIPCHK(e, up(gb[6], gx->cand_n, (size_t)n * 4));
+            HIPCHK(e, up(gb[7], gx->cand_node, rows * 4));
+            HIPCHK(e, up(gb[8], gx->cand_kind, rows * 4));
+            HIPCHK(e, up(gb[9], gx->cand_sites, (size_t)n * 4));
+            HIPCHK(e, up(gb[10], gx->cand_len, rows * 8));
+            HIPCHK(e, hipMalloc(&gb[11].p, std::max<size_t>(rows * 8, 8)));
+            A.cand_tg = (const int8_t *)gb[4].p;
+            A.cand_branch = (const int32_t *)gb[5].p;
+            A.cand_n = (const int32_t *)gb[6].p;
+            A.cand_node = (const int32_t *)gb[7].p;
+            A.cand_kind = (const int32_t *)gb[8].p;
+            A.cand_sites = (const int32_t *)gb[9].p;
+            A.cand_len = (const double *)gb[10].p;
+            A.cand_max = gx->cand_max;
+            A.cand_score = (double *)gb[11].p;
+        }
+    }
     st = run_kernel(e, sorted.data(), n, precision == PA_FP64, A, bufs);
     if (st != PA_OK) return st;
     std::vector<double> ll(n);
@@ -1253,9 +1312,117 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
         HIPCHK(e, hipMemcpy(ll.data(), pre[1].p, (size_t)n * 8, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; i++) out1[order[i]] = ll[i];
     }
+    if (gx && gx->hist) HIPCHK(e, hipMemcpy(gx->hist, gb[2].p, (size_t)n * N * gx->hist_stride, hipMemcpyDeviceToHost));
+    if (gx && gx->cand_score)
+        HIPCHK(e, hipMemcpy(gx->cand_score, gb[11].p, (size_t)n * gx->cand_max * 8, hipMemcpyDeviceToHost));
     if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
     return PA_OK;
 }
+
+// Element id's node table under the tree children [N*2] (the root is the
+// node no other node has as a child), as a gene tree: missing leaves from
+// the element, an internal node missing when both children are.
+static pa_status gene_elem(pa_engine *e, int32_t id, const int32_t *children, HostElem &out)
+{
+    uint8_t lm[MAXN];
+    pa_status st = element_gene_tree(e, id, lm, nullptr);
+    if (st != PA_OK) return st;
+    const int N = e->tm.N, S = e->tm.S;
+    std::vector<int> ch2(children, children + 2 * N), par2(N, N);
+    std::vector<uint8_t> seen(N, 0);
+    for (int v = 0; v < S; v++)
+        if (ch2[2 * v] != -1 || ch2[2 * v + 1] != -1) return fail(e, PA_E_INPUT, "gene tree leaves must have no children");
+    for (int v = S; v < N; v++)
+        for (int k = 0; k < 2; k++) {
+            const int x = ch2[2 * v + k];
+            if (x < 0 || x >= N || x == v || seen[x]) return fail(e, PA_E_INPUT, "gene tree is not a binary tree over the leaves");
+            seen[x] = 1;
+            par2[x] = v;
+        }
+    int root = -1;
+    for (int v = S; v < N; v++)
+        if (!seen[v]) { if (root >= 0) return fail(e, PA_E_INPUT, "gene tree has two roots"); root = v; }
+    if (root < 0) return fail(e, PA_E_INPUT, "gene tree has no root");
+    std::vector<int> post;
+    {
+        std::vector<std::pair<int, int>> stk{{root, 0}};
+        while (!stk.empty() && (int)post.size() <= N) {
+            const int v = stk.back().first, k = stk.back().second;
+            if (v >= S && k < 2) { stk.back().second++; stk.push_back({ch2[2 * v + k], 0}); continue; }
+            post.push_back(v);
+            stk.pop_back();
+        }
+    }
+    if ((int)post.size() != N) return fail(e, PA_E_INPUT, "gene tree does not reach every node");
+    std::vector<uint8_t> missing(N, 0);
+    for (int v : post) missing[v] = v < S ? lm[v] : (missing[ch2[2 * v]] && missing[ch2[2 * v + 1]]);
+    fill_tree_table(e->tm, ch2, par2, std::vector<double>(N, 0.0), missing, true, root, out);
+    return PA_OK;
+}
+
+namespace pa {
+
+pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr)
+{
+    HostElem tmp;
+    pa_status st = gene_elem(e, id, children, tmp);
+    if (st != PA_OK) return st;
+    memcpy(T, tmp.tab.data(), (size_t)tab_words(e->tm.N) * 4);
+    hdr[0] = tmp.root; hdr[1] = (int32_t)tmp.nodes.size(); hdr[2] = tmp.n_up; hdr[3] = 0;
+    return PA_OK;
+}
+
+pa_status element_set_tree(pa_engine *e, int32_t id, const int32_t *children)
+{
+    HostElem tmp;
+    pa_status st = gene_elem(e, id, children, tmp);
+    if (st != PA_OK) return st;
+    const int N = e->tm.N;
+    HostElem &E = e->elems[id];
+    // gene trees take their branch lengths per call; the stored ones stay
+    memcpy(tmp.tab.data() + tab_dist(N), E.tab.data() + tab_dist(N), (size_t)(tab_words(N) - tab_dist(N)) * 4);
+    E.tab = std::move(tmp.tab);
+    E.root = tmp.root; E.n_up = tmp.n_up; E.n_down = tmp.n_down;
+    E.nodes = std::move(tmp.nodes);
+    E.internal = std::move(tmp.internal);
+    if (!e->dev_dirty && e->d_tab && id < (int)e->dev_elems.size()) {
+        ElemDev &x = e->dev_elems[id];
+        x.root = E.root; x.n_nodes = (int)E.nodes.size(); x.n_int = (int)E.internal.size();
+        x.n_up = E.n_up; x.n_down = E.n_down;
+        HIPCHK(e, hipMemcpy(e->d_tab + x.off_tab, E.tab.data(), (size_t)tab_words(N) * 4, hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemcpy(e->d_elems + id, &x, sizeof x, hipMemcpyHostToDevice));
+    }
+    return PA_OK;
+}
+
+pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths, const double *piA,
+                    const int32_t *lens0, const int32_t *lens1, int32_t precision, double *out0, double *out1,
+                    const GtExtra &x)
+{
+    if (n == 0) return PA_OK;
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    for (int i = 0; i < n; i++) {
+        const int GG = e->elems[ids[i]].GG;
+        if (lens0 && (lens1[i] < 0 || lens1[i] > lens0[i] || lens0[i] > GG))
+            return fail(e, PA_E_INPUT, "prefix lengths must satisfy 0 <= lens1 <= lens0 <= the element's sites");
+        if (x.hist && (x.hist_len[i] < 0 || x.hist_len[i] > GG || x.hist_len[i] > x.hist_stride))
+            return fail(e, PA_E_INPUT, "history length beyond the element's sites or the stride");
+        if (x.cand_score && (x.cand_sites[i] < 0 || x.cand_sites[i] > GG || x.cand_sites[i] > x.hist_stride ||
+                             x.cand_n[i] < 0 || x.cand_n[i] > x.cand_max))
+            return fail(e, PA_E_INPUT, "candidate sites or count out of range");
+        if (x.cand_score)
+            for (int q = 0; q < x.cand_n[i]; q++) {
+                const int t = x.cand_node[(size_t)i * x.cand_max + q], k = x.cand_kind[(size_t)i * x.cand_max + q];
+                if (k < 0 || k > 2 || (k == 0 && (t < e->tm.S || t >= e->tm.N)) || x.cand_branch[i] < 0 ||
+                    x.cand_branch[i] >= e->tm.N)
+                    return fail(e, PA_E_INPUT, "bad candidate");
+            }
+    }
+    return loglik_lengths_impl(e, ids, n, lengths, piA, precision, out0, lens0, lens1, out1, &x);
+}
+
+}  // namespace pa
 
 // The gene-tree log-likelihood over two prefixes of each element's sites,
 // the first lens0[i] and the first lens1[i] (Wang-Landau blocks of

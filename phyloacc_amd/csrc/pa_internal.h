@@ -20,4 +20,27 @@ pa_status engine_fail(pa_engine *e, pa_status st, const std::string &msg);
 // have the children in children [N*2] (either order) when that is given.
 pa_status element_gene_tree(pa_engine *e, int32_t id, uint8_t *leaf_missing, const int32_t *children);
 
+// Extra work of a gene-tree likelihood launch; arrays are per caller row.
+struct GtExtra {
+    const int32_t *tabs = nullptr, *hdrs = nullptr;   // per-row trees [n][tab_words(N)], [n][4] (gene_table)
+    int8_t *hist = nullptr;                           // out: history [n][N][hist_stride]
+    const int32_t *hist_len = nullptr;
+    int hist_stride = 0, hist_sweep = 0;
+    const int8_t *cand_tg = nullptr;                  // history [n][N][hist_stride] read by the scores
+    const int32_t *cand_branch = nullptr, *cand_n = nullptr, *cand_node = nullptr, *cand_kind = nullptr,
+                  *cand_sites = nullptr;
+    const double *cand_len = nullptr;
+    int cand_max = 0;
+    double *cand_score = nullptr;                     // out: [n][cand_max]
+};
+// One gene-tree likelihood launch (pa_loglik_lengths_split's, lens0 may be
+// null) with the extra work of x.
+pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths, const double *piA,
+                    const int32_t *lens0, const int32_t *lens1, int32_t precision, double *out0, double *out1,
+                    const GtExtra &x);
+// Node table of element id's leaves under the tree children [N*2].
+pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr);
+// Makes children [N*2] the element's stored tree (host and, when uploaded, device).
+pa_status element_set_tree(pa_engine *e, int32_t id, const int32_t *children);
+
 }  // namespace pa

@@ -63,6 +63,32 @@ struct LoglikArgs {
     const int32_t *wpre;      // [n_work][2][wpre_stride]
     int wpre_stride;
     double *ll_out2;          // [n_work]
+    // PhyloAcc-GT gene-tree moves.  Arrays below are per caller row: work
+    // position w uses row len_row[w] (or w).
+    // Per-call trees, or null: row r prunes the tree of node table
+    // tab_call[r] (tab_words(N) words; children, parents, flags, post-order,
+    // levels) with root, node count and level count in hdr_call[r][0..2].
+    const int32_t *tab_call;
+    const int32_t *hdr_call;  // [rows][4]
+    const int32_t *site;      // the site -> pattern arena (ElemDev::off_site)
+    // History draws, or null: after pruning, sites g < hist_len[r] get the
+    // base of every node in hist[r][v][g] (hist_stride per node; leaves
+    // 0-3, -1 ambiguous, 4 no base; internal nodes drawn top-down, 4 when
+    // missing), draws of sweep hist_sweep (tag GT_HIST, sub-stream g, slot v).
+    int8_t *hist;
+    const int32_t *hist_len;
+    int hist_stride, hist_sweep;
+    // Candidate scores, or null: for row r, branch cand_branch[r] and
+    // candidates q < cand_n[r] (node cand_node, kind cand_kind, distance
+    // cand_len; cand_max per row), sum over sites g < cand_sites[r] the score
+    // of kind 0 (node's base, or the pi-average when it has none), 1 (the
+    // pi-average) or 2 (the branch's own base's pi-row) into cand_score,
+    // reading the history cand_tg (hist_stride per node).
+    const int8_t *cand_tg;
+    const int32_t *cand_branch, *cand_n, *cand_node, *cand_kind, *cand_sites;
+    const double *cand_len;
+    int cand_max;
+    double *cand_score;
 };
 
 constexpr int GB_CLASSES = 6, GB_EIG_WORDS = 36;

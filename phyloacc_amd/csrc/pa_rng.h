@@ -26,7 +26,7 @@
 namespace pa {
 
 enum { TAG_ANC = 1, TAG_Z = 2, TAG_R2 = 3, TAG_R1 = 4, TAG_TRANS = 5, TAG_PI = 6, TAG_WL = 7, TAG_GT_BRANCH = 8,
-       TAG_GT_TREE = 9 };
+       TAG_GT_TREE = 9, TAG_GT_HIST = 10 };
 
 struct U4 { uint32_t x, y, z, w; };
 

@@ -34,7 +34,8 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_serve_submit_lengths", "pa_release_scratch", "pa_loglik_gbgc",
            "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats",
            "pa_loglik_lengths_pi", "pa_sample_pi", "pa_loglik_lengths_split", "pa_wl_step",
-           "pa_gt_prior", "pa_gt_logpz", "pa_gt_lengths", "pa_gt_branch_len"]
+           "pa_gt_prior", "pa_gt_logpz", "pa_gt_lengths", "pa_gt_branch_len", "pa_gt_sample_history",
+           "pa_gt_sample_tree"]
 
 
 class SpeciesTree(C.Structure):
@@ -210,6 +211,13 @@ def lib():
                                        C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, PD, PD,
                                        PD, PI]
         L.pa_gt_branch_len.restype = C.c_int
+        L.pa_gt_sample_history.argtypes = [C.c_void_p, P(SpeciesTree), PI, C.c_int32, PI, PI, PD, PI, PD, PD, PI,
+                                           C.c_int32, C.c_int32, P(C.c_int8), C.c_int32]
+        L.pa_gt_sample_history.restype = C.c_int
+        L.pa_gt_sample_tree.argtypes = [C.c_void_p, P(SpeciesTree), PI, C.c_int32, PI, PI, PD, PI, PD, PD, PI, PI,
+                                        PI, P(C.c_int8), C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_int32, PD, PD, PI]
+        L.pa_gt_sample_tree.restype = C.c_int
         L.pa_loglik_last_stats.argtypes = [C.c_void_p, P(RunStats)]
         L.pa_loglik_last_stats.restype = C.c_int
         L.pa_batch_create.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(C.c_void_p)]
@@ -457,6 +465,58 @@ class Engine:
             _p(g, C.c_int32), _p(ind, C.c_int32), _p(ln, C.c_int32), float(delta), float(cons_to_mis),
             float(ncons_to_mis), int(m), int(slot), int(precision), _p(loglik, C.c_double), _p(logpz, C.c_double),
             _p(prior, C.c_double), _p(acc, C.c_int32)))
+        return acc
+
+    def gt_sample_history(self, sp, ids, children, parent, heights, Z, rates, length, m, piA=None,
+                          precision=PA_FP64, stride=None):
+        """Batched Update_Tg(tosample=true): int8 [n, N, stride] histories of
+        the first length[i] sites."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        n = len(ids)
+        ch = np.ascontiguousarray(children, np.int32).reshape(n, self.N, 2)
+        par = np.ascontiguousarray(parent, np.int32).reshape(n, self.N)
+        h = np.ascontiguousarray(heights, np.float64).reshape(n, self.N)
+        z = np.ascontiguousarray(Z, np.int32).reshape(n, self.N)
+        r = np.ascontiguousarray(rates, np.float64).reshape(n, 2)
+        ln = np.ascontiguousarray(np.broadcast_to(length, (n,)), np.int32)
+        stride = int(max(1, ln.max() if n else 1) if stride is None else stride)
+        a = None if piA is None else np.ascontiguousarray(piA, np.float64)
+        out = np.zeros((n, self.N, stride), np.int8)
+        self._check(lib().pa_gt_sample_history(
+            self._h, C.byref(sp), _p(ids, C.c_int32), n, _p(ch, C.c_int32), _p(par, C.c_int32), _p(h, C.c_double),
+            _p(z, C.c_int32), _p(r, C.c_double), None if a is None else _p(a, C.c_double), _p(ln, C.c_int32), int(m),
+            int(precision), _p(out, C.c_int8), stride))
+        return out
+
+    def gt_sample_tree(self, sp, ids, children, parent, heights, Z, rates, branch, indicator, lens, Tg, loglik,
+                       logpz, m, slot, cons_to_mis=0.01, ncons_to_mis=0.5, root_ingrp=None, piA=None,
+                       precision=PA_FP64):
+        """One batched Sample_tree2 move.  children [n, N, 2], parent,
+        heights [n, N], loglik, logpz [n, 2] are updated in place (C order;
+        int32 / float64); Tg [n, N, stride] as gt_sample_history returns it.
+        Returns accepted [n]."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        n = len(ids)
+        for a, t in ((children, np.int32), (parent, np.int32), (heights, np.float64), (loglik, np.float64),
+                     (logpz, np.float64)):
+            assert a.dtype == t and a.flags.c_contiguous
+        assert children.shape == (n, self.N, 2) and parent.shape == (n, self.N) and heights.shape == (n, self.N)
+        z = np.ascontiguousarray(Z, np.int32).reshape(n, self.N)
+        r = np.ascontiguousarray(rates, np.float64).reshape(n, 2)
+        b = np.ascontiguousarray(np.broadcast_to(branch, (n,)), np.int32)
+        ind = np.ascontiguousarray(np.broadcast_to(indicator, (n,)), np.int32)
+        ln = np.ascontiguousarray(np.broadcast_to(lens, (n, 2)), np.int32)
+        T = np.ascontiguousarray(Tg, np.int8)
+        assert T.shape[:2] == (n, self.N)
+        a = None if piA is None else np.ascontiguousarray(piA, np.float64)
+        acc = np.zeros(n, np.int32)
+        ri = self.N - 1 if root_ingrp is None else int(root_ingrp)
+        self._check(lib().pa_gt_sample_tree(
+            self._h, C.byref(sp), _p(ids, C.c_int32), n, _p(children, C.c_int32), _p(parent, C.c_int32),
+            _p(heights, C.c_double), _p(z, C.c_int32), _p(r, C.c_double), None if a is None else _p(a, C.c_double),
+            _p(b, C.c_int32), _p(ind, C.c_int32), _p(ln, C.c_int32), _p(T, C.c_int8), T.shape[2],
+            float(cons_to_mis), float(ncons_to_mis), ri, int(m), int(slot), int(precision), _p(loglik, C.c_double),
+            _p(logpz, C.c_double), _p(acc, C.c_int32)))
         return acc
 
     def loglik_stats(self):
