@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="time direct launches instead of the captured graph")
     ap.add_argument("--gt-cache", default=None,
                     help="config 5: load the gene-tree workload from this file, or simulate and save it there")
+    ap.add_argument("--gt-sweeps", type=int, default=1,
+                    help="config 5: timed sweeps of the gene-tree moves (0 skips them)")
     ap.add_argument("--table", action="store_true",
                     help="per-configuration figures (batched FP32/FP64, plain FP64, CPU, agreement) as one JSON line")
     ap.add_argument("--no-extra", action="store_true",
@@ -598,7 +600,89 @@ def gene_trees(args, E, synth):
     out = gene_trees_block(E, w, args.steps, args.warmup, args.precision, tr, oracle=not args.no_cpu)
     out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None})
+    if args.gt_sweeps > 0:
+        out["gene_tree_moves"] = gt_moves_block(E, w, args.gt_sweeps, args.precision)
     print(json.dumps(out))
+
+
+def gt_moves_block(E, w, sweeps=1, precision="fp32"):
+    """One PhyloAcc-GT sweep of the gene-tree moves over every element, as
+    the reference's Gibbs orders them (src/PhyloAcc-GT/bpp_c.cpp:560-680):
+    the history draw, one regraft (Sample_tree2) per element, the height move
+    (Sample_BranchLen) at every internal gene node, and one sample_pi step;
+    every move of every element batched into one launch per move (the
+    regraft takes two).  Z and the rates stay fixed (the species-branch
+    classes of the workload).  Wall clock per phase, one warmup sweep."""
+    prec = E.PA_FP32 if precision == "fp32" else E.PA_FP64
+    eng = E.Engine.from_workload(w, E.default_options(min_len=1))
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    keep = np.flatnonzero(st == E.PA_ELEM_OK)
+    ids = ids[keep]
+    n, S, N = len(ids), w.tree.S, w.tree.N
+    par_sp = np.where(w.tree.parent < 0, N, w.tree.parent)
+    sp = E.SpeciesTree.make(S, par_sp, w.sp_height * w.scale, np.full(N, 2 * w.scale))
+    ch = np.ascontiguousarray(w.children()[keep])
+    par = np.ascontiguousarray(np.stack([np.where(w.genes[c].parent < 0, N, w.genes[c].parent) for c in keep]),
+                               dtype=np.int32)
+    H = np.ascontiguousarray(np.stack([w.genes[c].height * w.scale for c in keep]))
+    Z = np.ascontiguousarray(w.Z[keep])
+    rates = np.ascontiguousarray(np.stack([w.r1[keep], w.r2[keep]], 1))
+    sizes = w.sizes[keep].astype(np.int32)
+    lens = np.ascontiguousarray(np.stack([sizes, sizes // 2], 1), dtype=np.int32)
+    piA = np.full(n, 0.25)
+    L = np.stack([E.gt_lengths(sp, ch[i], par[i], H[i], Z[i], rates[i, 0], rates[i, 1]) for i in range(n)])
+    f0, f1 = eng.loglik_lengths_split(ids, L, lens[:, 0], lens[:, 1], piA=piA, precision=prec)
+    ll = np.ascontiguousarray(np.stack([f0, f1], 1))
+    lz = np.zeros((n, 2))
+    prior = np.array([E.gt_prior(sp, ch[i], par[i], H[i]) for i in range(n)])
+    phases = {"history": 0.0, "regraft": 0.0, "heights": 0.0, "pi": 0.0}
+    acc = {"regraft": 0, "heights": 0, "pi": 0}
+    moves = 0
+    for m in range(sweeps + 1):
+        timed = m > 0
+        ind = np.zeros(n, np.int32)
+        t0 = time.time()
+        hist = eng.gt_sample_history(sp, ids, ch, par, H, Z, rates, lens[:, 0], m, piA=piA, precision=prec)
+        t1 = time.time()
+        # the (7m + i)-th gene node of element i, the root skipped
+        root = np.argmax(par == N, axis=1)
+        branch = (m * 7 + np.arange(n)) % (N - 1)
+        branch = (branch + (branch >= root)).astype(np.int32)
+        a = eng.gt_sample_tree(sp, ids, ch, par, H, Z, rates, branch, ind, lens, hist, ll, lz, m, 0, piA=piA,
+                               precision=prec)
+        t2 = time.time()
+        prior = np.array([E.gt_prior(sp, ch[i], par[i], H[i]) for i in range(n)])
+        t2b = time.time()
+        ah = 0
+        for b, v in enumerate(range(S, N)):
+            ah += int(eng.gt_branch_len(sp, ids, ch, par, H, Z, rates, v, ind, lens, ll, lz, prior, m, 1 + b,
+                                        piA=piA, precision=prec).sum())
+        t3 = time.time()
+        L = np.stack([E.gt_lengths(sp, ch[i], par[i], H[i], Z[i], rates[i, 0], rates[i, 1]) for i in range(n)])
+        t3b = time.time()
+        piA, l_pi, ap = eng.sample_pi(ids, L, m, piA, ll[:, 0], precision=prec)
+        ll[:, 0] = l_pi
+        t4 = time.time()
+        if timed:
+            phases["history"] += t1 - t0
+            phases["regraft"] += t2 - t1
+            phases["heights"] += t3 - t2b
+            phases["pi"] += t4 - t3b
+            acc["regraft"] += int(a.sum())
+            acc["heights"] += ah
+            acc["pi"] += int(ap.sum())
+            moves += n * (1 + (N - S) + 1)
+    eng.close()
+    tot = sum(phases.values())
+    return {"metric": "gene-tree MCMC moves/s (history, regraft, every node height, pi; batched over elements)",
+            "value": round(moves / tot, 1), "unit": "element moves/s", "sweeps": sweeps, "elements": n,
+            "ms_per_sweep": round(tot / sweeps * 1e3, 1),
+            "phase_ms": {k: round(v / sweeps * 1e3, 1) for k, v in phases.items()},
+            "launches_per_sweep": 1 + 2 + (N - S) + 1,
+            "accept_rate": {k: round(v / (n * sweeps * ((N - S) if k == "heights" else 1)), 4)
+                            for k, v in acc.items()},
+            "precision": precision,
+            "data": "synthetic gene trees of the workload; Z and rates fixed"}
 
 
 def gene_trees_block(E, w, steps=3, warmup=1, precision="fp32", traffic=None, oracle=True):

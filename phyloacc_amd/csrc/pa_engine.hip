@@ -63,8 +63,8 @@ struct Ctx {
     int S, N, P, Ppad, root, n_nodes, n_up;
     const GBL uint8_t *codes;
     const GBL int32_t *wts;
-    const GBL int32_t *wts1;   // weights of a second root sum (a second site prefix), or null
-    double sum1;               // that sum
+    int pre0, pre1;            // site-prefix root sums (pre0 < 0: the pattern sum)
+    double sum1;               // the second prefix's sum
     // workgroup slot in HBM
     GBL V4<R> *parts;      // [I][Ppad]
     GBL int32_t *exps;     // [I][Ppad]
@@ -377,14 +377,37 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
     }
     // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
     const int r = c->root;
-    const GBL int32_t *wts1 = uni_ptr(c->wts1);
+    const int pre0 = uni(c->pre0);
     double acc = 0, acc1 = 0;
+    if (pre0 >= 0 && r >= S) {
+        // two site prefixes: each site reads its pattern's root partial,
+        // which another wave may have written, so the pass must finish first
+        __syncthreads();
+        const int pre1 = uni(c->pre1);
+        const GBL int32_t *site = uni_ptr(c->site);
+        const GBL V4<R> *rp = parts + (size_t)(r - S) * Ppad;
+        const GBL int32_t *re = exps + (size_t)(r - S) * Ppad;
+        const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
+        for (int g = tid; g < pre0; g += BS) {
+            const int p = site[g];
+            const int ex = re[p];
+            if (ex == GAPT) continue;
+            const V4<R> av = ld4<R>(rp + p);
+            const double L = p0 * (double)av.x + p1 * (double)av.y + p2 * (double)av.z + p3 * (double)av.w;
+            const double t = log(L) + (double)ex * 0.69314718055994530942;
+            acc += t;
+            if (g < pre1) acc1 += t;
+        }
+        const double s1 = block_sum(acc1, c->red);
+        if (threadIdx.x == 0) c->sum1 = s1;
+        return block_sum(acc, c->red);
+    }
     if (r >= S) {
         const GBL V4<R> *rp = parts + (size_t)(r - S) * Ppad;
         const GBL int32_t *re = exps + (size_t)(r - S) * Ppad;
         const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
         for (int pb = tid; pb < P; pb += U * BS) {
-            int ex[U], wt[U], wt1[U];
+            int ex[U], wt[U];
             V4<R> av[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -393,7 +416,6 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
                 if (p < P) {
                     ex[u] = re[p];
                     wt[u] = c->wts[p];
-                    wt1[u] = wts1 ? wts1[p] : 0;
                     av[u] = ld4<R>(rp + p);
                 }
             }
@@ -403,13 +425,8 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
                 const double L = p0 * (double)av[u].x + p1 * (double)av[u].y + p2 * (double)av[u].z + p3 * (double)av[u].w;
                 const double t = log(L) + (double)ex[u] * 0.69314718055994530942;
                 acc += (double)wt[u] * t;
-                acc1 += (double)wt1[u] * t;
             }
         }
-    }
-    if (wts1) {
-        const double s1 = block_sum(acc1, c->red);
-        if (threadIdx.x == 0) c->sum1 = s1;
     }
     return block_sum(acc, c->red);
 }
@@ -594,13 +611,9 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
         c->n_nodes = hdr ? hdr[1] : E.n_nodes;
         c->n_up = hdr ? hdr[2] : E.n_up;
         c->codes = (const GBL uint8_t *)(A->codes + E.off_codes);
-        if (A->wpre) {
-            c->wts = (const GBL int32_t *)(A->wpre + (size_t)w * 2 * A->wpre_stride);
-            c->wts1 = c->wts + A->wpre_stride;
-        } else {
-            c->wts = (const GBL int32_t *)(A->wts + E.off_w);
-            c->wts1 = nullptr;
-        }
+        c->wts = (const GBL int32_t *)(A->wts + E.off_w);
+        c->pre0 = A->pre_len ? A->pre_len[2 * row] : -1;
+        c->pre1 = A->pre_len ? A->pre_len[2 * row + 1] : -1;
         const int64_t bo = A->big_off ? A->big_off[w] : -1;
         char *base = bo >= 0 ? A->big + bo : A->scratch + (int64_t)blockIdx.x * A->slot_bytes;
         c->parts = (GBL V4<R> *)base;
@@ -699,7 +712,7 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
         }
         if (threadIdx.x == 0) {
             A->ll_out[w] = l;
-            if (A->wpre) A->ll_out2[w] = c->sum1;
+            if (A->pre_len) A->ll_out2[w] = c->sum1;
             // root partial + exponent + pattern weight
             if (A->counters) atomicAdd(A->counters, bytes + (unsigned long long)c->P * (4 * sizeof(R) + 8));
         }

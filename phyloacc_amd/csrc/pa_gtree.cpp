@@ -161,6 +161,25 @@ void lengths_of(const Embed &g, const int32_t *Z, double r1, double r2, double *
     }
 }
 
+// Runs fn(i) for i < n on up to 16 host threads (one below 256 elements);
+// fn must write only what belongs to element i.
+template <typename F>
+void for_elements(int n, F &&fn)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 256) nt = 1;
+    if (nt == 1) {
+        for (int i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (int i = (int)t; i < n; i += (int)nt) fn(i);
+        });
+    for (auto &x : th) x.join();
+}
+
 pa_status check_sp(pa_engine *e, const pa_species_tree *t, Sp &s)
 {
     if (!t || !t->parent || !t->height || !t->theta) return engine_fail(e, PA_E_INPUT, "null species tree");
@@ -287,62 +306,72 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
     if (s.S != ev.S || s.N != ev.N) return engine_fail(e, PA_E_INPUT, "species tree size differs from the model's");
     if (n == 0) return PA_OK;
     const int N = s.N;
+    // the trees must be the elements' stored trees (serial: cheap, and it may record an error)
+    std::vector<uint8_t> lm((size_t)n * s.S);
+    for (int i = 0; i < n; i++) {
+        if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * s.S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
+        if (gnode[i] < s.S || gnode[i] >= N) return engine_fail(e, PA_E_INPUT, "gnode must be an internal gene node");
+        if (indicator[i] != 0 && indicator[i] != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
+        if (lens[2 * i + 1] < 0 || lens[2 * i + indicator[i]] < 0) return engine_fail(e, PA_E_INPUT, "negative prefix length");
+    }
     std::vector<Embed> emb(n);
     std::vector<HeightMove> win(n);
     std::vector<std::vector<uint8_t>> miss(n);
-    std::vector<double> prop(n, 0.0), rdraw(n, 0.0);
+    std::vector<std::vector<double>> newh(n);
+    std::vector<double> prop(n, 0.0), rdraw(n, 0.0), rlen((size_t)n * N, 0.0);
+    std::vector<int> bad(n, 0);
     const Key key{ev.key0, ev.key1};
     const uint32_t c3 = tagword(0, TAG_GT_BRANCH), c2 = (uint32_t)m;
-    for (int i = 0; i < n; i++) {
-        const int32_t *ch = gchildren + (size_t)i * 2 * N, *pa = gparent + (size_t)i * N;
-        std::vector<uint8_t> lm(s.S);
-        if ((st = element_gene_tree(e, ids[i], lm.data(), ch)) != PA_OK) return st;
-        if (!emb[i].build(s, ch, pa, gheight + (size_t)i * N))
-            return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
-        missing_of(emb[i], lm.data(), miss[i]);
-        const int v = gnode[i], ind = indicator[i];
-        if (v < s.S || v >= N || emb[i].where[v] < 0)
-            return engine_fail(e, PA_E_INPUT, "gnode must be an internal gene node");
-        if (ind != 0 && ind != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
-        if (lens[2 * i + 1] < 0 || lens[2 * i + ind] < 0) return engine_fail(e, PA_E_INPUT, "negative prefix length");
-        win[i] = height_window(emb[i], v, delta);
-        if (win[i].status < 0)
-            return engine_fail(e, PA_E_INPUT, "element " + std::to_string(i) + ": no species holds both children of gene node " +
-                                                  std::to_string(v));
-        if (win[i].status == 0) {
-            const double u = slot_uniform(key, (uint32_t)slot, c2, c3, 0);
-            prop[i] = std::max(u, 1e-5) * (win[i].upper - win[i].lower) + win[i].lower;
-            rdraw[i] = slot_uniform(key, (uint32_t)slot, c2, c3, 1);
-        }
+    // per element: window, draws, and the proposal's branch lengths
+    for_elements(n, [&](int i) {
         accepted[i] = 0;
-    }
-    // the proposals' lengths and their prefix likelihoods, one launch
-    std::vector<int32_t> run, rid, l0, l1;
-    std::vector<double> rlen, rpi;
-    std::vector<std::vector<double>> newh(n);
-    for (int i = 0; i < n; i++) {
-        if (win[i].status != 0) continue;
+        const int32_t *ch = gchildren + (size_t)i * 2 * N, *pa = gparent + (size_t)i * N;
+        if (!emb[i].build(s, ch, pa, gheight + (size_t)i * N)) { bad[i] = 1; return; }
+        missing_of(emb[i], &lm[(size_t)i * s.S], miss[i]);
+        win[i] = height_window(emb[i], gnode[i], delta);
+        if (win[i].status < 0) { bad[i] = 2; return; }
+        if (win[i].status != 0) return;
+        const double u = slot_uniform(key, (uint32_t)slot, c2, c3, 0);
+        prop[i] = std::max(u, 1e-5) * (win[i].upper - win[i].lower) + win[i].lower;
+        rdraw[i] = slot_uniform(key, (uint32_t)slot, c2, c3, 1);
         newh[i].assign(gheight + (size_t)i * N, gheight + (size_t)(i + 1) * N);
         newh[i][gnode[i]] = prop[i];
         Embed g2;
-        g2.build(s, emb[i].ch, emb[i].par, newh[i].data());
+        g2.build(s, ch, pa, newh[i].data());
         emb[i] = std::move(g2);
+        lengths_of(emb[i], Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &rlen[(size_t)i * N]);
+    });
+    for (int i = 0; i < n; i++) {
+        if (bad[i] == 1) return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
+        if (bad[i] == 2)
+            return engine_fail(e, PA_E_INPUT, "element " + std::to_string(i) + ": no species holds both children of gene node " +
+                                                  std::to_string(gnode[i]));
+    }
+    // the proposals' prefix likelihoods, one launch
+    std::vector<int32_t> run, rid, l0, l1;
+    std::vector<double> rl, rpi;
+    for (int i = 0; i < n; i++) {
+        if (win[i].status != 0) continue;
         run.push_back(i);
         rid.push_back(ids[i]);
         const int ind = indicator[i];
         l0.push_back(lens[2 * i + ind]);
         l1.push_back(std::min(lens[2 * i + 1], lens[2 * i + ind]));
-        rlen.resize(rlen.size() + N);
-        lengths_of(emb[i], Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &rlen[rlen.size() - N]);
         if (piA) rpi.push_back(piA[i]);
     }
     const int k = (int)run.size();
     if (k == 0) return PA_OK;
+    const double *lenp = rlen.data();
+    if (k < n) {
+        rl.resize((size_t)k * N);
+        for (int j = 0; j < k; j++) memcpy(&rl[(size_t)j * N], &rlen[(size_t)run[j] * N], sizeof(double) * N);
+        lenp = rl.data();
+    }
     std::vector<double> o0(k), o1(k);
-    if ((st = pa_loglik_lengths_split(e, rid.data(), k, rlen.data(), piA ? rpi.data() : nullptr, l0.data(), l1.data(),
+    if ((st = pa_loglik_lengths_split(e, rid.data(), k, lenp, piA ? rpi.data() : nullptr, l0.data(), l1.data(),
                                       precision, o0.data(), o1.data())) != PA_OK)
         return st;
-    for (int j = 0; j < k; j++) {
+    for_elements(k, [&](int j) {
         const int i = run[j], ind = indicator[i];
         const HeightMove &w = win[i];
         const double lz = logpz_of(emb[i], miss[i], Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
@@ -361,7 +390,7 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
             else logpz[2 * i] = logpz[2 * i + 1] = lz;
             prior[i] = pr;
         }
-    }
+    });
     return PA_OK;
 }
 
@@ -498,11 +527,15 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
     for (int i = 0; i < n; i++) {
         uint8_t lm[512];
         if ((st = element_gene_tree(e, ids[i], lm, gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
-        Embed g;
-        if (!g.build(s, gchildren + (size_t)i * 2 * N, gparent + (size_t)i * N, gheight + (size_t)i * N))
-            return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
-        lengths_of(g, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &lens[(size_t)i * N]);
     }
+    std::vector<int> bad(n, 0);
+    for_elements(n, [&](int i) {
+        Embed g;
+        if (!g.build(s, gchildren + (size_t)i * 2 * N, gparent + (size_t)i * N, gheight + (size_t)i * N)) { bad[i] = 1; return; }
+        lengths_of(g, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &lens[(size_t)i * N]);
+    });
+    for (int i = 0; i < n; i++)
+        if (bad[i]) return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
     GtExtra x;
     x.hist = Tg;
     x.hist_len = len;
@@ -544,25 +577,30 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     const uint32_t c3 = tagword(0, TAG_GT_TREE), c2 = (uint32_t)m;
     std::vector<Embed> emb(n);
     std::vector<TreeMove> mv(n);
-    std::vector<std::vector<uint8_t>> miss0(n);
-    int cmax = 1;
+    std::vector<uint8_t> lm((size_t)n * s.S);
     std::vector<double> cur_len((size_t)n * N);
     for (int i = 0; i < n; i++) {
-        int32_t *ch = gchildren + (size_t)i * 2 * N, *pa = gparent + (size_t)i * N;
-        uint8_t lm[512];
-        if ((st = element_gene_tree(e, ids[i], lm, ch)) != PA_OK) return st;
-        if (!emb[i].build(s, ch, pa, gheight + (size_t)i * N))
-            return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
-        const int b = branch[i], ind = indicator[i];
-        if (b < 0 || b >= N || b == emb[i].root || emb[i].where[b] < 0)
-            return engine_fail(e, PA_E_INPUT, "branch must be a gene node other than the root");
+        if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * s.S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
+        const int ind = indicator[i];
         if (ind != 0 && ind != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
         if (lens[2 * i + ind] < 0 || lens[2 * i + 1] < 0 || lens[2 * i + ind] > stride)
             return engine_fail(e, PA_E_INPUT, "prefix length negative or beyond the history's stride");
-        missing_of(emb[i], lm, miss0[i]);
-        tree_candidates(emb[i], b, root_ingrp, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], mv[i]);
-        cmax = std::max(cmax, (int)mv[i].cnode.size());
+        if (branch[i] < 0 || branch[i] >= N || gparent[(size_t)i * N + branch[i]] == N)
+            return engine_fail(e, PA_E_INPUT, "branch must be a gene node other than the root");
+    }
+    std::vector<int> bad(n, 0);
+    for_elements(n, [&](int i) {
+        if (!emb[i].build(s, gchildren + (size_t)i * 2 * N, gparent + (size_t)i * N, gheight + (size_t)i * N)) {
+            bad[i] = 1;
+            return;
+        }
+        tree_candidates(emb[i], branch[i], root_ingrp, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], mv[i]);
         lengths_of(emb[i], Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &cur_len[(size_t)i * N]);
+    });
+    int cmax = 1;
+    for (int i = 0; i < n; i++) {
+        if (bad[i]) return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
+        cmax = std::max(cmax, (int)mv[i].cnode.size());
     }
     // launch 1: the current trees' partials score every candidate
     std::vector<int32_t> cb(n), cn(n), cnode((size_t)n * cmax, 0), ckind((size_t)n * cmax, 1), csites(n);
@@ -594,7 +632,8 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<double> new_unif(n), rdraw(n);
     std::vector<int32_t> tabs((size_t)n * tab_words(N)), hdrs((size_t)n * 4), l0(n), l1(n);
     std::vector<double> nlen((size_t)n * N), rpi;
-    for (int i = 0; i < n; i++) {
+    std::vector<int> gbad(n, 0);
+    for_elements(n, [&](int i) {
         TreeMove &t = mv[i];
         const Embed &g = emb[i];
         const int nc = (int)t.cnode.size();
@@ -646,16 +685,19 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         nch[i][2 * bp + t.cb] = branch[i];
         nch[i][2 * bp + 1 - t.cb] = nsib;
         if (it2p < N) nch[i][2 * it2p + (t.ch2[2 * it2p] == nsib ? 0 : 1)] = bp;
-        if ((st = gene_table(e, ids[i], nch[i].data(), &tabs[(size_t)i * tab_words(N)], &hdrs[(size_t)i * 4])) != PA_OK)
-            return st;
+        if (gene_table(e, ids[i], nch[i].data(), &tabs[(size_t)i * tab_words(N)], &hdrs[(size_t)i * 4]) != PA_OK) {
+            gbad[i] = 1;
+            return;
+        }
         Embed g2;
-        if (!g2.build(s, nch[i].data(), npar[i].data(), nh[i].data()))
-            return engine_fail(e, PA_E_INPUT, "regrafted gene tree is not valid");
+        if (!g2.build(s, nch[i].data(), npar[i].data(), nh[i].data())) { gbad[i] = 1; return; }
         lengths_of(g2, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &nlen[(size_t)i * N]);
         const int ind = indicator[i];
         l0[i] = lens[2 * i + ind];
         l1[i] = std::min(lens[2 * i + 1], lens[2 * i + ind]);
-    }
+    });
+    for (int i = 0; i < n; i++)
+        if (gbad[i]) return engine_fail(e, PA_E_INPUT, "element " + std::to_string(i) + ": regrafted gene tree is not valid");
     // launch 2: the proposed trees' prefix likelihoods
     std::vector<double> o0(n), o1(n);
     {
@@ -665,15 +707,13 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         if ((st = gt_launch(e, ids, n, nlen.data(), piA, l0.data(), l1.data(), precision, o0.data(), o1.data(), x)) != PA_OK)
             return st;
     }
-    for (int i = 0; i < n; i++) {
+    for_elements(n, [&](int i) {
         accepted[i] = 0;
-        if (pos[i] < 0) continue;
+        if (pos[i] < 0) return;
         Embed g2;
         g2.build(s, nch[i].data(), npar[i].data(), nh[i].data());
-        uint8_t lm[512];
-        if ((st = element_gene_tree(e, ids[i], lm, nullptr)) != PA_OK) return st;
         std::vector<uint8_t> miss;
-        missing_of(g2, lm, miss);
+        missing_of(g2, &lm[(size_t)i * s.S], miss);
         const double lz = logpz_of(g2, miss, Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
         const int ind = indicator[i];
         const double lnew[2] = {o0[i], o1[i]};
@@ -681,7 +721,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         if (logpz[2 * i + ind] != 0) d += lz - logpz[2 * i + ind];
         d += prior_of(g2) - mv[i].curG + mv[i].cur_unif - new_unif[i] + std::log(sprob[i][pos[i]]) -
              std::log(sprob[i][get[i]]);
-        if (!(std::log(rdraw[i]) < d)) continue;
+        if (!(std::log(rdraw[i]) < d)) return;
         accepted[i] = 1;
         memcpy(gchildren + (size_t)i * 2 * N, nch[i].data(), sizeof(int32_t) * 2 * N);
         memcpy(gparent + (size_t)i * N, npar[i].data(), sizeof(int32_t) * N);
@@ -690,7 +730,9 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         loglik[2 * i + 1] = lnew[1];
         if (logpz[2 * i + 1] == 0) logpz[2 * i] = lz;
         else logpz[2 * i] = logpz[2 * i + 1] = lz;
-        if ((st = element_set_tree(e, ids[i], nch[i].data())) != PA_OK) return st;
-    }
+    });
+    // the accepted trees become the elements' stored trees
+    for (int i = 0; i < n; i++)
+        if (accepted[i] && (st = element_set_tree(e, ids[i], nch[i].data())) != PA_OK) return st;
     return PA_OK;
 }

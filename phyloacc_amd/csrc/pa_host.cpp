@@ -8,6 +8,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -445,6 +446,7 @@ struct pa_engine {
     double Q0[16];                  // RATE_MAT (the gBGC classes are built from it)
     TreeModel tm;
     std::string err;
+    std::mutex err_mu;
     std::vector<HostElem> elems;
     bool dev_dirty = true;
     std::vector<struct pa_batch *> batches;   // prepared batches, freed with the engine
@@ -475,7 +477,10 @@ struct pa_engine {
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
 {
-    if (e) e->err = msg;
+    if (e) {
+        std::lock_guard<std::mutex> lock(e->err_mu);   // host phases may run on several threads
+        e->err = msg;
+    }
     return st;
 }
 
@@ -1239,25 +1244,15 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
     for (int i = 0; i < n; i++) sorted[i] = ids[order[i]];
     DevBuf pre[2];
     if (lens0) {
-        // per work position, the pattern weights of its first lens0 and lens1 sites
-        int stride = 1;
-        for (int i = 0; i < n; i++) stride = std::max(stride, e->elems[sorted[i]].Ppad);
-        std::vector<int32_t> wp((size_t)n * 2 * stride, 0);
-        for (int i = 0; i < n; i++) {
-            const HostElem &E = e->elems[sorted[i]];
-            const int L0 = lens0[order[i]], L1 = lens1[order[i]];
-            int32_t *w0 = &wp[(size_t)i * 2 * stride], *w1 = w0 + stride;
-            for (int g = 0; g < L0; g++) {
-                w0[E.site[g]]++;
-                if (g < L1) w1[E.site[g]]++;
-            }
-        }
-        HIPCHK(e, hipMalloc(&pre[0].p, wp.size() * 4));
+        // the two site prefixes per caller row; the root sums walk the sites
+        std::vector<int32_t> pl((size_t)n * 2);
+        for (int i = 0; i < n; i++) { pl[2 * i] = lens0[i]; pl[2 * i + 1] = lens1[i]; }
+        HIPCHK(e, hipMalloc(&pre[0].p, pl.size() * 4));
         HIPCHK(e, hipMalloc(&pre[1].p, (size_t)n * 8));
-        HIPCHK(e, hipMemcpy(pre[0].p, wp.data(), wp.size() * 4, hipMemcpyHostToDevice));
-        A.wpre = (const int32_t *)pre[0].p;
-        A.wpre_stride = stride;
+        HIPCHK(e, hipMemcpy(pre[0].p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice));
+        A.pre_len = (const int32_t *)pre[0].p;
         A.ll_out2 = (double *)pre[1].p;
+        A.site = e->d_site;
     }
     // gene-tree moves: per-row trees, history draws, candidate scores (rows in the caller's order)
     DevBuf gb[12];

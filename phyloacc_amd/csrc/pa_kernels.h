@@ -57,11 +57,11 @@ struct LoglikArgs {
     // (gt_eig_kernel).
     const double *el_eig;     // [n_work][GB_EIG_WORDS]
     const double *el_pi;      // [n_work][4]
-    // Two site prefixes per work position, or null: the root sums use the
-    // pattern weights wpre[w][0][.] (into ll_out) and wpre[w][1][.] (into
-    // ll_out2) instead of the element's weights, each row wpre_stride long.
-    const int32_t *wpre;      // [n_work][2][wpre_stride]
-    int wpre_stride;
+    // Two site prefixes per caller row, or null: instead of the pattern sum,
+    // the root sums run over the element's sites, g < pre_len[r][0] into
+    // ll_out and g < pre_len[r][1] into ll_out2 (r = len_row[w], or w; the
+    // site -> pattern map is `site`).
+    const int32_t *pre_len;   // [rows][2]
     double *ll_out2;          // [n_work]
     // PhyloAcc-GT gene-tree moves.  Arrays below are per caller row: work
     // position w uses row len_row[w] (or w).
