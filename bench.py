@@ -638,25 +638,28 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
     phases = {"history": 0.0, "regraft": 0.0, "heights": 0.0, "pi": 0.0}
     acc = {"regraft": 0, "heights": 0, "pi": 0}
     moves = 0
+    kernel_heights = 0.0
     for m in range(sweeps + 1):
         timed = m > 0
         ind = np.zeros(n, np.int32)
         t0 = time.time()
-        hist = eng.gt_sample_history(sp, ids, ch, par, H, Z, rates, lens[:, 0], m, piA=piA, precision=prec)
+        eng.gt_sample_history(sp, ids, ch, par, H, Z, rates, lens[:, 0], m, piA=piA, precision=prec, to_host=False)
         t1 = time.time()
         # the (7m + i)-th gene node of element i, the root skipped
         root = np.argmax(par == N, axis=1)
         branch = (m * 7 + np.arange(n)) % (N - 1)
         branch = (branch + (branch >= root)).astype(np.int32)
-        a = eng.gt_sample_tree(sp, ids, ch, par, H, Z, rates, branch, ind, lens, hist, ll, lz, m, 0, piA=piA,
+        a = eng.gt_sample_tree(sp, ids, ch, par, H, Z, rates, branch, ind, lens, None, ll, lz, m, 0, piA=piA,
                                precision=prec)
         t2 = time.time()
         prior = np.array([E.gt_prior(sp, ch[i], par[i], H[i]) for i in range(n)])
         t2b = time.time()
         ah = 0
+        kms = 0.0
         for b, v in enumerate(range(S, N)):
             ah += int(eng.gt_branch_len(sp, ids, ch, par, H, Z, rates, v, ind, lens, ll, lz, prior, m, 1 + b,
                                         piA=piA, precision=prec).sum())
+            kms += eng.loglik_stats()["elapsed_ms"]
         t3 = time.time()
         L = np.stack([E.gt_lengths(sp, ch[i], par[i], H[i], Z[i], rates[i, 0], rates[i, 1]) for i in range(n)])
         t3b = time.time()
@@ -667,6 +670,7 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
             phases["history"] += t1 - t0
             phases["regraft"] += t2 - t1
             phases["heights"] += t3 - t2b
+            kernel_heights += kms
             phases["pi"] += t4 - t3b
             acc["regraft"] += int(a.sum())
             acc["heights"] += ah
@@ -678,6 +682,7 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
             "value": round(moves / tot, 1), "unit": "element moves/s", "sweeps": sweeps, "elements": n,
             "ms_per_sweep": round(tot / sweeps * 1e3, 1),
             "phase_ms": {k: round(v / sweeps * 1e3, 1) for k, v in phases.items()},
+            "heights_kernel_ms": round(kernel_heights / sweeps, 1),
             "launches_per_sweep": 1 + 2 + (N - S) + 1,
             "accept_rate": {k: round(v / (n * sweeps * ((N - S) if k == "heights" else 1)), 4)
                             for k, v in acc.items()},

@@ -277,7 +277,9 @@ pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *sp, const int32_
  * 1416-1532) for n gene-tree elements in one launch: after the pruning pass
  * at the gene trees' heights (gchildren, gparent, gheight as
  * pa_gt_branch_len; Z, rates, piA too), sites g < len[i] get the base of
- * every gene node in Tg[(i * N + v) * stride + g]: leaves their data (0-3
+ * every gene node.  The histories stay in the engine's device memory for the
+ * next pa_gt_sample_tree of the same ids (Tg NULL there); Tg, when not NULL,
+ * also receives them at Tg[(i * N + v) * stride + g]: leaves their data (0-3
  * a/c/g/t, -1 an ambiguity code, 4 no base), the gene root drawn from
  * pi * L_root, every other internal gene node from P(parent's base -> .) * L
  * over its whole branch (the law of the reference's steps through species
@@ -298,7 +300,9 @@ pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *sp, const in
  * root at 10 theta_root / 2 (or h(root) + 5 theta_root / 2) in the species
  * root; the walk stops after species root_ingrp once a candidate exists.
  * Each candidate is scored over the sites g < lens[2i + indicator[i]] with
- * the history Tg (as pa_gt_sample_history writes it, same stride) and
+ * the history Tg (as pa_gt_sample_history writes it, same stride; NULL: the
+ * device histories of the last pa_gt_sample_history, which must have had
+ * the same ids in the same order, else PA_E_STATE) and
  * branch's partial, all in one launch; a candidate is drawn with probability
  * proportional to exp(score), then the new sibling (one of its two
  * children: u < 1/2 the first) and bp's height uniformly from where both

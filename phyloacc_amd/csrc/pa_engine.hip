@@ -466,8 +466,8 @@ template <typename R, int BS>
 __device__ __noinline__ void gt_history(MP M, CP<R> c, AP A, int row)
 {
     const int N = c->N, S = c->S, Ppad = c->Ppad, root = c->root, n_nodes = c->n_nodes;
-    const int len = A->hist_len[row], stride = A->hist_stride;
-    GBL int8_t *out = (GBL int8_t *)A->hist + (size_t)row * N * stride;
+    const int len = A->hist_len[row], stride = A->hist_ns[row];
+    GBL int8_t *out = (GBL int8_t *)A->hist + A->hist_off[row];
     const GBL int32_t *site = c->site;
     const Key key{M->key0, M->key1};
     const uint32_t c2 = (uint32_t)A->hist_sweep, c3 = tagword(0, TAG_GT_HIST);
@@ -526,8 +526,8 @@ __device__ inline void full_transition(PE lam, PE V, PE Vi, double d, LDS double
 template <typename R, int BS>
 __device__ __noinline__ void gt_candidates(MP M, CP<R> c, AP A, int w, int row)
 {
-    const int N = c->N, S = c->S, Ppad = c->Ppad, stride = A->hist_stride, cm = A->cand_max;
-    const GBL int8_t *tg = (const GBL int8_t *)A->cand_tg + (size_t)row * N * stride;
+    const int S = c->S, Ppad = c->Ppad, stride = A->hist_ns[row], cm = A->cand_max;
+    const GBL int8_t *tg = (const GBL int8_t *)A->hist + A->hist_off[row];
     const int b = A->cand_branch[row], nc = A->cand_n[row], len = A->cand_sites[row];
     const GBL int32_t *site = c->site;
     const double *E = A->el_eig ? A->el_eig + (size_t)w * GB_EIG_WORDS : nullptr;
@@ -705,9 +705,9 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
         bind_element<R, BS>(M, c, A, w);
         unsigned long long bytes = 0;
         const double l = prune<R, BS>(M, c, A->counters && threadIdx.x == 0 ? &bytes : nullptr);
-        if (A->hist || A->cand_score) {
+        if (A->hist_draw || A->cand_score) {
             const int row = A->len_row ? A->len_row[w] : w;
-            if (A->hist) gt_history<R, BS>(M, c, A, row);
+            if (A->hist_draw) gt_history<R, BS>(M, c, A, row);
             if (A->cand_score) gt_candidates<R, BS>(M, c, A, w, row);
         }
         if (threadIdx.x == 0) {

@@ -11,6 +11,7 @@
 // species, on the way up from its first leaf, whose parent species is not
 // lower than the node, and a gene lineage crosses every species from its own
 // node's up to its parent's.
+#include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -29,8 +30,10 @@ namespace {
 
 struct Sp {
     int S, N;
-    const int32_t *up;     // parent species, N above the root
+    const int32_t *up;     // parent species, N above the root (the root is N - 1)
     const double *H, *theta;
+    std::vector<int> post;              // species in post-order
+    std::vector<int> kid0, kid1;        // children of each internal species
     double top(int s) const { return up[s] < N ? H[up[s]] : INFINITY; }
 };
 
@@ -54,8 +57,10 @@ struct Embed {
             if (par[v] == N) { if (root >= 0) return false; root = v; }
         if (root < 0) return false;
         // post-order from the root; a node's species continues its first child's climb
-        std::vector<uint8_t> seen(N, 0);
-        std::vector<std::pair<int, int>> st{{root, 0}};
+        thread_local std::vector<uint8_t> seen;
+        thread_local std::vector<std::pair<int, int>> st;
+        seen.assign(N, 0);
+        st.assign(1, {root, 0});
         seen[root] = 1;
         int leaves = 0;
         while (!st.empty()) {
@@ -82,6 +87,18 @@ struct Embed {
         for (int v : order) end[v] = v == root ? N - 1 : where[par[v]];
         return true;
     }
+    // the same tree with gene node v at height height[v] (children and
+    // parent unchanged, the new height between them): v's species and the
+    // end of its children's lineages follow
+    void moved(const Embed &g, int v, const double *height)
+    {
+        *this = g;
+        h = height;
+        int x = where[ch[2 * v]];
+        while (sp->up[x] < N && sp->H[sp->up[x]] < h[v]) x = sp->up[x];
+        where[v] = x;
+        end[ch[2 * v]] = end[ch[2 * v + 1]] = x;
+    }
     bool internal(int v) const { return v >= sp->S; }
     // lineage v crosses species x
     bool crosses(int v, int x) const
@@ -96,25 +113,35 @@ struct Embed {
 double prior_of(const Embed &g)
 {
     const Sp &s = *g.sp;
-    std::vector<int> enter(s.N, 0);
-    std::vector<std::vector<double>> born(s.N);
-    for (int v : g.order) {
-        if (g.internal(v)) born[g.where[v]].push_back(g.h[v]);
-        for (int x = g.where[v]; x != g.end[v] && s.up[x] < s.N;) {
-            x = s.up[x];
-            enter[x]++;
-        }
-    }
+    const int N = s.N;
+    // gene nodes born in each species (heights grouped by species), then the
+    // lineages entering each species from below: the sum over its child
+    // species of those leaving their tops (a leaf species sends its gene)
+    thread_local std::vector<int> cnt, off, enter, leave;
+    thread_local std::vector<double> hs;
+    cnt.assign(N, 0);
+    for (int v : g.order)
+        if (g.internal(v)) cnt[g.where[v]]++;
+    off.assign(N + 1, 0);
+    for (int x = 0; x < N; x++) off[x + 1] = off[x] + cnt[x];
+    hs.resize(off[N]);
+    for (int v : g.order)
+        if (g.internal(v)) hs[off[g.where[v] + 1] - cnt[g.where[v]]--] = g.h[v];
+    enter.assign(N, 0);
+    leave.assign(N, 0);
     double lp = 0;
-    for (int x = s.S; x < s.N; x++) {
-        int k = enter[x];
+    for (int x : s.post) {
+        if (x < s.S) { leave[x] = 1; continue; }
+        int k = leave[s.kid0[x]] + leave[s.kid1[x]];
+        const int a = off[x], b = off[x + 1];
+        leave[x] = k - (b - a);
         if (k == 0) continue;
-        std::sort(born[x].begin(), born[x].end());
+        std::sort(hs.begin() + a, hs.begin() + b);
         const double rate = 1.0 / s.theta[x], ldens = std::log(2.0 * rate);
         double t = s.H[x];
-        for (double b : born[x]) {
-            lp += ldens - (b - t) * k * (k - 1) * rate;
-            t = b;
+        for (int q = a; q < b; q++) {
+            lp += ldens - (hs[q] - t) * k * (k - 1) * rate;
+            t = hs[q];
             k--;
         }
         if (k > 1 && s.up[x] != s.N) lp -= (s.H[s.up[x]] - t) * k * (k - 1) * rate;
@@ -192,7 +219,32 @@ pa_status check_sp(pa_engine *e, const pa_species_tree *t, Sp &s)
             return engine_fail(e, PA_E_INPUT, "species node " + std::to_string(x) + ": bad parent or height");
         if (x >= s.S && !(s.theta[x] > 0)) return engine_fail(e, PA_E_INPUT, "theta must be positive on internal species");
     }
-    if (roots != 1) return engine_fail(e, PA_E_INPUT, "species tree must have one root");
+    if (roots != 1 || s.up[s.N - 1] != s.N) return engine_fail(e, PA_E_INPUT, "species tree must have one root, N - 1");
+    s.kid0.assign(s.N, -1);
+    s.kid1.assign(s.N, -1);
+    for (int x = 0; x < s.N - 1; x++) {
+        const int p = s.up[x];
+        if (s.kid0[p] < 0) s.kid0[p] = x;
+        else if (s.kid1[p] < 0) s.kid1[p] = x;
+        else return engine_fail(e, PA_E_INPUT, "species tree is not binary");
+    }
+    for (int x = s.S; x < s.N; x++)
+        if (s.kid1[x] < 0) return engine_fail(e, PA_E_INPUT, "species tree is not binary");
+    for (int x = 0; x < s.S; x++)
+        if (s.kid0[x] >= 0) return engine_fail(e, PA_E_INPUT, "species leaves must be 0..S-1");
+    s.post.clear();
+    std::vector<std::pair<int, int>> st{{s.N - 1, 0}};
+    while (!st.empty()) {
+        const int x = st.back().first, k = st.back().second;
+        if (x >= s.S && k < 2) {
+            st.back().second++;
+            st.push_back({k ? s.kid1[x] : s.kid0[x], 0});
+            continue;
+        }
+        s.post.push_back(x);
+        st.pop_back();
+    }
+    if ((int)s.post.size() != s.N) return engine_fail(e, PA_E_INPUT, "species tree does not reach every node");
     return PA_OK;
 }
 
@@ -337,7 +389,7 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
         newh[i].assign(gheight + (size_t)i * N, gheight + (size_t)(i + 1) * N);
         newh[i][gnode[i]] = prop[i];
         Embed g2;
-        g2.build(s, ch, pa, newh[i].data());
+        g2.moved(emb[i], gnode[i], newh[i].data());
         emb[i] = std::move(g2);
         lengths_of(emb[i], Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &rlen[(size_t)i * N]);
     });
@@ -504,7 +556,9 @@ void tree_candidates(const Embed &g, int branch, int root_ingrp, const int32_t *
 
 }  // namespace
 
-// Batched GTree::Update_Tg with tosample = true.
+// Batched GTree::Update_Tg with tosample = true: the histories stay in the
+// engine's device arena (rows packed, node-major) for the next regraft; Tg,
+// when given, also receives them in the caller's [n][N][stride] layout.
 extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t, const int32_t *ids, int32_t n,
                                           const int32_t *gchildren, const int32_t *gparent, const double *gheight,
                                           const int32_t *Z, const double *rates, const double *piA,
@@ -512,9 +566,9 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
                                           int32_t stride)
 {
     if (!e) return PA_E_INPUT;
-    if (n < 0 || (n > 0 && (!ids || !gchildren || !gparent || !gheight || !Z || !rates || !len || !Tg)))
+    if (n < 0 || (n > 0 && (!ids || !gchildren || !gparent || !gheight || !Z || !rates || !len)))
         return engine_fail(e, PA_E_INPUT, "null argument");
-    if (stride <= 0) return engine_fail(e, PA_E_INPUT, "stride must be positive");
+    if (Tg && stride <= 0) return engine_fail(e, PA_E_INPUT, "stride must be positive");
     Sp s;
     pa_status st = check_sp(e, t, s);
     if (st != PA_OK) return st;
@@ -524,9 +578,16 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
     if (n == 0) return PA_OK;
     const int N = s.N;
     std::vector<double> lens((size_t)n * N);
+    std::vector<int64_t> off(n);
+    std::vector<int32_t> ns(n);
+    int64_t total = 0;
     for (int i = 0; i < n; i++) {
         uint8_t lm[512];
         if ((st = element_gene_tree(e, ids[i], lm, gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
+        if (len[i] < 0 || (Tg && len[i] > stride)) return engine_fail(e, PA_E_INPUT, "history length negative or beyond stride");
+        off[i] = total;
+        ns[i] = len[i];
+        total += (int64_t)N * len[i];
     }
     std::vector<int> bad(n, 0);
     for_elements(n, [&](int i) {
@@ -536,13 +597,32 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
     });
     for (int i = 0; i < n; i++)
         if (bad[i]) return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
+    GtHistory &H = engine_history(e);
+    H.valid = false;
+    if ((st = engine_history_reserve(e, (size_t)std::max<int64_t>(total, 8))) != PA_OK) return st;
     GtExtra x;
-    x.hist = Tg;
+    x.d_hist = H.d;
+    x.hist_off = off.data();
+    x.hist_ns = ns.data();
     x.hist_len = len;
-    x.hist_stride = stride;
+    x.hist_draw = 1;
     x.hist_sweep = m;
     std::vector<double> o(n), o1(n);
-    return gt_launch(e, ids, n, lens.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x);
+    if ((st = gt_launch(e, ids, n, lens.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x)) != PA_OK)
+        return st;
+    H.ids.assign(ids, ids + n);
+    H.len.assign(len, len + n);
+    H.off = off;
+    H.valid = true;
+    if (Tg) {
+        std::vector<int8_t> host((size_t)std::max<int64_t>(total, 1));
+        if (hipMemcpy(host.data(), H.d, (size_t)total, hipMemcpyDeviceToHost) != hipSuccess)
+            return engine_fail(e, PA_E_DEVICE, "history copy failed");
+        for (int i = 0; i < n; i++)
+            for (int v = 0; v < N; v++)
+                memcpy(Tg + ((size_t)i * N + v) * stride, &host[(size_t)off[i] + (size_t)v * len[i]], (size_t)len[i]);
+    }
+    return PA_OK;
 }
 
 // Batched GTree::Sample_tree2 (src/PhyloAcc-GT/genetree.cpp:2138-2518).
@@ -556,7 +636,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
 {
     if (!e) return PA_E_INPUT;
     if (n < 0 || (n > 0 && (!ids || !gchildren || !gparent || !gheight || !Z || !rates || !branch || !indicator ||
-                            !lens || !Tg || !loglik || !logpz || !accepted)))
+                            !lens || !loglik || !logpz || !accepted)))
         return engine_fail(e, PA_E_INPUT, "null argument");
     Sp s;
     pa_status st = check_sp(e, t, s);
@@ -583,7 +663,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * s.S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
         const int ind = indicator[i];
         if (ind != 0 && ind != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
-        if (lens[2 * i + ind] < 0 || lens[2 * i + 1] < 0 || lens[2 * i + ind] > stride)
+        if (lens[2 * i + ind] < 0 || lens[2 * i + 1] < 0 || (Tg && lens[2 * i + ind] > stride))
             return engine_fail(e, PA_E_INPUT, "prefix length negative or beyond the history's stride");
         if (branch[i] < 0 || branch[i] >= N || gparent[(size_t)i * N + branch[i]] == N)
             return engine_fail(e, PA_E_INPUT, "branch must be a gene node other than the root");
@@ -616,14 +696,44 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         }
     }
     {
+        // the histories: the engine's last draw (Tg null), or the caller's
         GtExtra x;
-        x.hist_stride = stride;
-        x.cand_tg = Tg;
+        std::vector<int64_t> off(n);
+        std::vector<int32_t> ns(n);
+        int8_t *dh = nullptr;
+        if (!Tg) {
+            GtHistory &Hs = engine_history(e);
+            if (!Hs.valid || (int)Hs.ids.size() != n || !std::equal(Hs.ids.begin(), Hs.ids.end(), ids))
+                return engine_fail(e, PA_E_STATE, "no device history for these elements: call pa_gt_sample_history "
+                                                  "with the same ids first, or pass Tg");
+            for (int i = 0; i < n; i++) {
+                if (Hs.len[i] < csites[i]) return engine_fail(e, PA_E_INPUT, "the device history covers fewer sites");
+                off[i] = Hs.off[i];
+                ns[i] = Hs.len[i];
+            }
+            dh = Hs.d;
+        } else {
+            int64_t total = 0;
+            for (int i = 0; i < n; i++) { off[i] = total; ns[i] = csites[i]; total += (int64_t)N * csites[i]; }
+            std::vector<int8_t> host((size_t)std::max<int64_t>(total, 1));
+            for (int i = 0; i < n; i++)
+                for (int v = 0; v < N; v++)
+                    memcpy(&host[(size_t)off[i] + (size_t)v * ns[i]], Tg + ((size_t)i * N + v) * stride, (size_t)ns[i]);
+            // a separate arena: the engine's history stays as it was
+            if (hipMalloc(&dh, host.size()) != hipSuccess || hipMemcpy(dh, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipFree(dh);
+                return engine_fail(e, PA_E_DEVICE, "history upload failed");
+            }
+        }
+        x.d_hist = dh;
+        x.hist_off = off.data();
+        x.hist_ns = ns.data();
         x.cand_branch = cb.data(); x.cand_n = cn.data(); x.cand_node = cnode.data(); x.cand_kind = ckind.data();
         x.cand_sites = csites.data(); x.cand_len = clen.data(); x.cand_max = cmax; x.cand_score = cscore.data();
         std::vector<double> o(n), o1(n);
-        if ((st = gt_launch(e, ids, n, cur_len.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x)) != PA_OK)
-            return st;
+        st = gt_launch(e, ids, n, cur_len.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x);
+        if (Tg) (void)hipFree(dh);
+        if (st != PA_OK) return st;
     }
     // draw each element's regraft; build the proposed trees
     std::vector<std::vector<int32_t>> nch(n), npar(n);

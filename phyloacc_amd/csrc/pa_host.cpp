@@ -473,6 +473,7 @@ struct pa_engine {
     int max_ppad = 0;
     double mean_p = 0;
     std::vector<ElemDev> dev_elems;          // host copy of the uploaded descriptors (tree updates patch both)
+    GtHistory hist;                          // gene-node histories of the last history draw (device)
 };
 
 static pa_status fail(pa_engine *e, pa_status st, const std::string &msg)
@@ -625,6 +626,7 @@ extern "C" void pa_engine_destroy(pa_engine *e)
         (void)hipFree(e->d_queue);
         (void)hipFree(e->d_ll_scratch);
         (void)hipFree(e->d_ll_len);
+        (void)hipFree(e->hist.d);
         if (e->ll_t0) (void)hipEventDestroy(e->ll_t0);
         if (e->ll_t1) (void)hipEventDestroy(e->ll_t1);
         (void)hipStreamDestroy(e->stream);
@@ -1269,17 +1271,21 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
             A.tab_call = (const int32_t *)gb[0].p;
             A.hdr_call = (const int32_t *)gb[1].p;
         }
-        A.hist_stride = gx->hist_stride;
-        if (gx->hist) {
-            HIPCHK(e, hipMalloc(&gb[2].p, std::max<size_t>((size_t)n * N * gx->hist_stride, 8)));
-            HIPCHK(e, up(gb[3], gx->hist_len, (size_t)n * 4));
-            A.hist = (int8_t *)gb[2].p;
-            A.hist_len = (const int32_t *)gb[3].p;
-            A.hist_sweep = gx->hist_sweep;
+        if (gx->d_hist) {
+            HIPCHK(e, up(gb[2], gx->hist_off, (size_t)n * 8));
+            HIPCHK(e, up(gb[3], gx->hist_ns, (size_t)n * 4));
+            A.hist = gx->d_hist;
+            A.hist_off = (const int64_t *)gb[2].p;
+            A.hist_ns = (const int32_t *)gb[3].p;
+            if (gx->hist_draw) {
+                HIPCHK(e, up(gb[4], gx->hist_len, (size_t)n * 4));
+                A.hist_len = (const int32_t *)gb[4].p;
+                A.hist_draw = 1;
+                A.hist_sweep = gx->hist_sweep;
+            }
         }
         if (gx->cand_score) {
             const size_t rows = (size_t)n * gx->cand_max;
-            HIPCHK(e, up(gb[4], gx->cand_tg, (size_t)n * N * gx->hist_stride));
             HIPCHK(e, up(gb[5], gx->cand_branch, (size_t)n * 4));
             HIPCHK(e, up(gb[6], gx->cand_n, (size_t)n * 4));
             HIPCHK(e, up(gb[7], gx->cand_node, rows * 4));
@@ -1287,7 +1293,6 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
             HIPCHK(e, up(gb[9], gx->cand_sites, (size_t)n * 4));
             HIPCHK(e, up(gb[10], gx->cand_len, rows * 8));
             HIPCHK(e, hipMalloc(&gb[11].p, std::max<size_t>(rows * 8, 8)));
-            A.cand_tg = (const int8_t *)gb[4].p;
             A.cand_branch = (const int32_t *)gb[5].p;
             A.cand_n = (const int32_t *)gb[6].p;
             A.cand_node = (const int32_t *)gb[7].p;
@@ -1307,7 +1312,6 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
         HIPCHK(e, hipMemcpy(ll.data(), pre[1].p, (size_t)n * 8, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; i++) out1[order[i]] = ll[i];
     }
-    if (gx && gx->hist) HIPCHK(e, hipMemcpy(gx->hist, gb[2].p, (size_t)n * N * gx->hist_stride, hipMemcpyDeviceToHost));
     if (gx && gx->cand_score)
         HIPCHK(e, hipMemcpy(gx->cand_score, gb[11].p, (size_t)n * gx->cand_max * 8, hipMemcpyDeviceToHost));
     if (e->ll_scratch_bytes + e->ll_len_bytes > PA_RETAIN_MAX) release_ll_buffers(e);
@@ -1390,6 +1394,23 @@ pa_status element_set_tree(pa_engine *e, int32_t id, const int32_t *children)
     return PA_OK;
 }
 
+GtHistory &engine_history(pa_engine *e) { return e->hist; }
+
+pa_status engine_history_reserve(pa_engine *e, size_t bytes)
+{
+    pa_status st = ensure_device(e);
+    if (st != PA_OK) return st;
+    if (bytes <= e->hist.cap) return PA_OK;
+    HIPCHK(e, hipDeviceSynchronize());
+    (void)hipFree(e->hist.d);
+    e->hist.d = nullptr;
+    e->hist.cap = 0;
+    e->hist.valid = false;
+    HIPCHK(e, hipMalloc(&e->hist.d, bytes));
+    e->hist.cap = bytes;
+    return PA_OK;
+}
+
 pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths, const double *piA,
                     const int32_t *lens0, const int32_t *lens1, int32_t precision, double *out0, double *out1,
                     const GtExtra &x)
@@ -1401,9 +1422,10 @@ pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *l
         const int GG = e->elems[ids[i]].GG;
         if (lens0 && (lens1[i] < 0 || lens1[i] > lens0[i] || lens0[i] > GG))
             return fail(e, PA_E_INPUT, "prefix lengths must satisfy 0 <= lens1 <= lens0 <= the element's sites");
-        if (x.hist && (x.hist_len[i] < 0 || x.hist_len[i] > GG || x.hist_len[i] > x.hist_stride))
-            return fail(e, PA_E_INPUT, "history length beyond the element's sites or the stride");
-        if (x.cand_score && (x.cand_sites[i] < 0 || x.cand_sites[i] > GG || x.cand_sites[i] > x.hist_stride ||
+        if (x.hist_draw && (x.hist_len[i] < 0 || x.hist_len[i] > GG || x.hist_len[i] > x.hist_ns[i]))
+            return fail(e, PA_E_INPUT, "history length beyond the element's sites or its row");
+        if ((x.hist_draw || x.cand_score) && !x.d_hist) return fail(e, PA_E_INPUT, "no history arena");
+        if (x.cand_score && (x.cand_sites[i] < 0 || x.cand_sites[i] > GG || x.cand_sites[i] > x.hist_ns[i] ||
                              x.cand_n[i] < 0 || x.cand_n[i] > x.cand_max))
             return fail(e, PA_E_INPUT, "candidate sites or count out of range");
         if (x.cand_score)

@@ -3,6 +3,7 @@
 #pragma once
 #include <stdint.h>
 #include <string>
+#include <vector>
 #include "../../include/phyloacc_gpu.h"
 
 namespace pa {
@@ -20,19 +21,33 @@ pa_status engine_fail(pa_engine *e, pa_status st, const std::string &msg);
 // have the children in children [N*2] (either order) when that is given.
 pa_status element_gene_tree(pa_engine *e, int32_t id, uint8_t *leaf_missing, const int32_t *children);
 
-// Extra work of a gene-tree likelihood launch; arrays are per caller row.
+// Extra work of a gene-tree likelihood launch; arrays are per caller row
+// (host memory unless marked device).
 struct GtExtra {
     const int32_t *tabs = nullptr, *hdrs = nullptr;   // per-row trees [n][tab_words(N)], [n][4] (gene_table)
-    int8_t *hist = nullptr;                           // out: history [n][N][hist_stride]
-    const int32_t *hist_len = nullptr;
-    int hist_stride = 0, hist_sweep = 0;
-    const int8_t *cand_tg = nullptr;                  // history [n][N][hist_stride] read by the scores
+    int8_t *d_hist = nullptr;                         // device: the histories (row r at hist_off[r],
+    const int64_t *hist_off = nullptr;                //   node v at v * hist_ns[r] from there)
+    const int32_t *hist_ns = nullptr;
+    const int32_t *hist_len = nullptr;                // draw mode: sites to draw per row
+    int hist_draw = 0, hist_sweep = 0;
     const int32_t *cand_branch = nullptr, *cand_n = nullptr, *cand_node = nullptr, *cand_kind = nullptr,
                   *cand_sites = nullptr;
     const double *cand_len = nullptr;
     int cand_max = 0;
     double *cand_score = nullptr;                     // out: [n][cand_max]
 };
+// The engine's gene-node histories: the device arena of the last
+// pa_gt_sample_history call and which rows it holds.
+struct GtHistory {
+    int8_t *d = nullptr;
+    size_t cap = 0;
+    bool valid = false;
+    std::vector<int32_t> ids, len;
+    std::vector<int64_t> off;
+};
+GtHistory &engine_history(pa_engine *e);
+// grows the history arena to at least `bytes` (device memory)
+pa_status engine_history_reserve(pa_engine *e, size_t bytes);
 // One gene-tree likelihood launch (pa_loglik_lengths_split's, lens0 may be
 // null) with the extra work of x.
 pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths, const double *piA,

@@ -71,20 +71,22 @@ struct LoglikArgs {
     const int32_t *tab_call;
     const int32_t *hdr_call;  // [rows][4]
     const int32_t *site;      // the site -> pattern arena (ElemDev::off_site)
-    // History draws, or null: after pruning, sites g < hist_len[r] get the
-    // base of every node in hist[r][v][g] (hist_stride per node; leaves
-    // 0-3, -1 ambiguous, 4 no base; internal nodes drawn top-down, 4 when
-    // missing), draws of sweep hist_sweep (tag GT_HIST, sub-stream g, slot v).
+    // Gene-node histories, or null: row r's lives at hist + hist_off[r],
+    // node v's sites at v * hist_ns[r] from there (int8: leaves 0-3, -1
+    // ambiguous, 4 no base; internal nodes their drawn base, 4 when missing).
+    // hist_draw: after pruning, draw sites g < hist_len[r] there (tag
+    // GT_HIST, sub-stream g, sweep hist_sweep, slot v); else the history is
+    // only read, by the candidate scores.
     int8_t *hist;
-    const int32_t *hist_len;
-    int hist_stride, hist_sweep;
+    const int64_t *hist_off;
+    const int32_t *hist_ns, *hist_len;
+    int hist_draw, hist_sweep;
     // Candidate scores, or null: for row r, branch cand_branch[r] and
     // candidates q < cand_n[r] (node cand_node, kind cand_kind, distance
     // cand_len; cand_max per row), sum over sites g < cand_sites[r] the score
     // of kind 0 (node's base, or the pi-average when it has none), 1 (the
     // pi-average) or 2 (the branch's own base's pi-row) into cand_score,
-    // reading the history cand_tg (hist_stride per node).
-    const int8_t *cand_tg;
+    // reading the history.
     const int32_t *cand_branch, *cand_n, *cand_node, *cand_kind, *cand_sites;
     const double *cand_len;
     int cand_max;

@@ -468,9 +468,10 @@ class Engine:
         return acc
 
     def gt_sample_history(self, sp, ids, children, parent, heights, Z, rates, length, m, piA=None,
-                          precision=PA_FP64, stride=None):
-        """Batched Update_Tg(tosample=true): int8 [n, N, stride] histories of
-        the first length[i] sites."""
+                          precision=PA_FP64, stride=None, to_host=True):
+        """Batched Update_Tg(tosample=true): the histories of the first
+        length[i] sites stay on the device for the next gt_sample_tree of the
+        same ids; with to_host also returned as int8 [n, N, stride]."""
         ids = np.ascontiguousarray(ids, np.int32)
         n = len(ids)
         ch = np.ascontiguousarray(children, np.int32).reshape(n, self.N, 2)
@@ -481,11 +482,11 @@ class Engine:
         ln = np.ascontiguousarray(np.broadcast_to(length, (n,)), np.int32)
         stride = int(max(1, ln.max() if n else 1) if stride is None else stride)
         a = None if piA is None else np.ascontiguousarray(piA, np.float64)
-        out = np.zeros((n, self.N, stride), np.int8)
+        out = np.zeros((n, self.N, stride), np.int8) if to_host else None
         self._check(lib().pa_gt_sample_history(
             self._h, C.byref(sp), _p(ids, C.c_int32), n, _p(ch, C.c_int32), _p(par, C.c_int32), _p(h, C.c_double),
             _p(z, C.c_int32), _p(r, C.c_double), None if a is None else _p(a, C.c_double), _p(ln, C.c_int32), int(m),
-            int(precision), _p(out, C.c_int8), stride))
+            int(precision), None if out is None else _p(out, C.c_int8), stride))
         return out
 
     def gt_sample_tree(self, sp, ids, children, parent, heights, Z, rates, branch, indicator, lens, Tg, loglik,
@@ -493,8 +494,9 @@ class Engine:
                        precision=PA_FP64):
         """One batched Sample_tree2 move.  children [n, N, 2], parent,
         heights [n, N], loglik, logpz [n, 2] are updated in place (C order;
-        int32 / float64); Tg [n, N, stride] as gt_sample_history returns it.
-        Returns accepted [n]."""
+        int32 / float64); Tg [n, N, stride] as gt_sample_history returns it,
+        or None for the device histories of the last gt_sample_history of
+        these ids.  Returns accepted [n]."""
         ids = np.ascontiguousarray(ids, np.int32)
         n = len(ids)
         for a, t in ((children, np.int32), (parent, np.int32), (heights, np.float64), (loglik, np.float64),
@@ -506,15 +508,16 @@ class Engine:
         b = np.ascontiguousarray(np.broadcast_to(branch, (n,)), np.int32)
         ind = np.ascontiguousarray(np.broadcast_to(indicator, (n,)), np.int32)
         ln = np.ascontiguousarray(np.broadcast_to(lens, (n, 2)), np.int32)
-        T = np.ascontiguousarray(Tg, np.int8)
-        assert T.shape[:2] == (n, self.N)
+        T = None if Tg is None else np.ascontiguousarray(Tg, np.int8)
+        assert T is None or T.shape[:2] == (n, self.N)
         a = None if piA is None else np.ascontiguousarray(piA, np.float64)
         acc = np.zeros(n, np.int32)
         ri = self.N - 1 if root_ingrp is None else int(root_ingrp)
         self._check(lib().pa_gt_sample_tree(
             self._h, C.byref(sp), _p(ids, C.c_int32), n, _p(children, C.c_int32), _p(parent, C.c_int32),
             _p(heights, C.c_double), _p(z, C.c_int32), _p(r, C.c_double), None if a is None else _p(a, C.c_double),
-            _p(b, C.c_int32), _p(ind, C.c_int32), _p(ln, C.c_int32), _p(T, C.c_int8), T.shape[2],
+            _p(b, C.c_int32), _p(ind, C.c_int32), _p(ln, C.c_int32), None if T is None else _p(T, C.c_int8),
+            0 if T is None else T.shape[2],
             float(cons_to_mis), float(ncons_to_mis), ri, int(m), int(slot), int(precision), _p(loglik, C.c_double),
             _p(logpz, C.c_double), _p(acc, C.c_int32)))
         return acc

@@ -68,8 +68,9 @@ def test_history_and_tree_moves_match_the_oracle(require_gpu, use_pi):
             np.testing.assert_array_equal(hist[c, :, :L[c]], ho, err_msg=f"history m={m} c={c}")
         branch = np.array([[v for v in range(N) if par[c, v] != N][(m * 3 + c) % (N - 1)] for c in range(n)], np.int32)
         ch0 = ch.copy()
-        acc = eng.gt_sample_tree(sp, ids, ch, par, H, w.Z, rates, branch, ind, lens, hist, ll, lz, m, 0,
-                                 cons_to_mis=c2m, ncons_to_mis=n2m, piA=piA)
+        # one variant reads the histories the device kept, the other the caller's copy
+        acc = eng.gt_sample_tree(sp, ids, ch, par, H, w.Z, rates, branch, ind, lens, None if use_pi else hist, ll,
+                                 lz, m, 0, cons_to_mis=c2m, ncons_to_mis=n2m, piA=piA)
         for c in range(n):
             code = O.gt_sample_tree(m_or, osp, og[c], w.Z[c], w.r1[c], w.r2[c], c2m, n2m, branch[c], int(ind[c]),
                                     lens[c], hist[c, :, :L[c]], oll[c], olz[c], SEED, m, 0, w.align, w.starts[c],
@@ -121,6 +122,10 @@ def test_tree_move_input_checks(require_gpu):
     ll, lz = np.zeros((n, 2)), np.zeros((n, 2))
     with pytest.raises(E.PaError):                       # the root is no branch
         eng.gt_sample_tree(sp, ids, ch, par, H, w.Z, rates, N - 1, 0, lens, hist, ll, lz, 0, 0)
+    with pytest.raises(E.PaError) as ei:                 # no device history for other ids
+        eng.gt_sample_tree(sp, ids[::-1].copy(), ch[::-1].copy(), par[::-1].copy(), H[::-1].copy(), w.Z[::-1],
+                           rates[::-1], 0, 0, lens, None, ll, lz, 0, 0)
+    assert ei.value.status == E.PA_E_STATE
     with pytest.raises(E.PaError):                       # one element twice
         two = np.array([ids[0], ids[0]], np.int32)
         eng.gt_sample_tree(sp, two, ch[[0, 0]].copy(), par[[0, 0]].copy(), H[[0, 0]].copy(), w.Z[[0, 0]], rates[[0, 0]],
