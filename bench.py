@@ -27,7 +27,9 @@ Also reported (same JSON line):
                  throughput and p50/p99 completion latency;
   config5      - 20,000 elements with their own 120-species gene trees:
                  batched log-likelihood evaluations/s, FP32 and FP64 errors
-                 against the oracle on a sample.
+                 against the oracle on a sample, and one sweep of the
+                 PhyloAcc-GT gene-tree moves (history, regraft, every node
+                 height, pi) over all of them.
 """
 from __future__ import annotations
 
@@ -197,6 +199,8 @@ def main():
         blocks["config4"] = serve(args, E, synth, None, 2000.0, 2000, args.lanes)
         blocks["config5"] = gene_trees_block(
             E, gt_w, traffic=measured_traffic(args, None, f"config=5 elements={gt_w.C} seed={args.seed} precision=fp32")[0])
+        if args.gt_sweeps > 0:
+            blocks["config5"]["gene_tree_moves"] = gt_moves_block(E, gt_w, args.gt_sweeps)
         del gt_w
         for k, v in blocks.items():
             log(f"{k}: {v}")
