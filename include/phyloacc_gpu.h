@@ -450,8 +450,10 @@ pa_status pa_set_tuning(pa_engine *e, const pa_tuning *t);
 pa_status pa_get_tuning(pa_engine *e, pa_tuning *t);
 /* Times every block size (64, 128, 256) and shape on the batch (or, with
  * sample > 0, on `sample` of its elements evenly spaced by pattern count)
- * with chains of `states` states (0: 8) and keeps the fastest pair for later
- * pa_run_chains / pa_batch_create calls with block_threads = 0. */
+ * with chains of `states` states (0: 8), times the three fastest again twice
+ * at 2 * states and keeps the best of those (rate: its iterations per second
+ * there) for later pa_run_chains / pa_batch_create calls with
+ * block_threads = 0. */
 pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, int32_t precision, int32_t sample,
                       int32_t states, pa_tuning *out);
 

@@ -2145,27 +2145,47 @@ extern "C" pa_status pa_autotune(pa_engine *e, const int32_t *ids, int32_t n, in
     pa_tuning best = {0, 0, 0, 0, -1.0};
     const int cand[3] = {64, 128, 256};
     const pa_tuning keep = e->tune;
+    // one timed launch of `T`-state chains at block size bs and shape sh
+    auto time_one = [&](int bs, int sh, int T, double *rate) -> pa_status {
+        DevBuf bufs[8];
+        pa_run_stats rs;
+        chain::ChainArgs A;
+        memset(&A, 0, sizeof A);
+        if (hipMalloc(&bufs[3].p, (size_t)m * 3 * chain::RES_D * 8) != hipSuccess ||
+            hipMalloc(&bufs[4].p, (size_t)m * 3 * chain::RES_I * 4) != hipSuccess ||
+            hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) return fail(e, PA_E_OOM, "autotune buffers");
+        A.res_d = (double *)bufs[3].p;
+        A.res_i = (int32_t *)bufs[4].p;
+        A.maxZ = (int8_t *)bufs[5].p;
+        e->tune.shape = sh;
+        e->M.T = T;
+        e->M.burnin = T / 4;
+        e->M.mcmc = T - T / 4;
+        pa_status r = run_chain_kernel(e, sorted.data(), m, precision == PA_FP64, PA_RUN_BATCHED, bs, A, bufs, &rs);
+        if (r == PA_OK) *rate = (double)m * 3 * (T - 1) / (rs.elapsed_ms / 1e3);
+        return r;
+    };
+    // every (block, shape) once, then the three fastest again at twice the
+    // chain length, twice each, keeping each one's better time: neighbours
+    // that differ by a percent or two are then told apart reliably
+    std::vector<std::pair<double, std::pair<int, int>>> first;
     for (int a = 0; a < 3 && st == PA_OK; a++)
         for (int sh = 0; sh < chain::N_SHAPES && st == PA_OK; sh++) {
-            DevBuf bufs[8];
-            pa_run_stats rs;
-            chain::ChainArgs A;
-            memset(&A, 0, sizeof A);
-            if (hipMalloc(&bufs[3].p, (size_t)m * 3 * chain::RES_D * 8) != hipSuccess ||
-                hipMalloc(&bufs[4].p, (size_t)m * 3 * chain::RES_I * 4) != hipSuccess ||
-                hipMalloc(&bufs[5].p, (size_t)m * 3 * N) != hipSuccess) { st = fail(e, PA_E_OOM, "autotune buffers"); break; }
-            A.res_d = (double *)bufs[3].p;
-            A.res_i = (int32_t *)bufs[4].p;
-            A.maxZ = (int8_t *)bufs[5].p;
-            e->tune.shape = sh;
-            st = run_chain_kernel(e, sorted.data(), m, precision == PA_FP64, PA_RUN_BATCHED, cand[a], A, bufs, &rs);
-            if (st != PA_OK) break;
-            const double rate = (double)m * 3 * (states - 1) / (rs.elapsed_ms / 1e3);
-            if (rate > best.rate) {
-                best = {cand[a], sh, 0, 0, rate};
-                shape_dims(precision == PA_FP64, sh, &best.round_nodes, &best.chunk_nodes);
-            }
+            double rate = 0;
+            if ((st = time_one(cand[a], sh, states, &rate)) == PA_OK) first.push_back({rate, {cand[a], sh}});
         }
+    std::sort(first.begin(), first.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    for (size_t q = 0; q < first.size() && q < 3 && st == PA_OK; q++) {
+        double r = 0;
+        for (int rep = 0; rep < 2 && st == PA_OK; rep++) {
+            double x = 0;
+            if ((st = time_one(first[q].second.first, first[q].second.second, 2 * states, &x)) == PA_OK) r = std::max(r, x);
+        }
+        if (st == PA_OK && r > best.rate) {
+            best = {first[q].second.first, first[q].second.second, 0, 0, r};
+            shape_dims(precision == PA_FP64, best.shape, &best.round_nodes, &best.chunk_nodes);
+        }
+    }
     e->tune = keep;
     e->M = saved;
     if (st != PA_OK) return st;
