@@ -339,7 +339,7 @@ pa_status pa_loglik_gbgc(pa_engine *e, const int32_t *ids, int32_t n, const int3
                          const double *sel, int32_t precision, double *out);
 
 /* pa_loglik and pa_loglik_lengths keep their device scratch (workgroup slots
- * and per-call length buffer) for the next call when it is at most 2 GiB;
+ * and per-call length buffer) for the next call when it is at most 8 GiB;
  * this frees it now (no reference counterpart: the reference allocates per
  * element on the host).  Call it before a large pa_run_chains or
  * pa_serve_create when device memory is tight. */

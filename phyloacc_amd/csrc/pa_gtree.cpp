@@ -87,13 +87,11 @@ struct Embed {
         for (int v : order) end[v] = v == root ? N - 1 : where[par[v]];
         return true;
     }
-    // the same tree with gene node v at height height[v] (children and
-    // parent unchanged, the new height between them): v's species and the
-    // end of its children's lineages follow
-    void moved(const Embed &g, int v, const double *height)
+    // gene node v's height has changed in place (children and parent
+    // unchanged, the new height between them): v's species and the end of its
+    // children's lineages follow
+    void repoint(int v)
     {
-        *this = g;
-        h = height;
         int x = where[ch[2 * v]];
         while (sp->up[x] < N && sp->H[sp->up[x]] < h[v]) x = sp->up[x];
         where[v] = x;
@@ -366,32 +364,37 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
         if (indicator[i] != 0 && indicator[i] != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
         if (lens[2 * i + 1] < 0 || lens[2 * i + indicator[i]] < 0) return engine_fail(e, PA_E_INPUT, "negative prefix length");
     }
-    std::vector<Embed> emb(n);
     std::vector<HeightMove> win(n);
-    std::vector<std::vector<uint8_t>> miss(n);
-    std::vector<std::vector<double>> newh(n);
-    std::vector<double> prop(n, 0.0), rdraw(n, 0.0), rlen((size_t)n * N, 0.0);
+    std::vector<double> prop(n, 0.0), rdraw(n, 0.0), prior_new(n, 0.0), lz_new(n, 0.0);
+    std::vector<double> rlen((size_t)n * N, 0.0);
     std::vector<int> bad(n, 0);
     const Key key{ev.key0, ev.key1};
     const uint32_t c3 = tagword(0, TAG_GT_BRANCH), c2 = (uint32_t)m;
-    // per element: window, draws, and the proposal's branch lengths
+    // per element, with one reused embedding per host thread: the window and
+    // draws, then, with the proposed height written into the caller's row for
+    // the moment, the proposal's branch lengths, prior and missing-gene term
     for_elements(n, [&](int i) {
         accepted[i] = 0;
+        thread_local Embed g;
+        thread_local std::vector<uint8_t> miss;
         const int32_t *ch = gchildren + (size_t)i * 2 * N, *pa = gparent + (size_t)i * N;
-        if (!emb[i].build(s, ch, pa, gheight + (size_t)i * N)) { bad[i] = 1; return; }
-        missing_of(emb[i], &lm[(size_t)i * s.S], miss[i]);
-        win[i] = height_window(emb[i], gnode[i], delta);
+        double *h = gheight + (size_t)i * N;
+        g.root = -1;
+        if (!g.build(s, ch, pa, h)) { bad[i] = 1; return; }
+        win[i] = height_window(g, gnode[i], delta);
         if (win[i].status < 0) { bad[i] = 2; return; }
         if (win[i].status != 0) return;
         const double u = slot_uniform(key, (uint32_t)slot, c2, c3, 0);
         prop[i] = std::max(u, 1e-5) * (win[i].upper - win[i].lower) + win[i].lower;
         rdraw[i] = slot_uniform(key, (uint32_t)slot, c2, c3, 1);
-        newh[i].assign(gheight + (size_t)i * N, gheight + (size_t)(i + 1) * N);
-        newh[i][gnode[i]] = prop[i];
-        Embed g2;
-        g2.moved(emb[i], gnode[i], newh[i].data());
-        emb[i] = std::move(g2);
-        lengths_of(emb[i], Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &rlen[(size_t)i * N]);
+        const double old = h[gnode[i]];
+        h[gnode[i]] = prop[i];
+        g.repoint(gnode[i]);
+        lengths_of(g, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &rlen[(size_t)i * N]);
+        prior_new[i] = prior_of(g);
+        missing_of(g, &lm[(size_t)i * s.S], miss);
+        lz_new[i] = logpz_of(g, miss, Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
+        h[gnode[i]] = old;
     });
     for (int i = 0; i < n; i++) {
         if (bad[i] == 1) return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
@@ -423,26 +426,24 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
     if ((st = pa_loglik_lengths_split(e, rid.data(), k, lenp, piA ? rpi.data() : nullptr, l0.data(), l1.data(),
                                       precision, o0.data(), o1.data())) != PA_OK)
         return st;
-    for_elements(k, [&](int j) {
+    for (int j = 0; j < k; j++) {
         const int i = run[j], ind = indicator[i];
         const HeightMove &w = win[i];
-        const double lz = logpz_of(emb[i], miss[i], Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
-        const double pr = prior_of(emb[i]);
         const double back = std::min(w.upper1, prop[i] + w.half) - std::max(w.lower1, prop[i] - w.half);
         const double lnew[2] = {o0[j], o1[j]};
         double d = lnew[ind] - loglik[2 * i + ind] + std::log(w.upper - w.lower) - std::log(back);
-        if (logpz[2 * i + ind] != 0) d += lz - logpz[2 * i + ind];
-        d += pr - prior[i];
+        if (logpz[2 * i + ind] != 0) d += lz_new[i] - logpz[2 * i + ind];
+        d += prior_new[i] - prior[i];
         if (std::log(rdraw[i]) < d) {
             accepted[i] = 1;
             gheight[(size_t)i * N + gnode[i]] = prop[i];
             loglik[2 * i] = lnew[0];
             loglik[2 * i + 1] = lnew[1];
-            if (logpz[2 * i + 1] == 0) logpz[2 * i] = lz;
-            else logpz[2 * i] = logpz[2 * i + 1] = lz;
-            prior[i] = pr;
+            if (logpz[2 * i + 1] == 0) logpz[2 * i] = lz_new[i];
+            else logpz[2 * i] = logpz[2 * i + 1] = lz_new[i];
+            prior[i] = prior_new[i];
         }
-    });
+    }
     return PA_OK;
 }
 
