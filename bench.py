@@ -660,10 +660,13 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
         t2b = time.time()
         ah = 0
         kms = 0.0
+        # the trees stay on the device for the whole pass over the nodes
+        ses = E.GtSession(eng, sp, ids, ch, par, H, Z, rates, ll, lz, prior, piA=piA)
         for b, v in enumerate(range(S, N)):
-            ah += int(eng.gt_branch_len(sp, ids, ch, par, H, Z, rates, v, ind, lens, ll, lz, prior, m, 1 + b,
-                                        piA=piA, precision=prec).sum())
+            ah += int(ses.branch_len(v, ind, lens, m, 1 + b, precision=prec).sum())
             kms += eng.loglik_stats()["elapsed_ms"]
+        H[:], ll[:], lz[:], prior[:] = ses.read()
+        ses.close()
         t3 = time.time()
         L = np.stack([E.gt_lengths(sp, ch[i], par[i], H[i], Z[i], rates[i, 0], rates[i, 1]) for i in range(n)])
         t3b = time.time()
@@ -690,7 +693,7 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
             "launches_per_sweep": 1 + 2 + (N - S) + 1,
             "accept_rate": {k: round(v / (n * sweeps * ((N - S) if k == "heights" else 1)), 4)
                             for k, v in acc.items()},
-            "precision": precision,
+            "precision": precision, "heights_path": "device-resident session (pa_gt_session_*)",
             "data": "synthetic gene trees of the workload; Z and rates fixed"}
 
 

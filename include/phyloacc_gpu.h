@@ -324,6 +324,30 @@ pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *sp, const int32
                             double ncons_to_mis, int32_t root_ingrp, int32_t m, int32_t slot, int32_t precision,
                             double *loglik, double *logpz, int32_t *accepted);
 
+/* A device-resident batch of gene trees for the node-height moves: the trees,
+ * their classes and rates, base frequencies (piA may be NULL) and chain state
+ * (loglik [n*2], logpz [n*2], prior [n], as pa_gt_branch_len keeps them) are
+ * copied to HBM once; each pa_gt_session_branch_len is then one
+ * GTree::Sample_BranchLen move for every element with the proposal, the
+ * likelihood launch and the decision all on the device (one lane per element
+ * for the move's logic, the same code as pa_gt_branch_len), so a sweep of
+ * moves needs no host round trip; pa_gt_session_read returns the heights and
+ * state.  The trees must be the elements' stored trees and keep their
+ * topology while the session lives (a regraft needs a new session).  Moves,
+ * draws and decisions are pa_gt_branch_len's; accepted (n, or NULL) receives
+ * the decisions. */
+typedef struct pa_gt_session pa_gt_session;
+pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *sp, const int32_t *ids, int32_t n,
+                               const int32_t *gchildren, const int32_t *gparent, const double *gheight,
+                               const int32_t *Z, const double *rates, const double *piA, const double *loglik,
+                               const double *logpz, const double *prior, double cons_to_mis, double ncons_to_mis,
+                               pa_gt_session **out);
+pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *gnode, const int32_t *indicator,
+                                   const int32_t *lens, double delta, int32_t m, int32_t slot, int32_t precision,
+                                   int32_t *accepted);
+pa_status pa_gt_session_read(pa_gt_session *s, double *gheight, double *loglik, double *logpz, double *prior);
+void pa_gt_session_destroy(pa_gt_session *s);
+
 /* Per-element log-likelihood under the gBGC substitution model, on the device.
  * Replaces the pruning of PhyloAcc-ST-GBGC over log_cache_TM_{null,cons,neut}
  * and their _B twins (src/PhyloAcc-ST-GBGC/SRC/bpp_c.cpp:1563-1621), whose

@@ -713,6 +713,11 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
         if (threadIdx.x == 0) {
             A->ll_out[w] = l;
             if (A->pre_len) A->ll_out2[w] = c->sum1;
+            if (A->row_out0) {
+                const int row = A->len_row ? A->len_row[w] : w;
+                A->row_out0[row] = l;
+                if (A->pre_len) A->row_out1[row] = c->sum1;
+            }
             // root partial + exponent + pattern weight
             if (A->counters) atomicAdd(A->counters, bytes + (unsigned long long)c->P * (4 * sizeof(R) + 8));
         }

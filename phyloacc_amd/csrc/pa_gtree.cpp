@@ -13,6 +13,7 @@
 // node's up to its parent's.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -21,6 +22,8 @@
 
 #include "../../include/phyloacc_gpu.h"
 #include "pa_common.h"
+#include "pa_gtcore.h"
+#include "pa_gtdev.h"
 #include "pa_internal.h"
 #include "pa_rng.h"
 
@@ -32,9 +35,10 @@ struct Sp {
     int S, N;
     const int32_t *up;     // parent species, N above the root (the root is N - 1)
     const double *H, *theta;
-    std::vector<int> post;              // species in post-order
-    std::vector<int> kid0, kid1;        // children of each internal species
+    std::vector<int32_t> post;          // species in post-order
+    std::vector<int32_t> kid0, kid1;    // children of each internal species
     double top(int s) const { return up[s] < N ? H[up[s]] : INFINITY; }
+    gt::SpView view() const { return {S, N, up, H, theta, post.data(), kid0.data(), kid1.data()}; }
 };
 
 // One gene tree with its embedding.  where[v]: species branch holding gene
@@ -45,7 +49,17 @@ struct Embed {
     int N, root = -1;
     const int32_t *ch, *par;
     const double *h;
-    std::vector<int> where, end, order;
+    std::vector<int32_t> where, end, order;
+
+    gt::TreeView view()
+    {
+        return {sp->S, N, root, ch, par, order.data(), h, where.data(), end.data()};
+    }
+    gt::TreeView view() const
+    {
+        return {sp->S, N, root, ch, par, order.data(), h, const_cast<int32_t *>(where.data()),
+                const_cast<int32_t *>(end.data())};
+    }
 
     bool build(const Sp &s, const int32_t *children, const int32_t *parent, const double *height)
     {
@@ -74,116 +88,48 @@ struct Embed {
                 continue;
             }
             leaves += v < s.S;
-            if (v < s.S) where[v] = v;
-            else {
-                int x = where[ch[2 * v]];
-                while (s.up[x] < N && s.H[s.up[x]] < h[v]) x = s.up[x];
-                where[v] = x;
-            }
             order.push_back(v);
             st.pop_back();
         }
-        if (leaves != s.S) return false;
-        for (int v : order) end[v] = v == root ? N - 1 : where[par[v]];
+        if (leaves != s.S || (int)order.size() != N) return false;
+        gt::TreeView g = view();
+        gt::embed(s.view(), g);
         return true;
     }
     // gene node v's height has changed in place (children and parent
-    // unchanged, the new height between them): v's species and the end of its
-    // children's lineages follow
+    // unchanged, the new height between them)
     void repoint(int v)
     {
-        int x = where[ch[2 * v]];
-        while (sp->up[x] < N && sp->H[sp->up[x]] < h[v]) x = sp->up[x];
-        where[v] = x;
-        end[ch[2 * v]] = end[ch[2 * v + 1]] = x;
+        gt::TreeView g = view();
+        gt::repoint(sp->view(), g, v);
     }
     bool internal(int v) const { return v >= sp->S; }
     // lineage v crosses species x
-    bool crosses(int v, int x) const
-    {
-        for (int s = where[v];; s = sp->up[s]) {
-            if (s == x) return true;
-            if (s == end[v] || sp->up[s] >= N) return false;
-        }
-    }
+    bool crosses(int v, int x) const { return gt::crosses(sp->view(), view(), v, x); }
 };
 
 double prior_of(const Embed &g)
 {
-    const Sp &s = *g.sp;
-    const int N = s.N;
-    // gene nodes born in each species (heights grouped by species), then the
-    // lineages entering each species from below: the sum over its child
-    // species of those leaving their tops (a leaf species sends its gene)
-    thread_local std::vector<int> cnt, off, enter, leave;
+    thread_local std::vector<int32_t> cnt, off, leave;
     thread_local std::vector<double> hs;
-    cnt.assign(N, 0);
-    for (int v : g.order)
-        if (g.internal(v)) cnt[g.where[v]]++;
-    off.assign(N + 1, 0);
-    for (int x = 0; x < N; x++) off[x + 1] = off[x] + cnt[x];
-    hs.resize(off[N]);
-    for (int v : g.order)
-        if (g.internal(v)) hs[off[g.where[v] + 1] - cnt[g.where[v]]--] = g.h[v];
-    enter.assign(N, 0);
-    leave.assign(N, 0);
-    double lp = 0;
-    for (int x : s.post) {
-        if (x < s.S) { leave[x] = 1; continue; }
-        int k = leave[s.kid0[x]] + leave[s.kid1[x]];
-        const int a = off[x], b = off[x + 1];
-        leave[x] = k - (b - a);
-        if (k == 0) continue;
-        std::sort(hs.begin() + a, hs.begin() + b);
-        const double rate = 1.0 / s.theta[x], ldens = std::log(2.0 * rate);
-        double t = s.H[x];
-        for (int q = a; q < b; q++) {
-            lp += ldens - (hs[q] - t) * k * (k - 1) * rate;
-            t = hs[q];
-            k--;
-        }
-        if (k > 1 && s.up[x] != s.N) lp -= (s.H[s.up[x]] - t) * k * (k - 1) * rate;
-    }
-    return lp;
+    cnt.resize(g.N); off.resize(g.N + 1); leave.resize(g.N); hs.resize(g.N);
+    return gt::prior(g.sp->view(), g.view(), cnt.data(), off.data(), leave.data(), hs.data());
 }
 
 void missing_of(const Embed &g, const uint8_t *leaf_missing, std::vector<uint8_t> &miss)
 {
     miss.assign(g.N, 0);
-    for (int v : g.order) miss[v] = g.internal(v) ? (miss[g.ch[2 * v]] && miss[g.ch[2 * v + 1]]) : leaf_missing[v];
+    gt::missing(g.view(), leaf_missing, miss.data());
 }
 
 double logpz_of(const Embed &g, const std::vector<uint8_t> &miss, const int32_t *Z, double c2m, double n2m)
 {
-    const double lc[2] = {std::log(1 - c2m), std::log(c2m)}, ln[2] = {std::log(1 - n2m), std::log(n2m)};
-    auto term = [&](int v, int x) { return Z[x] == 1 ? lc[miss[v]] : ln[miss[v]]; };
-    double lp = 0;
-    for (int v : g.order) {
-        if (!g.internal(v)) lp += term(v, v);
-        else if (g.where[v] >= g.sp->S && g.h[v] >= g.sp->H[g.where[v]]) lp += term(v, g.where[v]);
-    }
-    return lp;
+    return gt::logpz(g.sp->view(), g.view(), miss.data(), Z, c2m, n2m);
 }
 
-// Substitution length of every gene branch: its pieces in the species it
-// crosses, each at that species' rate (Z 2: r2, Z 1: r1, else 1); pieces
-// shorter than 1e-8 are the identity of getlogTMc and add nothing.
 void lengths_of(const Embed &g, const int32_t *Z, double r1, double r2, double *len)
 {
-    const Sp &s = *g.sp;
-    for (int v = 0; v < g.N; v++) len[v] = 0;
-    for (int v : g.order) {
-        if (v == g.root) continue;
-        const double hp = g.h[g.par[v]];
-        double acc = 0;
-        for (int x = g.where[v];; x = s.up[x]) {
-            const bool last = x == g.end[v] || s.up[x] >= s.N;
-            const double lo = std::max(g.h[v], s.H[x]), hi = last ? hp : s.top(x);
-            if (hi - lo >= 1e-8) acc += (hi - lo) * (Z[x] == 2 ? r2 : (Z[x] == 1 ? r1 : 1.0));
-            if (last) break;
-        }
-        len[v] = acc;
-    }
+    gt::lengths(g.sp->view(), g.view(), Z, r1, r2, len);
 }
 
 // Runs fn(i) for i < n on up to 16 host threads (one below 256 elements);
@@ -289,48 +235,9 @@ extern "C" pa_status pa_gt_lengths(const pa_species_tree *t, const int32_t *chil
 
 namespace {
 
-// The window of one height move and where it came from.
-struct HeightMove {
-    int status = 0;          // 0: a proposal, 1: empty window, -1: no species holds both lineages
-    double lower1 = 0, upper1 = 0, half = 0, lower = 0, upper = 0;
-};
+using HeightMove = gt::Window;
 
-// GTree::Sample_BranchLen's window (src/PhyloAcc-GT/genetree.cpp:1658-1774):
-// cnode is the child with the shorter branch (the second child on a tie);
-// the node may go down to the bottom of css, the lowest species that both
-// children's lineages cross, or to cnode, and up to its parent gene node when
-// that is born in the same species, else to the top of its species; the
-// gene root goes up to 10 theta_root / 2 (h(root) + 5 theta_root / 2 when the
-// root is already higher).  The window's half-width is delta * theta / 2 of
-// the node's species.
-HeightMove height_window(const Embed &g, int v, double delta)
-{
-    const Sp &s = *g.sp;
-    HeightMove w;
-    const int a = g.ch[2 * v], b = g.ch[2 * v + 1];
-    const bool first = g.h[v] - g.h[a] < g.h[v] - g.h[b];
-    const int cnode = first ? a : b, sib = first ? b : a;
-    const int ss = g.where[v];
-    // lowest species on cnode's climb (up to ss) that sib's lineage also crosses
-    int css = -1;
-    for (int x = g.where[cnode];; x = s.up[x]) {
-        if (g.crosses(sib, x)) { css = x; break; }
-        if (x == ss || s.up[x] >= s.N) break;
-    }
-    if (css < 0) { w.status = -1; return w; }
-    const int p = g.par[v];
-    w.upper1 = p < s.N && g.where[p] == ss ? g.h[p] : s.top(ss);
-    if (v == g.root) {
-        w.upper1 = 10 * s.theta[s.N - 1] / 2;
-        if (w.upper1 < g.h[v]) w.upper1 = g.h[v] + 5 * s.theta[s.N - 1] / 2;
-    }
-    w.lower1 = std::max(s.H[css], g.h[cnode]);
-    w.half = delta * s.theta[ss] / 2;
-    w.upper = std::min(w.upper1, g.h[v] + w.half);
-    w.lower = std::max(w.lower1, g.h[v] - w.half);
-    if (w.upper - w.lower == 0) w.status = 1;
-    return w;
-}
+HeightMove height_window(const Embed &g, int v, double delta) { return gt::window(g.sp->view(), g.view(), v, delta); }
 
 }  // namespace
 
@@ -428,12 +335,9 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
         return st;
     for (int j = 0; j < k; j++) {
         const int i = run[j], ind = indicator[i];
-        const HeightMove &w = win[i];
-        const double back = std::min(w.upper1, prop[i] + w.half) - std::max(w.lower1, prop[i] - w.half);
         const double lnew[2] = {o0[j], o1[j]};
-        double d = lnew[ind] - loglik[2 * i + ind] + std::log(w.upper - w.lower) - std::log(back);
-        if (logpz[2 * i + ind] != 0) d += lz_new[i] - logpz[2 * i + ind];
-        d += prior_new[i] - prior[i];
+        const double d = gt::height_ratio(win[i], prop[i], ind, lnew, loglik + 2 * i, logpz + 2 * i, lz_new[i],
+                                          prior_new[i], prior[i]);
         if (std::log(rdraw[i]) < d) {
             accepted[i] = 1;
             gheight[(size_t)i * N + gnode[i]] = prop[i];
@@ -846,4 +750,190 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     for (int i = 0; i < n; i++)
         if (accepted[i] && (st = element_set_tree(e, ids[i], nch[i].data())) != PA_OK) return st;
     return PA_OK;
+}
+
+// ------------------------------------------------ device-resident height moves
+
+struct pa_gt_session {
+    pa_engine *e = nullptr;
+    int n = 0;
+    Sp sp;
+    std::vector<int32_t> up_h;
+    std::vector<double> H_h, th_h;
+    std::vector<int32_t> ids;
+    std::vector<double> piA;
+    double c2m = 0, n2m = 0;
+    std::vector<void *> bufs;
+    // device arrays
+    int32_t *up = nullptr, *post = nullptr, *kid0 = nullptr, *kid1 = nullptr;
+    double *H = nullptr, *theta = nullptr;
+    int32_t *ch = nullptr, *par = nullptr, *order = nullptr, *root = nullptr, *Z = nullptr;
+    double *h = nullptr, *rates = nullptr, *ll = nullptr, *lz = nullptr, *prior = nullptr;
+    uint8_t *lm = nullptr, *ws_u = nullptr;
+    int32_t *ws_i = nullptr, *gnode = nullptr, *ind = nullptr, *acc = nullptr;
+    double *ws_d = nullptr, *len = nullptr, *prop = nullptr, *rdraw = nullptr, *prior_new = nullptr, *lz_new = nullptr;
+    double *out0 = nullptr, *out1 = nullptr;
+    gt::Window *win = nullptr;
+
+    template <typename T>
+    hipError_t get(T **p, size_t count, const void *src = nullptr)
+    {
+        void *q = nullptr;
+        hipError_t r = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 8));
+        if (r != hipSuccess) return r;
+        bufs.push_back(q);
+        *p = (T *)q;
+        if (src && count) r = hipMemcpy(q, src, count * sizeof(T), hipMemcpyHostToDevice);
+        return r;
+    }
+    ~pa_gt_session()
+    {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+};
+
+extern "C" pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *t, const int32_t *ids, int32_t n,
+                                          const int32_t *gchildren, const int32_t *gparent, const double *gheight,
+                                          const int32_t *Z, const double *rates, const double *piA,
+                                          const double *loglik, const double *logpz, const double *prior,
+                                          double cons_to_mis, double ncons_to_mis, pa_gt_session **out)
+{
+    if (!e || !out) return PA_E_INPUT;
+    *out = nullptr;
+    if (n <= 0 || !ids || !gchildren || !gparent || !gheight || !Z || !rates || !loglik || !logpz || !prior)
+        return engine_fail(e, PA_E_INPUT, "null argument or empty batch");
+    if (!t || !t->parent || !t->height || !t->theta) return engine_fail(e, PA_E_INPUT, "null species tree");
+    std::unique_ptr<pa_gt_session> s(new pa_gt_session);
+    s->e = e;
+    s->n = n;
+    // the session keeps its own copy of the species tree
+    s->up_h.assign(t->parent, t->parent + t->N);
+    s->H_h.assign(t->height, t->height + t->N);
+    s->th_h.assign(t->theta, t->theta + t->N);
+    pa_species_tree own{t->S, t->N, s->up_h.data(), s->H_h.data(), s->th_h.data()};
+    pa_status st = check_sp(e, &own, s->sp);
+    if (st != PA_OK) return st;
+    EngineView ev;
+    engine_view(e, &ev);
+    if (s->sp.S != ev.S || s->sp.N != ev.N) return engine_fail(e, PA_E_INPUT, "species tree size differs from the model's");
+    const int S = s->sp.S, N = s->sp.N;
+    std::vector<uint8_t> lm((size_t)n * S);
+    for (int i = 0; i < n; i++)
+        if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
+    std::vector<int32_t> order((size_t)n * N), root(n);
+    std::vector<int> bad(n, 0);
+    const Sp &sp = s->sp;
+    for_elements(n, [&](int i) {
+        thread_local Embed g;
+        g.root = -1;
+        if (!g.build(sp, gchildren + (size_t)i * 2 * N, gparent + (size_t)i * N, gheight + (size_t)i * N)) { bad[i] = 1; return; }
+        memcpy(&order[(size_t)i * N], g.order.data(), sizeof(int32_t) * N);
+        root[i] = g.root;
+    });
+    for (int i = 0; i < n; i++)
+        if (bad[i]) return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
+    s->ids.assign(ids, ids + n);
+    if (piA) {
+        for (int i = 0; i < n; i++)
+            if (!(piA[i] > 0 && piA[i] < 0.5)) return engine_fail(e, PA_E_INPUT, "piA must lie in (0, 1/2)");
+        s->piA.assign(piA, piA + n);
+    }
+    s->c2m = cons_to_mis;
+    s->n2m = ncons_to_mis;
+    hipStream_t stream;
+    if ((st = engine_stream(e, &stream)) != PA_OK) return st;
+    const size_t nN = (size_t)n * N;
+    hipError_t r = hipSuccess;
+    auto chk = [&](hipError_t x) { if (r == hipSuccess) r = x; };
+    chk(s->get(&s->up, N, sp.up)); chk(s->get(&s->H, N, sp.H)); chk(s->get(&s->theta, N, sp.theta));
+    chk(s->get(&s->post, N, sp.post.data())); chk(s->get(&s->kid0, N, sp.kid0.data())); chk(s->get(&s->kid1, N, sp.kid1.data()));
+    chk(s->get(&s->ch, 2 * nN, gchildren)); chk(s->get(&s->par, nN, gparent)); chk(s->get(&s->order, nN, order.data()));
+    chk(s->get(&s->root, n, root.data())); chk(s->get(&s->h, nN, gheight)); chk(s->get(&s->Z, nN, Z));
+    chk(s->get(&s->rates, 2 * (size_t)n, rates)); chk(s->get(&s->lm, (size_t)n * S, lm.data()));
+    chk(s->get(&s->ll, 2 * (size_t)n, loglik)); chk(s->get(&s->lz, 2 * (size_t)n, logpz)); chk(s->get(&s->prior, n, prior));
+    chk(s->get(&s->ws_i, (size_t)n * (5 * N + 1))); chk(s->get(&s->ws_d, nN)); chk(s->get(&s->ws_u, nN));
+    chk(s->get(&s->len, nN)); chk(s->get(&s->win, n)); chk(s->get(&s->prop, n)); chk(s->get(&s->rdraw, n));
+    chk(s->get(&s->prior_new, n)); chk(s->get(&s->lz_new, n)); chk(s->get(&s->out0, n)); chk(s->get(&s->out1, n));
+    chk(s->get(&s->gnode, n)); chk(s->get(&s->ind, n)); chk(s->get(&s->acc, n));
+    if (r != hipSuccess) return engine_fail(e, PA_E_OOM, std::string("gene-tree session buffers: ") + hipGetErrorString(r));
+    *out = s.release();
+    return PA_OK;
+}
+
+extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *gnode, const int32_t *indicator,
+                                              const int32_t *lens, double delta, int32_t m, int32_t slot,
+                                              int32_t precision, int32_t *accepted)
+{
+    if (!s) return PA_E_INPUT;
+    pa_engine *e = s->e;
+    if (!gnode || !indicator || !lens) return engine_fail(e, PA_E_INPUT, "null argument");
+    if (!(delta > 0)) return engine_fail(e, PA_E_INPUT, "delta must be positive");
+    const int n = s->n, S = s->sp.S, N = s->sp.N;
+    std::vector<int32_t> l0(n), l1(n);
+    for (int i = 0; i < n; i++) {
+        if (gnode[i] < S || gnode[i] >= N) return engine_fail(e, PA_E_INPUT, "gnode must be an internal gene node");
+        if (indicator[i] != 0 && indicator[i] != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
+        if (lens[2 * i + 1] < 0 || lens[2 * i + indicator[i]] < 0) return engine_fail(e, PA_E_INPUT, "negative prefix length");
+        l0[i] = lens[2 * i + indicator[i]];
+        l1[i] = std::min(lens[2 * i + 1], l0[i]);
+    }
+    hipStream_t stream;
+    pa_status st = engine_stream(e, &stream);
+    if (st != PA_OK) return st;
+    if (hipMemcpy(s->gnode, gnode, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->ind, indicator, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return engine_fail(e, PA_E_DEVICE, "session upload failed");
+    EngineView ev;
+    engine_view(e, &ev);
+    GtPropArgs a;
+    a.sp = {S, N, s->up, s->H, s->theta, s->post, s->kid0, s->kid1};
+    a.n = n; a.S = S; a.N = N;
+    a.ch = s->ch; a.par = s->par; a.order = s->order; a.root = s->root;
+    a.h = s->h; a.Z = s->Z; a.rates = s->rates; a.lm = s->lm; a.gnode = s->gnode;
+    a.delta = delta; a.c2m = s->c2m; a.n2m = s->n2m;
+    a.key0 = ev.key0; a.key1 = ev.key1; a.m = m; a.slot = slot;
+    a.ws_i = s->ws_i; a.ws_d = s->ws_d; a.ws_u = s->ws_u;
+    a.len = s->len; a.win = s->win; a.prop = s->prop; a.rdraw = s->rdraw; a.prior_new = s->prior_new; a.lz_new = s->lz_new;
+    if (launch_gt_hprop(a, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height proposal launch failed");
+    GtExtra x;
+    x.d_len = s->len;
+    x.d_row0 = s->out0;
+    x.d_row1 = s->out1;
+    std::vector<double> o0(n), o1(n);
+    if ((st = gt_launch(e, s->ids.data(), n, nullptr, s->piA.empty() ? nullptr : s->piA.data(), l0.data(), l1.data(),
+                        precision, o0.data(), o1.data(), x)) != PA_OK)
+        return st;
+    GtDecideArgs d;
+    d.n = n; d.N = N;
+    d.win = s->win; d.prop = s->prop; d.rdraw = s->rdraw; d.prior_new = s->prior_new; d.lz_new = s->lz_new;
+    d.lnew0 = s->out0; d.lnew1 = s->out1; d.gnode = s->gnode; d.indicator = s->ind;
+    d.h = s->h; d.loglik = s->ll; d.logpz = s->lz; d.prior = s->prior; d.accepted = s->acc;
+    if (launch_gt_hdecide(d, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height decision launch failed");
+    if (accepted) {
+        if (hipMemcpy(accepted, s->acc, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+            return engine_fail(e, PA_E_DEVICE, "session download failed");
+    } else if (hipStreamSynchronize(stream) != hipSuccess) {
+        return engine_fail(e, PA_E_DEVICE, "height moves failed");
+    }
+    return PA_OK;
+}
+
+extern "C" pa_status pa_gt_session_read(pa_gt_session *s, double *gheight, double *loglik, double *logpz, double *prior)
+{
+    if (!s) return PA_E_INPUT;
+    const size_t n = s->n, N = s->sp.N;
+    hipError_t r = hipSuccess;
+    if (gheight && r == hipSuccess) r = hipMemcpy(gheight, s->h, sizeof(double) * n * N, hipMemcpyDeviceToHost);
+    if (loglik && r == hipSuccess) r = hipMemcpy(loglik, s->ll, sizeof(double) * 2 * n, hipMemcpyDeviceToHost);
+    if (logpz && r == hipSuccess) r = hipMemcpy(logpz, s->lz, sizeof(double) * 2 * n, hipMemcpyDeviceToHost);
+    if (prior && r == hipSuccess) r = hipMemcpy(prior, s->prior, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (r != hipSuccess) return engine_fail(s->e, PA_E_DEVICE, "session download failed");
+    return PA_OK;
+}
+
+extern "C" void pa_gt_session_destroy(pa_gt_session *s)
+{
+    if (!s) return;
+    (void)hipDeviceSynchronize();
+    delete s;
 }

@@ -1210,7 +1210,8 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
                                      const int32_t *lens0 = nullptr, const int32_t *lens1 = nullptr,
                                      double *out1 = nullptr, const GtExtra *gx = nullptr)
 {
-    if (!e || !ids || !lengths || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
+    const bool dev_len = gx && gx->d_len;
+    if (!e || !ids || (!lengths && !dev_len) || !out || n < 0) return fail(e, PA_E_INPUT, "null argument");
     if (n == 0) return PA_OK;
     pa_status st = check_ids(e, ids, n);
     if (st != PA_OK) return st;
@@ -1222,19 +1223,20 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
     DevBuf bufs[10];
     HIPCHK(e, hipMalloc(&bufs[5].p, (size_t)n * 8));
     HIPCHK(e, hipMalloc(&bufs[6].p, (size_t)n * 4));
-    if (lbytes > e->ll_len_bytes) {
+    if (!dev_len && lbytes > e->ll_len_bytes) {
         (void)hipFree(e->d_ll_len);
         e->d_ll_len = nullptr;
         e->ll_len_bytes = 0;
         HIPCHK(e, hipMalloc(&e->d_ll_len, lbytes));
         e->ll_len_bytes = lbytes;
     }
-    HIPCHK(e, hipMemcpy(e->d_ll_len, lengths, lbytes, hipMemcpyHostToDevice));
+    if (!dev_len) HIPCHK(e, hipMemcpy(e->d_ll_len, lengths, lbytes, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(bufs[6].p, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
     LoglikArgs A;
     memset(&A, 0, sizeof A);
     A.ll_out = (double *)bufs[5].p;
-    A.len_in = (const double *)e->d_ll_len;
+    A.len_in = dev_len ? gx->d_len : (const double *)e->d_ll_len;
+    if (gx) { A.row_out0 = gx->d_row0; A.row_out1 = gx->d_row1; }
     A.len_row = (const int32_t *)bufs[6].p;
     if (piA) {
         // BPP's exchangeabilities from RATE_MAT and BACKGROUND, with the
@@ -1411,6 +1413,15 @@ pa_status element_set_tree(pa_engine *e, int32_t id, const int32_t *children)
 }
 
 GtHistory &engine_history(pa_engine *e) { return e->hist; }
+
+pa_status engine_stream(pa_engine *e, hipStream_t *st)
+{
+    pa_status r = ensure_device(e);
+    if (r != PA_OK) return r;
+    if ((r = upload(e)) != PA_OK) return r;
+    *st = e->stream;
+    return PA_OK;
+}
 
 pa_status engine_history_reserve(pa_engine *e, size_t bytes)
 {

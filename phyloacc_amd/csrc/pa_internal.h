@@ -2,6 +2,7 @@
 // (pa_host.cpp owns the struct).
 #pragma once
 #include <stdint.h>
+#include <hip/hip_runtime.h>
 #include <string>
 #include <vector>
 #include "../../include/phyloacc_gpu.h"
@@ -24,6 +25,8 @@ pa_status element_gene_tree(pa_engine *e, int32_t id, uint8_t *leaf_missing, con
 // Extra work of a gene-tree likelihood launch; arrays are per caller row
 // (host memory unless marked device).
 struct GtExtra {
+    const double *d_len = nullptr;                    // device: the lengths [n][N] (the host lengths are then unused)
+    double *d_row0 = nullptr, *d_row1 = nullptr;      // device: the log-likelihoods by row as well
     const int32_t *tabs = nullptr, *hdrs = nullptr;   // per-row trees [n][tab_words(N)], [n][4] (gene_table)
     int8_t *d_hist = nullptr;                         // device: the histories (row r at hist_off[r],
     const int64_t *hist_off = nullptr;                //   node v at v * hist_ns[r] from there)
@@ -57,5 +60,8 @@ pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *l
 pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr);
 // Makes children [N*2] the element's stored tree (host and, when uploaded, device).
 pa_status element_set_tree(pa_engine *e, int32_t id, const int32_t *children);
+// The engine's stream (after the device is set up), for launches that a
+// likelihood launch on it must follow.
+pa_status engine_stream(pa_engine *e, hipStream_t *st);
 
 }  // namespace pa

@@ -63,6 +63,9 @@ struct LoglikArgs {
     // site -> pattern map is `site`).
     const int32_t *pre_len;   // [rows][2]
     double *ll_out2;          // [n_work]
+    // The log-likelihoods by caller row as well, or null (device-resident
+    // callers): row_out0[r] = ll_out, row_out1[r] = ll_out2.
+    double *row_out0, *row_out1;
     // PhyloAcc-GT gene-tree moves.  Arrays below are per caller row: work
     // position w uses row len_row[w] (or w).
     // Per-call trees, or null: row r prunes the tree of node table

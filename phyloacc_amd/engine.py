@@ -35,7 +35,8 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats",
            "pa_loglik_lengths_pi", "pa_sample_pi", "pa_loglik_lengths_split", "pa_wl_step",
            "pa_gt_prior", "pa_gt_logpz", "pa_gt_lengths", "pa_gt_branch_len", "pa_gt_sample_history",
-           "pa_gt_sample_tree"]
+           "pa_gt_sample_tree", "pa_gt_session_create", "pa_gt_session_branch_len", "pa_gt_session_read",
+           "pa_gt_session_destroy"]
 
 
 class SpeciesTree(C.Structure):
@@ -218,6 +219,15 @@ def lib():
                                         PI, P(C.c_int8), C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                         C.c_int32, C.c_int32, PD, PD, PI]
         L.pa_gt_sample_tree.restype = C.c_int
+        L.pa_gt_session_create.argtypes = [C.c_void_p, P(SpeciesTree), PI, C.c_int32, PI, PI, PD, PI, PD, PD, PD, PD,
+                                           PD, C.c_double, C.c_double, P(C.c_void_p)]
+        L.pa_gt_session_create.restype = C.c_int
+        L.pa_gt_session_branch_len.argtypes = [C.c_void_p, PI, PI, PI, C.c_double, C.c_int32, C.c_int32, C.c_int32, PI]
+        L.pa_gt_session_branch_len.restype = C.c_int
+        L.pa_gt_session_read.argtypes = [C.c_void_p, PD, PD, PD, PD]
+        L.pa_gt_session_read.restype = C.c_int
+        L.pa_gt_session_destroy.argtypes = [C.c_void_p]
+        L.pa_gt_session_destroy.restype = None
         L.pa_loglik_last_stats.argtypes = [C.c_void_p, P(RunStats)]
         L.pa_loglik_last_stats.restype = C.c_int
         L.pa_batch_create.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32, P(RunConfig), P(C.c_void_p)]
@@ -687,6 +697,62 @@ def gene_lengths(N, seg_gene, seg_species, seg_len, Z, r1, r2):
     if st != PA_OK:
         raise PaError(st, "pa_gene_lengths failed")
     return out
+
+
+class GtSession:
+    """Device-resident gene trees for batched node-height moves
+    (pa_gt_session_*): created from the trees and chain state, moved on the
+    device, read back at the end."""
+
+    def __init__(self, eng, sp, ids, children, parent, heights, Z, rates, loglik, logpz, prior, piA=None,
+                 cons_to_mis=0.01, ncons_to_mis=0.5):
+        self.eng = eng
+        ids = np.ascontiguousarray(ids, np.int32)
+        self.n, self.N = len(ids), eng.N
+        arrs = [np.ascontiguousarray(children, np.int32), np.ascontiguousarray(parent, np.int32),
+                np.ascontiguousarray(heights, np.float64), np.ascontiguousarray(Z, np.int32),
+                np.ascontiguousarray(rates, np.float64), np.ascontiguousarray(loglik, np.float64),
+                np.ascontiguousarray(logpz, np.float64), np.ascontiguousarray(prior, np.float64)]
+        a = None if piA is None else np.ascontiguousarray(piA, np.float64)
+        h = C.c_void_p()
+        eng._check(lib().pa_gt_session_create(
+            eng._h, C.byref(sp), _p(ids, C.c_int32), self.n, _p(arrs[0], C.c_int32), _p(arrs[1], C.c_int32),
+            _p(arrs[2], C.c_double), _p(arrs[3], C.c_int32), _p(arrs[4], C.c_double),
+            None if a is None else _p(a, C.c_double), _p(arrs[5], C.c_double), _p(arrs[6], C.c_double),
+            _p(arrs[7], C.c_double), float(cons_to_mis), float(ncons_to_mis), C.byref(h)))
+        self._h = h
+        self._sp = sp
+
+    def branch_len(self, gnode, indicator, lens, m, slot, delta=1.0, precision=PA_FP64, want_accepted=True):
+        g = np.ascontiguousarray(np.broadcast_to(gnode, (self.n,)), np.int32)
+        ind = np.ascontiguousarray(np.broadcast_to(indicator, (self.n,)), np.int32)
+        ln = np.ascontiguousarray(np.broadcast_to(lens, (self.n, 2)), np.int32)
+        acc = np.zeros(self.n, np.int32) if want_accepted else None
+        self.eng._check(lib().pa_gt_session_branch_len(self._h, _p(g, C.c_int32), _p(ind, C.c_int32),
+                                                       _p(ln, C.c_int32), float(delta), int(m), int(slot),
+                                                       int(precision), None if acc is None else _p(acc, C.c_int32)))
+        return acc
+
+    def read(self):
+        """(heights [n, N], loglik [n, 2], logpz [n, 2], prior [n])."""
+        h = np.zeros((self.n, self.N))
+        ll = np.zeros((self.n, 2))
+        lz = np.zeros((self.n, 2))
+        pr = np.zeros(self.n)
+        self.eng._check(lib().pa_gt_session_read(self._h, _p(h, C.c_double), _p(ll, C.c_double),
+                                                 _p(lz, C.c_double), _p(pr, C.c_double)))
+        return h, ll, lz, pr
+
+    def close(self):
+        if self._h:
+            lib().pa_gt_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _gt_call(name, sp, children, parent, heights, *rest):

@@ -152,3 +152,49 @@ def test_branch_len_sweeps_match_the_oracle(require_gpu, use_pi):
                 assert prior[c] == pytest.approx(oprior[c], rel=1e-12, abs=1e-12)
             total += acc.sum()
     assert 0 < total < 4 * n * (N - S)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_pi", [False, True])
+def test_device_session_equals_the_host_moves(require_gpu, use_pi):
+    """The device-resident height moves (pa_gt_session_*) give, move for move,
+    the decisions, heights and state of pa_gt_branch_len (FP64), and read
+    back the same final state; both paths are held to the oracle above."""
+    w = _gt_workload(seed=8, n=7, S=9)
+    opts = E.default_options(min_len=1, seed=SEED)
+    eng = E.Engine.from_workload(w, opts)
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    assert (st == E.PA_ELEM_OK).all()
+    sp, osp = _species(w)
+    S, N, n = w.tree.S, w.tree.N, w.C
+    rng = np.random.default_rng(4)
+    piA = rng.uniform(0.15, 0.35, n) if use_pi else None
+    ch = np.zeros((n, N, 2), np.int32)
+    par = np.zeros((n, N), np.int32)
+    H = np.zeros((n, N))
+    for c in range(n):
+        ch[c], par[c], H[c] = _state(w, c)
+    rates = np.stack([w.r1, w.r2], 1)
+    lens = np.stack([w.sizes, w.sizes // 3], 1).astype(np.int32)
+    f0, f1 = eng.loglik_lengths_split(ids, w.lengths(), lens[:, 0], lens[:, 1], piA=piA)
+    ll = np.stack([f0, f1], 1)
+    lz = np.full((n, 2), -3.0)
+    prior = np.array([E.gt_prior(sp, ch[c], par[c], H[c]) for c in range(n)])
+    ses = E.GtSession(eng, sp, ids, ch, par, H, w.Z, rates, ll, lz, prior, piA=piA, cons_to_mis=0.05,
+                      ncons_to_mis=0.4)
+    total = 0
+    for m in range(3):
+        ind = np.full(n, m % 2, np.int32)
+        for b, v in enumerate(range(S, N)):
+            a_host = eng.gt_branch_len(sp, ids, ch, par, H, w.Z, rates, v, ind, lens, ll, lz, prior, m, b,
+                                       cons_to_mis=0.05, ncons_to_mis=0.4, piA=piA)
+            a_dev = ses.branch_len(v, ind, lens, m, b)
+            np.testing.assert_array_equal(a_dev, a_host, err_msg=f"m={m} v={v}")
+            total += a_host.sum()
+        h2, ll2, lz2, pr2 = ses.read()
+        np.testing.assert_array_equal(h2, H)
+        np.testing.assert_allclose(ll2, ll, rtol=1e-12)
+        np.testing.assert_array_equal(lz2, lz)
+        np.testing.assert_allclose(pr2, prior, rtol=1e-13, atol=1e-13)
+    assert total > 0
+    ses.close()
