@@ -76,26 +76,37 @@ GT_HD bool crosses(const SpView &s, const TreeView &g, int v, int x)
     }
 }
 
-// Substitution length of every gene branch: its pieces in the species it
+// Substitution length of gene branch v: its pieces in the species it
 // crosses, each at that species' rate (Z 2: r2, Z 1: r1, else 1); pieces
-// shorter than 1e-8 are the identity of getlogTMc and add nothing.
+// shorter than 1e-8 are the identity of getlogTMc and add nothing.  The
+// root's is 0.  Needs the embedding of every node, nothing of other branches.
+GT_HD double branch_length(const SpView &s, const TreeView &g, int v, const int32_t *Z, double r1, double r2)
+{
+    if (v == g.root) return 0;
+    const double hp = g.h[g.par[v]];
+    double acc = 0;
+    for (int x = g.where[v];; x = s.up[x]) {
+        const bool last = x == g.end[v] || s.up[x] >= s.N;
+        const double lo = fmax(g.h[v], s.H[x]), hi = last ? hp : sp_top(s, x);
+        if (hi - lo >= 1e-8) acc += (hi - lo) * (Z[x] == 2 ? r2 : (Z[x] == 1 ? r1 : 1.0));
+        if (last) break;
+    }
+    return acc;
+}
+
 GT_HD void lengths(const SpView &s, const TreeView &g, const int32_t *Z, double r1, double r2, double *len)
 {
-    for (int v = 0; v < g.N; v++) len[v] = 0;
-    for (int q = 0; q < g.N; q++) {
-        const int v = g.order[q];
-        if (v == g.root) continue;
-        const double hp = g.h[g.par[v]];
-        double acc = 0;
-        for (int x = g.where[v];; x = s.up[x]) {
-            const bool last = x == g.end[v] || s.up[x] >= s.N;
-            const double lo = fmax(g.h[v], s.H[x]), hi = last ? hp : sp_top(s, x);
-            if (hi - lo >= 1e-8) acc += (hi - lo) * (Z[x] == 2 ? r2 : (Z[x] == 1 ? r1 : 1.0));
-            if (last) break;
-        }
-        len[v] = acc;
-    }
+    for (int v = 0; v < g.N; v++) len[v] = branch_length(s, g, v, Z, r1, r2);
 }
+
+// One coalescence among k lineages after a wait hi - lo, and the wait of k
+// lineages that do not coalesce before the top hi of the species; rate is
+// 1 / theta and ldens log(2 rate).
+GT_HD double coal_term(double ldens, double rate, double hi, double lo, int k)
+{
+    return ldens - (hi - lo) * k * (k - 1) * rate;
+}
+GT_HD double coal_tail(double rate, double hi, double lo, int k) { return (hi - lo) * k * (k - 1) * rate; }
 
 // The coalescent prior: per internal species, the lineages entering from
 // below (the sum over its child species of those leaving their tops; a leaf
@@ -125,11 +136,11 @@ GT_HD double prior(const SpView &s, const TreeView &g, int32_t *cnt, int32_t *of
         const double rate = 1.0 / s.theta[x], ldens = log(2.0 * rate);
         double t = s.H[x];
         for (int i = a; i < b; i++) {
-            lp += ldens - (hs[i] - t) * k * (k - 1) * rate;
+            lp += coal_term(ldens, rate, hs[i], t, k);
             t = hs[i];
             k--;
         }
-        if (k > 1 && s.up[x] != s.N) lp -= (s.H[s.up[x]] - t) * k * (k - 1) * rate;
+        if (k > 1 && s.up[x] != s.N) lp -= coal_tail(rate, s.H[s.up[x]], t, k);
     }
     return lp;
 }
@@ -144,20 +155,36 @@ GT_HD void missing(const TreeView &g, const uint8_t *leaf_missing, uint8_t *miss
 }
 
 // log P(missing | Z) over the leaves and the gene nodes born in internal
-// species; P(missing | Z = 1) = c2m, P(missing | Z != 1) = n2m.
+// species; P(missing | Z = 1) = c2m, P(missing | Z != 1) = n2m.  The four
+// logarithms: log(1 - c2m), log(c2m), log(1 - n2m), log(n2m).
+GT_HD void logpz_consts(double c2m, double n2m, double *lc)
+{
+    lc[0] = log(1 - c2m);
+    lc[1] = log(c2m);
+    lc[2] = log(1 - n2m);
+    lc[3] = log(n2m);
+}
+
+// gene node v's term (false: v adds nothing)
+GT_HD bool logpz_term(const SpView &s, const TreeView &g, const uint8_t *miss, const int32_t *Z, const double *lc,
+                      int v, double *term)
+{
+    int x = v;
+    if (v >= g.S) {
+        x = g.where[v];
+        if (x < s.S || g.h[v] < s.H[x]) return false;
+    }
+    *term = lc[(Z[x] == 1 ? 0 : 2) + (miss[v] ? 1 : 0)];
+    return true;
+}
+
 GT_HD double logpz(const SpView &s, const TreeView &g, const uint8_t *miss, const int32_t *Z, double c2m, double n2m)
 {
-    const double lc0 = log(1 - c2m), lc1 = log(c2m), ln0 = log(1 - n2m), ln1 = log(n2m);
-    double lp = 0;
-    for (int q = 0; q < g.N; q++) {
-        const int v = g.order[q];
-        int x = v;
-        if (v >= g.S) {
-            x = g.where[v];
-            if (x < s.S || g.h[v] < s.H[x]) continue;
-        }
-        lp += Z[x] == 1 ? (miss[v] ? lc1 : lc0) : (miss[v] ? ln1 : ln0);
-    }
+    double lc[4];
+    logpz_consts(c2m, n2m, lc);
+    double lp = 0, t;
+    for (int q = 0; q < g.N; q++)
+        if (logpz_term(s, g, miss, Z, lc, g.order[q], &t)) lp += t;
     return lp;
 }
 

@@ -8,38 +8,49 @@
 
 namespace pa {
 
-// Per element i (rows of N nodes): the gene tree, its species-branch
-// classes and rates, leaf missing flags, the node to move; per-element
-// workspace (5N + 1 ints, N doubles, N bytes) and outputs.
+// Per element i (rows of N nodes): the gene tree with its embedding (the
+// species of every gene node, kept current by the decisions), its missing
+// flags, species-branch classes and rates, the node to move, and outputs.
+// Per species: the leaves below it, 1 / theta and log(2 / theta); lc: the
+// missing-gene logarithms (log(1 - c2m), log(c2m), log(1 - n2m), log(n2m)).
+// The proposal works on a copy of the element's rows in LDS.
 struct GtPropArgs {
-    gt::SpView sp;               // device arrays
+    gt::SpView sp;               // device arrays (up, H, theta, post)
+    const int32_t *nleaf;
+    const double *rate, *ldens;
+    double lc[4];
     int n, S, N;
-    const int32_t *ch, *par, *order, *root;
-    double *h;                   // [n][N]: written for a moment, restored
+    const int32_t *ch, *par, *order, *root, *where;
+    const double *h;             // [n][N]
+    const uint8_t *miss;         // [n][N]
     const int32_t *Z;
     const double *rates;         // [n][2] (r1 conserved, r2 accelerated)
-    const uint8_t *lm;           // [n][S]
     const int32_t *gnode;        // [n]
-    double delta, c2m, n2m;
+    double delta;
     uint32_t key0, key1;
     int m, slot;
-    int32_t *ws_i;
-    double *ws_d;
-    uint8_t *ws_u;
     double *len;                 // [n][N] out: the proposal's branch lengths (the current ones for an empty window)
     gt::Window *win;             // [n] out
     double *prop, *rdraw, *prior_new, *lz_new;   // [n] out
+    int32_t *wnew;               // [n] out: the moved node's species under the proposal
 };
 
 struct GtDecideArgs {
     int n, N;
     const gt::Window *win;
     const double *prop, *rdraw, *prior_new, *lz_new;
+    const int32_t *wnew;
     const double *lnew0, *lnew1;  // [n] the proposals' prefix log-likelihoods
     const int32_t *gnode, *indicator;
     double *h, *loglik, *logpz, *prior;   // the chain state, updated on acceptance
+    int32_t *where;
     int32_t *accepted;
 };
+
+// LDS bytes of one proposal workgroup at N species-tree nodes (the species
+// tree, the element's rows and the move's workspace)
+inline size_t gt_hprop_lds(int N) { return (size_t)122 * N + 4; }
+constexpr size_t GT_HPROP_LDS_MAX = 64 * 1024 - 64;   // and 48 bytes of static LDS
 
 hipError_t launch_gt_hprop(const GtPropArgs &a, hipStream_t st);
 hipError_t launch_gt_hdecide(const GtDecideArgs &a, hipStream_t st);

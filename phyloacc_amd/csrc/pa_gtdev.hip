@@ -2,13 +2,12 @@
 // (GTree::Sample_BranchLen, src/PhyloAcc-GT/genetree.cpp:1658-1965) for a
 // batch of gene-tree elements whose trees stay in HBM between moves.
 //
-// One lane per element runs the move's scalar logic -- the embedding of its
-// gene tree in the species tree, the window, the draws, the proposal's branch
-// lengths, coalescent prior and missing-gene term -- with the same code the
-// host path runs (pa_gtcore.h), on the element's own workspace rows.  The
-// work is pointer-chasing over a few hundred nodes, not a dense contraction:
-// its place is beside the likelihood launch, not in it, and it takes the host
-// threads and the PCIe round trip of every move out of the sweep.
+// A proposal kernel (one wave per element) draws each element's new height
+// and computes the proposed tree's branch lengths, coalescent prior and
+// missing-gene term with the host path's own gene-tree functions
+// (pa_gtcore.h); the likelihood launch reads those lengths in place; a
+// decision kernel (one lane per element) accepts or rejects and updates the
+// chain state.  No host threads and no PCIe round trip per move.
 //
 // No multiply-add contraction in this file: the proposal heights, lengths
 // and priors round every product and sum the way the host path does, so the
@@ -19,45 +18,150 @@
 
 namespace pa {
 
-__global__ void gt_hprop_kernel(GtPropArgs a)
+// One 64-lane workgroup (one wave) per element; the element's rows and the
+// species tree are copied into LDS.  Lane 0 takes the move's few scalar
+// steps (the window, the draws, the moved node's new species); the rest runs
+// across the lanes, with every sum still taken in the host's order so both
+// paths give the same bits:
+// - the prior (GTree::priorTree): gene nodes born per species by LDS atomics;
+//   the lineages entering species x, the leaves below x less the gene nodes
+//   born in internal species strictly below it; each born node's place in its
+//   species' height order by ranking; its coalescence term at that place;
+//   lane 0 adds the terms and each species' tail in species post-order;
+// - the missing-gene term: a term per gene node, added in gene post-order;
+// - the branch lengths, one node per lane.
+__global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int status_sh;
+    __shared__ double lc[4];
+    const int i = blockIdx.x, t = threadIdx.x;
     const int N = a.N, S = a.S;
-    int32_t *where = a.ws_i + (size_t)i * (5 * N + 1);
-    int32_t *end = where + N, *cnt = end + N, *off = cnt + N, *leave = off + N + 1;
-    double *hs = a.ws_d + (size_t)i * N;
-    uint8_t *miss = a.ws_u + (size_t)i * N;
-    double *h = a.h + (size_t)i * N;
-    const int32_t *Z = a.Z + (size_t)i * N;
-    const double r1 = a.rates[2 * i], r2 = a.rates[2 * i + 1];
-    gt::TreeView g = {S, N, a.root[i], a.ch + (size_t)i * 2 * N, a.par + (size_t)i * N, a.order + (size_t)i * N, h,
-                      where, end};
-    gt::embed(a.sp, g);
-    const int v = a.gnode[i];
-    const gt::Window w = gt::window(a.sp, g, v, a.delta);
-    a.win[i] = w;
-    double *len = a.len + (size_t)i * N;
-    if (w.status != 0) {
-        // no proposal: the launch scores the current tree, and the decision skips it
-        gt::lengths(a.sp, g, Z, r1, r2, len);
-        a.prop[i] = h[v];
-        return;
+    double *H = (double *)lds, *theta = H + N, *rate = theta + N, *ldens = rate + N, *h = ldens + N;
+    double *hs = h + N, *tm = hs + N, *lzt = tm + N;
+    int32_t *up = (int32_t *)(lzt + N), *post = up + N, *nleaf = post + N, *ch = nleaf + N, *par = ch + 2 * N;
+    int32_t *order = par + N, *Z = order + N, *where = Z + N, *end = where + N, *cnt = end + N, *below = cnt + N;
+    int32_t *pos = below + N, *off = pos + N;
+    uint8_t *miss = (uint8_t *)(off + N + 1), *lzs = miss + N;
+    const size_t r = (size_t)i * N;
+    for (int k = t; k < N; k += 64) {
+        H[k] = a.sp.H[k];
+        theta[k] = a.sp.theta[k];
+        rate[k] = a.rate[k];
+        ldens[k] = a.ldens[k];
+        up[k] = a.sp.up[k];
+        post[k] = a.sp.post[k];
+        nleaf[k] = a.nleaf[k];
+        h[k] = a.h[r + k];
+        par[k] = a.par[r + k];
+        order[k] = a.order[r + k];
+        Z[k] = a.Z[r + k];
+        where[k] = a.where[r + k];
+        miss[k] = a.miss[r + k];
+        cnt[k] = 0;
+        below[k] = 0;
     }
-    const Key key{a.key0, a.key1};
-    const uint32_t c3 = tagword(0, TAG_GT_BRANCH), c2 = (uint32_t)a.m;
-    const double u = slot_uniform(key, (uint32_t)a.slot, c2, c3, 0);
-    const double prop = fmax(u, 1e-5) * (w.upper - w.lower) + w.lower;
-    a.prop[i] = prop;
-    a.rdraw[i] = slot_uniform(key, (uint32_t)a.slot, c2, c3, 1);
-    const double old = h[v];
-    h[v] = prop;
-    gt::repoint(a.sp, g, v);
-    gt::lengths(a.sp, g, Z, r1, r2, len);
-    a.prior_new[i] = gt::prior(a.sp, g, cnt, off, leave, hs);
-    gt::missing(g, a.lm + (size_t)i * S, miss);
-    a.lz_new[i] = gt::logpz(a.sp, g, miss, Z, a.c2m, a.n2m);
-    h[v] = old;
+    for (int k = t; k < 2 * N; k += 64) ch[k] = a.ch[2 * r + k];
+    if (t < 4) lc[t] = a.lc[t];
+    const int root = a.root[i], v = a.gnode[i];
+    __syncthreads();
+    for (int k = t; k < N; k += 64) end[k] = k == root ? N - 1 : where[par[k]];
+    __syncthreads();
+    const gt::SpView sp = {S, N, up, H, theta, post, nullptr, nullptr};
+    gt::TreeView g = {S, N, root, ch, par, order, h, where, end};
+    if (t == 0) {
+        const gt::Window w = gt::window(sp, g, v, a.delta);
+        a.win[i] = w;
+        status_sh = w.status;
+        if (w.status != 0) {
+            // no proposal: the launch scores the current tree, and the decision skips it
+            a.prop[i] = h[v];
+        } else {
+            const Key key{a.key0, a.key1};
+            const uint32_t c3 = tagword(0, TAG_GT_BRANCH), c2 = (uint32_t)a.m;
+            const double u = slot_uniform(key, (uint32_t)a.slot, c2, c3, 0);
+            const double prop = fmax(u, 1e-5) * (w.upper - w.lower) + w.lower;
+            a.prop[i] = prop;
+            a.rdraw[i] = slot_uniform(key, (uint32_t)a.slot, c2, c3, 1);
+            h[v] = prop;
+            gt::repoint(sp, g, v);
+            a.wnew[i] = where[v];
+        }
+    }
+    __syncthreads();
+    if (status_sh == 0) {
+        for (int u = S + t; u < N; u += 64) {
+            const int x = where[u];
+            atomicAdd(&cnt[x], 1);
+            if (x >= S)
+                for (int z = up[x]; z < N; z = up[z]) atomicAdd(&below[z], 1);
+        }
+        __syncthreads();
+        {
+            // off: the exclusive prefix of cnt, in chunks per lane and a shuffle scan
+            const int c = (N + 63) / 64, lo = min(N, t * c), hi = min(N, lo + c);
+            int sum = 0;
+            for (int x = lo; x < hi; x++) sum += cnt[x];
+            int inc = sum;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y = __shfl_up(inc, d, 64);
+                if (t >= d) inc += y;
+            }
+            int run = inc - sum;
+            for (int x = lo; x < hi; x++) {
+                off[x] = run;
+                run += cnt[x];
+            }
+            if (t == 63) off[N] = inc;
+        }
+        __syncthreads();
+        for (int u = S + t; u < N; u += 64) {
+            const int x = where[u];
+            const double hu = h[u];
+            int rk = 0;
+            for (int q = S; q < N; q++) {
+                const double hq = h[q];
+                rk += where[q] == x && (hq < hu || (hq == hu && q < u));
+            }
+            pos[u] = off[x] + rk;
+            hs[off[x] + rk] = hu;
+        }
+        __syncthreads();
+        for (int u = S + t; u < N; u += 64) {
+            const int x = where[u], ke = nleaf[x] - below[x];
+            if (x < S || ke == 0) continue;
+            const int p = pos[u], rk = p - off[x];
+            tm[p] = gt::coal_term(ldens[x], rate[x], hs[p], rk == 0 ? H[x] : hs[p - 1], ke - rk);
+        }
+        for (int u = t; u < N; u += 64) {
+            double tv = 0;
+            lzs[u] = gt::logpz_term(sp, g, miss, Z, lc, u, &tv) ? 1 : 0;
+            lzt[u] = tv;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double lp = 0;
+            for (int q = 0; q < N; q++) {
+                const int x = post[q];
+                if (x < S) continue;
+                const int ke = nleaf[x] - below[x];
+                if (ke == 0) continue;
+                const int a0 = off[x], b0 = off[x + 1];
+                for (int j = a0; j < b0; j++) lp += tm[j];
+                const int kf = ke - (b0 - a0);
+                if (kf > 1 && up[x] != N) lp -= gt::coal_tail(rate[x], H[up[x]], b0 > a0 ? hs[b0 - 1] : H[x], kf);
+            }
+            a.prior_new[i] = lp;
+            double lz = 0;
+            for (int q = 0; q < N; q++) {
+                const int u = order[q];
+                if (lzs[u]) lz += lzt[u];
+            }
+            a.lz_new[i] = lz;
+        }
+    }
+    const double r1 = a.rates[2 * i], r2 = a.rates[2 * i + 1];
+    for (int k = t; k < N; k += 64) a.len[r + k] = gt::branch_length(sp, g, k, Z, r1, r2);
 }
 
 __global__ void gt_hdecide_kernel(GtDecideArgs a)
@@ -74,6 +178,7 @@ __global__ void gt_hdecide_kernel(GtDecideArgs a)
     if (!(log(a.rdraw[i]) < d)) return;
     a.accepted[i] = 1;
     a.h[(size_t)i * a.N + a.gnode[i]] = a.prop[i];
+    a.where[(size_t)i * a.N + a.gnode[i]] = a.wnew[i];
     a.loglik[2 * i] = lnew[0];
     a.loglik[2 * i + 1] = lnew[1];
     if (a.logpz[2 * i + 1] == 0) a.logpz[2 * i] = a.lz_new[i];
@@ -84,7 +189,9 @@ __global__ void gt_hdecide_kernel(GtDecideArgs a)
 hipError_t launch_gt_hprop(const GtPropArgs &a, hipStream_t st)
 {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gt_hprop_kernel, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
+    const size_t lds = gt_hprop_lds(a.N);
+    if (lds > GT_HPROP_LDS_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gt_hprop_kernel, dim3(a.n), dim3(64), lds, st, a);
     return hipGetLastError();
 }
 

@@ -762,16 +762,16 @@ struct pa_gt_session {
     std::vector<double> H_h, th_h;
     std::vector<int32_t> ids;
     std::vector<double> piA;
-    double c2m = 0, n2m = 0;
     std::vector<void *> bufs;
     // device arrays
-    int32_t *up = nullptr, *post = nullptr, *kid0 = nullptr, *kid1 = nullptr;
-    double *H = nullptr, *theta = nullptr;
-    int32_t *ch = nullptr, *par = nullptr, *order = nullptr, *root = nullptr, *Z = nullptr;
+    double lc[4] = {0, 0, 0, 0};
+    int32_t *up = nullptr, *post = nullptr, *nleaf = nullptr;
+    double *H = nullptr, *theta = nullptr, *rate = nullptr, *ldens = nullptr;
+    int32_t *ch = nullptr, *par = nullptr, *order = nullptr, *root = nullptr, *Z = nullptr, *where = nullptr;
     double *h = nullptr, *rates = nullptr, *ll = nullptr, *lz = nullptr, *prior = nullptr;
-    uint8_t *lm = nullptr, *ws_u = nullptr;
-    int32_t *ws_i = nullptr, *gnode = nullptr, *ind = nullptr, *acc = nullptr;
-    double *ws_d = nullptr, *len = nullptr, *prop = nullptr, *rdraw = nullptr, *prior_new = nullptr, *lz_new = nullptr;
+    uint8_t *miss = nullptr;
+    int32_t *gnode = nullptr, *ind = nullptr, *acc = nullptr, *wnew = nullptr;
+    double *len = nullptr, *prop = nullptr, *rdraw = nullptr, *prior_new = nullptr, *lz_new = nullptr;
     double *out0 = nullptr, *out1 = nullptr;
     gt::Window *win = nullptr;
 
@@ -817,19 +817,39 @@ extern "C" pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *t
     engine_view(e, &ev);
     if (s->sp.S != ev.S || s->sp.N != ev.N) return engine_fail(e, PA_E_INPUT, "species tree size differs from the model's");
     const int S = s->sp.S, N = s->sp.N;
+    if (gt_hprop_lds(N) > GT_HPROP_LDS_MAX)
+        return engine_fail(e, PA_E_INPUT, "species tree too large for the device session (its LDS copy exceeds 64 KiB)");
     std::vector<uint8_t> lm((size_t)n * S);
     for (int i = 0; i < n; i++)
         if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
-    std::vector<int32_t> order((size_t)n * N), root(n);
+    // per element: the post-order, root, embedding and missing flags; the
+    // trees keep their topology for the session's life, so the flags hold
+    std::vector<int32_t> order((size_t)n * N), root(n), where((size_t)n * N);
+    std::vector<uint8_t> miss((size_t)n * N);
     std::vector<int> bad(n, 0);
     const Sp &sp = s->sp;
     for_elements(n, [&](int i) {
         thread_local Embed g;
+        thread_local std::vector<uint8_t> mi;
         g.root = -1;
         if (!g.build(sp, gchildren + (size_t)i * 2 * N, gparent + (size_t)i * N, gheight + (size_t)i * N)) { bad[i] = 1; return; }
         memcpy(&order[(size_t)i * N], g.order.data(), sizeof(int32_t) * N);
+        memcpy(&where[(size_t)i * N], g.where.data(), sizeof(int32_t) * N);
+        missing_of(g, &lm[(size_t)i * S], mi);
+        memcpy(&miss[(size_t)i * N], mi.data(), N);
         root[i] = g.root;
     });
+    // per species: the leaves below it, and the prior's 1 / theta and log(2 / theta)
+    // with the host's own arithmetic
+    std::vector<int32_t> nleaf(N, 0);
+    std::vector<double> rate(N), ldens(N);
+    for (int q = 0; q < N; q++) {
+        const int x = sp.post[q];
+        nleaf[x] = x < S ? 1 : nleaf[sp.kid0[x]] + nleaf[sp.kid1[x]];
+        rate[x] = 1.0 / sp.theta[x];
+        ldens[x] = std::log(2.0 * rate[x]);
+    }
+    gt::logpz_consts(cons_to_mis, ncons_to_mis, s->lc);
     for (int i = 0; i < n; i++)
         if (bad[i]) return engine_fail(e, PA_E_INPUT, std::string(BAD_GENE_TREE) + " (element " + std::to_string(i) + ")");
     s->ids.assign(ids, ids + n);
@@ -838,20 +858,19 @@ extern "C" pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *t
             if (!(piA[i] > 0 && piA[i] < 0.5)) return engine_fail(e, PA_E_INPUT, "piA must lie in (0, 1/2)");
         s->piA.assign(piA, piA + n);
     }
-    s->c2m = cons_to_mis;
-    s->n2m = ncons_to_mis;
     hipStream_t stream;
     if ((st = engine_stream(e, &stream)) != PA_OK) return st;
     const size_t nN = (size_t)n * N;
     hipError_t r = hipSuccess;
     auto chk = [&](hipError_t x) { if (r == hipSuccess) r = x; };
     chk(s->get(&s->up, N, sp.up)); chk(s->get(&s->H, N, sp.H)); chk(s->get(&s->theta, N, sp.theta));
-    chk(s->get(&s->post, N, sp.post.data())); chk(s->get(&s->kid0, N, sp.kid0.data())); chk(s->get(&s->kid1, N, sp.kid1.data()));
+    chk(s->get(&s->post, N, sp.post.data())); chk(s->get(&s->nleaf, N, nleaf.data()));
+    chk(s->get(&s->rate, N, rate.data())); chk(s->get(&s->ldens, N, ldens.data()));
+    chk(s->get(&s->where, nN, where.data())); chk(s->get(&s->miss, nN, miss.data())); chk(s->get(&s->wnew, n));
     chk(s->get(&s->ch, 2 * nN, gchildren)); chk(s->get(&s->par, nN, gparent)); chk(s->get(&s->order, nN, order.data()));
     chk(s->get(&s->root, n, root.data())); chk(s->get(&s->h, nN, gheight)); chk(s->get(&s->Z, nN, Z));
-    chk(s->get(&s->rates, 2 * (size_t)n, rates)); chk(s->get(&s->lm, (size_t)n * S, lm.data()));
+    chk(s->get(&s->rates, 2 * (size_t)n, rates));
     chk(s->get(&s->ll, 2 * (size_t)n, loglik)); chk(s->get(&s->lz, 2 * (size_t)n, logpz)); chk(s->get(&s->prior, n, prior));
-    chk(s->get(&s->ws_i, (size_t)n * (5 * N + 1))); chk(s->get(&s->ws_d, nN)); chk(s->get(&s->ws_u, nN));
     chk(s->get(&s->len, nN)); chk(s->get(&s->win, n)); chk(s->get(&s->prop, n)); chk(s->get(&s->rdraw, n));
     chk(s->get(&s->prior_new, n)); chk(s->get(&s->lz_new, n)); chk(s->get(&s->out0, n)); chk(s->get(&s->out1, n));
     chk(s->get(&s->gnode, n)); chk(s->get(&s->ind, n)); chk(s->get(&s->acc, n));
@@ -886,14 +905,16 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
     EngineView ev;
     engine_view(e, &ev);
     GtPropArgs a;
-    a.sp = {S, N, s->up, s->H, s->theta, s->post, s->kid0, s->kid1};
+    a.sp = {S, N, s->up, s->H, s->theta, s->post, nullptr, nullptr};
+    a.nleaf = s->nleaf; a.rate = s->rate; a.ldens = s->ldens;
+    for (int k = 0; k < 4; k++) a.lc[k] = s->lc[k];
     a.n = n; a.S = S; a.N = N;
-    a.ch = s->ch; a.par = s->par; a.order = s->order; a.root = s->root;
-    a.h = s->h; a.Z = s->Z; a.rates = s->rates; a.lm = s->lm; a.gnode = s->gnode;
-    a.delta = delta; a.c2m = s->c2m; a.n2m = s->n2m;
+    a.ch = s->ch; a.par = s->par; a.order = s->order; a.root = s->root; a.where = s->where;
+    a.h = s->h; a.miss = s->miss; a.Z = s->Z; a.rates = s->rates; a.gnode = s->gnode;
+    a.delta = delta;
     a.key0 = ev.key0; a.key1 = ev.key1; a.m = m; a.slot = slot;
-    a.ws_i = s->ws_i; a.ws_d = s->ws_d; a.ws_u = s->ws_u;
     a.len = s->len; a.win = s->win; a.prop = s->prop; a.rdraw = s->rdraw; a.prior_new = s->prior_new; a.lz_new = s->lz_new;
+    a.wnew = s->wnew;
     if (launch_gt_hprop(a, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height proposal launch failed");
     GtExtra x;
     x.d_len = s->len;
@@ -906,6 +927,7 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
     GtDecideArgs d;
     d.n = n; d.N = N;
     d.win = s->win; d.prop = s->prop; d.rdraw = s->rdraw; d.prior_new = s->prior_new; d.lz_new = s->lz_new;
+    d.wnew = s->wnew; d.where = s->where;
     d.lnew0 = s->out0; d.lnew1 = s->out1; d.gnode = s->gnode; d.indicator = s->ind;
     d.h = s->h; d.loglik = s->ll; d.logpz = s->lz; d.prior = s->prior; d.accepted = s->acc;
     if (launch_gt_hdecide(d, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height decision launch failed");
