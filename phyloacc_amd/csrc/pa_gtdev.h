@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "pa_gtcore.h"
+#include "pa_common.h"
 
 namespace pa {
 
@@ -53,6 +54,12 @@ inline size_t gt_hprop_lds(int N) { return (size_t)122 * N + 4; }
 constexpr size_t GT_HPROP_LDS_MAX = 64 * 1024 - 64;   // and 48 bytes of static LDS
 
 hipError_t launch_gt_hprop(const GtPropArgs &a, hipStream_t st);
+
+// Element k's new node table (W words at src + k W) to d_tab + dst_off[k],
+// its record to d_elems[dst_id[k]]: one workgroup per element.
+hipError_t launch_gt_scatter_tables(const int32_t *src, int W, const int64_t *dst_off, int32_t *d_tab,
+                                    const ElemDev *rec, const int32_t *dst_id, ElemDev *d_elems, int n,
+                                    hipStream_t st);
 hipError_t launch_gt_hdecide(const GtDecideArgs &a, hipStream_t st);
 
 }  // namespace pa

@@ -186,6 +186,30 @@ __global__ void gt_hdecide_kernel(GtDecideArgs a)
     a.prior[i] = a.prior_new[i];
 }
 
+__global__ __launch_bounds__(256) void gt_scatter_tables_kernel(const int32_t *src, int W, const int64_t *dst_off,
+                                                                 int32_t *d_tab, const ElemDev *rec,
+                                                                 const int32_t *dst_id, ElemDev *d_elems)
+{
+    const int k = blockIdx.x;
+    int32_t *dst = d_tab + dst_off[k];
+    const int32_t *s = src + (size_t)k * W;
+    for (int w = threadIdx.x; w < W; w += blockDim.x) dst[w] = s[w];
+    constexpr int RW = (int)(sizeof(ElemDev) / 4);
+    if (threadIdx.x < RW)
+        ((int32_t *)(d_elems + dst_id[k]))[threadIdx.x] = ((const int32_t *)(rec + k))[threadIdx.x];
+}
+
+hipError_t launch_gt_scatter_tables(const int32_t *src, int W, const int64_t *dst_off, int32_t *d_tab,
+                                    const ElemDev *rec, const int32_t *dst_id, ElemDev *d_elems, int n,
+                                    hipStream_t st)
+{
+    static_assert(sizeof(ElemDev) % 4 == 0 && sizeof(ElemDev) / 4 <= 256, "element record as words");
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gt_scatter_tables_kernel, dim3(n), dim3(256), 0, st, src, W, dst_off, d_tab, rec, dst_id,
+                       d_elems);
+    return hipGetLastError();
+}
+
 hipError_t launch_gt_hprop(const GtPropArgs &a, hipStream_t st)
 {
     if (a.n <= 0) return hipSuccess;

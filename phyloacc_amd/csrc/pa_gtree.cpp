@@ -747,9 +747,14 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         else logpz[2 * i] = logpz[2 * i + 1] = lz;
     });
     // the accepted trees become the elements' stored trees
+    std::vector<int32_t> sid;
+    std::vector<const int32_t *> sch;
     for (int i = 0; i < n; i++)
-        if (accepted[i] && (st = element_set_tree(e, ids[i], nch[i].data())) != PA_OK) return st;
-    return PA_OK;
+        if (accepted[i]) {
+            sid.push_back(ids[i]);
+            sch.push_back(nch[i].data());
+        }
+    return element_set_trees(e, sid.data(), (int)sid.size(), sch.data());
 }
 
 // ------------------------------------------------ device-resident height moves

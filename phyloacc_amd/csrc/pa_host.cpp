@@ -25,6 +25,7 @@
 #include "pa_rng.h"
 #include "pa_eig.h"
 #include "pa_internal.h"
+#include "pa_gtdev.h"
 
 using namespace pa;
 
@@ -1389,26 +1390,64 @@ pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t 
     return PA_OK;
 }
 
-pa_status element_set_tree(pa_engine *e, int32_t id, const int32_t *children)
+// Many elements' trees at once: every table is built first (nothing
+// changes if one fails), then the host copies change and, when the device
+// copy is current, one upload of the new tables and records and one scatter
+// launch replace them on the device.
+pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32_t *const *children)
 {
-    HostElem tmp;
-    pa_status st = gene_elem(e, id, children, tmp);
-    if (st != PA_OK) return st;
-    const int N = e->tm.N;
-    HostElem &E = e->elems[id];
-    // gene trees take their branch lengths per call; the stored ones stay
-    memcpy(tmp.tab.data() + tab_dist(N), E.tab.data() + tab_dist(N), (size_t)(tab_words(N) - tab_dist(N)) * 4);
-    E.tab = std::move(tmp.tab);
-    E.root = tmp.root; E.n_up = tmp.n_up; E.n_down = tmp.n_down;
-    E.nodes = std::move(tmp.nodes);
-    E.internal = std::move(tmp.internal);
-    if (!e->dev_dirty && e->d_tab && id < (int)e->dev_elems.size()) {
+    if (n <= 0) return PA_OK;
+    const int N = e->tm.N, W = tab_words(N);
+    std::vector<HostElem> tmp(n);
+    std::vector<pa_status> sts(n, PA_OK);
+    {
+        const unsigned nt = n < 256 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        auto work = [&](unsigned t) {
+            for (int k = (int)t; k < n; k += (int)nt) sts[k] = gene_elem(e, ids[k], children[k], tmp[k]);
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
+        work(0);
+        for (auto &x : th) x.join();
+    }
+    for (int k = 0; k < n; k++)
+        if (sts[k] != PA_OK) return sts[k];
+    const bool dev = !e->dev_dirty && e->d_tab && e->d_elems;
+    std::vector<int32_t> tabs(dev ? (size_t)n * W : 0), dst_id(dev ? n : 0);
+    std::vector<int64_t> dst_off(dev ? n : 0);
+    std::vector<ElemDev> recs(dev ? n : 0);
+    for (int k = 0; k < n; k++) {
+        const int id = ids[k];
+        HostElem &E = e->elems[id];
+        HostElem &t = tmp[k];
+        // gene trees take their branch lengths per call; the stored ones stay
+        memcpy(t.tab.data() + tab_dist(N), E.tab.data() + tab_dist(N), (size_t)(W - tab_dist(N)) * 4);
+        E.tab = std::move(t.tab);
+        E.root = t.root; E.n_up = t.n_up; E.n_down = t.n_down;
+        E.nodes = std::move(t.nodes);
+        E.internal = std::move(t.internal);
+        if (!dev || id >= (int)e->dev_elems.size()) continue;
         ElemDev &x = e->dev_elems[id];
         x.root = E.root; x.n_nodes = (int)E.nodes.size(); x.n_int = (int)E.internal.size();
         x.n_up = E.n_up; x.n_down = E.n_down;
-        HIPCHK(e, hipMemcpy(e->d_tab + x.off_tab, E.tab.data(), (size_t)tab_words(N) * 4, hipMemcpyHostToDevice));
-        HIPCHK(e, hipMemcpy(e->d_elems + id, &x, sizeof x, hipMemcpyHostToDevice));
+        memcpy(&tabs[(size_t)k * W], E.tab.data(), (size_t)W * 4);
+        dst_id[k] = id;
+        dst_off[k] = x.off_tab;
+        recs[k] = x;
     }
+    if (!dev) return PA_OK;
+    DevBuf b[4];
+    HIPCHK(e, hipMalloc(&b[0].p, tabs.size() * 4));
+    HIPCHK(e, hipMalloc(&b[1].p, (size_t)n * sizeof(ElemDev)));
+    HIPCHK(e, hipMalloc(&b[2].p, (size_t)n * 8));
+    HIPCHK(e, hipMalloc(&b[3].p, (size_t)n * 4));
+    HIPCHK(e, hipMemcpy(b[0].p, tabs.data(), tabs.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[1].p, recs.data(), (size_t)n * sizeof(ElemDev), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[2].p, dst_off.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[3].p, dst_id.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, launch_gt_scatter_tables((const int32_t *)b[0].p, W, (const int64_t *)b[2].p, e->d_tab,
+                                       (const ElemDev *)b[1].p, (const int32_t *)b[3].p, e->d_elems, n, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
     return PA_OK;
 }
 

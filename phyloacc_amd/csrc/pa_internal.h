@@ -58,8 +58,8 @@ pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *l
                     const GtExtra &x);
 // Node table of element id's leaves under the tree children [N*2].
 pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr);
-// Makes children [N*2] the element's stored tree (host and, when uploaded, device).
-pa_status element_set_tree(pa_engine *e, int32_t id, const int32_t *children);
+// Makes children[k] [N*2] the stored tree of element ids[k] (host and, when uploaded, device).
+pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32_t *const *children);
 // The engine's stream (after the device is set up), for launches that a
 // likelihood launch on it must follow.
 pa_status engine_stream(pa_engine *e, hipStream_t *st);
