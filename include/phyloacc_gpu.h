@@ -329,13 +329,15 @@ pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *sp, const int32
  * (loglik [n*2], logpz [n*2], prior [n], as pa_gt_branch_len keeps them) are
  * copied to HBM once; each pa_gt_session_branch_len is then one
  * GTree::Sample_BranchLen move for every element with the proposal, the
- * likelihood launch and the decision all on the device (one lane per element
- * for the move's logic, the same code as pa_gt_branch_len), so a sweep of
- * moves needs no host round trip; pa_gt_session_read returns the heights and
- * state.  The trees must be the elements' stored trees and keep their
- * topology while the session lives (a regraft needs a new session).  Moves,
- * draws and decisions are pa_gt_branch_len's; accepted (n, or NULL) receives
- * the decisions. */
+ * likelihood launch and the decision all on the device (one wave per element
+ * for the move's logic, the same gene-tree code as pa_gt_branch_len), so a
+ * sweep of moves needs no host round trip; pa_gt_session_read returns the
+ * heights and state.  The trees must be the elements' stored trees and keep
+ * their topology while the session lives (a regraft needs a new session).
+ * Moves, draws, decisions, heights, prior and missing-gene term are
+ * pa_gt_branch_len's, bit for bit; accepted (n, or NULL) receives the
+ * decisions.  Species trees up to N = 536 nodes (an LDS copy per element);
+ * larger ones give PA_E_INPUT. */
 typedef struct pa_gt_session pa_gt_session;
 pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *sp, const int32_t *ids, int32_t n,
                                const int32_t *gchildren, const int32_t *gparent, const double *gheight,
