@@ -69,6 +69,9 @@ __device__ __forceinline__ double block_sum(double v, LDS double *red)
 // the node is a cherry (a non-root internal node whose children are both
 // leaves), whose partial is recomputed where it is read.
 constexpr uint8_t NF_CHERRY = 0x80;
+// Node flag of resident partials: the node's partial lives in its second
+// slot (at node + N - S) rather than its first.
+constexpr uint8_t NF_ALT = 0x40;
 
 template <typename R>
 __device__ __forceinline__ void leaf_mul(const LDS R *PT, int code, R &a0, R &a1, R &a2, R &a3)

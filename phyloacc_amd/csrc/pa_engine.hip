@@ -61,6 +61,7 @@ template <typename R>
 struct Ctx {
     // element
     int S, N, P, Ppad, root, n_nodes, n_up;
+    int h0;                    // first tree level the pass visits
     const GBL uint8_t *codes;
     const GBL int32_t *wts;
     int pre0, pre1;            // site-prefix root sums (pre0 < 0: the pattern sum)
@@ -122,15 +123,26 @@ size_t loglik_smem_bytes(int N, int fp64)
            (fp64 ? sizeof(Ctx<double>) : sizeof(Ctx<float>)) + sizeof(ModelDev) + sizeof(LoglikArgs);
 }
 
-// Byte offset of the exponents in a slot, and the slot size.
-__host__ __device__ inline size_t slot_exps_off(int N, int S, int Ppad, int fp64)
+// Byte offset of the exponents in a slot of I node slots, and the slot size.
+__host__ __device__ inline size_t slot_exps_off(int I, int Ppad, int fp64)
 {
-    return al((size_t)(N - S) * Ppad * 4 * (fp64 ? 8 : 4), 256);
+    return al((size_t)I * Ppad * 4 * (fp64 ? 8 : 4), 256);
 }
 
 size_t loglik_slot_bytes(int N, int S, int Pcap, int fp64)
 {
-    return al(slot_exps_off(N, S, Pcap, fp64) + (size_t)(N - S) * Pcap * 4, 256);
+    return al(slot_exps_off(N - S, Pcap, fp64) + (size_t)(N - S) * Pcap * 4, 256);
+}
+
+size_t loglik_resident_bytes(int N, int S, int Ppad, int fp64)
+{
+    return al(slot_exps_off(2 * (N - S), Ppad, fp64) + (size_t)2 * (N - S) * Ppad * 4, 256);
+}
+
+// Node x's slot (x >= S) under its flags f: its first, or with NF_ALT its second.
+__device__ __forceinline__ size_t node_slot(int x, int f, int S, int N)
+{
+    return (size_t)(x - S + ((f & NF_ALT) ? N - S : 0));
 }
 
 // ------------------------------------------------------------------ helpers
@@ -276,7 +288,7 @@ template <typename R, int BS>
 __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
 {
     const int tid = threadIdx.x;
-    const int P = uni(c->P), Ppad = uni(c->Ppad), S = uni(c->S);
+    const int P = uni(c->P), Ppad = uni(c->Ppad), S = uni(c->S), N = uni(c->N);
     GBL V4<R> *parts = uni_ptr(c->parts);
     GBL int32_t *exps = uni_ptr(c->exps);
     const GBL uint8_t *codes = uni_ptr(c->codes);
@@ -285,7 +297,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
     const int n_up = uni(c->n_up);
     // every node of the first level (height 1) is a cherry, unless the tree is
     // one cherry: its root
-    for (int h = n_up > 1 ? 1 : 0; h < n_up; h++) {
+    for (int h = uni(c->h0); h < n_up; h++) {
         // level h: up[upoff[h-1] .. upoff[h]) (from 0 for the first level)
         const int a = uni(h == 0 ? 0 : c->upoff[h - 1]), b = uni(c->upoff[h]);
         const int items = (b - a) * Pb;
@@ -314,8 +326,8 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
                 const bool leaf0 = x0 < S, leaf1 = x1 < S;
                 const int f0 = uni(c->flags[x0]), f1 = uni(c->flags[x1]);
                 const bool miss0 = f0 & NF_MISSING, miss1 = f1 & NF_MISSING;
-                const size_t o0 = leaf0 ? 0 : (size_t)(x0 - S) * Ppad;
-                const size_t o1 = leaf1 ? 0 : (size_t)(x1 - S) * Ppad;
+                const size_t o0 = leaf0 ? 0 : node_slot(x0, f0, S, N) * Ppad;
+                const size_t o1 = leaf1 ? 0 : node_slot(x1, f1, S, N) * Ppad;
                 if (p < P) {
                     // a cherry child: its two leaf codes, packed
                     if (f0 & NF_CHERRY)
@@ -340,7 +352,8 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
                 const int s = uni(c->up[a + q]);
                 const int x0 = uni(c->ch0[s]), x1 = uni(c->ch1[s]);
                 const bool leaf0 = x0 < S, leaf1 = x1 < S;
-                const bool miss_s = uni(c->flags[s]) & NF_MISSING;
+                const int fs = uni(c->flags[s]);
+                const bool miss_s = fs & NF_MISSING;
                 const int f0 = uni(c->flags[x0]), f1 = uni(c->flags[x1]);
                 const bool miss0 = f0 & NF_MISSING, miss1 = f1 & NF_MISSING;
                 if (f0 & NF_CHERRY) {
@@ -353,8 +366,8 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
                     in1[u] = cherry_in<R>(c->PT + 16 * yA, c->PT + 16 * yB, uni(c->flags[yA]) & NF_MISSING,
                                           uni(c->flags[yB]) & NF_MISSING, miss1, in1[u].x & 255, in1[u].x >> 8);
                 }
-                GBL V4<R> *dstp = parts + (size_t)(s - S) * Ppad;
-                GBL int32_t *dste = exps + (size_t)(s - S) * Ppad;
+                GBL V4<R> *dstp = parts + node_slot(s, fs, S, N) * Ppad;
+                GBL int32_t *dste = exps + node_slot(s, fs, S, N) * Ppad;
                 R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
                 int e = 0;
                 bool g0, g1;
@@ -385,8 +398,9 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
         __syncthreads();
         const int pre1 = uni(c->pre1);
         const GBL int32_t *site = uni_ptr(c->site);
-        const GBL V4<R> *rp = parts + (size_t)(r - S) * Ppad;
-        const GBL int32_t *re = exps + (size_t)(r - S) * Ppad;
+        const size_t ro = node_slot(r, uni(c->flags[r]), S, N) * Ppad;
+        const GBL V4<R> *rp = parts + ro;
+        const GBL int32_t *re = exps + ro;
         const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
         for (int g = tid; g < pre0; g += BS) {
             const int p = site[g];
@@ -403,8 +417,9 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
         return block_sum(acc, c->red);
     }
     if (r >= S) {
-        const GBL V4<R> *rp = parts + (size_t)(r - S) * Ppad;
-        const GBL int32_t *re = exps + (size_t)(r - S) * Ppad;
+        const size_t ro = node_slot(r, uni(c->flags[r]), S, N) * Ppad;
+        const GBL V4<R> *rp = parts + ro;
+        const GBL int32_t *re = exps + ro;
         const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
         for (int pb = tid; pb < P; pb += U * BS) {
             int ex[U], wt[U];
@@ -445,8 +460,9 @@ __device__ inline ChildIn<R> node_in(CP<R> c, int v, int p)
                             c->flags[v] & NF_MISSING, c->codes[(size_t)yA * Ppad + p], c->codes[(size_t)yB * Ppad + p]);
     }
     ChildIn<R> in;
-    in.x = c->exps[(size_t)(v - S) * Ppad + p];
-    in.v = ld4<R>(c->parts + (size_t)(v - S) * Ppad + p);
+    const size_t o = node_slot(v, c->flags[v], S, c->N) * Ppad;
+    in.x = c->exps[o + p];
+    in.v = ld4<R>(c->parts + o + p);
     return in;
 }
 
@@ -617,7 +633,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
         const int64_t bo = A->big_off ? A->big_off[w] : -1;
         char *base = bo >= 0 ? A->big + bo : A->scratch + (int64_t)blockIdx.x * A->slot_bytes;
         c->parts = (GBL V4<R> *)base;
-        c->exps = (GBL int32_t *)(base + slot_exps_off(N, M->S, E.Ppad, sizeof(R) == 8));
+        c->exps = (GBL int32_t *)(base + slot_exps_off((A->alt ? 2 : 1) * (N - M->S), E.Ppad, sizeof(R) == 8));
     }
     const int32_t *tab = A->tab_call ? A->tab_call + (size_t)row * tab_words(N) : A->tab + E.off_tab;
     for (int i = tid; i < N; i += BS) {
@@ -632,6 +648,32 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
     // cherries (both children leaves; never the root) are recomputed where read
     if (c->n_up > 1)
         for (int q = tid; q < c->upoff[0]; q += BS) c->flags[c->up[q]] |= NF_CHERRY;
+    __syncthreads();
+    if (A->alt)
+        for (int i = tid; i < N; i += BS)
+            if (A->alt[(size_t)row * N + i]) c->flags[i] |= NF_ALT;
+    __syncthreads();
+    if (tid == 0) {
+        c->h0 = c->n_up > 1 ? 1 : 0;
+        const int v = A->path_node ? A->path_node[row] : -1;
+        if (v >= c->S && v < N) {
+            // the pass visits the path from v to the root (cherries are
+            // recomputed where read), one node per level, writing each into
+            // the slot it does not hold now
+            int k = 0;
+            for (int x = v; x >= 0 && x < N && k < N; x = tab[tab_par(N) + x]) {
+                c->flags[x] ^= NF_ALT;
+                if (!(c->flags[x] & NF_CHERRY)) {
+                    c->up[k] = (int16_t)x;
+                    c->upoff[k] = (int16_t)(k + 1);
+                    k++;
+                }
+                if (x == c->root) break;
+            }
+            c->n_up = k;
+            c->h0 = 0;
+        }
+    }
     const bool own = A->len_in != nullptr, gb = A->gb_eig != nullptr;
     const double r1 = own || gb ? 1.0 : A->rates_in[2 * w], r2 = own || gb ? 1.0 : A->rates_in[2 * w + 1];
     const double *dist = (const double *)(tab + tab_dist(N));

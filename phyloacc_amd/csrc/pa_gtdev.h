@@ -30,6 +30,7 @@ struct GtPropArgs {
     double delta;
     uint32_t key0, key1;
     int m, slot;
+    int lengths_only;            // 1: no move, the current trees' lengths only
     double *len;                 // [n][N] out: the proposal's branch lengths (the current ones for an empty window)
     gt::Window *win;             // [n] out
     double *prop, *rdraw, *prior_new, *lz_new;   // [n] out
@@ -45,6 +46,10 @@ struct GtDecideArgs {
     const int32_t *gnode, *indicator;
     double *h, *loglik, *logpz, *prior;   // the chain state, updated on acceptance
     int32_t *where;
+    // resident partials: on acceptance the nodes from gnode to the root
+    // switch slots (the launch wrote their new partials into the other one)
+    const int32_t *par, *root;
+    uint8_t *alt;                // [n][N]
     int32_t *accepted;
 };
 

@@ -69,7 +69,9 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
     __syncthreads();
     const gt::SpView sp = {S, N, up, H, theta, post, nullptr, nullptr};
     gt::TreeView g = {S, N, root, ch, par, order, h, where, end};
-    if (t == 0) {
+    if (t == 0 && a.lengths_only) {
+        status_sh = 1;
+    } else if (t == 0) {
         const gt::Window w = gt::window(sp, g, v, a.delta);
         a.win[i] = w;
         status_sh = w.status;
@@ -179,6 +181,14 @@ __global__ void gt_hdecide_kernel(GtDecideArgs a)
     a.accepted[i] = 1;
     a.h[(size_t)i * a.N + a.gnode[i]] = a.prop[i];
     a.where[(size_t)i * a.N + a.gnode[i]] = a.wnew[i];
+    if (a.alt) {
+        const int32_t *par = a.par + (size_t)i * a.N;
+        uint8_t *alt = a.alt + (size_t)i * a.N;
+        for (int x = a.gnode[i], k = 0; x >= 0 && x < a.N && k < a.N; x = par[x], k++) {
+            alt[x] ^= 1;
+            if (x == a.root[i]) break;
+        }
+    }
     a.loglik[2 * i] = lnew[0];
     a.loglik[2 * i + 1] = lnew[1];
     if (a.logpz[2 * i + 1] == 0) a.logpz[2 * i] = a.lz_new[i];

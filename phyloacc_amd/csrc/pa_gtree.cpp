@@ -24,6 +24,7 @@
 #include "pa_common.h"
 #include "pa_gtcore.h"
 #include "pa_gtdev.h"
+#include "pa_kernels.h"
 #include "pa_internal.h"
 #include "pa_rng.h"
 
@@ -779,6 +780,11 @@ struct pa_gt_session {
     double *len = nullptr, *prop = nullptr, *rdraw = nullptr, *prior_new = nullptr, *lz_new = nullptr;
     double *out0 = nullptr, *out1 = nullptr;
     gt::Window *win = nullptr;
+    // resident partials of the current trees, at one precision
+    int res_prec = -1;
+    char *big = nullptr;
+    uint8_t *alt = nullptr;
+    std::vector<int64_t> big_off;
 
     template <typename T>
     hipError_t get(T **p, size_t count, const void *src = nullptr)
@@ -794,8 +800,68 @@ struct pa_gt_session {
     ~pa_gt_session()
     {
         for (void *p : bufs) (void)hipFree(p);
+        (void)hipFree(big);
     }
 };
+
+static void session_prop_args(pa_gt_session *s, GtPropArgs &a)
+{
+    const int S = s->sp.S, N = s->sp.N;
+    a.sp = {S, N, s->up, s->H, s->theta, s->post, nullptr, nullptr};
+    a.nleaf = s->nleaf; a.rate = s->rate; a.ldens = s->ldens;
+    for (int k = 0; k < 4; k++) a.lc[k] = s->lc[k];
+    a.n = s->n; a.S = S; a.N = N;
+    a.ch = s->ch; a.par = s->par; a.order = s->order; a.root = s->root; a.where = s->where;
+    a.h = s->h; a.miss = s->miss; a.Z = s->Z; a.rates = s->rates; a.gnode = s->gnode;
+    a.delta = 1.0; a.key0 = a.key1 = 0; a.m = a.slot = 0; a.lengths_only = 0;
+    a.len = s->len; a.win = s->win; a.prop = s->prop; a.rdraw = s->rdraw; a.prior_new = s->prior_new; a.lz_new = s->lz_new;
+    a.wnew = s->wnew;
+}
+
+// The partials of every element's current tree, kept in HBM at this
+// precision (two slots per internal node, all in their first): the current
+// trees' lengths, then one full pass that stores every node's partial.
+static pa_status session_resident(pa_gt_session *s, int32_t precision)
+{
+    pa_engine *e = s->e;
+    const int n = s->n, S = s->sp.S, N = s->sp.N, fp64 = precision == PA_FP64;
+    if (precision != PA_FP32 && precision != PA_FP64) return engine_fail(e, PA_E_INPUT, "precision must be PA_FP32 or PA_FP64");
+    hipStream_t stream;
+    pa_status st = engine_stream(e, &stream);
+    if (st != PA_OK) return st;
+    (void)hipStreamSynchronize(stream);
+    s->res_prec = -1;
+    (void)hipFree(s->big);
+    s->big = nullptr;
+    s->big_off.assign(n, 0);
+    size_t total = 0;
+    for (int i = 0; i < n; i++) {
+        const int pp = element_ppad(e, s->ids[i]);
+        if (pp < 0) return engine_fail(e, PA_E_INPUT, "unknown element id");
+        s->big_off[i] = (int64_t)total;
+        total += loglik_resident_bytes(N, S, pp, fp64);
+    }
+    if (hipMalloc(&s->big, std::max<size_t>(total, 256)) != hipSuccess)
+        return engine_fail(e, PA_E_OOM, "gene-tree session: no device memory for the resident partials (" +
+                                         std::to_string(total >> 20) + " MiB)");
+    if (!s->alt && s->get(&s->alt, (size_t)n * N) != hipSuccess) return engine_fail(e, PA_E_OOM, "session slot bits");
+    if (hipMemsetAsync(s->alt, 0, (size_t)n * N, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "session slot bits");
+    GtPropArgs a;
+    session_prop_args(s, a);
+    a.lengths_only = 1;
+    if (launch_gt_hprop(a, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "session lengths launch failed");
+    GtExtra x;
+    x.d_len = s->len;
+    x.d_big = s->big;
+    x.row_big_off = s->big_off.data();
+    x.d_alt = s->alt;
+    std::vector<double> o0(n), o1(n);
+    if ((st = gt_launch(e, s->ids.data(), n, nullptr, s->piA.empty() ? nullptr : s->piA.data(), nullptr, nullptr,
+                        precision, o0.data(), o1.data(), x)) != PA_OK)
+        return st;
+    s->res_prec = precision;
+    return PA_OK;
+}
 
 extern "C" pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *t, const int32_t *ids, int32_t n,
                                           const int32_t *gchildren, const int32_t *gparent, const double *gheight,
@@ -901,30 +967,30 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
         l0[i] = lens[2 * i + indicator[i]];
         l1[i] = std::min(lens[2 * i + 1], l0[i]);
     }
+    pa_status st;
+    if (s->res_prec != precision && (st = session_resident(s, precision)) != PA_OK) return st;
     hipStream_t stream;
-    pa_status st = engine_stream(e, &stream);
-    if (st != PA_OK) return st;
+    if ((st = engine_stream(e, &stream)) != PA_OK) return st;
     if (hipMemcpy(s->gnode, gnode, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->ind, indicator, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess)
         return engine_fail(e, PA_E_DEVICE, "session upload failed");
     EngineView ev;
     engine_view(e, &ev);
     GtPropArgs a;
-    a.sp = {S, N, s->up, s->H, s->theta, s->post, nullptr, nullptr};
-    a.nleaf = s->nleaf; a.rate = s->rate; a.ldens = s->ldens;
-    for (int k = 0; k < 4; k++) a.lc[k] = s->lc[k];
-    a.n = n; a.S = S; a.N = N;
-    a.ch = s->ch; a.par = s->par; a.order = s->order; a.root = s->root; a.where = s->where;
-    a.h = s->h; a.miss = s->miss; a.Z = s->Z; a.rates = s->rates; a.gnode = s->gnode;
+    session_prop_args(s, a);
     a.delta = delta;
     a.key0 = ev.key0; a.key1 = ev.key1; a.m = m; a.slot = slot;
-    a.len = s->len; a.win = s->win; a.prop = s->prop; a.rdraw = s->rdraw; a.prior_new = s->prior_new; a.lz_new = s->lz_new;
-    a.wnew = s->wnew;
     if (launch_gt_hprop(a, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height proposal launch failed");
+    // the proposals' likelihoods: each element's path from the moved node to
+    // the root, into the slots its current partials do not use
     GtExtra x;
     x.d_len = s->len;
     x.d_row0 = s->out0;
     x.d_row1 = s->out1;
+    x.d_big = s->big;
+    x.row_big_off = s->big_off.data();
+    x.d_alt = s->alt;
+    x.d_path = s->gnode;
     std::vector<double> o0(n), o1(n);
     if ((st = gt_launch(e, s->ids.data(), n, nullptr, s->piA.empty() ? nullptr : s->piA.data(), l0.data(), l1.data(),
                         precision, o0.data(), o1.data(), x)) != PA_OK)
@@ -933,6 +999,7 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
     d.n = n; d.N = N;
     d.win = s->win; d.prop = s->prop; d.rdraw = s->rdraw; d.prior_new = s->prior_new; d.lz_new = s->lz_new;
     d.wnew = s->wnew; d.where = s->where;
+    d.par = s->par; d.root = s->root; d.alt = s->alt;
     d.lnew0 = s->out0; d.lnew1 = s->out1; d.gnode = s->gnode; d.indicator = s->ind;
     d.h = s->h; d.loglik = s->ll; d.logpz = s->lz; d.prior = s->prior; d.accepted = s->acc;
     if (launch_gt_hdecide(d, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height decision launch failed");

@@ -947,11 +947,14 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     const size_t smem = loglik_smem_bytes(N, fp64);
     if (smem > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
     const int grid = std::min(n, loglik_grid_max(e, fp64, BS));
-    std::vector<int> ppad(n);
-    for (int i = 0; i < n; i++) ppad[i] = e->elems[ids[i]].Ppad;
-    const ScratchPlan pl = plan_scratch(ppad, grid, 0, [&](int p) { return loglik_slot_bytes(N, S, p, fp64); });
+    // resident partials come with their own arena and offsets
+    const bool resident = A.alt != nullptr;
+    std::vector<int> ppad(resident ? 0 : n);
+    for (int i = 0; i < (int)ppad.size(); i++) ppad[i] = e->elems[ids[i]].Ppad;
+    const ScratchPlan pl = resident ? ScratchPlan()
+                                    : plan_scratch(ppad, grid, 0, [&](int p) { return loglik_slot_bytes(N, S, p, fp64); });
     // the previous call synchronised its stream, so its scratch is free to reuse
-    if (pl.total > e->ll_scratch_bytes) {
+    if (!resident && pl.total > e->ll_scratch_bytes) {
         (void)hipFree(e->d_ll_scratch);
         e->d_ll_scratch = nullptr;
         e->ll_scratch_bytes = 0;
@@ -962,7 +965,7 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     int64_t *d_bo = (int64_t *)((char *)bufs[1].p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
     unsigned long long *d_cnt = (unsigned long long *)(d_bo + n);
     HIPCHK(e, hipMemcpyAsync(bufs[1].p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(e, hipMemcpyAsync(d_bo, pl.big_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, e->stream));
+    if (!resident) HIPCHK(e, hipMemcpyAsync(d_bo, pl.big_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipMemsetAsync(d_cnt, 0, 8, e->stream));
     A.counters = d_cnt;
     A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
@@ -970,10 +973,12 @@ static pa_status run_kernel(pa_engine *e, const int32_t *ids, int32_t n, int fp6
     A.work = (const int32_t *)bufs[1].p;
     A.n_work = n;
     A.queue = e->d_queue;
-    A.scratch = (char *)e->d_ll_scratch;
-    A.slot_bytes = (int64_t)pl.slot;
-    A.big = (char *)e->d_ll_scratch + pl.slots;
-    A.big_off = pl.big ? d_bo : nullptr;
+    if (!resident) {
+        A.scratch = (char *)e->d_ll_scratch;
+        A.slot_bytes = (int64_t)pl.slot;
+        A.big = (char *)e->d_ll_scratch + pl.slots;
+        A.big_off = pl.big ? d_bo : nullptr;
+    }
     HIPCHK(e, launch_reset(A.queue, e->stream));
     HIPCHK(e, hipEventRecord(e->ll_t0, e->stream));
     HIPCHK(e, launch_loglik(e->M, A, fp64, BS, grid, e->stream));
@@ -1275,6 +1280,19 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
         A.ll_out2 = (double *)pre[1].p;
         A.site = e->d_site;
     }
+    // resident partials: every work position's region in the caller's arena
+    DevBuf rb;
+    if (gx && gx->d_big) {
+        if (!gx->d_alt || !gx->row_big_off) return fail(e, PA_E_INPUT, "resident partials need slot bits and offsets");
+        std::vector<int64_t> bo(n);
+        for (int i = 0; i < n; i++) bo[i] = gx->row_big_off[order[i]];
+        HIPCHK(e, hipMalloc(&rb.p, (size_t)n * 8));
+        HIPCHK(e, hipMemcpy(rb.p, bo.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        A.big = gx->d_big;
+        A.big_off = (const int64_t *)rb.p;
+        A.alt = gx->d_alt;
+        A.path_node = gx->d_path;
+    }
     // gene-tree moves: per-row trees, history draws, candidate scores (rows in the caller's order)
     DevBuf gb[12];
     auto up = [&](DevBuf &b, const void *src, size_t bytes) -> hipError_t {
@@ -1452,6 +1470,11 @@ pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32
 }
 
 GtHistory &engine_history(pa_engine *e) { return e->hist; }
+
+int element_ppad(pa_engine *e, int32_t id)
+{
+    return id >= 0 && id < (int)e->elems.size() ? e->elems[id].Ppad : -1;
+}
 
 pa_status engine_stream(pa_engine *e, hipStream_t *st)
 {

@@ -38,6 +38,13 @@ struct GtExtra {
     const double *cand_len = nullptr;
     int cand_max = 0;
     double *cand_score = nullptr;                     // out: [n][cand_max]
+    // resident partials: row r's at d_big + row_big_off[r] (loglik_resident_bytes),
+    // node slots chosen by d_alt [n][N]; rows with d_path[r] >= 0 prune that
+    // node's path to the root only (device arrays, rows in the caller's order)
+    char *d_big = nullptr;
+    const int64_t *row_big_off = nullptr;             // host
+    const uint8_t *d_alt = nullptr;
+    const int32_t *d_path = nullptr;
 };
 // The engine's gene-node histories: the device arena of the last
 // pa_gt_sample_history call and which rows it holds.
@@ -49,6 +56,8 @@ struct GtHistory {
     std::vector<int64_t> off;
 };
 GtHistory &engine_history(pa_engine *e);
+// element id's padded pattern count (-1 for an unknown id)
+int element_ppad(pa_engine *e, int32_t id);
 // grows the history arena to at least `bytes` (device memory)
 pa_status engine_history_reserve(pa_engine *e, size_t bytes);
 // One gene-tree likelihood launch (pa_loglik_lengths_split's, lens0 may be

@@ -39,6 +39,14 @@ struct LoglikArgs {
     // or big_off null, means the workgroup's slot.
     char *big;
     const int64_t *big_off;
+    // Resident partials (gene-tree height moves), or null: every work
+    // position's partials stay at big + big_off[w] (loglik_resident_bytes of
+    // its Ppad: two slots per internal node), node x of row r in its second
+    // slot when alt[r][x] != 0.  A row with path_node[r] >= 0 prunes only the
+    // path from that node to the root, into the slots alt does not select,
+    // and reads every other node's partial where it is.
+    const uint8_t *alt;
+    const int32_t *path_node;
     const int32_t *Zin;       // [n_work][N] (when len_in is null)
     const double *rates_in;   // [n_work][2] (r1, r2) (when len_in is null)
     const double *len_in;     // [rows][N] per-call branch lengths, or null
@@ -117,6 +125,9 @@ size_t loglik_smem_bytes(int N, int fp64);
 // Bytes of one workgroup's HBM slot for elements of up to Pcap (padded)
 // patterns: the partials [I][Pcap] (4 values each) and their exponents.
 size_t loglik_slot_bytes(int N, int S, int Pcap, int fp64);
+// Bytes of one element's resident partials at Ppad patterns (two slots per
+// internal node and their exponents).
+size_t loglik_resident_bytes(int N, int S, int Ppad, int fp64);
 // Dynamic LDS bytes of one workgroup (the launch's dynamic allocation).
 size_t loglik_dyn_smem_bytes(int N, int fp64);
 // Workgroups of the likelihood kernel resident on one CU (registers, LDS, launch bound).
