@@ -25,24 +25,27 @@ namespace pa {
 // paths give the same bits:
 // - the prior (GTree::priorTree): gene nodes born per species by LDS atomics;
 //   the lineages entering species x, the leaves below x less the gene nodes
-//   born in internal species strictly below it; each born node's place in its
-//   species' height order by ranking; its coalescence term at that place;
-//   lane 0 adds the terms and each species' tail in species post-order;
-// - the missing-gene term: a term per gene node, added in gene post-order;
+//   born in internal species strictly below it; each born node's rank in its
+//   species' height order, and the height before it, by ranking; each
+//   coalescence term and each species' tail written at its place in the
+//   host's sequence (species post-order, born nodes by height, then the tail;
+//   +0.0 where the host adds nothing, which leaves a sum unchanged);
+// - the missing-gene term: a term per gene node at its gene post-order place;
+// - lane 0 adds the prior sequence while lane 1 adds the missing-gene one;
 // - the branch lengths, one node per lane.
 __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int status_sh;
+    __shared__ int status_sh, total_sh;
     __shared__ double lc[4];
     const int i = blockIdx.x, t = threadIdx.x;
     const int N = a.N, S = a.S;
     double *H = (double *)lds, *theta = H + N, *rate = theta + N, *ldens = rate + N, *h = ldens + N;
-    double *hs = h + N, *tm = hs + N, *lzt = tm + N;
-    int32_t *up = (int32_t *)(lzt + N), *post = up + N, *nleaf = post + N, *ch = nleaf + N, *par = ch + 2 * N;
-    int32_t *order = par + N, *Z = order + N, *where = Z + N, *end = where + N, *cnt = end + N, *below = cnt + N;
-    int32_t *pos = below + N, *off = pos + N;
-    uint8_t *miss = (uint8_t *)(off + N + 1), *lzs = miss + N;
+    double *seq = h + N, *lzseq = seq + 2 * N;
+    int32_t *up = (int32_t *)(lzseq + N), *post = up + N, *ipost = post + N, *nleaf = ipost + N, *ch = nleaf + N;
+    int32_t *par = ch + 2 * N, *Z = par + N, *where = Z + N, *end = where + N, *cnt = end + N, *below = cnt + N;
+    int32_t *soff = below + N;
+    uint8_t *miss = (uint8_t *)(soff + N + 1);
     const size_t r = (size_t)i * N;
     for (int k = t; k < N; k += 64) {
         H[k] = a.sp.H[k];
@@ -50,11 +53,12 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
         rate[k] = a.rate[k];
         ldens[k] = a.ldens[k];
         up[k] = a.sp.up[k];
-        post[k] = a.sp.post[k];
+        const int pk = a.sp.post[k];
+        post[k] = pk;
+        ipost[pk] = k;
         nleaf[k] = a.nleaf[k];
         h[k] = a.h[r + k];
         par[k] = a.par[r + k];
-        order[k] = a.order[r + k];
         Z[k] = a.Z[r + k];
         where[k] = a.where[r + k];
         miss[k] = a.miss[r + k];
@@ -68,7 +72,7 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
     for (int k = t; k < N; k += 64) end[k] = k == root ? N - 1 : where[par[k]];
     __syncthreads();
     const gt::SpView sp = {S, N, up, H, theta, post, nullptr, nullptr};
-    gt::TreeView g = {S, N, root, ch, par, order, h, where, end};
+    gt::TreeView g = {S, N, root, ch, par, nullptr, h, where, end};
     if (t == 0 && a.lengths_only) {
         status_sh = 1;
     } else if (t == 0) {
@@ -100,66 +104,66 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
         }
         __syncthreads();
         {
-            // off: the exclusive prefix of cnt, in chunks per lane and a shuffle scan
+            // soff: each species' start in the prior sequence (species
+            // post-order; cnt + 1 places per internal species), chunks per
+            // lane and a shuffle scan
             const int c = (N + 63) / 64, lo = min(N, t * c), hi = min(N, lo + c);
             int sum = 0;
-            for (int x = lo; x < hi; x++) sum += cnt[x];
+            for (int q = lo; q < hi; q++) sum += post[q] >= S ? cnt[post[q]] + 1 : 0;
             int inc = sum;
             for (int d = 1; d < 64; d <<= 1) {
                 const int y = __shfl_up(inc, d, 64);
                 if (t >= d) inc += y;
             }
             int run = inc - sum;
-            for (int x = lo; x < hi; x++) {
-                off[x] = run;
-                run += cnt[x];
+            for (int q = lo; q < hi; q++) {
+                soff[q] = run;
+                run += post[q] >= S ? cnt[post[q]] + 1 : 0;
             }
-            if (t == 63) off[N] = inc;
+            if (t == 63) total_sh = inc;
         }
         __syncthreads();
+        // born nodes: rank in the species' height order (ties by index), the
+        // height before, the term; the last one also writes the tail
         for (int u = S + t; u < N; u += 64) {
             const int x = where[u];
+            if (x < S) continue;
             const double hu = h[u];
             int rk = 0;
+            double prev = H[x];
             for (int q = S; q < N; q++) {
                 const double hq = h[q];
-                rk += where[q] == x && (hq < hu || (hq == hu && q < u));
+                if (where[q] == x && (hq < hu || (hq == hu && q < u))) {
+                    rk++;
+                    prev = fmax(prev, hq);
+                }
             }
-            pos[u] = off[x] + rk;
-            hs[off[x] + rk] = hu;
+            const int base = soff[ipost[x]], ke = nleaf[x] - below[x], n_b = cnt[x];
+            seq[base + rk] = ke == 0 ? 0.0 : gt::coal_term(ldens[x], rate[x], hu, prev, ke - rk);
+            if (rk == n_b - 1) {
+                const int kf = ke - n_b;
+                seq[base + n_b] =
+                    ke != 0 && kf > 1 && up[x] != N ? -gt::coal_tail(rate[x], H[up[x]], hu, kf) : 0.0;
+            }
         }
-        __syncthreads();
-        for (int u = S + t; u < N; u += 64) {
-            const int x = where[u], ke = nleaf[x] - below[x];
-            if (x < S || ke == 0) continue;
-            const int p = pos[u], rk = p - off[x];
-            tm[p] = gt::coal_term(ldens[x], rate[x], hs[p], rk == 0 ? H[x] : hs[p - 1], ke - rk);
+        // internal species with no born node: the tail alone
+        for (int x = S + t; x < N; x += 64) {
+            if (cnt[x] != 0) continue;
+            const int ke = nleaf[x] - below[x];
+            seq[soff[ipost[x]]] = ke != 0 && ke > 1 && up[x] != N ? -gt::coal_tail(rate[x], H[up[x]], H[x], ke) : 0.0;
         }
-        for (int u = t; u < N; u += 64) {
+        for (int q = t; q < N; q += 64) {
             double tv = 0;
-            lzs[u] = gt::logpz_term(sp, g, miss, Z, lc, u, &tv) ? 1 : 0;
-            lzt[u] = tv;
+            lzseq[q] = gt::logpz_term(sp, g, miss, Z, lc, a.order[r + q], &tv) ? tv : 0.0;
         }
         __syncthreads();
-        if (t == 0) {
-            double lp = 0;
-            for (int q = 0; q < N; q++) {
-                const int x = post[q];
-                if (x < S) continue;
-                const int ke = nleaf[x] - below[x];
-                if (ke == 0) continue;
-                const int a0 = off[x], b0 = off[x + 1];
-                for (int j = a0; j < b0; j++) lp += tm[j];
-                const int kf = ke - (b0 - a0);
-                if (kf > 1 && up[x] != N) lp -= gt::coal_tail(rate[x], H[up[x]], b0 > a0 ? hs[b0 - 1] : H[x], kf);
-            }
-            a.prior_new[i] = lp;
-            double lz = 0;
-            for (int q = 0; q < N; q++) {
-                const int u = order[q];
-                if (lzs[u]) lz += lzt[u];
-            }
-            a.lz_new[i] = lz;
+        if (t < 2) {
+            const double *arr = t == 0 ? seq : lzseq;
+            const int len = t == 0 ? total_sh : N;
+            double acc = 0;
+            for (int j = 0; j < len; j++) acc += arr[j];
+            if (t == 0) a.prior_new[i] = acc;
+            else a.lz_new[i] = acc;
         }
     }
     const double r1 = a.rates[2 * i], r2 = a.rates[2 * i + 1];

@@ -785,6 +785,8 @@ struct pa_gt_session {
     char *big = nullptr;
     uint8_t *alt = nullptr;
     std::vector<int64_t> big_off;
+    GtPlan *plan = nullptr;
+    std::vector<int32_t> h_gnode, h_ind, h_acc, l0, l1;   // staged for the asynchronous copies
 
     template <typename T>
     hipError_t get(T **p, size_t count, const void *src = nullptr)
@@ -799,6 +801,7 @@ struct pa_gt_session {
     }
     ~pa_gt_session()
     {
+        gt_plan_free(plan);
         for (void *p : bufs) (void)hipFree(p);
         (void)hipFree(big);
     }
@@ -831,6 +834,8 @@ static pa_status session_resident(pa_gt_session *s, int32_t precision)
     if (st != PA_OK) return st;
     (void)hipStreamSynchronize(stream);
     s->res_prec = -1;
+    gt_plan_free(s->plan);
+    s->plan = nullptr;
     (void)hipFree(s->big);
     s->big = nullptr;
     s->big_off.assign(n, 0);
@@ -852,12 +857,15 @@ static pa_status session_resident(pa_gt_session *s, int32_t precision)
     if (launch_gt_hprop(a, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "session lengths launch failed");
     GtExtra x;
     x.d_len = s->len;
+    x.d_row0 = s->out0;
+    x.d_row1 = s->out1;
     x.d_big = s->big;
     x.row_big_off = s->big_off.data();
     x.d_alt = s->alt;
-    std::vector<double> o0(n), o1(n);
-    if ((st = gt_launch(e, s->ids.data(), n, nullptr, s->piA.empty() ? nullptr : s->piA.data(), nullptr, nullptr,
-                        precision, o0.data(), o1.data(), x)) != PA_OK)
+    if ((st = gt_plan_build(e, s->ids.data(), n, s->piA.empty() ? nullptr : s->piA.data(), precision, x, &s->plan)) !=
+        PA_OK)
+        return st;
+    if ((st = gt_plan_run(e, s->plan, nullptr, nullptr, nullptr)) != PA_OK || (st = gt_plan_stats(e, s->plan)) != PA_OK)
         return st;
     s->res_prec = precision;
     return PA_OK;
@@ -959,20 +967,23 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
     if (!gnode || !indicator || !lens) return engine_fail(e, PA_E_INPUT, "null argument");
     if (!(delta > 0)) return engine_fail(e, PA_E_INPUT, "delta must be positive");
     const int n = s->n, S = s->sp.S, N = s->sp.N;
-    std::vector<int32_t> l0(n), l1(n);
+    s->l0.resize(n);
+    s->l1.resize(n);
     for (int i = 0; i < n; i++) {
         if (gnode[i] < S || gnode[i] >= N) return engine_fail(e, PA_E_INPUT, "gnode must be an internal gene node");
         if (indicator[i] != 0 && indicator[i] != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
         if (lens[2 * i + 1] < 0 || lens[2 * i + indicator[i]] < 0) return engine_fail(e, PA_E_INPUT, "negative prefix length");
-        l0[i] = lens[2 * i + indicator[i]];
-        l1[i] = std::min(lens[2 * i + 1], l0[i]);
+        s->l0[i] = lens[2 * i + indicator[i]];
+        s->l1[i] = std::min(lens[2 * i + 1], s->l0[i]);
     }
     pa_status st;
     if (s->res_prec != precision && (st = session_resident(s, precision)) != PA_OK) return st;
     hipStream_t stream;
     if ((st = engine_stream(e, &stream)) != PA_OK) return st;
-    if (hipMemcpy(s->gnode, gnode, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->ind, indicator, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess)
+    s->h_gnode.assign(gnode, gnode + n);
+    s->h_ind.assign(indicator, indicator + n);
+    if (hipMemcpyAsync(s->gnode, s->h_gnode.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(s->ind, s->h_ind.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream) != hipSuccess)
         return engine_fail(e, PA_E_DEVICE, "session upload failed");
     EngineView ev;
     engine_view(e, &ev);
@@ -983,18 +994,7 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
     if (launch_gt_hprop(a, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height proposal launch failed");
     // the proposals' likelihoods: each element's path from the moved node to
     // the root, into the slots its current partials do not use
-    GtExtra x;
-    x.d_len = s->len;
-    x.d_row0 = s->out0;
-    x.d_row1 = s->out1;
-    x.d_big = s->big;
-    x.row_big_off = s->big_off.data();
-    x.d_alt = s->alt;
-    x.d_path = s->gnode;
-    std::vector<double> o0(n), o1(n);
-    if ((st = gt_launch(e, s->ids.data(), n, nullptr, s->piA.empty() ? nullptr : s->piA.data(), l0.data(), l1.data(),
-                        precision, o0.data(), o1.data(), x)) != PA_OK)
-        return st;
+    if ((st = gt_plan_run(e, s->plan, s->l0.data(), s->l1.data(), s->gnode)) != PA_OK) return st;
     GtDecideArgs d;
     d.n = n; d.N = N;
     d.win = s->win; d.prop = s->prop; d.rdraw = s->rdraw; d.prior_new = s->prior_new; d.lz_new = s->lz_new;
@@ -1004,12 +1004,13 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
     d.h = s->h; d.loglik = s->ll; d.logpz = s->lz; d.prior = s->prior; d.accepted = s->acc;
     if (launch_gt_hdecide(d, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height decision launch failed");
     if (accepted) {
-        if (hipMemcpy(accepted, s->acc, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        s->h_acc.resize(n);
+        if (hipMemcpyAsync(s->h_acc.data(), s->acc, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream) != hipSuccess)
             return engine_fail(e, PA_E_DEVICE, "session download failed");
-    } else if (hipStreamSynchronize(stream) != hipSuccess) {
-        return engine_fail(e, PA_E_DEVICE, "height moves failed");
     }
-    return PA_OK;
+    if (hipStreamSynchronize(stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height moves failed");
+    if (accepted) memcpy(accepted, s->h_acc.data(), sizeof(int32_t) * n);
+    return gt_plan_stats(e, s->plan);
 }
 
 extern "C" pa_status pa_gt_session_read(pa_gt_session *s, double *gheight, double *loglik, double *logpz, double *prior)

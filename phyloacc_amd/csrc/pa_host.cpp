@@ -1211,6 +1211,20 @@ extern "C" pa_status pa_loglik_last_stats(pa_engine *e, pa_run_stats *out)
 // from pa_gene_lengths.
 // Own-tree log-likelihoods at per-call lengths, with the model's base
 // frequencies or, when piA is given, each element's strand-symmetric ones.
+// BPP's exchangeabilities from RATE_MAT and BACKGROUND, with the reference's
+// assignment (src/PhyloAcc-GT/bpp.hpp:346-353: CG takes AC's value, CT takes
+// C->G / pi_G, GT takes AG's value).
+static void gt_exchangeabilities(const pa_engine *e, double *inst)
+{
+    const double *Q = e->Q0, *p0 = e->M.pi;
+    inst[0] = Q[0 * 4 + 1] / p0[1];
+    inst[1] = Q[0 * 4 + 2] / p0[2];
+    inst[2] = Q[0 * 4 + 3] / p0[3];
+    inst[3] = inst[0];
+    inst[4] = Q[1 * 4 + 2] / p0[2];
+    inst[5] = inst[1];
+}
+
 static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths,
                                      const double *piA, int32_t precision, double *out,
                                      const int32_t *lens0 = nullptr, const int32_t *lens1 = nullptr,
@@ -1245,17 +1259,8 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
     if (gx) { A.row_out0 = gx->d_row0; A.row_out1 = gx->d_row1; }
     A.len_row = (const int32_t *)bufs[6].p;
     if (piA) {
-        // BPP's exchangeabilities from RATE_MAT and BACKGROUND, with the
-        // reference's assignment (src/PhyloAcc-GT/bpp.hpp:346-353: CG takes
-        // AC's value, CT takes C->G / pi_G, GT takes AG's value)
-        const double *Q = e->Q0, *p0 = e->M.pi;
         double inst[6];
-        inst[0] = Q[0 * 4 + 1] / p0[1];
-        inst[1] = Q[0 * 4 + 2] / p0[2];
-        inst[2] = Q[0 * 4 + 3] / p0[3];
-        inst[3] = inst[0];
-        inst[4] = Q[1 * 4 + 2] / p0[2];
-        inst[5] = inst[1];
+        gt_exchangeabilities(e, inst);
         std::vector<double> pw(n);
         for (int i = 0; i < n; i++) pw[i] = piA[order[i]];
         HIPCHK(e, hipMalloc(&bufs[7].p, (size_t)n * 8));
@@ -1527,6 +1532,135 @@ pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *l
     }
     return loglik_lengths_impl(e, ids, n, lengths, piA, precision, out0, lens0, lens1, out1, &x);
 }
+
+// A resident-partials likelihood launch prepared once for a fixed batch
+// (the gene-tree height session): work order, block size, per-work-position
+// arena offsets, the elements' eigensystems and the result rows stay on the
+// device; a run uploads the prefix lengths and launches, without waiting.
+struct GtPlan {
+    int n = 0, fp64 = 0, BS = 0, grid = 0;
+    std::vector<int32_t> pl;          // prefix lengths staged for the last run
+    std::vector<int32_t> GG;          // sites per row
+    DevBuf work, row, bo, pre, out, out2, eig, pi, cnt;
+    LoglikArgs A;
+};
+
+pa_status gt_plan_build(pa_engine *e, const int32_t *ids, int32_t n, const double *piA, int32_t precision,
+                        const GtExtra &x, GtPlan **out)
+{
+    *out = nullptr;
+    if (n <= 0 || !x.d_len || !x.d_big || !x.row_big_off || !x.d_alt) return fail(e, PA_E_INPUT, "incomplete plan");
+    pa_status st = check_ids(e, ids, n);
+    if (st != PA_OK) return st;
+    if ((st = upload(e)) != PA_OK) return st;
+    std::unique_ptr<GtPlan> p(new GtPlan);
+    const int N = e->tm.N, fp64 = precision == PA_FP64;
+    p->n = n;
+    p->fp64 = fp64;
+    const std::vector<int32_t> order = schedule(e, ids, n, PA_RUN_BATCHED);
+    std::vector<int32_t> sorted(n);
+    std::vector<int64_t> bo(n);
+    p->GG.resize(n);
+    for (int i = 0; i < n; i++) {
+        sorted[i] = ids[order[i]];
+        bo[i] = x.row_big_off[order[i]];
+        p->GG[i] = e->elems[ids[i]].GG;
+    }
+    p->BS = loglik_block(e, sorted.data(), n, fp64);
+    if (loglik_smem_bytes(N, fp64) > e->lds_per_cu) return fail(e, PA_E_INPUT, "tree too large for the LDS layout");
+    p->grid = std::min(n, loglik_grid_max(e, fp64, p->BS));
+    HIPCHK(e, hipMalloc(&p->work.p, (size_t)n * 4));
+    HIPCHK(e, hipMalloc(&p->row.p, (size_t)n * 4));
+    HIPCHK(e, hipMalloc(&p->bo.p, (size_t)n * 8));
+    HIPCHK(e, hipMalloc(&p->pre.p, (size_t)n * 8));
+    HIPCHK(e, hipMalloc(&p->out.p, (size_t)n * 8));
+    HIPCHK(e, hipMalloc(&p->out2.p, (size_t)n * 8));
+    HIPCHK(e, hipMalloc(&p->cnt.p, 8));
+    HIPCHK(e, hipMemcpy(p->work.p, sorted.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(p->row.p, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(p->bo.p, bo.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    LoglikArgs &A = p->A;
+    memset(&A, 0, sizeof A);
+    if (piA) {
+        double inst[6];
+        gt_exchangeabilities(e, inst);
+        std::vector<double> pw(n);
+        for (int i = 0; i < n; i++) pw[i] = piA[order[i]];
+        DevBuf dpw;
+        HIPCHK(e, hipMalloc(&dpw.p, (size_t)n * 8));
+        HIPCHK(e, hipMalloc(&p->eig.p, (size_t)n * GB_EIG_WORDS * 8));
+        HIPCHK(e, hipMalloc(&p->pi.p, (size_t)n * 4 * 8));
+        HIPCHK(e, hipMemcpy(dpw.p, pw.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        HIPCHK(e, launch_gt_eig(inst, (const double *)dpw.p, n, (double *)p->eig.p, (double *)p->pi.p, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        A.el_eig = (const double *)p->eig.p;
+        A.el_pi = (const double *)p->pi.p;
+    }
+    A.ll_out = (double *)p->out.p;
+    A.len_in = x.d_len;
+    A.len_row = (const int32_t *)p->row.p;
+    A.row_out0 = x.d_row0;
+    A.row_out1 = x.d_row1;
+    A.big = x.d_big;
+    A.big_off = (const int64_t *)p->bo.p;
+    A.alt = x.d_alt;
+    A.work = (const int32_t *)p->work.p;
+    A.n_work = n;
+    A.counters = (unsigned long long *)p->cnt.p;
+    *out = p.release();
+    return PA_OK;
+}
+
+pa_status gt_plan_run(pa_engine *e, GtPlan *p, const int32_t *lens0, const int32_t *lens1, const int32_t *d_path)
+{
+    pa_status st = upload(e);
+    if (st != PA_OK) return st;
+    const int n = p->n;
+    LoglikArgs A = p->A;
+    if (lens0) {
+        p->pl.resize((size_t)n * 2);
+        for (int i = 0; i < n; i++) {
+            if (lens1[i] < 0 || lens1[i] > lens0[i] || lens0[i] > p->GG[i])
+                return fail(e, PA_E_INPUT, "prefix lengths must satisfy 0 <= lens1 <= lens0 <= the element's sites");
+            p->pl[2 * i] = lens0[i];
+            p->pl[2 * i + 1] = lens1[i];
+        }
+        HIPCHK(e, hipMemcpyAsync(p->pre.p, p->pl.data(), (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
+        A.pre_len = (const int32_t *)p->pre.p;
+        A.ll_out2 = (double *)p->out2.p;
+    }
+    // the engine's arenas as they are now
+    A.elems = e->d_elems; A.codes = e->d_codes; A.wts = e->d_w; A.tab = e->d_tab;
+    A.n_elems = (int)e->elems.size();
+    A.site = e->d_site;
+    A.queue = e->d_queue;
+    A.path_node = d_path;
+    HIPCHK(e, hipMemsetAsync(p->cnt.p, 0, 8, e->stream));
+    HIPCHK(e, launch_reset(A.queue, e->stream));
+    HIPCHK(e, hipEventRecord(e->ll_t0, e->stream));
+    HIPCHK(e, launch_loglik(e->M, A, p->fp64, p->BS, p->grid, e->stream));
+    HIPCHK(e, hipEventRecord(e->ll_t1, e->stream));
+    e->ll_stats.grid = p->grid;
+    e->ll_stats.block_threads = p->BS;
+    e->ll_stats.lds_bytes = (int32_t)loglik_smem_bytes(e->tm.N, p->fp64);
+    e->ll_stats.n_launches = 1;
+    e->ll_stats.scratch_bytes = 0;
+    return PA_OK;
+}
+
+pa_status gt_plan_stats(pa_engine *e, GtPlan *p)
+{
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    float ms = 0;
+    unsigned long long cnt = 0;
+    HIPCHK(e, hipEventElapsedTime(&ms, e->ll_t0, e->ll_t1));
+    HIPCHK(e, hipMemcpy(&cnt, p->cnt.p, sizeof cnt, hipMemcpyDeviceToHost));
+    e->ll_stats.elapsed_ms = ms;
+    e->ll_stats.algorithmic_bytes = (double)cnt;
+    return PA_OK;
+}
+
+void gt_plan_free(GtPlan *p) { delete p; }
 
 }  // namespace pa
 

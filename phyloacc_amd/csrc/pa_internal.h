@@ -56,6 +56,17 @@ struct GtHistory {
     std::vector<int64_t> off;
 };
 GtHistory &engine_history(pa_engine *e);
+// A resident-partials likelihood launch prepared once for a fixed batch
+// (x: d_len, d_big, row_big_off, d_alt; d_row0/1 optional); each run uploads
+// the prefix lengths (lens0 null: the pattern sum, no prefixes) and launches
+// on the engine stream without waiting; gt_plan_stats waits and records the
+// launch's time and counted bytes.
+struct GtPlan;
+pa_status gt_plan_build(pa_engine *e, const int32_t *ids, int32_t n, const double *piA, int32_t precision,
+                        const GtExtra &x, GtPlan **out);
+pa_status gt_plan_run(pa_engine *e, GtPlan *p, const int32_t *lens0, const int32_t *lens1, const int32_t *d_path);
+pa_status gt_plan_stats(pa_engine *e, GtPlan *p);
+void gt_plan_free(GtPlan *p);
 // element id's padded pattern count (-1 for an unknown id)
 int element_ppad(pa_engine *e, int32_t id);
 // grows the history arena to at least `bytes` (device memory)
