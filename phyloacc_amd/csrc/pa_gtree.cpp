@@ -152,6 +152,19 @@ void for_elements(int n, F &&fn)
     for (auto &x : th) x.join();
 }
 
+// The trees must be the elements' stored trees: checked on the element
+// threads; on a mismatch the first failing element (in batch order) is
+// checked again alone, so the recorded message is that element's.  lm
+// receives the leaves' missing flags [n][S].
+pa_status stored_trees(pa_engine *e, const int32_t *ids, int n, int S, int N, const int32_t *gchildren, uint8_t *lm)
+{
+    std::vector<pa_status> st(n, PA_OK);
+    for_elements(n, [&](int i) { st[i] = element_gene_tree(e, ids[i], lm + (size_t)i * S, gchildren + (size_t)i * 2 * N); });
+    for (int i = 0; i < n; i++)
+        if (st[i] != PA_OK) return element_gene_tree(e, ids[i], lm + (size_t)i * S, gchildren + (size_t)i * 2 * N);
+    return PA_OK;
+}
+
 pa_status check_sp(pa_engine *e, const pa_species_tree *t, Sp &s)
 {
     if (!t || !t->parent || !t->height || !t->theta) return engine_fail(e, PA_E_INPUT, "null species tree");
@@ -264,10 +277,9 @@ extern "C" pa_status pa_gt_branch_len(pa_engine *e, const pa_species_tree *t, co
     if (s.S != ev.S || s.N != ev.N) return engine_fail(e, PA_E_INPUT, "species tree size differs from the model's");
     if (n == 0) return PA_OK;
     const int N = s.N;
-    // the trees must be the elements' stored trees (serial: cheap, and it may record an error)
     std::vector<uint8_t> lm((size_t)n * s.S);
+    if ((st = stored_trees(e, ids, n, s.S, N, gchildren, lm.data())) != PA_OK) return st;
     for (int i = 0; i < n; i++) {
-        if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * s.S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
         if (gnode[i] < s.S || gnode[i] >= N) return engine_fail(e, PA_E_INPUT, "gnode must be an internal gene node");
         if (indicator[i] != 0 && indicator[i] != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
         if (lens[2 * i + 1] < 0 || lens[2 * i + indicator[i]] < 0) return engine_fail(e, PA_E_INPUT, "negative prefix length");
@@ -487,9 +499,11 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
     std::vector<int64_t> off(n);
     std::vector<int32_t> ns(n);
     int64_t total = 0;
+    {
+        std::vector<uint8_t> lm((size_t)n * s.S);
+        if ((st = stored_trees(e, ids, n, s.S, N, gchildren, lm.data())) != PA_OK) return st;
+    }
     for (int i = 0; i < n; i++) {
-        uint8_t lm[512];
-        if ((st = element_gene_tree(e, ids[i], lm, gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
         if (len[i] < 0 || (Tg && len[i] > stride)) return engine_fail(e, PA_E_INPUT, "history length negative or beyond stride");
         off[i] = total;
         ns[i] = len[i];
@@ -565,8 +579,8 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<TreeMove> mv(n);
     std::vector<uint8_t> lm((size_t)n * s.S);
     std::vector<double> cur_len((size_t)n * N);
+    if ((st = stored_trees(e, ids, n, s.S, N, gchildren, lm.data())) != PA_OK) return st;
     for (int i = 0; i < n; i++) {
-        if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * s.S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
         const int ind = indicator[i];
         if (ind != 0 && ind != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
         if (lens[2 * i + ind] < 0 || lens[2 * i + 1] < 0 || (Tg && lens[2 * i + ind] > stride))
@@ -649,6 +663,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<int32_t> tabs((size_t)n * tab_words(N)), hdrs((size_t)n * 4), l0(n), l1(n);
     std::vector<double> nlen((size_t)n * N), rpi;
     std::vector<int> gbad(n, 0);
+    std::vector<Embed> emb2(n);   // the proposed trees' embeddings
     for_elements(n, [&](int i) {
         TreeMove &t = mv[i];
         const Embed &g = emb[i];
@@ -705,7 +720,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
             gbad[i] = 1;
             return;
         }
-        Embed g2;
+        Embed &g2 = emb2[i];
         if (!g2.build(s, nch[i].data(), npar[i].data(), nh[i].data())) { gbad[i] = 1; return; }
         lengths_of(g2, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &nlen[(size_t)i * N]);
         const int ind = indicator[i];
@@ -726,8 +741,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     for_elements(n, [&](int i) {
         accepted[i] = 0;
         if (pos[i] < 0) return;
-        Embed g2;
-        g2.build(s, nch[i].data(), npar[i].data(), nh[i].data());
+        const Embed &g2 = emb2[i];
         std::vector<uint8_t> miss;
         missing_of(g2, &lm[(size_t)i * s.S], miss);
         const double lz = logpz_of(g2, miss, Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
@@ -749,13 +763,14 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     });
     // the accepted trees become the elements' stored trees
     std::vector<int32_t> sid;
-    std::vector<const int32_t *> sch;
+    std::vector<const int32_t *> stab, shdr;
     for (int i = 0; i < n; i++)
         if (accepted[i]) {
             sid.push_back(ids[i]);
-            sch.push_back(nch[i].data());
+            stab.push_back(&tabs[(size_t)i * tab_words(N)]);
+            shdr.push_back(&hdrs[(size_t)i * 4]);
         }
-    return element_set_trees(e, sid.data(), (int)sid.size(), sch.data());
+    return element_set_trees(e, sid.data(), (int)sid.size(), stab.data(), shdr.data());
 }
 
 // ------------------------------------------------ device-resident height moves
@@ -899,8 +914,7 @@ extern "C" pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *t
     if (gt_hprop_lds(N) > GT_HPROP_LDS_MAX)
         return engine_fail(e, PA_E_INPUT, "species tree too large for the device session (its LDS copy exceeds 64 KiB)");
     std::vector<uint8_t> lm((size_t)n * S);
-    for (int i = 0; i < n; i++)
-        if ((st = element_gene_tree(e, ids[i], &lm[(size_t)i * S], gchildren + (size_t)i * 2 * N)) != PA_OK) return st;
+    if ((st = stored_trees(e, ids, n, S, N, gchildren, lm.data())) != PA_OK) return st;
     // per element: the post-order, root, embedding and missing flags; the
     // trees keep their topology for the session's life, so the flags hold
     std::vector<int32_t> order((size_t)n * N), root(n), where((size_t)n * N);

@@ -284,10 +284,13 @@ void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std
                      HostElem &E)
 {
     const int S = tm.S, N = tm.N;
+    // scratch kept per thread: tables are rebuilt for every proposed gene tree
+    thread_local std::vector<int> nodes, internal, height, depth, cnt;
+    thread_local std::vector<std::pair<int, int>> st;
     // post-order of the element's tree (BPP_C::getSubtree, bpp_c.cpp:44-60)
-    std::vector<int> nodes;
+    nodes.clear();
     {
-        std::vector<std::pair<int, int>> st{{root, 0}};
+        st.assign(1, {root, 0});
         while (!st.empty()) {
             auto &[v, k] = st.back();
             if (v >= S && k < 2) { int x = ch2[2 * v + k]; k++; if (x != -1) st.push_back({x, 0}); continue; }
@@ -295,10 +298,11 @@ void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std
             st.pop_back();
         }
     }
-    std::vector<int> internal;
+    internal.clear();
     for (int v : nodes) if (v >= S) internal.push_back(v);
     // levels for the Z forward-backward
-    std::vector<int> height(N, 0), depth(N, 0);
+    height.assign(N, 0);
+    depth.assign(N, 0);
     int hmax = 0, dmax = 0;
     for (int v : nodes)
         if (v >= S) { height[v] = 1 + std::max(height[ch2[2 * v]], height[ch2[2 * v + 1]]); hmax = std::max(hmax, height[v]); }
@@ -322,15 +326,30 @@ void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std
         T[tab_tz(N) + nodes[k]] = (int)nodes.size() - 1 - (int)k;
     }
     for (size_t k = 0; k < internal.size(); k++) T[tab_int(N) + k] = internal[k];
-    int q = 0;
-    for (int h = 1; h <= hmax; h++) {
-        for (int v : internal) if (height[v] == h) T[tab_up(N) + q++] = v;
-        T[tab_upoff(N) + h - 1] = q;
-    }
-    q = 0;
-    for (int d = 1; d <= dmax; d++) {
-        for (int k = (int)nodes.size() - 1; k >= 0; k--) if (depth[nodes[k]] == d) T[tab_down(N) + q++] = nodes[k];
-        T[tab_downoff(N) + d - 1] = q;
+    // up: internal nodes by height (post-order within a level); down: non-root
+    // nodes by depth (reverse post-order within a level); stable counting sorts
+    {
+        cnt.assign(std::max(hmax, dmax) + 2, 0);
+        for (int v : internal) cnt[height[v]]++;
+        int q = 0;
+        for (int h = 1; h <= hmax; h++) {
+            const int c = cnt[h];
+            cnt[h] = q;
+            q += c;
+            T[tab_upoff(N) + h - 1] = q;
+        }
+        for (int v : internal) T[tab_up(N) + cnt[height[v]]++] = v;
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int v : nodes) if (depth[v] >= 1) cnt[depth[v]]++;
+        q = 0;
+        for (int d = 1; d <= dmax; d++) {
+            const int c = cnt[d];
+            cnt[d] = q;
+            q += c;
+            T[tab_downoff(N) + d - 1] = q;
+        }
+        for (int k = (int)nodes.size() - 1; k >= 0; k--)
+            if (depth[nodes[k]] >= 1) T[tab_down(N) + cnt[depth[nodes[k]]]++] = nodes[k];
     }
     double *D = (double *)(T + tab_dist(N));
     for (int i = 0; i < N; i++) D[i] = dist2[i];
@@ -1369,8 +1388,13 @@ static pa_status gene_elem(pa_engine *e, int32_t id, const int32_t *children, Ho
     pa_status st = element_gene_tree(e, id, lm, nullptr);
     if (st != PA_OK) return st;
     const int N = e->tm.N, S = e->tm.S;
-    std::vector<int> ch2(children, children + 2 * N), par2(N, N);
-    std::vector<uint8_t> seen(N, 0);
+    thread_local std::vector<int> ch2, par2, post;
+    thread_local std::vector<uint8_t> seen, missing;
+    thread_local std::vector<double> zero;
+    thread_local std::vector<std::pair<int, int>> stk;
+    ch2.assign(children, children + 2 * N);
+    par2.assign(N, N);
+    seen.assign(N, 0);
     for (int v = 0; v < S; v++)
         if (ch2[2 * v] != -1 || ch2[2 * v + 1] != -1) return fail(e, PA_E_INPUT, "gene tree leaves must have no children");
     for (int v = S; v < N; v++)
@@ -1384,9 +1408,9 @@ static pa_status gene_elem(pa_engine *e, int32_t id, const int32_t *children, Ho
     for (int v = S; v < N; v++)
         if (!seen[v]) { if (root >= 0) return fail(e, PA_E_INPUT, "gene tree has two roots"); root = v; }
     if (root < 0) return fail(e, PA_E_INPUT, "gene tree has no root");
-    std::vector<int> post;
+    post.clear();
     {
-        std::vector<std::pair<int, int>> stk{{root, 0}};
+        stk.assign(1, {root, 0});
         while (!stk.empty() && (int)post.size() <= N) {
             const int v = stk.back().first, k = stk.back().second;
             if (v >= S && k < 2) { stk.back().second++; stk.push_back({ch2[2 * v + k], 0}); continue; }
@@ -1395,9 +1419,10 @@ static pa_status gene_elem(pa_engine *e, int32_t id, const int32_t *children, Ho
         }
     }
     if ((int)post.size() != N) return fail(e, PA_E_INPUT, "gene tree does not reach every node");
-    std::vector<uint8_t> missing(N, 0);
+    missing.assign(N, 0);
     for (int v : post) missing[v] = v < S ? lm[v] : (missing[ch2[2 * v]] && missing[ch2[2 * v + 1]]);
-    fill_tree_table(e->tm, ch2, par2, std::vector<double>(N, 0.0), missing, true, root, out);
+    zero.assign(N, 0.0);
+    fill_tree_table(e->tm, ch2, par2, zero, missing, true, root, out);
     return PA_OK;
 }
 
@@ -1405,66 +1430,58 @@ namespace pa {
 
 pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr)
 {
-    HostElem tmp;
+    thread_local HostElem tmp;
     pa_status st = gene_elem(e, id, children, tmp);
     if (st != PA_OK) return st;
     memcpy(T, tmp.tab.data(), (size_t)tab_words(e->tm.N) * 4);
-    hdr[0] = tmp.root; hdr[1] = (int32_t)tmp.nodes.size(); hdr[2] = tmp.n_up; hdr[3] = 0;
+    hdr[0] = tmp.root; hdr[1] = (int32_t)tmp.nodes.size(); hdr[2] = tmp.n_up; hdr[3] = tmp.n_down;
     return PA_OK;
 }
 
-// Many elements' trees at once: every table is built first (nothing
-// changes if one fails), then the host copies change and, when the device
-// copy is current, one upload of the new tables and records and one scatter
-// launch replace them on the device.
-pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32_t *const *children)
+// Many elements' trees at once, from their node tables (gene_table's):
+// the host copies change, and when the device copy is current one upload of
+// the new tables and records and one scatter launch replace them there.
+pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32_t *const *tabs,
+                            const int32_t *const *hdrs)
 {
     if (n <= 0) return PA_OK;
     const int N = e->tm.N, W = tab_words(N);
-    std::vector<HostElem> tmp(n);
-    std::vector<pa_status> sts(n, PA_OK);
-    {
-        const unsigned nt = n < 256 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        auto work = [&](unsigned t) {
-            for (int k = (int)t; k < n; k += (int)nt) sts[k] = gene_elem(e, ids[k], children[k], tmp[k]);
-        };
-        std::vector<std::thread> th;
-        for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
-        work(0);
-        for (auto &x : th) x.join();
-    }
     for (int k = 0; k < n; k++)
-        if (sts[k] != PA_OK) return sts[k];
+        if (ids[k] < 0 || ids[k] >= (int)e->elems.size() || hdrs[k][1] < 1 || hdrs[k][1] > N)
+            return fail(e, PA_E_INPUT, "bad element or node table");
     const bool dev = !e->dev_dirty && e->d_tab && e->d_elems;
-    std::vector<int32_t> tabs(dev ? (size_t)n * W : 0), dst_id(dev ? n : 0);
+    std::vector<int32_t> stage(dev ? (size_t)n * W : 0), dst_id(dev ? n : 0);
     std::vector<int64_t> dst_off(dev ? n : 0);
     std::vector<ElemDev> recs(dev ? n : 0);
     for (int k = 0; k < n; k++) {
         const int id = ids[k];
         HostElem &E = e->elems[id];
-        HostElem &t = tmp[k];
+        const int32_t *T = tabs[k], *hd = hdrs[k];
         // gene trees take their branch lengths per call; the stored ones stay
-        memcpy(t.tab.data() + tab_dist(N), E.tab.data() + tab_dist(N), (size_t)(W - tab_dist(N)) * 4);
-        E.tab = std::move(t.tab);
-        E.root = t.root; E.n_up = t.n_up; E.n_down = t.n_down;
-        E.nodes = std::move(t.nodes);
-        E.internal = std::move(t.internal);
+        std::vector<int32_t> tab(T, T + W);
+        memcpy(tab.data() + tab_dist(N), E.tab.data() + tab_dist(N), (size_t)(W - tab_dist(N)) * 4);
+        E.tab = std::move(tab);
+        E.root = hd[0];
+        E.n_up = hd[2];
+        E.n_down = hd[3];
+        E.nodes.assign(T + tab_nodes(N), T + tab_nodes(N) + hd[1]);
+        E.internal.assign(T + tab_int(N), T + tab_int(N) + (hd[1] - 1) / 2);
         if (!dev || id >= (int)e->dev_elems.size()) continue;
         ElemDev &x = e->dev_elems[id];
         x.root = E.root; x.n_nodes = (int)E.nodes.size(); x.n_int = (int)E.internal.size();
         x.n_up = E.n_up; x.n_down = E.n_down;
-        memcpy(&tabs[(size_t)k * W], E.tab.data(), (size_t)W * 4);
+        memcpy(&stage[(size_t)k * W], E.tab.data(), (size_t)W * 4);
         dst_id[k] = id;
         dst_off[k] = x.off_tab;
         recs[k] = x;
     }
     if (!dev) return PA_OK;
     DevBuf b[4];
-    HIPCHK(e, hipMalloc(&b[0].p, tabs.size() * 4));
+    HIPCHK(e, hipMalloc(&b[0].p, stage.size() * 4));
     HIPCHK(e, hipMalloc(&b[1].p, (size_t)n * sizeof(ElemDev)));
     HIPCHK(e, hipMalloc(&b[2].p, (size_t)n * 8));
     HIPCHK(e, hipMalloc(&b[3].p, (size_t)n * 4));
-    HIPCHK(e, hipMemcpy(b[0].p, tabs.data(), tabs.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[0].p, stage.data(), stage.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b[1].p, recs.data(), (size_t)n * sizeof(ElemDev), hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b[2].p, dst_off.data(), (size_t)n * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b[3].p, dst_id.data(), (size_t)n * 4, hipMemcpyHostToDevice));

@@ -76,10 +76,14 @@ pa_status engine_history_reserve(pa_engine *e, size_t bytes);
 pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *lengths, const double *piA,
                     const int32_t *lens0, const int32_t *lens1, int32_t precision, double *out0, double *out1,
                     const GtExtra &x);
-// Node table of element id's leaves under the tree children [N*2].
+// Node table of element id's leaves under the tree children [N*2], and its
+// header: root, node count, height levels, depth levels.
 pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr);
-// Makes children[k] [N*2] the stored tree of element ids[k] (host and, when uploaded, device).
-pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32_t *const *children);
+// Makes the node table tabs[k] (with its header hdrs[k], both from
+// gene_table) the stored tree of element ids[k] (host and, when uploaded,
+// device).
+pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32_t *const *tabs,
+                            const int32_t *const *hdrs);
 // The engine's stream (after the device is set up), for launches that a
 // likelihood launch on it must follow.
 pa_status engine_stream(pa_engine *e, hipStream_t *st);
