@@ -337,7 +337,10 @@ pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *sp, const int32
  * Moves, draws, decisions, heights, prior and missing-gene term are
  * pa_gt_branch_len's, bit for bit; accepted (n, or NULL) receives the
  * decisions.  Species trees up to N = 536 nodes (an LDS copy per element);
- * larger ones give PA_E_INPUT. */
+ * larger ones give PA_E_INPUT.  The session keeps every element's partials
+ * on the device (two slots per internal node: 2 (N - S) Ppad 20 bytes per
+ * element in FP32, 40 in FP64), so a move prunes only the path from the
+ * moved node to the root; not enough device memory gives PA_E_OOM. */
 typedef struct pa_gt_session pa_gt_session;
 pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *sp, const int32_t *ids, int32_t n,
                                const int32_t *gchildren, const int32_t *gparent, const double *gheight,

@@ -155,13 +155,15 @@ def test_branch_len_sweeps_match_the_oracle(require_gpu, use_pi):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_pi,S", [(False, 9), (True, 9), (False, 41)])
-def test_device_session_equals_the_host_moves(require_gpu, use_pi, S):
+@pytest.mark.parametrize("use_pi,S,precs", [(False, 9, "ddd"), (True, 9, "ddd"), (False, 41, "ddd"),
+                                            (True, 9, "dff")])
+def test_device_session_equals_the_host_moves(require_gpu, use_pi, S, precs):
     """The device-resident height moves (pa_gt_session_*) give, move for move,
-    the decisions, heights and state of pa_gt_branch_len (FP64), and read
-    back the same final state, the prior and missing-gene term to the bit;
-    both paths are held to the oracle above.  S = 41 puts more gene nodes
-    than a wave has lanes."""
+    the decisions, heights and state of pa_gt_branch_len, and read back the
+    same final state, the prior and missing-gene term to the bit; both paths
+    are held to the oracle above.  S = 41 puts more gene nodes than a wave
+    has lanes.  precs: each sweep's precision (d FP64, f FP32); a change
+    rebuilds the session's resident partials."""
     w = _gt_workload(seed=8, n=7, S=S)
     opts = E.default_options(min_len=1, seed=SEED)
     eng = E.Engine.from_workload(w, opts)
@@ -187,10 +189,11 @@ def test_device_session_equals_the_host_moves(require_gpu, use_pi, S):
     total = 0
     for m in range(3):
         ind = np.full(n, m % 2, np.int32)
+        prec = E.PA_FP64 if precs[m] == "d" else E.PA_FP32
         for b, v in enumerate(range(S, N)):
             a_host = eng.gt_branch_len(sp, ids, ch, par, H, w.Z, rates, v, ind, lens, ll, lz, prior, m, b,
-                                       cons_to_mis=0.05, ncons_to_mis=0.4, piA=piA)
-            a_dev = ses.branch_len(v, ind, lens, m, b)
+                                       cons_to_mis=0.05, ncons_to_mis=0.4, piA=piA, precision=prec)
+            a_dev = ses.branch_len(v, ind, lens, m, b, precision=prec)
             np.testing.assert_array_equal(a_dev, a_host, err_msg=f"m={m} v={v}")
             total += a_host.sum()
         h2, ll2, lz2, pr2 = ses.read()
