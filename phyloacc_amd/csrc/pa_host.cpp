@@ -467,6 +467,10 @@ struct pa_engine {
     TreeModel tm;
     std::string err;
     std::mutex err_mu;
+    // the last batch's schedule and likelihood scratch plan: a batch that is
+    // launched again (an MCMC loop over the same elements) reuses them
+    std::mutex sched_mu;
+    std::vector<int32_t> sched_ids, sched_order;
     std::vector<HostElem> elems;
     bool dev_dirty = true;
     std::vector<struct pa_batch *> batches;   // prepared batches, freed with the engine
@@ -486,6 +490,8 @@ struct pa_engine {
     // between calls and grown when a call needs more (freed with the engine)
     void *d_ll_scratch = nullptr, *d_ll_len = nullptr;
     size_t ll_scratch_bytes = 0, ll_len_bytes = 0;
+    void *h_ll_len = nullptr;       // pinned staging of the per-call lengths
+    size_t h_ll_len_bytes = 0;
     size_t ll_last_plan_bytes = 0;           // scratch the last likelihood launch used
     // the allocation-free likelihood entry point: this device's copy of its
     // scratch array, and the arenas' longest element and mean pattern count
@@ -646,6 +652,7 @@ extern "C" void pa_engine_destroy(pa_engine *e)
         (void)hipFree(e->d_queue);
         (void)hipFree(e->d_ll_scratch);
         (void)hipFree(e->d_ll_len);
+        (void)hipHostFree(e->h_ll_len);
         (void)hipFree(e->hist.d);
         if (e->ll_t0) (void)hipEventDestroy(e->ll_t0);
         if (e->ll_t1) (void)hipEventDestroy(e->ll_t1);
@@ -937,8 +944,41 @@ static void release_ll_buffers(pa_engine *e)
 {
     (void)hipFree(e->d_ll_scratch);
     (void)hipFree(e->d_ll_len);
+    (void)hipHostFree(e->h_ll_len);
     e->d_ll_scratch = e->d_ll_len = nullptr;
-    e->ll_scratch_bytes = e->ll_len_bytes = 0;
+    e->h_ll_len = nullptr;
+    e->ll_scratch_bytes = e->ll_len_bytes = e->h_ll_len_bytes = 0;
+}
+
+// Per-call lengths to the device: copied into a retained pinned buffer by up
+// to 16 threads, then one asynchronous copy on the engine stream (the
+// launch that reads them follows on that stream and the call waits for it).
+static pa_status upload_lengths(pa_engine *e, const double *lengths, size_t bytes)
+{
+    // no copy of an earlier call may still read the staging buffer
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (bytes > e->h_ll_len_bytes) {
+        (void)hipHostFree(e->h_ll_len);
+        e->h_ll_len = nullptr;
+        e->h_ll_len_bytes = 0;
+        HIPCHK(e, hipHostMalloc(&e->h_ll_len, bytes));
+        e->h_ll_len_bytes = bytes;
+    }
+    const size_t chunk = (size_t)4 << 20;
+    const int parts = (int)std::min<size_t>(16, (bytes + chunk - 1) / chunk);
+    if (parts <= 1) {
+        memcpy(e->h_ll_len, lengths, bytes);
+    } else {
+        const size_t per = (bytes / parts + 63) & ~(size_t)63;
+        std::vector<std::thread> th;
+        for (int t = 0; t < parts; t++) {
+            const size_t a = std::min(bytes, per * t), b = std::min(bytes, per * (t + 1));
+            if (b > a) th.emplace_back([=]() { memcpy((char *)e->h_ll_len + a, (const char *)lengths + a, b - a); });
+        }
+        for (auto &x : th) x.join();
+    }
+    HIPCHK(e, hipMemcpyAsync(e->d_ll_len, e->h_ll_len, bytes, hipMemcpyHostToDevice, e->stream));
+    return PA_OK;
 }
 
 // Fixed-(Z, r) log-likelihood launch: one persistent launch over ids (in
@@ -1114,11 +1154,25 @@ static std::vector<int32_t> schedule(pa_engine *e, const int32_t *ids, int32_t n
 {
     std::vector<int32_t> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
-    if (mode == PA_RUN_BATCHED)
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            const HostElem &x = e->elems[ids[a]], &y = e->elems[ids[b]];
-            return (int64_t)x.GG * 3 + x.P > (int64_t)y.GG * 3 + y.P;
-        });
+    if (mode != PA_RUN_BATCHED || n == 0) return order;
+    {
+        std::lock_guard<std::mutex> lock(e->sched_mu);
+        if ((int)e->sched_ids.size() == n && memcmp(e->sched_ids.data(), ids, sizeof(int32_t) * n) == 0)
+            return e->sched_order;
+    }
+    // longest first (GG * 3 + P descending), ties in batch order: one sort of
+    // packed (inverted key, position) words
+    std::vector<uint64_t> kk(n);
+    for (int i = 0; i < n; i++) {
+        const HostElem &x = e->elems[ids[i]];
+        const int64_t key = (int64_t)x.GG * 3 + x.P;
+        kk[i] = ((uint64_t)(INT64_C(0x7fffffff) - std::min<int64_t>(key, INT64_C(0x7fffffff))) << 32) | (uint32_t)i;
+    }
+    std::sort(kk.begin(), kk.end());
+    for (int i = 0; i < n; i++) order[i] = (int32_t)(kk[i] & 0xffffffffu);
+    std::lock_guard<std::mutex> lock(e->sched_mu);
+    e->sched_ids.assign(ids, ids + n);
+    e->sched_order = order;
     return order;
 }
 
@@ -1269,7 +1323,7 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
         HIPCHK(e, hipMalloc(&e->d_ll_len, lbytes));
         e->ll_len_bytes = lbytes;
     }
-    if (!dev_len) HIPCHK(e, hipMemcpy(e->d_ll_len, lengths, lbytes, hipMemcpyHostToDevice));
+    if (!dev_len && (st = upload_lengths(e, lengths, lbytes)) != PA_OK) return st;
     HIPCHK(e, hipMemcpy(bufs[6].p, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
     LoglikArgs A;
     memset(&A, 0, sizeof A);
