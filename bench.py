@@ -615,7 +615,9 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
     the history draw, one regraft (Sample_tree2) per element, the height move
     (Sample_BranchLen) at every internal gene node, and one sample_pi step;
     every move of every element batched into one launch per move (the
-    regraft takes two).  Z and the rates stay fixed (the species-branch
+    regraft takes two; the height moves run in a device-resident session
+    whose first full pass stores every element's partials, then one path
+    pass per move).  Z and the rates stay fixed (the species-branch
     classes of the workload).  Wall clock per phase, one warmup sweep."""
     prec = E.PA_FP32 if precision == "fp32" else E.PA_FP64
     eng = E.Engine.from_workload(w, E.default_options(min_len=1))
@@ -690,7 +692,9 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
             "ms_per_sweep": round(tot / sweeps * 1e3, 1),
             "phase_ms": {k: round(v / sweeps * 1e3, 1) for k, v in phases.items()},
             "heights_kernel_ms": round(kernel_heights / sweeps, 1),
-            "launches_per_sweep": 1 + 2 + (N - S) + 1,
+            # likelihood launches: history, two for the regraft, the session's
+            # first full pass, one path pass per height move, the pi step
+            "launches_per_sweep": 1 + 2 + 1 + (N - S) + 1,
             "accept_rate": {k: round(v / (n * sweeps * ((N - S) if k == "heights" else 1)), 4)
                             for k, v in acc.items()},
             "precision": precision, "heights_path": "device-resident session (pa_gt_session_*)",
