@@ -1,6 +1,7 @@
 // pa_gtdev.h -- launch interface of the device-resident gene-tree height
-// moves (pa_gtdev.hip): one lane per element proposes, a likelihood launch
-// scores the proposals, one lane per element decides.
+// moves (pa_gtdev.hip): one wave per element proposes, a path likelihood
+// launch over the resident partials scores the proposals, one lane per
+// element decides.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
