@@ -437,10 +437,21 @@ void tree_candidates(const Embed &g, int branch, int root_ingrp, const int32_t *
     t.par2[t.bp] = -1;
     // gene nodes born in each species, by height (bp is gone; branch's own
     // subtree stays, below branch, and is passed over)
-    std::vector<std::vector<int>> born(N);
+    // (per-species ranges of one list: born[boff[x] .. boff[x + 1]))
+    thread_local std::vector<int> bcnt, boff, born;
+    bcnt.assign(N, 0);
     for (int v : g.order)
-        if (g.internal(v) && v != t.bp) born[g.where[v]].push_back(v);
-    for (auto &b : born) std::sort(b.begin(), b.end(), [&](int a, int c) { return g.h[a] < g.h[c]; });
+        if (g.internal(v) && v != t.bp) bcnt[g.where[v]]++;
+    boff.resize(N + 1);
+    boff[0] = 0;
+    for (int x = 0; x < N; x++) boff[x + 1] = boff[x] + bcnt[x];
+    born.resize(boff[N]);
+    for (int x = 0; x < N; x++) bcnt[x] = boff[x];
+    for (int v : g.order)
+        if (g.internal(v) && v != t.bp) born[bcnt[g.where[v]]++] = v;
+    for (int x = 0; x < N; x++)
+        if (boff[x + 1] - boff[x] > 1)
+            std::sort(born.begin() + boff[x], born.begin() + boff[x + 1], [&](int a, int c) { return g.h[a] < g.h[c]; });
     auto rate = [&](int x) { return Z[x] == 2 ? r2 : (Z[x] == 1 ? r1 : 1.0); };
     auto push = [&](int node, int kind, int x, double len) {
         t.cnode.push_back(node); t.ckind.push_back(kind); t.csp.push_back(x); t.clen.push_back(len);
@@ -449,7 +460,7 @@ void tree_candidates(const Embed &g, int branch, int root_ingrp, const int32_t *
     for (int x = t.ss; x < N;) {
         const double base = std::max(s.H[x], g.h[branch]);
         const int up = s.up[x];
-        if (born[x].empty()) {
+        if (boff[x] == boff[x + 1]) {
             if (up == N) {
                 push(N, t.cnode.empty() ? 3 : 2, x, rate(x) * (t.curN - base) + cur);
                 break;
@@ -458,7 +469,8 @@ void tree_candidates(const Embed &g, int branch, int root_ingrp, const int32_t *
             x = up;
             continue;
         }
-        for (int v : born[x]) {
+        for (int j = boff[x]; j < boff[x + 1]; j++) {
+            const int v = born[j];
             if (x == t.ss && g.h[v] <= g.h[branch]) continue;
             push(v, 0, x, rate(x) * (g.h[v] - base) + cur);
         }

@@ -279,9 +279,12 @@ struct Collapse {
 // under root (BPP_C::getSubtree, src/PhyloAcc-ST/bpp_c.cpp:44-60), the
 // internal nodes by height level, the non-root nodes by depth level, the
 // Z-draw slots and the branch lengths.  own: a gene tree (no outgroup flags).
-void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std::vector<int> &par2,
-                     const std::vector<double> &dist2, const std::vector<uint8_t> &missing, bool own, int root,
-                     HostElem &E)
+// The table into T (tab_words(N) words, every word written); the post-order
+// and internal-node lists stay in per-thread storage behind nodes_out /
+// internal_out until the thread's next call.
+static void fill_table(const TreeModel &tm, const std::vector<int> &ch2, const std::vector<int> &par2,
+                       const double *dist2, const std::vector<uint8_t> &missing, bool own, int root, int32_t *T,
+                       int &n_up, int &n_down, const std::vector<int> **nodes_out, const std::vector<int> **internal_out)
 {
     const int S = tm.S, N = tm.N;
     // scratch kept per thread: tables are rebuilt for every proposed gene tree
@@ -311,8 +314,7 @@ void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std
         depth[v] = v == root ? 0 : depth[par2[v]] + 1;
         dmax = std::max(dmax, depth[v]);
     }
-    E.tab.assign(tab_words(N), 0);
-    int32_t *T = E.tab.data();
+    memset(T, 0, sizeof(int32_t) * tab_words(N));
     for (int i = 0; i < N; i++) {
         T[tab_ch0(N) + i] = ch2[2 * i];
         T[tab_ch1(N) + i] = ch2[2 * i + 1];
@@ -353,11 +355,22 @@ void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std
     }
     double *D = (double *)(T + tab_dist(N));
     for (int i = 0; i < N; i++) D[i] = dist2[i];
-    E.n_up = hmax;
-    E.n_down = dmax;
+    n_up = hmax;
+    n_down = dmax;
+    *nodes_out = &nodes;
+    *internal_out = &internal;
+}
+
+void fill_tree_table(const TreeModel &tm, const std::vector<int> &ch2, const std::vector<int> &par2,
+                     const std::vector<double> &dist2, const std::vector<uint8_t> &missing, bool own, int root,
+                     HostElem &E)
+{
+    E.tab.resize(tab_words(tm.N));
+    const std::vector<int> *nodes, *internal;
+    fill_table(tm, ch2, par2, dist2.data(), missing, own, root, E.tab.data(), E.n_up, E.n_down, &nodes, &internal);
     E.root = root;
-    E.nodes = nodes;
-    E.internal = internal;
+    E.nodes = *nodes;
+    E.internal = *internal;
 }
 
 void encode_element(const TreeModel &tm, const pa_options &o, const char *const *rows, int64_t start, int len,
@@ -1436,7 +1449,7 @@ static pa_status loglik_lengths_impl(pa_engine *e, const int32_t *ids, int32_t n
 // Element id's node table under the tree children [N*2] (the root is the
 // node no other node has as a child), as a gene tree: missing leaves from
 // the element, an internal node missing when both children are.
-static pa_status gene_elem(pa_engine *e, int32_t id, const int32_t *children, HostElem &out)
+static pa_status gene_elem(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr)
 {
     uint8_t lm[MAXN];
     pa_status st = element_gene_tree(e, id, lm, nullptr);
@@ -1476,7 +1489,10 @@ static pa_status gene_elem(pa_engine *e, int32_t id, const int32_t *children, Ho
     missing.assign(N, 0);
     for (int v : post) missing[v] = v < S ? lm[v] : (missing[ch2[2 * v]] && missing[ch2[2 * v + 1]]);
     zero.assign(N, 0.0);
-    fill_tree_table(e->tm, ch2, par2, zero, missing, true, root, out);
+    const std::vector<int> *nodes, *internal;
+    int n_up = 0, n_down = 0;
+    fill_table(e->tm, ch2, par2, zero.data(), missing, true, root, T, n_up, n_down, &nodes, &internal);
+    hdr[0] = root; hdr[1] = (int32_t)nodes->size(); hdr[2] = n_up; hdr[3] = n_down;
     return PA_OK;
 }
 
@@ -1484,12 +1500,7 @@ namespace pa {
 
 pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr)
 {
-    thread_local HostElem tmp;
-    pa_status st = gene_elem(e, id, children, tmp);
-    if (st != PA_OK) return st;
-    memcpy(T, tmp.tab.data(), (size_t)tab_words(e->tm.N) * 4);
-    hdr[0] = tmp.root; hdr[1] = (int32_t)tmp.nodes.size(); hdr[2] = tmp.n_up; hdr[3] = tmp.n_down;
-    return PA_OK;
+    return gene_elem(e, id, children, T, hdr);
 }
 
 // Many elements' trees at once, from their node tables (gene_table's):
