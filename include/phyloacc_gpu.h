@@ -336,7 +336,7 @@ pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *sp, const int32
  * their topology while the session lives (a regraft needs a new session).
  * Moves, draws, decisions, heights, prior and missing-gene term are
  * pa_gt_branch_len's, bit for bit; accepted (n, or NULL) receives the
- * decisions.  Species trees up to N = 536 nodes (an LDS copy per element);
+ * decisions.  Species trees up to N = 507 nodes (an LDS copy per element);
  * larger ones give PA_E_INPUT.  The session keeps every element's partials
  * on the device (two slots per internal node: 2 (N - S) Ppad 20 bytes per
  * element in FP32, 40 in FP64), so a move prunes only the path from the

@@ -56,7 +56,7 @@ struct GtDecideArgs {
 
 // LDS bytes of one proposal workgroup at N species-tree nodes (the species
 // tree, the element's rows and the move's workspace)
-inline size_t gt_hprop_lds(int N) { return (size_t)117 * N + 4; }
+inline size_t gt_hprop_lds(int N) { return (size_t)129 * N + 8; }
 constexpr size_t GT_HPROP_LDS_MAX = 64 * 1024 - 64;   // and 48 bytes of static LDS
 
 hipError_t launch_gt_hprop(const GtPropArgs &a, hipStream_t st);

@@ -44,8 +44,8 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
     double *seq = h + N, *lzseq = seq + 2 * N;
     int32_t *up = (int32_t *)(lzseq + N), *post = up + N, *ipost = post + N, *nleaf = ipost + N, *ch = nleaf + N;
     int32_t *par = ch + 2 * N, *Z = par + N, *where = Z + N, *end = where + N, *cnt = end + N, *below = cnt + N;
-    int32_t *soff = below + N;
-    uint8_t *miss = (uint8_t *)(soff + N + 1);
+    int32_t *soff = below + N, *fill = soff + N + 1, *blist = fill + N;
+    uint8_t *miss = (uint8_t *)(blist + 2 * N);
     const size_t r = (size_t)i * N;
     for (int k = t; k < N; k += 64) {
         H[k] = a.sp.H[k];
@@ -64,6 +64,7 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
         miss[k] = a.miss[r + k];
         cnt[k] = 0;
         below[k] = 0;
+        fill[k] = 0;
     }
     for (int k = t; k < 2 * N; k += 64) ch[k] = a.ch[2 * r + k];
     if (t < 4) lc[t] = a.lc[t];
@@ -123,22 +124,30 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
             if (t == 63) total_sh = inc;
         }
         __syncthreads();
+        // each internal species' born nodes as a list (in any order) at its
+        // sequence place
+        for (int u = S + t; u < N; u += 64) {
+            const int x = where[u];
+            if (x >= S) blist[soff[ipost[x]] + atomicAdd(&fill[x], 1)] = u;
+        }
+        __syncthreads();
         // born nodes: rank in the species' height order (ties by index), the
         // height before, the term; the last one also writes the tail
         for (int u = S + t; u < N; u += 64) {
             const int x = where[u];
             if (x < S) continue;
             const double hu = h[u];
+            const int base = soff[ipost[x]], ke = nleaf[x] - below[x], n_b = cnt[x];
             int rk = 0;
             double prev = H[x];
-            for (int q = S; q < N; q++) {
+            for (int j = 0; j < n_b; j++) {
+                const int q = blist[base + j];
                 const double hq = h[q];
-                if (where[q] == x && (hq < hu || (hq == hu && q < u))) {
+                if (hq < hu || (hq == hu && q < u)) {
                     rk++;
                     prev = fmax(prev, hq);
                 }
             }
-            const int base = soff[ipost[x]], ke = nleaf[x] - below[x], n_b = cnt[x];
             seq[base + rk] = ke == 0 ? 0.0 : gt::coal_term(ldens[x], rate[x], hu, prev, ke - rk);
             if (rk == n_b - 1) {
                 const int kf = ke - n_b;
@@ -161,6 +170,7 @@ __global__ __launch_bounds__(64) void gt_hprop_kernel(GtPropArgs a)
             const double *arr = t == 0 ? seq : lzseq;
             const int len = t == 0 ? total_sh : N;
             double acc = 0;
+#pragma unroll 8
             for (int j = 0; j < len; j++) acc += arr[j];
             if (t == 0) a.prior_new[i] = acc;
             else a.lz_new[i] = acc;
