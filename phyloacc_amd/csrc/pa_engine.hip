@@ -62,6 +62,7 @@ struct Ctx {
     // element
     int S, N, P, Ppad, root, n_nodes, n_up;
     int h0;                    // first tree level the pass visits
+    int path;                  // 1: the levels are one path (up[k]'s child on the path is up[k - 1])
     const GBL uint8_t *codes;
     const GBL int32_t *wts;
     int pre0, pre1;            // site-prefix root sums (pre0 < 0: the pattern sum)
@@ -272,6 +273,211 @@ hipError_t launch_gbgc_eig(const double *Q0, const double *sel, int n, double *e
 template <typename R>
 struct Unroll { static constexpr int U = sizeof(R) == 4 ? 4 : 2; };
 
+// The root sum of a pass (log_f_Xz, src/PhyloAcc-ST/bpp_c2.cpp:392-425):
+// log sum_j pi_j L_j + exponent * ln 2 per pattern (or per site over the two
+// prefixes), FP64, identical on every thread.
+template <typename R, int BS>
+__device__ __forceinline__ double root_sum(CP<R> c)
+{
+    const int tid = threadIdx.x;
+    const int P = uni(c->P), Ppad = uni(c->Ppad), S = uni(c->S), N = uni(c->N);
+    GBL V4<R> *parts = uni_ptr(c->parts);
+    GBL int32_t *exps = uni_ptr(c->exps);
+    constexpr int U = Unroll<R>::U;
+    // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
+    const int r = c->root;
+    const int pre0 = uni(c->pre0);
+    double acc = 0, acc1 = 0;
+    if (pre0 >= 0 && r >= S) {
+        // two site prefixes: each site reads its pattern's root partial,
+        // which another wave may have written, so the pass must finish first
+        __syncthreads();
+        const int pre1 = uni(c->pre1);
+        const GBL int32_t *site = uni_ptr(c->site);
+        const size_t ro = node_slot(r, uni(c->flags[r]), S, N) * Ppad;
+        const GBL V4<R> *rp = parts + ro;
+        const GBL int32_t *re = exps + ro;
+        const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
+        for (int g = tid; g < pre0; g += BS) {
+            const int p = site[g];
+            const int ex = re[p];
+            if (ex == GAPT) continue;
+            const V4<R> av = ld4<R>(rp + p);
+            const double L = p0 * (double)av.x + p1 * (double)av.y + p2 * (double)av.z + p3 * (double)av.w;
+            const double t = log(L) + (double)ex * 0.69314718055994530942;
+            acc += t;
+            if (g < pre1) acc1 += t;
+        }
+        const double s1 = block_sum(acc1, c->red);
+        if (threadIdx.x == 0) c->sum1 = s1;
+        return block_sum(acc, c->red);
+    }
+    if (r >= S) {
+        const size_t ro = node_slot(r, uni(c->flags[r]), S, N) * Ppad;
+        const GBL V4<R> *rp = parts + ro;
+        const GBL int32_t *re = exps + ro;
+        const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
+        for (int pb = tid; pb < P; pb += U * BS) {
+            int ex[U], wt[U];
+            V4<R> av[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int p = pb + u * BS;
+                ex[u] = GAPT;
+                if (p < P) {
+                    ex[u] = re[p];
+                    wt[u] = c->wts[p];
+                    av[u] = ld4<R>(rp + p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (ex[u] == GAPT) continue;
+                const double L = p0 * (double)av[u].x + p1 * (double)av[u].y + p2 * (double)av[u].z + p3 * (double)av[u].w;
+                const double t = log(L) + (double)ex[u] * 0.69314718055994530942;
+                acc += (double)wt[u] * t;
+            }
+        }
+    }
+    return block_sum(acc, c->red);
+}
+
+// Node x's input at pattern p as the pruning pass loads it: a leaf's code, a
+// cherry's two leaf codes packed, or a stored node's partial and exponent.
+template <typename R>
+__device__ __forceinline__ ChildIn<R> raw_in(CP<R> c, const GBL uint8_t *codes, const GBL V4<R> *parts,
+                                             const GBL int32_t *exps, int x, int f, int S, int N, int Ppad, int p)
+{
+    ChildIn<R> in;
+    in.v.x = in.v.y = in.v.z = in.v.w = (R)0;
+    if (x < S) {
+        in.x = codes[(size_t)x * Ppad + p];
+    } else if (f & NF_CHERRY) {
+        in.x = codes[(size_t)c->ch0[x] * Ppad + p] | (codes[(size_t)c->ch1[x] * Ppad + p] << 8);
+    } else {
+        const size_t o = node_slot(x, f, S, N) * Ppad;
+        in.x = exps[o + p];
+        if (!(f & NF_MISSING)) in.v = ld4<R>(parts + o + p);
+    }
+    return in;
+}
+
+// A cherry's packed codes into its scaled partial (cherry_in); other inputs as they are.
+template <typename R>
+__device__ __forceinline__ ChildIn<R> cooked_in(CP<R> c, int x, int f, ChildIn<R> in)
+{
+    if (!(f & NF_CHERRY)) return in;
+    const int yA = c->ch0[x], yB = c->ch1[x];
+    return cherry_in<R>(c->PT + 16 * yA, c->PT + 16 * yB, c->flags[yA] & NF_MISSING, c->flags[yB] & NF_MISSING,
+                        f & NF_MISSING, in.x & 255, in.x >> 8);
+}
+
+// Node s at pattern p from its children's inputs (child 0 first, as the
+// pruning pass applies them), stored into s's slot; returns what a later
+// load of s at p would read.
+template <typename R>
+__device__ __forceinline__ ChildIn<R> node_at(CP<R> c, GBL V4<R> *parts, GBL int32_t *exps, int s, int x0, int x1,
+                                              const ChildIn<R> &in0, const ChildIn<R> &in1, int S, int N, int Ppad,
+                                              int p)
+{
+    const int fs = c->flags[s], f0 = c->flags[x0], f1 = c->flags[x1];
+    const size_t o = node_slot(s, fs, S, N) * Ppad;
+    R a0 = 1, a1 = 1, a2 = 1, a3 = 1;
+    int e = 0;
+    bool g0, g1;
+    child_apply<R>(x0 < S, f0 & NF_MISSING, c->PT + 16 * x0, in0, a0, a1, a2, a3, e, g0);
+    child_apply<R>(x1 < S, f1 & NF_MISSING, c->PT + 16 * x1, in1, a0, a1, a2, a3, e, g1);
+    ChildIn<R> out;
+    out.v.x = out.v.y = out.v.z = out.v.w = (R)0;
+    if (g0 && g1) {
+        out.x = GAPT;
+    } else if (fs & NF_MISSING) {
+        out.x = 0;
+    } else {
+        const R mx = fmax(fmax(a0, a1), fmax(a2, a3));
+        int kx = 0;
+        frexp(mx, &kx);
+        a0 = ldexp(a0, -kx); a1 = ldexp(a1, -kx); a2 = ldexp(a2, -kx); a3 = ldexp(a3, -kx);
+        st4<R>(parts + o + p, a0, a1, a2, a3);
+        out.v.x = a0; out.v.y = a1; out.v.z = a2; out.v.w = a3;
+        out.x = e + kx;
+    }
+    exps[o + p] = out.x;
+    return out;
+}
+
+// The levels of a path pass (the height moves' resident partials): level k
+// is the one node up[k], and its child on the path is up[k - 1].  A thread
+// takes each of its patterns through every level: the child on the path
+// comes from the previous level in registers, and the loads of the next
+// levels' other children are in flight PD levels ahead, so the levels do not
+// each wait for a memory round trip.  The values are the level-by-level
+// pass's, node by node and pattern by pattern.
+template <typename R, int BS>
+__device__ __noinline__ double prune_path(CP<R> c, unsigned long long *bytes)
+{
+    constexpr int PD = sizeof(R) == 4 ? 8 : 4;
+    const int tid = threadIdx.x;
+    const int P = uni(c->P), Ppad = uni(c->Ppad), S = uni(c->S), N = uni(c->N), K = uni(c->n_up);
+    GBL V4<R> *parts = uni_ptr(c->parts);
+    GBL int32_t *exps = uni_ptr(c->exps);
+    const GBL uint8_t *codes = uni_ptr(c->codes);
+    if (bytes)
+        for (int q = 0; q < K; q++) {
+            // as a level-by-level pass counts its nodes
+            const int s = uni(c->up[q]);
+            unsigned long long per = 4 * sizeof(R) + 4;
+            for (int j = 0; j < 2; j++) {
+                const int x = uni(j ? c->ch1[s] : c->ch0[s]);
+                per += x < S ? 1 : ((uni(c->flags[x]) & NF_CHERRY) ? 2 : 4 * sizeof(R) + 4);
+            }
+            *bytes += (unsigned long long)P * per;
+        }
+    for (int p = tid; p < P; p += BS) {
+        // level 0: both children are read
+        const int s0 = uni(c->up[0]);
+        const int a0 = uni(c->ch0[s0]), a1 = uni(c->ch1[s0]);
+        const int fa0 = uni(c->flags[a0]), fa1 = uni(c->flags[a1]);
+        ChildIn<R> i0 = raw_in<R>(c, codes, parts, exps, a0, fa0, S, N, Ppad, p);
+        ChildIn<R> i1 = raw_in<R>(c, codes, parts, exps, a1, fa1, S, N, Ppad, p);
+        // the other children of levels 1 .. PD, in flight
+        ChildIn<R> ring[PD];
+#pragma unroll
+        for (int j = 0; j < PD; j++) {
+            const int h = 1 + j;
+            if (h < K) {
+                const int s = uni(c->up[h]), prev = uni(c->up[h - 1]);
+                const int x0 = uni(c->ch0[s]), x1 = uni(c->ch1[s]);
+                const int xo = x0 == prev ? x1 : x0;
+                ring[j] = raw_in<R>(c, codes, parts, exps, xo, uni(c->flags[xo]), S, N, Ppad, p);
+            }
+        }
+        ChildIn<R> carry = node_at<R>(c, parts, exps, s0, a0, a1, cooked_in<R>(c, a0, fa0, i0),
+                                      cooked_in<R>(c, a1, fa1, i1), S, N, Ppad, p);
+        for (int h0 = 1; h0 < K; h0 += PD) {
+#pragma unroll
+            for (int j = 0; j < PD; j++) {
+                const int h = h0 + j;
+                if (h >= K) break;
+                const int s = uni(c->up[h]), prev = uni(c->up[h - 1]);
+                const int x0 = uni(c->ch0[s]), x1 = uni(c->ch1[s]);
+                const bool on0 = x0 == prev;
+                const int xo = on0 ? x1 : x0;
+                const ChildIn<R> other = cooked_in<R>(c, xo, uni(c->flags[xo]), ring[j]);
+                if (h + PD < K) {
+                    const int sn = uni(c->up[h + PD]), pn = uni(c->up[h + PD - 1]);
+                    const int y0 = uni(c->ch0[sn]), y1 = uni(c->ch1[sn]);
+                    const int yo = y0 == pn ? y1 : y0;
+                    ring[j] = raw_in<R>(c, codes, parts, exps, yo, uni(c->flags[yo]), S, N, Ppad, p);
+                }
+                carry = on0 ? node_at<R>(c, parts, exps, s, x0, x1, carry, other, S, N, Ppad, p)
+                            : node_at<R>(c, parts, exps, s, x0, x1, other, carry, S, N, Ppad, p);
+            }
+        }
+    }
+    return root_sum<R, BS>(c);
+}
+
 // The pruning pass over every internal node of the element's tree
 // (BPP_C::Update_Tg(tosample=false), src/PhyloAcc-ST/bpp_c.cpp:644-671) and
 // the root sum (log_f_Xz, src/PhyloAcc-ST/bpp_c2.cpp:392-425).  Missing
@@ -388,62 +594,7 @@ __device__ __noinline__ double prune(MP M, CP<R> c, unsigned long long *bytes)
             }
         }
     }
-    // root: log sum_j pi_j L_j + exponent * ln 2; all-gap columns contribute 0
-    const int r = c->root;
-    const int pre0 = uni(c->pre0);
-    double acc = 0, acc1 = 0;
-    if (pre0 >= 0 && r >= S) {
-        // two site prefixes: each site reads its pattern's root partial,
-        // which another wave may have written, so the pass must finish first
-        __syncthreads();
-        const int pre1 = uni(c->pre1);
-        const GBL int32_t *site = uni_ptr(c->site);
-        const size_t ro = node_slot(r, uni(c->flags[r]), S, N) * Ppad;
-        const GBL V4<R> *rp = parts + ro;
-        const GBL int32_t *re = exps + ro;
-        const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
-        for (int g = tid; g < pre0; g += BS) {
-            const int p = site[g];
-            const int ex = re[p];
-            if (ex == GAPT) continue;
-            const V4<R> av = ld4<R>(rp + p);
-            const double L = p0 * (double)av.x + p1 * (double)av.y + p2 * (double)av.z + p3 * (double)av.w;
-            const double t = log(L) + (double)ex * 0.69314718055994530942;
-            acc += t;
-            if (g < pre1) acc1 += t;
-        }
-        const double s1 = block_sum(acc1, c->red);
-        if (threadIdx.x == 0) c->sum1 = s1;
-        return block_sum(acc, c->red);
-    }
-    if (r >= S) {
-        const size_t ro = node_slot(r, uni(c->flags[r]), S, N) * Ppad;
-        const GBL V4<R> *rp = parts + ro;
-        const GBL int32_t *re = exps + ro;
-        const double p0 = c->pi[0], p1 = c->pi[1], p2 = c->pi[2], p3 = c->pi[3];
-        for (int pb = tid; pb < P; pb += U * BS) {
-            int ex[U], wt[U];
-            V4<R> av[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int p = pb + u * BS;
-                ex[u] = GAPT;
-                if (p < P) {
-                    ex[u] = re[p];
-                    wt[u] = c->wts[p];
-                    av[u] = ld4<R>(rp + p);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (ex[u] == GAPT) continue;
-                const double L = p0 * (double)av[u].x + p1 * (double)av[u].y + p2 * (double)av[u].z + p3 * (double)av[u].w;
-                const double t = log(L) + (double)ex[u] * 0.69314718055994530942;
-                acc += (double)wt[u] * t;
-            }
-        }
-    }
-    return block_sum(acc, c->red);
+    return root_sum<R, BS>(c);
 }
 
 // ------------------------------------------- gene-tree moves (PhyloAcc-GT)
@@ -655,6 +806,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
     __syncthreads();
     if (tid == 0) {
         c->h0 = c->n_up > 1 ? 1 : 0;
+        c->path = 0;
         const int v = A->path_node ? A->path_node[row] : -1;
         if (v >= c->S && v < N) {
             // the pass visits the path from v to the root (cherries are
@@ -672,6 +824,7 @@ __device__ __noinline__ void bind_element(MP M, CP<R> c, AP A, int w)
             }
             c->n_up = k;
             c->h0 = 0;
+            c->path = 1;
         }
     }
     const bool own = A->len_in != nullptr, gb = A->gb_eig != nullptr;
@@ -746,7 +899,8 @@ __global__ void __launch_bounds__(BS, 3) loglik_kernel(ModelDev Mk, LoglikArgs A
         }
         bind_element<R, BS>(M, c, A, w);
         unsigned long long bytes = 0;
-        const double l = prune<R, BS>(M, c, A->counters && threadIdx.x == 0 ? &bytes : nullptr);
+        unsigned long long *bp = A->counters && threadIdx.x == 0 ? &bytes : nullptr;
+        const double l = uni(c->path) ? prune_path<R, BS>(c, bp) : prune<R, BS>(M, c, bp);
         if (A->hist_draw || A->cand_score) {
             const int row = A->len_row ? A->len_row[w] : w;
             if (A->hist_draw) gt_history<R, BS>(M, c, A, row);
