@@ -645,6 +645,7 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
     acc = {"regraft": 0, "heights": 0, "pi": 0}
     moves = 0
     kernel_heights = 0.0
+    bytes_heights = 0.0
     for m in range(sweeps + 1):
         timed = m > 0
         ind = np.zeros(n, np.int32)
@@ -662,11 +663,14 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
         t2b = time.time()
         ah = 0
         kms = 0.0
+        kbytes = 0.0
         # the trees stay on the device for the whole pass over the nodes
         ses = E.GtSession(eng, sp, ids, ch, par, H, Z, rates, ll, lz, prior, piA=piA)
         for b, v in enumerate(range(S, N)):
             ah += int(ses.branch_len(v, ind, lens, m, 1 + b, precision=prec).sum())
-            kms += eng.loglik_stats()["elapsed_ms"]
+            lst = eng.loglik_stats()
+            kms += lst["elapsed_ms"]
+            kbytes += lst["algorithmic_bytes"]
         H[:], ll[:], lz[:], prior[:] = ses.read()
         ses.close()
         t3 = time.time()
@@ -680,6 +684,7 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
             phases["regraft"] += t2 - t1
             phases["heights"] += t3 - t2b
             kernel_heights += kms
+            bytes_heights += kbytes
             phases["pi"] += t4 - t3b
             acc["regraft"] += int(a.sum())
             acc["heights"] += ah
@@ -692,6 +697,11 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
             "ms_per_sweep": round(tot / sweeps * 1e3, 1),
             "phase_ms": {k: round(v / sweeps * 1e3, 1) for k, v in phases.items()},
             "heights_kernel_ms": round(kernel_heights / sweeps, 1),
+            # the path passes' counted algorithmic bytes over their HIP-event time
+            "heights_kernel_roofline": {"bound": "hbm", "achieved": round(bytes_heights / (kernel_heights * 1e-3) / 1e9, 1),
+                                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                        "frac": round(bytes_heights / (kernel_heights * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                        "bytes_per_sweep": bytes_heights / sweeps},
             # likelihood launches: history, two for the regraft, the session's
             # first full pass, one path pass per height move, the pi step
             "launches_per_sweep": 1 + 2 + 1 + (N - S) + 1,
