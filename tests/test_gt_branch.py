@@ -156,14 +156,15 @@ def test_branch_len_sweeps_match_the_oracle(require_gpu, use_pi):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("use_pi,S,precs", [(False, 9, "ddd"), (True, 9, "ddd"), (False, 41, "ddd"),
-                                            (True, 9, "dff")])
+                                            (True, 9, "dff"), (True, 41, "fff")])
 def test_device_session_equals_the_host_moves(require_gpu, use_pi, S, precs):
     """The device-resident height moves (pa_gt_session_*) give, move for move,
     the decisions, heights and state of pa_gt_branch_len, and read back the
     same final state, the prior and missing-gene term to the bit; both paths
     are held to the oracle above.  S = 41 puts more gene nodes than a wave
-    has lanes.  precs: each sweep's precision (d FP64, f FP32); a change
-    rebuilds the session's resident partials."""
+    has lanes, and paths deeper than the path pass's loads in flight (4 in
+    FP64, 8 in FP32).  precs: each sweep's precision (d FP64, f FP32); a
+    change rebuilds the session's resident partials."""
     w = _gt_workload(seed=8, n=7, S=S)
     opts = E.default_options(min_len=1, seed=SEED)
     eng = E.Engine.from_workload(w, opts)
