@@ -661,16 +661,12 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
         t2 = time.time()
         prior = np.array([E.gt_prior(sp, ch[i], par[i], H[i]) for i in range(n)])
         t2b = time.time()
-        ah = 0
-        kms = 0.0
-        kbytes = 0.0
-        # the trees stay on the device for the whole pass over the nodes
+        # the trees stay on the device for the whole pass over the nodes,
+        # every move queued at once (pa_gt_session_sweep)
         ses = E.GtSession(eng, sp, ids, ch, par, H, Z, rates, ll, lz, prior, piA=piA)
-        for b, v in enumerate(range(S, N)):
-            ah += int(ses.branch_len(v, ind, lens, m, 1 + b, precision=prec).sum())
-            lst = eng.loglik_stats()
-            kms += lst["elapsed_ms"]
-            kbytes += lst["algorithmic_bytes"]
+        acc_h, kms = ses.sweep(np.arange(S, N, dtype=np.int32), ind, lens, m, 1, precision=prec)
+        ah = int(acc_h.sum())
+        kbytes = eng.loglik_stats()["algorithmic_bytes"]
         H[:], ll[:], lz[:], prior[:] = ses.read()
         ses.close()
         t3 = time.time()
@@ -707,7 +703,7 @@ def gt_moves_block(E, w, sweeps=1, precision="fp32"):
             "launches_per_sweep": 1 + 2 + 1 + (N - S) + 1,
             "accept_rate": {k: round(v / (n * sweeps * ((N - S) if k == "heights" else 1)), 4)
                             for k, v in acc.items()},
-            "precision": precision, "heights_path": "device-resident session (pa_gt_session_*)",
+            "precision": precision, "heights_path": "device-resident session, one sweep call (pa_gt_session_sweep)",
             "data": "synthetic gene trees of the workload; Z and rates fixed"}
 
 

@@ -350,6 +350,16 @@ pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *sp, const in
 pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *gnode, const int32_t *indicator,
                                    const int32_t *lens, double delta, int32_t m, int32_t slot, int32_t precision,
                                    int32_t *accepted);
+/* k height moves in a row, e.g. GT's Gibbs step at every internal gene node
+ * (GT bpp_c.cpp:640-660): move j moves gene node gnodes[j*n + i] of element
+ * i with draw slot slot0 + j, exactly as k pa_gt_session_branch_len calls
+ * with slots slot0, slot0 + 1, ... would; all k moves are queued on the
+ * device at once and the call waits once.  accepted: [k*n] or NULL;
+ * kernel_ms (or NULL): the summed time of the k likelihood launches, whose
+ * counted bytes pa_loglik_last_stats then reports in total. */
+pa_status pa_gt_session_sweep(pa_gt_session *s, const int32_t *gnodes, int32_t k, const int32_t *indicator,
+                              const int32_t *lens, double delta, int32_t m, int32_t slot0, int32_t precision,
+                              int32_t *accepted, double *kernel_ms);
 pa_status pa_gt_session_read(pa_gt_session *s, double *gheight, double *loglik, double *logpz, double *prior);
 void pa_gt_session_destroy(pa_gt_session *s);
 

@@ -814,6 +814,10 @@ struct pa_gt_session {
     std::vector<int64_t> big_off;
     GtPlan *plan = nullptr;
     std::vector<int32_t> h_gnode, h_ind, h_acc, l0, l1;   // staged for the asynchronous copies
+    // sweeps: every move's nodes and decisions on the device, one event pair per move
+    int32_t *sw_gnode = nullptr, *sw_acc = nullptr;
+    size_t sw_cap = 0;
+    std::vector<hipEvent_t> ev;
 
     template <typename T>
     hipError_t get(T **p, size_t count, const void *src = nullptr)
@@ -831,6 +835,9 @@ struct pa_gt_session {
         gt_plan_free(plan);
         for (void *p : bufs) (void)hipFree(p);
         (void)hipFree(big);
+        (void)hipFree(sw_gnode);
+        (void)hipFree(sw_acc);
+        for (hipEvent_t x : ev) (void)hipEventDestroy(x);
     }
 };
 
@@ -1036,6 +1043,93 @@ extern "C" pa_status pa_gt_session_branch_len(pa_gt_session *s, const int32_t *g
     }
     if (hipStreamSynchronize(stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height moves failed");
     if (accepted) memcpy(accepted, s->h_acc.data(), sizeof(int32_t) * n);
+    return gt_plan_stats(e, s->plan);
+}
+
+extern "C" pa_status pa_gt_session_sweep(pa_gt_session *s, const int32_t *gnodes, int32_t k, const int32_t *indicator,
+                                         const int32_t *lens, double delta, int32_t m, int32_t slot0,
+                                         int32_t precision, int32_t *accepted, double *kernel_ms)
+{
+    if (!s) return PA_E_INPUT;
+    pa_engine *e = s->e;
+    if (!gnodes || !indicator || !lens || k < 0) return engine_fail(e, PA_E_INPUT, "null argument");
+    if (!(delta > 0)) return engine_fail(e, PA_E_INPUT, "delta must be positive");
+    if (k == 0) {
+        if (kernel_ms) *kernel_ms = 0;
+        return PA_OK;
+    }
+    const int n = s->n, S = s->sp.S, N = s->sp.N;
+    const size_t kn = (size_t)k * n;
+    for (size_t q = 0; q < kn; q++)
+        if (gnodes[q] < S || gnodes[q] >= N) return engine_fail(e, PA_E_INPUT, "gnode must be an internal gene node");
+    s->l0.resize(n);
+    s->l1.resize(n);
+    for (int i = 0; i < n; i++) {
+        if (indicator[i] != 0 && indicator[i] != 1) return engine_fail(e, PA_E_INPUT, "indicator must be 0 or 1");
+        if (lens[2 * i + 1] < 0 || lens[2 * i + indicator[i]] < 0) return engine_fail(e, PA_E_INPUT, "negative prefix length");
+        s->l0[i] = lens[2 * i + indicator[i]];
+        s->l1[i] = std::min(lens[2 * i + 1], s->l0[i]);
+    }
+    pa_status st;
+    if (s->res_prec != precision && (st = session_resident(s, precision)) != PA_OK) return st;
+    hipStream_t stream;
+    if ((st = engine_stream(e, &stream)) != PA_OK) return st;
+    if (hipStreamSynchronize(stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "session stream failed");
+    if (kn > s->sw_cap) {
+        (void)hipFree(s->sw_gnode);
+        (void)hipFree(s->sw_acc);
+        s->sw_gnode = s->sw_acc = nullptr;
+        s->sw_cap = 0;
+        if (hipMalloc(&s->sw_gnode, kn * 4) != hipSuccess || hipMalloc(&s->sw_acc, kn * 4) != hipSuccess)
+            return engine_fail(e, PA_E_OOM, "session sweep buffers");
+        s->sw_cap = kn;
+    }
+    while ((int)s->ev.size() < 2 * k) {
+        hipEvent_t x;
+        if (hipEventCreate(&x) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "session events");
+        s->ev.push_back(x);
+    }
+    s->h_ind.assign(indicator, indicator + n);
+    if (hipMemcpy(s->sw_gnode, gnodes, kn * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->ind, s->h_ind.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return engine_fail(e, PA_E_DEVICE, "session upload failed");
+    EngineView ev;
+    engine_view(e, &ev);
+    // every move on the stream in turn: each reads the state the previous
+    // one's decisions left, so nothing waits on the host in between
+    for (int j = 0; j < k; j++) {
+        int32_t *gn = s->sw_gnode + (size_t)j * n;
+        GtPropArgs a;
+        session_prop_args(s, a);
+        a.gnode = gn;
+        a.delta = delta;
+        a.key0 = ev.key0; a.key1 = ev.key1; a.m = m; a.slot = slot0 + j;
+        if (launch_gt_hprop(a, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height proposal launch failed");
+        if (hipEventRecord(s->ev[2 * j], stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "session events");
+        if ((st = gt_plan_run(e, s->plan, s->l0.data(), s->l1.data(), gn, j > 0, j > 0)) != PA_OK) return st;
+        if (hipEventRecord(s->ev[2 * j + 1], stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "session events");
+        GtDecideArgs d;
+        d.n = n; d.N = N;
+        d.win = s->win; d.prop = s->prop; d.rdraw = s->rdraw; d.prior_new = s->prior_new; d.lz_new = s->lz_new;
+        d.wnew = s->wnew; d.where = s->where;
+        d.par = s->par; d.root = s->root; d.alt = s->alt;
+        d.lnew0 = s->out0; d.lnew1 = s->out1; d.gnode = gn; d.indicator = s->ind;
+        d.h = s->h; d.loglik = s->ll; d.logpz = s->lz; d.prior = s->prior; d.accepted = s->sw_acc + (size_t)j * n;
+        if (launch_gt_hdecide(d, stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height decision launch failed");
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) return engine_fail(e, PA_E_DEVICE, "height moves failed");
+    if (accepted && hipMemcpy(accepted, s->sw_acc, kn * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return engine_fail(e, PA_E_DEVICE, "session download failed");
+    if (kernel_ms) {
+        double tot = 0;
+        for (int j = 0; j < k; j++) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, s->ev[2 * j], s->ev[2 * j + 1]) != hipSuccess)
+                return engine_fail(e, PA_E_DEVICE, "session events");
+            tot += ms;
+        }
+        *kernel_ms = tot;
+    }
     return gt_plan_stats(e, s->plan);
 }
 

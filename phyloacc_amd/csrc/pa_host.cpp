@@ -1693,13 +1693,18 @@ pa_status gt_plan_build(pa_engine *e, const int32_t *ids, int32_t n, const doubl
     return PA_OK;
 }
 
-pa_status gt_plan_run(pa_engine *e, GtPlan *p, const int32_t *lens0, const int32_t *lens1, const int32_t *d_path)
+pa_status gt_plan_run(pa_engine *e, GtPlan *p, const int32_t *lens0, const int32_t *lens1, const int32_t *d_path,
+                      bool same_lens, bool add_count)
 {
     pa_status st = upload(e);
     if (st != PA_OK) return st;
     const int n = p->n;
     LoglikArgs A = p->A;
-    if (lens0) {
+    if (lens0 && same_lens && (int)p->pl.size() == 2 * n) {
+        // the prefix lengths of the previous run, already on the device
+        A.pre_len = (const int32_t *)p->pre.p;
+        A.ll_out2 = (double *)p->out2.p;
+    } else if (lens0) {
         p->pl.resize((size_t)n * 2);
         for (int i = 0; i < n; i++) {
             if (lens1[i] < 0 || lens1[i] > lens0[i] || lens0[i] > p->GG[i])
@@ -1717,7 +1722,7 @@ pa_status gt_plan_run(pa_engine *e, GtPlan *p, const int32_t *lens0, const int32
     A.site = e->d_site;
     A.queue = e->d_queue;
     A.path_node = d_path;
-    HIPCHK(e, hipMemsetAsync(p->cnt.p, 0, 8, e->stream));
+    if (!add_count) HIPCHK(e, hipMemsetAsync(p->cnt.p, 0, 8, e->stream));
     HIPCHK(e, launch_reset(A.queue, e->stream));
     HIPCHK(e, hipEventRecord(e->ll_t0, e->stream));
     HIPCHK(e, launch_loglik(e->M, A, p->fp64, p->BS, p->grid, e->stream));

@@ -64,7 +64,10 @@ GtHistory &engine_history(pa_engine *e);
 struct GtPlan;
 pa_status gt_plan_build(pa_engine *e, const int32_t *ids, int32_t n, const double *piA, int32_t precision,
                         const GtExtra &x, GtPlan **out);
-pa_status gt_plan_run(pa_engine *e, GtPlan *p, const int32_t *lens0, const int32_t *lens1, const int32_t *d_path);
+// same_lens: lens0/lens1 are those of the previous run, already on the
+// device; add_count: the counted bytes add to the previous run's.
+pa_status gt_plan_run(pa_engine *e, GtPlan *p, const int32_t *lens0, const int32_t *lens1, const int32_t *d_path,
+                      bool same_lens = false, bool add_count = false);
 pa_status gt_plan_stats(pa_engine *e, GtPlan *p);
 void gt_plan_free(GtPlan *p);
 // element id's padded pattern count (-1 for an unknown id)

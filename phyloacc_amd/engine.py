@@ -35,8 +35,8 @@ EXPORTS = ["pa_default_options", "pa_parse_mod", "pa_engine_create", "pa_engine_
            "pa_batch_create", "pa_batch_run", "pa_batch_destroy", "pa_loglik_last_stats",
            "pa_loglik_lengths_pi", "pa_sample_pi", "pa_loglik_lengths_split", "pa_wl_step",
            "pa_gt_prior", "pa_gt_logpz", "pa_gt_lengths", "pa_gt_branch_len", "pa_gt_sample_history",
-           "pa_gt_sample_tree", "pa_gt_session_create", "pa_gt_session_branch_len", "pa_gt_session_read",
-           "pa_gt_session_destroy"]
+           "pa_gt_sample_tree", "pa_gt_session_create", "pa_gt_session_branch_len", "pa_gt_session_sweep",
+           "pa_gt_session_read", "pa_gt_session_destroy"]
 
 
 class SpeciesTree(C.Structure):
@@ -224,6 +224,9 @@ def lib():
         L.pa_gt_session_create.restype = C.c_int
         L.pa_gt_session_branch_len.argtypes = [C.c_void_p, PI, PI, PI, C.c_double, C.c_int32, C.c_int32, C.c_int32, PI]
         L.pa_gt_session_branch_len.restype = C.c_int
+        L.pa_gt_session_sweep.argtypes = [C.c_void_p, PI, C.c_int32, PI, PI, C.c_double, C.c_int32, C.c_int32,
+                                          C.c_int32, PI, PD]
+        L.pa_gt_session_sweep.restype = C.c_int
         L.pa_gt_session_read.argtypes = [C.c_void_p, PD, PD, PD, PD]
         L.pa_gt_session_read.restype = C.c_int
         L.pa_gt_session_destroy.argtypes = [C.c_void_p]
@@ -732,6 +735,24 @@ class GtSession:
                                                        _p(ln, C.c_int32), float(delta), int(m), int(slot),
                                                        int(precision), None if acc is None else _p(acc, C.c_int32)))
         return acc
+
+    def sweep(self, gnodes, indicator, lens, m, slot0, delta=1.0, precision=PA_FP64):
+        """k moves in a row (gnodes [k, n], or [k] for the same node in
+        every element), queued at once; returns (accepted [k, n], the
+        likelihood launches' summed ms)."""
+        g = np.asarray(gnodes, np.int32)
+        if g.ndim == 1:
+            g = np.repeat(g[:, None], self.n, axis=1)
+        g = np.ascontiguousarray(g, np.int32)
+        k = g.shape[0]
+        ind = np.ascontiguousarray(np.broadcast_to(indicator, (self.n,)), np.int32)
+        ln = np.ascontiguousarray(np.broadcast_to(lens, (self.n, 2)), np.int32)
+        acc = np.zeros((k, self.n), np.int32)
+        ms = C.c_double(0)
+        self.eng._check(lib().pa_gt_session_sweep(self._h, _p(g, C.c_int32), k, _p(ind, C.c_int32),
+                                                  _p(ln, C.c_int32), float(delta), int(m), int(slot0),
+                                                  int(precision), _p(acc, C.c_int32), C.byref(ms)))
+        return acc, ms.value
 
     def read(self):
         """(heights [n, N], loglik [n, 2], logpz [n, 2], prior [n])."""

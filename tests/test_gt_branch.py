@@ -204,3 +204,46 @@ def test_device_session_equals_the_host_moves(require_gpu, use_pi, S, precs):
         np.testing.assert_array_equal(pr2, prior)
     assert total > 0
     ses.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [E.PA_FP64, E.PA_FP32])
+def test_session_sweep_equals_single_moves(require_gpu, prec):
+    """pa_gt_session_sweep (every internal gene node in turn, queued at
+    once) leaves two sessions from the same state in the same place as one
+    pa_gt_session_branch_len call per node: the same decisions, heights and
+    state, to the bit; a second sweep continues from there."""
+    w = _gt_workload(seed=11, n=6, S=33)
+    opts = E.default_options(min_len=1, seed=SEED)
+    eng = E.Engine.from_workload(w, opts)
+    ids, st = eng.add_elements_with_trees(w.align, w.starts, w.sizes, w.children())
+    assert (st == E.PA_ELEM_OK).all()
+    sp, _ = _species(w)
+    S, N, n = w.tree.S, w.tree.N, w.C
+    ch = np.zeros((n, N, 2), np.int32)
+    par = np.zeros((n, N), np.int32)
+    H = np.zeros((n, N))
+    for c in range(n):
+        ch[c], par[c], H[c] = _state(w, c)
+    rates = np.stack([w.r1, w.r2], 1)
+    lens = np.stack([w.sizes, w.sizes // 2], 1).astype(np.int32)
+    piA = np.random.default_rng(5).uniform(0.2, 0.3, n)
+    f0, f1 = eng.loglik_lengths_split(ids, w.lengths(), lens[:, 0], lens[:, 1], piA=piA, precision=prec)
+    ll = np.stack([f0, f1], 1)
+    lz = np.full((n, 2), -2.0)
+    prior = np.array([E.gt_prior(sp, ch[c], par[c], H[c]) for c in range(n)])
+    one = E.GtSession(eng, sp, ids, ch, par, H, w.Z, rates, ll, lz, prior, piA=piA)
+    many = E.GtSession(eng, sp, ids, ch, par, H, w.Z, rates, ll, lz, prior, piA=piA)
+    rng = np.random.default_rng(6)
+    for m in range(2):
+        ind = np.full(n, m % 2, np.int32)
+        g = np.stack([rng.permutation(np.arange(S, N)) for _ in range(n)], 1).astype(np.int32)   # [k, n]
+        acc1 = np.stack([one.branch_len(g[j], ind, lens, m, 3 + j, precision=prec) for j in range(len(g))])
+        acc2, ms = many.sweep(g, ind, lens, m, 3, precision=prec)
+        np.testing.assert_array_equal(acc2, acc1)
+        assert ms > 0
+        for a, b in zip(one.read(), many.read()):
+            np.testing.assert_array_equal(b, a)
+    assert acc1.sum() > 0
+    one.close()
+    many.close()
