@@ -6,8 +6,9 @@ Remove_branch / Graft; src/PhyloAcc-GT/genetree.cpp:1416-1532,2138-2518,
 FP64, element by element over several sweeps: identical histories, the
 same accepted moves and trees (topology, heights), log-likelihoods within
 1e-9; the height moves run between the topology moves on the regrafted
-trees, so the element's stored tree must follow every accepted regraft on
-the device as well.  The Wang-Landau indicator alternates, element 0 has a
+trees (by the host-driven move, or by a device session's sweep), so the
+element's stored tree must follow every accepted regraft on the device as
+well.  The Wang-Landau indicator alternates, element 0 has a
 missing species, and a second run uses per-element base frequencies."""
 import numpy as np
 import pytest
@@ -29,8 +30,11 @@ def _setup(use_pi):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_pi", [False, True])
-def test_history_and_tree_moves_match_the_oracle(require_gpu, use_pi):
+@pytest.mark.parametrize("use_pi,heights", [(False, "host"), (True, "host"), (True, "session")])
+def test_history_and_tree_moves_match_the_oracle(require_gpu, use_pi, heights):
+    """heights: the height moves by pa_gt_branch_len, or by a device
+    session's sweep (pa_gt_session_sweep) read back before the next sweep's
+    history draw and regraft."""
     w, eng, ids = _setup(use_pi)
     m_or = O.parse_mod(w.mod_text)[0]
     sp, osp = _species(w)
@@ -86,17 +90,32 @@ def test_history_and_tree_moves_match_the_oracle(require_gpu, use_pi):
         # height moves on the (possibly regrafted) trees
         prior = np.array([O.gt_prior(osp, og[c]) for c in range(n)])
         oprior = prior.copy()
+        if heights == "session":
+            ses = E.GtSession(eng, sp, ids, ch, par, H, w.Z, rates, ll, lz, prior, piA=piA, cons_to_mis=c2m,
+                              ncons_to_mis=n2m)
+            swept, _ = ses.sweep(np.arange(S, N, dtype=np.int32), ind, lens, m, 1)
+            H[:], ll[:], lz[:], prior[:] = ses.read()
+            ses.close()
         for b, v in enumerate(range(S, N)):
-            a2 = eng.gt_branch_len(sp, ids, ch, par, H, w.Z, rates, v, ind, lens, ll, lz, prior, m, 1 + b,
-                                   cons_to_mis=c2m, ncons_to_mis=n2m, piA=piA)
+            if heights == "session":
+                a2 = swept[b]
+            else:
+                a2 = eng.gt_branch_len(sp, ids, ch, par, H, w.Z, rates, v, ind, lens, ll, lz, prior, m, 1 + b,
+                                       cons_to_mis=c2m, ncons_to_mis=n2m, piA=piA)
             for c in range(n):
                 code, oprior[c] = O.gt_branch_len(m_or, osp, og[c], w.Z[c], w.r1[c], w.r2[c], c2m, n2m, v, 1.0,
                                                   int(ind[c]), lens[c], oll[c], olz[c], oprior[c], SEED, m, 1 + b,
                                                   w.align, w.starts[c], w.sizes[c],
                                                   piA=0.0 if piA is None else piA[c])
                 assert code == a2[c], (m, v, c)
-                np.testing.assert_array_equal(H[c], og[c].h)
-                np.testing.assert_allclose(ll[c], oll[c], rtol=1e-9)
+                if heights == "host":
+                    np.testing.assert_array_equal(H[c], og[c].h)
+                    np.testing.assert_allclose(ll[c], oll[c], rtol=1e-9)
+        for c in range(n):
+            np.testing.assert_array_equal(H[c], og[c].h)
+            np.testing.assert_allclose(ll[c], oll[c], rtol=1e-9)
+            np.testing.assert_allclose(lz[c], olz[c], rtol=1e-12)
+            assert prior[c] == pytest.approx(oprior[c], rel=1e-12, abs=1e-12)
     assert accepted > 0 and moved > 0
     # the stored trees followed: a plain likelihood call at the final trees
     Lf = np.stack([E.gt_lengths(sp, ch[c], par[c], H[c], w.Z[c], w.r1[c], w.r2[c]) for c in range(n)])
