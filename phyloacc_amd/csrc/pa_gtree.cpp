@@ -507,7 +507,7 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
     if (s.S != ev.S || s.N != ev.N) return engine_fail(e, PA_E_INPUT, "species tree size differs from the model's");
     if (n == 0) return PA_OK;
     const int N = s.N;
-    std::vector<double> lens((size_t)n * N);
+    std::unique_ptr<double[]> lens(new double[(size_t)n * N]);   // every entry written below
     std::vector<int64_t> off(n);
     std::vector<int32_t> ns(n);
     int64_t total = 0;
@@ -540,7 +540,7 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
     x.hist_draw = 1;
     x.hist_sweep = m;
     std::vector<double> o(n), o1(n);
-    if ((st = gt_launch(e, ids, n, lens.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x)) != PA_OK)
+    if ((st = gt_launch(e, ids, n, lens.get(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x)) != PA_OK)
         return st;
     H.ids.assign(ids, ids + n);
     H.len.assign(len, len + n);
@@ -590,7 +590,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<Embed> emb(n);
     std::vector<TreeMove> mv(n);
     std::vector<uint8_t> lm((size_t)n * s.S);
-    std::vector<double> cur_len((size_t)n * N);
+    std::unique_ptr<double[]> cur_len(new double[(size_t)n * N]);   // every entry written below
     if ((st = stored_trees(e, ids, n, s.S, N, gchildren, lm.data())) != PA_OK) return st;
     for (int i = 0; i < n; i++) {
         const int ind = indicator[i];
@@ -663,7 +663,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         x.cand_branch = cb.data(); x.cand_n = cn.data(); x.cand_node = cnode.data(); x.cand_kind = ckind.data();
         x.cand_sites = csites.data(); x.cand_len = clen.data(); x.cand_max = cmax; x.cand_score = cscore.data();
         std::vector<double> o(n), o1(n);
-        st = gt_launch(e, ids, n, cur_len.data(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x);
+        st = gt_launch(e, ids, n, cur_len.get(), piA, nullptr, nullptr, precision, o.data(), o1.data(), x);
         if (Tg) (void)hipFree(dh);
         if (st != PA_OK) return st;
     }
@@ -673,7 +673,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<int> get(n), pos(n);
     std::vector<double> new_unif(n), rdraw(n);
     std::vector<int32_t> tabs((size_t)n * tab_words(N)), hdrs((size_t)n * 4), l0(n), l1(n);
-    std::vector<double> nlen((size_t)n * N), rpi;
+    std::unique_ptr<double[]> nlen(new double[(size_t)n * N]);   // every entry written below
     std::vector<int> gbad(n, 0);
     std::vector<Embed> emb2(n);   // the proposed trees' embeddings
     for_elements(n, [&](int i) {
@@ -747,7 +747,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         GtExtra x;
         x.tabs = tabs.data();
         x.hdrs = hdrs.data();
-        if ((st = gt_launch(e, ids, n, nlen.data(), piA, l0.data(), l1.data(), precision, o0.data(), o1.data(), x)) != PA_OK)
+        if ((st = gt_launch(e, ids, n, nlen.get(), piA, l0.data(), l1.data(), precision, o0.data(), o1.data(), x)) != PA_OK)
             return st;
     }
     for_elements(n, [&](int i) {

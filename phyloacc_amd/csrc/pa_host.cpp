@@ -222,7 +222,23 @@ struct HostElem {
     std::vector<int32_t> w, site;     // [P], [GG]
     std::vector<uint8_t> site_codes;  // [GG][S]
     std::vector<int> nodes, internal;
+    // gene trees: the stored children of each internal node (smaller id
+    // first) and the leaves' missing flags, kept compact for the check that
+    // a caller's tree is the stored one
+    std::vector<int16_t> kids;        // [2 (N - S)]
+    std::vector<uint8_t> leaf_miss;   // [S]
 };
+
+// The compact form of a gene tree's internal nodes' children (smaller id first).
+static void compact_kids(const int32_t *ch0, const int32_t *ch1, int S, int N, std::vector<int16_t> &kids)
+{
+    kids.resize(2 * (size_t)(N - S));
+    for (int v = S; v < N; v++) {
+        const int a = ch0[v], b = ch1[v];
+        kids[2 * (v - S)] = (int16_t)std::min(a, b);
+        kids[2 * (v - S) + 1] = (int16_t)std::max(a, b);
+    }
+}
 
 inline uint8_t code_of(int y, int gapchar)
 {
@@ -434,6 +450,12 @@ void encode_element(const TreeModel &tm, const pa_options &o, const char *const 
     par2[root] = N;
     if (root < S) { E.filtered = 1; return; }   // nothing left to prune over
     fill_tree_table(tm, ch2, par2, dist2, missing, own_children != nullptr, root, E);
+    if (own_children) {
+        const int32_t *T = E.tab.data();
+        compact_kids(T + tab_ch0(N), T + tab_ch1(N), S, N, E.kids);
+        E.leaf_miss.resize(S);
+        for (int q = 0; q < S; q++) E.leaf_miss[q] = (T[tab_flags(N) + q] & NF_MISSING) ? 1 : 0;
+    }
     // pattern compression (first-appearance order) or one pattern per site
     E.site.resize(GG);
     std::vector<int> first;
@@ -793,10 +815,21 @@ pa_status element_gene_tree(pa_engine *e, int32_t id, uint8_t *leaf_missing, con
     if (E.filtered) return fail(e, PA_E_INPUT, "element " + std::to_string(id) + " is filtered");
     const int N = e->tm.N, S = e->tm.S;
     const int32_t *T = E.tab.data();
-    for (int s = 0; s < S; s++) leaf_missing[s] = (T[tab_flags(N) + s] & NF_MISSING) ? 1 : 0;
+    const bool compact = (int)E.kids.size() == 2 * (N - S) && (int)E.leaf_miss.size() == S;
+    if (compact) memcpy(leaf_missing, E.leaf_miss.data(), S);
+    else
+        for (int s = 0; s < S; s++) leaf_missing[s] = (T[tab_flags(N) + s] & NF_MISSING) ? 1 : 0;
     if (children)
         for (int v = S; v < N; v++) {
-            const int a = T[tab_ch0(N) + v], b = T[tab_ch1(N) + v], x = children[2 * v], y = children[2 * v + 1];
+            const int x = children[2 * v], y = children[2 * v + 1];
+            int a, b;
+            if (compact) {
+                a = E.kids[2 * (v - S)];
+                b = E.kids[2 * (v - S) + 1];
+            } else {
+                a = T[tab_ch0(N) + v];
+                b = T[tab_ch1(N) + v];
+            }
             if (!((a == x && b == y) || (a == y && b == x)))
                 return fail(e, PA_E_INPUT, "element " + std::to_string(id) + ": gene node " + std::to_string(v) +
                                                " has other children than the element's stored tree");
@@ -1529,6 +1562,7 @@ pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32
         E.root = hd[0];
         E.n_up = hd[2];
         E.n_down = hd[3];
+        compact_kids(E.tab.data() + tab_ch0(N), E.tab.data() + tab_ch1(N), e->tm.S, N, E.kids);
         E.nodes.assign(T + tab_nodes(N), T + tab_nodes(N) + hd[1]);
         E.internal.assign(T + tab_int(N), T + tab_int(N) + (hd[1] - 1) / 2);
         if (!dev || id >= (int)e->dev_elems.size()) continue;
