@@ -672,7 +672,9 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<std::vector<double>> nh(n), sprob(n);
     std::vector<int> get(n), pos(n);
     std::vector<double> new_unif(n), rdraw(n);
-    std::vector<int32_t> tabs((size_t)n * tab_words(N)), hdrs((size_t)n * 4), l0(n), l1(n);
+    // every row of tabs is written by gene_table before the launch reads it (a failure returns first)
+    std::unique_ptr<int32_t[]> tabs(new int32_t[(size_t)n * tab_words(N)]);
+    std::vector<int32_t> hdrs((size_t)n * 4), l0(n), l1(n);
     std::unique_ptr<double[]> nlen(new double[(size_t)n * N]);   // every entry written below
     std::vector<int> gbad(n, 0);
     std::vector<Embed> emb2(n);   // the proposed trees' embeddings
@@ -745,7 +747,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<double> o0(n), o1(n);
     {
         GtExtra x;
-        x.tabs = tabs.data();
+        x.tabs = tabs.get();
         x.hdrs = hdrs.data();
         if ((st = gt_launch(e, ids, n, nlen.get(), piA, l0.data(), l1.data(), precision, o0.data(), o1.data(), x)) != PA_OK)
             return st;

@@ -1548,9 +1548,12 @@ pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32
         if (ids[k] < 0 || ids[k] >= (int)e->elems.size() || hdrs[k][1] < 1 || hdrs[k][1] > N)
             return fail(e, PA_E_INPUT, "bad element or node table");
     const bool dev = !e->dev_dirty && e->d_tab && e->d_elems;
-    std::vector<int32_t> stage(dev ? (size_t)n * W : 0), dst_id(dev ? n : 0);
+    // the staged rows (ns of them, each copied in full) go to the device
+    std::unique_ptr<int32_t[]> stage(dev ? new int32_t[(size_t)n * W] : nullptr);
+    std::vector<int32_t> dst_id(dev ? n : 0);
     std::vector<int64_t> dst_off(dev ? n : 0);
     std::vector<ElemDev> recs(dev ? n : 0);
+    int ns = 0;
     for (int k = 0; k < n; k++) {
         const int id = ids[k];
         HostElem &E = e->elems[id];
@@ -1569,23 +1572,24 @@ pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32
         ElemDev &x = e->dev_elems[id];
         x.root = E.root; x.n_nodes = (int)E.nodes.size(); x.n_int = (int)E.internal.size();
         x.n_up = E.n_up; x.n_down = E.n_down;
-        memcpy(&stage[(size_t)k * W], E.tab.data(), (size_t)W * 4);
-        dst_id[k] = id;
-        dst_off[k] = x.off_tab;
-        recs[k] = x;
+        memcpy(&stage[(size_t)ns * W], E.tab.data(), (size_t)W * 4);
+        dst_id[ns] = id;
+        dst_off[ns] = x.off_tab;
+        recs[ns] = x;
+        ns++;
     }
-    if (!dev) return PA_OK;
+    if (!dev || ns == 0) return PA_OK;
     DevBuf b[4];
-    HIPCHK(e, hipMalloc(&b[0].p, stage.size() * 4));
-    HIPCHK(e, hipMalloc(&b[1].p, (size_t)n * sizeof(ElemDev)));
-    HIPCHK(e, hipMalloc(&b[2].p, (size_t)n * 8));
-    HIPCHK(e, hipMalloc(&b[3].p, (size_t)n * 4));
-    HIPCHK(e, hipMemcpy(b[0].p, stage.data(), stage.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(e, hipMemcpy(b[1].p, recs.data(), (size_t)n * sizeof(ElemDev), hipMemcpyHostToDevice));
-    HIPCHK(e, hipMemcpy(b[2].p, dst_off.data(), (size_t)n * 8, hipMemcpyHostToDevice));
-    HIPCHK(e, hipMemcpy(b[3].p, dst_id.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMalloc(&b[0].p, (size_t)ns * W * 4));
+    HIPCHK(e, hipMalloc(&b[1].p, (size_t)ns * sizeof(ElemDev)));
+    HIPCHK(e, hipMalloc(&b[2].p, (size_t)ns * 8));
+    HIPCHK(e, hipMalloc(&b[3].p, (size_t)ns * 4));
+    HIPCHK(e, hipMemcpy(b[0].p, stage.get(), (size_t)ns * W * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[1].p, recs.data(), (size_t)ns * sizeof(ElemDev), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[2].p, dst_off.data(), (size_t)ns * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[3].p, dst_id.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
     HIPCHK(e, launch_gt_scatter_tables((const int32_t *)b[0].p, W, (const int64_t *)b[2].p, e->d_tab,
-                                       (const ElemDev *)b[1].p, (const int32_t *)b[3].p, e->d_elems, n, e->stream));
+                                       (const ElemDev *)b[1].p, (const int32_t *)b[3].p, e->d_elems, ns, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return PA_OK;
 }
