@@ -938,8 +938,10 @@ extern "C" pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *t
     if ((st = stored_trees(e, ids, n, S, N, gchildren, lm.data())) != PA_OK) return st;
     // per element: the post-order, root, embedding and missing flags; the
     // trees keep their topology for the session's life, so the flags hold
-    std::vector<int32_t> order((size_t)n * N), root(n), where((size_t)n * N);
-    std::vector<uint8_t> miss((size_t)n * N);
+    // every element's rows are written below before they are uploaded (a bad tree returns first)
+    std::unique_ptr<int32_t[]> order(new int32_t[(size_t)n * N]), where(new int32_t[(size_t)n * N]);
+    std::unique_ptr<uint8_t[]> miss(new uint8_t[(size_t)n * N]);
+    std::vector<int32_t> root(n);
     std::vector<int> bad(n, 0);
     const Sp &sp = s->sp;
     for_elements(n, [&](int i) {
@@ -980,8 +982,8 @@ extern "C" pa_status pa_gt_session_create(pa_engine *e, const pa_species_tree *t
     chk(s->get(&s->up, N, sp.up)); chk(s->get(&s->H, N, sp.H)); chk(s->get(&s->theta, N, sp.theta));
     chk(s->get(&s->post, N, sp.post.data())); chk(s->get(&s->nleaf, N, nleaf.data()));
     chk(s->get(&s->rate, N, rate.data())); chk(s->get(&s->ldens, N, ldens.data()));
-    chk(s->get(&s->where, nN, where.data())); chk(s->get(&s->miss, nN, miss.data())); chk(s->get(&s->wnew, n));
-    chk(s->get(&s->ch, 2 * nN, gchildren)); chk(s->get(&s->par, nN, gparent)); chk(s->get(&s->order, nN, order.data()));
+    chk(s->get(&s->where, nN, where.get())); chk(s->get(&s->miss, nN, miss.get())); chk(s->get(&s->wnew, n));
+    chk(s->get(&s->ch, 2 * nN, gchildren)); chk(s->get(&s->par, nN, gparent)); chk(s->get(&s->order, nN, order.get()));
     chk(s->get(&s->root, n, root.data())); chk(s->get(&s->h, nN, gheight)); chk(s->get(&s->Z, nN, Z));
     chk(s->get(&s->rates, 2 * (size_t)n, rates));
     chk(s->get(&s->ll, 2 * (size_t)n, loglik)); chk(s->get(&s->lz, 2 * (size_t)n, logpz)); chk(s->get(&s->prior, n, prior));
