@@ -527,6 +527,8 @@ struct pa_engine {
     size_t ll_scratch_bytes = 0, ll_len_bytes = 0;
     void *h_ll_len = nullptr;       // pinned staging of the per-call lengths
     size_t h_ll_len_bytes = 0;
+    void *h_tab_stage = nullptr;    // pinned staging of accepted gene trees' node tables
+    size_t h_tab_stage_bytes = 0;
     size_t ll_last_plan_bytes = 0;           // scratch the last likelihood launch used
     // the allocation-free likelihood entry point: this device's copy of its
     // scratch array, and the arenas' longest element and mean pattern count
@@ -688,6 +690,7 @@ extern "C" void pa_engine_destroy(pa_engine *e)
         (void)hipFree(e->d_ll_scratch);
         (void)hipFree(e->d_ll_len);
         (void)hipHostFree(e->h_ll_len);
+        (void)hipHostFree(e->h_tab_stage);
         (void)hipFree(e->hist.d);
         if (e->ll_t0) (void)hipEventDestroy(e->ll_t0);
         if (e->ll_t1) (void)hipEventDestroy(e->ll_t1);
@@ -991,9 +994,10 @@ static void release_ll_buffers(pa_engine *e)
     (void)hipFree(e->d_ll_scratch);
     (void)hipFree(e->d_ll_len);
     (void)hipHostFree(e->h_ll_len);
+    (void)hipHostFree(e->h_tab_stage);
     e->d_ll_scratch = e->d_ll_len = nullptr;
-    e->h_ll_len = nullptr;
-    e->ll_scratch_bytes = e->ll_len_bytes = e->h_ll_len_bytes = 0;
+    e->h_ll_len = e->h_tab_stage = nullptr;
+    e->ll_scratch_bytes = e->ll_len_bytes = e->h_ll_len_bytes = e->h_tab_stage_bytes = 0;
 }
 
 // Per-call lengths to the device: copied into a retained pinned buffer by up
@@ -1543,48 +1547,78 @@ pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32
                             const int32_t *const *hdrs)
 {
     if (n <= 0) return PA_OK;
-    const int N = e->tm.N, W = tab_words(N);
+    const int N = e->tm.N, S = e->tm.S, W = tab_words(N), D = tab_dist(N);
     for (int k = 0; k < n; k++)
         if (ids[k] < 0 || ids[k] >= (int)e->elems.size() || hdrs[k][1] < 1 || hdrs[k][1] > N)
             return fail(e, PA_E_INPUT, "bad element or node table");
     const bool dev = !e->dev_dirty && e->d_tab && e->d_elems;
-    // the staged rows (ns of them, each copied in full) go to the device
-    std::unique_ptr<int32_t[]> stage(dev ? new int32_t[(size_t)n * W] : nullptr);
-    std::vector<int32_t> dst_id(dev ? n : 0);
-    std::vector<int64_t> dst_off(dev ? n : 0);
-    std::vector<ElemDev> recs(dev ? n : 0);
+    // the rows that go to the device (elements with a device record), in
+    // batch order, staged in a retained pinned buffer
+    std::vector<int> row(n, -1);
     int ns = 0;
-    for (int k = 0; k < n; k++) {
+    if (dev)
+        for (int k = 0; k < n; k++)
+            if (ids[k] < (int)e->dev_elems.size()) row[k] = ns++;
+    const size_t stage_bytes = (size_t)ns * W * 4;
+    if (stage_bytes > e->h_tab_stage_bytes) {
+        (void)hipHostFree(e->h_tab_stage);
+        e->h_tab_stage = nullptr;
+        e->h_tab_stage_bytes = 0;
+        HIPCHK(e, hipHostMalloc(&e->h_tab_stage, stage_bytes));
+        e->h_tab_stage_bytes = stage_bytes;
+    }
+    int32_t *stage = (int32_t *)e->h_tab_stage;
+    std::vector<int32_t> dst_id(ns);
+    std::vector<int64_t> dst_off(ns);
+    std::vector<ElemDev> recs(ns);
+    // one element per index on each thread (the ids are distinct, so no two
+    // threads touch one element)
+    auto one = [&](int k) {
         const int id = ids[k];
         HostElem &E = e->elems[id];
         const int32_t *T = tabs[k], *hd = hdrs[k];
         // gene trees take their branch lengths per call; the stored ones stay
-        std::vector<int32_t> tab(T, T + W);
-        memcpy(tab.data() + tab_dist(N), E.tab.data() + tab_dist(N), (size_t)(W - tab_dist(N)) * 4);
-        E.tab = std::move(tab);
+        if ((int)E.tab.size() == W) {
+            memcpy(E.tab.data(), T, (size_t)D * 4);
+        } else {
+            std::vector<int32_t> tab(T, T + W);
+            if ((int)E.tab.size() >= W) memcpy(tab.data() + D, E.tab.data() + D, (size_t)(W - D) * 4);
+            E.tab = std::move(tab);
+        }
         E.root = hd[0];
         E.n_up = hd[2];
         E.n_down = hd[3];
-        compact_kids(E.tab.data() + tab_ch0(N), E.tab.data() + tab_ch1(N), e->tm.S, N, E.kids);
+        compact_kids(E.tab.data() + tab_ch0(N), E.tab.data() + tab_ch1(N), S, N, E.kids);
         E.nodes.assign(T + tab_nodes(N), T + tab_nodes(N) + hd[1]);
         E.internal.assign(T + tab_int(N), T + tab_int(N) + (hd[1] - 1) / 2);
-        if (!dev || id >= (int)e->dev_elems.size()) continue;
+        const int r = row[k];
+        if (r < 0) return;
         ElemDev &x = e->dev_elems[id];
         x.root = E.root; x.n_nodes = (int)E.nodes.size(); x.n_int = (int)E.internal.size();
         x.n_up = E.n_up; x.n_down = E.n_down;
-        memcpy(&stage[(size_t)ns * W], E.tab.data(), (size_t)W * 4);
-        dst_id[ns] = id;
-        dst_off[ns] = x.off_tab;
-        recs[ns] = x;
-        ns++;
+        memcpy(&stage[(size_t)r * W], E.tab.data(), (size_t)W * 4);
+        dst_id[r] = id;
+        dst_off[r] = x.off_tab;
+        recs[r] = x;
+    };
+    const int nt = n < 256 ? 1 : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nt == 1) {
+        for (int k = 0; k < n; k++) one(k);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; t++)
+            th.emplace_back([&, t]() {
+                for (int k = t; k < n; k += nt) one(k);
+            });
+        for (auto &x : th) x.join();
     }
-    if (!dev || ns == 0) return PA_OK;
+    if (ns == 0) return PA_OK;
     DevBuf b[4];
-    HIPCHK(e, hipMalloc(&b[0].p, (size_t)ns * W * 4));
+    HIPCHK(e, hipMalloc(&b[0].p, stage_bytes));
     HIPCHK(e, hipMalloc(&b[1].p, (size_t)ns * sizeof(ElemDev)));
     HIPCHK(e, hipMalloc(&b[2].p, (size_t)ns * 8));
     HIPCHK(e, hipMalloc(&b[3].p, (size_t)ns * 4));
-    HIPCHK(e, hipMemcpy(b[0].p, stage.get(), (size_t)ns * W * 4, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(b[0].p, stage, stage_bytes, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b[1].p, recs.data(), (size_t)ns * sizeof(ElemDev), hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b[2].p, dst_off.data(), (size_t)ns * 8, hipMemcpyHostToDevice));
     HIPCHK(e, hipMemcpy(b[3].p, dst_id.data(), (size_t)ns * 4, hipMemcpyHostToDevice));

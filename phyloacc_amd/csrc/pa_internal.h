@@ -84,7 +84,7 @@ pa_status gt_launch(pa_engine *e, const int32_t *ids, int32_t n, const double *l
 pa_status gene_table(pa_engine *e, int32_t id, const int32_t *children, int32_t *T, int32_t *hdr);
 // Makes the node table tabs[k] (with its header hdrs[k], both from
 // gene_table) the stored tree of element ids[k] (host and, when uploaded,
-// device).
+// device).  The ids must be distinct: the elements are updated in parallel.
 pa_status element_set_trees(pa_engine *e, const int32_t *ids, int n, const int32_t *const *tabs,
                             const int32_t *const *hdrs);
 // The engine's stream (after the device is set up), for launches that a
