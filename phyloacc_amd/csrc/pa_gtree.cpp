@@ -523,7 +523,8 @@ extern "C" pa_status pa_gt_sample_history(pa_engine *e, const pa_species_tree *t
     }
     std::vector<int> bad(n, 0);
     for_elements(n, [&](int i) {
-        Embed g;
+        thread_local Embed g;   // reused by the thread's elements; build expects no root yet
+        g.root = -1;
         if (!g.build(s, gchildren + (size_t)i * 2 * N, gparent + (size_t)i * N, gheight + (size_t)i * N)) { bad[i] = 1; return; }
         lengths_of(g, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &lens[(size_t)i * N]);
     });
@@ -668,8 +669,14 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         if (st != PA_OK) return st;
     }
     // draw each element's regraft; build the proposed trees
-    std::vector<std::vector<int32_t>> nch(n), npar(n);
-    std::vector<std::vector<double>> nh(n), sprob(n);
+    // the proposed trees and candidate probabilities, one row per element
+    // (every row written in the draw below before it is read)
+    std::unique_ptr<int32_t[]> nch_buf(new int32_t[(size_t)n * 2 * N]), npar_buf(new int32_t[(size_t)n * N]);
+    std::unique_ptr<double[]> nh_buf(new double[(size_t)n * N]), sp_buf(new double[(size_t)n * cmax]);
+    auto nch = [&](int i) { return &nch_buf[(size_t)i * 2 * N]; };
+    auto npar = [&](int i) { return &npar_buf[(size_t)i * N]; };
+    auto nh = [&](int i) { return &nh_buf[(size_t)i * N]; };
+    auto sprob = [&](int i) { return &sp_buf[(size_t)i * cmax]; };
     std::vector<int> get(n), pos(n);
     std::vector<double> new_unif(n), rdraw(n);
     // every row of tabs is written by gene_table before the launch reads it (a failure returns first)
@@ -682,19 +689,20 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         TreeMove &t = mv[i];
         const Embed &g = emb[i];
         const int nc = (int)t.cnode.size();
-        std::vector<double> &p = sprob[i];
-        p.assign(nc, 1.0);
+        double *p = sprob(i);
+        std::fill(p, p + nc, 1.0);
         get[i] = 0;
         if (nc > 1) {
             double mx = -INFINITY, tot = 0;
-            std::vector<double> sc(nc);
+            thread_local std::vector<double> sc;
+            sc.resize(nc);
             for (int q = 0; q < nc; q++) {
                 sc[q] = t.ckind[q] == 3 ? 0.0 : cscore[(size_t)i * cmax + q];
                 mx = std::max(mx, sc[q]);
             }
             for (int q = 0; q < nc; q++) { p[q] = std::exp(sc[q] - mx); tot += p[q]; }
             for (int q = 0; q < nc; q++) p[q] /= tot;
-            get[i] = categorical_n(p.data(), nc, slot_uniform(key, (uint32_t)slot, c2, c3, 0));
+            get[i] = categorical_n(p, nc, slot_uniform(key, (uint32_t)slot, c2, c3, 0));
         }
         pos[i] = -1;
         for (int q = 0; q < nc; q++) if (t.cnode[q] == (t.bpp < N ? t.bpp : N)) { pos[i] = q; break; }
@@ -720,22 +728,24 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         new_unif[i] = -std::log(upp - low);
         rdraw[i] = slot_uniform(key, (uint32_t)slot, c2, c3, 3);
         // Graft: bp above nsib, branch keeps its side of bp
-        nch[i] = t.ch2;
-        npar[i] = t.par2;
-        nh[i].assign(g.h, g.h + N);
-        const int bp = t.bp, it2p = npar[i][nsib];
-        nh[i][bp] = u * (upp - low) + low;
-        npar[i][bp] = it2p;
-        npar[i][nsib] = bp;
-        nch[i][2 * bp + t.cb] = branch[i];
-        nch[i][2 * bp + 1 - t.cb] = nsib;
-        if (it2p < N) nch[i][2 * it2p + (t.ch2[2 * it2p] == nsib ? 0 : 1)] = bp;
-        if (gene_table(e, ids[i], nch[i].data(), &tabs[(size_t)i * tab_words(N)], &hdrs[(size_t)i * 4]) != PA_OK) {
+        int32_t *ch_i = nch(i), *par_i = npar(i);
+        double *h_i = nh(i);
+        memcpy(ch_i, t.ch2.data(), sizeof(int32_t) * 2 * N);
+        memcpy(par_i, t.par2.data(), sizeof(int32_t) * N);
+        memcpy(h_i, g.h, sizeof(double) * N);
+        const int bp = t.bp, it2p = par_i[nsib];
+        h_i[bp] = u * (upp - low) + low;
+        par_i[bp] = it2p;
+        par_i[nsib] = bp;
+        ch_i[2 * bp + t.cb] = branch[i];
+        ch_i[2 * bp + 1 - t.cb] = nsib;
+        if (it2p < N) ch_i[2 * it2p + (t.ch2[2 * it2p] == nsib ? 0 : 1)] = bp;
+        if (gene_table(e, ids[i], ch_i, &tabs[(size_t)i * tab_words(N)], &hdrs[(size_t)i * 4]) != PA_OK) {
             gbad[i] = 1;
             return;
         }
         Embed &g2 = emb2[i];
-        if (!g2.build(s, nch[i].data(), npar[i].data(), nh[i].data())) { gbad[i] = 1; return; }
+        if (!g2.build(s, ch_i, par_i, h_i)) { gbad[i] = 1; return; }
         lengths_of(g2, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &nlen[(size_t)i * N]);
         const int ind = indicator[i];
         l0[i] = lens[2 * i + ind];
@@ -756,20 +766,20 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         accepted[i] = 0;
         if (pos[i] < 0) return;
         const Embed &g2 = emb2[i];
-        std::vector<uint8_t> miss;
+        thread_local std::vector<uint8_t> miss;
         missing_of(g2, &lm[(size_t)i * s.S], miss);
         const double lz = logpz_of(g2, miss, Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
         const int ind = indicator[i];
         const double lnew[2] = {o0[i], o1[i]};
         double d = lnew[ind] - loglik[2 * i + ind];
         if (logpz[2 * i + ind] != 0) d += lz - logpz[2 * i + ind];
-        d += prior_of(g2) - mv[i].curG + mv[i].cur_unif - new_unif[i] + std::log(sprob[i][pos[i]]) -
-             std::log(sprob[i][get[i]]);
+        d += prior_of(g2) - mv[i].curG + mv[i].cur_unif - new_unif[i] + std::log(sprob(i)[pos[i]]) -
+             std::log(sprob(i)[get[i]]);
         if (!(std::log(rdraw[i]) < d)) return;
         accepted[i] = 1;
-        memcpy(gchildren + (size_t)i * 2 * N, nch[i].data(), sizeof(int32_t) * 2 * N);
-        memcpy(gparent + (size_t)i * N, npar[i].data(), sizeof(int32_t) * N);
-        memcpy(gheight + (size_t)i * N, nh[i].data(), sizeof(double) * N);
+        memcpy(gchildren + (size_t)i * 2 * N, nch(i), sizeof(int32_t) * 2 * N);
+        memcpy(gparent + (size_t)i * N, npar(i), sizeof(int32_t) * N);
+        memcpy(gheight + (size_t)i * N, nh(i), sizeof(double) * N);
         loglik[2 * i] = lnew[0];
         loglik[2 * i + 1] = lnew[1];
         if (logpz[2 * i + 1] == 0) logpz[2 * i] = lz;
@@ -783,10 +793,6 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
         emb[i] = Embed();
         emb2[i] = Embed();
         mv[i] = TreeMove();
-        std::vector<int32_t>().swap(nch[i]);
-        std::vector<int32_t>().swap(npar[i]);
-        std::vector<double>().swap(nh[i]);
-        std::vector<double>().swap(sprob[i]);
     });
     // the accepted trees become the elements' stored trees
     std::vector<int32_t> sid;
