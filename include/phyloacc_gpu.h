@@ -379,8 +379,9 @@ pa_status pa_loglik_gbgc(pa_engine *e, const int32_t *ids, int32_t n, const int3
 
 /* pa_loglik and pa_loglik_lengths keep their device scratch (workgroup slots
  * and per-call length buffer) for the next call when it is at most 8 GiB;
- * this frees it now (no reference counterpart: the reference allocates per
- * element on the host).  Call it before a large pa_run_chains or
+ * this frees it now, with the pinned host buffers that stage the lengths and
+ * the regrafted trees' node tables (no reference counterpart: the reference
+ * allocates per element on the host).  Call it before a large pa_run_chains or
  * pa_serve_create when device memory is tight. */
 pa_status pa_release_scratch(pa_engine *e);
 
