@@ -684,7 +684,7 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     std::vector<int32_t> hdrs((size_t)n * 4), l0(n), l1(n);
     std::unique_ptr<double[]> nlen(new double[(size_t)n * N]);   // every entry written below
     std::vector<int> gbad(n, 0);
-    std::vector<Embed> emb2(n);   // the proposed trees' embeddings
+    std::vector<double> prior2(n), lz2(n);   // the proposed trees' prior and missing-gene term
     for_elements(n, [&](int i) {
         TreeMove &t = mv[i];
         const Embed &g = emb[i];
@@ -744,9 +744,15 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
             gbad[i] = 1;
             return;
         }
-        Embed &g2 = emb2[i];
+        thread_local Embed g2;   // the proposed tree's embedding, reused by the thread's elements
+        g2.root = -1;
         if (!g2.build(s, ch_i, par_i, h_i)) { gbad[i] = 1; return; }
         lengths_of(g2, Z + (size_t)i * N, rates[2 * i], rates[2 * i + 1], &nlen[(size_t)i * N]);
+        // the decision's terms that do not need the likelihoods
+        thread_local std::vector<uint8_t> miss;
+        missing_of(g2, &lm[(size_t)i * s.S], miss);
+        lz2[i] = logpz_of(g2, miss, Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
+        prior2[i] = prior_of(g2);
         const int ind = indicator[i];
         l0[i] = lens[2 * i + ind];
         l1[i] = std::min(lens[2 * i + 1], lens[2 * i + ind]);
@@ -765,15 +771,12 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     for_elements(n, [&](int i) {
         accepted[i] = 0;
         if (pos[i] < 0) return;
-        const Embed &g2 = emb2[i];
-        thread_local std::vector<uint8_t> miss;
-        missing_of(g2, &lm[(size_t)i * s.S], miss);
-        const double lz = logpz_of(g2, miss, Z + (size_t)i * N, cons_to_mis, ncons_to_mis);
+        const double lz = lz2[i];
         const int ind = indicator[i];
         const double lnew[2] = {o0[i], o1[i]};
         double d = lnew[ind] - loglik[2 * i + ind];
         if (logpz[2 * i + ind] != 0) d += lz - logpz[2 * i + ind];
-        d += prior_of(g2) - mv[i].curG + mv[i].cur_unif - new_unif[i] + std::log(sprob(i)[pos[i]]) -
+        d += prior2[i] - mv[i].curG + mv[i].cur_unif - new_unif[i] + std::log(sprob(i)[pos[i]]) -
              std::log(sprob(i)[get[i]]);
         if (!(std::log(rdraw[i]) < d)) return;
         accepted[i] = 1;
@@ -791,7 +794,6 @@ extern "C" pa_status pa_gt_sample_tree(pa_engine *e, const pa_species_tree *t, c
     // a quarter of the call)
     for_elements(n, [&](int i) {
         emb[i] = Embed();
-        emb2[i] = Embed();
         mv[i] = TreeMove();
     });
     // the accepted trees become the elements' stored trees
